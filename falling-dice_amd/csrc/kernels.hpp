@@ -1,0 +1,124 @@
+// kernels.hpp — device data layout and kernels of the gfx950 equilibrium solver.
+//
+// Layout in HBM (all structure-of-arrays, one plane per component):
+//   nodes   base position bx|by|bz (FP64), base orientation bq w|x|y|z (FP64),
+//           mass, lock bitmask (bit d locks DOF d)
+//   state   x (3 planes FP64) and R (4 planes, element type T) of the current
+//           evaluation point, produced once per objective evaluation
+//   beams   a, b, L0, radius, area, I, e0 (3 planes), plastic (7 planes FP64)
+//   vectors z, grad, L-BFGS s_h / y_h: 6 DOF planes of N doubles each
+//   records per-beam element output (9 planes: chord force f, rotational
+//           gradients of ends a and b), per-shape contact forces
+// Gradients are assembled per node through a CSR beam-incidence list in a fixed
+// order: no floating-point atomics anywhere, so every evaluation is
+// bit-reproducible.  Reductions are two-level (wave shuffle + LDS per block,
+// fixed-order sum of block partials in the consumer kernel).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "devmath.hpp"
+
+namespace mmc {
+
+constexpr int kBlock = 256;
+constexpr int kWave = 64;
+
+struct Material {
+  double E, G, sy, H, hb_frac, ht_frac, shear, density;
+};
+
+// Read-only lattice description on the device.
+template <class T>
+struct Lat {
+  int N, B;
+  const double* bpos;   // [3][N]
+  const double* bquat;  // [4][N]
+  const double* mass;   // [N]
+  const uint8_t* lock;  // [N]
+  const int* ba;        // [B]
+  const int* bb;        // [B]
+  const double* L0;     // [B]
+  const double* rad;    // [B]
+  const double* area;   // [B]
+  const double* inert;  // [B]
+  const double* e0;     // [3][B]
+  const double* plastic;  // [7][B]
+  const int* inc_off;     // [N+1]
+  const int* inc;         // beam*2 + side
+  const int* adj_off;     // [N+1]
+  const int* adj;         // sorted neighbour lists (contact exclusion)
+};
+
+// Per-evaluation scalars handed between kernels.
+struct Scalars {
+  double ts;          // translation scale
+  double jr;          // joint sphere radius
+  double gravity;
+  double kc;          // contact stiffness
+  double margin;
+  double centering;
+  double top;         // platen position
+  double cell;        // broadphase cell
+  double inv_cell;
+  unsigned table_mask;
+  int entry_cap;
+};
+
+// ---- block reduction helpers (wave64 butterfly + LDS across waves) ----
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+  return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, kWave));
+  return v;
+}
+// Reduces K values per thread; mode bit k set = max, else sum.  Result valid in
+// thread 0.  Fixed combination order: deterministic.
+template <int K>
+__device__ __forceinline__ void block_reduce(double (&v)[K], unsigned maxmask) {
+  __shared__ double sh[K][kBlock / kWave];
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+#pragma unroll
+  for (int k = 0; k < K; k++) v[k] = ((maxmask >> k) & 1) ? wave_max(v[k]) : wave_sum(v[k]);
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < K; k++) sh[k][w] = v[k];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      double a = sh[k][0];
+      for (int i = 1; i < kBlock / kWave; i++) a = ((maxmask >> k) & 1) ? fmax(a, sh[k][i]) : a + sh[k][i];
+      v[k] = a;
+    }
+  }
+  __syncthreads();
+}
+// Sum (or max) of P block partials laid out [K][P], evaluated identically by
+// every block that calls it (all threads participate; result broadcast).
+template <int K>
+__device__ __forceinline__ void reduce_partials(const double* part, int P, double (&out)[K], unsigned maxmask) {
+  double v[K];
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    bool mx = (maxmask >> k) & 1;
+    double a = mx ? 0.0 : 0.0;
+    for (int i = threadIdx.x; i < P; i += kBlock) a = mx ? fmax(a, part[k * P + i]) : a + part[k * P + i];
+    v[k] = a;
+  }
+  block_reduce<K>(v, maxmask);
+  __shared__ double bc[K];
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int k = 0; k < K; k++) bc[k] = v[k];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < K; k++) out[k] = bc[k];
+  __syncthreads();
+}
+
+}  // namespace mmc

@@ -1,0 +1,1553 @@
+// mmc.hip — gfx950 kernels and the C-ABI context of the beam-lattice
+// equilibrium solver (the hot path of Simulator::objective/minimize/commit,
+// projects/metamaterial-compression/industry-grade/src/sim.cpp:136-212).
+//
+// One objective evaluation is five kernels on one stream:
+//   k_state   node DOFs -> pose (x, R), gravity, platen contact, centroid sums
+//   k_count / scan / k_fill / k_sort
+//             uniform-grid spatial hash of joint spheres and strut capsules
+//   k_narrow  per-shape narrowphase; each shape accumulates its own contact
+//             force over its candidate partners in a fixed order
+//   k_beam    per-beam axial/shear/bending/torsion energy, gradient record and
+//             trial plastic state
+//   k_gather  per-node deterministic assembly over the CSR incidence list,
+//             gradient pack, lock mask and convergence norms
+//   k_final   one block: fixed-order sum of every block partial -> record
+// L-BFGS vector algebra (two-loop recursion, clamp, pair update) runs in
+// fused thread-per-node kernels over the 6-plane DOF vectors; its scalars
+// travel between kernels as block partials, never through the host.
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/mmc.h"
+#include "kernels.hpp"
+
+using namespace mmc;
+
+namespace {
+
+__device__ __forceinline__ unsigned cell_hash(int i, int j, int k, unsigned mask) {
+  return ((unsigned)i * 73856093u ^ (unsigned)j * 19349663u ^ (unsigned)k * 83492791u) & mask;
+}
+__device__ __forceinline__ float round_down(double v) {
+  float f = (float)v;
+  return ((double)f > v) ? nextafterf(f, -INFINITY) : f;
+}
+__device__ __forceinline__ float round_up(double v) {
+  float f = (float)v;
+  return ((double)f < v) ? nextafterf(f, INFINITY) : f;
+}
+__device__ __forceinline__ int cell_of(float v, double inv) { return (int)floor((double)v * inv); }
+
+template <class T>
+__device__ __forceinline__ v3<double> node_x(const double* x, int N, int i) {
+  return v3<double>{x[i], x[N + i], x[2 * N + i]};
+}
+template <class T>
+__device__ __forceinline__ quat<T> node_R(const T* R, int N, int i) {
+  return quat<T>{R[i], R[N + i], R[2 * N + i], R[3 * N + i]};
+}
+
+// ---------------------------------------------------------------------------
+// k_state: z (+ step p) -> x, R; gravity, platens, centroid.  MC/src/sim.cpp:136,142-143,168-171
+// partials [7][P]: E_gravity, E_platen, reaction, max_pen, platen_contacts, sum_x, sum_z
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_state(Lat<T> L, Scalars S, const double* __restrict__ z,
+                                                  const double* __restrict__ p, double step, double* __restrict__ zc,
+                                                  double* __restrict__ x, T* __restrict__ R,
+                                                  double* __restrict__ part) {
+  const int N = L.N, i = blockIdx.x * kBlock + threadIdx.x, P = gridDim.x;
+  double v[7] = {0, 0, 0, 0, 0, 0, 0};
+  if (i < N) {
+    const unsigned lk = L.lock[i];
+    double q[6];
+#pragma unroll
+    for (int d = 0; d < 6; d++) {
+      double zz = z[d * N + i];
+      if (p) zz = zz + step * p[d * N + i];
+      if ((lk >> d) & 1u) zz = 0.0;
+      q[d] = zz;
+      if (zc) zc[d * N + i] = zz;
+    }
+    v3<double> pos{L.bpos[i] + q[0] * S.ts, L.bpos[N + i] + q[1] * S.ts, L.bpos[2 * N + i] + q[2] * S.ts};
+    x[i] = pos.x;
+    x[N + i] = pos.y;
+    x[2 * N + i] = pos.z;
+    quat<double> b{L.bquat[i], L.bquat[N + i], L.bquat[2 * N + i], L.bquat[3 * N + i]};
+    quat<double> r = qnormalize(qmul(qexp(v3<double>{q[3], q[4], q[5]}), b));
+    R[i] = T(r.w);
+    R[N + i] = T(r.x);
+    R[2 * N + i] = T(r.y);
+    R[3 * N + i] = T(r.z);
+    const double m = L.mass[i];
+    v[0] = m * S.gravity * pos.y;
+    double pb = S.jr - pos.y;
+    if (pb > 0) {
+      v[1] += 0.5 * S.kc * pb * pb;
+      v[3] = fmax(v[3], pb);
+      v[4] += 1;
+    }
+    double pt = pos.y + S.jr - S.top;
+    if (pt > 0) {
+      v[1] += 0.5 * S.kc * pt * pt;
+      v[2] += S.kc * pt;
+      v[3] = fmax(v[3], pt);
+      v[4] += 1;
+    }
+    v[5] = pos.x;
+    v[6] = pos.z;
+  }
+  block_reduce<7>(v, 1u << 3);
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int k = 0; k < 7; k++) part[k * P + blockIdx.x] = v[k];
+}
+
+// ---------------------------------------------------------------------------
+// Broadphase: shapes 0..N-1 are joint spheres, N..N+B-1 strut capsules
+// (MC/src/sim.cpp:148-151).  Each shape stores a conservative float AABB padded
+// by r + margin; the same AABB drives counting, filling and the narrowphase.
+// ---------------------------------------------------------------------------
+template <class T>
+__device__ __forceinline__ void shape_geom(const Lat<T>& L, const Scalars& S, const double* x, int s, v3<double>& a,
+                                           v3<double>& b, double& r, int& n0, int& n1) {
+  const int N = L.N;
+  if (s < N) {
+    a = b = node_x<T>(x, N, s);
+    r = S.jr;
+    n0 = s;
+    n1 = -1;
+  } else {
+    int k = s - N;
+    n0 = L.ba[k];
+    n1 = L.bb[k];
+    a = node_x<T>(x, N, n0);
+    b = node_x<T>(x, N, n1);
+    r = L.rad[k];
+  }
+}
+
+constexpr int kMaxCellsPerShape = 512;
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_count(Lat<T> L, Scalars S, const double* __restrict__ x,
+                                                  float* __restrict__ box, unsigned* __restrict__ cnt,
+                                                  int* __restrict__ flags) {
+  const int NS = L.N + L.B, s = blockIdx.x * kBlock + threadIdx.x;
+  if (s >= NS) return;
+  v3<double> a, b;
+  double r;
+  int n0, n1;
+  shape_geom(L, S, x, s, a, b, r, n0, n1);
+  double pad = r + S.margin;
+  float lo[3] = {round_down(fmin(a.x, b.x) - pad), round_down(fmin(a.y, b.y) - pad), round_down(fmin(a.z, b.z) - pad)};
+  float hi[3] = {round_up(fmax(a.x, b.x) + pad), round_up(fmax(a.y, b.y) + pad), round_up(fmax(a.z, b.z) + pad)};
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    box[d * NS + s] = lo[d];
+    box[(3 + d) * NS + s] = hi[d];
+  }
+  int c0[3], c1[3];
+  bool ok = true;
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    ok = ok && isfinite(lo[d]) && isfinite(hi[d]);
+    c0[d] = cell_of(lo[d], S.inv_cell);
+    c1[d] = cell_of(hi[d], S.inv_cell);
+  }
+  long ncell = ok ? (long)(c1[0] - c0[0] + 1) * (c1[1] - c0[1] + 1) * (c1[2] - c0[2] + 1) : 0;
+  if (!ok || ncell > kMaxCellsPerShape) {
+    // A non-finite pose already makes the energy non-finite; an absurdly large
+    // box marks the evaluation non-finite (k_final) so that the line search
+    // rejects it.  Either way the shape is left out of the grid.
+    if (ok) atomicOr(flags, 1);
+    box[s] = INFINITY;
+    return;
+  }
+  for (int j = c0[1]; j <= c1[1]; j++)
+    for (int k = c0[2]; k <= c1[2]; k++)
+      for (int i = c0[0]; i <= c1[0]; i++) atomicAdd(&cnt[cell_hash(i, j, k, S.table_mask)], 1u);
+}
+
+__device__ __forceinline__ bool load_cells(const float* box, int NS, int s, double inv, float lo[3], float hi[3],
+                                           int c0[3], int c1[3]) {
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    lo[d] = box[d * NS + s];
+    hi[d] = box[(3 + d) * NS + s];
+  }
+  if (!(lo[0] < INFINITY)) return false;
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    c0[d] = cell_of(lo[d], inv);
+    c1[d] = cell_of(hi[d], inv);
+  }
+  return true;
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill(int NS, Scalars S, const float* __restrict__ box,
+                                                 unsigned* __restrict__ cnt, const unsigned* __restrict__ start,
+                                                 int* __restrict__ entries, int* __restrict__ flags) {
+  const int s = blockIdx.x * kBlock + threadIdx.x;
+  if (s >= NS) return;
+  float lo[3], hi[3];
+  int c0[3], c1[3];
+  if (!load_cells(box, NS, s, S.inv_cell, lo, hi, c0, c1)) return;
+  for (int j = c0[1]; j <= c1[1]; j++)
+    for (int k = c0[2]; k <= c1[2]; k++)
+      for (int i = c0[0]; i <= c1[0]; i++) {
+        unsigned h = cell_hash(i, j, k, S.table_mask);
+        unsigned slot = atomicSub(&cnt[h], 1u) - 1u;  // also returns cnt[] to zero
+        unsigned idx = start[h] + slot;
+        if (idx < (unsigned)S.entry_cap)
+          entries[idx] = s;
+        else
+          atomicOr(flags, 4);
+      }
+}
+
+// Orders each bucket ascending so that every shape visits its partners in a
+// fixed order (bit-reproducible contact sums).
+__global__ __launch_bounds__(kBlock) void k_sort(unsigned T, Scalars S, const unsigned* __restrict__ start,
+                                                 int* __restrict__ entries) {
+  const unsigned h = blockIdx.x * kBlock + threadIdx.x;
+  if (h >= T) return;
+  unsigned a = start[h], b = min(start[h + 1], (unsigned)S.entry_cap);
+  for (unsigned i = a + 1; i < b; i++) {
+    int v = entries[i];
+    unsigned j = i;
+    while (j > a && entries[j - 1] > v) {
+      entries[j] = entries[j - 1];
+      j--;
+    }
+    entries[j] = v;
+  }
+}
+
+template <class T>
+__device__ __forceinline__ bool adjacent(const Lat<T>& L, int u, int v) {
+  for (int e = L.adj_off[u]; e < L.adj_off[u + 1]; e++)
+    if (L.adj[e] == v) return true;
+  return false;
+}
+// localShapes, MC/src/sim.cpp:24-39
+template <class T>
+__device__ __forceinline__ bool excluded(const Lat<T>& L, int a0, int a1, int b0, int b1) {
+  int ea[2] = {a0, a1}, eb[2] = {b0, b1};
+  for (int i = 0; i < 2; i++) {
+    if (ea[i] < 0) continue;
+    for (int j = 0; j < 2; j++) {
+      if (eb[j] < 0) continue;
+      if (ea[i] == eb[j] || adjacent(L, ea[i], eb[j])) return true;
+    }
+  }
+  return false;
+}
+
+// closestShapes, MC/src/sim.cpp:17-22 (A is the lower shape index)
+__device__ __forceinline__ closest shape_closest(bool capA, v3<double> a0, v3<double> a1, bool capB, v3<double> b0,
+                                                 v3<double> b1) {
+  if (capA && capB) return seg_seg(a0, a1, b0, b1);
+  if (capA) {
+    v3<double> d = a1 - a0;
+    double s = sat(dot(b0 - a0, d) / fmax(1e-30, norm2(d)));
+    return {s, 0, a0 + d * s, b0};
+  }
+  if (capB) {
+    v3<double> d = b1 - b0;
+    double t = sat(dot(a0 - b0, d) / fmax(1e-30, norm2(d)));
+    return {0, t, a0, b0 + d * t};
+  }
+  return {0, 0, a0, b0};
+}
+
+// k_narrow: MC/src/sim.cpp:159-165.  partials [3][P]: E_contact, max_pen, contacts
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_narrow(Lat<T> L, Scalars S, const double* __restrict__ x,
+                                                   const float* __restrict__ box,
+                                                   const unsigned* __restrict__ start,
+                                                   const int* __restrict__ entries, double* __restrict__ cf_sph,
+                                                   double* __restrict__ cf_cap, double* __restrict__ part) {
+  const int N = L.N, B = L.B, NS = N + B, s = blockIdx.x * kBlock + threadIdx.x, P = gridDim.x;
+  double v[3] = {0, 0, 0};
+  if (s < NS) {
+    v3<double> f0{0, 0, 0}, f1{0, 0, 0};
+    float lo[3], hi[3];
+    int c0[3], c1[3];
+    v3<double> sa, sb;
+    double sr;
+    int s0, s1;
+    shape_geom(L, S, x, s, sa, sb, sr, s0, s1);
+    const bool scap = s >= N;
+    if (load_cells(box, NS, s, S.inv_cell, lo, hi, c0, c1)) {
+      for (int cj = c0[1]; cj <= c1[1]; cj++)
+        for (int ck = c0[2]; ck <= c1[2]; ck++)
+          for (int ci = c0[0]; ci <= c1[0]; ci++) {
+            unsigned h = cell_hash(ci, cj, ck, S.table_mask);
+            unsigned e0 = start[h], e1 = min(start[h + 1], (unsigned)S.entry_cap);
+            for (unsigned e = e0; e < e1; e++) {
+              int o = entries[e];
+              if (o == s) continue;
+              float olo[3], ohi[3];
+#pragma unroll
+              for (int d = 0; d < 3; d++) {
+                olo[d] = box[d * NS + o];
+                ohi[d] = box[(3 + d) * NS + o];
+              }
+              if (!(olo[0] <= hi[0] && lo[0] <= ohi[0] && olo[1] <= hi[1] && lo[1] <= ohi[1] && olo[2] <= hi[2] &&
+                    lo[2] <= ohi[2]))
+                continue;
+              // home cell of the pair: the cell holding the low corner of the overlap
+              if (cell_of(fmaxf(lo[0], olo[0]), S.inv_cell) != ci || cell_of(fmaxf(lo[1], olo[1]), S.inv_cell) != cj ||
+                  cell_of(fmaxf(lo[2], olo[2]), S.inv_cell) != ck)
+                continue;
+              v3<double> oa, ob;
+              double orr;
+              int o0, o1;
+              shape_geom(L, S, x, o, oa, ob, orr, o0, o1);
+              if (excluded(L, s0, s1, o0, o1)) continue;
+              const bool low = s < o;
+              const bool ocap = o >= N;
+              closest cp = low ? shape_closest(scap, sa, sb, ocap, oa, ob) : shape_closest(ocap, oa, ob, scap, sa, sb);
+              v3<double> delta = cp.pb - cp.pa;
+              double dist = norm(delta);
+              double target = (low ? sr + orr : orr + sr) + S.margin;
+              if (dist >= target) continue;
+              int ia = low ? s : o, ib = low ? o : s;
+              v3<double> n = dist > 1e-12 ? over(delta, dist)
+                                          : unit(v3<double>{0.37 + 0.001 * ia, 0.73, -0.41 + 0.001 * ib});
+              double pen = target - dist, mag = S.kc * pen;
+              v3<double> fv = low ? n * mag : n * (-mag);
+              double u = low ? cp.s : cp.t;
+              if (scap) {
+                f0 = f0 + fv * (1 - u);
+                f1 = f1 + fv * u;
+              } else {
+                f0 = f0 + fv;
+              }
+              if (low) {
+                v[0] += 0.5 * S.kc * pen * pen;
+                v[1] = fmax(v[1], pen);
+                v[2] += 1;
+              }
+            }
+          }
+    }
+    if (scap) {
+      int k = s - N;
+      cf_cap[k] = f0.x;
+      cf_cap[B + k] = f0.y;
+      cf_cap[2 * B + k] = f0.z;
+      cf_cap[3 * B + k] = f1.x;
+      cf_cap[4 * B + k] = f1.y;
+      cf_cap[5 * B + k] = f1.z;
+    } else {
+      cf_sph[s] = f0.x;
+      cf_sph[N + s] = f0.y;
+      cf_sph[2 * N + s] = f0.z;
+    }
+  }
+  block_reduce<3>(v, 1u << 1);
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int k = 0; k < 3; k++) part[k * P + blockIdx.x] = v[k];
+}
+
+// ---------------------------------------------------------------------------
+// k_beam: per-beam energy, gradient record and trial state, MC/src/sim.cpp:139-141.
+// rec [9][B]: f (force on b, -f on a), rotational gradient of a, of b.
+// partials [3][P]: energy, elastic, plastic.
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_beam(Lat<T> L, Material M, const double* __restrict__ x,
+                                                 const T* __restrict__ R, double* __restrict__ rec,
+                                                 double* __restrict__ trial, double* __restrict__ part) {
+  const int N = L.N, B = L.B, k = blockIdx.x * kBlock + threadIdx.x, P = gridDim.x;
+  double v[3] = {0, 0, 0};
+  if (k < B) {
+    const int ia = L.ba[k], ib = L.bb[k];
+    v3<double> d = node_x<T>(x, N, ib) - node_x<T>(x, N, ia);
+    double Ln = norm(d);
+    double out[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    double tr[MMC_TRIAL_STRIDE];
+#pragma unroll
+    for (int c = 0; c < MMC_TRIAL_STRIDE; c++) tr[c] = 0;
+    if (Ln < 1e-10) {
+      v[0] = 1e8 * ((1e-10 - Ln) * (1e-10 - Ln));
+    } else {
+      const double L0 = L.L0[k], r = L.rad[k], A = L.area[k], I = L.inert[k], J = 2 * I;
+      double pl[7];
+#pragma unroll
+      for (int c = 0; c < 7; c++) pl[c] = L.plastic[c * B + k];
+      v3<double> t = over(d, Ln);
+      double eps = (Ln - L0) / L0;
+      ret1 ax = return1(eps, pl[0], pl[1], M.E, M.sy, M.H);
+      double uax = A * L0 * (ax.energy + ax.hard), dax = A * L0 * M.sy * ax.dg;
+      v3<double> f = t * (A * ax.force);
+      // shear: material directors against the chord
+      quat<T> Ra = node_R<T>(R, N, ia), Rb = node_R<T>(R, N, ib);
+      v3<T> e0{T(L.e0[k]), T(L.e0[B + k]), T(L.e0[2 * B + k])};
+      v3<T> tT = cvt<T>(t);
+      v3<T> da = qrot(Ra, e0), db = qrot(Rb, e0);
+      T cs = T(0.5 * M.shear * M.G * A * L0);
+      v3<T> ma = da - tT, mb = db - tT;
+      double us = double(T(0.5) * cs * (norm2(ma) + norm2(mb)));
+      v3<T> gt = ((tT * T(2) - da) - db) * cs;
+      v3<T> trans = over(gt - tT * dot(gt, tT), T(Ln));
+      f = f + cvt<double>(trans);
+      v3<T> ga = -(cross(da, tT) * cs), gb = -(cross(db, tT) * cs);
+      // bending + torsion from the relative rotation
+      quat<T> Q = qnormalize(qmul(qconj(Ra), Rb));
+      v3<T> phiT = qlog(Q), axT = unit(qrotinv(Ra, tT));
+      v3<double> phi = cvt<double>(phiT), axis = cvt<double>(axT);
+      v3<double> rotp{pl[2], pl[3], pl[4]};
+      double tw = dot(phi, axis), ptw = dot(rotp, axis);
+      v3<double> bend = phi - axis * tw, pb = rotp - axis * ptw;
+      double kb = M.E * I / L0, kt = M.G * J / L0, my = M.sy * I / r;
+      double ty = (M.sy / sqrt(3.0)) * J / r;
+      ret3 br = return3(bend, pb, pl[5], kb, my, M.hb_frac * kb);
+      ret1 to = return1(tw, ptw, pl[6], kt, ty, M.ht_frac * kt);
+      v3<double> nrotp = br.p + axis * to.p;
+      v3<double> mom = br.force + axis * to.force;
+      double ur = br.energy + br.hard + to.energy + to.hard, dr = my * br.dg + ty * to.dg;
+      v3<T> world = qrot(Ra, jlinvt(phiT, cvt<T>(mom)));
+      ga = ga - world;
+      gb = gb + world;
+      v[0] = (uax + dax) + us + (ur + dr);
+      v[1] = uax + us + ur;
+      v[2] = dax + dr;
+      out[0] = f.x;
+      out[1] = f.y;
+      out[2] = f.z;
+      out[3] = double(ga.x);
+      out[4] = double(ga.y);
+      out[5] = double(ga.z);
+      out[6] = double(gb.x);
+      out[7] = double(gb.y);
+      out[8] = double(gb.z);
+      if (trial) {
+        tr[0] = ax.p;
+        tr[1] = ax.a;
+        tr[2] = nrotp.x;
+        tr[3] = nrotp.y;
+        tr[4] = nrotp.z;
+        tr[5] = br.a;
+        tr[6] = to.a;
+        tr[7] = eps;
+        tr[8] = ax.force;
+        tr[9] = mom.x;
+        tr[10] = mom.y;
+        tr[11] = mom.z;
+        tr[12] = uax + us + ur;
+        tr[13] = dax + dr;
+        tr[14] = ax.yielded;
+        tr[15] = br.yielded;
+        tr[16] = to.yielded;
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < 9; c++) rec[c * B + k] = out[c];
+    if (trial)
+#pragma unroll
+      for (int c = 0; c < MMC_TRIAL_STRIDE; c++) trial[c * B + k] = tr[c];
+  }
+  block_reduce<3>(v, 0u);
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int c = 0; c < 3; c++) part[c * P + blockIdx.x] = v[c];
+}
+
+// ---------------------------------------------------------------------------
+// k_gather: per-node assembly + pack, MC/src/sim.cpp:139-143,164,168-172.
+// partials [2][P]: sum of squares and max |g| over unlocked DOFs.
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_gather(Lat<T> L, Scalars S, const double* __restrict__ x,
+                                                   const double* __restrict__ rec, const double* __restrict__ cf_sph,
+                                                   const double* __restrict__ cf_cap,
+                                                   const double* __restrict__ spart, int Ps,
+                                                   double* __restrict__ g, double* __restrict__ part) {
+  const int N = L.N, B = L.B, i = blockIdx.x * kBlock + threadIdx.x, P = gridDim.x;
+  double cen[2];
+  {
+    double t[2];
+    reduce_partials<2>(spart + 5 * Ps, Ps, t, 0u);
+    cen[0] = t[0] * (1.0 / N);
+    cen[1] = t[1] * (1.0 / N);
+  }
+  double v[2] = {0, 0};
+  if (i < N) {
+    v3<double> gx{0, 0, 0}, gr{0, 0, 0};
+    for (int e = L.inc_off[i]; e < L.inc_off[i + 1]; e++) {
+      const int code = L.inc[e], k = code >> 1, side = code & 1;
+      v3<double> f{rec[k], rec[B + k], rec[2 * B + k]};
+      gx = side ? gx + f : gx - f;
+      const int o = 3 + 3 * side;
+      gr = gr + v3<double>{rec[o * B + k], rec[(o + 1) * B + k], rec[(o + 2) * B + k]};
+      const int c = 3 * side;
+      gx = gx + v3<double>{cf_cap[c * B + k], cf_cap[(c + 1) * B + k], cf_cap[(c + 2) * B + k]};
+    }
+    gx = gx + v3<double>{cf_sph[i], cf_sph[N + i], cf_sph[2 * N + i]};
+    gx.y += L.mass[i] * S.gravity;
+    gx.x += S.centering * cen[0] / N;
+    gx.z += S.centering * cen[1] / N;
+    const double y = x[N + i];
+    double pb = S.jr - y;
+    if (pb > 0) gx.y -= S.kc * pb;
+    double pt = y + S.jr - S.top;
+    if (pt > 0) gx.y += S.kc * pt;
+    double gv[6] = {S.ts * gx.x, S.ts * gx.y, S.ts * gx.z, gr.x, gr.y, gr.z};
+    const unsigned lk = L.lock[i];
+#pragma unroll
+    for (int d = 0; d < 6; d++) {
+      double gd = ((lk >> d) & 1u) ? 0.0 : gv[d];
+      g[d * N + i] = gd;
+      if (!((lk >> d) & 1u)) {
+        v[0] += gd * gd;
+        v[1] = fmax(v[1], fabs(gd));
+      }
+    }
+  }
+  block_reduce<2>(v, 1u << 1);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = v[0];
+    part[P + blockIdx.x] = v[1];
+  }
+}
+
+// Record layout (doubles): 0 energy, 1 elastic, 2 plastic, 3 contact, 4 reaction,
+// 5 max_pen, 6 rms, 7 max, 8 contacts, 9 finite, 10 flags
+constexpr int kRec = 12;
+
+__global__ __launch_bounds__(kBlock) void k_final(int N, double centering, double n_free,
+                                                  const double* __restrict__ spart, int Ps,
+                                                  const double* __restrict__ cpart, int Pc,
+                                                  const double* __restrict__ bpart, int Pb,
+                                                  const double* __restrict__ gpart, int Pg,
+                                                  int* __restrict__ flags, double* __restrict__ out) {
+  double s[7], c[3], b[3], g[2];
+  reduce_partials<7>(spart, Ps, s, 1u << 3);
+  reduce_partials<3>(cpart, Pc, c, 1u << 1);
+  reduce_partials<3>(bpart, Pb, b, 0u);
+  reduce_partials<2>(gpart, Pg, g, 1u << 1);
+  if (threadIdx.x == 0) {
+    const int fl = flags[0];
+    flags[0] = 0;
+    double cx = s[5] * (1.0 / N), cz = s[6] * (1.0 / N);
+    double ecen = 0.5 * centering * (cx * cx + cz * cz);
+    double energy = b[0] + s[0] + ecen + c[0] + s[1];
+    bool fin = isfinite(energy) && !(fl & 1);
+    out[0] = fin ? energy : INFINITY;
+    out[1] = b[1];
+    out[2] = b[2];
+    out[3] = c[0] + s[1];
+    out[4] = s[2];
+    out[5] = fmax(s[3], c[1]);
+    out[6] = fin ? (n_free > 0 ? sqrt(g[0] / n_free) : 0.0) : 0.0;
+    out[7] = fin ? g[1] : 0.0;
+    out[8] = s[4] + c[2];
+    out[9] = fin ? 1.0 : 0.0;
+    out[10] = (double)fl;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// beamPotential, MC/src/sim.cpp:468-509 (objectivity check)
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_beam_potential(Lat<T> L, Material M, const double* __restrict__ x,
+                                                           const double* __restrict__ R, double* __restrict__ part) {
+  const int N = L.N, B = L.B, k = blockIdx.x * kBlock + threadIdx.x, P = gridDim.x;
+  double v[1] = {0};
+  if (k < B) {
+    const int ia = L.ba[k], ib = L.bb[k];
+    v3<double> d = node_x<T>(x, N, ib) - node_x<T>(x, N, ia);
+    double Ln = fmax(1e-12, norm(d));
+    v3<double> t = over(d, Ln);
+    const double L0 = L.L0[k], r = L.rad[k], A = L.area[k], I = L.inert[k], J = 2 * I;
+    double pl[7];
+    for (int c = 0; c < 7; c++) pl[c] = L.plastic[c * B + k];
+    double eps = (Ln - L0) / L0;
+    ret1 ax = return1(eps, pl[0], pl[1], M.E, M.sy, M.H);
+    double total = A * L0 * (ax.energy + ax.hard);
+    quat<double> Ra = node_R<double>(R, N, ia), Rb = node_R<double>(R, N, ib);
+    v3<double> e0{L.e0[k], L.e0[B + k], L.e0[2 * B + k]};
+    v3<double> da = qrot(Ra, e0), db = qrot(Rb, e0);
+    double cs = 0.5 * M.shear * M.G * A * L0;
+    total += 0.5 * cs * (norm2(da - t) + norm2(db - t));
+    quat<double> Q = qnormalize(qmul(qconj(Ra), Rb));
+    v3<double> phi = qlog(Q), axis = unit(qrotinv(Ra, t));
+    v3<double> rotp{pl[2], pl[3], pl[4]};
+    double tw = dot(phi, axis), ptw = dot(rotp, axis);
+    v3<double> bend = phi - axis * tw, pb = rotp - axis * ptw;
+    double kb = M.E * I / L0, kt = M.G * J / L0, my = M.sy * I / r, ty = (M.sy / sqrt(3.0)) * J / r;
+    ret3 br = return3(bend, pb, pl[5], kb, my, M.hb_frac * kb);
+    ret1 to = return1(tw, ptw, pl[6], kt, ty, M.ht_frac * kt);
+    v[0] = total + (br.energy + br.hard + to.energy + to.hard);
+  }
+  block_reduce<1>(v, 0u);
+  if (threadIdx.x == 0) part[blockIdx.x] = v[0];
+}
+
+__global__ __launch_bounds__(kBlock) void k_sum1(const double* __restrict__ part, int P, double* __restrict__ out) {
+  double s[1];
+  reduce_partials<1>(part, P, s, 0u);
+  if (threadIdx.x == 0) out[0] = s[0];
+}
+
+// ---------------------------------------------------------------------------
+// L-BFGS vector kernels (thread per node, 6 DOF planes).  MC/src/sim.cpp:43-46,184-191,205-206
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double plane_dot(const double* a, const double* b, int N, int i) {
+  double s = 0;
+  if (i < N)
+#pragma unroll
+    for (int d = 0; d < 6; d++) s += a[d * N + i] * b[d * N + i];
+  return s;
+}
+__device__ __forceinline__ void write_dot_partial(double v, double* part) {
+  double t[1] = {v};
+  block_reduce<1>(t, 0u);
+  if (threadIdx.x == 0) part[blockIdx.x] = t[0];
+}
+
+// p = src * scale; partial dot(D, p) when D is given
+__global__ __launch_bounds__(kBlock) void k_tl_init(int N, const double* __restrict__ src, double scale,
+                                                    double* __restrict__ p, const double* __restrict__ D,
+                                                    double* __restrict__ part) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < N)
+#pragma unroll
+    for (int d = 0; d < 6; d++) p[d * N + i] = src[d * N + i] * scale;
+  if (D) write_dot_partial(plane_dot(D, p, N, i), part);
+}
+
+// backward step h: alpha_h = rho_h * (s_h . p);  p -= alpha_h y_h;  (last: p *= H0)
+// writes alpha_h to alpha[h]; partial dot(D, p) for the next step
+__global__ __launch_bounds__(kBlock) void k_tl_back(int N, double* __restrict__ p, const double* __restrict__ Y,
+                                                    const double* __restrict__ rho, int h,
+                                                    const double* __restrict__ part_in, int Pin,
+                                                    double* __restrict__ alpha, double post,
+                                                    const double* __restrict__ D, double* __restrict__ part) {
+  double sdp[1];
+  reduce_partials<1>(part_in, Pin, sdp, 0u);
+  const double al = rho[h] * sdp[0];
+  if (blockIdx.x == 0 && threadIdx.x == 0) alpha[h] = al;
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < N)
+#pragma unroll
+    for (int d = 0; d < 6; d++) {
+      double v = p[d * N + i] - al * Y[d * N + i];
+      p[d * N + i] = v * post;
+    }
+  write_dot_partial(plane_dot(D, p, N, i), part);
+}
+
+// Direction epilogue: p = -p, zero locked DOFs, clamp per node (clampDirection,
+// MC/src/sim.cpp:46), partial dot(g, p).
+__device__ __forceinline__ void finish_node(double (&q)[6], unsigned lk, double mt, double mr) {
+#pragma unroll
+  for (int d = 0; d < 6; d++)
+    if ((lk >> d) & 1u) q[d] = 0.0;
+  double lt = sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2]);
+  if (lt > mt) {
+    double s = mt / lt;
+    q[0] *= s;
+    q[1] *= s;
+    q[2] *= s;
+  }
+  double lr = sqrt(q[3] * q[3] + q[4] * q[4] + q[5] * q[5]);
+  if (lr > mr) {
+    double s = mr / lr;
+    q[3] *= s;
+    q[4] *= s;
+    q[5] *= s;
+  }
+#pragma unroll
+  for (int d = 0; d < 6; d++)
+    if ((lk >> d) & 1u) q[d] = 0.0;
+}
+
+// forward step h: beta = rho_h (y_h . p); p += s_h (alpha_h - beta); partial dot(D, p).
+// With finish: then p = -p, lock, clamp and partial dot(g, p) instead.
+__global__ __launch_bounds__(kBlock) void k_tl_fwd(int N, double* __restrict__ p, const double* __restrict__ Sh,
+                                                   const double* __restrict__ rho, const double* __restrict__ alpha,
+                                                   int h, const double* __restrict__ part_in, int Pin,
+                                                   const double* __restrict__ D, int finish,
+                                                   const uint8_t* __restrict__ lock, const double* __restrict__ g,
+                                                   double mt, double mr, double* __restrict__ part) {
+  double ydp[1];
+  reduce_partials<1>(part_in, Pin, ydp, 0u);
+  const double coef = alpha[h] - rho[h] * ydp[0];
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  double dv = 0;
+  if (i < N) {
+    double q[6];
+#pragma unroll
+    for (int d = 0; d < 6; d++) q[d] = p[d * N + i] + Sh[d * N + i] * coef;
+    if (finish) {
+#pragma unroll
+      for (int d = 0; d < 6; d++) q[d] = -q[d];
+      finish_node(q, lock[i], mt, mr);
+    }
+#pragma unroll
+    for (int d = 0; d < 6; d++) p[d * N + i] = q[d];
+    const double* Dv = finish ? g : D;
+    if (Dv)
+#pragma unroll
+      for (int d = 0; d < 6; d++) dv += Dv[d * N + i] * q[d];
+  }
+  if (finish || D) write_dot_partial(dv, part);
+}
+
+// p = -(src * scale), lock, clamp; partial dot(g, p)  (H0 direction with an
+// empty history, or the steepest-descent reset)
+__global__ __launch_bounds__(kBlock) void k_tl_scaled(int N, const double* __restrict__ src, double scale,
+                                                      double* __restrict__ p, const uint8_t* __restrict__ lock,
+                                                      const double* __restrict__ g, double mt, double mr,
+                                                      double* __restrict__ part) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  double dv = 0;
+  if (i < N) {
+    double q[6];
+#pragma unroll
+    for (int d = 0; d < 6; d++) q[d] = -(src[d * N + i] * scale);
+    finish_node(q, lock[i], mt, mr);
+#pragma unroll
+    for (int d = 0; d < 6; d++) {
+      p[d * N + i] = q[d];
+      dv += g[d * N + i] * q[d];
+    }
+  }
+  write_dot_partial(dv, part);
+}
+
+// s = zc - z, y = gc - g into the history slot; partials [3][P]: s.y, s.s, y.y
+__global__ __launch_bounds__(kBlock) void k_pair(int N, const double* __restrict__ z, const double* __restrict__ zc,
+                                                 const double* __restrict__ g, const double* __restrict__ gc,
+                                                 double* __restrict__ S, double* __restrict__ Y,
+                                                 double* __restrict__ part) {
+  const int i = blockIdx.x * kBlock + threadIdx.x, P = gridDim.x;
+  double v[3] = {0, 0, 0};
+  if (i < N)
+#pragma unroll
+    for (int d = 0; d < 6; d++) {
+      double s = zc[d * N + i] - z[d * N + i], y = gc[d * N + i] - g[d * N + i];
+      S[d * N + i] = s;
+      Y[d * N + i] = y;
+      v[0] += s * y;
+      v[1] += s * s;
+      v[2] += y * y;
+    }
+  block_reduce<3>(v, 0u);
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int k = 0; k < 3; k++) part[k * P + blockIdx.x] = v[k];
+}
+
+__global__ __launch_bounds__(kBlock) void k_sum3(const double* __restrict__ part, int P, double* __restrict__ out) {
+  double s[3];
+  reduce_partials<3>(part, P, s, 0u);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 3; k++) out[k] = s[k];
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill_zero(double* __restrict__ a, long n) {
+  long i = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) a[i] = 0.0;
+}
+
+// commit, MC/src/sim.cpp:212: base pose <- state(z); plastic <- trial.next
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_commit_nodes(int N, const double* __restrict__ x, const T* __restrict__ R,
+                                                         double* __restrict__ bpos, double* __restrict__ bquat) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= N) return;
+  for (int d = 0; d < 3; d++) bpos[d * N + i] = x[d * N + i];
+  for (int d = 0; d < 4; d++) bquat[d * N + i] = double(R[d * N + i]);
+}
+__global__ __launch_bounds__(kBlock) void k_commit_beams(int B, const double* __restrict__ trial,
+                                                         double* __restrict__ plastic) {
+  const int k = blockIdx.x * kBlock + threadIdx.x;
+  if (k >= B) return;
+  for (int c = 0; c < 7; c++) plastic[c * B + k] = trial[c * B + k];
+}
+
+// diagnostics reductions, MC/src/sim.cpp:339-370.
+// partials [8][P]: sum|epsp|, max|epsp|, sum|rotp|, max|rotp|, #plastic beams,
+// #hinges, min y, max y (min stored negated so that all maxima share one op)
+__global__ __launch_bounds__(kBlock) void k_stats(int N, int B, const double* __restrict__ plastic,
+                                                  const double* __restrict__ bpos, double* __restrict__ part) {
+  const int i = blockIdx.x * kBlock + threadIdx.x, P = gridDim.x;
+  double v[8] = {0, 0, 0, 0, 0, 0, -INFINITY, -INFINITY};
+  if (i < B) {
+    double pa = fabs(plastic[i]);
+    double pr = sqrt(plastic[2 * B + i] * plastic[2 * B + i] + plastic[3 * B + i] * plastic[3 * B + i] +
+                     plastic[4 * B + i] * plastic[4 * B + i]);
+    v[0] = pa;
+    v[1] = pa;
+    v[2] = pr;
+    v[3] = pr;
+    v[4] = pa > 1e-5 ? 1 : 0;
+    v[5] = pr > 1e-4 ? 1 : 0;
+  }
+  if (i < N) {
+    v[6] = -bpos[N + i];
+    v[7] = bpos[N + i];
+  }
+  block_reduce<8>(v, (1u << 1) | (1u << 3) | (1u << 6) | (1u << 7));
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 8; k++) part[k * P + blockIdx.x] = v[k];
+}
+__global__ __launch_bounds__(kBlock) void k_stats_final(const double* __restrict__ part, int P,
+                                                        double* __restrict__ out) {
+  double s[8];
+  reduce_partials<8>(part, P, s, (1u << 1) | (1u << 3) | (1u << 6) | (1u << 7));
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 8; k++) out[k] = s[k];
+}
+
+}  // namespace
+
+// ===========================================================================
+// Host side: the context
+// ===========================================================================
+#define HIPCALL(expr)                                                                        \
+  do {                                                                                       \
+    hipError_t e_ = (expr);                                                                  \
+    if (e_ != hipSuccess) {                                                                  \
+      throw HipFail(std::string(#expr) + ": " + hipGetErrorString(e_));                      \
+    }                                                                                        \
+  } while (0)
+
+namespace {
+struct HipFail : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+struct Fail : std::runtime_error {
+  int code;
+  Fail(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+std::string g_create_error;
+
+int blocks(long n) { return int((n + kBlock - 1) / kBlock); }
+
+template <class X>
+X* dalloc(size_t n) {
+  X* p = nullptr;
+  if (n == 0) n = 1;
+  HIPCALL(hipMalloc(&p, n * sizeof(X)));
+  return p;
+}
+}  // namespace
+
+struct mmc_ctx {
+  std::string err;
+  int precision = MMC_FP64;
+  int N = 0, B = 0, NS = 0;
+  mmc_config cfg{};
+  double jr = 0, n_free = 0;
+  hipStream_t stream = nullptr;
+  Material M{};
+  Scalars S{};
+  // device buffers
+  double *bpos = nullptr, *bquat = nullptr, *mass = nullptr;
+  uint8_t* lock = nullptr;
+  int *ba = nullptr, *bb = nullptr;
+  double *L0 = nullptr, *rad = nullptr, *area = nullptr, *inert = nullptr, *e0 = nullptr, *plastic = nullptr;
+  int *inc_off = nullptr, *inc = nullptr, *adj_off = nullptr, *adj = nullptr;
+  double *x = nullptr;
+  void* R = nullptr;  // double or float planes
+  double *rec = nullptr, *trial = nullptr, *cf_sph = nullptr, *cf_cap = nullptr;
+  float* box = nullptr;
+  unsigned *cnt = nullptr, *start = nullptr;
+  int* entries = nullptr;
+  int* flags = nullptr;
+  void* scan_tmp = nullptr;
+  size_t scan_bytes = 0;
+  unsigned table = 0;
+  double *spart = nullptr, *cpart = nullptr, *bpart = nullptr, *gpart = nullptr, *vpart = nullptr;
+  double* stpart = nullptr;
+  int Ps = 0, Pc = 0, Pb = 0;
+  double* record = nullptr;  // kRec per slot, slots: 0 current, 1 candidate, 2 scratch
+  double* scal = nullptr;    // misc scalars (dots)
+  double *z = nullptr, *g = nullptr, *p = nullptr, *zc = nullptr, *gc = nullptr, *zt = nullptr, *zin = nullptr;
+  double *Sh = nullptr, *Yh = nullptr;  // [m][6N]
+  double *rho = nullptr, *alpha = nullptr;
+  double* host = nullptr;  // pinned readback
+  bool have_trial = false;
+  int evaluations = 0;
+  int64_t total_evaluations = 0;
+  // kernel timer
+  int timed = -1;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int64_t timed_launches = 0;
+  double timed_ms = 0;
+  void tick(int id) {
+    if (id == timed) HIPCALL(hipEventRecord(ev0, stream));
+  }
+  void tock(int id) {
+    if (id == timed) HIPCALL(hipEventRecord(ev1, stream));
+  }
+
+  Lat<double> lat64() const {
+    return Lat<double>{N, B, bpos, bquat, mass, lock, ba, bb, L0, rad, area, inert, e0, plastic, inc_off, inc, adj_off, adj};
+  }
+  Lat<float> lat32() const {
+    return Lat<float>{N, B, bpos, bquat, mass, lock, ba, bb, L0, rad, area, inert, e0, plastic, inc_off, inc, adj_off, adj};
+  }
+  ~mmc_ctx();
+  void setup(const mmc_lattice* L, const mmc_config* c, int prec);
+  void ensure_entries(size_t need);
+  void evaluate(const double* zsrc, const double* psrc, double step, double* zdst, double* gdst, double top,
+                bool trials, double* rec_out);
+  template <class T>
+  void evaluate_t(Lat<T> L, const double* zsrc, const double* psrc, double step, double* zdst, double* gdst,
+                  double top, bool trials, double* rec_out);
+  void read(void* dst, const void* src, size_t bytes) {
+    HIPCALL(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream));
+    HIPCALL(hipStreamSynchronize(stream));
+  }
+  void zero(double* a, long n) { k_fill_zero<<<blocks(n), kBlock, 0, stream>>>(a, n); }
+};
+
+mmc_ctx::~mmc_ctx() {
+  void* ptrs[] = {bpos, bquat, mass, lock, ba, bb, L0, rad, area, inert, e0, plastic, inc_off, inc, adj_off, adj, x, R,
+                  rec, trial, cf_sph, cf_cap, box, cnt, start, entries, flags, scan_tmp, spart, cpart, bpart, gpart,
+                  vpart, stpart, record, scal, z, g, p, zc, gc, zt, zin, Sh, Yh, rho, alpha};
+  for (void* q : ptrs)
+    if (q) (void)hipFree(q);
+  if (host) (void)hipHostFree(host);
+  if (ev0) (void)hipEventDestroy(ev0);
+  if (ev1) (void)hipEventDestroy(ev1);
+  if (stream) (void)hipStreamDestroy(stream);
+}
+
+void mmc_ctx::setup(const mmc_lattice* L, const mmc_config* c, int prec) {
+  cfg = *c;
+  cfg.translation_scale = cfg.cell_size;  // MC/src/sim.cpp:49
+  precision = prec;
+  N = L->n_nodes;
+  B = L->n_beams;
+  NS = N + B;
+  if (N <= 0 || B <= 0) throw Fail(MMC_ERR_ARG, "lattice has no nodes or beams");
+  if (cfg.history < 0 || cfg.history > 64) throw Fail(MMC_ERR_ARG, "history must be in [0, 64]");
+  jr = cfg.joint_scale * cfg.max_radius;
+  HIPCALL(hipStreamCreate(&stream));
+  const mmc_material& m = cfg.material;
+  M = Material{m.young, m.young / (2 * (1 + m.nu)), m.yield, m.hardening, m.bend_hardening, m.torsion_hardening,
+               cfg.shear_correction, m.density};
+  // ---- host-side SoA staging ----
+  std::vector<double> h(7 * size_t(std::max(N, B)));
+  auto up = [&](double* dst, const double* src, size_t n) {
+    HIPCALL(hipMemcpy(dst, src, n * sizeof(double), hipMemcpyHostToDevice));
+  };
+  auto soa = [&](const double* aos, int n, int k) {
+    for (int i = 0; i < n; i++)
+      for (int d = 0; d < k; d++) h[size_t(d) * n + i] = aos[size_t(k) * i + d];
+  };
+  bpos = dalloc<double>(3 * size_t(N));
+  soa(L->x, N, 3);
+  up(bpos, h.data(), 3 * size_t(N));
+  bquat = dalloc<double>(4 * size_t(N));
+  if (L->q) {
+    soa(L->q, N, 4);
+  } else {
+    for (int i = 0; i < N; i++) {
+      h[i] = 1;
+      h[N + i] = h[2 * N + i] = h[3 * N + i] = 0;
+    }
+  }
+  up(bquat, h.data(), 4 * size_t(N));
+  mass = dalloc<double>(N);
+  up(mass, L->mass, N);
+  std::vector<uint8_t> lk(N, 0);
+  n_free = 0;
+  for (int i = 0; i < N; i++)
+    for (int d = 0; d < 6; d++) {
+      if (L->locked && L->locked[6 * i + d]) lk[i] |= uint8_t(1u << d);
+      else n_free += 1;
+    }
+  lock = dalloc<uint8_t>(N);
+  HIPCALL(hipMemcpy(lock, lk.data(), N, hipMemcpyHostToDevice));
+  for (int k = 0; k < B; k++)
+    if (L->beam_a[k] < 0 || L->beam_a[k] >= N || L->beam_b[k] < 0 || L->beam_b[k] >= N)
+      throw Fail(MMC_ERR_ARG, "beam endpoint out of range");
+  ba = dalloc<int>(B);
+  bb = dalloc<int>(B);
+  HIPCALL(hipMemcpy(ba, L->beam_a, B * sizeof(int), hipMemcpyHostToDevice));
+  HIPCALL(hipMemcpy(bb, L->beam_b, B * sizeof(int), hipMemcpyHostToDevice));
+  L0 = dalloc<double>(B);
+  up(L0, L->L0, B);
+  rad = dalloc<double>(B);
+  up(rad, L->radius, B);
+  area = dalloc<double>(B);
+  up(area, L->area, B);
+  inert = dalloc<double>(B);
+  up(inert, L->inertia, B);
+  e0 = dalloc<double>(3 * size_t(B));
+  soa(L->e0, B, 3);
+  up(e0, h.data(), 3 * size_t(B));
+  plastic = dalloc<double>(7 * size_t(B));
+  if (L->plastic) {
+    soa(L->plastic, B, 7);
+  } else {
+    std::fill(h.begin(), h.begin() + 7 * size_t(B), 0.0);
+  }
+  up(plastic, h.data(), 7 * size_t(B));
+  // ---- CSR beam incidence, ordered by beam index ----
+  std::vector<int> off(N + 1, 0), list(2 * size_t(B));
+  for (int k = 0; k < B; k++) {
+    off[L->beam_a[k] + 1]++;
+    off[L->beam_b[k] + 1]++;
+  }
+  for (int i = 0; i < N; i++) off[i + 1] += off[i];
+  {
+    std::vector<int> cur(off.begin(), off.end() - 1);
+    for (int k = 0; k < B; k++) {
+      list[cur[L->beam_a[k]]++] = 2 * k;
+      list[cur[L->beam_b[k]]++] = 2 * k + 1;
+    }
+  }
+  inc_off = dalloc<int>(N + 1);
+  inc = dalloc<int>(2 * size_t(B));
+  HIPCALL(hipMemcpy(inc_off, off.data(), (N + 1) * sizeof(int), hipMemcpyHostToDevice));
+  HIPCALL(hipMemcpy(inc, list.data(), 2 * size_t(B) * sizeof(int), hipMemcpyHostToDevice));
+  // ---- exclusion adjacency from the connected pair set ----
+  std::vector<std::vector<int>> nb(N);
+  for (int64_t c2 = 0; c2 < L->n_connected; c2++) {
+    uint64_t key = L->connected[c2];
+    int u = int(uint32_t(key >> 32)), v = int(uint32_t(key & 0xffffffffu));
+    if (u < 0 || u >= N || v < 0 || v >= N) throw Fail(MMC_ERR_ARG, "connected pair out of range");
+    nb[u].push_back(v);
+    nb[v].push_back(u);
+  }
+  std::vector<int> aoff(N + 1, 0), alist;
+  for (int i = 0; i < N; i++) {
+    std::sort(nb[i].begin(), nb[i].end());
+    nb[i].erase(std::unique(nb[i].begin(), nb[i].end()), nb[i].end());
+    aoff[i + 1] = aoff[i] + int(nb[i].size());
+    alist.insert(alist.end(), nb[i].begin(), nb[i].end());
+  }
+  adj_off = dalloc<int>(N + 1);
+  adj = dalloc<int>(alist.size());
+  HIPCALL(hipMemcpy(adj_off, aoff.data(), (N + 1) * sizeof(int), hipMemcpyHostToDevice));
+  if (!alist.empty()) HIPCALL(hipMemcpy(adj, alist.data(), alist.size() * sizeof(int), hipMemcpyHostToDevice));
+  // ---- evaluation buffers ----
+  x = dalloc<double>(3 * size_t(N));
+  if (precision == MMC_MIXED) R = dalloc<float>(4 * size_t(N));
+  else R = dalloc<double>(4 * size_t(N));
+  rec = dalloc<double>(9 * size_t(B));
+  trial = dalloc<double>(size_t(MMC_TRIAL_STRIDE) * B);
+  cf_sph = dalloc<double>(3 * size_t(N));
+  cf_cap = dalloc<double>(6 * size_t(B));
+  box = dalloc<float>(6 * size_t(NS));
+  table = 1024;
+  while (table < 2u * unsigned(NS)) table <<= 1;
+  cnt = dalloc<unsigned>(table + 1);
+  start = dalloc<unsigned>(table + 1);
+  HIPCALL(hipMemset(cnt, 0, (table + 1) * sizeof(unsigned)));
+  flags = dalloc<int>(4);
+  HIPCALL(hipMemset(flags, 0, 4 * sizeof(int)));
+  HIPCALL(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, cnt, start, table + 1, stream));
+  scan_tmp = dalloc<char>(scan_bytes);
+  ensure_entries(size_t(NS) * 16);
+  Ps = blocks(N);
+  Pc = blocks(NS);
+  Pb = blocks(B);
+  spart = dalloc<double>(7 * size_t(Ps));
+  cpart = dalloc<double>(3 * size_t(Pc));
+  bpart = dalloc<double>(3 * size_t(Pb));
+  gpart = dalloc<double>(2 * size_t(Ps));
+  vpart = dalloc<double>(3 * size_t(Ps) * 2);
+  stpart = dalloc<double>(8 * size_t(blocks(std::max(N, B))));
+  record = dalloc<double>(kRec * 4);
+  scal = dalloc<double>(16);
+  size_t n6 = 6 * size_t(N);
+  z = dalloc<double>(n6);
+  g = dalloc<double>(n6);
+  p = dalloc<double>(n6);
+  zc = dalloc<double>(n6);
+  gc = dalloc<double>(n6);
+  zt = dalloc<double>(n6);
+  zin = dalloc<double>(n6);
+  const size_t hm = size_t(std::max(1, cfg.history));
+  Sh = dalloc<double>(n6 * hm);
+  Yh = dalloc<double>(n6 * hm);
+  rho = dalloc<double>(hm);
+  alpha = dalloc<double>(hm);
+  HIPCALL(hipHostMalloc((void**)&host, 64 * sizeof(double), hipHostMallocDefault));
+  S.ts = cfg.translation_scale;
+  S.jr = jr;
+  S.gravity = cfg.gravity;
+  S.kc = cfg.contact_stiffness;
+  S.margin = cfg.contact_margin;
+  S.centering = cfg.centering_stiffness;
+  S.cell = std::max(2.8 * jr, 0.72 * cfg.cell_size);  // MC/src/sim.cpp:151
+  S.inv_cell = 1.0 / S.cell;
+  S.table_mask = table - 1;
+  HIPCALL(hipStreamSynchronize(stream));
+}
+
+void mmc_ctx::ensure_entries(size_t need) {
+  if (entries && size_t(S.entry_cap) >= need) return;
+  if (entries) HIPCALL(hipFree(entries));
+  size_t cap = std::max<size_t>(need, 1024);
+  if (cap > size_t(0x7fffffff)) throw Fail(MMC_ERR_CAPACITY, "contact grid entry capacity exceeds 2^31");
+  entries = dalloc<int>(cap);
+  S.entry_cap = int(cap);
+}
+
+template <class T>
+void mmc_ctx::evaluate_t(Lat<T> Lt, const double* zsrc, const double* psrc, double step, double* zdst, double* gdst,
+                         double top, bool trials, double* rec_out) {
+  S.top = top;
+  for (int attempt = 0; attempt < 3; attempt++) {
+    tick(MMC_K_STATE);
+    k_state<T><<<Ps, kBlock, 0, stream>>>(Lt, S, zsrc, psrc, step, psrc ? zdst : nullptr, x, (T*)R, spart);
+    tock(MMC_K_STATE);
+    tick(MMC_K_COUNT);
+    k_count<T><<<Pc, kBlock, 0, stream>>>(Lt, S, x, box, cnt, flags);
+    tock(MMC_K_COUNT);
+    HIPCALL(hipcub::DeviceScan::ExclusiveSum(scan_tmp, scan_bytes, cnt, start, table + 1, stream));
+    tick(MMC_K_FILL);
+    k_fill<<<Pc, kBlock, 0, stream>>>(NS, S, box, cnt, start, entries, flags);
+    tock(MMC_K_FILL);
+    tick(MMC_K_SORT);
+    k_sort<<<blocks(table), kBlock, 0, stream>>>(table, S, start, entries);
+    tock(MMC_K_SORT);
+    tick(MMC_K_NARROW);
+    k_narrow<T><<<Pc, kBlock, 0, stream>>>(Lt, S, x, box, start, entries, cf_sph, cf_cap, cpart);
+    tock(MMC_K_NARROW);
+    tick(MMC_K_BEAM);
+    k_beam<T><<<Pb, kBlock, 0, stream>>>(Lt, M, x, (const T*)R, rec, trials ? trial : nullptr, bpart);
+    tock(MMC_K_BEAM);
+    tick(MMC_K_GATHER);
+    k_gather<T><<<Ps, kBlock, 0, stream>>>(Lt, S, x, rec, cf_sph, cf_cap, spart, Ps, gdst, gpart);
+    tock(MMC_K_GATHER);
+    tick(MMC_K_FINAL);
+    k_final<<<1, kBlock, 0, stream>>>(N, cfg.centering_stiffness, n_free, spart, Ps, cpart, Pc, bpart, Pb, gpart, Ps,
+                                      flags, rec_out);
+    tock(MMC_K_FINAL);
+    HIPCALL(hipGetLastError());
+    evaluations++;
+    total_evaluations++;
+    // the entry count is only known after the scan: read it with the record
+    HIPCALL(hipMemcpyAsync(host, rec_out, kRec * sizeof(double), hipMemcpyDeviceToHost, stream));
+    HIPCALL(hipMemcpyAsync(host + kRec, start + table, sizeof(unsigned), hipMemcpyDeviceToHost, stream));
+    HIPCALL(hipStreamSynchronize(stream));
+    if (timed >= 0) {
+      float ms = 0;
+      HIPCALL(hipEventElapsedTime(&ms, ev0, ev1));
+      timed_ms += ms;
+      timed_launches++;
+    }
+    unsigned total;
+    std::memcpy(&total, host + kRec, sizeof(unsigned));
+    int fl = int(host[10]);
+    if (!(fl & 4)) return;
+    // entry buffer overflow: grow and evaluate again
+    ensure_entries(size_t(total) + size_t(total) / 2 + 1024);
+  }
+  throw Fail(MMC_ERR_CAPACITY, "contact grid overflow persisted");
+}
+
+void mmc_ctx::evaluate(const double* zsrc, const double* psrc, double step, double* zdst, double* gdst, double top,
+                       bool trials, double* rec_out) {
+  if (precision == MMC_MIXED)
+    evaluate_t<float>(lat32(), zsrc, psrc, step, zdst, gdst, top, trials, rec_out);
+  else
+    evaluate_t<double>(lat64(), zsrc, psrc, step, zdst, gdst, top, trials, rec_out);
+}
+
+namespace {
+void fill_out(const double* r, mmc_objective_out* o) {
+  if (!o) return;
+  o->energy = r[0];
+  o->elastic = r[1];
+  o->plastic = r[2];
+  o->contact = r[3];
+  o->reaction = r[4];
+  o->max_pen = r[5];
+  o->rms = r[6];
+  o->max = r[7];
+  o->contacts = int32_t(r[8]);
+  o->finite = int32_t(r[9]);
+}
+template <class F>
+int guarded(mmc_ctx* c, F&& f) {
+  if (!c) return MMC_ERR_ARG;
+  try {
+    f();
+    return MMC_OK;
+  } catch (const Fail& e) {
+    c->err = e.what();
+    return e.code;
+  } catch (const HipFail& e) {
+    c->err = e.what();
+    return MMC_ERR_HIP;
+  } catch (const std::exception& e) {
+    c->err = e.what();
+    return MMC_ERR_STATE;
+  }
+}
+void aos_from_soa(const double* soa, double* aos, int n, int k) {
+  for (int i = 0; i < n; i++)
+    for (int d = 0; d < k; d++) aos[size_t(k) * i + d] = soa[size_t(d) * n + i];
+}
+void soa_from_aos(const double* aos, double* soa, int n, int k) {
+  for (int i = 0; i < n; i++)
+    for (int d = 0; d < k; d++) soa[size_t(d) * n + i] = aos[size_t(k) * i + d];
+}
+}  // namespace
+
+extern "C" {
+
+void mmc_config_defaults(mmc_config* c) {
+  std::memset(c, 0, sizeof(*c));
+  c->cells_x = 4;
+  c->cells_y = 6;
+  c->cells_z = 4;
+  c->elements_per_strut = 3;
+  c->cell_size = .010;
+  c->min_radius = .00082;
+  c->max_radius = .00122;
+  c->joint_scale = 1.18;
+  c->trigger_scale = .38;
+  c->imperfection = .00022;
+  c->member_imperfection = .00022;
+  c->material = mmc_material{1180, 26e6, .485, 5.2e6, 1.4e6, .06, .045};
+  c->gravity = 9.81;
+  c->shear_correction = .62;
+  c->contact_stiffness = 1.2e6;
+  c->contact_margin = 2e-5;
+  c->centering_stiffness = 1.5;
+  c->contact_samples = 3;
+  c->load_steps = 52;
+  c->unload_steps = 30;
+  c->max_iterations = 130;
+  c->history = 10;
+  c->max_line_search = 22;
+  c->max_adaptive_refinements = 7;
+  c->fps = 24;
+  c->frames = 83;
+  c->threads = 0;
+  c->max_strain = .62;
+  c->step_time = .06;
+  c->translation_scale = .010;
+  c->max_translation_step = .0022;
+  c->max_rotation_step = .30;
+  c->gradient_rms_tolerance = 1.2e-3;
+  c->gradient_max_tolerance = 1.5e-2;
+  c->armijo = 1e-4;
+  c->min_line_step = 1e-9;
+  c->penetration_tolerance = .00012;
+  c->seed = 0x494e445553545259ULL;
+  c->lattice = MMC_LATTICE_BCC;
+  c->relative_escape = 1;
+  c->relative_tolerance = 1.5e-3;
+}
+
+int mmc_create(const mmc_lattice* L, const mmc_config* cfg, int32_t precision, mmc_ctx** out) {
+  if (!L || !cfg || !out || (precision != MMC_FP64 && precision != MMC_MIXED)) {
+    g_create_error = "mmc_create: invalid argument";
+    return MMC_ERR_ARG;
+  }
+  *out = nullptr;
+  mmc_ctx* c = new mmc_ctx();
+  try {
+    c->setup(L, cfg, precision);
+  } catch (const Fail& e) {
+    g_create_error = e.what();
+    delete c;
+    return e.code;
+  } catch (const std::exception& e) {
+    g_create_error = e.what();
+    delete c;
+    return MMC_ERR_HIP;
+  }
+  *out = c;
+  return MMC_OK;
+}
+
+void mmc_destroy(mmc_ctx* c) { delete c; }
+
+const char* mmc_last_error(const mmc_ctx* c) { return c ? c->err.c_str() : g_create_error.c_str(); }
+
+int mmc_objective(mmc_ctx* c, const double* zh, double top, int32_t trials, mmc_objective_out* out, double* grad) {
+  return guarded(c, [&] {
+    if (!zh) throw Fail(MMC_ERR_ARG, "z is NULL");
+    const int N = c->N;
+    std::vector<double> s(6 * size_t(N));
+    soa_from_aos(zh, s.data(), N, 6);
+    HIPCALL(hipMemcpyAsync(c->zin, s.data(), s.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    c->evaluate(c->zin, nullptr, 0.0, nullptr, c->gc, top, trials != 0, c->record + 2 * kRec);
+    fill_out(c->host, out);
+    if (trials) {
+      HIPCALL(hipMemcpyAsync(c->zt, c->zin, s.size() * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+      c->have_trial = true;
+    }
+    if (grad) {
+      c->read(s.data(), c->gc, s.size() * sizeof(double));
+      if (c->host[9] == 0.0)
+        std::fill(grad, grad + s.size(), 0.0);
+      else
+        aos_from_soa(s.data(), grad, N, 6);
+    }
+    HIPCALL(hipStreamSynchronize(c->stream));
+  });
+}
+
+// Simulator::minimize, MC/src/sim.cpp:174-211.  The line-search decisions read
+// one 12-double record per objective evaluation; every vector stays on the device.
+int mmc_minimize(mmc_ctx* c, double top, mmc_opt_stats* stats, mmc_objective_out* fin, double* z_out) {
+  return guarded(c, [&] {
+    const int N = c->N, P = c->Ps;
+    const size_t n6 = 6 * size_t(N);
+    const mmc_config& cf = c->cfg;
+    const int m = cf.history;
+    const double mt = cf.max_translation_step / cf.translation_scale, mr = cf.max_rotation_step;
+    c->evaluations = 0;
+    mmc_opt_stats st{};
+    st.rms = st.max = st.energy = INFINITY;
+    c->zero(c->z, long(n6));
+    double cur[kRec], cand[kRec];
+    c->evaluate(c->z, nullptr, 0.0, nullptr, c->g, top, false, c->record);
+    std::memcpy(cur, c->host, sizeof cur);
+    auto converged = [&](const double* r) {
+      double rms = r[6], mx = r[7];
+      double rel = rms / std::max(1e-6, cf.translation_scale * std::max(1.0, r[4]));
+      bool ok = (rms <= cf.gradient_rms_tolerance && mx <= cf.gradient_max_tolerance) ||
+                (cf.relative_escape && rel < cf.relative_tolerance);
+      return ok && r[5] <= cf.penetration_tolerance;
+    };
+    // history ring: pair j (0 = oldest) lives in slot (head + j) % m
+    int count = 0, head = 0;
+    std::vector<double> rho_h(std::max(1, m)), sy_last(1), yy_last(1);
+    double last_sy = 0, last_yy = 0;
+    auto slot = [&](int j) { return (head + j) % std::max(1, m); };
+    auto read_scalar = [&](const double* part, int Pn) {
+      k_sum1<<<1, kBlock, 0, c->stream>>>(part, Pn, c->scal);
+      c->read(c->host + 32, c->scal, sizeof(double));
+      return c->host[32];
+    };
+    auto steepest = [&]() {
+      k_tl_scaled<<<P, kBlock, 0, c->stream>>>(N, c->g, 1.0, c->p, c->lock, c->g, mt, mr, c->vpart);
+      return read_scalar(c->vpart, P);
+    };
+    auto line_search = [&](double dg) {
+      double step = 1.0;
+      for (int bt = 0; bt < cf.max_line_search; bt++) {
+        c->evaluate(c->z, c->p, step, c->zc, c->gc, top, false, c->record + kRec);
+        std::memcpy(cand, c->host, sizeof cand);
+        st.backtracks++;
+        if (cand[9] != 0.0 && cand[0] <= cur[0] + cf.armijo * step * dg) return true;
+        step *= 0.5;
+        if (step < cf.min_line_step) break;
+      }
+      return false;
+    };
+    for (int it = 0; it < cf.max_iterations; it++) {
+      st.iterations = it + 1;
+      st.rms = cur[6];
+      st.max = cur[7];
+      st.energy = cur[0];
+      if (converged(cur)) {
+        st.converged = 1;
+        break;
+      }
+      // two-loop recursion, MC/src/sim.cpp:184-190
+      double dg;
+      if (count == 0) {
+        k_tl_scaled<<<P, kBlock, 0, c->stream>>>(N, c->g, 0.10, c->p, c->lock, c->g, mt, mr, c->vpart);
+        dg = read_scalar(c->vpart, P);
+      } else {
+        HIPCALL(hipMemcpyAsync(c->rho, rho_h.data(), m * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        double H0 = 0.10;
+        if (last_sy > 1e-20 && last_yy > 1e-20) H0 = std::min(1e4, std::max(1e-7, last_sy / last_yy));
+        double* part_a = c->vpart;
+        double* part_b = c->vpart + 3 * P;
+        k_tl_init<<<P, kBlock, 0, c->stream>>>(N, c->g, 1.0, c->p, c->Sh + n6 * slot(count - 1), part_a);
+        for (int j = count - 1; j >= 0; j--) {
+          const double* D = j > 0 ? c->Sh + n6 * slot(j - 1) : c->Yh + n6 * slot(0);
+          k_tl_back<<<P, kBlock, 0, c->stream>>>(N, c->p, c->Yh + n6 * slot(j), c->rho, slot(j), part_a, P, c->alpha,
+                                                  j == 0 ? H0 : 1.0, D, part_b);
+          std::swap(part_a, part_b);
+        }
+        for (int j = 0; j < count; j++) {
+          const bool last = j == count - 1;
+          const double* D = last ? nullptr : c->Yh + n6 * slot(j + 1);
+          k_tl_fwd<<<P, kBlock, 0, c->stream>>>(N, c->p, c->Sh + n6 * slot(j), c->rho, c->alpha, slot(j), part_a, P,
+                                                 D, last ? 1 : 0, c->lock, c->g, mt, mr, part_b);
+          std::swap(part_a, part_b);
+        }
+        dg = read_scalar(part_a, P);
+      }
+      if (!(dg < 0)) {
+        count = 0;
+        dg = steepest();
+      }
+      bool ok = line_search(dg);
+      if (!ok) {
+        count = 0;
+        dg = steepest();
+        ok = line_search(dg);
+      }
+      if (!ok) break;
+      // curvature pair, MC/src/sim.cpp:205-206
+      int target = count == m ? slot(0) : slot(count);
+      if (m > 0) {
+        k_pair<<<P, kBlock, 0, c->stream>>>(N, c->z, c->zc, c->g, c->gc, c->Sh + n6 * target, c->Yh + n6 * target,
+                                            c->vpart);
+        k_sum3<<<1, kBlock, 0, c->stream>>>(c->vpart, P, c->scal);
+        c->read(c->host + 32, c->scal, 3 * sizeof(double));
+        double sy = c->host[32], ss = c->host[33], yy = c->host[34];
+        if (sy > 1e-12 * std::sqrt(std::max(1e-30, ss * yy))) {
+          if (count == m) {
+            head = (head + 1) % m;  // drop the oldest; the new pair took its slot
+          } else {
+            count++;
+          }
+          rho_h[target] = 1 / sy;
+          last_sy = sy;
+          last_yy = yy;
+        } else {
+          count = 0;
+          head = 0;
+        }
+      }
+      std::swap(c->z, c->zc);
+      std::swap(c->g, c->gc);
+      std::memcpy(cur, cand, sizeof cur);
+    }
+    // Armijo steps decrease the energy strictly, so the reference's best-iterate
+    // fallback (MC/src/sim.cpp:209) always selects the current iterate.
+    c->evaluate(c->z, nullptr, 0.0, nullptr, c->gc, top, true, c->record + 2 * kRec);
+    double f2[kRec];
+    std::memcpy(f2, c->host, sizeof f2);
+    HIPCALL(hipMemcpyAsync(c->zt, c->z, n6 * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    c->have_trial = true;
+    st.rms = f2[6];
+    st.max = f2[7];
+    st.energy = f2[0];
+    st.converged = converged(f2) ? 1 : 0;
+    st.evaluations = c->evaluations;
+    if (stats) *stats = st;
+    fill_out(f2, fin);
+    if (z_out) {
+      std::vector<double> s(n6);
+      c->read(s.data(), c->z, n6 * sizeof(double));
+      aos_from_soa(s.data(), z_out, N, 6);
+    }
+    HIPCALL(hipStreamSynchronize(c->stream));
+  });
+}
+
+int mmc_commit(mmc_ctx* c) {
+  return guarded(c, [&] {
+    if (!c->have_trial) throw Fail(MMC_ERR_STATE, "missing constitutive trial states");
+    // recompute state(z) of the trial evaluation in FP64 (for either precision
+    // policy the committed pose is FP64), then copy it into the base pose
+    k_state<double><<<c->Ps, kBlock, 0, c->stream>>>(c->lat64(), c->S, c->zt, nullptr, 0.0, nullptr, c->x, c->zin,
+                                                     c->spart);
+    k_commit_nodes<double><<<c->Ps, kBlock, 0, c->stream>>>(c->N, c->x, c->zin, c->bpos, c->bquat);
+    k_commit_beams<<<c->Pb, kBlock, 0, c->stream>>>(c->B, c->trial, c->plastic);
+    HIPCALL(hipGetLastError());
+    HIPCALL(hipStreamSynchronize(c->stream));
+    c->have_trial = false;
+  });
+}
+
+int mmc_read_state(mmc_ctx* c, double* xh, double* qh, double* ph) {
+  return guarded(c, [&] {
+    std::vector<double> s(7 * size_t(std::max(c->N, c->B)));
+    if (xh) {
+      c->read(s.data(), c->bpos, 3 * size_t(c->N) * sizeof(double));
+      aos_from_soa(s.data(), xh, c->N, 3);
+    }
+    if (qh) {
+      c->read(s.data(), c->bquat, 4 * size_t(c->N) * sizeof(double));
+      aos_from_soa(s.data(), qh, c->N, 4);
+    }
+    if (ph) {
+      c->read(s.data(), c->plastic, 7 * size_t(c->B) * sizeof(double));
+      aos_from_soa(s.data(), ph, c->B, 7);
+    }
+  });
+}
+
+int mmc_read_trial(mmc_ctx* c, double* t) {
+  return guarded(c, [&] {
+    if (!t) throw Fail(MMC_ERR_ARG, "trial is NULL");
+    std::vector<double> s(size_t(MMC_TRIAL_STRIDE) * c->B);
+    c->read(s.data(), c->trial, s.size() * sizeof(double));
+    aos_from_soa(s.data(), t, c->B, MMC_TRIAL_STRIDE);
+  });
+}
+
+int mmc_read_stats(mmc_ctx* c, mmc_state_stats* o) {
+  return guarded(c, [&] {
+    int P = blocks(std::max(c->N, c->B));
+    double* part = c->stpart;
+    k_stats<<<P, kBlock, 0, c->stream>>>(c->N, c->B, c->plastic, c->bpos, part);
+    k_stats_final<<<1, kBlock, 0, c->stream>>>(part, P, c->scal);
+    c->read(c->host + 40, c->scal, 8 * sizeof(double));
+    const double* r = c->host + 40;
+    o->sum_axial = r[0];
+    o->max_axial = r[1];
+    o->sum_rot = r[2];
+    o->max_rot = r[3];
+    o->plastic_beams = int32_t(r[4]);
+    o->plastic_hinges = int32_t(r[5]);
+    o->y_min = -r[6];
+    o->y_max = r[7];
+  });
+}
+
+int mmc_kernel_timer(mmc_ctx* c, int32_t kernel) {
+  return guarded(c, [&] {
+    if (kernel > MMC_K_FINAL) throw Fail(MMC_ERR_ARG, "unknown kernel id");
+    if (!c->ev0) {
+      HIPCALL(hipEventCreate(&c->ev0));
+      HIPCALL(hipEventCreate(&c->ev1));
+    }
+    c->timed = kernel < 0 ? -1 : kernel;
+    c->timed_launches = 0;
+    c->timed_ms = 0;
+  });
+}
+
+int mmc_kernel_timer_read(mmc_ctx* c, int64_t* launches, double* mean_ms) {
+  return guarded(c, [&] {
+    if (launches) *launches = c->timed_launches;
+    if (mean_ms) *mean_ms = c->timed_launches ? c->timed_ms / double(c->timed_launches) : 0.0;
+  });
+}
+
+int64_t mmc_evaluations(const mmc_ctx* c) { return c ? c->total_evaluations : 0; }
+
+int mmc_beam_potential(mmc_ctx* c, const double* xh, const double* qh, double* out) {
+  return guarded(c, [&] {
+    if (!xh || !qh || !out) throw Fail(MMC_ERR_ARG, "NULL argument");
+    const int N = c->N;
+    std::vector<double> s(4 * size_t(N));
+    soa_from_aos(xh, s.data(), N, 3);
+    HIPCALL(hipMemcpyAsync(c->x, s.data(), 3 * size_t(N) * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCALL(hipStreamSynchronize(c->stream));
+    soa_from_aos(qh, s.data(), N, 4);
+    HIPCALL(hipMemcpyAsync(c->zin, s.data(), 4 * size_t(N) * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    k_beam_potential<double><<<c->Pb, kBlock, 0, c->stream>>>(c->lat64(), c->M, c->x, c->zin, c->bpart);
+    k_sum1<<<1, kBlock, 0, c->stream>>>(c->bpart, c->Pb, c->scal);
+    c->read(c->host + 48, c->scal, sizeof(double));
+    *out = c->host[48];
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,757 @@
+// simulator.cpp — the Simulator facade over the C-ABI (see simulator.hpp) and
+// the mmc_sim_* C entry points.  Reference: projects/metamaterial-compression/
+// industry-grade/src/sim.cpp (MC/src/sim.cpp below).
+#include "simulator.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <functional>
+#include <iomanip>
+#include <iostream>
+#include <limits>
+#include <sstream>
+#include <stdexcept>
+
+namespace meta {
+namespace {
+
+constexpr double kPi = 3.1415926535897932384626433832795;
+constexpr double kInf = std::numeric_limits<double>::infinity();
+
+Vec3 add(Vec3 a, Vec3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+Vec3 sub(Vec3 a, Vec3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+Vec3 mul(Vec3 a, double s) { return {a.x * s, a.y * s, a.z * s}; }
+double dot3(Vec3 a, Vec3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+Vec3 cross3(Vec3 a, Vec3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+double len3(Vec3 a) { return std::sqrt(dot3(a, a)); }
+Vec3 unit3(Vec3 v) {
+  double l2 = dot3(v, v);
+  return (std::isfinite(l2) && l2 > 1e-30) ? mul(v, 1.0 / std::sqrt(l2)) : Vec3{0, 1, 0};
+}
+Quat qn(Quat q) {
+  double n = std::sqrt(q.w * q.w + q.x * q.x + q.y * q.y + q.z * q.z);
+  if (n < 1e-30) return Quat{};
+  return {q.w / n, q.x / n, q.y / n, q.z / n};
+}
+Quat qm(Quat a, Quat b) {
+  return {a.w * b.w - a.x * b.x - a.y * b.y - a.z * b.z, a.w * b.x + a.x * b.w + a.y * b.z - a.z * b.y,
+          a.w * b.y - a.x * b.z + a.y * b.w + a.z * b.x, a.w * b.z + a.x * b.y - a.y * b.x + a.z * b.w};
+}
+Quat qc(Quat q) { return {q.w, -q.x, -q.y, -q.z}; }
+Vec3 qr(Quat q, Vec3 v) {
+  Vec3 u{q.x, q.y, q.z};
+  Vec3 t = mul(cross3(u, v), 2.0);
+  return add(add(v, mul(t, q.w)), cross3(u, t));
+}
+Quat q_axis_angle(Vec3 axis, double angle) {
+  axis = unit3(axis);
+  double s = std::sin(0.5 * angle);
+  return qn({std::cos(0.5 * angle), axis.x * s, axis.y * s, axis.z * s});
+}
+Vec3 q_log(Quat q) {  // MC/include/core.hpp:52
+  q = qn(q);
+  if (q.w < 0) q = {-q.w, -q.x, -q.y, -q.z};
+  double s = std::sqrt(q.x * q.x + q.y * q.y + q.z * q.z);
+  if (s < 1e-12) return {2 * q.x, 2 * q.y, 2 * q.z};
+  double a = 2 * std::atan2(s, std::max(-1.0, std::min(1.0, q.w)));
+  return mul(Vec3{q.x, q.y, q.z}, a / s);
+}
+// Rotation matrix columns -> quaternion (MC/include/core.hpp:43)
+Quat q_from_axes(Vec3 X, Vec3 Y, Vec3 Z) {
+  double m00 = X.x, m01 = Y.x, m02 = Z.x, m10 = X.y, m11 = Y.y, m12 = Z.y, m20 = X.z, m21 = Y.z, m22 = Z.z;
+  Quat q;
+  double tr = m00 + m11 + m22;
+  if (tr > 0) {
+    double s = std::sqrt(tr + 1) * 2;
+    q = {0.25 * s, (m21 - m12) / s, (m02 - m20) / s, (m10 - m01) / s};
+  } else if (m00 > m11 && m00 > m22) {
+    double s = std::sqrt(1 + m00 - m11 - m22) * 2;
+    q = {(m21 - m12) / s, 0.25 * s, (m01 + m10) / s, (m02 + m20) / s};
+  } else if (m11 > m22) {
+    double s = std::sqrt(1 + m11 - m00 - m22) * 2;
+    q = {(m02 - m20) / s, (m01 + m10) / s, 0.25 * s, (m12 + m21) / s};
+  } else {
+    double s = std::sqrt(1 + m22 - m00 - m11) * 2;
+    q = {(m10 - m01) / s, (m02 + m20) / s, (m12 + m21) / s, 0.25 * s};
+  }
+  return qn(q);
+}
+// Orientation whose x axis follows d (MC/src/sim.cpp:41)
+Quat align_x(Vec3 d) {
+  Vec3 X = unit3(d);
+  Vec3 h = std::abs(X.z) < 0.8 ? Vec3{0, 0, 1} : Vec3{0, 1, 0};
+  Vec3 Y = unit3(cross3(h, X));
+  Vec3 Z = unit3(cross3(X, Y));
+  Y = unit3(cross3(Z, X));
+  return q_from_axes(X, Y, Z);
+}
+uint64_t splitmix(uint64_t v) {
+  v += 0x9e3779b97f4a7c15ULL;
+  v = (v ^ (v >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  v = (v ^ (v >> 27)) * 0x94d049bb133111ebULL;
+  return v ^ (v >> 31);
+}
+
+}  // namespace
+
+mmc_config to_c(const SimulationConfig& s) {
+  mmc_config c;
+  mmc_config_defaults(&c);
+  c.cells_x = s.cellsX;
+  c.cells_y = s.cellsY;
+  c.cells_z = s.cellsZ;
+  c.elements_per_strut = s.elementsPerStrut;
+  c.cell_size = s.cellSize;
+  c.min_radius = s.minRadius;
+  c.max_radius = s.maxRadius;
+  c.joint_scale = s.jointScale;
+  c.trigger_scale = s.triggerScale;
+  c.imperfection = s.imperfection;
+  c.member_imperfection = s.memberImperfection;
+  c.material = mmc_material{s.material.density, s.material.young, s.material.nu, s.material.yield,
+                            s.material.hardening, s.material.bendHardening, s.material.torsionHardening};
+  c.gravity = s.gravity;
+  c.shear_correction = s.shearCorrection;
+  c.contact_stiffness = s.contactStiffness;
+  c.contact_margin = s.contactMargin;
+  c.centering_stiffness = s.centeringStiffness;
+  c.contact_samples = s.contactSamples;
+  c.load_steps = s.loadSteps;
+  c.unload_steps = s.unloadSteps;
+  c.max_iterations = s.maxIterations;
+  c.history = s.history;
+  c.max_line_search = s.maxLineSearch;
+  c.max_adaptive_refinements = s.maxAdaptiveRefinements;
+  c.fps = s.fps;
+  c.frames = s.frames;
+  c.threads = s.threads;
+  c.max_strain = s.maxStrain;
+  c.step_time = s.stepTime;
+  c.translation_scale = s.translationScale;
+  c.max_translation_step = s.maxTranslationStep;
+  c.max_rotation_step = s.maxRotationStep;
+  c.gradient_rms_tolerance = s.gradientRmsTolerance;
+  c.gradient_max_tolerance = s.gradientMaxTolerance;
+  c.armijo = s.armijo;
+  c.min_line_step = s.minLineStep;
+  c.penetration_tolerance = s.penetrationTolerance;
+  c.seed = s.seed;
+  c.lattice = s.lattice;
+  c.relative_escape = s.relativeEscape ? 1 : 0;
+  c.relative_tolerance = s.relativeTolerance;
+  c.position_sigma = s.positionSigma;
+  c.radius_mode = s.radiusMode;
+  return c;
+}
+
+SimulationConfig from_c(const mmc_config& c) {
+  SimulationConfig s;
+  s.cellsX = c.cells_x;
+  s.cellsY = c.cells_y;
+  s.cellsZ = c.cells_z;
+  s.elementsPerStrut = c.elements_per_strut;
+  s.cellSize = c.cell_size;
+  s.minRadius = c.min_radius;
+  s.maxRadius = c.max_radius;
+  s.jointScale = c.joint_scale;
+  s.triggerScale = c.trigger_scale;
+  s.imperfection = c.imperfection;
+  s.memberImperfection = c.member_imperfection;
+  s.material.density = c.material.density;
+  s.material.young = c.material.young;
+  s.material.nu = c.material.nu;
+  s.material.yield = c.material.yield;
+  s.material.hardening = c.material.hardening;
+  s.material.bendHardening = c.material.bend_hardening;
+  s.material.torsionHardening = c.material.torsion_hardening;
+  s.gravity = c.gravity;
+  s.shearCorrection = c.shear_correction;
+  s.contactStiffness = c.contact_stiffness;
+  s.contactMargin = c.contact_margin;
+  s.centeringStiffness = c.centering_stiffness;
+  s.contactSamples = c.contact_samples;
+  s.loadSteps = c.load_steps;
+  s.unloadSteps = c.unload_steps;
+  s.maxIterations = c.max_iterations;
+  s.history = c.history;
+  s.maxLineSearch = c.max_line_search;
+  s.maxAdaptiveRefinements = c.max_adaptive_refinements;
+  s.fps = c.fps;
+  s.frames = c.frames;
+  s.threads = c.threads;
+  s.maxStrain = c.max_strain;
+  s.stepTime = c.step_time;
+  s.translationScale = c.translation_scale;
+  s.maxTranslationStep = c.max_translation_step;
+  s.maxRotationStep = c.max_rotation_step;
+  s.gradientRmsTolerance = c.gradient_rms_tolerance;
+  s.gradientMaxTolerance = c.gradient_max_tolerance;
+  s.armijo = c.armijo;
+  s.minLineStep = c.min_line_step;
+  s.penetrationTolerance = c.penetration_tolerance;
+  s.seed = c.seed;
+  s.lattice = c.lattice;
+  s.relativeEscape = c.relative_escape != 0;
+  s.relativeTolerance = c.relative_tolerance;
+  s.positionSigma = c.position_sigma;
+  s.radiusMode = c.radius_mode;
+  return s;
+}
+
+// MC/src/sim.cpp:49
+Simulator::Simulator(SimulationConfig cfg) : cfg_(std::move(cfg)) {
+  cfg_.translationScale = cfg_.cellSize;
+  c_ = to_c(cfg_);
+  lat_ = build_lattice(c_);
+  mmc_lattice v = lat_.view();
+  int rc = mmc_create(&v, &c_, cfg_.precision, &ctx_);
+  if (rc != MMC_OK) throw std::runtime_error(mmc_last_error(nullptr));
+}
+
+Simulator::~Simulator() { mmc_destroy(ctx_); }
+
+void Simulator::check(int rc) const {
+  if (rc != MMC_OK) throw std::runtime_error(mmc_last_error(ctx_));
+}
+
+void Simulator::exportSTL(const std::filesystem::path&, int) const {
+  throw std::runtime_error("STL export is not part of the gfx950 equilibrium solver");
+}
+
+// Host mirror of the committed lattice state (read from the device on demand).
+void Simulator::sync() const {
+  if (synced_) return;
+  const int N = lat_.n_nodes, B = lat_.n_beams;
+  std::vector<double> x(3 * size_t(N)), q(4 * size_t(N)), p(7 * size_t(B));
+  check(mmc_read_state(ctx_, x.data(), q.data(), p.data()));
+  nodes_.resize(size_t(N));
+  beams_.resize(size_t(B));
+  for (int i = 0; i < N; i++) {
+    Node& n = nodes_[size_t(i)];
+    n.x0 = Vec3{lat_.x[3 * size_t(i)], lat_.x[3 * size_t(i) + 1], lat_.x[3 * size_t(i) + 2]};
+    n.x = Vec3{x[3 * size_t(i)], x[3 * size_t(i) + 1], x[3 * size_t(i) + 2]};
+    n.R = Quat{q[4 * size_t(i)], q[4 * size_t(i) + 1], q[4 * size_t(i) + 2], q[4 * size_t(i) + 3]};
+    n.mass = lat_.mass[size_t(i)];
+    n.layer = lat_.node_layer[size_t(i)];
+  }
+  for (int k = 0; k < B; k++) {
+    Beam& e = beams_[size_t(k)];
+    size_t s = size_t(k);
+    e.a = lat_.a[s];
+    e.b = lat_.b[s];
+    e.layer = lat_.layer[s];
+    e.family = lat_.family[s];
+    e.L0 = lat_.L0[s];
+    e.radius = lat_.radius[s];
+    e.area = lat_.area[s];
+    e.I = lat_.inertia[s];
+    e.J = 2 * e.I;
+    e.e0 = Vec3{lat_.e0[3 * s], lat_.e0[3 * s + 1], lat_.e0[3 * s + 2]};
+    const double* pp = &p[7 * s];
+    e.plastic = PlasticState{pp[0], pp[1], Vec3{pp[2], pp[3], pp[4]}, pp[5], pp[6]};
+  }
+  synced_ = true;
+}
+
+const std::vector<Node>& Simulator::nodes() const {
+  sync();
+  return nodes_;
+}
+const std::vector<Beam>& Simulator::beams() const {
+  sync();
+  return beams_;
+}
+
+// MC/src/sim.cpp:213
+double Simulator::topY(double s, bool loading) const {
+  double low = lat_.top0 - cfg_.maxStrain * lat_.height;
+  double t = std::max(0.0, std::min(1.0, s));
+  return loading ? lat_.top0 * (1.0 - t) + low * t : low * (1.0 - t) + lat_.top0 * t;
+}
+
+// MC/src/sim.cpp:215-310
+std::vector<Snapshot> Simulator::capture(double top, const std::vector<Vec3>* prevX,
+                                         const std::vector<Quat>* prevR) const {
+  sync();
+  std::vector<Snapshot> out;
+  out.reserve(beams_.size() + nodes_.size() + 10);
+  const double dt = std::max(1e-6, cfg_.stepTime);
+  auto vel = [&](int i) {
+    if (!prevX || prevX->size() != nodes_.size()) return Vec3{};
+    return mul(sub(nodes_[size_t(i)].x, (*prevX)[size_t(i)]), 1.0 / dt);
+  };
+  auto omega = [&](int i) {
+    if (!prevR || prevR->size() != nodes_.size()) return Vec3{};
+    Quat dq = qn(qm(nodes_[size_t(i)].R, qc((*prevR)[size_t(i)])));
+    return mul(q_log(dq), 1.0 / dt);
+  };
+  for (size_t bi = 0; bi < beams_.size(); ++bi) {
+    const Beam& e = beams_[bi];
+    const Vec3 a = nodes_[size_t(e.a)].x, b = nodes_[size_t(e.b)].x, d = sub(b, a);
+    const double L = std::max(1e-8, len3(d));
+    const double severity = std::max(std::abs(e.plastic.epsp) / 0.10, len3(e.plastic.rotp) / 0.42);
+    int material = 0;
+    if (severity > 0.28) material = 1;
+    if (severity > 0.62) material = 2;
+    if (severity > 1.05) material = 3;
+    Snapshot s;
+    s.p = mul(add(a, b), 0.5);
+    s.q = align_x(d);
+    s.v = mul(add(vel(e.a), vel(e.b)), 0.5);
+    s.w = mul(add(omega(e.a), omega(e.b)), 0.5);
+    s.half = {0.5 * L, e.radius, e.radius};
+    s.material = material;
+    s.id = int(bi);
+    s.kind = int(BodyKind::Strut);
+    s.moduleClass = e.family;
+    s.ring = e.layer;
+    s.slot = int(bi);
+    out.push_back(s);
+  }
+  const double jr = cfg_.jointScale * cfg_.maxRadius;
+  for (size_t ni = 0; ni < nodes_.size(); ++ni) {
+    Snapshot s;
+    s.p = nodes_[ni].x;
+    s.q = nodes_[ni].R;
+    s.v = vel(int(ni));
+    s.w = omega(int(ni));
+    s.half = Vec3{jr, jr, jr};
+    s.material = 4;
+    s.id = int(beams_.size() + ni);
+    s.kind = int(BodyKind::Joint);
+    s.moduleClass = nodes_[ni].layer;
+    s.ring = nodes_[ni].layer;
+    s.slot = int(ni);
+    out.push_back(s);
+  }
+  int id = int(beams_.size() + nodes_.size());
+  auto box = [&](Vec3 p, Vec3 half, int material, BodyKind kind) {
+    Snapshot s;
+    s.p = p;
+    s.q = Quat{};
+    s.half = half;
+    s.material = material;
+    s.id = id++;
+    s.kind = int(kind);
+    out.push_back(s);
+  };
+  // rigid test machine: platens at y = 0 and y = top, base, frame (MC/src/sim.cpp:297-307)
+  box({0, -0.0030, 0}, {0.026, 0.0030, 0.026}, 5, BodyKind::Platen);
+  box({0, top + 0.0030, 0}, {0.026, 0.0030, 0.026}, 5, BodyKind::Platen);
+  box({0, -0.0095, 0}, {0.036, 0.0035, 0.036}, 6, BodyKind::Machine);
+  const double columnY = 0.054;
+  box({-0.037, columnY, -0.049}, {0.0026, 0.061, 0.0026}, 6, BodyKind::Machine);
+  box({0.037, columnY, -0.049}, {0.0026, 0.061, 0.0026}, 6, BodyKind::Machine);
+  box({0, 0.111, -0.049}, {0.042, 0.0032, 0.0036}, 6, BodyKind::Machine);
+  box({0, top + 0.0080, -0.001}, {0.0105, 0.0025, 0.0105}, 7, BodyKind::Machine);
+  return out;
+}
+
+// MC/src/sim.cpp:312-372; plastic and height reductions come from the device.
+FrameDiagnostics Simulator::diagnostics(double time, double top, bool loading, const mmc_objective_out& o,
+                                        const mmc_opt_stats& s) const {
+  FrameDiagnostics d;
+  const double height = lat_.height;
+  d.time = time;
+  d.platenY = top;
+  d.displacement = lat_.top0 - top;
+  d.engineeringStrain = d.displacement / std::max(1e-12, height);
+  d.reactionForce = std::max(0.0, o.reaction - reactionOffset_);
+  d.engineeringStress = d.reactionForce / std::max(1e-12, lat_.area_ref);
+  d.loadingWork = loadingWork_;
+  d.recoveredWork = recoveredWork_;
+  d.dissipatedWork = std::max(0.0, loadingWork_ - recoveredWork_);
+  d.specificEnergyAbsorption = d.dissipatedWork / std::max(1e-12, lat_.total_mass);
+  d.elasticEnergy = o.elastic;
+  d.plasticDissipation = o.plastic;
+  d.contactEnergy = o.contact;
+  d.residualRms = s.rms;
+  d.residualMax = s.max;
+  d.maxPenetration = o.max_pen;
+  d.activeContacts = o.contacts;
+  d.iterations = s.iterations;
+  d.converged = s.converged != 0;
+  d.loading = loading;
+  mmc_state_stats st{};
+  check(mmc_read_stats(ctx_, &st));
+  const double B = double(lat_.n_beams);
+  if (lat_.n_beams > 0) {
+    d.meanPlasticStrain = st.sum_axial / B;
+    d.maxPlasticStrain = st.max_axial;
+    d.meanPlasticRotation = st.sum_rot / B;
+    d.maxPlasticRotation = st.max_rot;
+  }
+  d.plasticBeams = st.plastic_beams;
+  d.plasticHinges = st.plastic_hinges;
+  const double hcur = std::max(0.0, st.y_max - st.y_min);
+  d.permanentSet = std::max(0.0, 1.0 - hcur / std::max(1e-12, height));
+  return d;
+}
+
+// MC/src/sim.cpp:379-388
+void Simulator::begin() {
+  loadingWork_ = recoveredWork_ = prevDisp_ = prevForce_ = reactionOffset_ = 0.0;
+  time_ = 0.0;
+  currentTop_ = lat_.top0;
+  first_ = true;
+  acceptedSteps_ = 0;
+  cursor_ = 0;
+}
+
+// One equilibrium increment with adaptive bisection, MC/src/sim.cpp:390-454.
+void Simulator::advanceTo(double targetTop, bool loading, int refinement, SimulationResult& result) {
+  std::vector<Vec3> previousX;
+  std::vector<Quat> previousR;
+  if (cfg_.captureFrames) {
+    sync();
+    previousX.resize(nodes_.size());
+    previousR.resize(nodes_.size());
+    for (size_t i = 0; i < nodes_.size(); ++i) {
+      previousX[i] = nodes_[i].x;
+      previousR[i] = nodes_[i].R;
+    }
+  }
+  mmc_opt_stats stats{};
+  mmc_objective_out fin{};
+  check(mmc_minimize(ctx_, targetTop, &stats, &fin, nullptr));
+  const bool acceptable = stats.converged && fin.max_pen <= cfg_.penetrationTolerance && std::isfinite(fin.energy);
+  if (!acceptable && refinement < cfg_.maxAdaptiveRefinements && std::abs(targetTop - currentTop_) > 2.5e-5) {
+    const double midpoint = 0.5 * (currentTop_ + targetTop);
+    advanceTo(midpoint, loading, refinement + 1, result);
+    advanceTo(targetTop, loading, refinement + 1, result);
+    return;
+  }
+  if (!acceptable) {
+    std::ostringstream message;
+    message << "quasi-static equilibrium failed at platen=" << targetTop << " m, residual=" << stats.rms
+            << ", penetration=" << fin.max_pen * 1000.0 << " mm after " << stats.iterations << " iterations";
+    throw std::runtime_error(message.str());
+  }
+  check(mmc_commit(ctx_));
+  synced_ = false;
+  if (first_) reactionOffset_ = fin.reaction;
+  const double correctedForce = std::max(0.0, fin.reaction - reactionOffset_);
+  const double displacement = lat_.top0 - targetTop;
+  if (!first_) {
+    const double dd = displacement - prevDisp_;
+    const double work = 0.5 * (prevForce_ + correctedForce) * dd;
+    if (dd >= 0.0)
+      loadingWork_ += std::max(0.0, work);
+    else
+      recoveredWork_ += std::max(0.0, -work);
+  }
+  prevDisp_ = displacement;
+  prevForce_ = correctedForce;
+  currentTop_ = targetTop;
+  if (cfg_.captureFrames)
+    result.frames.push_back(capture(targetTop, first_ ? nullptr : &previousX, first_ ? nullptr : &previousR));
+  result.diagnostics.push_back(diagnostics(time_, targetTop, loading, fin, stats));
+  const FrameDiagnostics& d = result.diagnostics.back();
+  if (cfg_.verbose)
+    std::cout << "[equilibrium] " << std::setw(3) << acceptedSteps_ << " " << (loading ? "load  " : "unload")
+              << " strain=" << std::fixed << std::setprecision(4) << d.engineeringStrain
+              << " force=" << std::setprecision(3) << d.reactionForce << " N"
+              << " residual=" << std::scientific << std::setprecision(2) << d.residualRms
+              << " pen=" << std::fixed << std::setprecision(3) << d.maxPenetration * 1000.0 << " mm"
+              << " iter=" << d.iterations << (refinement ? " [REFINED]" : " [OK]") << '\n';
+  acceptedSteps_++;
+  first_ = false;
+  time_ += cfg_.stepTime;
+}
+
+// Next scheduled increment, MC/src/sim.cpp:456-465.  Returns false once the
+// schedule is complete.
+bool Simulator::advance(SimulationResult& result) {
+  const int L = cfg_.loadSteps, U = cfg_.unloadSteps;
+  if (cursor_ > L + U) return false;
+  const int k = cursor_++;
+  if (k == 0)
+    advanceTo(lat_.top0, true, 0, result);  // gravity/contact equilibrium and load-cell tare
+  else if (k <= L)
+    advanceTo(topY(double(k) / std::max(1, L), true), true, 0, result);
+  else
+    advanceTo(topY(double(k - L) / std::max(1, U), false), false, 0, result);
+  return cursor_ <= L + U;
+}
+
+// MC/src/sim.cpp:374-467
+SimulationResult Simulator::run() {
+  SimulationResult result;
+  result.frames.reserve(size_t(cfg_.loadSteps + cfg_.unloadSteps + 24));
+  result.diagnostics.reserve(size_t(cfg_.loadSteps + cfg_.unloadSteps + 24));
+  begin();
+  while (advance(result)) {
+  }
+  return result;
+}
+
+// MC/src/sim.cpp:511-536; the potential is evaluated on the device.
+bool Simulator::rigidRotationTest(double* error) const {
+  const int N = lat_.n_nodes;
+  const double jr = cfg_.jointScale * cfg_.maxRadius, height = lat_.height;
+  std::vector<Vec3> x(static_cast<size_t>(N));
+  std::vector<Quat> R(static_cast<size_t>(N));
+  for (int i = 0; i < N; ++i) {
+    Vec3 x0{lat_.x[3 * size_t(i)], lat_.x[3 * size_t(i) + 1], lat_.x[3 * size_t(i) + 2]};
+    double yf = std::max(0.0, std::min(1.0, (x0.y - jr) / std::max(1e-12, height)));
+    x[size_t(i)] = x0;
+    x[size_t(i)].y -= 0.018 * height * yf;
+    x[size_t(i)].x += 0.00018 * std::sin(2.0 * kPi * yf + 0.17 * double(i));
+    R[size_t(i)] = q_axis_angle({0, 0, 1}, 0.035 * yf);
+  }
+  auto potential = [&](const std::vector<Vec3>& xs, const std::vector<Quat>& Rs) {
+    std::vector<double> xa(3 * size_t(N)), qa(4 * size_t(N));
+    for (int i = 0; i < N; i++) {
+      xa[3 * size_t(i)] = xs[size_t(i)].x;
+      xa[3 * size_t(i) + 1] = xs[size_t(i)].y;
+      xa[3 * size_t(i) + 2] = xs[size_t(i)].z;
+      qa[4 * size_t(i)] = Rs[size_t(i)].w;
+      qa[4 * size_t(i) + 1] = Rs[size_t(i)].x;
+      qa[4 * size_t(i) + 2] = Rs[size_t(i)].y;
+      qa[4 * size_t(i) + 3] = Rs[size_t(i)].z;
+    }
+    double e = 0;
+    check(mmc_beam_potential(ctx_, xa.data(), qa.data(), &e));
+    return e;
+  };
+  const double e0 = potential(x, R);
+  const Quat Q = q_axis_angle(unit3(Vec3{0.37, 0.81, -0.44}), 1.137);
+  const Vec3 shift{0.081, -0.024, 0.053};
+  std::vector<Vec3> xr(static_cast<size_t>(N));
+  std::vector<Quat> Rr(static_cast<size_t>(N));
+  for (int i = 0; i < N; ++i) {
+    xr[size_t(i)] = add(qr(Q, x[size_t(i)]), shift);
+    Rr[size_t(i)] = qn(qm(Q, R[size_t(i)]));
+  }
+  const double e1 = potential(xr, Rr);
+  const double rel = std::abs(e1 - e0) / std::max({1e-12, std::abs(e0), std::abs(e1)});
+  if (error) *error = rel;
+  return rel < 1e-9;
+}
+
+// MC/src/sim.cpp:538-577; every objective evaluation runs on the device.
+bool Simulator::gradientCheck(double* relativeError) const {
+  const int N = lat_.n_nodes;
+  std::vector<double> z(6 * size_t(N), 0.0);
+  // the reference RNG stream (MC/include/core.hpp:46) seeded with seed ^ "GRADCHEK"
+  uint64_t state = splitmix(cfg_.seed ^ 0x475241444348454bULL);
+  auto uniform = [&]() {
+    state = splitmix(state);
+    return (double(uint32_t(state >> 32)) + 0.5) / 4294967296.0;
+  };
+  auto signed_uniform = [&]() { return 2 * uniform() - 1; };
+  for (int i = 0; i < N; ++i) {
+    const size_t b = 6 * size_t(i);
+    z[b + 0] = 0.003 * signed_uniform();
+    z[b + 1] = 0.100 + 0.003 * signed_uniform();  // lift off the lower platen
+    z[b + 2] = 0.003 * signed_uniform();
+    z[b + 3] = 0.004 * signed_uniform();
+    z[b + 4] = 0.004 * signed_uniform();
+    z[b + 5] = 0.004 * signed_uniform();
+    for (int d = 0; d < 6; ++d)
+      if (lat_.locked[b + size_t(d)]) z[b + size_t(d)] = 0.0;
+  }
+  const double top = lat_.top0 + 0.020;
+  std::vector<double> grad(z.size());
+  mmc_objective_out o{};
+  check(mmc_objective(ctx_, z.data(), top, 0, &o, grad.data()));
+  double num = 0, den = 0;
+  int checked = 0;
+  for (size_t k = 0; k < z.size() && checked < 42; ++k) {
+    if (lat_.locked[k]) continue;
+    if ((splitmix(k * 0x9e3779b97f4a7c15ULL + cfg_.seed) & 15ULL) > 1ULL) continue;
+    const double h = (k % 6 < 3) ? 2e-6 : 1e-6;
+    std::vector<double> zp = z, zm = z;
+    zp[k] += h;
+    zm[k] -= h;
+    mmc_objective_out op{}, om{};
+    check(mmc_objective(ctx_, zp.data(), top, 0, &op, nullptr));
+    check(mmc_objective(ctx_, zm.data(), top, 0, &om, nullptr));
+    const double numeric = (op.energy - om.energy) / (2.0 * h), analytic = grad[k];
+    num += (numeric - analytic) * (numeric - analytic);
+    den += numeric * numeric + analytic * analytic;
+    ++checked;
+  }
+  const double rel = std::sqrt(num / std::max(1e-30, den));
+  if (relativeError) *relativeError = rel;
+  return checked >= 24 && rel < 0.075;
+}
+
+}  // namespace meta
+
+// ===========================================================================
+// C entry points of the facade
+// ===========================================================================
+struct mmc_sim {
+  std::unique_ptr<meta::Simulator> sim;
+  std::string err;
+};
+
+namespace {
+std::string g_sim_create_error;
+mmc_frame to_frame(const meta::FrameDiagnostics& d) {
+  mmc_frame f;
+  std::memset(&f, 0, sizeof f);
+  f.time = d.time;
+  f.platen_y = d.platenY;
+  f.displacement = d.displacement;
+  f.engineering_strain = d.engineeringStrain;
+  f.engineering_stress = d.engineeringStress;
+  f.reaction_force = d.reactionForce;
+  f.loading_work = d.loadingWork;
+  f.recovered_work = d.recoveredWork;
+  f.dissipated_work = d.dissipatedWork;
+  f.specific_energy_absorption = d.specificEnergyAbsorption;
+  f.elastic_energy = d.elasticEnergy;
+  f.plastic_dissipation = d.plasticDissipation;
+  f.contact_energy = d.contactEnergy;
+  f.residual_rms = d.residualRms;
+  f.residual_max = d.residualMax;
+  f.max_penetration = d.maxPenetration;
+  f.mean_plastic_strain = d.meanPlasticStrain;
+  f.max_plastic_strain = d.maxPlasticStrain;
+  f.mean_plastic_rotation = d.meanPlasticRotation;
+  f.max_plastic_rotation = d.maxPlasticRotation;
+  f.permanent_set = d.permanentSet;
+  f.active_contacts = d.activeContacts;
+  f.plastic_beams = d.plasticBeams;
+  f.plastic_hinges = d.plasticHinges;
+  f.iterations = d.iterations;
+  f.converged = d.converged ? 1u : 0u;
+  f.loading = d.loading ? 1u : 0u;
+  return f;
+}
+template <class F>
+int sim_guard(mmc_sim* s, F&& f) {
+  if (!s || !s->sim) return MMC_ERR_ARG;
+  try {
+    f();
+    return MMC_OK;
+  } catch (const std::exception& e) {
+    s->err = e.what();
+    return MMC_ERR_EQUILIBRIUM;
+  }
+}
+}  // namespace
+
+extern "C" {
+
+int mmc_sim_create(const mmc_config* cfg, int32_t precision, mmc_sim** out) {
+  if (!cfg || !out) return MMC_ERR_ARG;
+  *out = nullptr;
+  auto* s = new mmc_sim();
+  try {
+    meta::SimulationConfig c = meta::from_c(*cfg);
+    c.precision = precision;
+    c.captureFrames = false;
+    c.verbose = false;
+    s->sim = std::make_unique<meta::Simulator>(c);
+  } catch (const std::exception& e) {
+    s->err = e.what();
+    g_sim_create_error = e.what();
+    *out = s;
+    return MMC_ERR_ARG;
+  }
+  *out = s;
+  return MMC_OK;
+}
+
+void mmc_sim_destroy(mmc_sim* s) { delete s; }
+
+const char* mmc_sim_last_error(const mmc_sim* s) { return s ? s->err.c_str() : g_sim_create_error.c_str(); }
+
+int mmc_sim_info(const mmc_sim* s, int32_t* nn, int32_t* nb, double* height, double* mass, double* area) {
+  if (!s || !s->sim) return MMC_ERR_ARG;
+  const meta::HostLattice& L = s->sim->lattice();
+  if (nn) *nn = L.n_nodes;
+  if (nb) *nb = L.n_beams;
+  if (height) *height = s->sim->initialHeight();
+  if (mass) *mass = s->sim->mass();
+  if (area) *area = s->sim->area();
+  return MMC_OK;
+}
+
+int mmc_sim_lattice(const mmc_sim* s, double* x, double* mass, uint8_t* locked, int32_t* a, int32_t* b, double* L0,
+                    double* radius, double* e0) {
+  if (!s || !s->sim) return MMC_ERR_ARG;
+  const meta::HostLattice& L = s->sim->lattice();
+  if (x) std::memcpy(x, L.x.data(), L.x.size() * sizeof(double));
+  if (mass) std::memcpy(mass, L.mass.data(), L.mass.size() * sizeof(double));
+  if (locked) std::memcpy(locked, L.locked.data(), L.locked.size());
+  if (a) std::memcpy(a, L.a.data(), L.a.size() * sizeof(int32_t));
+  if (b) std::memcpy(b, L.b.data(), L.b.size() * sizeof(int32_t));
+  if (L0) std::memcpy(L0, L.L0.data(), L.L0.size() * sizeof(double));
+  if (radius) std::memcpy(radius, L.radius.data(), L.radius.size() * sizeof(double));
+  if (e0) std::memcpy(e0, L.e0.data(), L.e0.size() * sizeof(double));
+  return MMC_OK;
+}
+
+int mmc_sim_run(mmc_sim* s, mmc_frame* frames, int32_t cap, int32_t* n, int32_t verbose) {
+  return sim_guard(s, [&] {
+    s->sim->setVerbose(verbose != 0);
+    meta::SimulationResult r;
+    std::string failure;
+    try {
+      r = s->sim->run();
+    } catch (const std::exception& e) {
+      failure = e.what();
+    }
+    (void)r;
+    if (!failure.empty()) throw std::runtime_error(failure);
+    int m = std::min<int>(cap, int(r.diagnostics.size()));
+    for (int i = 0; i < m && frames; i++) frames[i] = to_frame(r.diagnostics[size_t(i)]);
+    if (n) *n = int32_t(r.diagnostics.size());
+  });
+}
+
+int mmc_sim_begin(mmc_sim* s) {
+  return sim_guard(s, [&] { s->sim->begin(); });
+}
+
+int mmc_sim_advance(mmc_sim* s, mmc_frame* frames, int32_t cap, int32_t* n, int32_t* done) {
+  return sim_guard(s, [&] {
+    meta::SimulationResult r;
+    bool more = s->sim->advance(r);
+    int m = std::min<int>(cap, int(r.diagnostics.size()));
+    for (int i = 0; i < m && frames; i++) frames[i] = to_frame(r.diagnostics[size_t(i)]);
+    if (n) *n = int32_t(r.diagnostics.size());
+    if (done) *done = more ? 0 : 1;
+  });
+}
+
+int mmc_sim_rigid_rotation_test(mmc_sim* s, double* error, int32_t* pass) {
+  return sim_guard(s, [&] {
+    bool ok = s->sim->rigidRotationTest(error);
+    if (pass) *pass = ok;
+  });
+}
+
+int mmc_sim_gradient_check(mmc_sim* s, double* error, int32_t* pass) {
+  return sim_guard(s, [&] {
+    bool ok = s->sim->gradientCheck(error);
+    if (pass) *pass = ok;
+  });
+}
+
+mmc_ctx* mmc_sim_context(mmc_sim* s) { return (s && s->sim) ? s->sim->context() : nullptr; }
+
+int mmc_lattice_build(const mmc_config* cfg, mmc_lattice* view, void** handle) {
+  if (!cfg || !view || !handle) return MMC_ERR_ARG;
+  *handle = nullptr;
+  try {
+    mmc_config c = *cfg;
+    c.translation_scale = c.cell_size;
+    auto* L = new meta::HostLattice(meta::build_lattice(c));
+    *view = L->view();
+    *handle = L;
+  } catch (const std::exception& e) {
+    g_sim_create_error = e.what();
+    return MMC_ERR_ARG;
+  }
+  return MMC_OK;
+}
+
+void mmc_lattice_free(void* handle) { delete static_cast<meta::HostLattice*>(handle); }
+
+}  // extern "C"

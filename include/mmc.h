@@ -1,0 +1,241 @@
+/*
+ * mmc.h — C-ABI boundary of the gfx950 beam-lattice compression solver.
+ *
+ * The reference has no FFI: its boundary is the C++ class meta::Simulator
+ * (projects/metamaterial-compression/industry-grade/include/sim.hpp:35-54),
+ * whose private hot path is objective()/minimize()/commit()/state()
+ * (sim.hpp:48-53, src/sim.cpp:136-212).  This header is the thin C layer that
+ * a C++20 Simulator facade with the reference's public signatures calls into.
+ * Plain pointers and sizes only; every entry point returns an int status
+ * (MMC_OK = 0) and never throws; the message of the last failure on a context
+ * is returned by mmc_last_error().
+ *
+ * Paths below abbreviate projects/metamaterial-compression/industry-grade/ as MC/.
+ */
+#ifndef MMC_H
+#define MMC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes -------------------------------------------------------- */
+enum {
+  MMC_OK = 0,
+  MMC_ERR_ARG = 1,         /* bad argument / size mismatch                       */
+  MMC_ERR_HIP = 2,         /* HIP runtime failure (message names the call)       */
+  MMC_ERR_STATE = 3,       /* e.g. commit without trial states (MC/src/sim.cpp:212) */
+  MMC_ERR_EQUILIBRIUM = 4, /* equilibrium failed after bisection (MC/src/sim.cpp:412-418) */
+  MMC_ERR_CAPACITY = 5     /* a device buffer bound was exceeded                 */
+};
+
+/* ---- precision policies -------------------------------------------------- */
+enum {
+  MMC_FP64 = 0,  /* every element operation in FP64 (the reference's arithmetic) */
+  MMC_MIXED = 1  /* FP32 rotational element math; FP64 positions, strains, return
+                    maps, plastic state, L-BFGS vectors/dots and line search     */
+};
+
+/* ---- lattice families ---------------------------------------------------- */
+enum {
+  MMC_LATTICE_BCC = 0,   /* graded BCC accordion lattice of MC/src/sim.cpp:51-132 */
+  MMC_LATTICE_CUBIC = 1  /* simple cubic lattice, one element per cell edge       */
+};
+
+/* Material parameters: MaterialConfig, MC/include/sim.hpp:16. */
+typedef struct mmc_material {
+  double density, young, nu, yield, hardening, bend_hardening, torsion_hardening;
+} mmc_material;
+
+/* Solver configuration: SimulationConfig, MC/include/sim.hpp:17-26, same field
+ * meaning and defaults (mmc_config_defaults), followed by additive fields that
+ * have no counterpart in the reference. */
+typedef struct mmc_config {
+  int32_t cells_x, cells_y, cells_z;
+  int32_t elements_per_strut;
+  double cell_size, min_radius, max_radius, joint_scale, trigger_scale;
+  double imperfection, member_imperfection;
+  mmc_material material;
+  double gravity, shear_correction, contact_stiffness, contact_margin, centering_stiffness;
+  int32_t contact_samples, load_steps, unload_steps, max_iterations;
+  int32_t history, max_line_search, max_adaptive_refinements, fps;
+  int32_t frames, threads;
+  double max_strain, step_time, translation_scale, max_translation_step, max_rotation_step;
+  double gradient_rms_tolerance, gradient_max_tolerance, armijo, min_line_step;
+  double penetration_tolerance;
+  uint64_t seed;
+  /* additive: */
+  int32_t lattice;            /* MMC_LATTICE_*                                         */
+  int32_t relative_escape;    /* 1: accept rms/max(1e-6,ts*max(1,F)) < relative_tolerance
+                                 as the reference does (MC/src/sim.cpp:177); 0: off     */
+  double relative_tolerance;  /* 1.5e-3 in the reference                                */
+  double position_sigma;      /* cubic: Gaussian node jitter std (m); 0 selects the
+                                 reference's corner-node imperfection (sim.cpp:63-69)   */
+  int32_t radius_mode;        /* cubic: 0 linear grading min->max along y; 1 per-beam
+                                 uniform in [min_radius, max_radius]                    */
+  int32_t reserved;
+} mmc_config;
+
+/* Fills *cfg with the reference defaults of MC/include/sim.hpp:16-26 (BCC lattice,
+ * relative escape on). */
+void mmc_config_defaults(mmc_config* cfg);
+
+/* A lattice handed across the boundary.  Caller-owned; read only during the call.
+ * Node-major AoS for host convenience; the device stores SoA. */
+typedef struct mmc_lattice {
+  int32_t n_nodes, n_beams;
+  const double* x;        /* 3*n_nodes base positions (Node::x, sim.hpp:27)           */
+  const double* q;        /* 4*n_nodes base orientations w,x,y,z                      */
+  const double* mass;     /* n_nodes lumped masses (distributeMass, sim.cpp:134)       */
+  const uint8_t* locked;  /* 6*n_nodes DOF locks (sim.cpp:134)                          */
+  const int32_t* beam_a;  /* n_beams                                                  */
+  const int32_t* beam_b;  /* n_beams                                                  */
+  const double* L0;       /* n_beams rest lengths                                      */
+  const double* radius;   /* n_beams                                                   */
+  const double* area;     /* n_beams  pi r^2                                           */
+  const double* inertia;  /* n_beams  pi r^4 / 4  (J = 2 I)                             */
+  const double* e0;       /* 3*n_beams material directors                               */
+  const double* plastic;  /* 7*n_beams PlasticState epsp, alpha, rotp[3], alphaB, alphaT */
+  int64_t n_connected;    /* node pairs excluded from self-contact (sim.cpp:115,117)    */
+  const uint64_t* connected; /* pairKey(a,b) = min<<32 | max (MC/include/core.hpp:47)   */
+  const int32_t* node_layer;  /* n_nodes Node::layer (may be NULL)                        */
+  const int32_t* beam_layer;  /* n_beams Beam::layer (may be NULL)                        */
+  const int32_t* beam_family; /* n_beams Beam::family (may be NULL)                       */
+  double width, height, depth; /* specimen envelope (sim.cpp:56)                          */
+} mmc_lattice;
+
+/* Objective record: Objective/ObjectiveDiag, MC/include/sim.hpp:31-32, plus the
+ * unlocked-DOF gradient norms used by the convergence test (sim.cpp:44-45). */
+typedef struct mmc_objective_out {
+  double energy, elastic, plastic, contact, reaction, max_pen;
+  double rms, max;
+  int32_t contacts, finite;
+} mmc_objective_out;
+
+/* OptStats, MC/include/sim.hpp:33. */
+typedef struct mmc_opt_stats {
+  int32_t iterations, backtracks;
+  double rms, max, energy;
+  int32_t converged, evaluations;
+} mmc_opt_stats;
+
+/* Per-frame record: byte layout of DiskDiagnostics, MC/src/main.cpp:24-31
+ * (FrameDiagnostics, MC/include/sim.hpp:7-13). */
+typedef struct mmc_frame {
+  double time, platen_y, displacement, engineering_strain, engineering_stress, reaction_force;
+  double loading_work, recovered_work, dissipated_work, specific_energy_absorption;
+  double elastic_energy, plastic_dissipation, contact_energy, residual_rms, residual_max,
+      max_penetration;
+  double mean_plastic_strain, max_plastic_strain, mean_plastic_rotation, max_plastic_rotation,
+      permanent_set;
+  int32_t active_contacts, plastic_beams, plastic_hinges, iterations;
+  uint32_t converged, loading;
+} mmc_frame;
+
+/* Per-beam trial record: TrialBeam, MC/include/sim.hpp:30, flattened to 17 doubles:
+ * next[7], strain, stress, moment[3], elastic, dissipation, axialPlastic,
+ * bendPlastic, torsionPlastic (flags as 0/1). */
+#define MMC_TRIAL_STRIDE 17
+
+/* Plastic-state and geometry reductions of diagnostics(), MC/src/sim.cpp:339-370. */
+typedef struct mmc_state_stats {
+  double sum_axial, max_axial, sum_rot, max_rot, y_min, y_max;
+  int32_t plastic_beams, plastic_hinges;
+} mmc_state_stats;
+
+/* ==== single-lattice context (the hot path) ================================ */
+typedef struct mmc_ctx mmc_ctx;
+
+/* Uploads the lattice SoA to device 0's current HIP device and sizes every work
+ * buffer.  Replaces the setup half of Simulator::Simulator (MC/src/sim.cpp:49). */
+int mmc_create(const mmc_lattice* lat, const mmc_config* cfg, int32_t precision, mmc_ctx** out);
+void mmc_destroy(mmc_ctx* ctx);
+/* Message of the last failure on ctx (or of the last failed mmc_create when ctx
+ * is NULL).  Never NULL. */
+const char* mmc_last_error(const mmc_ctx* ctx);
+
+/* Simulator::objective(z, top, trials), MC/src/sim.cpp:138-172.  z and grad are
+ * 6*n_nodes node-major (DOF order tx,ty,tz,rx,ry,rz); grad may be NULL.  With
+ * trials != 0 the per-beam trial states stay on the device for mmc_commit and
+ * mmc_read_trial. */
+int mmc_objective(mmc_ctx* ctx, const double* z, double top, int32_t trials,
+                  mmc_objective_out* out, double* grad);
+
+/* Simulator::minimize(top, z, final), MC/src/sim.cpp:174-211, started from z = 0
+ * as advanceTo does (sim.cpp:398).  The minimiser, its final objective and the
+ * trial states stay on the device; z_out (6*n_nodes, may be NULL) receives z. */
+int mmc_minimize(mmc_ctx* ctx, double top, mmc_opt_stats* stats, mmc_objective_out* final_out,
+                 double* z_out);
+
+/* Simulator::commit(z, final), MC/src/sim.cpp:212: base pose <- state(z),
+ * plastic <- trial.next.  Fails with MMC_ERR_STATE if no trial states exist. */
+int mmc_commit(mmc_ctx* ctx);
+
+/* Reads the committed node poses (3N, 4N) and beam plastic states (7B); any may be NULL. */
+int mmc_read_state(mmc_ctx* ctx, double* x, double* q, double* plastic);
+/* Reads the last trial records (MMC_TRIAL_STRIDE * n_beams). */
+int mmc_read_trial(mmc_ctx* ctx, double* trial);
+/* Device reductions over the committed state for diagnostics (sim.cpp:339-370). */
+int mmc_read_stats(mmc_ctx* ctx, mmc_state_stats* out);
+
+/* Kernel timer (no reference counterpart): HIP events recorded on the context
+ * stream around every launch of one objective kernel (MMC_K_*) while enabled.
+ * mmc_kernel_timer(ctx, id) starts (and clears) it, id < 0 stops it;
+ * mmc_kernel_timer_read returns the launch count and the mean duration. */
+enum { MMC_K_STATE = 0, MMC_K_COUNT = 1, MMC_K_FILL = 2, MMC_K_SORT = 3, MMC_K_NARROW = 4,
+       MMC_K_BEAM = 5, MMC_K_GATHER = 6, MMC_K_FINAL = 7 };
+int mmc_kernel_timer(mmc_ctx* ctx, int32_t kernel);
+int mmc_kernel_timer_read(mmc_ctx* ctx, int64_t* launches, double* mean_ms);
+/* Objective evaluations performed by the context since creation. */
+int64_t mmc_evaluations(const mmc_ctx* ctx);
+
+/* Simulator::beamPotential(x, R), MC/src/sim.cpp:468-509, evaluated on the device
+ * against the committed plastic state.  x is 3N, q is 4N (w,x,y,z). */
+int mmc_beam_potential(mmc_ctx* ctx, const double* x, const double* q, double* out);
+
+/* ==== host lattice construction (no device work) ========================== */
+/* Builds the lattice the facade would build for cfg (MC/src/sim.cpp:49-135:
+ * build + distributeMass, or the simple cubic family) without touching a GPU.
+ * *view points into storage owned by *handle until mmc_lattice_free(handle). */
+int mmc_lattice_build(const mmc_config* cfg, mmc_lattice* view, void** handle);
+void mmc_lattice_free(void* handle);
+
+/* ==== Simulator facade (MC/include/sim.hpp:35-54) exposed for C callers ===== */
+typedef struct mmc_sim mmc_sim;
+
+/* Simulator::Simulator(cfg): builds the lattice on the host (BCC or cubic),
+ * distributes mass, sets the locks and creates the device context. */
+int mmc_sim_create(const mmc_config* cfg, int32_t precision, mmc_sim** out);
+void mmc_sim_destroy(mmc_sim* sim);
+const char* mmc_sim_last_error(const mmc_sim* sim);
+/* nodes().size(), beams().size(), initialHeight(), mass(), area(). */
+int mmc_sim_info(const mmc_sim* sim, int32_t* n_nodes, int32_t* n_beams, double* height,
+                 double* mass, double* area);
+/* The host lattice the facade built (same layout as mmc_lattice; arrays sized by
+ * mmc_sim_info).  Any pointer may be NULL. */
+int mmc_sim_lattice(const mmc_sim* sim, double* x, double* mass, uint8_t* locked, int32_t* beam_a,
+                    int32_t* beam_b, double* L0, double* radius, double* e0);
+/* Simulator::run(), MC/src/sim.cpp:374-467: tare + load + unload schedule with
+ * adaptive bisection.  Writes up to cap frames; *n_frames gets the count.
+ * With verbose != 0 prints the reference's "[equilibrium]" line per step. */
+int mmc_sim_run(mmc_sim* sim, mmc_frame* frames, int32_t cap, int32_t* n_frames, int32_t verbose);
+/* Stepwise form of mmc_sim_run: mmc_sim_begin resets the schedule (tare, load,
+ * unload; work accumulators as in MC/src/sim.cpp:379-387); each
+ * mmc_sim_advance solves the next scheduled increment (with its bisection
+ * sub-steps) and writes the accepted frames.  *done becomes 1 after the last
+ * unload increment. */
+int mmc_sim_begin(mmc_sim* sim);
+int mmc_sim_advance(mmc_sim* sim, mmc_frame* frames, int32_t cap, int32_t* n_frames, int32_t* done);
+/* Simulator::rigidRotationTest / gradientCheck, MC/src/sim.cpp:511-577. */
+int mmc_sim_rigid_rotation_test(mmc_sim* sim, double* error, int32_t* pass);
+int mmc_sim_gradient_check(mmc_sim* sim, double* error, int32_t* pass);
+/* The device context under the facade (for objective-level callers). */
+mmc_ctx* mmc_sim_context(mmc_sim* sim);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MMC_H */

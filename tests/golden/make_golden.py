@@ -1,0 +1,131 @@
+"""Writes the golden fixtures under tests/golden/.
+
+Reference vectors come from the reference solver compiled from its own sources
+(oracle/_ref/libmmcref.so, built by oracle/Makefile from
+/root/reference/projects/metamaterial-compression/industry-grade); the long
+cubic curve comes from the oracle once the oracle has been pinned against those
+vectors.  Run from the repository root:  python tests/golden/make_golden.py
+
+Fixtures:
+  bcc_222_ref.npz          2x4x2 graded BCC specimen (MC/docs/VALIDATION.md:9-11):
+                           lattice, objective/gradient/trial records at two
+                           seeded states, objectivity and FD-check values
+  bcc_222_run_ref.npy      its 10 + 7 increment run (the reference's "coarse"
+                           convergence case, MC/src/main.cpp:213-221)
+  cubic_444_ref.npz        4x4x4 cubic lattice injected into the reference:
+                           objective records at seeded, squashed and yielded
+                           states and a 4 + 2 increment run
+  cubic_8x8x4_oracle_frames.npy
+                           the 8x8x4 cubic workload, 60 + 60 increments, gradient
+                           tolerance 1e-6 (oracle)
+"""
+import sys
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parents[2]
+sys.path.insert(0, str(ROOT / "falling-dice_amd"))
+sys.path.insert(0, str(ROOT / "oracle"))
+import mmc  # noqa: E402
+import py_oracle as po  # noqa: E402
+
+OUT = Path(__file__).resolve().parent
+
+
+def seeded_state(n, seed, squash=0.0, y=None, jr=0.0, ts=1.0):
+    rng = np.random.default_rng(seed)
+    z = np.empty((n, 6))
+    z[:, :3] = rng.normal(0, 0.02, (n, 3))
+    z[:, 3:] = rng.normal(0, 0.02, (n, 3))
+    if squash:
+        z[:, 1] -= squash * (y - jr) / ts
+    return z.reshape(-1)
+
+
+def random_plastic(B, seed):
+    rng = np.random.default_rng(seed)
+    p = np.zeros((B, 7))
+    p[:, 0] = rng.normal(0, 0.01, B)
+    p[:, 1] = np.abs(rng.normal(0, 0.01, B))
+    p[:, 2:5] = rng.normal(0, 0.02, (B, 3))
+    p[:, 5] = np.abs(rng.normal(0, 0.01, B))
+    p[:, 6] = np.abs(rng.normal(0, 0.01, B))
+    return p.reshape(-1)
+
+
+def bcc_222_config():
+    return mmc.default_config().replace(cells_x=2, cells_y=4, cells_z=2, max_strain=0.5, load_steps=10,
+                                        unload_steps=7, max_iterations=500, contact_stiffness=1e6)
+
+
+def objective_record(s, z, top):
+    out, g, t = s.objective(z, top, trials=True)
+    rec = np.array([out.energy, out.elastic, out.plastic, out.contact, out.reaction, out.max_pen, out.rms,
+                    out.max, out.contacts])
+    return rec, g, t
+
+
+def cubic_444_config():
+    return mmc.cubic_config(4, 4, 4, load=4, unload=2, grad_tol=1e-3).replace(
+        relative_escape=1, gradient_max_tolerance=1.5e-2, max_iterations=600)
+
+
+def main():
+    if not po.available("ref"):
+        raise SystemExit("oracle/_ref/libmmcref.so missing: run make -C oracle with the reference present")
+    # ---- 2x4x2 BCC specimen ----
+    cfg = bcc_222_config()
+    r = po.Solver(cfg, "ref")
+    L = r.lattice()
+    y = L.x.reshape(-1, 3)[:, 1]
+    data = dict(x=L.x, mass=L.mass, locked=L.locked, a=L.a, b=L.b, L0=L.L0, radius=L.radius, e0=L.e0,
+                connected=np.sort(L.connected), node_layer=L.node_layer, env=np.array(
+                    [r.width, r.height, r.depth, r.area, r.mass, r.top0]))
+    jr = cfg.joint_scale * cfg.max_radius
+    for name, z, top in [("s0", seeded_state(r.n_nodes, 1), r.top0 - 0.004),
+                         ("s1", seeded_state(r.n_nodes, 2, squash=0.5, y=y, jr=jr, ts=cfg.cell_size),
+                          r.top0 - 0.01)]:
+        rec, g, t = objective_record(r, z, top)
+        data[f"{name}_z"], data[f"{name}_top"] = z, np.array([top])
+        data[f"{name}_rec"], data[f"{name}_grad"], data[f"{name}_trial"] = rec, g, t
+    ok, e = r.rigid_rotation_test()
+    data["objectivity"] = np.array([e, ok])
+    ok, e = r.gradient_check()
+    data["fdcheck"] = np.array([e, ok])
+    np.savez_compressed(OUT / "bcc_222_ref.npz", **data)
+    frames = po.Solver(cfg, "ref").run()
+    np.save(OUT / "bcc_222_run_ref.npy", frames)
+    print("bcc_222:", r.n_nodes, r.n_beams, "frames", len(frames), "peak", frames["reaction_force"].max())
+
+    # ---- 4x4x4 cubic lattice injected into the reference ----
+    cfg = cubic_444_config()
+    lat = po.Solver(cfg, "orc").lattice()
+    r = po.Solver(cfg, "ref", lattice=lat)
+    y = lat.x.reshape(-1, 3)[:, 1]
+    jr = cfg.joint_scale * cfg.max_radius
+    data = {}
+    states = [("random", seeded_state(r.n_nodes, 3), r.top0 - 0.01 * r.height, None),
+              ("squash", seeded_state(r.n_nodes, 4, 0.8, y, jr, cfg.cell_size), r.top0 - 0.1 * r.height, None),
+              ("yielded", seeded_state(r.n_nodes, 5), r.top0 - 0.02 * r.height, random_plastic(r.n_beams, 6))]
+    for name, z, top, plastic in states:
+        rr = po.Solver(cfg, "ref", lattice=lat)
+        if plastic is not None:
+            rr.set_plastic(plastic)
+            data[f"{name}_plastic"] = plastic
+        rec, g, t = objective_record(rr, z, top)
+        data[f"{name}_z"], data[f"{name}_top"] = z, np.array([top])
+        data[f"{name}_rec"], data[f"{name}_grad"], data[f"{name}_trial"] = rec, g, t
+    frames = r.run()
+    data["run_frames"] = frames
+    np.savez_compressed(OUT / "cubic_444_ref.npz", **data)
+    print("cubic_444: contacts", [int(data[f"{n}_rec"][8]) for n, *_ in states], "frames", len(frames))
+
+    # ---- 8x8x4 cubic workload curve (oracle) ----
+    frames = po.Solver(mmc.cubic_config(8, 8, 4), "orc").run()
+    np.save(OUT / "cubic_8x8x4_oracle_frames.npy", frames)
+    print("cubic_8x8x4: frames", len(frames), "peak", frames["reaction_force"].max())
+
+
+if __name__ == "__main__":
+    main()
