@@ -61,6 +61,7 @@ struct Scalars {
   double top;         // platen position
   double cell;        // broadphase cell
   double inv_cell;
+  double skin;        // Verlet skin of the contact pair list
   unsigned table_mask;
   int entry_cap;
 };
@@ -106,7 +107,7 @@ __device__ __forceinline__ void reduce_partials(const double* part, int P, doubl
 #pragma unroll
   for (int k = 0; k < K; k++) {
     bool mx = (maxmask >> k) & 1;
-    double a = mx ? 0.0 : 0.0;
+    double a = mx ? -INFINITY : 0.0;
     for (int i = threadIdx.x; i < P; i += kBlock) a = mx ? fmax(a, part[k * P + i]) : a + part[k * P + i];
     v[k] = a;
   }

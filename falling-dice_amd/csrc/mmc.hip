@@ -56,15 +56,17 @@ __device__ __forceinline__ quat<T> node_R(const T* R, int N, int i) {
 
 // ---------------------------------------------------------------------------
 // k_state: z (+ step p) -> x, R; gravity, platens, centroid.  MC/src/sim.cpp:136,142-143,168-171
-// partials [7][P]: E_gravity, E_platen, reaction, max_pen, platen_contacts, sum_x, sum_z
+// partials [8][P]: E_gravity, E_platen, reaction, max_pen, platen_contacts,
+// sum_x, sum_z, max displacement from xref (the positions of the last contact
+// list build)
 // ---------------------------------------------------------------------------
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_state(Lat<T> L, Scalars S, const double* __restrict__ z,
                                                   const double* __restrict__ p, double step, double* __restrict__ zc,
                                                   double* __restrict__ x, T* __restrict__ R,
-                                                  double* __restrict__ part) {
+                                                  const double* __restrict__ xref, double* __restrict__ part) {
   const int N = L.N, i = blockIdx.x * kBlock + threadIdx.x, P = gridDim.x;
-  double v[7] = {0, 0, 0, 0, 0, 0, 0};
+  double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   if (i < N) {
     const unsigned lk = L.lock[i];
     double q[6];
@@ -103,17 +105,31 @@ __global__ __launch_bounds__(kBlock) void k_state(Lat<T> L, Scalars S, const dou
     }
     v[5] = pos.x;
     v[6] = pos.z;
+    if (xref) {
+      v3<double> d{pos.x - xref[i], pos.y - xref[N + i], pos.z - xref[2 * N + i]};
+      double dn = norm(d);
+      v[7] = isfinite(dn) ? dn : INFINITY;
+    }
   }
-  block_reduce<7>(v, 1u << 3);
+  block_reduce<8>(v, (1u << 3) | (1u << 7));
   if (threadIdx.x == 0)
 #pragma unroll
-    for (int k = 0; k < 7; k++) part[k * P + blockIdx.x] = v[k];
+    for (int k = 0; k < 8; k++) part[k * P + blockIdx.x] = v[k];
 }
 
 // ---------------------------------------------------------------------------
-// Broadphase: shapes 0..N-1 are joint spheres, N..N+B-1 strut capsules
-// (MC/src/sim.cpp:148-151).  Each shape stores a conservative float AABB padded
-// by r + margin; the same AABB drives counting, filling and the narrowphase.
+// Contact.  Shapes 0..N-1 are joint spheres, N..N+B-1 strut capsules
+// (MC/src/sim.cpp:148-151).  Candidate partners come from a Verlet list: every
+// non-excluded pair closer than target + skin at the positions x_ref of the
+// last build.  While no node has moved by skin/2 or more since that build,
+// every pair closer than its contact target is in the list (a segment point
+// moves by at most the larger of its endpoint displacements), so evaluating
+// the list gives exactly the contact set of the full broadphase.  The list is
+// rebuilt with a uniform-grid spatial hash: each shape's float AABB, padded by
+// r + margin + skin/2, is counted and scattered into hashed cell buckets; a pair
+// is enumerated only in the cell that holds the low corner of the two boxes'
+// overlap (each overlapping pair once per partner); buckets are sorted so the
+// partner order, and therefore every contact sum, is reproducible.
 // ---------------------------------------------------------------------------
 template <class T>
 __device__ __forceinline__ void shape_geom(const Lat<T>& L, const Scalars& S, const double* x, int s, v3<double>& a,
@@ -146,7 +162,7 @@ __global__ __launch_bounds__(kBlock) void k_count(Lat<T> L, Scalars S, const dou
   double r;
   int n0, n1;
   shape_geom(L, S, x, s, a, b, r, n0, n1);
-  double pad = r + S.margin;
+  double pad = r + S.margin + 0.5 * S.skin;
   float lo[3] = {round_down(fmin(a.x, b.x) - pad), round_down(fmin(a.y, b.y) - pad), round_down(fmin(a.z, b.z) - pad)};
   float hi[3] = {round_up(fmax(a.x, b.x) + pad), round_up(fmax(a.y, b.y) + pad), round_up(fmax(a.z, b.z) + pad)};
 #pragma unroll
@@ -164,9 +180,9 @@ __global__ __launch_bounds__(kBlock) void k_count(Lat<T> L, Scalars S, const dou
   }
   long ncell = ok ? (long)(c1[0] - c0[0] + 1) * (c1[1] - c0[1] + 1) * (c1[2] - c0[2] + 1) : 0;
   if (!ok || ncell > kMaxCellsPerShape) {
-    // A non-finite pose already makes the energy non-finite; an absurdly large
-    // box marks the evaluation non-finite (k_final) so that the line search
-    // rejects it.  Either way the shape is left out of the grid.
+    // Left out of the grid.  A non-finite pose already makes the energy
+    // non-finite; an absurdly large box is reported (flag 1) and the
+    // evaluation is marked non-finite so that the line search rejects it.
     if (ok) atomicOr(flags, 1);
     box[s] = INFINITY;
     return;
@@ -268,79 +284,118 @@ __device__ __forceinline__ closest shape_closest(bool capA, v3<double> a0, v3<do
   return {0, 0, a0, b0};
 }
 
-// k_narrow: MC/src/sim.cpp:159-165.  partials [3][P]: E_contact, max_pen, contacts
-template <class T>
-__global__ __launch_bounds__(kBlock) void k_narrow(Lat<T> L, Scalars S, const double* __restrict__ x,
-                                                   const float* __restrict__ box,
-                                                   const unsigned* __restrict__ start,
-                                                   const int* __restrict__ entries, double* __restrict__ cf_sph,
-                                                   double* __restrict__ cf_cap, double* __restrict__ part) {
-  const int N = L.N, B = L.B, NS = N + B, s = blockIdx.x * kBlock + threadIdx.x, P = gridDim.x;
-  double v[3] = {0, 0, 0};
-  if (s < NS) {
-    v3<double> f0{0, 0, 0}, f1{0, 0, 0};
-    float lo[3], hi[3];
-    int c0[3], c1[3];
+// Verlet-list build: per shape, partners (either side) closer than target +
+// skin.  WRITE = false counts them into pcount; WRITE = true writes them at
+// poff[s] in the enumeration order (cells in fixed order, sorted buckets).
+template <class T, bool WRITE>
+__global__ __launch_bounds__(kBlock) void k_pairs(Lat<T> L, Scalars S, const double* __restrict__ x,
+                                                  const float* __restrict__ box, const unsigned* __restrict__ start,
+                                                  const int* __restrict__ entries, unsigned* __restrict__ pcount,
+                                                  const unsigned* __restrict__ poff, int* __restrict__ plist,
+                                                  unsigned cap, int* __restrict__ flags) {
+  const int N = L.N, NS = N + L.B, s = blockIdx.x * kBlock + threadIdx.x;
+  if (s >= NS) return;
+  float lo[3], hi[3];
+  int c0[3], c1[3];
+  unsigned n = 0, base = WRITE ? poff[s] : 0u;
+  if (load_cells(box, NS, s, S.inv_cell, lo, hi, c0, c1)) {
     v3<double> sa, sb;
     double sr;
     int s0, s1;
     shape_geom(L, S, x, s, sa, sb, sr, s0, s1);
     const bool scap = s >= N;
-    if (load_cells(box, NS, s, S.inv_cell, lo, hi, c0, c1)) {
-      for (int cj = c0[1]; cj <= c1[1]; cj++)
-        for (int ck = c0[2]; ck <= c1[2]; ck++)
-          for (int ci = c0[0]; ci <= c1[0]; ci++) {
-            unsigned h = cell_hash(ci, cj, ck, S.table_mask);
-            unsigned e0 = start[h], e1 = min(start[h + 1], (unsigned)S.entry_cap);
-            for (unsigned e = e0; e < e1; e++) {
-              int o = entries[e];
-              if (o == s) continue;
-              float olo[3], ohi[3];
+    for (int cj = c0[1]; cj <= c1[1]; cj++)
+      for (int ck = c0[2]; ck <= c1[2]; ck++)
+        for (int ci = c0[0]; ci <= c1[0]; ci++) {
+          unsigned h = cell_hash(ci, cj, ck, S.table_mask);
+          unsigned e0 = start[h], e1 = min(start[h + 1], (unsigned)S.entry_cap);
+          for (unsigned e = e0; e < e1; e++) {
+            int o = entries[e];
+            if (o == s) continue;
+            float olo[3], ohi[3];
 #pragma unroll
-              for (int d = 0; d < 3; d++) {
-                olo[d] = box[d * NS + o];
-                ohi[d] = box[(3 + d) * NS + o];
-              }
-              if (!(olo[0] <= hi[0] && lo[0] <= ohi[0] && olo[1] <= hi[1] && lo[1] <= ohi[1] && olo[2] <= hi[2] &&
-                    lo[2] <= ohi[2]))
-                continue;
-              // home cell of the pair: the cell holding the low corner of the overlap
-              if (cell_of(fmaxf(lo[0], olo[0]), S.inv_cell) != ci || cell_of(fmaxf(lo[1], olo[1]), S.inv_cell) != cj ||
-                  cell_of(fmaxf(lo[2], olo[2]), S.inv_cell) != ck)
-                continue;
-              v3<double> oa, ob;
-              double orr;
-              int o0, o1;
-              shape_geom(L, S, x, o, oa, ob, orr, o0, o1);
-              if (excluded(L, s0, s1, o0, o1)) continue;
-              const bool low = s < o;
-              const bool ocap = o >= N;
-              closest cp = low ? shape_closest(scap, sa, sb, ocap, oa, ob) : shape_closest(ocap, oa, ob, scap, sa, sb);
-              v3<double> delta = cp.pb - cp.pa;
-              double dist = norm(delta);
-              double target = (low ? sr + orr : orr + sr) + S.margin;
-              if (dist >= target) continue;
-              int ia = low ? s : o, ib = low ? o : s;
-              v3<double> n = dist > 1e-12 ? over(delta, dist)
-                                          : unit(v3<double>{0.37 + 0.001 * ia, 0.73, -0.41 + 0.001 * ib});
-              double pen = target - dist, mag = S.kc * pen;
-              v3<double> fv = low ? n * mag : n * (-mag);
-              double u = low ? cp.s : cp.t;
-              if (scap) {
-                f0 = f0 + fv * (1 - u);
-                f1 = f1 + fv * u;
-              } else {
-                f0 = f0 + fv;
-              }
-              if (low) {
-                v[0] += 0.5 * S.kc * pen * pen;
-                v[1] = fmax(v[1], pen);
-                v[2] += 1;
-              }
+            for (int d = 0; d < 3; d++) {
+              olo[d] = box[d * NS + o];
+              ohi[d] = box[(3 + d) * NS + o];
             }
+            if (!(olo[0] <= hi[0] && lo[0] <= ohi[0] && olo[1] <= hi[1] && lo[1] <= ohi[1] && olo[2] <= hi[2] &&
+                  lo[2] <= ohi[2]))
+              continue;
+            if (cell_of(fmaxf(lo[0], olo[0]), S.inv_cell) != ci || cell_of(fmaxf(lo[1], olo[1]), S.inv_cell) != cj ||
+                cell_of(fmaxf(lo[2], olo[2]), S.inv_cell) != ck)
+              continue;
+            v3<double> oa, ob;
+            double orr;
+            int o0, o1;
+            shape_geom(L, S, x, o, oa, ob, orr, o0, o1);
+            if (excluded(L, s0, s1, o0, o1)) continue;
+            const bool low = s < o, ocap = o >= N;
+            closest cp = low ? shape_closest(scap, sa, sb, ocap, oa, ob) : shape_closest(ocap, oa, ob, scap, sa, sb);
+            double dist = norm(cp.pb - cp.pa);
+            if (!(dist < sr + orr + S.margin + S.skin)) continue;
+            if (WRITE) {
+              if (base + n < cap)
+                plist[base + n] = o;
+              else
+                atomicOr(flags, 8);
+            }
+            n++;
           }
+        }
+  }
+  if (!WRITE) pcount[s] = n;
+}
+
+// k_contact: MC/src/sim.cpp:159-165 over the Verlet list.  Each shape
+// accumulates its own force; each pair's energy is counted by its lower shape.
+// partials [3][P]: E_contact, max_pen, contacts
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_contact(Lat<T> L, Scalars S, const double* __restrict__ x,
+                                                    const unsigned* __restrict__ poff, const int* __restrict__ plist,
+                                                    double* __restrict__ cf_sph, double* __restrict__ cf_cap,
+                                                    double* __restrict__ part) {
+  const int N = L.N, B = L.B, NS = N + B, s = blockIdx.x * kBlock + threadIdx.x, P = gridDim.x;
+  double v[3] = {0, 0, 0};
+  if (s < NS) {
+    v3<double> f0{0, 0, 0}, f1{0, 0, 0};
+    const unsigned e0 = poff[s], e1 = poff[s + 1];
+    if (e1 > e0) {
+      v3<double> sa, sb;
+      double sr;
+      int s0, s1;
+      shape_geom(L, S, x, s, sa, sb, sr, s0, s1);
+      const bool scap = s >= N;
+      for (unsigned e = e0; e < e1; e++) {
+        const int o = plist[e];
+        v3<double> oa, ob;
+        double orr;
+        int o0, o1;
+        shape_geom(L, S, x, o, oa, ob, orr, o0, o1);
+        const bool low = s < o, ocap = o >= N;
+        closest cp = low ? shape_closest(scap, sa, sb, ocap, oa, ob) : shape_closest(ocap, oa, ob, scap, sa, sb);
+        v3<double> delta = cp.pb - cp.pa;
+        double dist = norm(delta);
+        double target = (low ? sr + orr : orr + sr) + S.margin;
+        if (dist >= target) continue;
+        int ia = low ? s : o, ib = low ? o : s;
+        v3<double> n = dist > 1e-12 ? over(delta, dist) : unit(v3<double>{0.37 + 0.001 * ia, 0.73, -0.41 + 0.001 * ib});
+        double pen = target - dist, mag = S.kc * pen;
+        v3<double> fv = low ? n * mag : n * (-mag);
+        double u = low ? cp.s : cp.t;
+        if (scap) {
+          f0 = f0 + fv * (1 - u);
+          f1 = f1 + fv * u;
+        } else {
+          f0 = f0 + fv;
+        }
+        if (low) {
+          v[0] += 0.5 * S.kc * pen * pen;
+          v[1] = fmax(v[1], pen);
+          v[2] += 1;
+        }
+      }
     }
-    if (scap) {
+    if (s >= N) {
       int k = s - N;
       cf_cap[k] = f0.x;
       cf_cap[B + k] = f0.y;
@@ -523,7 +578,8 @@ __global__ __launch_bounds__(kBlock) void k_gather(Lat<T> L, Scalars S, const do
 }
 
 // Record layout (doubles): 0 energy, 1 elastic, 2 plastic, 3 contact, 4 reaction,
-// 5 max_pen, 6 rms, 7 max, 8 contacts, 9 finite, 10 flags
+// 5 max_pen, 6 rms, 7 max, 8 contacts, 9 finite, 10 flags, 11 largest node
+// displacement since the contact list was built
 constexpr int kRec = 12;
 
 __global__ __launch_bounds__(kBlock) void k_final(int N, double centering, double n_free,
@@ -532,8 +588,8 @@ __global__ __launch_bounds__(kBlock) void k_final(int N, double centering, doubl
                                                   const double* __restrict__ bpart, int Pb,
                                                   const double* __restrict__ gpart, int Pg,
                                                   int* __restrict__ flags, double* __restrict__ out) {
-  double s[7], c[3], b[3], g[2];
-  reduce_partials<7>(spart, Ps, s, 1u << 3);
+  double s[8], c[3], b[3], g[2];
+  reduce_partials<8>(spart, Ps, s, (1u << 3) | (1u << 7));
   reduce_partials<3>(cpart, Pc, c, 1u << 1);
   reduce_partials<3>(bpart, Pb, b, 0u);
   reduce_partials<2>(gpart, Pg, g, 1u << 1);
@@ -555,6 +611,7 @@ __global__ __launch_bounds__(kBlock) void k_final(int N, double centering, doubl
     out[8] = s[4] + c[2];
     out[9] = fin ? 1.0 : 0.0;
     out[10] = (double)fl;
+    out[11] = s[7];
   }
 }
 
@@ -603,55 +660,33 @@ __global__ __launch_bounds__(kBlock) void k_sum1(const double* __restrict__ part
 }
 
 // ---------------------------------------------------------------------------
-// L-BFGS vector kernels (thread per node, 6 DOF planes).  MC/src/sim.cpp:43-46,184-191,205-206
+// L-BFGS vector kernels (thread per node, 6 DOF planes), MC/src/sim.cpp:43-46,
+// 184-191, 205-206.  The direction p = -H g of the two-loop recursion is applied
+// in the compact form of Byrd, Nocedal & Schnabel (Math. Prog. 63, 1994, eq.
+// 2.6): H g = gamma g + sum_j a_j s_j + c_j y_j, with the small coefficient
+// system solved from the pair inner products.  Two vector passes per iteration
+// (pair + dots, direction) replace the recursion's 2m dependent passes.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ double plane_dot(const double* a, const double* b, int N, int i) {
-  double s = 0;
-  if (i < N)
-#pragma unroll
-    for (int d = 0; d < 6; d++) s += a[d * N + i] * b[d * N + i];
-  return s;
-}
+constexpr int kMaxHist = 16;
+struct DirCoef {
+  double cg;
+  int k;
+  int slot[kMaxHist];
+  double cs[kMaxHist], cy[kMaxHist];
+};
+struct PairArgs {
+  int t, nold;
+  int slot[kMaxHist];
+};
+
 __device__ __forceinline__ void write_dot_partial(double v, double* part) {
   double t[1] = {v};
   block_reduce<1>(t, 0u);
   if (threadIdx.x == 0) part[blockIdx.x] = t[0];
 }
 
-// p = src * scale; partial dot(D, p) when D is given
-__global__ __launch_bounds__(kBlock) void k_tl_init(int N, const double* __restrict__ src, double scale,
-                                                    double* __restrict__ p, const double* __restrict__ D,
-                                                    double* __restrict__ part) {
-  const int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i < N)
-#pragma unroll
-    for (int d = 0; d < 6; d++) p[d * N + i] = src[d * N + i] * scale;
-  if (D) write_dot_partial(plane_dot(D, p, N, i), part);
-}
-
-// backward step h: alpha_h = rho_h * (s_h . p);  p -= alpha_h y_h;  (last: p *= H0)
-// writes alpha_h to alpha[h]; partial dot(D, p) for the next step
-__global__ __launch_bounds__(kBlock) void k_tl_back(int N, double* __restrict__ p, const double* __restrict__ Y,
-                                                    const double* __restrict__ rho, int h,
-                                                    const double* __restrict__ part_in, int Pin,
-                                                    double* __restrict__ alpha, double post,
-                                                    const double* __restrict__ D, double* __restrict__ part) {
-  double sdp[1];
-  reduce_partials<1>(part_in, Pin, sdp, 0u);
-  const double al = rho[h] * sdp[0];
-  if (blockIdx.x == 0 && threadIdx.x == 0) alpha[h] = al;
-  const int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i < N)
-#pragma unroll
-    for (int d = 0; d < 6; d++) {
-      double v = p[d * N + i] - al * Y[d * N + i];
-      p[d * N + i] = v * post;
-    }
-  write_dot_partial(plane_dot(D, p, N, i), part);
-}
-
-// Direction epilogue: p = -p, zero locked DOFs, clamp per node (clampDirection,
-// MC/src/sim.cpp:46), partial dot(g, p).
+// Direction epilogue: zero locked DOFs, clamp per node (clampDirection,
+// MC/src/sim.cpp:46).
 __device__ __forceinline__ void finish_node(double (&q)[6], unsigned lk, double mt, double mr) {
 #pragma unroll
   for (int d = 0; d < 6; d++)
@@ -675,88 +710,102 @@ __device__ __forceinline__ void finish_node(double (&q)[6], unsigned lk, double 
     if ((lk >> d) & 1u) q[d] = 0.0;
 }
 
-// forward step h: beta = rho_h (y_h . p); p += s_h (alpha_h - beta); partial dot(D, p).
-// With finish: then p = -p, lock, clamp and partial dot(g, p) instead.
-__global__ __launch_bounds__(kBlock) void k_tl_fwd(int N, double* __restrict__ p, const double* __restrict__ Sh,
-                                                   const double* __restrict__ rho, const double* __restrict__ alpha,
-                                                   int h, const double* __restrict__ part_in, int Pin,
-                                                   const double* __restrict__ D, int finish,
-                                                   const uint8_t* __restrict__ lock, const double* __restrict__ g,
-                                                   double mt, double mr, double* __restrict__ part) {
-  double ydp[1];
-  reduce_partials<1>(part_in, Pin, ydp, 0u);
-  const double coef = alpha[h] - rho[h] * ydp[0];
-  const int i = blockIdx.x * kBlock + threadIdx.x;
-  double dv = 0;
-  if (i < N) {
-    double q[6];
-#pragma unroll
-    for (int d = 0; d < 6; d++) q[d] = p[d * N + i] + Sh[d * N + i] * coef;
-    if (finish) {
-#pragma unroll
-      for (int d = 0; d < 6; d++) q[d] = -q[d];
-      finish_node(q, lock[i], mt, mr);
-    }
-#pragma unroll
-    for (int d = 0; d < 6; d++) p[d * N + i] = q[d];
-    const double* Dv = finish ? g : D;
-    if (Dv)
-#pragma unroll
-      for (int d = 0; d < 6; d++) dv += Dv[d * N + i] * q[d];
-  }
-  if (finish || D) write_dot_partial(dv, part);
-}
-
-// p = -(src * scale), lock, clamp; partial dot(g, p)  (H0 direction with an
-// empty history, or the steepest-descent reset)
-__global__ __launch_bounds__(kBlock) void k_tl_scaled(int N, const double* __restrict__ src, double scale,
-                                                      double* __restrict__ p, const uint8_t* __restrict__ lock,
-                                                      const double* __restrict__ g, double mt, double mr,
+// p = -(cg g + sum_j cs_j s_j + cy_j y_j), locked and clamped; partial g.p
+__global__ __launch_bounds__(kBlock) void k_direction(int N, size_t n6, const double* __restrict__ g,
+                                                      const double* __restrict__ Sh, const double* __restrict__ Yh,
+                                                      DirCoef C, double* __restrict__ p,
+                                                      const uint8_t* __restrict__ lock, double mt, double mr,
                                                       double* __restrict__ part) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
   double dv = 0;
   if (i < N) {
-    double q[6];
+    double gv[6], q[6];
 #pragma unroll
-    for (int d = 0; d < 6; d++) q[d] = -(src[d * N + i] * scale);
+    for (int d = 0; d < 6; d++) {
+      gv[d] = g[d * N + i];
+      q[d] = C.cg * gv[d];
+    }
+    for (int j = 0; j < C.k; j++) {
+      const double* s = Sh + n6 * C.slot[j];
+      const double* y = Yh + n6 * C.slot[j];
+#pragma unroll
+      for (int d = 0; d < 6; d++) q[d] += C.cs[j] * s[d * N + i] + C.cy[j] * y[d * N + i];
+    }
+#pragma unroll
+    for (int d = 0; d < 6; d++) q[d] = -q[d];
     finish_node(q, lock[i], mt, mr);
 #pragma unroll
     for (int d = 0; d < 6; d++) {
       p[d * N + i] = q[d];
-      dv += g[d * N + i] * q[d];
+      dv += gv[d] * q[d];
     }
   }
   write_dot_partial(dv, part);
 }
 
-// s = zc - z, y = gc - g into the history slot; partials [3][P]: s.y, s.s, y.y
-__global__ __launch_bounds__(kBlock) void k_pair(int N, const double* __restrict__ z, const double* __restrict__ zc,
-                                                 const double* __restrict__ g, const double* __restrict__ gc,
-                                                 double* __restrict__ S, double* __restrict__ Y,
-                                                 double* __restrict__ part) {
+// s = zc - z, y = gc - g into history slot t.  Partials [4*nold + 5][P]: for
+// each retained pair j: s_j.y, y_j.y, s_j.gc, y_j.gc; then s.y, s.s, y.y,
+// s.gc, y.gc.
+__global__ __launch_bounds__(kBlock) void k_pair_dots(int N, size_t n6, const double* __restrict__ z,
+                                                      const double* __restrict__ zc, const double* __restrict__ g,
+                                                      const double* __restrict__ gc, double* __restrict__ Sh,
+                                                      double* __restrict__ Yh, PairArgs A, double* __restrict__ part) {
   const int i = blockIdx.x * kBlock + threadIdx.x, P = gridDim.x;
-  double v[3] = {0, 0, 0};
-  if (i < N)
+  double sv[6] = {0, 0, 0, 0, 0, 0}, yv[6] = {0, 0, 0, 0, 0, 0}, gv[6] = {0, 0, 0, 0, 0, 0};
+  if (i < N) {
+    double* st = Sh + n6 * A.t;
+    double* yt = Yh + n6 * A.t;
 #pragma unroll
     for (int d = 0; d < 6; d++) {
-      double s = zc[d * N + i] - z[d * N + i], y = gc[d * N + i] - g[d * N + i];
-      S[d * N + i] = s;
-      Y[d * N + i] = y;
-      v[0] += s * y;
-      v[1] += s * s;
-      v[2] += y * y;
+      gv[d] = gc[d * N + i];
+      sv[d] = zc[d * N + i] - z[d * N + i];
+      yv[d] = gv[d] - g[d * N + i];
+      st[d * N + i] = sv[d];
+      yt[d * N + i] = yv[d];
     }
-  block_reduce<3>(v, 0u);
+  }
+  for (int jj = 0; jj < A.nold; jj++) {
+    double v[4] = {0, 0, 0, 0};
+    if (i < N) {
+      const double* sj = Sh + n6 * A.slot[jj];
+      const double* yj = Yh + n6 * A.slot[jj];
+#pragma unroll
+      for (int d = 0; d < 6; d++) {
+        const double a = sj[d * N + i], b = yj[d * N + i];
+        v[0] += a * yv[d];
+        v[1] += b * yv[d];
+        v[2] += a * gv[d];
+        v[3] += b * gv[d];
+      }
+    }
+    block_reduce<4>(v, 0u);
+    if (threadIdx.x == 0)
+#pragma unroll
+      for (int c = 0; c < 4; c++) part[(4 * jj + c) * P + blockIdx.x] = v[c];
+  }
+  double w[5] = {0, 0, 0, 0, 0};
+#pragma unroll
+  for (int d = 0; d < 6; d++) {
+    w[0] += sv[d] * yv[d];
+    w[1] += sv[d] * sv[d];
+    w[2] += yv[d] * yv[d];
+    w[3] += sv[d] * gv[d];
+    w[4] += yv[d] * gv[d];
+  }
+  block_reduce<5>(w, 0u);
   if (threadIdx.x == 0)
 #pragma unroll
-    for (int k = 0; k < 3; k++) part[k * P + blockIdx.x] = v[k];
+    for (int c = 0; c < 5; c++) part[(4 * A.nold + c) * P + blockIdx.x] = w[c];
 }
 
-__global__ __launch_bounds__(kBlock) void k_sum3(const double* __restrict__ part, int P, double* __restrict__ out) {
-  double s[3];
-  reduce_partials<3>(part, P, s, 0u);
-  if (threadIdx.x == 0)
-    for (int k = 0; k < 3; k++) out[k] = s[k];
+// Sums K rows of block partials [K][P] into out[K] (one block, fixed order).
+__global__ __launch_bounds__(kBlock) void k_sum_rows(const double* __restrict__ part, int P, int K,
+                                                     double* __restrict__ out) {
+  for (int r = 0; r < K; r++) {
+    double v[1];
+    reduce_partials<1>(part + size_t(r) * P, P, v, 0u);
+    if (threadIdx.x == 0) out[r] = v[0];
+  }
 }
 
 __global__ __launch_bounds__(kBlock) void k_fill_zero(double* __restrict__ a, long n) {
@@ -848,6 +897,12 @@ X* dalloc(size_t n) {
 }
 }  // namespace
 
+// Device scalar scratch and pinned host readback sizes (doubles).  The largest
+// readback is the pair inner products: 4 * kMaxHist + 5.
+constexpr int kScal = 128;
+constexpr int kHost = 256;
+static_assert(4 * kMaxHist + 5 <= kScal && 128 + 4 * kMaxHist + 5 <= kHost, "scalar buffers too small");
+
 struct mmc_ctx {
   std::string err;
   int precision = MMC_FP64;
@@ -870,6 +925,13 @@ struct mmc_ctx {
   unsigned *cnt = nullptr, *start = nullptr;
   int* entries = nullptr;
   int* flags = nullptr;
+  // Verlet contact list
+  double* xref = nullptr;
+  unsigned *pcount = nullptr, *poff = nullptr;
+  int* plist = nullptr;
+  size_t plist_cap = 0;
+  bool list_valid = false;
+  int64_t rebuilds = 0;
   void* scan_tmp = nullptr;
   size_t scan_bytes = 0;
   unsigned table = 0;
@@ -880,7 +942,6 @@ struct mmc_ctx {
   double* scal = nullptr;    // misc scalars (dots)
   double *z = nullptr, *g = nullptr, *p = nullptr, *zc = nullptr, *gc = nullptr, *zt = nullptr, *zin = nullptr;
   double *Sh = nullptr, *Yh = nullptr;  // [m][6N]
-  double *rho = nullptr, *alpha = nullptr;
   double* host = nullptr;  // pinned readback
   bool have_trial = false;
   int evaluations = 0;
@@ -890,11 +951,25 @@ struct mmc_ctx {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int64_t timed_launches = 0;
   double timed_ms = 0;
+  bool timer_pending = false;
   void tick(int id) {
     if (id == timed) HIPCALL(hipEventRecord(ev0, stream));
   }
   void tock(int id) {
-    if (id == timed) HIPCALL(hipEventRecord(ev1, stream));
+    if (id == timed) {
+      HIPCALL(hipEventRecord(ev1, stream));
+      timer_pending = true;
+    }
+  }
+  // called after every stream synchronisation: at most one timed launch lies
+  // between two synchronisations
+  void harvest() {
+    if (!timer_pending) return;
+    float ms = 0;
+    HIPCALL(hipEventElapsedTime(&ms, ev0, ev1));
+    timed_ms += ms;
+    timed_launches++;
+    timer_pending = false;
   }
 
   Lat<double> lat64() const {
@@ -906,6 +981,8 @@ struct mmc_ctx {
   ~mmc_ctx();
   void setup(const mmc_lattice* L, const mmc_config* c, int prec);
   void ensure_entries(size_t need);
+  template <class T>
+  void build_list(Lat<T> L);
   void evaluate(const double* zsrc, const double* psrc, double step, double* zdst, double* gdst, double top,
                 bool trials, double* rec_out);
   template <class T>
@@ -914,6 +991,7 @@ struct mmc_ctx {
   void read(void* dst, const void* src, size_t bytes) {
     HIPCALL(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream));
     HIPCALL(hipStreamSynchronize(stream));
+    harvest();
   }
   void zero(double* a, long n) { k_fill_zero<<<blocks(n), kBlock, 0, stream>>>(a, n); }
 };
@@ -921,7 +999,7 @@ struct mmc_ctx {
 mmc_ctx::~mmc_ctx() {
   void* ptrs[] = {bpos, bquat, mass, lock, ba, bb, L0, rad, area, inert, e0, plastic, inc_off, inc, adj_off, adj, x, R,
                   rec, trial, cf_sph, cf_cap, box, cnt, start, entries, flags, scan_tmp, spart, cpart, bpart, gpart,
-                  vpart, stpart, record, scal, z, g, p, zc, gc, zt, zin, Sh, Yh, rho, alpha};
+                  vpart, stpart, record, scal, z, g, p, zc, gc, zt, zin, Sh, Yh, xref, pcount, poff, plist};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   if (host) (void)hipHostFree(host);
@@ -938,7 +1016,7 @@ void mmc_ctx::setup(const mmc_lattice* L, const mmc_config* c, int prec) {
   B = L->n_beams;
   NS = N + B;
   if (N <= 0 || B <= 0) throw Fail(MMC_ERR_ARG, "lattice has no nodes or beams");
-  if (cfg.history < 0 || cfg.history > 64) throw Fail(MMC_ERR_ARG, "history must be in [0, 64]");
+  if (cfg.history < 0 || cfg.history > kMaxHist) throw Fail(MMC_ERR_ARG, "history must be in [0, 16]");
   jr = cfg.joint_scale * cfg.max_radius;
   HIPCALL(hipStreamCreate(&stream));
   const mmc_material& m = cfg.material;
@@ -1056,20 +1134,31 @@ void mmc_ctx::setup(const mmc_lattice* L, const mmc_config* c, int prec) {
   HIPCALL(hipMemset(cnt, 0, (table + 1) * sizeof(unsigned)));
   flags = dalloc<int>(4);
   HIPCALL(hipMemset(flags, 0, 4 * sizeof(int)));
-  HIPCALL(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, cnt, start, table + 1, stream));
+  xref = dalloc<double>(3 * size_t(N));
+  pcount = dalloc<unsigned>(size_t(NS) + 1);
+  poff = dalloc<unsigned>(size_t(NS) + 1);
+  HIPCALL(hipMemset(pcount, 0, (size_t(NS) + 1) * sizeof(unsigned)));
+  {
+    size_t b1 = 0, b2 = 0;
+    HIPCALL(hipcub::DeviceScan::ExclusiveSum(nullptr, b1, cnt, start, table + 1, stream));
+    HIPCALL(hipcub::DeviceScan::ExclusiveSum(nullptr, b2, pcount, poff, NS + 1, stream));
+    scan_bytes = std::max(b1, b2);
+  }
   scan_tmp = dalloc<char>(scan_bytes);
   ensure_entries(size_t(NS) * 16);
+  plist_cap = std::max<size_t>(1024, size_t(NS));
+  plist = dalloc<int>(plist_cap);
   Ps = blocks(N);
   Pc = blocks(NS);
   Pb = blocks(B);
-  spart = dalloc<double>(7 * size_t(Ps));
+  spart = dalloc<double>(8 * size_t(Ps));
   cpart = dalloc<double>(3 * size_t(Pc));
   bpart = dalloc<double>(3 * size_t(Pb));
   gpart = dalloc<double>(2 * size_t(Ps));
-  vpart = dalloc<double>(3 * size_t(Ps) * 2);
+  vpart = dalloc<double>((4 * size_t(kMaxHist) + 8) * size_t(Ps));
   stpart = dalloc<double>(8 * size_t(blocks(std::max(N, B))));
   record = dalloc<double>(kRec * 4);
-  scal = dalloc<double>(16);
+  scal = dalloc<double>(kScal);
   size_t n6 = 6 * size_t(N);
   z = dalloc<double>(n6);
   g = dalloc<double>(n6);
@@ -1081,9 +1170,7 @@ void mmc_ctx::setup(const mmc_lattice* L, const mmc_config* c, int prec) {
   const size_t hm = size_t(std::max(1, cfg.history));
   Sh = dalloc<double>(n6 * hm);
   Yh = dalloc<double>(n6 * hm);
-  rho = dalloc<double>(hm);
-  alpha = dalloc<double>(hm);
-  HIPCALL(hipHostMalloc((void**)&host, 64 * sizeof(double), hipHostMallocDefault));
+  HIPCALL(hipHostMalloc((void**)&host, kHost * sizeof(double), hipHostMallocDefault));
   S.ts = cfg.translation_scale;
   S.jr = jr;
   S.gravity = cfg.gravity;
@@ -1092,6 +1179,7 @@ void mmc_ctx::setup(const mmc_lattice* L, const mmc_config* c, int prec) {
   S.centering = cfg.centering_stiffness;
   S.cell = std::max(2.8 * jr, 0.72 * cfg.cell_size);  // MC/src/sim.cpp:151
   S.inv_cell = 1.0 / S.cell;
+  S.skin = 0.25 * S.cell;
   S.table_mask = table - 1;
   HIPCALL(hipStreamSynchronize(stream));
 }
@@ -1105,30 +1193,58 @@ void mmc_ctx::ensure_entries(size_t need) {
   S.entry_cap = int(cap);
 }
 
+// Contact list rebuild at the current positions x (rare: only when some node
+// has moved skin/2 since the last build).  Two host synchronisations size the
+// grid entries and the pair list exactly.
+template <class T>
+void mmc_ctx::build_list(Lat<T> Lt) {
+  k_count<T><<<Pc, kBlock, 0, stream>>>(Lt, S, x, box, cnt, flags);
+  HIPCALL(hipcub::DeviceScan::ExclusiveSum(scan_tmp, scan_bytes, cnt, start, table + 1, stream));
+  unsigned total = 0;
+  read(&total, start + table, sizeof(unsigned));
+  ensure_entries(size_t(total) + 1024);
+  tick(MMC_K_FILL);
+  k_fill<<<Pc, kBlock, 0, stream>>>(NS, S, box, cnt, start, entries, flags);
+  tock(MMC_K_FILL);
+  k_sort<<<blocks(table), kBlock, 0, stream>>>(table, S, start, entries);
+  k_pairs<T, false><<<Pc, kBlock, 0, stream>>>(Lt, S, x, box, start, entries, pcount, poff, plist, 0, flags);
+  HIPCALL(hipcub::DeviceScan::ExclusiveSum(scan_tmp, scan_bytes, pcount, poff, NS + 1, stream));
+  unsigned pairs = 0;
+  read(&pairs, poff + NS, sizeof(unsigned));
+  if (size_t(pairs) > plist_cap) {
+    HIPCALL(hipFree(plist));
+    plist = nullptr;
+    plist_cap = size_t(pairs) + size_t(pairs) / 2 + 1024;
+    plist = dalloc<int>(plist_cap);
+  }
+  k_pairs<T, true><<<Pc, kBlock, 0, stream>>>(Lt, S, x, box, start, entries, pcount, poff, plist,
+                                              unsigned(plist_cap), flags);
+  HIPCALL(hipMemcpyAsync(xref, x, 3 * size_t(N) * sizeof(double), hipMemcpyDeviceToDevice, stream));
+  HIPCALL(hipGetLastError());
+  list_valid = true;
+  rebuilds++;
+}
+
 template <class T>
 void mmc_ctx::evaluate_t(Lat<T> Lt, const double* zsrc, const double* psrc, double step, double* zdst, double* gdst,
                          double top, bool trials, double* rec_out) {
   S.top = top;
-  for (int attempt = 0; attempt < 3; attempt++) {
-    tick(MMC_K_STATE);
-    k_state<T><<<Ps, kBlock, 0, stream>>>(Lt, S, zsrc, psrc, step, psrc ? zdst : nullptr, x, (T*)R, spart);
-    tock(MMC_K_STATE);
-    tick(MMC_K_COUNT);
-    k_count<T><<<Pc, kBlock, 0, stream>>>(Lt, S, x, box, cnt, flags);
-    tock(MMC_K_COUNT);
-    HIPCALL(hipcub::DeviceScan::ExclusiveSum(scan_tmp, scan_bytes, cnt, start, table + 1, stream));
-    tick(MMC_K_FILL);
-    k_fill<<<Pc, kBlock, 0, stream>>>(NS, S, box, cnt, start, entries, flags);
-    tock(MMC_K_FILL);
-    tick(MMC_K_SORT);
-    k_sort<<<blocks(table), kBlock, 0, stream>>>(table, S, start, entries);
-    tock(MMC_K_SORT);
-    tick(MMC_K_NARROW);
-    k_narrow<T><<<Pc, kBlock, 0, stream>>>(Lt, S, x, box, start, entries, cf_sph, cf_cap, cpart);
-    tock(MMC_K_NARROW);
-    tick(MMC_K_BEAM);
-    k_beam<T><<<Pb, kBlock, 0, stream>>>(Lt, M, x, (const T*)R, rec, trials ? trial : nullptr, bpart);
-    tock(MMC_K_BEAM);
+  tick(MMC_K_STATE);
+  k_state<T><<<Ps, kBlock, 0, stream>>>(Lt, S, zsrc, psrc, step, psrc ? zdst : nullptr, x, (T*)R,
+                                        list_valid ? xref : nullptr, spart);
+  tock(MMC_K_STATE);
+  bool built = false;
+  if (!list_valid) {
+    build_list<T>(Lt);
+    built = true;
+  }
+  tick(MMC_K_BEAM);
+  k_beam<T><<<Pb, kBlock, 0, stream>>>(Lt, M, x, (const T*)R, rec, trials ? trial : nullptr, bpart);
+  tock(MMC_K_BEAM);
+  for (;;) {
+    tick(MMC_K_CONTACT);
+    k_contact<T><<<Pc, kBlock, 0, stream>>>(Lt, S, x, poff, plist, cf_sph, cf_cap, cpart);
+    tock(MMC_K_CONTACT);
     tick(MMC_K_GATHER);
     k_gather<T><<<Ps, kBlock, 0, stream>>>(Lt, S, x, rec, cf_sph, cf_cap, spart, Ps, gdst, gpart);
     tock(MMC_K_GATHER);
@@ -1137,26 +1253,17 @@ void mmc_ctx::evaluate_t(Lat<T> Lt, const double* zsrc, const double* psrc, doub
                                       flags, rec_out);
     tock(MMC_K_FINAL);
     HIPCALL(hipGetLastError());
-    evaluations++;
-    total_evaluations++;
-    // the entry count is only known after the scan: read it with the record
     HIPCALL(hipMemcpyAsync(host, rec_out, kRec * sizeof(double), hipMemcpyDeviceToHost, stream));
-    HIPCALL(hipMemcpyAsync(host + kRec, start + table, sizeof(unsigned), hipMemcpyDeviceToHost, stream));
     HIPCALL(hipStreamSynchronize(stream));
-    if (timed >= 0) {
-      float ms = 0;
-      HIPCALL(hipEventElapsedTime(&ms, ev0, ev1));
-      timed_ms += ms;
-      timed_launches++;
-    }
-    unsigned total;
-    std::memcpy(&total, host + kRec, sizeof(unsigned));
-    int fl = int(host[10]);
-    if (!(fl & 4)) return;
-    // entry buffer overflow: grow and evaluate again
-    ensure_entries(size_t(total) + size_t(total) / 2 + 1024);
+    harvest();
+    // a stale list (some node moved skin/2 or more since the build) cannot be
+    // trusted: rebuild at these positions and redo the contact part
+    if (built || !(host[11] >= 0.5 * S.skin)) break;
+    build_list<T>(Lt);
+    built = true;
   }
-  throw Fail(MMC_ERR_CAPACITY, "contact grid overflow persisted");
+  evaluations++;
+  total_evaluations++;
 }
 
 void mmc_ctx::evaluate(const double* zsrc, const double* psrc, double step, double* zdst, double* gdst, double top,
@@ -1305,14 +1412,81 @@ int mmc_objective(mmc_ctx* c, const double* zh, double top, int32_t trials, mmc_
   });
 }
 
-// Simulator::minimize, MC/src/sim.cpp:174-211.  The line-search decisions read
-// one 12-double record per objective evaluation; every vector stays on the device.
+namespace {
+// Host half of the compact L-BFGS direction: from the pair inner products of the
+// retained history (oldest first) and the current gradient's products, the
+// coefficients of H g = gamma g + sum_j a_j s_j + c_j y_j (Byrd, Nocedal &
+// Schnabel 1994, eq. 2.6):  w = R^{-1} u,  a = R^{-T} ((D + gamma Y'Y) w - gamma v),
+// c = -gamma w, where R is the upper triangle of S'Y, D its diagonal,
+// u = S'g, v = Y'g.  gamma is the reference's H0 (MC/src/sim.cpp:186).
+struct History {
+  int m = 0, count = 0, head = 0;
+  double SY[kMaxHist][kMaxHist] = {};  // by slot: s_i . y_j (i not newer than j)
+  double YY[kMaxHist][kMaxHist] = {};  // by slot: y_i . y_j
+  double u[kMaxHist] = {}, v[kMaxHist] = {};  // s_j . g, y_j . g for the current g
+  int slot(int j) const { return (head + j) % m; }
+  void clear() {
+    count = 0;
+    head = 0;
+  }
+  DirCoef coefficients() const {
+    DirCoef C{};
+    const int k = count;
+    if (k == 0) {
+      C.cg = 0.10;
+      C.k = 0;
+      return C;
+    }
+    double Rm[kMaxHist][kMaxHist], Dg[kMaxHist], Ym[kMaxHist][kMaxHist], uu[kMaxHist], vv[kMaxHist];
+    int sl[kMaxHist];
+    for (int i = 0; i < k; i++) sl[i] = slot(i);
+    for (int i = 0; i < k; i++) {
+      for (int j = 0; j < k; j++) {
+        Rm[i][j] = i <= j ? SY[sl[i]][sl[j]] : 0.0;
+        Ym[i][j] = YY[sl[i]][sl[j]];
+      }
+      Dg[i] = SY[sl[i]][sl[i]];
+      uu[i] = u[sl[i]];
+      vv[i] = v[sl[i]];
+    }
+    const double sy = Dg[k - 1], yy = Ym[k - 1][k - 1];
+    const double gamma = (sy > 1e-20 && yy > 1e-20) ? std::min(1e4, std::max(1e-7, sy / yy)) : 0.10;
+    double w[kMaxHist], t[kMaxHist], a[kMaxHist];
+    for (int i = k - 1; i >= 0; i--) {
+      double acc = uu[i];
+      for (int j = i + 1; j < k; j++) acc -= Rm[i][j] * w[j];
+      w[i] = acc / Rm[i][i];
+    }
+    for (int i = 0; i < k; i++) {
+      double acc = Dg[i] * w[i] - gamma * vv[i];
+      for (int j = 0; j < k; j++) acc += gamma * Ym[i][j] * w[j];
+      t[i] = acc;
+    }
+    for (int i = 0; i < k; i++) {
+      double acc = t[i];
+      for (int j = 0; j < i; j++) acc -= Rm[j][i] * a[j];
+      a[i] = acc / Rm[i][i];
+    }
+    C.cg = gamma;
+    C.k = k;
+    for (int i = 0; i < k; i++) {
+      C.slot[i] = sl[i];
+      C.cs[i] = a[i];
+      C.cy[i] = -gamma * w[i];
+    }
+    return C;
+  }
+};
+}  // namespace
+
+// Simulator::minimize, MC/src/sim.cpp:174-211.  Every vector stays on the
+// device; the host reads one 12-double record per objective evaluation, the
+// direction's g.p, and the new pair's inner products.
 int mmc_minimize(mmc_ctx* c, double top, mmc_opt_stats* stats, mmc_objective_out* fin, double* z_out) {
   return guarded(c, [&] {
     const int N = c->N, P = c->Ps;
     const size_t n6 = 6 * size_t(N);
     const mmc_config& cf = c->cfg;
-    const int m = cf.history;
     const double mt = cf.max_translation_step / cf.translation_scale, mr = cf.max_rotation_step;
     c->evaluations = 0;
     mmc_opt_stats st{};
@@ -1328,20 +1502,18 @@ int mmc_minimize(mmc_ctx* c, double top, mmc_opt_stats* stats, mmc_objective_out
                 (cf.relative_escape && rel < cf.relative_tolerance);
       return ok && r[5] <= cf.penetration_tolerance;
     };
-    // history ring: pair j (0 = oldest) lives in slot (head + j) % m
-    int count = 0, head = 0;
-    std::vector<double> rho_h(std::max(1, m)), sy_last(1), yy_last(1);
-    double last_sy = 0, last_yy = 0;
-    auto slot = [&](int j) { return (head + j) % std::max(1, m); };
-    auto read_scalar = [&](const double* part, int Pn) {
-      k_sum1<<<1, kBlock, 0, c->stream>>>(part, Pn, c->scal);
+    History H;
+    H.m = cf.history;
+    auto direction = [&](const DirCoef& C) {
+      c->tick(MMC_K_DIRECTION);
+      k_direction<<<P, kBlock, 0, c->stream>>>(N, n6, c->g, c->Sh, c->Yh, C, c->p, c->lock, mt, mr, c->vpart);
+      c->tock(MMC_K_DIRECTION);
+      k_sum_rows<<<1, kBlock, 0, c->stream>>>(c->vpart, P, 1, c->scal);
       c->read(c->host + 32, c->scal, sizeof(double));
       return c->host[32];
     };
-    auto steepest = [&]() {
-      k_tl_scaled<<<P, kBlock, 0, c->stream>>>(N, c->g, 1.0, c->p, c->lock, c->g, mt, mr, c->vpart);
-      return read_scalar(c->vpart, P);
-    };
+    DirCoef steepest{};
+    steepest.cg = 1.0;
     auto line_search = [&](double dg) {
       double step = 1.0;
       for (int bt = 0; bt < cf.max_line_search; bt++) {
@@ -1363,64 +1535,51 @@ int mmc_minimize(mmc_ctx* c, double top, mmc_opt_stats* stats, mmc_objective_out
         st.converged = 1;
         break;
       }
-      // two-loop recursion, MC/src/sim.cpp:184-190
-      double dg;
-      if (count == 0) {
-        k_tl_scaled<<<P, kBlock, 0, c->stream>>>(N, c->g, 0.10, c->p, c->lock, c->g, mt, mr, c->vpart);
-        dg = read_scalar(c->vpart, P);
-      } else {
-        HIPCALL(hipMemcpyAsync(c->rho, rho_h.data(), m * sizeof(double), hipMemcpyHostToDevice, c->stream));
-        double H0 = 0.10;
-        if (last_sy > 1e-20 && last_yy > 1e-20) H0 = std::min(1e4, std::max(1e-7, last_sy / last_yy));
-        double* part_a = c->vpart;
-        double* part_b = c->vpart + 3 * P;
-        k_tl_init<<<P, kBlock, 0, c->stream>>>(N, c->g, 1.0, c->p, c->Sh + n6 * slot(count - 1), part_a);
-        for (int j = count - 1; j >= 0; j--) {
-          const double* D = j > 0 ? c->Sh + n6 * slot(j - 1) : c->Yh + n6 * slot(0);
-          k_tl_back<<<P, kBlock, 0, c->stream>>>(N, c->p, c->Yh + n6 * slot(j), c->rho, slot(j), part_a, P, c->alpha,
-                                                  j == 0 ? H0 : 1.0, D, part_b);
-          std::swap(part_a, part_b);
-        }
-        for (int j = 0; j < count; j++) {
-          const bool last = j == count - 1;
-          const double* D = last ? nullptr : c->Yh + n6 * slot(j + 1);
-          k_tl_fwd<<<P, kBlock, 0, c->stream>>>(N, c->p, c->Sh + n6 * slot(j), c->rho, c->alpha, slot(j), part_a, P,
-                                                 D, last ? 1 : 0, c->lock, c->g, mt, mr, part_b);
-          std::swap(part_a, part_b);
-        }
-        dg = read_scalar(part_a, P);
-      }
-      if (!(dg < 0)) {
-        count = 0;
-        dg = steepest();
+      double dg = direction(H.coefficients());
+      if (!(dg < 0)) {  // MC/src/sim.cpp:191
+        H.clear();
+        dg = direction(steepest);
       }
       bool ok = line_search(dg);
-      if (!ok) {
-        count = 0;
-        dg = steepest();
+      if (!ok) {  // steepest-descent retry, MC/src/sim.cpp:193-203
+        H.clear();
+        dg = direction(steepest);
         ok = line_search(dg);
       }
       if (!ok) break;
-      // curvature pair, MC/src/sim.cpp:205-206
-      int target = count == m ? slot(0) : slot(count);
-      if (m > 0) {
-        k_pair<<<P, kBlock, 0, c->stream>>>(N, c->z, c->zc, c->g, c->gc, c->Sh + n6 * target, c->Yh + n6 * target,
-                                            c->vpart);
-        k_sum3<<<1, kBlock, 0, c->stream>>>(c->vpart, P, c->scal);
-        c->read(c->host + 32, c->scal, 3 * sizeof(double));
-        double sy = c->host[32], ss = c->host[33], yy = c->host[34];
+      if (H.m > 0) {  // curvature pair, MC/src/sim.cpp:205-206
+        PairArgs A{};
+        A.t = H.count == H.m ? H.slot(0) : H.slot(H.count);
+        A.nold = 0;
+        for (int j = 0; j < H.count; j++)
+          if (H.slot(j) != A.t) A.slot[A.nold++] = H.slot(j);
+        c->tick(MMC_K_PAIR);
+        k_pair_dots<<<P, kBlock, 0, c->stream>>>(N, n6, c->z, c->zc, c->g, c->gc, c->Sh, c->Yh, A, c->vpart);
+        c->tock(MMC_K_PAIR);
+        const int K = 4 * A.nold + 5;
+        k_sum_rows<<<1, kBlock, 0, c->stream>>>(c->vpart, P, K, c->scal);
+        c->read(c->host + 128, c->scal, K * sizeof(double));
+        const double* d = c->host + 128;
+        const double* nw = d + 4 * A.nold;
+        const double sy = nw[0], ss = nw[1], yy = nw[2];
         if (sy > 1e-12 * std::sqrt(std::max(1e-30, ss * yy))) {
-          if (count == m) {
-            head = (head + 1) % m;  // drop the oldest; the new pair took its slot
-          } else {
-            count++;
+          for (int jj = 0; jj < A.nold; jj++) {
+            const int j = A.slot[jj];
+            H.SY[j][A.t] = d[4 * jj + 0];
+            H.YY[j][A.t] = H.YY[A.t][j] = d[4 * jj + 1];
+            H.u[j] = d[4 * jj + 2];
+            H.v[j] = d[4 * jj + 3];
           }
-          rho_h[target] = 1 / sy;
-          last_sy = sy;
-          last_yy = yy;
+          H.SY[A.t][A.t] = sy;
+          H.YY[A.t][A.t] = yy;
+          H.u[A.t] = nw[3];
+          H.v[A.t] = nw[4];
+          if (H.count == H.m)
+            H.head = (H.head + 1) % H.m;  // the new pair took the oldest pair's slot
+          else
+            H.count++;
         } else {
-          count = 0;
-          head = 0;
+          H.clear();
         }
       }
       std::swap(c->z, c->zc);
@@ -1456,7 +1615,7 @@ int mmc_commit(mmc_ctx* c) {
     // recompute state(z) of the trial evaluation in FP64 (for either precision
     // policy the committed pose is FP64), then copy it into the base pose
     k_state<double><<<c->Ps, kBlock, 0, c->stream>>>(c->lat64(), c->S, c->zt, nullptr, 0.0, nullptr, c->x, c->zin,
-                                                     c->spart);
+                                                     nullptr, c->spart);
     k_commit_nodes<double><<<c->Ps, kBlock, 0, c->stream>>>(c->N, c->x, c->zin, c->bpos, c->bquat);
     k_commit_beams<<<c->Pb, kBlock, 0, c->stream>>>(c->B, c->trial, c->plastic);
     HIPCALL(hipGetLastError());
@@ -1513,12 +1672,13 @@ int mmc_read_stats(mmc_ctx* c, mmc_state_stats* o) {
 
 int mmc_kernel_timer(mmc_ctx* c, int32_t kernel) {
   return guarded(c, [&] {
-    if (kernel > MMC_K_FINAL) throw Fail(MMC_ERR_ARG, "unknown kernel id");
+    if (kernel > MMC_K_PAIR) throw Fail(MMC_ERR_ARG, "unknown kernel id");
     if (!c->ev0) {
       HIPCALL(hipEventCreate(&c->ev0));
       HIPCALL(hipEventCreate(&c->ev1));
     }
     c->timed = kernel < 0 ? -1 : kernel;
+    c->timer_pending = false;
     c->timed_launches = 0;
     c->timed_ms = 0;
   });

@@ -459,7 +459,7 @@ class Simulator:
         return int(lib().mmc_evaluations(self.context()))
 
 
-KERNELS = {"state": 0, "count": 1, "fill": 2, "sort": 3, "narrow": 4, "beam": 5, "gather": 6, "final": 7}
+KERNELS = {"state": 0, "fill": 2, "contact": 4, "beam": 5, "gather": 6, "final": 7, "direction": 8, "pair": 9}
 
 
 def device_name() -> str:

@@ -185,8 +185,8 @@ int mmc_read_stats(mmc_ctx* ctx, mmc_state_stats* out);
  * stream around every launch of one objective kernel (MMC_K_*) while enabled.
  * mmc_kernel_timer(ctx, id) starts (and clears) it, id < 0 stops it;
  * mmc_kernel_timer_read returns the launch count and the mean duration. */
-enum { MMC_K_STATE = 0, MMC_K_COUNT = 1, MMC_K_FILL = 2, MMC_K_SORT = 3, MMC_K_NARROW = 4,
-       MMC_K_BEAM = 5, MMC_K_GATHER = 6, MMC_K_FINAL = 7 };
+enum { MMC_K_STATE = 0, MMC_K_FILL = 2, MMC_K_CONTACT = 4, MMC_K_BEAM = 5, MMC_K_GATHER = 6,
+       MMC_K_FINAL = 7, MMC_K_DIRECTION = 8, MMC_K_PAIR = 9 };
 int mmc_kernel_timer(mmc_ctx* ctx, int32_t kernel);
 int mmc_kernel_timer_read(mmc_ctx* ctx, int64_t* launches, double* mean_ms);
 /* Objective evaluations performed by the context since creation. */
