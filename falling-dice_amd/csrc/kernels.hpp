@@ -62,6 +62,7 @@ struct Scalars {
   double cell;        // broadphase cell
   double inv_cell;
   double skin;        // Verlet skin of the contact pair list
+  int slab;           // partner slots per shape in the pair list
   unsigned table_mask;
   int entry_cap;
 };

@@ -177,7 +177,7 @@ void usage() {
                "gfx950 build: sim, validate, convergence (all = sim + validate; render/hero/stl not built)\n"
                "additive options: --lattice bcc|cubic --cell-size M --min-radius M --max-radius M --nu V\n"
                "  --density V --seed N --history N --grad-tol V --grad-rms-tol V --grad-max-tol V\n"
-               "  --no-relative-escape --position-sigma M --radius-mode 0|1 --imperfection M\n"
+               "  --no-relative-escape --position-sigma M --radius-mode 0|1 --line-search 0|1 --imperfection M\n"
                "  --member-imperfection M --joint-scale V --precision fp64|mixed\n";
 }
 
@@ -232,6 +232,7 @@ Args parse_args(int argc, char** argv) {
     else if (o == "--no-relative-escape") a.sim.relativeEscape = false;
     else if (o == "--position-sigma") a.sim.positionSigma = std::stod(value());
     else if (o == "--radius-mode") a.sim.radiusMode = std::stoi(value());
+    else if (o == "--line-search") a.sim.lineSearch = std::stoi(value());
     else if (o == "--imperfection") a.sim.imperfection = std::stod(value());
     else if (o == "--member-imperfection") a.sim.memberImperfection = std::stod(value());
     else if (o == "--joint-scale") a.sim.jointScale = std::stod(value());

@@ -21,8 +21,10 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/mmc.h"
@@ -45,6 +47,14 @@ __device__ __forceinline__ float round_up(double v) {
 }
 __device__ __forceinline__ int cell_of(float v, double inv) { return (int)floor((double)v * inv); }
 
+// Per-item energies for the per-term Armijo test: the item's energy at the
+// reference point (the current iterate) and at this evaluation.  Either may be
+// null (reference-style total-energy test).
+struct TermBufs {
+  const double* ref;
+  double* out;
+};
+
 template <class T>
 __device__ __forceinline__ v3<double> node_x(const double* x, int N, int i) {
   return v3<double>{x[i], x[N + i], x[2 * N + i]};
@@ -56,17 +66,18 @@ __device__ __forceinline__ quat<T> node_R(const T* R, int N, int i) {
 
 // ---------------------------------------------------------------------------
 // k_state: z (+ step p) -> x, R; gravity, platens, centroid.  MC/src/sim.cpp:136,142-143,168-171
-// partials [8][P]: E_gravity, E_platen, reaction, max_pen, platen_contacts,
+// partials [9][P]: E_gravity, E_platen, reaction, max_pen, platen_contacts,
 // sum_x, sum_z, max displacement from xref (the positions of the last contact
-// list build)
+// list build), sum of per-node energy changes against nt.ref
 // ---------------------------------------------------------------------------
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_state(Lat<T> L, Scalars S, const double* __restrict__ z,
+__device__ __forceinline__ void state_body(Lat<T> L, Scalars S, const double* __restrict__ z,
                                                   const double* __restrict__ p, double step, double* __restrict__ zc,
                                                   double* __restrict__ x, T* __restrict__ R,
-                                                  const double* __restrict__ xref, double* __restrict__ part) {
+                                                  const double* __restrict__ xref, TermBufs nt,
+                                                  double* __restrict__ part) {
   const int N = L.N, i = blockIdx.x * kBlock + threadIdx.x, P = gridDim.x;
-  double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   if (i < N) {
     const unsigned lk = L.lock[i];
     double q[6];
@@ -110,12 +121,24 @@ __global__ __launch_bounds__(kBlock) void k_state(Lat<T> L, Scalars S, const dou
       double dn = norm(d);
       v[7] = isfinite(dn) ? dn : INFINITY;
     }
+    const double en = v[0] + v[1];
+    if (nt.out) nt.out[i] = en;
+    if (nt.ref) v[8] = en - nt.ref[i];
   }
-  block_reduce<8>(v, (1u << 3) | (1u << 7));
+  block_reduce<9>(v, (1u << 3) | (1u << 7));
   if (threadIdx.x == 0)
 #pragma unroll
-    for (int k = 0; k < 8; k++) part[k * P + blockIdx.x] = v[k];
+    for (int k = 0; k < 9; k++) part[k * P + blockIdx.x] = v[k];
 }
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_state(Lat<T> L, Scalars S, const double* __restrict__ z,
+                                                  const double* __restrict__ p, double step, double* __restrict__ zc,
+                                                  double* __restrict__ x, T* __restrict__ R,
+                                                  const double* __restrict__ xref, TermBufs nt,
+                                                  double* __restrict__ part) {
+  state_body<T>(L, S, z, p, step, zc, x, R, xref, nt, part);
+}
+
 
 // ---------------------------------------------------------------------------
 // Contact.  Shapes 0..N-1 are joint spheres, N..N+B-1 strut capsules
@@ -128,8 +151,9 @@ __global__ __launch_bounds__(kBlock) void k_state(Lat<T> L, Scalars S, const dou
 // rebuilt with a uniform-grid spatial hash: each shape's float AABB, padded by
 // r + margin + skin/2, is counted and scattered into hashed cell buckets; a pair
 // is enumerated only in the cell that holds the low corner of the two boxes'
-// overlap (each overlapping pair once per partner); buckets are sorted so the
-// partner order, and therefore every contact sum, is reproducible.
+// overlap (each overlapping pair once per partner); each shape's partner list
+// is sorted, so the partner order, and therefore every contact sum, does not
+// depend on the order of the atomic bucket fill.
 // ---------------------------------------------------------------------------
 template <class T>
 __device__ __forceinline__ void shape_geom(const Lat<T>& L, const Scalars& S, const double* x, int s, v3<double>& a,
@@ -165,11 +189,8 @@ __global__ __launch_bounds__(kBlock) void k_count(Lat<T> L, Scalars S, const dou
   double pad = r + S.margin + 0.5 * S.skin;
   float lo[3] = {round_down(fmin(a.x, b.x) - pad), round_down(fmin(a.y, b.y) - pad), round_down(fmin(a.z, b.z) - pad)};
   float hi[3] = {round_up(fmax(a.x, b.x) + pad), round_up(fmax(a.y, b.y) + pad), round_up(fmax(a.z, b.z) + pad)};
-#pragma unroll
-  for (int d = 0; d < 3; d++) {
-    box[d * NS + s] = lo[d];
-    box[(3 + d) * NS + s] = hi[d];
-  }
+  reinterpret_cast<float4*>(box)[2 * s] = make_float4(lo[0], lo[1], lo[2], 0.f);
+  reinterpret_cast<float4*>(box)[2 * s + 1] = make_float4(hi[0], hi[1], hi[2], 0.f);
   int c0[3], c1[3];
   bool ok = true;
 #pragma unroll
@@ -184,7 +205,7 @@ __global__ __launch_bounds__(kBlock) void k_count(Lat<T> L, Scalars S, const dou
     // non-finite; an absurdly large box is reported (flag 1) and the
     // evaluation is marked non-finite so that the line search rejects it.
     if (ok) atomicOr(flags, 1);
-    box[s] = INFINITY;
+    reinterpret_cast<float4*>(box)[2 * s] = make_float4(INFINITY, INFINITY, INFINITY, 0.f);
     return;
   }
   for (int j = c0[1]; j <= c1[1]; j++)
@@ -192,13 +213,16 @@ __global__ __launch_bounds__(kBlock) void k_count(Lat<T> L, Scalars S, const dou
       for (int i = c0[0]; i <= c1[0]; i++) atomicAdd(&cnt[cell_hash(i, j, k, S.table_mask)], 1u);
 }
 
+// Boxes are packed as two float4 (lo, hi) per shape: one cache line per lookup.
 __device__ __forceinline__ bool load_cells(const float* box, int NS, int s, double inv, float lo[3], float hi[3],
                                            int c0[3], int c1[3]) {
-#pragma unroll
-  for (int d = 0; d < 3; d++) {
-    lo[d] = box[d * NS + s];
-    hi[d] = box[(3 + d) * NS + s];
-  }
+  const float4 l = reinterpret_cast<const float4*>(box)[2 * s], h = reinterpret_cast<const float4*>(box)[2 * s + 1];
+  lo[0] = l.x;
+  lo[1] = l.y;
+  lo[2] = l.z;
+  hi[0] = h.x;
+  hi[1] = h.y;
+  hi[2] = h.z;
   if (!(lo[0] < INFINITY)) return false;
 #pragma unroll
   for (int d = 0; d < 3; d++) {
@@ -227,24 +251,6 @@ __global__ __launch_bounds__(kBlock) void k_fill(int NS, Scalars S, const float*
         else
           atomicOr(flags, 4);
       }
-}
-
-// Orders each bucket ascending so that every shape visits its partners in a
-// fixed order (bit-reproducible contact sums).
-__global__ __launch_bounds__(kBlock) void k_sort(unsigned T, Scalars S, const unsigned* __restrict__ start,
-                                                 int* __restrict__ entries) {
-  const unsigned h = blockIdx.x * kBlock + threadIdx.x;
-  if (h >= T) return;
-  unsigned a = start[h], b = min(start[h + 1], (unsigned)S.entry_cap);
-  for (unsigned i = a + 1; i < b; i++) {
-    int v = entries[i];
-    unsigned j = i;
-    while (j > a && entries[j - 1] > v) {
-      entries[j] = entries[j - 1];
-      j--;
-    }
-    entries[j] = v;
-  }
 }
 
 template <class T>
@@ -285,19 +291,21 @@ __device__ __forceinline__ closest shape_closest(bool capA, v3<double> a0, v3<do
 }
 
 // Verlet-list build: per shape, partners (either side) closer than target +
-// skin.  WRITE = false counts them into pcount; WRITE = true writes them at
-// poff[s] in the enumeration order (cells in fixed order, sorted buckets).
-template <class T, bool WRITE>
+// skin, written to the shape's slab of S.slab slots in the enumeration order
+// (cells in fixed order, sorted buckets); pcount gets the true count, and a
+// count above the slab raises flag 8 (the host widens the slab and rebuilds).
+template <class T>
 __global__ __launch_bounds__(kBlock) void k_pairs(Lat<T> L, Scalars S, const double* __restrict__ x,
                                                   const float* __restrict__ box, const unsigned* __restrict__ start,
                                                   const int* __restrict__ entries, unsigned* __restrict__ pcount,
-                                                  const unsigned* __restrict__ poff, int* __restrict__ plist,
-                                                  unsigned cap, int* __restrict__ flags) {
+                                                  int* __restrict__ plist, int* __restrict__ flags) {
   const int N = L.N, NS = N + L.B, s = blockIdx.x * kBlock + threadIdx.x;
   if (s >= NS) return;
   float lo[3], hi[3];
   int c0[3], c1[3];
-  unsigned n = 0, base = WRITE ? poff[s] : 0u;
+  const unsigned slab = unsigned(S.slab);
+  const size_t base = size_t(s) * slab;
+  unsigned n = 0;
   if (load_cells(box, NS, s, S.inv_cell, lo, hi, c0, c1)) {
     v3<double> sa, sb;
     double sr;
@@ -312,12 +320,9 @@ __global__ __launch_bounds__(kBlock) void k_pairs(Lat<T> L, Scalars S, const dou
           for (unsigned e = e0; e < e1; e++) {
             int o = entries[e];
             if (o == s) continue;
-            float olo[3], ohi[3];
-#pragma unroll
-            for (int d = 0; d < 3; d++) {
-              olo[d] = box[d * NS + o];
-              ohi[d] = box[(3 + d) * NS + o];
-            }
+            const float4 ol = reinterpret_cast<const float4*>(box)[2 * o];
+            const float4 oh = reinterpret_cast<const float4*>(box)[2 * o + 1];
+            const float olo[3] = {ol.x, ol.y, ol.z}, ohi[3] = {oh.x, oh.y, oh.z};
             if (!(olo[0] <= hi[0] && lo[0] <= ohi[0] && olo[1] <= hi[1] && lo[1] <= ohi[1] && olo[2] <= hi[2] &&
                   lo[2] <= ohi[2]))
               continue;
@@ -333,39 +338,53 @@ __global__ __launch_bounds__(kBlock) void k_pairs(Lat<T> L, Scalars S, const dou
             closest cp = low ? shape_closest(scap, sa, sb, ocap, oa, ob) : shape_closest(ocap, oa, ob, scap, sa, sb);
             double dist = norm(cp.pb - cp.pa);
             if (!(dist < sr + orr + S.margin + S.skin)) continue;
-            if (WRITE) {
-              if (base + n < cap)
-                plist[base + n] = o;
-              else
-                atomicOr(flags, 8);
-            }
+            if (n < slab) plist[base + n] = o;
             n++;
           }
         }
   }
-  if (!WRITE) pcount[s] = n;
+  pcount[s] = n;
+  if (n > slab) {
+    atomicOr(flags, 8);
+    return;
+  }
+  // partners ascending: the contact sums do not depend on bucket order
+  for (unsigned i = 1; i < n; i++) {
+    const int v = plist[base + i];
+    unsigned j = i;
+    while (j > 0 && plist[base + j - 1] > v) {
+      plist[base + j] = plist[base + j - 1];
+      j--;
+    }
+    plist[base + j] = v;
+  }
+  // a shape covering two cells that hash to one bucket meets a partner twice
+  unsigned u = 0;
+  for (unsigned i = 0; i < n; i++)
+    if (u == 0 || plist[base + u - 1] != plist[base + i]) plist[base + u++] = plist[base + i];
+  pcount[s] = u;
 }
 
 // k_contact: MC/src/sim.cpp:159-165 over the Verlet list.  Each shape
 // accumulates its own force; each pair's energy is counted by its lower shape.
-// partials [3][P]: E_contact, max_pen, contacts
+// partials [4][P]: E_contact, max_pen, contacts, sum of per-shape energy changes
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_contact(Lat<T> L, Scalars S, const double* __restrict__ x,
-                                                    const unsigned* __restrict__ poff, const int* __restrict__ plist,
+__device__ __forceinline__ void contact_body(Lat<T> L, Scalars S, const double* __restrict__ x,
+                                                    const unsigned* __restrict__ pcount, const int* __restrict__ plist,
                                                     double* __restrict__ cf_sph, double* __restrict__ cf_cap,
-                                                    double* __restrict__ part) {
+                                                    TermBufs ct, double* __restrict__ part) {
   const int N = L.N, B = L.B, NS = N + B, s = blockIdx.x * kBlock + threadIdx.x, P = gridDim.x;
-  double v[3] = {0, 0, 0};
+  double v[4] = {0, 0, 0, 0};
   if (s < NS) {
     v3<double> f0{0, 0, 0}, f1{0, 0, 0};
-    const unsigned e0 = poff[s], e1 = poff[s + 1];
+    const size_t e0 = size_t(s) * S.slab, e1 = e0 + min(pcount[s], unsigned(S.slab));
     if (e1 > e0) {
       v3<double> sa, sb;
       double sr;
       int s0, s1;
       shape_geom(L, S, x, s, sa, sb, sr, s0, s1);
       const bool scap = s >= N;
-      for (unsigned e = e0; e < e1; e++) {
+      for (size_t e = e0; e < e1; e++) {
         const int o = plist[e];
         v3<double> oa, ob;
         double orr;
@@ -408,24 +427,35 @@ __global__ __launch_bounds__(kBlock) void k_contact(Lat<T> L, Scalars S, const d
       cf_sph[N + s] = f0.y;
       cf_sph[2 * N + s] = f0.z;
     }
+    if (ct.out) ct.out[s] = v[0];
+    if (ct.ref) v[3] = v[0] - ct.ref[s];
   }
-  block_reduce<3>(v, 1u << 1);
+  block_reduce<4>(v, 1u << 1);
   if (threadIdx.x == 0)
 #pragma unroll
-    for (int k = 0; k < 3; k++) part[k * P + blockIdx.x] = v[k];
+    for (int k = 0; k < 4; k++) part[k * P + blockIdx.x] = v[k];
 }
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_contact(Lat<T> L, Scalars S, const double* __restrict__ x,
+                                                    const unsigned* __restrict__ pcount, const int* __restrict__ plist,
+                                                    double* __restrict__ cf_sph, double* __restrict__ cf_cap,
+                                                    TermBufs ct, double* __restrict__ part) {
+  contact_body<T>(L, S, x, pcount, plist, cf_sph, cf_cap, ct, part);
+}
+
 
 // ---------------------------------------------------------------------------
 // k_beam: per-beam energy, gradient record and trial state, MC/src/sim.cpp:139-141.
 // rec [9][B]: f (force on b, -f on a), rotational gradient of a, of b.
-// partials [3][P]: energy, elastic, plastic.
+// partials [4][P]: energy, elastic, plastic, sum of per-beam energy changes.
 // ---------------------------------------------------------------------------
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_beam(Lat<T> L, Material M, const double* __restrict__ x,
+__device__ __forceinline__ void beam_body(Lat<T> L, Material M, const double* __restrict__ x,
                                                  const T* __restrict__ R, double* __restrict__ rec,
-                                                 double* __restrict__ trial, double* __restrict__ part) {
+                                                 double* __restrict__ trial, TermBufs bt,
+                                                 double* __restrict__ part) {
   const int N = L.N, B = L.B, k = blockIdx.x * kBlock + threadIdx.x, P = gridDim.x;
-  double v[3] = {0, 0, 0};
+  double v[4] = {0, 0, 0, 0};
   if (k < B) {
     const int ia = L.ba[k], ib = L.bb[k];
     v3<double> d = node_x<T>(x, N, ib) - node_x<T>(x, N, ia);
@@ -512,23 +542,34 @@ __global__ __launch_bounds__(kBlock) void k_beam(Lat<T> L, Material M, const dou
     if (trial)
 #pragma unroll
       for (int c = 0; c < MMC_TRIAL_STRIDE; c++) trial[c * B + k] = tr[c];
+    if (bt.out) bt.out[k] = v[0];
+    if (bt.ref) v[3] = v[0] - bt.ref[k];
   }
-  block_reduce<3>(v, 0u);
+  block_reduce<4>(v, 0u);
   if (threadIdx.x == 0)
 #pragma unroll
-    for (int c = 0; c < 3; c++) part[c * P + blockIdx.x] = v[c];
+    for (int c = 0; c < 4; c++) part[c * P + blockIdx.x] = v[c];
 }
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_beam(Lat<T> L, Material M, const double* __restrict__ x,
+                                                 const T* __restrict__ R, double* __restrict__ rec,
+                                                 double* __restrict__ trial, TermBufs bt, double* __restrict__ part) {
+  beam_body<T>(L, M, x, R, rec, trial, bt, part);
+}
+
 
 // ---------------------------------------------------------------------------
 // k_gather: per-node assembly + pack, MC/src/sim.cpp:139-143,164,168-172.
-// partials [2][P]: sum of squares and max |g| over unlocked DOFs.
+// partials [3][P]: sum of squares and max |g| over unlocked DOFs, g.p (the
+// search direction's directional derivative at this point, when p is given).
 // ---------------------------------------------------------------------------
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_gather(Lat<T> L, Scalars S, const double* __restrict__ x,
+__device__ __forceinline__ void gather_body(Lat<T> L, Scalars S, const double* __restrict__ x,
                                                    const double* __restrict__ rec, const double* __restrict__ cf_sph,
                                                    const double* __restrict__ cf_cap,
                                                    const double* __restrict__ spart, int Ps,
-                                                   double* __restrict__ g, double* __restrict__ part) {
+                                                   double* __restrict__ g, const double* __restrict__ p,
+                                                   double* __restrict__ part) {
   const int N = L.N, B = L.B, i = blockIdx.x * kBlock + threadIdx.x, P = gridDim.x;
   double cen[2];
   {
@@ -537,7 +578,7 @@ __global__ __launch_bounds__(kBlock) void k_gather(Lat<T> L, Scalars S, const do
     cen[0] = t[0] * (1.0 / N);
     cen[1] = t[1] * (1.0 / N);
   }
-  double v[2] = {0, 0};
+  double v[3] = {0, 0, 0};
   if (i < N) {
     v3<double> gx{0, 0, 0}, gr{0, 0, 0};
     for (int e = L.inc_off[i]; e < L.inc_off[i + 1]; e++) {
@@ -568,31 +609,75 @@ __global__ __launch_bounds__(kBlock) void k_gather(Lat<T> L, Scalars S, const do
         v[0] += gd * gd;
         v[1] = fmax(v[1], fabs(gd));
       }
+      if (p) v[2] += gd * p[d * N + i];
     }
   }
-  block_reduce<2>(v, 1u << 1);
+  block_reduce<3>(v, 1u << 1);
   if (threadIdx.x == 0) {
     part[blockIdx.x] = v[0];
     part[P + blockIdx.x] = v[1];
+    part[2 * P + blockIdx.x] = v[2];
   }
 }
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_gather(Lat<T> L, Scalars S, const double* __restrict__ x,
+                                                   const double* __restrict__ rec, const double* __restrict__ cf_sph,
+                                                   const double* __restrict__ cf_cap,
+                                                   const double* __restrict__ spart, int Ps,
+                                                   double* __restrict__ g, const double* __restrict__ p,
+                                                   double* __restrict__ part) {
+  gather_body<T>(L, S, x, rec, cf_sph, cf_cap, spart, Ps, g, p, part);
+}
+
 
 // Record layout (doubles): 0 energy, 1 elastic, 2 plastic, 3 contact, 4 reaction,
 // 5 max_pen, 6 rms, 7 max, 8 contacts, 9 finite, 10 flags, 11 largest node
-// displacement since the contact list was built
-constexpr int kRec = 12;
+// displacement since the contact list was built, 12 energy change against the
+// reference record summed per term (when per-term energies were kept),
+// 13 centering energy, 14 g.p
+constexpr int kRec = 16;
 
-__global__ __launch_bounds__(kBlock) void k_final(int N, double centering, double n_free,
+__device__ __forceinline__ void final_body(int N, double centering, double n_free,
                                                   const double* __restrict__ spart, int Ps,
                                                   const double* __restrict__ cpart, int Pc,
                                                   const double* __restrict__ bpart, int Pb,
                                                   const double* __restrict__ gpart, int Pg,
-                                                  int* __restrict__ flags, double* __restrict__ out) {
-  double s[8], c[3], b[3], g[2];
-  reduce_partials<8>(spart, Ps, s, (1u << 3) | (1u << 7));
-  reduce_partials<3>(cpart, Pc, c, 1u << 1);
-  reduce_partials<3>(bpart, Pb, b, 0u);
-  reduce_partials<2>(gpart, Pg, g, 1u << 1);
+                                                  int* __restrict__ flags, const double* __restrict__ refrec,
+                                                  double* __restrict__ out) {
+  // 20 partial rows in one pass: every thread accumulates its share of every
+  // row (independent loads), then one block reduction (fixed order)
+  constexpr int R = 20;
+  double acc[R];
+  const unsigned maxmask = (1u << 3) | (1u << 7) | (1u << 10) | (1u << 18);
+#pragma unroll
+  for (int r = 0; r < R; r++) acc[r] = ((maxmask >> r) & 1u) ? -INFINITY : 0.0;
+  for (int i = threadIdx.x; i < Ps; i += kBlock) {
+#pragma unroll
+    for (int r = 0; r < 9; r++) acc[r] = (r == 3 || r == 7) ? fmax(acc[r], spart[r * Ps + i]) : acc[r] + spart[r * Ps + i];
+  }
+  for (int i = threadIdx.x; i < Pc; i += kBlock) {
+#pragma unroll
+    for (int r = 0; r < 4; r++) acc[9 + r] = r == 1 ? fmax(acc[9 + r], cpart[r * Pc + i]) : acc[9 + r] + cpart[r * Pc + i];
+  }
+  for (int i = threadIdx.x; i < Pb; i += kBlock) {
+#pragma unroll
+    for (int r = 0; r < 4; r++) acc[13 + r] += bpart[r * Pb + i];
+  }
+  for (int i = threadIdx.x; i < Pg; i += kBlock) {
+    acc[17] += gpart[i];
+    acc[18] = fmax(acc[18], gpart[Pg + i]);
+    acc[19] += gpart[2 * Pg + i];
+  }
+  block_reduce<R>(acc, maxmask);
+  __shared__ double rows[R];
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int r = 0; r < R; r++) rows[r] = acc[r];
+  __syncthreads();
+  const double* s = rows;
+  const double* c = rows + 9;
+  const double* b = rows + 13;
+  const double* g = rows + 17;
   if (threadIdx.x == 0) {
     const int fl = flags[0];
     flags[0] = 0;
@@ -612,8 +697,22 @@ __global__ __launch_bounds__(kBlock) void k_final(int N, double centering, doubl
     out[9] = fin ? 1.0 : 0.0;
     out[10] = (double)fl;
     out[11] = s[7];
+    out[12] = refrec ? (s[8] + c[3] + b[3]) + (ecen - refrec[13]) : 0.0;
+    out[13] = ecen;
+    out[14] = g[2];
+    out[15] = 0.0;
   }
 }
+__global__ __launch_bounds__(kBlock) void k_final(int N, double centering, double n_free,
+                                                  const double* __restrict__ spart, int Ps,
+                                                  const double* __restrict__ cpart, int Pc,
+                                                  const double* __restrict__ bpart, int Pb,
+                                                  const double* __restrict__ gpart, int Pg,
+                                                  int* __restrict__ flags, const double* __restrict__ refrec,
+                                                  double* __restrict__ out) {
+  final_body(N, centering, n_free, spart, Ps, cpart, Pc, bpart, Pb, gpart, Pg, flags, refrec, out);
+}
+
 
 // ---------------------------------------------------------------------------
 // beamPotential, MC/src/sim.cpp:468-509 (objectivity check)
@@ -711,9 +810,9 @@ __device__ __forceinline__ void finish_node(double (&q)[6], unsigned lk, double 
 }
 
 // p = -(cg g + sum_j cs_j s_j + cy_j y_j), locked and clamped; partial g.p
-__global__ __launch_bounds__(kBlock) void k_direction(int N, size_t n6, const double* __restrict__ g,
+__device__ __forceinline__ void direction_body(int N, size_t n6, const double* __restrict__ g,
                                                       const double* __restrict__ Sh, const double* __restrict__ Yh,
-                                                      DirCoef C, double* __restrict__ p,
+                                                      const DirCoef& C, double* __restrict__ p,
                                                       const uint8_t* __restrict__ lock, double mt, double mr,
                                                       double* __restrict__ part) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
@@ -742,14 +841,22 @@ __global__ __launch_bounds__(kBlock) void k_direction(int N, size_t n6, const do
   }
   write_dot_partial(dv, part);
 }
+__global__ __launch_bounds__(kBlock) void k_direction(int N, size_t n6, const double* __restrict__ g,
+                                                      const double* __restrict__ Sh, const double* __restrict__ Yh,
+                                                      DirCoef C, double* __restrict__ p,
+                                                      const uint8_t* __restrict__ lock, double mt, double mr,
+                                                      double* __restrict__ part) {
+  direction_body(N, n6, g, Sh, Yh, C, p, lock, mt, mr, part);
+}
+
 
 // s = zc - z, y = gc - g into history slot t.  Partials [4*nold + 5][P]: for
 // each retained pair j: s_j.y, y_j.y, s_j.gc, y_j.gc; then s.y, s.s, y.y,
 // s.gc, y.gc.
-__global__ __launch_bounds__(kBlock) void k_pair_dots(int N, size_t n6, const double* __restrict__ z,
+__device__ __forceinline__ void pair_dots_body(int N, size_t n6, const double* __restrict__ z,
                                                       const double* __restrict__ zc, const double* __restrict__ g,
                                                       const double* __restrict__ gc, double* __restrict__ Sh,
-                                                      double* __restrict__ Yh, PairArgs A, double* __restrict__ part) {
+                                                      double* __restrict__ Yh, const PairArgs& A, double* __restrict__ part) {
   const int i = blockIdx.x * kBlock + threadIdx.x, P = gridDim.x;
   double sv[6] = {0, 0, 0, 0, 0, 0}, yv[6] = {0, 0, 0, 0, 0, 0}, gv[6] = {0, 0, 0, 0, 0, 0};
   if (i < N) {
@@ -764,25 +871,30 @@ __global__ __launch_bounds__(kBlock) void k_pair_dots(int N, size_t n6, const do
       yt[d * N + i] = yv[d];
     }
   }
-  for (int jj = 0; jj < A.nold; jj++) {
-    double v[4] = {0, 0, 0, 0};
-    if (i < N) {
-      const double* sj = Sh + n6 * A.slot[jj];
-      const double* yj = Yh + n6 * A.slot[jj];
+  // all 4*nold + 5 partial dots in registers (unrolled over the maximum
+  // history), then reduced 16 at a time
+  constexpr int KM = 4 * kMaxHist + 5;
+  double acc[KM];
 #pragma unroll
-      for (int d = 0; d < 6; d++) {
-        const double a = sj[d * N + i], b = yj[d * N + i];
-        v[0] += a * yv[d];
-        v[1] += b * yv[d];
-        v[2] += a * gv[d];
-        v[3] += b * gv[d];
+  for (int r = 0; r < KM; r++) acc[r] = 0.0;
+  if (i < N) {
+#pragma unroll
+    for (int jj = 0; jj < kMaxHist; jj++) {
+      if (jj < A.nold) {
+        const double* sj = Sh + n6 * A.slot[jj];
+        const double* yj = Yh + n6 * A.slot[jj];
+#pragma unroll
+        for (int d = 0; d < 6; d++) {
+          const double a = sj[d * N + i], b = yj[d * N + i];
+          acc[4 * jj + 0] += a * yv[d];
+          acc[4 * jj + 1] += b * yv[d];
+          acc[4 * jj + 2] += a * gv[d];
+          acc[4 * jj + 3] += b * gv[d];
+        }
       }
     }
-    block_reduce<4>(v, 0u);
-    if (threadIdx.x == 0)
-#pragma unroll
-      for (int c = 0; c < 4; c++) part[(4 * jj + c) * P + blockIdx.x] = v[c];
   }
+  // the new pair's products go right after the retained pairs' rows
   double w[5] = {0, 0, 0, 0, 0};
 #pragma unroll
   for (int d = 0; d < 6; d++) {
@@ -792,11 +904,30 @@ __global__ __launch_bounds__(kBlock) void k_pair_dots(int N, size_t n6, const do
     w[3] += sv[d] * gv[d];
     w[4] += yv[d] * gv[d];
   }
-  block_reduce<5>(w, 0u);
-  if (threadIdx.x == 0)
 #pragma unroll
-    for (int c = 0; c < 5; c++) part[(4 * A.nold + c) * P + blockIdx.x] = w[c];
+  for (int jj = 0; jj <= kMaxHist; jj++)
+    if (jj == A.nold)
+#pragma unroll
+      for (int c = 0; c < 5; c++) acc[4 * jj + c] = w[c];
+  const int K = 4 * A.nold + 5;
+#pragma unroll
+  for (int c0 = 0; c0 < KM; c0 += 16) {
+    if (c0 >= K) break;
+    double v[16];
+#pragma unroll
+    for (int r = 0; r < 16; r++) v[r] = (c0 + r < KM) ? acc[c0 + r] : 0.0;
+    block_reduce<16>(v, 0u);
+    if (threadIdx.x == 0)
+      for (int r = 0; r < 16 && c0 + r < K; r++) part[(c0 + r) * P + blockIdx.x] = v[r];
+  }
 }
+__global__ __launch_bounds__(kBlock) void k_pair_dots(int N, size_t n6, const double* __restrict__ z,
+                                                      const double* __restrict__ zc, const double* __restrict__ g,
+                                                      const double* __restrict__ gc, double* __restrict__ Sh,
+                                                      double* __restrict__ Yh, PairArgs A, double* __restrict__ part) {
+  pair_dots_body(N, n6, z, zc, g, gc, Sh, Yh, A, part);
+}
+
 
 // Sums K rows of block partials [K][P] into out[K] (one block, fixed order).
 __global__ __launch_bounds__(kBlock) void k_sum_rows(const double* __restrict__ part, int P, int K,
@@ -806,6 +937,341 @@ __global__ __launch_bounds__(kBlock) void k_sum_rows(const double* __restrict__ 
     reduce_partials<1>(part + size_t(r) * P, P, v, 0u);
     if (threadIdx.x == 0) out[r] = v[0];
   }
+}
+
+// ---------------------------------------------------------------------------
+// Device-resident minimiser, MC/src/sim.cpp:174-208.  A control block holds
+// the loop state; every kernel of one L-BFGS iteration is predicated on it, so
+// the iteration can be captured once as a hipGraph and replayed without host
+// round trips.  Phases of one iteration:
+//   DIRECTION  p = -H g (compact form), g.p                       k_dir/k_dgcheck
+//   STEEPEST   p = -g when g.p >= 0 or after a failed line search  (same, mode 1)
+//   SEARCH     one Armijo trial per replay: state, beam, contact,
+//              gather, final (decision: accept, halve, or fail)
+//   PAIR       s, y and their inner products; history update,
+//              convergence test of the new iterate, next coefficients
+// Only the single-block kernels write the control block; the wide kernels read
+// it before they start (kernel boundaries order the two).
+// ---------------------------------------------------------------------------
+enum { PH_DIRECTION = 0, PH_STEEPEST = 2, PH_SEARCH = 4, PH_PAIR = 5 };
+enum { ST_RUNNING = 0, ST_CONVERGED = 1, ST_FAILED = 2, ST_ITERCAP = 3, ST_HOLD = 4 };
+
+struct Ctl {
+  int status, phase, par, it, iterations, backtracks, evals, retry;
+  int bt, count, head, m, max_it, max_ls, rel_escape, ls_mode;
+  int wolfe, pad;
+  double step, dg, top, armijo, min_step, tol_rms, tol_max, rel_tol, pen_tol, ts, skin_half, wolfe_eps;
+  double cur[kRec], cand[kRec];
+  DirCoef coef;
+  PairArgs pair;
+  double SY[kMaxHist][kMaxHist];  // by slot: s_i . y_j (i not newer than j)
+  double YY[kMaxHist][kMaxHist];  // by slot: y_i . y_j
+  double u[kMaxHist], v[kMaxHist];  // s_j . g and y_j . g of the current gradient
+};
+
+// Coefficients of H g = gamma g + sum_j a_j s_j + c_j y_j for the retained
+// history (oldest first from slot head): Byrd, Nocedal & Schnabel 1994, eq. 2.6:
+// w = R^{-1} u, a = R^{-T}((D + gamma Y'Y) w - gamma v), c = -gamma w, with R
+// the upper triangle of S'Y and D its diagonal.  gamma is the reference's H0
+// (MC/src/sim.cpp:186); an empty history gives p = -0.1 g.
+__host__ __device__ inline void compact_coefficients(int m, int count, int head, const double (*SY)[kMaxHist],
+                                                     const double (*YY)[kMaxHist], const double* u, const double* v,
+                                                     DirCoef& C) {
+  const int k = count;
+  C.k = k;
+  if (k == 0) {
+    C.cg = 0.10;
+    return;
+  }
+  int sl[kMaxHist];
+  for (int i = 0; i < k; i++) sl[i] = (head + i) % m;
+  const double sy = SY[sl[k - 1]][sl[k - 1]], yy = YY[sl[k - 1]][sl[k - 1]];
+  const double gamma = (sy > 1e-20 && yy > 1e-20) ? fmin(1e4, fmax(1e-7, sy / yy)) : 0.10;
+  double w[kMaxHist], t[kMaxHist], a[kMaxHist];
+  for (int i = k - 1; i >= 0; i--) {
+    double acc = u[sl[i]];
+    for (int j = i + 1; j < k; j++) acc -= SY[sl[i]][sl[j]] * w[j];
+    w[i] = acc / SY[sl[i]][sl[i]];
+  }
+  for (int i = 0; i < k; i++) {
+    double acc = SY[sl[i]][sl[i]] * w[i] - gamma * v[sl[i]];
+    for (int j = 0; j < k; j++) acc += gamma * YY[sl[i]][sl[j]] * w[j];
+    t[i] = acc;
+  }
+  for (int i = 0; i < k; i++) {
+    double acc = t[i];
+    for (int j = 0; j < i; j++) acc -= SY[sl[j]][sl[i]] * a[j];
+    a[i] = acc / SY[sl[i]][sl[i]];
+  }
+  C.cg = gamma;
+  for (int i = 0; i < k; i++) {
+    C.slot[i] = sl[i];
+    C.cs[i] = a[i];
+    C.cy[i] = -gamma * w[i];
+  }
+}
+
+// Curvature-pair bookkeeping after the pair inner products d (layout of
+// k_pair_dots), MC/src/sim.cpp:205-206.  Shared by both minimiser drivers.
+__host__ __device__ inline void history_update(int m, int& count, int& head, double (*SY)[kMaxHist],
+                                               double (*YY)[kMaxHist], double* u, double* v, const PairArgs& A,
+                                               const double* d) {
+  const double* nw = d + 4 * A.nold;
+  const double sy = nw[0], ss = nw[1], yy = nw[2];
+  if (sy > 1e-12 * sqrt(fmax(1e-30, ss * yy))) {
+    for (int jj = 0; jj < A.nold; jj++) {
+      const int j = A.slot[jj];
+      SY[j][A.t] = d[4 * jj + 0];
+      YY[j][A.t] = YY[A.t][j] = d[4 * jj + 1];
+      u[j] = d[4 * jj + 2];
+      v[j] = d[4 * jj + 3];
+    }
+    SY[A.t][A.t] = sy;
+    YY[A.t][A.t] = yy;
+    u[A.t] = nw[3];
+    v[A.t] = nw[4];
+    if (count == m)
+      head = (head + 1) % m;  // the new pair took the oldest pair's slot
+    else
+      count++;
+  } else {
+    count = 0;
+    head = 0;
+  }
+}
+
+// Slot of the next pair and the retained pairs whose products it needs.
+__host__ __device__ inline PairArgs pair_slots(int m, int count, int head) {
+  PairArgs A{};
+  A.t = count == m ? head % m : (head + count) % m;
+  A.nold = 0;
+  for (int j = 0; j < count; j++) {
+    const int sj = (head + j) % m;
+    if (sj != A.t) A.slot[A.nold++] = sj;
+  }
+  return A;
+}
+
+// The single-block kernels copy the control block to LDS with one coalesced
+// round trip, run their scalar logic there, and copy it back.
+static_assert(sizeof(Ctl) % 8 == 0, "Ctl is copied as 8-byte words");
+__device__ __forceinline__ void ctl_load(const Ctl* C, Ctl& sc) {
+  const int W = int(sizeof(Ctl) / 8);
+  for (int e = threadIdx.x; e < W; e += kBlock) reinterpret_cast<double*>(&sc)[e] = reinterpret_cast<const double*>(C)[e];
+  __syncthreads();
+}
+__device__ __forceinline__ void ctl_store(Ctl* C, const Ctl& sc) {
+  __syncthreads();
+  const int W = int(sizeof(Ctl) / 8);
+  for (int e = threadIdx.x; e < W; e += kBlock) reinterpret_cast<double*>(C)[e] = reinterpret_cast<const double*>(&sc)[e];
+}
+
+__device__ __forceinline__ bool ctl_converged(const Ctl* C, const double* r) {
+  const double rms = r[6], mx = r[7];
+  const double rel = rms / fmax(1e-6, C->ts * fmax(1.0, r[4]));
+  const bool ok = (rms <= C->tol_rms && mx <= C->tol_max) || (C->rel_escape && rel < C->rel_tol);
+  return ok && r[5] <= C->pen_tol;
+}
+__device__ __forceinline__ bool ctl_gate(const Ctl* C, int phase) {
+  return C->status == ST_RUNNING && C->phase == phase;
+}
+
+// L-BFGS direction from the control block's coefficients (phase DIRECTION)
+// or -g (phase STEEPEST)
+__global__ __launch_bounds__(kBlock) void k_dir_g(const Ctl* __restrict__ C, int N, size_t n6,
+                                                  const double* __restrict__ g0, const double* __restrict__ g1,
+                                                  const double* __restrict__ Sh, const double* __restrict__ Yh,
+                                                  double* __restrict__ p, const uint8_t* __restrict__ lock, double mt,
+                                                  double mr, double* __restrict__ part) {
+  if (C->status != ST_RUNNING || (C->phase != PH_DIRECTION && C->phase != PH_STEEPEST)) return;
+  const double* g = C->par ? g1 : g0;
+  if (C->phase == PH_STEEPEST) {
+    DirCoef D{};
+    D.cg = 1.0;
+    direction_body(N, n6, g, Sh, Yh, D, p, lock, mt, mr, part);
+  } else {
+    direction_body(N, n6, g, Sh, Yh, C->coef, p, lock, mt, mr, part);
+  }
+}
+
+// descent check, MC/src/sim.cpp:191: g.p >= 0 resets to steepest descent,
+// computed by the next replay's direction kernel
+__global__ __launch_bounds__(kBlock) void k_dgcheck_g(Ctl* __restrict__ C, const double* __restrict__ part, int P) {
+  if (C->status != ST_RUNNING || (C->phase != PH_DIRECTION && C->phase != PH_STEEPEST)) return;
+  const int mode = C->phase == PH_STEEPEST;
+  double v[1];
+  reduce_partials<1>(part, P, v, 0u);
+  if (threadIdx.x == 0) {
+    if (mode == 0 && !(v[0] < 0)) {
+      C->count = 0;
+      C->head = 0;
+      C->phase = PH_STEEPEST;
+    } else {
+      C->dg = v[0];
+      C->step = 1.0;
+      C->bt = 0;
+      C->phase = PH_SEARCH;
+    }
+  }
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_state_g(Lat<T> L, Scalars S, const Ctl* __restrict__ C,
+                                                    double* __restrict__ z0, double* __restrict__ z1,
+                                                    const double* __restrict__ p, double* __restrict__ x,
+                                                    T* __restrict__ R, const double* __restrict__ xref,
+                                                    double* __restrict__ nb0, double* __restrict__ nb1,
+                                                    double* __restrict__ part) {
+  if (!ctl_gate(C, PH_SEARCH)) return;
+  S.top = C->top;
+  const int par = C->par;
+  TermBufs nt{par ? nb1 : nb0, par ? nb0 : nb1};
+  state_body<T>(L, S, par ? z1 : z0, p, C->step, par ? z0 : z1, x, R, xref, nt, part);
+}
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_beam_g(const Ctl* __restrict__ C, Lat<T> L, Material M,
+                                                   const double* __restrict__ x, const T* __restrict__ R,
+                                                   double* __restrict__ rec, double* __restrict__ ub0,
+                                                   double* __restrict__ ub1, double* __restrict__ part) {
+  if (!ctl_gate(C, PH_SEARCH)) return;
+  const int par = C->par;
+  TermBufs bt{par ? ub1 : ub0, par ? ub0 : ub1};
+  beam_body<T>(L, M, x, R, rec, nullptr, bt, part);
+}
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_contact_g(const Ctl* __restrict__ C, Lat<T> L, Scalars S,
+                                                      const double* __restrict__ x,
+                                                      const unsigned* __restrict__ pcount,
+                                                      const int* __restrict__ plist, double* __restrict__ cf_sph,
+                                                      double* __restrict__ cf_cap, double* __restrict__ cb0,
+                                                      double* __restrict__ cb1, double* __restrict__ part) {
+  if (!ctl_gate(C, PH_SEARCH)) return;
+  const int par = C->par;
+  TermBufs ct{par ? cb1 : cb0, par ? cb0 : cb1};
+  contact_body<T>(L, S, x, pcount, plist, cf_sph, cf_cap, ct, part);
+}
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_gather_g(Lat<T> L, Scalars S, const Ctl* __restrict__ C,
+                                                     const double* __restrict__ x, const double* __restrict__ rec,
+                                                     const double* __restrict__ cf_sph,
+                                                     const double* __restrict__ cf_cap,
+                                                     const double* __restrict__ spart, int Ps,
+                                                     double* __restrict__ g0, double* __restrict__ g1,
+                                                     const double* __restrict__ p, double* __restrict__ part) {
+  if (!ctl_gate(C, PH_SEARCH)) return;
+  S.top = C->top;
+  gather_body<T>(L, S, x, rec, cf_sph, cf_cap, spart, Ps, C->par ? g0 : g1, p, part);
+}
+
+// Candidate record and the Armijo decision, MC/src/sim.cpp:178-180,192-204.
+__global__ __launch_bounds__(kBlock) void k_final_g(Ctl* __restrict__ C, int N, double centering, double n_free,
+                                                    const double* __restrict__ spart, int Ps,
+                                                    const double* __restrict__ cpart, int Pc,
+                                                    const double* __restrict__ bpart, int Pb,
+                                                    const double* __restrict__ gpart, int Pg,
+                                                    int* __restrict__ flags) {
+  if (!ctl_gate(C, PH_SEARCH)) return;
+  __shared__ Ctl sc;
+  ctl_load(C, sc);
+  final_body(N, centering, n_free, spart, Ps, cpart, Pc, bpart, Pb, gpart, Pg, flags, sc.cur, sc.cand);
+  if (threadIdx.x == 0) {
+    Ctl* c = &sc;
+    const double* r = c->cand;
+    c->evals++;
+    if (r[11] >= c->skin_half) {  // contact list stale at this trial: rebuild, then redo it
+      c->status = ST_HOLD;
+    } else {
+      c->backtracks++;
+      // Armijo on the energy change: total energies (the reference) or the
+      // per-term sum; the mixed policy adds the approximate-Wolfe test of Hager
+      // & Zhang (SIAM J. Optim. 16, 2005) for changes below its energy noise
+      const double dE = c->ls_mode ? r[12] : r[0] - c->cur[0];
+      const double bound = c->armijo * c->step * c->dg;
+      bool ok = r[9] != 0.0 && dE <= bound;
+      if (!ok && c->wolfe && r[9] != 0.0 && fabs(dE) <= c->wolfe_eps * fabs(c->cur[0]))
+        ok = r[14] <= (2.0 * c->armijo - 1.0) * c->dg;
+      if (ok) {
+        c->phase = PH_PAIR;
+        if (c->m > 0) c->pair = pair_slots(c->m, c->count, c->head);
+      } else {
+        c->step *= 0.5;
+        c->bt++;
+        if (c->bt >= c->max_ls || c->step < c->min_step) {
+          if (c->retry == 0) {  // steepest-descent retry
+            c->retry = 1;
+            c->count = 0;
+            c->head = 0;
+            c->phase = PH_STEEPEST;
+          } else {
+            c->status = ST_FAILED;
+          }
+        }
+      }
+    }
+  }
+  ctl_store(C, sc);
+}
+
+__global__ __launch_bounds__(kBlock) void k_pair_g(const Ctl* __restrict__ C, int N, size_t n6,
+                                                   double* __restrict__ z0, double* __restrict__ z1,
+                                                   double* __restrict__ g0, double* __restrict__ g1,
+                                                   double* __restrict__ Sh, double* __restrict__ Yh,
+                                                   double* __restrict__ part) {
+  if (!ctl_gate(C, PH_PAIR) || C->m == 0) return;
+  const int par = C->par;
+  pair_dots_body(N, n6, par ? z1 : z0, par ? z0 : z1, par ? g1 : g0, par ? g0 : g1, Sh, Yh, C->pair, part);
+}
+
+// history update, swap of current/candidate buffers, next iteration's
+// convergence test and coefficients (MC/src/sim.cpp:181-183,205-207).  The
+// pair products are reduced one wave per row; the small history matrices are
+// staged in LDS for the scalar algebra.
+__global__ __launch_bounds__(kBlock) void k_update_g(Ctl* __restrict__ C, const double* __restrict__ part, int P) {
+  if (!ctl_gate(C, PH_PAIR)) return;
+  __shared__ Ctl sc;
+  __shared__ double d[4 * kMaxHist + 5];
+  ctl_load(C, sc);
+  const int m = sc.m;
+  if (m > 0) {
+    const int K = 4 * sc.pair.nold + 5;
+    for (int c0 = 0; c0 < K; c0 += 16) {  // 16 rows per pass, independent loads
+      double acc[16];
+#pragma unroll
+      for (int r = 0; r < 16; r++) acc[r] = 0.0;
+      for (int i = threadIdx.x; i < P; i += kBlock) {
+#pragma unroll
+        for (int r = 0; r < 16; r++)
+          if (c0 + r < K) acc[r] += part[size_t(c0 + r) * P + i];
+      }
+      block_reduce<16>(acc, 0u);
+      if (threadIdx.x == 0)
+        for (int r = 0; r < 16 && c0 + r < K; r++) d[c0 + r] = acc[r];
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    Ctl* c = &sc;
+    if (m > 0) history_update(m, c->count, c->head, c->SY, c->YY, c->u, c->v, c->pair, d);
+    c->par ^= 1;
+    for (int k = 0; k < kRec; k++) c->cur[k] = c->cand[k];
+    c->it++;
+    c->retry = 0;
+    if (c->it >= c->max_it) {
+      c->status = ST_ITERCAP;
+    } else {
+      c->iterations = c->it + 1;
+      const double* r = c->cur;
+      const double rel = r[6] / fmax(1e-6, c->ts * fmax(1.0, r[4]));
+      const bool ok = (r[6] <= c->tol_rms && r[7] <= c->tol_max) || (c->rel_escape && rel < c->rel_tol);
+      if (ok && r[5] <= c->pen_tol) {
+        c->status = ST_CONVERGED;
+      } else {
+        compact_coefficients(m, c->count, c->head, c->SY, c->YY, c->u, c->v, c->coef);
+        c->phase = PH_DIRECTION;
+      }
+    }
+  }
+  ctl_store(C, sc);
 }
 
 __global__ __launch_bounds__(kBlock) void k_fill_zero(double* __restrict__ a, long n) {
@@ -927,9 +1393,8 @@ struct mmc_ctx {
   int* flags = nullptr;
   // Verlet contact list
   double* xref = nullptr;
-  unsigned *pcount = nullptr, *poff = nullptr;
-  int* plist = nullptr;
-  size_t plist_cap = 0;
+  unsigned* pcount = nullptr;
+  int* plist = nullptr;  // [NS][S.slab]
   bool list_valid = false;
   int64_t rebuilds = 0;
   void* scan_tmp = nullptr;
@@ -943,6 +1408,21 @@ struct mmc_ctx {
   double *z = nullptr, *g = nullptr, *p = nullptr, *zc = nullptr, *gc = nullptr, *zt = nullptr, *zin = nullptr;
   double *Sh = nullptr, *Yh = nullptr;  // [m][6N]
   double* host = nullptr;  // pinned readback
+  // device-resident minimiser: control block, its pinned staging copy, the
+  // (current, candidate) buffer pairs selected by Ctl::par, and the graph of
+  // one iteration
+  Ctl* ctl = nullptr;
+  Ctl* hctl = nullptr;
+  double *zbuf[2] = {nullptr, nullptr}, *gbuf[2] = {nullptr, nullptr};
+  // per-item energies (nodes, beams, shapes) at the current iterate and at the
+  // candidate, for the per-term energy change of the line search
+  double *nbuf[2] = {nullptr, nullptr}, *ubuf[2] = {nullptr, nullptr}, *cbuf[2] = {nullptr, nullptr};
+  bool terms_out0 = false;  // the next host-driven evaluation writes slot 0 of those
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  const int* graph_plist = nullptr;  // pair-list pointer and slab baked into the graph
+  int graph_slab = 0;
+  int64_t graph_replays = 0;
   bool have_trial = false;
   int evaluations = 0;
   int64_t total_evaluations = 0;
@@ -983,6 +1463,10 @@ struct mmc_ctx {
   void ensure_entries(size_t need);
   template <class T>
   void build_list(Lat<T> L);
+  template <class T>
+  void capture_iteration(Lat<T> L);
+  void minimize_graph(double top, mmc_opt_stats& st);
+  void minimize_host(double top, mmc_opt_stats& st);
   void evaluate(const double* zsrc, const double* psrc, double step, double* zdst, double* gdst, double top,
                 bool trials, double* rec_out);
   template <class T>
@@ -999,10 +1483,21 @@ struct mmc_ctx {
 mmc_ctx::~mmc_ctx() {
   void* ptrs[] = {bpos, bquat, mass, lock, ba, bb, L0, rad, area, inert, e0, plastic, inc_off, inc, adj_off, adj, x, R,
                   rec, trial, cf_sph, cf_cap, box, cnt, start, entries, flags, scan_tmp, spart, cpart, bpart, gpart,
-                  vpart, stpart, record, scal, z, g, p, zc, gc, zt, zin, Sh, Yh, xref, pcount, poff, plist};
+                  vpart, stpart, record, scal, z, g, p, zc, gc, zt, zin, Sh, Yh, xref, pcount, plist};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   if (host) (void)hipHostFree(host);
+  if (hctl) (void)hipHostFree(hctl);
+  if (ctl) (void)hipFree(ctl);
+  for (int b = 0; b < 2; b++) {
+    if (zbuf[b]) (void)hipFree(zbuf[b]);
+    if (gbuf[b]) (void)hipFree(gbuf[b]);
+    if (nbuf[b]) (void)hipFree(nbuf[b]);
+    if (ubuf[b]) (void)hipFree(ubuf[b]);
+    if (cbuf[b]) (void)hipFree(cbuf[b]);
+  }
+  if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+  if (graph) (void)hipGraphDestroy(graph);
   if (ev0) (void)hipEventDestroy(ev0);
   if (ev1) (void)hipEventDestroy(ev1);
   if (stream) (void)hipStreamDestroy(stream);
@@ -1017,6 +1512,7 @@ void mmc_ctx::setup(const mmc_lattice* L, const mmc_config* c, int prec) {
   NS = N + B;
   if (N <= 0 || B <= 0) throw Fail(MMC_ERR_ARG, "lattice has no nodes or beams");
   if (cfg.history < 0 || cfg.history > kMaxHist) throw Fail(MMC_ERR_ARG, "history must be in [0, 16]");
+  if (cfg.line_search != 0 && cfg.line_search != 1) throw Fail(MMC_ERR_ARG, "line_search must be 0 or 1");
   jr = cfg.joint_scale * cfg.max_radius;
   HIPCALL(hipStreamCreate(&stream));
   const mmc_material& m = cfg.material;
@@ -1126,7 +1622,7 @@ void mmc_ctx::setup(const mmc_lattice* L, const mmc_config* c, int prec) {
   trial = dalloc<double>(size_t(MMC_TRIAL_STRIDE) * B);
   cf_sph = dalloc<double>(3 * size_t(N));
   cf_cap = dalloc<double>(6 * size_t(B));
-  box = dalloc<float>(6 * size_t(NS));
+  box = dalloc<float>(8 * size_t(NS));
   table = 1024;
   while (table < 2u * unsigned(NS)) table <<= 1;
   cnt = dalloc<unsigned>(table + 1);
@@ -1135,26 +1631,20 @@ void mmc_ctx::setup(const mmc_lattice* L, const mmc_config* c, int prec) {
   flags = dalloc<int>(4);
   HIPCALL(hipMemset(flags, 0, 4 * sizeof(int)));
   xref = dalloc<double>(3 * size_t(N));
-  pcount = dalloc<unsigned>(size_t(NS) + 1);
-  poff = dalloc<unsigned>(size_t(NS) + 1);
-  HIPCALL(hipMemset(pcount, 0, (size_t(NS) + 1) * sizeof(unsigned)));
-  {
-    size_t b1 = 0, b2 = 0;
-    HIPCALL(hipcub::DeviceScan::ExclusiveSum(nullptr, b1, cnt, start, table + 1, stream));
-    HIPCALL(hipcub::DeviceScan::ExclusiveSum(nullptr, b2, pcount, poff, NS + 1, stream));
-    scan_bytes = std::max(b1, b2);
-  }
+  pcount = dalloc<unsigned>(size_t(NS));
+  HIPCALL(hipMemset(pcount, 0, size_t(NS) * sizeof(unsigned)));
+  HIPCALL(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, cnt, start, table + 1, stream));
   scan_tmp = dalloc<char>(scan_bytes);
   ensure_entries(size_t(NS) * 16);
-  plist_cap = std::max<size_t>(1024, size_t(NS));
-  plist = dalloc<int>(plist_cap);
+  S.slab = 8;
+  plist = dalloc<int>(size_t(NS) * S.slab);
   Ps = blocks(N);
   Pc = blocks(NS);
   Pb = blocks(B);
-  spart = dalloc<double>(8 * size_t(Ps));
-  cpart = dalloc<double>(3 * size_t(Pc));
-  bpart = dalloc<double>(3 * size_t(Pb));
-  gpart = dalloc<double>(2 * size_t(Ps));
+  spart = dalloc<double>(9 * size_t(Ps));
+  cpart = dalloc<double>(4 * size_t(Pc));
+  bpart = dalloc<double>(4 * size_t(Pb));
+  gpart = dalloc<double>(3 * size_t(Ps));
   vpart = dalloc<double>((4 * size_t(kMaxHist) + 8) * size_t(Ps));
   stpart = dalloc<double>(8 * size_t(blocks(std::max(N, B))));
   record = dalloc<double>(kRec * 4);
@@ -1171,6 +1661,15 @@ void mmc_ctx::setup(const mmc_lattice* L, const mmc_config* c, int prec) {
   Sh = dalloc<double>(n6 * hm);
   Yh = dalloc<double>(n6 * hm);
   HIPCALL(hipHostMalloc((void**)&host, kHost * sizeof(double), hipHostMallocDefault));
+  HIPCALL(hipHostMalloc((void**)&hctl, sizeof(Ctl), hipHostMallocDefault));
+  ctl = dalloc<Ctl>(1);
+  for (int b = 0; b < 2; b++) {
+    zbuf[b] = dalloc<double>(n6);
+    gbuf[b] = dalloc<double>(n6);
+    nbuf[b] = dalloc<double>(size_t(N));
+    ubuf[b] = dalloc<double>(size_t(B));
+    cbuf[b] = dalloc<double>(size_t(NS));
+  }
   S.ts = cfg.translation_scale;
   S.jr = jr;
   S.gravity = cfg.gravity;
@@ -1179,7 +1678,7 @@ void mmc_ctx::setup(const mmc_lattice* L, const mmc_config* c, int prec) {
   S.centering = cfg.centering_stiffness;
   S.cell = std::max(2.8 * jr, 0.72 * cfg.cell_size);  // MC/src/sim.cpp:151
   S.inv_cell = 1.0 / S.cell;
-  S.skin = 0.25 * S.cell;
+  S.skin = 0.5 * S.cell;
   S.table_mask = table - 1;
   HIPCALL(hipStreamSynchronize(stream));
 }
@@ -1206,19 +1705,20 @@ void mmc_ctx::build_list(Lat<T> Lt) {
   tick(MMC_K_FILL);
   k_fill<<<Pc, kBlock, 0, stream>>>(NS, S, box, cnt, start, entries, flags);
   tock(MMC_K_FILL);
-  k_sort<<<blocks(table), kBlock, 0, stream>>>(table, S, start, entries);
-  k_pairs<T, false><<<Pc, kBlock, 0, stream>>>(Lt, S, x, box, start, entries, pcount, poff, plist, 0, flags);
-  HIPCALL(hipcub::DeviceScan::ExclusiveSum(scan_tmp, scan_bytes, pcount, poff, NS + 1, stream));
-  unsigned pairs = 0;
-  read(&pairs, poff + NS, sizeof(unsigned));
-  if (size_t(pairs) > plist_cap) {
+  for (;;) {
+    k_pairs<T><<<Pc, kBlock, 0, stream>>>(Lt, S, x, box, start, entries, pcount, plist, flags);
+    int fl = 0;
+    read(&fl, flags, sizeof(int));
+    if (!(fl & 8)) break;
+    // some shape has more partners than slots: widen the slab and enumerate again
+    int cleared = fl & ~8;
+    HIPCALL(hipMemcpy(flags, &cleared, sizeof(int), hipMemcpyHostToDevice));
     HIPCALL(hipFree(plist));
     plist = nullptr;
-    plist_cap = size_t(pairs) + size_t(pairs) / 2 + 1024;
-    plist = dalloc<int>(plist_cap);
+    if (S.slab >= (1 << 16)) throw Fail(MMC_ERR_CAPACITY, "contact partner slab exceeds 65536 per shape");
+    S.slab *= 2;
+    plist = dalloc<int>(size_t(NS) * S.slab);
   }
-  k_pairs<T, true><<<Pc, kBlock, 0, stream>>>(Lt, S, x, box, start, entries, pcount, poff, plist,
-                                              unsigned(plist_cap), flags);
   HIPCALL(hipMemcpyAsync(xref, x, 3 * size_t(N) * sizeof(double), hipMemcpyDeviceToDevice, stream));
   HIPCALL(hipGetLastError());
   list_valid = true;
@@ -1230,8 +1730,11 @@ void mmc_ctx::evaluate_t(Lat<T> Lt, const double* zsrc, const double* psrc, doub
                          double top, bool trials, double* rec_out) {
   S.top = top;
   tick(MMC_K_STATE);
+  const bool terms = terms_out0;
+  terms_out0 = false;
   k_state<T><<<Ps, kBlock, 0, stream>>>(Lt, S, zsrc, psrc, step, psrc ? zdst : nullptr, x, (T*)R,
-                                        list_valid ? xref : nullptr, spart);
+                                        list_valid ? xref : nullptr, TermBufs{nullptr, terms ? nbuf[0] : nullptr},
+                                        spart);
   tock(MMC_K_STATE);
   bool built = false;
   if (!list_valid) {
@@ -1239,18 +1742,20 @@ void mmc_ctx::evaluate_t(Lat<T> Lt, const double* zsrc, const double* psrc, doub
     built = true;
   }
   tick(MMC_K_BEAM);
-  k_beam<T><<<Pb, kBlock, 0, stream>>>(Lt, M, x, (const T*)R, rec, trials ? trial : nullptr, bpart);
+  k_beam<T><<<Pb, kBlock, 0, stream>>>(Lt, M, x, (const T*)R, rec, trials ? trial : nullptr,
+                                       TermBufs{nullptr, terms ? ubuf[0] : nullptr}, bpart);
   tock(MMC_K_BEAM);
   for (;;) {
     tick(MMC_K_CONTACT);
-    k_contact<T><<<Pc, kBlock, 0, stream>>>(Lt, S, x, poff, plist, cf_sph, cf_cap, cpart);
+    k_contact<T><<<Pc, kBlock, 0, stream>>>(Lt, S, x, pcount, plist, cf_sph, cf_cap,
+                                            TermBufs{nullptr, terms ? cbuf[0] : nullptr}, cpart);
     tock(MMC_K_CONTACT);
     tick(MMC_K_GATHER);
-    k_gather<T><<<Ps, kBlock, 0, stream>>>(Lt, S, x, rec, cf_sph, cf_cap, spart, Ps, gdst, gpart);
+    k_gather<T><<<Ps, kBlock, 0, stream>>>(Lt, S, x, rec, cf_sph, cf_cap, spart, Ps, gdst, nullptr, gpart);
     tock(MMC_K_GATHER);
     tick(MMC_K_FINAL);
     k_final<<<1, kBlock, 0, stream>>>(N, cfg.centering_stiffness, n_free, spart, Ps, cpart, Pc, bpart, Pb, gpart, Ps,
-                                      flags, rec_out);
+                                      flags, nullptr, rec_out);
     tock(MMC_K_FINAL);
     HIPCALL(hipGetLastError());
     HIPCALL(hipMemcpyAsync(host, rec_out, kRec * sizeof(double), hipMemcpyDeviceToHost, stream));
@@ -1412,182 +1917,226 @@ int mmc_objective(mmc_ctx* c, const double* zh, double top, int32_t trials, mmc_
   });
 }
 
+}  // extern "C"
+
 namespace {
-// Host half of the compact L-BFGS direction: from the pair inner products of the
-// retained history (oldest first) and the current gradient's products, the
-// coefficients of H g = gamma g + sum_j a_j s_j + c_j y_j (Byrd, Nocedal &
-// Schnabel 1994, eq. 2.6):  w = R^{-1} u,  a = R^{-T} ((D + gamma Y'Y) w - gamma v),
-// c = -gamma w, where R is the upper triangle of S'Y, D its diagonal,
-// u = S'g, v = Y'g.  gamma is the reference's H0 (MC/src/sim.cpp:186).
-struct History {
-  int m = 0, count = 0, head = 0;
-  double SY[kMaxHist][kMaxHist] = {};  // by slot: s_i . y_j (i not newer than j)
-  double YY[kMaxHist][kMaxHist] = {};  // by slot: y_i . y_j
-  double u[kMaxHist] = {}, v[kMaxHist] = {};  // s_j . g, y_j . g for the current g
-  int slot(int j) const { return (head + j) % m; }
-  void clear() {
-    count = 0;
-    head = 0;
-  }
-  DirCoef coefficients() const {
-    DirCoef C{};
-    const int k = count;
-    if (k == 0) {
-      C.cg = 0.10;
-      C.k = 0;
-      return C;
-    }
-    double Rm[kMaxHist][kMaxHist], Dg[kMaxHist], Ym[kMaxHist][kMaxHist], uu[kMaxHist], vv[kMaxHist];
-    int sl[kMaxHist];
-    for (int i = 0; i < k; i++) sl[i] = slot(i);
-    for (int i = 0; i < k; i++) {
-      for (int j = 0; j < k; j++) {
-        Rm[i][j] = i <= j ? SY[sl[i]][sl[j]] : 0.0;
-        Ym[i][j] = YY[sl[i]][sl[j]];
-      }
-      Dg[i] = SY[sl[i]][sl[i]];
-      uu[i] = u[sl[i]];
-      vv[i] = v[sl[i]];
-    }
-    const double sy = Dg[k - 1], yy = Ym[k - 1][k - 1];
-    const double gamma = (sy > 1e-20 && yy > 1e-20) ? std::min(1e4, std::max(1e-7, sy / yy)) : 0.10;
-    double w[kMaxHist], t[kMaxHist], a[kMaxHist];
-    for (int i = k - 1; i >= 0; i--) {
-      double acc = uu[i];
-      for (int j = i + 1; j < k; j++) acc -= Rm[i][j] * w[j];
-      w[i] = acc / Rm[i][i];
-    }
-    for (int i = 0; i < k; i++) {
-      double acc = Dg[i] * w[i] - gamma * vv[i];
-      for (int j = 0; j < k; j++) acc += gamma * Ym[i][j] * w[j];
-      t[i] = acc;
-    }
-    for (int i = 0; i < k; i++) {
-      double acc = t[i];
-      for (int j = 0; j < i; j++) acc -= Rm[j][i] * a[j];
-      a[i] = acc / Rm[i][i];
-    }
-    C.cg = gamma;
-    C.k = k;
-    for (int i = 0; i < k; i++) {
-      C.slot[i] = sl[i];
-      C.cs[i] = a[i];
-      C.cy[i] = -gamma * w[i];
-    }
-    return C;
-  }
-};
+bool host_converged(const mmc_config& cf, const double* r) {
+  double rms = r[6], mx = r[7];
+  double rel = rms / std::max(1e-6, cf.translation_scale * std::max(1.0, r[4]));
+  bool ok = (rms <= cf.gradient_rms_tolerance && mx <= cf.gradient_max_tolerance) ||
+            (cf.relative_escape && rel < cf.relative_tolerance);
+  return ok && r[5] <= cf.penetration_tolerance;
+}
+bool use_host_loop() {
+  const char* e = std::getenv("MMC_HOST_LOOP");
+  return e && e[0] == '1';
+}
 }  // namespace
 
+// kGraphIterations L-BFGS iterations as one graph of predicated kernels (see Ctl).
+constexpr int kGraphIterations = 4;
+template <class T>
+void mmc_ctx::capture_iteration(Lat<T> Lt) {
+  if (graph_exec) HIPCALL(hipGraphExecDestroy(graph_exec));
+  if (graph) HIPCALL(hipGraphDestroy(graph));
+  graph_exec = nullptr;
+  graph = nullptr;
+  const int P = Ps;
+  const size_t n6 = 6 * size_t(N);
+  const double mt = cfg.max_translation_step / cfg.translation_scale, mr = cfg.max_rotation_step;
+  HIPCALL(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+  for (int u = 0; u < kGraphIterations; u++) {
+  k_dir_g<<<P, kBlock, 0, stream>>>(ctl, N, n6, gbuf[0], gbuf[1], Sh, Yh, p, lock, mt, mr, vpart);
+  k_dgcheck_g<<<1, kBlock, 0, stream>>>(ctl, vpart, P);
+  k_state_g<T><<<Ps, kBlock, 0, stream>>>(Lt, S, ctl, zbuf[0], zbuf[1], p, x, (T*)R, xref, nbuf[0], nbuf[1], spart);
+  k_beam_g<T><<<Pb, kBlock, 0, stream>>>(ctl, Lt, M, x, (const T*)R, rec, ubuf[0], ubuf[1], bpart);
+  k_contact_g<T><<<Pc, kBlock, 0, stream>>>(ctl, Lt, S, x, pcount, plist, cf_sph, cf_cap, cbuf[0], cbuf[1], cpart);
+  k_gather_g<T><<<Ps, kBlock, 0, stream>>>(Lt, S, ctl, x, rec, cf_sph, cf_cap, spart, Ps, gbuf[0], gbuf[1], p, gpart);
+  k_final_g<<<1, kBlock, 0, stream>>>(ctl, N, cfg.centering_stiffness, n_free, spart, Ps, cpart, Pc, bpart, Pb,
+                                      gpart, Ps, flags);
+  k_pair_g<<<P, kBlock, 0, stream>>>(ctl, N, n6, zbuf[0], zbuf[1], gbuf[0], gbuf[1], Sh, Yh, vpart);
+  k_update_g<<<1, kBlock, 0, stream>>>(ctl, vpart, P);
+  }
+  HIPCALL(hipStreamEndCapture(stream, &graph));
+  HIPCALL(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0));
+  graph_plist = plist;
+  graph_slab = S.slab;
+}
+
+// Simulator::minimize, MC/src/sim.cpp:174-211, driven from the device: the host
+// replays the captured iteration in batches and reads the 16-word status head
+// of the control block after each batch.  It steps in only to rebuild a stale
+// contact list (status HOLD) and for the initial and final evaluations.
+void mmc_ctx::minimize_graph(double top, mmc_opt_stats& st) {
+  const int m = cfg.history;
+  zero(zbuf[0], long(6 * size_t(N)));
+  terms_out0 = true;
+  evaluate(zbuf[0], nullptr, 0.0, nullptr, gbuf[0], top, false, record);
+  Ctl& h = *hctl;
+  std::memset(&h, 0, sizeof(Ctl));
+  std::memcpy(h.cur, host, kRec * sizeof(double));
+  h.m = m;
+  h.max_it = cfg.max_iterations;
+  h.max_ls = cfg.max_line_search;
+  h.rel_escape = cfg.relative_escape;
+  h.top = top;
+  h.armijo = cfg.armijo;
+  h.min_step = cfg.min_line_step;
+  h.tol_rms = cfg.gradient_rms_tolerance;
+  h.tol_max = cfg.gradient_max_tolerance;
+  h.rel_tol = cfg.relative_tolerance;
+  h.pen_tol = cfg.penetration_tolerance;
+  h.ts = cfg.translation_scale;
+  h.skin_half = 0.5 * S.skin;
+  h.ls_mode = cfg.line_search;
+  h.wolfe = precision == MMC_MIXED;
+  h.wolfe_eps = 1e-6;
+  h.phase = PH_DIRECTION;
+  h.coef.cg = 0.10;
+  if (cfg.max_iterations <= 0) {
+    h.status = ST_ITERCAP;
+  } else {
+    h.iterations = 1;
+    h.status = host_converged(cfg, h.cur) ? ST_CONVERGED : ST_RUNNING;
+  }
+  HIPCALL(hipMemcpyAsync(ctl, &h, sizeof(Ctl), hipMemcpyHostToDevice, stream));
+  const bool mixed = precision == MMC_MIXED;
+  auto recapture = [&] {
+    if (mixed)
+      capture_iteration<float>(lat32());
+    else
+      capture_iteration<double>(lat64());
+  };
+  if (!graph_exec || graph_plist != plist || graph_slab != S.slab) recapture();
+  // every replay advances the phase machine by at least one step; bound the
+  // total so that a logic error cannot spin forever
+  const int64_t guard = int64_t(cfg.max_iterations + 1) * (cfg.max_line_search + 3) * 2 + 64;
+  int64_t replays = 0;
+  int batch = 2;
+  HIPCALL(hipMemcpyAsync(&h, ctl, 16 * sizeof(int), hipMemcpyDeviceToHost, stream));
+  HIPCALL(hipStreamSynchronize(stream));
+  while (h.status == ST_RUNNING || h.status == ST_HOLD) {
+    if (h.status == ST_HOLD) {
+      // the trial's positions are still in x: rebuild there and redo the trial
+      if (mixed)
+        build_list<float>(lat32());
+      else
+        build_list<double>(lat64());
+      if (graph_plist != plist || graph_slab != S.slab) recapture();
+      int* running = reinterpret_cast<int*>(host + kHost - 1);
+      *running = ST_RUNNING;
+      HIPCALL(hipMemcpyAsync(&ctl->status, running, sizeof(int), hipMemcpyHostToDevice, stream));
+    }
+    for (int r = 0; r < batch; r++) HIPCALL(hipGraphLaunch(graph_exec, stream));
+    replays += int64_t(batch) * kGraphIterations;
+    graph_replays += batch;
+    batch = std::min(8, batch * 2);
+    HIPCALL(hipMemcpyAsync(&h, ctl, 16 * sizeof(int), hipMemcpyDeviceToHost, stream));
+    HIPCALL(hipStreamSynchronize(stream));
+    if (replays > guard) throw Fail(MMC_ERR_STATE, "device minimiser exceeded its replay bound");
+  }
+  HIPCALL(hipMemcpyAsync(&h, ctl, sizeof(Ctl), hipMemcpyDeviceToHost, stream));
+  HIPCALL(hipStreamSynchronize(stream));
+  st.iterations = h.iterations;
+  st.backtracks = h.backtracks;
+  st.converged = h.status == ST_CONVERGED;
+  evaluations += h.evals;
+  total_evaluations += h.evals;
+  // the current iterate; Armijo steps decrease the energy strictly, so the
+  // reference's best-iterate fallback (MC/src/sim.cpp:209) selects it (the
+  // approximate-Wolfe acceptances of the mixed path admit changes within the
+  // energy's rounding noise, where "best" carries no information)
+  HIPCALL(hipMemcpyAsync(z, zbuf[h.par], 6 * size_t(N) * sizeof(double), hipMemcpyDeviceToDevice, stream));
+}
+
+// The same loop driven from the host (one synchronisation per trial): kept as
+// a cross-check of the device phase machine (MMC_HOST_LOOP=1).
+void mmc_ctx::minimize_host(double top, mmc_opt_stats& st) {
+  const int P = Ps;
+  const size_t n6 = 6 * size_t(N);
+  const mmc_config& cf = cfg;
+  const double mt = cf.max_translation_step / cf.translation_scale, mr = cf.max_rotation_step;
+  zero(z, long(n6));
+  double cur[kRec], cand[kRec];
+  evaluate(z, nullptr, 0.0, nullptr, g, top, false, record);
+  std::memcpy(cur, host, sizeof cur);
+  const int m = cf.history;
+  int count = 0, head = 0;
+  static thread_local double SY[kMaxHist][kMaxHist], YY[kMaxHist][kMaxHist], uu[kMaxHist], vv[kMaxHist];
+  auto direction = [&](const DirCoef& C) {
+    tick(MMC_K_DIRECTION);
+    k_direction<<<P, kBlock, 0, stream>>>(N, n6, g, Sh, Yh, C, p, lock, mt, mr, vpart);
+    tock(MMC_K_DIRECTION);
+    k_sum_rows<<<1, kBlock, 0, stream>>>(vpart, P, 1, scal);
+    read(host + 32, scal, sizeof(double));
+    return host[32];
+  };
+  DirCoef steepest{};
+  steepest.cg = 1.0;
+  auto line_search = [&](double dg) {
+    double step = 1.0;
+    for (int bt = 0; bt < cf.max_line_search; bt++) {
+      evaluate(z, p, step, zc, gc, top, false, record + kRec);
+      std::memcpy(cand, host, sizeof cand);
+      st.backtracks++;
+      if (cand[9] != 0.0 && cand[0] <= cur[0] + cf.armijo * step * dg) return true;
+      step *= 0.5;
+      if (step < cf.min_line_step) break;
+    }
+    return false;
+  };
+  for (int it = 0; it < cf.max_iterations; it++) {
+    st.iterations = it + 1;
+    if (host_converged(cf, cur)) {
+      st.converged = 1;
+      break;
+    }
+    DirCoef C{};
+    compact_coefficients(m, count, head, SY, YY, uu, vv, C);
+    double dg = direction(C);
+    if (!(dg < 0)) {  // MC/src/sim.cpp:191
+      count = head = 0;
+      dg = direction(steepest);
+    }
+    bool ok = line_search(dg);
+    if (!ok) {  // steepest-descent retry, MC/src/sim.cpp:193-203
+      count = head = 0;
+      dg = direction(steepest);
+      ok = line_search(dg);
+    }
+    if (!ok) break;
+    if (m > 0) {  // curvature pair, MC/src/sim.cpp:205-206
+      PairArgs A = pair_slots(m, count, head);
+      tick(MMC_K_PAIR);
+      k_pair_dots<<<P, kBlock, 0, stream>>>(N, n6, z, zc, g, gc, Sh, Yh, A, vpart);
+      tock(MMC_K_PAIR);
+      const int K = 4 * A.nold + 5;
+      k_sum_rows<<<1, kBlock, 0, stream>>>(vpart, P, K, scal);
+      read(host + 128, scal, K * sizeof(double));
+      history_update(m, count, head, SY, YY, uu, vv, A, host + 128);
+    }
+    std::swap(z, zc);
+    std::swap(g, gc);
+    std::memcpy(cur, cand, sizeof cur);
+  }
+}
+
+extern "C" {
+
 // Simulator::minimize, MC/src/sim.cpp:174-211.  Every vector stays on the
-// device; the host reads one 12-double record per objective evaluation, the
-// direction's g.p, and the new pair's inner products.
+// device; the final objective (with trial states) is evaluated at the last
+// iterate, which is left in z.
 int mmc_minimize(mmc_ctx* c, double top, mmc_opt_stats* stats, mmc_objective_out* fin, double* z_out) {
   return guarded(c, [&] {
-    const int N = c->N, P = c->Ps;
-    const size_t n6 = 6 * size_t(N);
-    const mmc_config& cf = c->cfg;
-    const double mt = cf.max_translation_step / cf.translation_scale, mr = cf.max_rotation_step;
+    const size_t n6 = 6 * size_t(c->N);
     c->evaluations = 0;
     mmc_opt_stats st{};
-    st.rms = st.max = st.energy = INFINITY;
-    c->zero(c->z, long(n6));
-    double cur[kRec], cand[kRec];
-    c->evaluate(c->z, nullptr, 0.0, nullptr, c->g, top, false, c->record);
-    std::memcpy(cur, c->host, sizeof cur);
-    auto converged = [&](const double* r) {
-      double rms = r[6], mx = r[7];
-      double rel = rms / std::max(1e-6, cf.translation_scale * std::max(1.0, r[4]));
-      bool ok = (rms <= cf.gradient_rms_tolerance && mx <= cf.gradient_max_tolerance) ||
-                (cf.relative_escape && rel < cf.relative_tolerance);
-      return ok && r[5] <= cf.penetration_tolerance;
-    };
-    History H;
-    H.m = cf.history;
-    auto direction = [&](const DirCoef& C) {
-      c->tick(MMC_K_DIRECTION);
-      k_direction<<<P, kBlock, 0, c->stream>>>(N, n6, c->g, c->Sh, c->Yh, C, c->p, c->lock, mt, mr, c->vpart);
-      c->tock(MMC_K_DIRECTION);
-      k_sum_rows<<<1, kBlock, 0, c->stream>>>(c->vpart, P, 1, c->scal);
-      c->read(c->host + 32, c->scal, sizeof(double));
-      return c->host[32];
-    };
-    DirCoef steepest{};
-    steepest.cg = 1.0;
-    auto line_search = [&](double dg) {
-      double step = 1.0;
-      for (int bt = 0; bt < cf.max_line_search; bt++) {
-        c->evaluate(c->z, c->p, step, c->zc, c->gc, top, false, c->record + kRec);
-        std::memcpy(cand, c->host, sizeof cand);
-        st.backtracks++;
-        if (cand[9] != 0.0 && cand[0] <= cur[0] + cf.armijo * step * dg) return true;
-        step *= 0.5;
-        if (step < cf.min_line_step) break;
-      }
-      return false;
-    };
-    for (int it = 0; it < cf.max_iterations; it++) {
-      st.iterations = it + 1;
-      st.rms = cur[6];
-      st.max = cur[7];
-      st.energy = cur[0];
-      if (converged(cur)) {
-        st.converged = 1;
-        break;
-      }
-      double dg = direction(H.coefficients());
-      if (!(dg < 0)) {  // MC/src/sim.cpp:191
-        H.clear();
-        dg = direction(steepest);
-      }
-      bool ok = line_search(dg);
-      if (!ok) {  // steepest-descent retry, MC/src/sim.cpp:193-203
-        H.clear();
-        dg = direction(steepest);
-        ok = line_search(dg);
-      }
-      if (!ok) break;
-      if (H.m > 0) {  // curvature pair, MC/src/sim.cpp:205-206
-        PairArgs A{};
-        A.t = H.count == H.m ? H.slot(0) : H.slot(H.count);
-        A.nold = 0;
-        for (int j = 0; j < H.count; j++)
-          if (H.slot(j) != A.t) A.slot[A.nold++] = H.slot(j);
-        c->tick(MMC_K_PAIR);
-        k_pair_dots<<<P, kBlock, 0, c->stream>>>(N, n6, c->z, c->zc, c->g, c->gc, c->Sh, c->Yh, A, c->vpart);
-        c->tock(MMC_K_PAIR);
-        const int K = 4 * A.nold + 5;
-        k_sum_rows<<<1, kBlock, 0, c->stream>>>(c->vpart, P, K, c->scal);
-        c->read(c->host + 128, c->scal, K * sizeof(double));
-        const double* d = c->host + 128;
-        const double* nw = d + 4 * A.nold;
-        const double sy = nw[0], ss = nw[1], yy = nw[2];
-        if (sy > 1e-12 * std::sqrt(std::max(1e-30, ss * yy))) {
-          for (int jj = 0; jj < A.nold; jj++) {
-            const int j = A.slot[jj];
-            H.SY[j][A.t] = d[4 * jj + 0];
-            H.YY[j][A.t] = H.YY[A.t][j] = d[4 * jj + 1];
-            H.u[j] = d[4 * jj + 2];
-            H.v[j] = d[4 * jj + 3];
-          }
-          H.SY[A.t][A.t] = sy;
-          H.YY[A.t][A.t] = yy;
-          H.u[A.t] = nw[3];
-          H.v[A.t] = nw[4];
-          if (H.count == H.m)
-            H.head = (H.head + 1) % H.m;  // the new pair took the oldest pair's slot
-          else
-            H.count++;
-        } else {
-          H.clear();
-        }
-      }
-      std::swap(c->z, c->zc);
-      std::swap(c->g, c->gc);
-      std::memcpy(cur, cand, sizeof cur);
+    if (use_host_loop()) {
+      if (c->cfg.line_search != 0) throw Fail(MMC_ERR_ARG, "the host-driven loop supports line_search 0 only");
+      c->minimize_host(top, st);
     }
-    // Armijo steps decrease the energy strictly, so the reference's best-iterate
-    // fallback (MC/src/sim.cpp:209) always selects the current iterate.
+    else
+      c->minimize_graph(top, st);
     c->evaluate(c->z, nullptr, 0.0, nullptr, c->gc, top, true, c->record + 2 * kRec);
     double f2[kRec];
     std::memcpy(f2, c->host, sizeof f2);
@@ -1596,14 +2145,14 @@ int mmc_minimize(mmc_ctx* c, double top, mmc_opt_stats* stats, mmc_objective_out
     st.rms = f2[6];
     st.max = f2[7];
     st.energy = f2[0];
-    st.converged = converged(f2) ? 1 : 0;
+    st.converged = host_converged(c->cfg, f2) ? 1 : 0;
     st.evaluations = c->evaluations;
     if (stats) *stats = st;
     fill_out(f2, fin);
     if (z_out) {
       std::vector<double> s(n6);
       c->read(s.data(), c->z, n6 * sizeof(double));
-      aos_from_soa(s.data(), z_out, N, 6);
+      aos_from_soa(s.data(), z_out, c->N, 6);
     }
     HIPCALL(hipStreamSynchronize(c->stream));
   });
@@ -1615,7 +2164,7 @@ int mmc_commit(mmc_ctx* c) {
     // recompute state(z) of the trial evaluation in FP64 (for either precision
     // policy the committed pose is FP64), then copy it into the base pose
     k_state<double><<<c->Ps, kBlock, 0, c->stream>>>(c->lat64(), c->S, c->zt, nullptr, 0.0, nullptr, c->x, c->zin,
-                                                     nullptr, c->spart);
+                                                     nullptr, TermBufs{nullptr, nullptr}, c->spart);
     k_commit_nodes<double><<<c->Ps, kBlock, 0, c->stream>>>(c->N, c->x, c->zin, c->bpos, c->bquat);
     k_commit_beams<<<c->Pb, kBlock, 0, c->stream>>>(c->B, c->trial, c->plastic);
     HIPCALL(hipGetLastError());
@@ -1692,6 +2241,52 @@ int mmc_kernel_timer_read(mmc_ctx* c, int64_t* launches, double* mean_ms) {
 }
 
 int64_t mmc_evaluations(const mmc_ctx* c) { return c ? c->total_evaluations : 0; }
+
+int mmc_time_kernel(mmc_ctx* c, int32_t kernel, int32_t launches, double* mean_ms) {
+  return guarded(c, [&] {
+    if (launches <= 0 || !mean_ms) throw Fail(MMC_ERR_ARG, "launches must be positive");
+    if (!c->ev0) {
+      HIPCALL(hipEventCreate(&c->ev0));
+      HIPCALL(hipEventCreate(&c->ev1));
+    }
+    auto run = [&](auto Lt, auto* R) {
+      using T = std::remove_pointer_t<decltype(R)>;
+      HIPCALL(hipEventRecord(c->ev0, c->stream));
+      for (int i = 0; i < launches; i++) {
+        switch (kernel) {
+          case MMC_K_BEAM:
+            k_beam<T><<<c->Pb, kBlock, 0, c->stream>>>(Lt, c->M, c->x, R, c->rec, nullptr, TermBufs{nullptr, nullptr},
+                                                       c->bpart);
+            break;
+          case MMC_K_CONTACT:
+            k_contact<T><<<c->Pc, kBlock, 0, c->stream>>>(Lt, c->S, c->x, c->pcount, c->plist, c->cf_sph, c->cf_cap,
+                                                          TermBufs{nullptr, nullptr}, c->cpart);
+            break;
+          case MMC_K_GATHER:
+            k_gather<T><<<c->Ps, kBlock, 0, c->stream>>>(Lt, c->S, c->x, c->rec, c->cf_sph, c->cf_cap, c->spart,
+                                                          c->Ps, c->gc, nullptr, c->gpart);
+            break;
+          case MMC_K_STATE:
+            k_state<T><<<c->Ps, kBlock, 0, c->stream>>>(Lt, c->S, c->z, nullptr, 0.0, nullptr, c->x, R, nullptr,
+                                                          TermBufs{nullptr, nullptr}, c->spart);
+            break;
+          default:
+            throw Fail(MMC_ERR_ARG, "kernel id cannot be timed in isolation");
+        }
+      }
+      HIPCALL(hipEventRecord(c->ev1, c->stream));
+      HIPCALL(hipEventSynchronize(c->ev1));
+      HIPCALL(hipGetLastError());
+      float ms = 0;
+      HIPCALL(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+      *mean_ms = double(ms) / launches;
+    };
+    if (c->precision == MMC_MIXED)
+      run(c->lat32(), (float*)c->R);
+    else
+      run(c->lat64(), (double*)c->R);
+  });
+}
 
 int mmc_beam_potential(mmc_ctx* c, const double* xh, const double* qh, double* out) {
   return guarded(c, [&] {

@@ -142,6 +142,7 @@ mmc_config to_c(const SimulationConfig& s) {
   c.relative_tolerance = s.relativeTolerance;
   c.position_sigma = s.positionSigma;
   c.radius_mode = s.radiusMode;
+  c.line_search = s.lineSearch;
   return c;
 }
 
@@ -196,6 +197,7 @@ SimulationConfig from_c(const mmc_config& c) {
   s.relativeTolerance = c.relative_tolerance;
   s.positionSigma = c.position_sigma;
   s.radiusMode = c.radius_mode;
+  s.lineSearch = c.line_search;
   return s;
 }
 

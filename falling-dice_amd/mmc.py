@@ -56,7 +56,7 @@ class Config(C.Structure):
         ("seed", C.c_uint64),
         ("lattice", C.c_int32), ("relative_escape", C.c_int32),
         ("relative_tolerance", C.c_double), ("position_sigma", C.c_double),
-        ("radius_mode", C.c_int32), ("reserved", C.c_int32),
+        ("radius_mode", C.c_int32), ("line_search", C.c_int32),
     ]
 
     def replace(self, **kw) -> "Config":
@@ -238,6 +238,7 @@ def lib():
                                       C.POINTER(C.c_int32)]
         L.mmc_kernel_timer.argtypes = [P, C.c_int32]
         L.mmc_kernel_timer_read.argtypes = [P, C.POINTER(C.c_int64), dp]
+        L.mmc_time_kernel.argtypes = [P, C.c_int32, C.c_int32, dp]
         L.mmc_evaluations.argtypes = [P]
         L.mmc_evaluations.restype = C.c_int64
         L.mmc_lattice_build.argtypes = [C.POINTER(Config), C.POINTER(Lattice), C.POINTER(P)]
@@ -457,6 +458,14 @@ class Simulator:
 
     def evaluations(self) -> int:
         return int(lib().mmc_evaluations(self.context()))
+
+    def time_kernel(self, kernel: str, launches: int = 200) -> float:
+        """Mean duration (ms) of one objective kernel, HIP events on the solver stream."""
+        ms = C.c_double()
+        rc = lib().mmc_time_kernel(self.context(), KERNELS[kernel], launches, C.byref(ms))
+        if rc != 0:
+            raise MMCError(lib().mmc_last_error(self.context()).decode())
+        return ms.value
 
 
 KERNELS = {"state": 0, "fill": 2, "contact": 4, "beam": 5, "gather": 6, "final": 7, "direction": 8, "pair": 9}

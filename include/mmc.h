@@ -76,7 +76,11 @@ typedef struct mmc_config {
                                  reference's corner-node imperfection (sim.cpp:63-69)   */
   int32_t radius_mode;        /* cubic: 0 linear grading min->max along y; 1 per-beam
                                  uniform in [min_radius, max_radius]                    */
-  int32_t reserved;
+  int32_t line_search;        /* Armijo energy test: 0 difference of total energies (the
+                                 reference, MC/src/sim.cpp:179); 1 sum of per-beam,
+                                 per-node and per-contact-shape energy differences (same
+                                 test in exact arithmetic, resolves decreases far below
+                                 the round-off of the total)                            */
 } mmc_config;
 
 /* Fills *cfg with the reference defaults of MC/include/sim.hpp:16-26 (BCC lattice,
@@ -191,6 +195,11 @@ int mmc_kernel_timer(mmc_ctx* ctx, int32_t kernel);
 int mmc_kernel_timer_read(mmc_ctx* ctx, int64_t* launches, double* mean_ms);
 /* Objective evaluations performed by the context since creation. */
 int64_t mmc_evaluations(const mmc_ctx* ctx);
+/* Mean duration of one objective kernel (MMC_K_STATE, _CONTACT, _BEAM,
+ * _GATHER) over `launches` back-to-back launches on the context stream at the
+ * state of the last evaluation, timed with HIP events (no reference
+ * counterpart). */
+int mmc_time_kernel(mmc_ctx* ctx, int32_t kernel, int32_t launches, double* mean_ms);
 
 /* Simulator::beamPotential(x, R), MC/src/sim.cpp:468-509, evaluated on the device
  * against the committed plastic state.  x is 3N, q is 4N (w,x,y,z). */

@@ -289,6 +289,10 @@ struct Eval {
   int contacts = 0;
   std::vector<double> grad;
   std::vector<Trial> trial;
+  // line_search 1 only: per-item energies (nodes [0,N), beams [N,N+B), shapes
+  // from N+B, a pair's energy held by its lower shape) and the centering energy
+  std::vector<double> terms;
+  double ecen = 0;
 };
 struct Stats {
   int iterations = 0, backtracks = 0, evaluations = 0;
@@ -579,6 +583,9 @@ struct orc {
     size_t N = nodes.size();
     o.grad.assign(N * 6, 0);
     if (trials) o.trial.resize(beams.size());
+    const bool per_term = cfg.line_search == 1;
+    const size_t NB = N + beams.size();
+    if (per_term) o.terms.assign(2 * NB, 0.0);  // N + B items, then N + B shapes
     std::vector<V3> x, gx(N), gr(N);
     std::vector<Q4> R;
     state(z, x, R);
@@ -591,6 +598,7 @@ struct orc {
       double L = len(d);
       if (L < 1e-10) {
         o.energy += 1e8 * sqr(1e-10 - L);
+        if (per_term) o.terms[N + bi] = 1e8 * sqr(1e-10 - L);
         continue;
       }
       V3 t = divs(d, L);
@@ -645,6 +653,7 @@ struct orc {
       o.plastic += dr;
       tr.elastic = uax + us + ur;
       tr.dissipation = dax + dr;
+      if (per_term) o.terms[N + bi] = ((uax + dax) + us) + (ur + dr);
       V3 world = qrot(R[ia], jlinvt(phi, tr.moment));
       subfrom(gr[ia], world);
       addto(gr[ib], world);
@@ -653,11 +662,13 @@ struct orc {
     for (size_t i = 0; i < N; i++) {
       o.energy += nodes[i].mass * cfg.gravity * x[i].y;
       gx[i].y += nodes[i].mass * cfg.gravity;
+      if (per_term) o.terms[i] = nodes[i].mass * cfg.gravity * x[i].y;
     }
     V3 cen;
     for (const V3& p : x) addto(cen, p);
     cen = divs(cen, double(N));
     o.energy += 0.5 * cfg.centering_stiffness * (sqr(cen.x) + sqr(cen.z));
+    o.ecen = 0.5 * cfg.centering_stiffness * (sqr(cen.x) + sqr(cen.z));
     for (V3& g : gx) {
       g.x += cfg.centering_stiffness * cen.x / double(N);
       g.z += cfg.centering_stiffness * cen.z / double(N);
@@ -674,6 +685,7 @@ struct orc {
         o.maxPen = std::max(o.maxPen, pb);
         o.contacts++;
         gx[i].y -= m;
+        if (per_term) o.terms[i] += en;
       }
       double pt = x[i].y + J - top;
       if (pt > 0) {
@@ -684,6 +696,7 @@ struct orc {
         o.maxPen = std::max(o.maxPen, pt);
         o.contacts++;
         gx[i].y += m;
+        if (per_term) o.terms[i] += en;
       }
     }
     double ts = cfg.translation_scale;
@@ -761,6 +774,7 @@ struct orc {
           o.contact += en;
           o.maxPen = std::max(o.maxPen, pen);
           o.contacts++;
+          if (!o.terms.empty()) o.terms[nodes.size() + beams.size() + size_t(std::min(ai, bi))] += en;
           spread(gx, A, cp.s, scl(n, mag));
           spread(gx, B, cp.t, scl(n, -mag));
         }
@@ -845,7 +859,17 @@ struct orc {
         for (size_t i = 0; i < z.size(); i++) zc[i] = locked[i] ? 0.0 : z[i] + step * p[i];
         cand = objective(zc, top, false);
         st.backtracks++;
-        if (std::isfinite(cand.energy) && cand.energy <= cur.energy + cfg.armijo * step * dg) return true;
+        bool accept;
+        if (cfg.line_search == 1) {
+          // Armijo on the energy change summed per term: free of the
+          // cancellation of two nearly equal totals
+          double acc = 0;
+          for (size_t i = 0; i < cand.terms.size(); i++) acc += cand.terms[i] - cur.terms[i];
+          accept = std::isfinite(cand.energy) && acc + (cand.ecen - cur.ecen) <= cfg.armijo * step * dg;
+        } else {
+          accept = std::isfinite(cand.energy) && cand.energy <= cur.energy + cfg.armijo * step * dg;
+        }
+        if (accept) return true;
         step *= 0.5;
         if (step < cfg.min_line_step) break;
       }

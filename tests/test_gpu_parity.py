@@ -216,3 +216,109 @@ def test_run_config1_parity(gpu, po):
     # gradient tolerance 1e-6 bounds the agreement of two correct FP64 solvers
     assert err["force"] <= 2e-4
     assert err["loading_work"] <= 2e-5 and err["dissipated"] <= 5e-5
+
+
+def test_device_loop_matches_host_loop(gpu, po, monkeypatch):
+    """The graph-replayed phase machine and the host-driven loop implement the
+    same algorithm: same convergence, equilibria inside the tolerance band."""
+    mmc = gpu
+    cfg = cases(mmc)["cubic_8x8x4"]
+    res = {}
+    for mode in ("0", "1"):
+        monkeypatch.setenv("MMC_HOST_LOOP", mode)
+        o, ctx, lat = make_pair(mmc, po, cfg)
+        out = []
+        for s in (0.0, 0.01, 0.02, 0.03):
+            st, fin, z = ctx.minimize(o.top0 - s * o.height)
+            ctx.commit()
+            out.append((st.converged, st.iterations, fin.reaction, fin.energy))
+        res[mode] = out
+    print(res)
+    for a, b in zip(res["0"], res["1"]):
+        assert a[0] == b[0] == 1
+        assert abs(a[2] - b[2]) <= 1e-5 * max(abs(b[2]), 1.0)
+        assert abs(a[3] - b[3]) <= 1e-6 * abs(b[3])
+        assert abs(a[1] - b[1]) <= 0.3 * b[1] + 5
+
+
+# ---------------------------------------------------------------------------
+# Per-term energy line search (line_search = 1) and the mixed-precision policy
+# (FP32 shear/rotation element arithmetic, FP64 axial strain, positions and
+# reductions, approximate-Wolfe acceptance below the FP32 energy noise)
+# ---------------------------------------------------------------------------
+def test_per_term_line_search_matches_oracle(gpu, po):
+    """line_search = 1 on both sides, at a gradient tolerance (1e-7) below the
+    total-energy Armijo floor of the 8x8x4 lattice: both converge, and agree
+    to the tighter band."""
+    mmc = gpu
+    cfg = cases(mmc)["cubic_8x8x4"].replace(line_search=1, gradient_rms_tolerance=1e-7,
+                                            gradient_max_tolerance=1e-7, relative_escape=0)
+    o, ctx, lat = make_pair(mmc, po, cfg)
+    for s in (0.0, 0.01, 0.02):
+        top = o.top0 - s * o.height
+        so, fo, zo = o.minimize(top)
+        sg, fg, zg = ctx.minimize(top)
+        print(s, so.iterations, sg.iterations, fo.reaction, fg.reaction)
+        assert so.converged and sg.converged
+        assert sg.rms <= 1e-7 and sg.max <= 1e-7
+        assert abs(fg.reaction - fo.reaction) <= 2e-6 * max(fo.reaction, 1.0)
+        o.commit()
+        ctx.commit()
+
+
+@pytest.mark.parametrize("case", ["cubic_8x8x4", "cubic_4x5x3_jitter"])
+@pytest.mark.parametrize("state", STATES)
+def test_objective_parity_mixed(gpu, po, case, state):
+    """Mixed-precision objective against the FP64 oracle: rotations and shear
+    in FP32 bound the agreement (energy 1e-7 relative, gradient 2e-5 of its
+    largest component); contacts and axial terms stay FP64."""
+    mmc = gpu
+    cfg = cases(mmc)[case]
+    o, ctx, lat = make_pair(mmc, po, cfg, plastic_seed=7 if state == "yielded" else None,
+                            precision=mmc.MMC_MIXED)
+    y = lat.x.reshape(-1, 3)[:, 1]
+    jr = cfg.joint_scale * cfg.max_radius
+    z = seeded_z(o.n_nodes, 11, squash=0.8 if state == "squash" else 0.0, y=y, jr=jr, ts=cfg.cell_size)
+    top = o.top0 - 0.02 * o.height
+    ro, go, to = o.objective(z, top, trials=True)
+    rg, gg = ctx.objective(z, top, trials=True)
+    e_rel = abs(rg.energy - ro.energy) / abs(ro.energy)
+    g_rel = rel(gg, go)
+    print(case, state, "energy rel", e_rel, "grad rel", g_rel)
+    assert rg.finite == 1 and rg.contacts == ro.contacts
+    assert e_rel <= 1e-7
+    assert g_rel <= 2e-5
+    assert abs(rg.reaction - ro.reaction) <= 1e-12 * max(ro.reaction, 1.0)
+
+
+def test_mixed_minimize_matches_fp64(gpu, po):
+    """Mixed-precision equilibria against the FP64 oracle at gradient
+    tolerance 1e-7: platen force within 1e-5 relative."""
+    mmc = gpu
+    cfg = cases(mmc)["cubic_8x8x4"].replace(line_search=1, gradient_rms_tolerance=1e-7,
+                                            gradient_max_tolerance=1e-7, relative_escape=0)
+    o, ctx, lat = make_pair(mmc, po, cfg, precision=mmc.MMC_MIXED)
+    for s in (0.0, 0.01, 0.02, 0.03):
+        top = o.top0 - s * o.height
+        so, fo, zo = o.minimize(top)
+        sg, fg, zg = ctx.minimize(top)
+        print(s, so.iterations, sg.iterations, fo.reaction, fg.reaction, sg.rms)
+        assert so.converged and sg.converged
+        assert abs(fg.reaction - fo.reaction) <= 1e-5 * max(fo.reaction, 1.0)
+        o.commit()
+        ctx.commit()
+
+
+def test_run_config1_mixed(gpu, po):
+    """Full 60 + 60 schedule of the 8x8x4 lattice with the mixed policy against
+    the oracle's FP64 curve (same band as the FP64 path at tolerance 1e-6)."""
+    mmc = gpu
+    cfg = mmc.cubic_config(8, 8, 4)
+    path = GOLDEN / "cubic_8x8x4_oracle_frames.npy"
+    fo = np.load(path) if path.exists() else po.Solver(cfg, "orc").run()
+    fg = mmc.Simulator(cfg, mmc.MMC_MIXED).run()
+    err = _frames_error(fg, fo)
+    print("config1 mixed errors", err, "iterations", fg["iterations"].sum(), fo["iterations"].sum())
+    assert np.all(fg["converged"] == 1)
+    assert err["force"] <= 2e-4
+    assert err["loading_work"] <= 2e-5 and err["dissipated"] <= 5e-5
