@@ -1,5 +1,6 @@
 #!/usr/bin/env python3
-"""Benchmark: load-steps/s of the quasi-static compression solve on one GPU.
+"""Benchmark: load-steps/s of the quasi-static compression solve on one GPU,
+mixed precision against plain FP64.
 
 Workload (the load-steps/s metric's configuration): simple cubic lattice
 32x32x16 cells (18,513 nodes, 53,328 beams), cell 5 mm, radius graded
@@ -8,17 +9,24 @@ hardening 5% E, platen to 30% strain in 60 load + 60 unload increments,
 L-BFGS m = 10, absolute gradient tolerance 1e-6, self-contact on.
 
 A "step" is one scheduled load increment (one equilibrium solve, bisection
-sub-steps included).  The solver state starts from the undeformed lattice on
-the device; --warmup increments (the tare first) run untimed, then exactly
---steps increments are timed between device synchronisations.
+sub-steps included).  Each leg builds its solver on the device; --warmup
+increments (the tare first) run untimed, then exactly --steps increments are
+timed between device synchronisations.  The mixed leg (FP32 shear/rotation
+element arithmetic, FP64 positions, axial strain, plastic state, reductions and
+L-BFGS algebra) gives `value`; the FP64 leg runs the same increments for the
+comparison, and the platen-force curves of the two legs are compared frame by
+frame (and against the oracle's FP64 curve when tests/golden holds it for this
+workload).
 
 The JSON line also carries:
-  roofline      the dominant kernel's algorithmic bytes per launch divided by
-                its mean launch duration, measured with HIP events on the solver
-                stream during the timed region (peak: MI355X HBM 8 TB/s)
+  roofline      the beam element kernel's algorithmic bytes per launch divided
+                by its mean launch duration, measured with HIP events on the
+                solver stream at the state the timed region ended in (peak:
+                MI355X HBM 8 TB/s); `kernel_share` lists the per-iteration
+                kernels' mean durations measured the same way
   cpu_baseline  the FP64 CPU oracle (single-threaded restatement of the
                 reference solver) on the host, on a bounded sample of the same
-                workload (the tare and the first load increments)
+                workload (the first load increments after the tare)
 """
 from __future__ import annotations
 
@@ -39,20 +47,65 @@ HBM_PEAK_GBS = 8000.0
 
 def workload(args):
     import mmc
-    cfg = mmc.cubic_config(args.nx, args.ny, args.nz, load=args.load_steps, unload=args.unload_steps)
+    cfg = mmc.cubic_config(args.nx, args.ny, args.nz, load=args.load_steps, unload=args.unload_steps,
+                           grad_tol=args.grad_tol, strain=args.strain, max_iterations=args.max_iterations)
+    cfg = cfg.replace(line_search=args.line_search)
     name = (f"cubic {args.nx}x{args.ny}x{args.nz} cells, cell 5 mm, r 0.4->0.6 mm, "
-            f"E 2.0 GPa, yield 50 MPa, H 5%E, 30% strain, {args.load_steps}+{args.unload_steps} increments, "
-            f"L-BFGS m=10, grad tol 1e-6")
+            f"E 2.0 GPa, yield 50 MPa, H 5%E, {args.strain * 100:g}% strain, "
+            f"{args.load_steps}+{args.unload_steps} increments, L-BFGS m=10, grad tol {args.grad_tol:g}, "
+            f"max {args.max_iterations} iterations" + (", per-term Armijo" if args.line_search else ""))
     return cfg, name
 
 
 def beam_kernel_bytes(N, B, precision):
-    """Compulsory HBM bytes of one k_beam launch: per beam the endpoints (8),
-    L0/radius/area/I (32), director (24), plastic state (56) and the 9-double
-    gradient record written (72); per node the pose read once (x 24 B, R 32 B
-    FP64 or 16 B FP32)."""
-    rbytes = 32 if precision == 0 else 16
-    return B * (8 + 32 + 24 + 56 + 72) + N * (24 + rbytes)
+    """Compulsory HBM bytes of one beam-kernel launch: per beam the endpoint
+    indices (8), L0/radius/area/I (32), director (24) and plastic state (56)
+    read and the 9-double gradient record written (72); per node the pose
+    read once (x 24 B, R 32 B; the pose is FP64 under both policies)."""
+    return B * (8 + 32 + 24 + 56 + 72) + N * (24 + 32)
+
+
+def run_leg(cfg, prec, warmup, steps, log):
+    import mmc
+    sim = mmc.Simulator(cfg, prec)
+    total = 1 + cfg.load_steps + cfg.unload_steps
+    warm = max(0, min(warmup, total - 1))
+    K = steps if steps > 0 else total - warm
+    K = min(K, total - warm)
+    sim.begin()
+    frames = []
+    for _ in range(warm):
+        f, _ = sim.advance()
+        frames.append(f)
+    log(f"{'mixed' if prec else 'fp64'}: {warm} warmup increments done")
+    ev0 = sim.evaluations()
+    t0 = time.perf_counter()
+    marks = []
+    for k in range(K):
+        f, _ = sim.advance()
+        frames.append(f)
+        marks.append(time.perf_counter())
+        if k % 20 == 19:
+            log(f"{'mixed' if prec else 'fp64'}: timed increment {k + 1}/{K}")
+    dt = time.perf_counter() - t0
+    evals = sim.evaluations() - ev0
+    fr = np.concatenate(frames)
+    # wall time of each timed increment (the host waits for every increment's
+    # status, so consecutive marks bracket one increment's device work)
+    sim.increment_s = np.diff(np.concatenate([[t0], marks]))
+    return sim, fr, warm, K, dt, evals
+
+
+def curve_error(fa, fb):
+    n = min(len(fa), len(fb))
+    if n == 0:
+        return None
+    peak = max(float(np.max(fb["reaction_force"][:n])), 1e-12)
+    d = np.abs(fa["reaction_force"][:n] - fb["reaction_force"][:n]) / peak
+    return {"frames": int(n), "max_rel_force": float(d.max()), "worst_frame": int(d.argmax()),
+            "loading_work_rel": float(abs(fa["loading_work"][n - 1] - fb["loading_work"][n - 1])
+                                      / max(abs(fb["loading_work"][n - 1]), 1e-12)),
+            "permanent_set_abs": float(abs(fa["permanent_set"][n - 1] - fb["permanent_set"][n - 1]))}
 
 
 def cpu_baseline(cfg, budget_s):
@@ -60,23 +113,23 @@ def cpu_baseline(cfg, budget_s):
     import py_oracle
     o = py_oracle.Solver(cfg, "orc")
     z = np.zeros(6 * o.n_nodes)
-    t0 = time.perf_counter()
-    steps = 0
-    # tare (untimed in both legs: it converges in one iteration), then load increments
+    # the tare (untimed in both legs: it converges in one iteration), then load increments
     o.minimize(o.top0, z)
     o.commit()
+    steps = 0
     t0 = time.perf_counter()
     while steps < cfg.load_steps:
         steps += 1
         top = o.top0 - cfg.max_strain * o.height * steps / cfg.load_steps
-        st, fin, _ = o.minimize(top)
+        o.minimize(top)
         o.commit()
         if time.perf_counter() - t0 > budget_s:
             break
     dt = time.perf_counter() - t0
-    return {"value": steps / dt, "unit": "load-steps/s", "cores": 1, "kind": "port",
-            "sample": f"load increments 1..{steps} after the tare ({dt:.1f} s on one host core)",
-            "steps": steps, "seconds": dt}
+    return {"value": round(steps / dt, 5), "unit": "load-steps/s", "cores": 1, "kind": "port",
+            "sample": f"load increments 1..{steps} after the tare ({dt:.1f} s on one host core; "
+                      f"early increments, lighter than the schedule's average)",
+            "steps": steps, "seconds": round(dt, 2)}
 
 
 def main():
@@ -84,13 +137,19 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=0, help="timed increments (0 = rest of the schedule)")
     ap.add_argument("--warmup", type=int, default=1, help="untimed increments before timing (tare first)")
-    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed", help="precision of the headline leg")
+    ap.add_argument("--no-compare", action="store_true", help="skip the FP64 comparison leg")
     ap.add_argument("--nx", type=int, default=32)
     ap.add_argument("--ny", type=int, default=32)
     ap.add_argument("--nz", type=int, default=16)
     ap.add_argument("--load-steps", type=int, default=60)
     ap.add_argument("--unload-steps", type=int, default=60)
-    ap.add_argument("--kernel", default="beam", help="kernel whose roofline is reported")
+    ap.add_argument("--grad-tol", type=float, default=1e-6)
+    ap.add_argument("--strain", type=float, default=0.25,
+                    help="platen travel as a fraction of the height (beyond ~0.27 the 32x32x16 lattice "
+                         "leaves the penetration tolerance and then collapses away from the platen)")
+    ap.add_argument("--max-iterations", type=int, default=20000)
+    ap.add_argument("--line-search", type=int, default=0, choices=[0, 1])
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -98,45 +157,29 @@ def main():
         print("bench.py measures one GPU: this solver has no multi-GPU path (pass --gpus 1)", file=sys.stderr)
         sys.exit(2)
     import mmc
+
+    def log(msg):
+        print(msg, file=sys.stderr, flush=True)
+
     prec = mmc.MMC_FP64 if args.precision == "fp64" else mmc.MMC_MIXED
     cfg, name = workload(args)
-    sim = mmc.Simulator(cfg, prec)
+    sim, fr, warm, K, dt, evals = run_leg(cfg, prec, args.warmup, args.steps, log)
     N, B = sim.n_nodes, sim.n_beams
     total = 1 + args.load_steps + args.unload_steps
-    warm = max(0, min(args.warmup, total - 1))
-    K = args.steps if args.steps > 0 else total - warm
-    K = min(K, total - warm)
-    sim.begin()
-    frames = []
-    for _ in range(warm):
-        f, _ = sim.advance()
-        frames.append(f)
-        print(f"warmup increment done ({len(frames)}/{warm})", file=sys.stderr, flush=True)
-    ev0 = sim.evaluations()
-    t0 = time.perf_counter()
-    for k in range(K):
-        f, _ = sim.advance()
-        frames.append(f)
-        if k % 10 == 9:
-            print(f"timed increment {k + 1}/{K}", file=sys.stderr, flush=True)
-    dt = time.perf_counter() - t0
-    evals = sim.evaluations() - ev0
-    # the dominant kernel, timed with HIP events on the solver stream at the
-    # state the timed region ended in (launches inside graph replays carry no
-    # per-launch events)
+    # kernel durations with HIP events on the solver stream, at the state the
+    # timed region ended in (launches inside graph replays carry no per-launch
+    # events, so each kernel is replayed in isolation on the same buffers)
     launches = 200
-    mean_ms = sim.time_kernel(args.kernel, launches)
-    others = {k: round(sim.time_kernel(k, 100) * 1e3, 3) for k in ("state", "contact", "gather") if k != args.kernel}
-    fr = np.concatenate(frames)
+    beam_ms = sim.time_kernel("beam", launches)
+    share = {k: round(sim.time_kernel(k, 100) * 1e3, 3) for k in ("state", "contact", "gather")}
+    share["beam"] = round(beam_ms * 1e3, 3)
+    bytes_launch = beam_kernel_bytes(N, B, prec)
+    ach = bytes_launch / (beam_ms * 1e-3) / 1e9
+    roof = {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_beam",
+            "bytes_per_launch": bytes_launch, "mean_launch_us": round(beam_ms * 1e3, 3),
+            "launches_timed": launches, "kernel_us": share}
     value = K / dt
-    bytes_launch = beam_kernel_bytes(N, B, prec) if args.kernel == "beam" else None
-    roof = None
-    if bytes_launch and mean_ms > 0:
-        ach = bytes_launch / (mean_ms * 1e-3) / 1e9
-        roof = {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": f"k_beam",
-                "bytes_per_launch": bytes_launch, "mean_launch_us": round(mean_ms * 1e3, 3),
-                "launches_timed": launches, "other_kernels_us": others}
     out = {
         "metric": "load-steps/sec, mixed vs FP64 on 1 GPU, at <=1e-5 rel platen-force error",
         "value": round(value, 4), "unit": "load-steps/s", "n_gpus": 1, "steps": K, "warmup": warm,
@@ -145,16 +188,44 @@ def main():
         "data": "synthetic (seeded cubic lattice)",
         "config": {"workload": name, "nodes": N, "beams": B, "precision": args.precision,
                    "increments": f"{warm} warmup + {K} timed of {total}"},
-        "lbfgs_iterations": int(fr["iterations"][warm:].sum()) if len(fr) > warm else None,
+        "lbfgs_iterations": int(fr["iterations"][warm:].sum()),
         "objective_evaluations": int(evals),
         "beam_evaluations_per_s": round(evals * B / dt, 1),
         "peak_force_n": float(fr["reaction_force"].max()),
         "all_converged": bool(np.all(fr["converged"] == 1)),
         "roofline": roof,
     }
+    inc_s = sim.increment_s
+    sim.close()
+    if not args.no_compare:
+        other = mmc.MMC_FP64 if prec == mmc.MMC_MIXED else mmc.MMC_MIXED
+        sim2, fr2, _, K2, dt2, _ = run_leg(cfg, other, args.warmup, args.steps, log)
+        sim2.close()
+        lbl = "fp64" if other == mmc.MMC_FP64 else "mixed"
+        out[f"{lbl}_leg"] = {"value": round(K2 / dt2, 4), "ms_per_step": round(dt2 / K2 * 1e3, 3),
+                             "lbfgs_iterations": int(fr2["iterations"][warm:].sum()),
+                             "all_converged": bool(np.all(fr2["converged"] == 1))}
+        mixed_fr, fp64_fr = (fr, fr2) if prec == mmc.MMC_MIXED else (fr2, fr)
+        out["speedup_mixed_vs_fp64"] = round((K / dt) / (K2 / dt2) if prec == mmc.MMC_MIXED
+                                             else (K2 / dt2) / (K / dt), 4)
+        out["force_error_mixed_vs_fp64"] = curve_error(mixed_fr, fp64_fr)
+        golden = ROOT / "tests" / "golden" / (
+            f"cubic_{args.nx}x{args.ny}x{args.nz}" + ("_tight" if args.line_search else "") + "_oracle_frames.npy")
+        full = args.load_steps == 60 and args.unload_steps == 60 and (
+            (args.grad_tol == 1e-6 and args.line_search == 0) or (args.grad_tol == 1e-7 and args.line_search == 1))
+        if golden.exists() and full:
+            fo = np.load(golden)
+            out["force_error_vs_oracle"] = {"mixed": curve_error(mixed_fr, fo), "fp64": curve_error(fp64_fr, fo)}
     if not args.no_cpu:
-        print("cpu baseline ...", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_budget)
+        log("cpu baseline ...")
+        cb = cpu_baseline(cfg, args.cpu_budget)
+        # the same increments on the GPU (headline leg), for a like-for-like ratio
+        first = 1 - warm  # timed index of load increment 1
+        if first >= 0 and first + cb["steps"] <= len(inc_s):
+            gs = float(np.sum(inc_s[first:first + cb["steps"]]))
+            cb["gpu_same_increments_value"] = round(cb["steps"] / gs, 4)
+            cb["gpu_over_cpu_same_increments"] = round(cb["steps"] / gs / cb["value"], 1)
+        out["cpu_baseline"] = cb
     print(json.dumps(out))
 
 

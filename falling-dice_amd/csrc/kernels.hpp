@@ -78,11 +78,11 @@ __device__ __forceinline__ double wave_max(double v) {
   for (int o = kWave / 2; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, kWave));
   return v;
 }
-// Reduces K values per thread; mode bit k set = max, else sum.  Result valid in
-// thread 0.  Fixed combination order: deterministic.
-template <int K>
+// Reduces K values per thread over a block of NT threads; mode bit k set = max,
+// else sum.  Result valid in thread 0.  Fixed combination order: deterministic.
+template <int K, int NT = kBlock>
 __device__ __forceinline__ void block_reduce(double (&v)[K], unsigned maxmask) {
-  __shared__ double sh[K][kBlock / kWave];
+  __shared__ double sh[K][NT / kWave];
   const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
 #pragma unroll
   for (int k = 0; k < K; k++) v[k] = ((maxmask >> k) & 1) ? wave_max(v[k]) : wave_sum(v[k]);
@@ -94,7 +94,7 @@ __device__ __forceinline__ void block_reduce(double (&v)[K], unsigned maxmask) {
 #pragma unroll
     for (int k = 0; k < K; k++) {
       double a = sh[k][0];
-      for (int i = 1; i < kBlock / kWave; i++) a = ((maxmask >> k) & 1) ? fmax(a, sh[k][i]) : a + sh[k][i];
+      for (int i = 1; i < NT / kWave; i++) a = ((maxmask >> k) & 1) ? fmax(a, sh[k][i]) : a + sh[k][i];
       v[k] = a;
     }
   }

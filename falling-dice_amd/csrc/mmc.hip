@@ -73,7 +73,7 @@ __device__ __forceinline__ quat<T> node_R(const T* R, int N, int i) {
 template <class T>
 __device__ __forceinline__ void state_body(Lat<T> L, Scalars S, const double* __restrict__ z,
                                                   const double* __restrict__ p, double step, double* __restrict__ zc,
-                                                  double* __restrict__ x, T* __restrict__ R,
+                                                  double* __restrict__ x, double* __restrict__ R,
                                                   const double* __restrict__ xref, TermBufs nt,
                                                   double* __restrict__ part) {
   const int N = L.N, i = blockIdx.x * kBlock + threadIdx.x, P = gridDim.x;
@@ -95,10 +95,10 @@ __device__ __forceinline__ void state_body(Lat<T> L, Scalars S, const double* __
     x[2 * N + i] = pos.z;
     quat<double> b{L.bquat[i], L.bquat[N + i], L.bquat[2 * N + i], L.bquat[3 * N + i]};
     quat<double> r = qnormalize(qmul(qexp(v3<double>{q[3], q[4], q[5]}), b));
-    R[i] = T(r.w);
-    R[N + i] = T(r.x);
-    R[2 * N + i] = T(r.y);
-    R[3 * N + i] = T(r.z);
+    R[i] = r.w;
+    R[N + i] = r.x;
+    R[2 * N + i] = r.y;
+    R[3 * N + i] = r.z;
     const double m = L.mass[i];
     v[0] = m * S.gravity * pos.y;
     double pb = S.jr - pos.y;
@@ -133,7 +133,7 @@ __device__ __forceinline__ void state_body(Lat<T> L, Scalars S, const double* __
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_state(Lat<T> L, Scalars S, const double* __restrict__ z,
                                                   const double* __restrict__ p, double step, double* __restrict__ zc,
-                                                  double* __restrict__ x, T* __restrict__ R,
+                                                  double* __restrict__ x, double* __restrict__ R,
                                                   const double* __restrict__ xref, TermBufs nt,
                                                   double* __restrict__ part) {
   state_body<T>(L, S, z, p, step, zc, x, R, xref, nt, part);
@@ -372,8 +372,8 @@ template <class T>
 __device__ __forceinline__ void contact_body(Lat<T> L, Scalars S, const double* __restrict__ x,
                                                     const unsigned* __restrict__ pcount, const int* __restrict__ plist,
                                                     double* __restrict__ cf_sph, double* __restrict__ cf_cap,
-                                                    TermBufs ct, double* __restrict__ part) {
-  const int N = L.N, B = L.B, NS = N + B, s = blockIdx.x * kBlock + threadIdx.x, P = gridDim.x;
+                                                    TermBufs ct, double* __restrict__ part, int blk, int P) {
+  const int N = L.N, B = L.B, NS = N + B, s = blk * kBlock + threadIdx.x;
   double v[4] = {0, 0, 0, 0};
   if (s < NS) {
     v3<double> f0{0, 0, 0}, f1{0, 0, 0};
@@ -433,14 +433,14 @@ __device__ __forceinline__ void contact_body(Lat<T> L, Scalars S, const double* 
   block_reduce<4>(v, 1u << 1);
   if (threadIdx.x == 0)
 #pragma unroll
-    for (int k = 0; k < 4; k++) part[k * P + blockIdx.x] = v[k];
+    for (int k = 0; k < 4; k++) part[k * P + blk] = v[k];
 }
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_contact(Lat<T> L, Scalars S, const double* __restrict__ x,
                                                     const unsigned* __restrict__ pcount, const int* __restrict__ plist,
                                                     double* __restrict__ cf_sph, double* __restrict__ cf_cap,
                                                     TermBufs ct, double* __restrict__ part) {
-  contact_body<T>(L, S, x, pcount, plist, cf_sph, cf_cap, ct, part);
+  contact_body<T>(L, S, x, pcount, plist, cf_sph, cf_cap, ct, part, blockIdx.x, gridDim.x);
 }
 
 
@@ -451,10 +451,10 @@ __global__ __launch_bounds__(kBlock) void k_contact(Lat<T> L, Scalars S, const d
 // ---------------------------------------------------------------------------
 template <class T>
 __device__ __forceinline__ void beam_body(Lat<T> L, Material M, const double* __restrict__ x,
-                                                 const T* __restrict__ R, double* __restrict__ rec,
+                                                 const double* __restrict__ R, double* __restrict__ rec,
                                                  double* __restrict__ trial, TermBufs bt,
-                                                 double* __restrict__ part) {
-  const int N = L.N, B = L.B, k = blockIdx.x * kBlock + threadIdx.x, P = gridDim.x;
+                                                 double* __restrict__ part, int blk, int P) {
+  const int N = L.N, B = L.B, k = blk * kBlock + threadIdx.x;
   double v[4] = {0, 0, 0, 0};
   if (k < B) {
     const int ia = L.ba[k], ib = L.bb[k];
@@ -476,21 +476,25 @@ __device__ __forceinline__ void beam_body(Lat<T> L, Material M, const double* __
       ret1 ax = return1(eps, pl[0], pl[1], M.E, M.sy, M.H);
       double uax = A * L0 * (ax.energy + ax.hard), dax = A * L0 * M.sy * ax.dg;
       v3<double> f = t * (A * ax.force);
-      // shear: material directors against the chord
-      quat<T> Ra = node_R<T>(R, N, ia), Rb = node_R<T>(R, N, ib);
-      v3<T> e0{T(L.e0[k]), T(L.e0[B + k]), T(L.e0[2 * B + k])};
+      // shear: material directors against the chord, in FP64 for either
+      // element type (the mismatch 2t - da - db cancels unit vectors to a few
+      // percent of their length)
+      quat<double> Ra = node_R<double>(R, N, ia), Rb = node_R<double>(R, N, ib);
+      v3<double> e0{L.e0[k], L.e0[B + k], L.e0[2 * B + k]};
+      v3<double> da = qrot(Ra, e0), db = qrot(Rb, e0);
+      double cs = 0.5 * M.shear * M.G * A * L0;
+      v3<double> ma = da - t, mb = db - t;
+      double us = 0.5 * cs * (norm2(ma) + norm2(mb));
+      v3<double> gt = ((t * 2.0 - da) - db) * cs;
+      v3<double> trans = over(gt - t * dot(gt, t), Ln);
+      f = f + trans;
+      v3<double> ga = -(cross(da, t) * cs), gb = -(cross(db, t) * cs);
+      // bending + torsion from the relative rotation (formed in FP64); its
+      // logarithm, the beam axis in the frame of a and the moment transport in T
+      quat<double> Q = qnormalize(qmul(qconj(Ra), Rb));
+      quat<T> QT{T(Q.w), T(Q.x), T(Q.y), T(Q.z)}, RaT{T(Ra.w), T(Ra.x), T(Ra.y), T(Ra.z)};
       v3<T> tT = cvt<T>(t);
-      v3<T> da = qrot(Ra, e0), db = qrot(Rb, e0);
-      T cs = T(0.5 * M.shear * M.G * A * L0);
-      v3<T> ma = da - tT, mb = db - tT;
-      double us = double(T(0.5) * cs * (norm2(ma) + norm2(mb)));
-      v3<T> gt = ((tT * T(2) - da) - db) * cs;
-      v3<T> trans = over(gt - tT * dot(gt, tT), T(Ln));
-      f = f + cvt<double>(trans);
-      v3<T> ga = -(cross(da, tT) * cs), gb = -(cross(db, tT) * cs);
-      // bending + torsion from the relative rotation
-      quat<T> Q = qnormalize(qmul(qconj(Ra), Rb));
-      v3<T> phiT = qlog(Q), axT = unit(qrotinv(Ra, tT));
+      v3<T> phiT = qlog(QT), axT = unit(qrotinv(RaT, tT));
       v3<double> phi = cvt<double>(phiT), axis = cvt<double>(axT);
       v3<double> rotp{pl[2], pl[3], pl[4]};
       double tw = dot(phi, axis), ptw = dot(rotp, axis);
@@ -502,7 +506,7 @@ __device__ __forceinline__ void beam_body(Lat<T> L, Material M, const double* __
       v3<double> nrotp = br.p + axis * to.p;
       v3<double> mom = br.force + axis * to.force;
       double ur = br.energy + br.hard + to.energy + to.hard, dr = my * br.dg + ty * to.dg;
-      v3<T> world = qrot(Ra, jlinvt(phiT, cvt<T>(mom)));
+      v3<double> world = cvt<double>(qrot(RaT, jlinvt(phiT, cvt<T>(mom))));
       ga = ga - world;
       gb = gb + world;
       v[0] = (uax + dax) + us + (ur + dr);
@@ -511,12 +515,12 @@ __device__ __forceinline__ void beam_body(Lat<T> L, Material M, const double* __
       out[0] = f.x;
       out[1] = f.y;
       out[2] = f.z;
-      out[3] = double(ga.x);
-      out[4] = double(ga.y);
-      out[5] = double(ga.z);
-      out[6] = double(gb.x);
-      out[7] = double(gb.y);
-      out[8] = double(gb.z);
+      out[3] = ga.x;
+      out[4] = ga.y;
+      out[5] = ga.z;
+      out[6] = gb.x;
+      out[7] = gb.y;
+      out[8] = gb.z;
       if (trial) {
         tr[0] = ax.p;
         tr[1] = ax.a;
@@ -548,13 +552,13 @@ __device__ __forceinline__ void beam_body(Lat<T> L, Material M, const double* __
   block_reduce<4>(v, 0u);
   if (threadIdx.x == 0)
 #pragma unroll
-    for (int c = 0; c < 4; c++) part[c * P + blockIdx.x] = v[c];
+    for (int c = 0; c < 4; c++) part[c * P + blk] = v[c];
 }
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_beam(Lat<T> L, Material M, const double* __restrict__ x,
-                                                 const T* __restrict__ R, double* __restrict__ rec,
+                                                 const double* __restrict__ R, double* __restrict__ rec,
                                                  double* __restrict__ trial, TermBufs bt, double* __restrict__ part) {
-  beam_body<T>(L, M, x, R, rec, trial, bt, part);
+  beam_body<T>(L, M, x, R, rec, trial, bt, part, blockIdx.x, gridDim.x);
 }
 
 
@@ -720,7 +724,7 @@ __global__ __launch_bounds__(kBlock) void k_final(int N, double centering, doubl
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_beam_potential(Lat<T> L, Material M, const double* __restrict__ x,
                                                            const double* __restrict__ R, double* __restrict__ part) {
-  const int N = L.N, B = L.B, k = blockIdx.x * kBlock + threadIdx.x, P = gridDim.x;
+  const int N = L.N, B = L.B, k = blockIdx.x * kBlock + threadIdx.x;
   double v[1] = {0};
   if (k < B) {
     const int ia = L.ba[k], ib = L.bb[k];
@@ -841,6 +845,46 @@ __device__ __forceinline__ void direction_body(int N, size_t n6, const double* _
   }
   write_dot_partial(dv, part);
 }
+// Element-parallel form for the device loop: blocks of 6 waves, wave d holds
+// DOF plane d of 64 nodes; the per-node clamp reads the node's other planes
+// from LDS.  Same arithmetic per element as direction_body.
+constexpr int kDirNodes = kWave, kDirThreads = 6 * kDirNodes;
+__device__ __forceinline__ void direction_elem(int N, size_t n6, const double* __restrict__ g,
+                                               const double* __restrict__ Sh, const double* __restrict__ Yh,
+                                               const DirCoef& C, double* __restrict__ p,
+                                               const uint8_t* __restrict__ lock, double mt, double mr,
+                                               double* __restrict__ part) {
+  __shared__ double qs[6][kDirNodes];
+  const int d = threadIdx.x / kDirNodes, l = threadIdx.x % kDirNodes;
+  const int i = blockIdx.x * kDirNodes + l;
+  const bool on = i < N;
+  const size_t e = size_t(d) * N + i;
+  const bool locked = on && ((lock[i] >> d) & 1u);
+  double gv = 0.0, q = 0.0;
+  if (on) {
+    gv = g[e];
+    q = C.cg * gv;
+    for (int j = 0; j < C.k; j++) q += C.cs[j] * Sh[n6 * C.slot[j] + e] + C.cy[j] * Yh[n6 * C.slot[j] + e];
+    q = -q;
+    if (locked) q = 0.0;
+  }
+  qs[d][l] = q;
+  __syncthreads();
+  double dv = 0.0;
+  if (on) {
+    const int b = d < 3 ? 0 : 3;
+    const double a0 = qs[b][l], a1 = qs[b + 1][l], a2 = qs[b + 2][l];
+    const double len = sqrt(a0 * a0 + a1 * a1 + a2 * a2), lim = d < 3 ? mt : mr;
+    if (len > lim) q *= lim / len;
+    if (locked) q = 0.0;
+    p[e] = q;
+    dv = gv * q;
+  }
+  double t[1] = {dv};
+  block_reduce<1, kDirThreads>(t, 0u);
+  if (threadIdx.x == 0) part[blockIdx.x] = t[0];
+}
+
 __global__ __launch_bounds__(kBlock) void k_direction(int N, size_t n6, const double* __restrict__ g,
                                                       const double* __restrict__ Sh, const double* __restrict__ Yh,
                                                       DirCoef C, double* __restrict__ p,
@@ -921,6 +965,58 @@ __device__ __forceinline__ void pair_dots_body(int N, size_t n6, const double* _
       for (int r = 0; r < 16 && c0 + r < K; r++) part[(c0 + r) * P + blockIdx.x] = v[r];
   }
 }
+// Element-parallel form for the device loop: one thread per vector element
+// (the 6N DOF planes are contiguous, so every vector is a flat array of n6),
+// ceil(n6 / kBlock) blocks; same partial layout [4*nold + 5][P].
+__device__ __forceinline__ void pair_dots_elem(size_t n6, const double* __restrict__ z, const double* __restrict__ zc,
+                                               const double* __restrict__ g, const double* __restrict__ gc,
+                                               double* __restrict__ Sh, double* __restrict__ Yh, const PairArgs& A,
+                                               double* __restrict__ part) {
+  const size_t e = size_t(blockIdx.x) * kBlock + threadIdx.x;
+  const int P = gridDim.x;
+  double sv = 0.0, yv = 0.0, gv = 0.0;
+  if (e < n6) {
+    gv = gc[e];
+    sv = zc[e] - z[e];
+    yv = gv - g[e];
+    Sh[n6 * A.t + e] = sv;
+    Yh[n6 * A.t + e] = yv;
+  }
+  constexpr int KM = 4 * kMaxHist + 5;
+  double acc[KM];
+#pragma unroll
+  for (int r = 0; r < KM; r++) acc[r] = 0.0;
+  if (e < n6) {
+#pragma unroll
+    for (int jj = 0; jj < kMaxHist; jj++) {
+      if (jj < A.nold) {
+        const double a = Sh[n6 * A.slot[jj] + e], b = Yh[n6 * A.slot[jj] + e];
+        acc[4 * jj + 0] = a * yv;
+        acc[4 * jj + 1] = b * yv;
+        acc[4 * jj + 2] = a * gv;
+        acc[4 * jj + 3] = b * gv;
+      }
+    }
+  }
+  const double w[5] = {sv * yv, sv * sv, yv * yv, sv * gv, yv * gv};
+#pragma unroll
+  for (int jj = 0; jj <= kMaxHist; jj++)
+    if (jj == A.nold)
+#pragma unroll
+      for (int c = 0; c < 5; c++) acc[4 * jj + c] = w[c];
+  const int K = 4 * A.nold + 5;
+#pragma unroll
+  for (int c0 = 0; c0 < KM; c0 += 16) {
+    if (c0 >= K) break;
+    double v[16];
+#pragma unroll
+    for (int r = 0; r < 16; r++) v[r] = (c0 + r < KM) ? acc[c0 + r] : 0.0;
+    block_reduce<16>(v, 0u);
+    if (threadIdx.x == 0)
+      for (int r = 0; r < 16 && c0 + r < K; r++) part[(c0 + r) * P + blockIdx.x] = v[r];
+  }
+}
+
 __global__ __launch_bounds__(kBlock) void k_pair_dots(int N, size_t n6, const double* __restrict__ z,
                                                       const double* __restrict__ zc, const double* __restrict__ g,
                                                       const double* __restrict__ gc, double* __restrict__ Sh,
@@ -944,14 +1040,18 @@ __global__ __launch_bounds__(kBlock) void k_sum_rows(const double* __restrict__ 
 // the loop state; every kernel of one L-BFGS iteration is predicated on it, so
 // the iteration can be captured once as a hipGraph and replayed without host
 // round trips.  Phases of one iteration:
-//   DIRECTION  p = -H g (compact form), g.p                       k_dir/k_dgcheck
+//   DIRECTION  p = -H g (compact form), g.p; the descent check     k_dir, k_state
+//              runs in the state kernel of the same replay
 //   STEEPEST   p = -g when g.p >= 0 or after a failed line search  (same, mode 1)
 //   SEARCH     one Armijo trial per replay: state, beam, contact,
 //              gather, final (decision: accept, halve, or fail)
 //   PAIR       s, y and their inner products; history update,
 //              convergence test of the new iterate, next coefficients
-// Only the single-block kernels write the control block; the wide kernels read
-// it before they start (kernel boundaries order the two).
+// The single-block kernels own the control block.  One wide kernel writes to
+// it: block 0 of the state kernel records the descent-check outcome in two
+// fields (srch_eval, srch_dg) that the state kernel itself never reads; every
+// other field a wide kernel reads is written only by earlier kernels (kernel
+// boundaries order the two).
 // ---------------------------------------------------------------------------
 enum { PH_DIRECTION = 0, PH_STEEPEST = 2, PH_SEARCH = 4, PH_PAIR = 5 };
 enum { ST_RUNNING = 0, ST_CONVERGED = 1, ST_FAILED = 2, ST_ITERCAP = 3, ST_HOLD = 4 };
@@ -959,7 +1059,8 @@ enum { ST_RUNNING = 0, ST_CONVERGED = 1, ST_FAILED = 2, ST_ITERCAP = 3, ST_HOLD 
 struct Ctl {
   int status, phase, par, it, iterations, backtracks, evals, retry;
   int bt, count, head, m, max_it, max_ls, rel_escape, ls_mode;
-  int wolfe, pad;
+  int wolfe, srch_eval;  // srch_eval: the state kernel of this replay evaluates a trial
+  double srch_dg;        // g.p of a fresh direction, reduced by the state kernel
   double step, dg, top, armijo, min_step, tol_rms, tol_max, rel_tol, pen_tol, ts, skin_half, wolfe_eps;
   double cur[kRec], cand[kRec];
   DirCoef coef;
@@ -1052,6 +1153,61 @@ __host__ __device__ inline PairArgs pair_slots(int m, int count, int head) {
   return A;
 }
 
+// compact_coefficients for one wavefront (all 64 lanes call it; lane l holds
+// history row l, oldest first).  The two triangular solves run by columns:
+// each step broadcasts one solved component and every lane updates its own
+// residual, so the dependent chain is k steps instead of k(k+1)/2.
+__device__ void compact_coefficients_wave(int m, int count, int head, const double (*SY)[kMaxHist],
+                                          const double (*YY)[kMaxHist], const double* u, const double* v,
+                                          DirCoef& C) {
+  const int k = count, l = threadIdx.x & (kWave - 1);
+  if (k == 0) {
+    if (l == 0) {
+      C.k = 0;
+      C.cg = 0.10;
+    }
+    return;
+  }
+  const bool on = l < k;
+  const int sl = on ? (head + l) % m : 0;
+  const int last = (head + k - 1) % m;
+  const double sy = SY[last][last], yy = YY[last][last];
+  const double gamma = (sy > 1e-20 && yy > 1e-20) ? fmin(1e4, fmax(1e-7, sy / yy)) : 0.10;
+  const double dl = on ? SY[sl][sl] : 1.0;
+  // w = R^{-1} u, R the upper triangle of S'Y (R_ij = s_i . y_j, i <= j)
+  double r = on ? u[sl] : 0.0, w = 0.0;
+  for (int i = k - 1; i >= 0; i--) {
+    const double wi = __shfl(r / dl, i, kWave);
+    const int si = (head + i) % m;
+    if (l == i) w = wi;
+    if (l < i) r -= SY[sl][si] * wi;
+  }
+  // t = (D + gamma Y'Y) w - gamma v
+  double t = on ? SY[sl][sl] * w - gamma * v[sl] : 0.0;
+  for (int j = 0; j < k; j++) {
+    const double wj = __shfl(w, j, kWave);
+    if (on) t += gamma * YY[sl][(head + j) % m] * wj;
+  }
+  // a = R^{-T} t
+  double a = 0.0;
+  r = t;
+  for (int i = 0; i < k; i++) {
+    const double ai = __shfl(r / dl, i, kWave);
+    const int si = (head + i) % m;
+    if (l == i) a = ai;
+    if (on && l > i) r -= SY[si][sl] * ai;
+  }
+  if (on) {
+    C.slot[l] = sl;
+    C.cs[l] = a;
+    C.cy[l] = -gamma * w;
+  }
+  if (l == 0) {
+    C.k = k;
+    C.cg = gamma;
+  }
+}
+
 // The single-block kernels copy the control block to LDS with one coalesced
 // round trip, run their scalar logic there, and copy it back.
 static_assert(sizeof(Ctl) % 8 == 0, "Ctl is copied as 8-byte words");
@@ -1066,19 +1222,21 @@ __device__ __forceinline__ void ctl_store(Ctl* C, const Ctl& sc) {
   for (int e = threadIdx.x; e < W; e += kBlock) reinterpret_cast<double*>(C)[e] = reinterpret_cast<const double*>(&sc)[e];
 }
 
-__device__ __forceinline__ bool ctl_converged(const Ctl* C, const double* r) {
-  const double rms = r[6], mx = r[7];
-  const double rel = rms / fmax(1e-6, C->ts * fmax(1.0, r[4]));
-  const bool ok = (rms <= C->tol_rms && mx <= C->tol_max) || (C->rel_escape && rel < C->rel_tol);
-  return ok && r[5] <= C->pen_tol;
-}
 __device__ __forceinline__ bool ctl_gate(const Ctl* C, int phase) {
   return C->status == ST_RUNNING && C->phase == phase;
 }
+// A replay evaluates a trial right after a fresh direction (phase DIRECTION or
+// STEEPEST: the descent check runs inside the state kernel) or on a backtrack
+// (phase SEARCH).
+__device__ __forceinline__ bool ctl_trial_phase(const Ctl* C) {
+  return C->status == ST_RUNNING &&
+         (C->phase == PH_SEARCH || C->phase == PH_DIRECTION || C->phase == PH_STEEPEST);
+}
+__device__ __forceinline__ bool ctl_trial(const Ctl* C) { return ctl_trial_phase(C) && C->srch_eval; }
 
 // L-BFGS direction from the control block's coefficients (phase DIRECTION)
 // or -g (phase STEEPEST)
-__global__ __launch_bounds__(kBlock) void k_dir_g(const Ctl* __restrict__ C, int N, size_t n6,
+__global__ __launch_bounds__(kDirThreads) void k_dir_g(const Ctl* __restrict__ C, int N, size_t n6,
                                                   const double* __restrict__ g0, const double* __restrict__ g1,
                                                   const double* __restrict__ Sh, const double* __restrict__ Yh,
                                                   double* __restrict__ p, const uint8_t* __restrict__ lock, double mt,
@@ -1088,68 +1246,66 @@ __global__ __launch_bounds__(kBlock) void k_dir_g(const Ctl* __restrict__ C, int
   if (C->phase == PH_STEEPEST) {
     DirCoef D{};
     D.cg = 1.0;
-    direction_body(N, n6, g, Sh, Yh, D, p, lock, mt, mr, part);
+    direction_elem(N, n6, g, Sh, Yh, D, p, lock, mt, mr, part);
   } else {
-    direction_body(N, n6, g, Sh, Yh, C->coef, p, lock, mt, mr, part);
-  }
-}
-
-// descent check, MC/src/sim.cpp:191: g.p >= 0 resets to steepest descent,
-// computed by the next replay's direction kernel
-__global__ __launch_bounds__(kBlock) void k_dgcheck_g(Ctl* __restrict__ C, const double* __restrict__ part, int P) {
-  if (C->status != ST_RUNNING || (C->phase != PH_DIRECTION && C->phase != PH_STEEPEST)) return;
-  const int mode = C->phase == PH_STEEPEST;
-  double v[1];
-  reduce_partials<1>(part, P, v, 0u);
-  if (threadIdx.x == 0) {
-    if (mode == 0 && !(v[0] < 0)) {
-      C->count = 0;
-      C->head = 0;
-      C->phase = PH_STEEPEST;
-    } else {
-      C->dg = v[0];
-      C->step = 1.0;
-      C->bt = 0;
-      C->phase = PH_SEARCH;
-    }
+    direction_elem(N, n6, g, Sh, Yh, C->coef, p, lock, mt, mr, part);
   }
 }
 
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_state_g(Lat<T> L, Scalars S, const Ctl* __restrict__ C,
+__global__ __launch_bounds__(kBlock) void k_state_g(Lat<T> L, Scalars S, const Ctl* C,
                                                     double* __restrict__ z0, double* __restrict__ z1,
                                                     const double* __restrict__ p, double* __restrict__ x,
-                                                    T* __restrict__ R, const double* __restrict__ xref,
+                                                    double* __restrict__ R, const double* __restrict__ xref,
                                                     double* __restrict__ nb0, double* __restrict__ nb1,
-                                                    double* __restrict__ part) {
-  if (!ctl_gate(C, PH_SEARCH)) return;
+                                                    const double* __restrict__ dpart, int Pd,
+                                                    double* __restrict__ part, Ctl* Cw) {
+  if (!ctl_trial_phase(C)) return;
+  double step = C->step;
+  if (C->phase != PH_SEARCH) {
+    // descent check of the fresh direction, MC/src/sim.cpp:191: every block
+    // reduces the same partials in the same order; block 0 records the outcome
+    // for the later kernels of this replay (fields this kernel never reads)
+    double dg[1];
+    reduce_partials<1>(dpart, Pd, dg, 0u);
+    const bool eval = !(C->phase == PH_DIRECTION && !(dg[0] < 0));
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      Cw->srch_eval = eval;
+      Cw->srch_dg = dg[0];
+    }
+    if (!eval) return;
+    step = 1.0;
+  } else if (blockIdx.x == 0 && threadIdx.x == 0) {
+    Cw->srch_eval = 1;
+  }
   S.top = C->top;
   const int par = C->par;
   TermBufs nt{par ? nb1 : nb0, par ? nb0 : nb1};
-  state_body<T>(L, S, par ? z1 : z0, p, C->step, par ? z0 : z1, x, R, xref, nt, part);
+  state_body<T>(L, S, par ? z1 : z0, p, step, par ? z0 : z1, x, R, xref, nt, part);
 }
+// beams and contact shapes of one trial in a single launch: blocks [0, Pb)
+// evaluate beams, blocks [Pb, Pb + Pc) contact shapes (independent work; one
+// launch gap fewer per trial)
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_beam_g(const Ctl* __restrict__ C, Lat<T> L, Material M,
-                                                   const double* __restrict__ x, const T* __restrict__ R,
+__global__ __launch_bounds__(kBlock) void k_elem_g(const Ctl* __restrict__ C, Lat<T> L, Material M, Scalars S,
+                                                   const double* __restrict__ x, const double* __restrict__ R,
                                                    double* __restrict__ rec, double* __restrict__ ub0,
-                                                   double* __restrict__ ub1, double* __restrict__ part) {
-  if (!ctl_gate(C, PH_SEARCH)) return;
+                                                   double* __restrict__ ub1, double* __restrict__ bpart, int Pb,
+                                                   const unsigned* __restrict__ pcount,
+                                                   const int* __restrict__ plist, double* __restrict__ cf_sph,
+                                                   double* __restrict__ cf_cap, double* __restrict__ cb0,
+                                                   double* __restrict__ cb1, double* __restrict__ cpart, int Pc) {
+  if (!ctl_trial(C)) return;
   const int par = C->par;
-  TermBufs bt{par ? ub1 : ub0, par ? ub0 : ub1};
-  beam_body<T>(L, M, x, R, rec, nullptr, bt, part);
+  if (int(blockIdx.x) < Pb) {
+    TermBufs bt{par ? ub1 : ub0, par ? ub0 : ub1};
+    beam_body<T>(L, M, x, R, rec, nullptr, bt, bpart, blockIdx.x, Pb);
+  } else {
+    TermBufs ct{par ? cb1 : cb0, par ? cb0 : cb1};
+    contact_body<T>(L, S, x, pcount, plist, cf_sph, cf_cap, ct, cpart, blockIdx.x - Pb, Pc);
+  }
 }
-template <class T>
-__global__ __launch_bounds__(kBlock) void k_contact_g(const Ctl* __restrict__ C, Lat<T> L, Scalars S,
-                                                      const double* __restrict__ x,
-                                                      const unsigned* __restrict__ pcount,
-                                                      const int* __restrict__ plist, double* __restrict__ cf_sph,
-                                                      double* __restrict__ cf_cap, double* __restrict__ cb0,
-                                                      double* __restrict__ cb1, double* __restrict__ part) {
-  if (!ctl_gate(C, PH_SEARCH)) return;
-  const int par = C->par;
-  TermBufs ct{par ? cb1 : cb0, par ? cb0 : cb1};
-  contact_body<T>(L, S, x, pcount, plist, cf_sph, cf_cap, ct, part);
-}
+
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_gather_g(Lat<T> L, Scalars S, const Ctl* __restrict__ C,
                                                      const double* __restrict__ x, const double* __restrict__ rec,
@@ -1158,7 +1314,7 @@ __global__ __launch_bounds__(kBlock) void k_gather_g(Lat<T> L, Scalars S, const 
                                                      const double* __restrict__ spart, int Ps,
                                                      double* __restrict__ g0, double* __restrict__ g1,
                                                      const double* __restrict__ p, double* __restrict__ part) {
-  if (!ctl_gate(C, PH_SEARCH)) return;
+  if (!ctl_trial(C)) return;
   S.top = C->top;
   gather_body<T>(L, S, x, rec, cf_sph, cf_cap, spart, Ps, C->par ? g0 : g1, p, part);
 }
@@ -1170,9 +1326,27 @@ __global__ __launch_bounds__(kBlock) void k_final_g(Ctl* __restrict__ C, int N, 
                                                     const double* __restrict__ bpart, int Pb,
                                                     const double* __restrict__ gpart, int Pg,
                                                     int* __restrict__ flags) {
-  if (!ctl_gate(C, PH_SEARCH)) return;
+  if (!ctl_trial_phase(C)) return;
   __shared__ Ctl sc;
   ctl_load(C, sc);
+  if (sc.phase != PH_SEARCH) {
+    if (!sc.srch_eval) {  // not a descent direction: steepest descent next
+      if (threadIdx.x == 0) {
+        sc.count = 0;
+        sc.head = 0;
+        sc.phase = PH_STEEPEST;
+      }
+      ctl_store(C, sc);
+      return;
+    }
+    if (threadIdx.x == 0) {
+      sc.dg = sc.srch_dg;
+      sc.step = 1.0;
+      sc.bt = 0;
+      sc.phase = PH_SEARCH;
+    }
+    __syncthreads();
+  }
   final_body(N, centering, n_free, spart, Ps, cpart, Pc, bpart, Pb, gpart, Pg, flags, sc.cur, sc.cand);
   if (threadIdx.x == 0) {
     Ctl* c = &sc;
@@ -1219,7 +1393,7 @@ __global__ __launch_bounds__(kBlock) void k_pair_g(const Ctl* __restrict__ C, in
                                                    double* __restrict__ part) {
   if (!ctl_gate(C, PH_PAIR) || C->m == 0) return;
   const int par = C->par;
-  pair_dots_body(N, n6, par ? z1 : z0, par ? z0 : z1, par ? g1 : g0, par ? g0 : g1, Sh, Yh, C->pair, part);
+  pair_dots_elem(n6, par ? z1 : z0, par ? z0 : z1, par ? g1 : g0, par ? g0 : g1, Sh, Yh, C->pair, part);
 }
 
 // history update, swap of current/candidate buffers, next iteration's
@@ -1230,22 +1404,17 @@ __global__ __launch_bounds__(kBlock) void k_update_g(Ctl* __restrict__ C, const 
   if (!ctl_gate(C, PH_PAIR)) return;
   __shared__ Ctl sc;
   __shared__ double d[4 * kMaxHist + 5];
+  __shared__ int need_coef;
   ctl_load(C, sc);
   const int m = sc.m;
   if (m > 0) {
-    const int K = 4 * sc.pair.nold + 5;
-    for (int c0 = 0; c0 < K; c0 += 16) {  // 16 rows per pass, independent loads
-      double acc[16];
-#pragma unroll
-      for (int r = 0; r < 16; r++) acc[r] = 0.0;
-      for (int i = threadIdx.x; i < P; i += kBlock) {
-#pragma unroll
-        for (int r = 0; r < 16; r++)
-          if (c0 + r < K) acc[r] += part[size_t(c0 + r) * P + i];
-      }
-      block_reduce<16>(acc, 0u);
-      if (threadIdx.x == 0)
-        for (int r = 0; r < 16 && c0 + r < K; r++) d[c0 + r] = acc[r];
+    // one wave per row (rows strided over the block's waves), fixed order
+    const int K = 4 * sc.pair.nold + 5, lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
+    for (int r = wv; r < K; r += kBlock / kWave) {
+      double a = 0.0;
+      for (int i = lane; i < P; i += kWave) a += part[size_t(r) * P + i];
+      a = wave_sum(a);
+      if (lane == 0) d[r] = a;
     }
   }
   __syncthreads();
@@ -1256,6 +1425,7 @@ __global__ __launch_bounds__(kBlock) void k_update_g(Ctl* __restrict__ C, const 
     for (int k = 0; k < kRec; k++) c->cur[k] = c->cand[k];
     c->it++;
     c->retry = 0;
+    need_coef = 0;
     if (c->it >= c->max_it) {
       c->status = ST_ITERCAP;
     } else {
@@ -1266,11 +1436,14 @@ __global__ __launch_bounds__(kBlock) void k_update_g(Ctl* __restrict__ C, const 
       if (ok && r[5] <= c->pen_tol) {
         c->status = ST_CONVERGED;
       } else {
-        compact_coefficients(m, c->count, c->head, c->SY, c->YY, c->u, c->v, c->coef);
+        need_coef = 1;
         c->phase = PH_DIRECTION;
       }
     }
   }
+  __syncthreads();
+  if (need_coef && threadIdx.x < kWave)
+    compact_coefficients_wave(m, sc.count, sc.head, sc.SY, sc.YY, sc.u, sc.v, sc.coef);
   ctl_store(C, sc);
 }
 
@@ -1402,7 +1575,7 @@ struct mmc_ctx {
   unsigned table = 0;
   double *spart = nullptr, *cpart = nullptr, *bpart = nullptr, *gpart = nullptr, *vpart = nullptr;
   double* stpart = nullptr;
-  int Ps = 0, Pc = 0, Pb = 0;
+  int Ps = 0, Pc = 0, Pb = 0, Pe = 0;  // blocks over nodes, shapes, beams, 6N vector elements
   double* record = nullptr;  // kRec per slot, slots: 0 current, 1 candidate, 2 scratch
   double* scal = nullptr;    // misc scalars (dots)
   double *z = nullptr, *g = nullptr, *p = nullptr, *zc = nullptr, *gc = nullptr, *zt = nullptr, *zin = nullptr;
@@ -1616,8 +1789,7 @@ void mmc_ctx::setup(const mmc_lattice* L, const mmc_config* c, int prec) {
   if (!alist.empty()) HIPCALL(hipMemcpy(adj, alist.data(), alist.size() * sizeof(int), hipMemcpyHostToDevice));
   // ---- evaluation buffers ----
   x = dalloc<double>(3 * size_t(N));
-  if (precision == MMC_MIXED) R = dalloc<float>(4 * size_t(N));
-  else R = dalloc<double>(4 * size_t(N));
+  R = dalloc<double>(4 * size_t(N));
   rec = dalloc<double>(9 * size_t(B));
   trial = dalloc<double>(size_t(MMC_TRIAL_STRIDE) * B);
   cf_sph = dalloc<double>(3 * size_t(N));
@@ -1645,7 +1817,8 @@ void mmc_ctx::setup(const mmc_lattice* L, const mmc_config* c, int prec) {
   cpart = dalloc<double>(4 * size_t(Pc));
   bpart = dalloc<double>(4 * size_t(Pb));
   gpart = dalloc<double>(3 * size_t(Ps));
-  vpart = dalloc<double>((4 * size_t(kMaxHist) + 8) * size_t(Ps));
+  Pe = blocks(6 * N);
+  vpart = dalloc<double>((4 * size_t(kMaxHist) + 8) * size_t(std::max({Ps, Pe, (N + kDirNodes - 1) / kDirNodes})));
   stpart = dalloc<double>(8 * size_t(blocks(std::max(N, B))));
   record = dalloc<double>(kRec * 4);
   scal = dalloc<double>(kScal);
@@ -1732,7 +1905,7 @@ void mmc_ctx::evaluate_t(Lat<T> Lt, const double* zsrc, const double* psrc, doub
   tick(MMC_K_STATE);
   const bool terms = terms_out0;
   terms_out0 = false;
-  k_state<T><<<Ps, kBlock, 0, stream>>>(Lt, S, zsrc, psrc, step, psrc ? zdst : nullptr, x, (T*)R,
+  k_state<T><<<Ps, kBlock, 0, stream>>>(Lt, S, zsrc, psrc, step, psrc ? zdst : nullptr, x, (double*)R,
                                         list_valid ? xref : nullptr, TermBufs{nullptr, terms ? nbuf[0] : nullptr},
                                         spart);
   tock(MMC_K_STATE);
@@ -1742,7 +1915,7 @@ void mmc_ctx::evaluate_t(Lat<T> Lt, const double* zsrc, const double* psrc, doub
     built = true;
   }
   tick(MMC_K_BEAM);
-  k_beam<T><<<Pb, kBlock, 0, stream>>>(Lt, M, x, (const T*)R, rec, trials ? trial : nullptr,
+  k_beam<T><<<Pb, kBlock, 0, stream>>>(Lt, M, x, (const double*)R, rec, trials ? trial : nullptr,
                                        TermBufs{nullptr, terms ? ubuf[0] : nullptr}, bpart);
   tock(MMC_K_BEAM);
   for (;;) {
@@ -1941,21 +2114,21 @@ void mmc_ctx::capture_iteration(Lat<T> Lt) {
   if (graph) HIPCALL(hipGraphDestroy(graph));
   graph_exec = nullptr;
   graph = nullptr;
-  const int P = Ps;
+  const int Pd = (N + kDirNodes - 1) / kDirNodes;
   const size_t n6 = 6 * size_t(N);
   const double mt = cfg.max_translation_step / cfg.translation_scale, mr = cfg.max_rotation_step;
   HIPCALL(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
   for (int u = 0; u < kGraphIterations; u++) {
-  k_dir_g<<<P, kBlock, 0, stream>>>(ctl, N, n6, gbuf[0], gbuf[1], Sh, Yh, p, lock, mt, mr, vpart);
-  k_dgcheck_g<<<1, kBlock, 0, stream>>>(ctl, vpart, P);
-  k_state_g<T><<<Ps, kBlock, 0, stream>>>(Lt, S, ctl, zbuf[0], zbuf[1], p, x, (T*)R, xref, nbuf[0], nbuf[1], spart);
-  k_beam_g<T><<<Pb, kBlock, 0, stream>>>(ctl, Lt, M, x, (const T*)R, rec, ubuf[0], ubuf[1], bpart);
-  k_contact_g<T><<<Pc, kBlock, 0, stream>>>(ctl, Lt, S, x, pcount, plist, cf_sph, cf_cap, cbuf[0], cbuf[1], cpart);
+  k_dir_g<<<Pd, kDirThreads, 0, stream>>>(ctl, N, n6, gbuf[0], gbuf[1], Sh, Yh, p, lock, mt, mr, vpart);
+  k_state_g<T><<<Ps, kBlock, 0, stream>>>(Lt, S, ctl, zbuf[0], zbuf[1], p, x, (double*)R, xref, nbuf[0], nbuf[1],
+                                          vpart, Pd, spart, ctl);
+  k_elem_g<T><<<Pb + Pc, kBlock, 0, stream>>>(ctl, Lt, M, S, x, (const double*)R, rec, ubuf[0], ubuf[1], bpart, Pb,
+                                              pcount, plist, cf_sph, cf_cap, cbuf[0], cbuf[1], cpart, Pc);
   k_gather_g<T><<<Ps, kBlock, 0, stream>>>(Lt, S, ctl, x, rec, cf_sph, cf_cap, spart, Ps, gbuf[0], gbuf[1], p, gpart);
   k_final_g<<<1, kBlock, 0, stream>>>(ctl, N, cfg.centering_stiffness, n_free, spart, Ps, cpart, Pc, bpart, Pb,
                                       gpart, Ps, flags);
-  k_pair_g<<<P, kBlock, 0, stream>>>(ctl, N, n6, zbuf[0], zbuf[1], gbuf[0], gbuf[1], Sh, Yh, vpart);
-  k_update_g<<<1, kBlock, 0, stream>>>(ctl, vpart, P);
+  k_pair_g<<<Pe, kBlock, 0, stream>>>(ctl, N, n6, zbuf[0], zbuf[1], gbuf[0], gbuf[1], Sh, Yh, vpart);
+  k_update_g<<<1, kBlock, 0, stream>>>(ctl, vpart, Pe);
   }
   HIPCALL(hipStreamEndCapture(stream, &graph));
   HIPCALL(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0));
@@ -2249,8 +2422,9 @@ int mmc_time_kernel(mmc_ctx* c, int32_t kernel, int32_t launches, double* mean_m
       HIPCALL(hipEventCreate(&c->ev0));
       HIPCALL(hipEventCreate(&c->ev1));
     }
-    auto run = [&](auto Lt, auto* R) {
-      using T = std::remove_pointer_t<decltype(R)>;
+    auto run = [&](auto Lt, auto* tag) {
+      using T = std::remove_pointer_t<decltype(tag)>;
+      double* R = static_cast<double*>(c->R);
       HIPCALL(hipEventRecord(c->ev0, c->stream));
       for (int i = 0; i < launches; i++) {
         switch (kernel) {
@@ -2282,9 +2456,9 @@ int mmc_time_kernel(mmc_ctx* c, int32_t kernel, int32_t launches, double* mean_m
       *mean_ms = double(ms) / launches;
     };
     if (c->precision == MMC_MIXED)
-      run(c->lat32(), (float*)c->R);
+      run(c->lat32(), (float*)nullptr);
     else
-      run(c->lat64(), (double*)c->R);
+      run(c->lat64(), (double*)nullptr);
   });
 }
 
