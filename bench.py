@@ -43,6 +43,10 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT / "falling-dice_amd"))
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per k_beam launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
+# passes, calibrated against device copies of known size (command and figures:
+# profiles/k_beam_cubic32x32x16.md); keyed by lattice cells
+BEAM_TRAFFIC_BYTES = {(32, 32, 16): 13.08e6}
 
 
 def workload(args):
@@ -176,7 +180,8 @@ def main():
     bytes_launch = beam_kernel_bytes(N, B, prec)
     ach = bytes_launch / (beam_ms * 1e-3) / 1e9
     roof = {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_beam",
+            "frac": round(ach / HBM_PEAK_GBS, 4),
+            "traffic": BEAM_TRAFFIC_BYTES.get((args.nx, args.ny, args.nz)), "kernel": "k_beam",
             "bytes_per_launch": bytes_launch, "mean_launch_us": round(beam_ms * 1e3, 3),
             "launches_timed": launches, "kernel_us": share}
     value = K / dt

@@ -1283,6 +1283,13 @@ __global__ __launch_bounds__(kBlock) void k_state_g(Lat<T> L, Scalars S, const C
   TermBufs nt{par ? nb1 : nb0, par ? nb0 : nb1};
   state_body<T>(L, S, par ? z1 : z0, p, step, par ? z0 : z1, x, R, xref, nt, part);
 }
+// Location of the Verlet pair list, read by the graph kernels at run time so
+// that a list reallocated with a wider slab needs no new graph.
+struct ListRef {
+  const int* plist;
+  int slab, pad;
+};
+
 // beams and contact shapes of one trial in a single launch: blocks [0, Pb)
 // evaluate beams, blocks [Pb, Pb + Pc) contact shapes (independent work; one
 // launch gap fewer per trial)
@@ -1292,7 +1299,7 @@ __global__ __launch_bounds__(kBlock) void k_elem_g(const Ctl* __restrict__ C, La
                                                    double* __restrict__ rec, double* __restrict__ ub0,
                                                    double* __restrict__ ub1, double* __restrict__ bpart, int Pb,
                                                    const unsigned* __restrict__ pcount,
-                                                   const int* __restrict__ plist, double* __restrict__ cf_sph,
+                                                   const ListRef* __restrict__ lr, double* __restrict__ cf_sph,
                                                    double* __restrict__ cf_cap, double* __restrict__ cb0,
                                                    double* __restrict__ cb1, double* __restrict__ cpart, int Pc) {
   if (!ctl_trial(C)) return;
@@ -1302,7 +1309,8 @@ __global__ __launch_bounds__(kBlock) void k_elem_g(const Ctl* __restrict__ C, La
     beam_body<T>(L, M, x, R, rec, nullptr, bt, bpart, blockIdx.x, Pb);
   } else {
     TermBufs ct{par ? cb1 : cb0, par ? cb0 : cb1};
-    contact_body<T>(L, S, x, pcount, plist, cf_sph, cf_cap, ct, cpart, blockIdx.x - Pb, Pc);
+    S.slab = lr->slab;
+    contact_body<T>(L, S, x, pcount, lr->plist, cf_sph, cf_cap, ct, cpart, blockIdx.x - Pb, Pc);
   }
 }
 
@@ -1593,8 +1601,7 @@ struct mmc_ctx {
   bool terms_out0 = false;  // the next host-driven evaluation writes slot 0 of those
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
-  const int* graph_plist = nullptr;  // pair-list pointer and slab baked into the graph
-  int graph_slab = 0;
+  ListRef* lref = nullptr;  // device copy of (plist, slab) for the graph kernels
   int64_t graph_replays = 0;
   bool have_trial = false;
   int evaluations = 0;
@@ -1662,6 +1669,7 @@ mmc_ctx::~mmc_ctx() {
   if (host) (void)hipHostFree(host);
   if (hctl) (void)hipHostFree(hctl);
   if (ctl) (void)hipFree(ctl);
+  if (lref) (void)hipFree(lref);
   for (int b = 0; b < 2; b++) {
     if (zbuf[b]) (void)hipFree(zbuf[b]);
     if (gbuf[b]) (void)hipFree(gbuf[b]);
@@ -1836,6 +1844,7 @@ void mmc_ctx::setup(const mmc_lattice* L, const mmc_config* c, int prec) {
   HIPCALL(hipHostMalloc((void**)&host, kHost * sizeof(double), hipHostMallocDefault));
   HIPCALL(hipHostMalloc((void**)&hctl, sizeof(Ctl), hipHostMallocDefault));
   ctl = dalloc<Ctl>(1);
+  lref = dalloc<ListRef>(1);
   for (int b = 0; b < 2; b++) {
     zbuf[b] = dalloc<double>(n6);
     gbuf[b] = dalloc<double>(n6);
@@ -1894,6 +1903,14 @@ void mmc_ctx::build_list(Lat<T> Lt) {
   }
   HIPCALL(hipMemcpyAsync(xref, x, 3 * size_t(N) * sizeof(double), hipMemcpyDeviceToDevice, stream));
   HIPCALL(hipGetLastError());
+  // publish the list location for the graph kernels (stream-ordered before
+  // any later replay; staged through pinned memory)
+  ListRef* hl = reinterpret_cast<ListRef*>(host + kHost - 4);
+  hl->plist = plist;
+  hl->slab = S.slab;
+  hl->pad = 0;
+  HIPCALL(hipMemcpyAsync(lref, hl, sizeof(ListRef), hipMemcpyHostToDevice, stream));
+  HIPCALL(hipStreamSynchronize(stream));
   list_valid = true;
   rebuilds++;
 }
@@ -2123,7 +2140,7 @@ void mmc_ctx::capture_iteration(Lat<T> Lt) {
   k_state_g<T><<<Ps, kBlock, 0, stream>>>(Lt, S, ctl, zbuf[0], zbuf[1], p, x, (double*)R, xref, nbuf[0], nbuf[1],
                                           vpart, Pd, spart, ctl);
   k_elem_g<T><<<Pb + Pc, kBlock, 0, stream>>>(ctl, Lt, M, S, x, (const double*)R, rec, ubuf[0], ubuf[1], bpart, Pb,
-                                              pcount, plist, cf_sph, cf_cap, cbuf[0], cbuf[1], cpart, Pc);
+                                              pcount, lref, cf_sph, cf_cap, cbuf[0], cbuf[1], cpart, Pc);
   k_gather_g<T><<<Ps, kBlock, 0, stream>>>(Lt, S, ctl, x, rec, cf_sph, cf_cap, spart, Ps, gbuf[0], gbuf[1], p, gpart);
   k_final_g<<<1, kBlock, 0, stream>>>(ctl, N, cfg.centering_stiffness, n_free, spart, Ps, cpart, Pc, bpart, Pb,
                                       gpart, Ps, flags);
@@ -2132,8 +2149,7 @@ void mmc_ctx::capture_iteration(Lat<T> Lt) {
   }
   HIPCALL(hipStreamEndCapture(stream, &graph));
   HIPCALL(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0));
-  graph_plist = plist;
-  graph_slab = S.slab;
+
 }
 
 // Simulator::minimize, MC/src/sim.cpp:174-211, driven from the device: the host
@@ -2180,7 +2196,7 @@ void mmc_ctx::minimize_graph(double top, mmc_opt_stats& st) {
     else
       capture_iteration<double>(lat64());
   };
-  if (!graph_exec || graph_plist != plist || graph_slab != S.slab) recapture();
+  if (!graph_exec) recapture();
   // every replay advances the phase machine by at least one step; bound the
   // total so that a logic error cannot spin forever
   const int64_t guard = int64_t(cfg.max_iterations + 1) * (cfg.max_line_search + 3) * 2 + 64;
@@ -2195,7 +2211,6 @@ void mmc_ctx::minimize_graph(double top, mmc_opt_stats& st) {
         build_list<float>(lat32());
       else
         build_list<double>(lat64());
-      if (graph_plist != plist || graph_slab != S.slab) recapture();
       int* running = reinterpret_cast<int*>(host + kHost - 1);
       *running = ST_RUNNING;
       HIPCALL(hipMemcpyAsync(&ctl->status, running, sizeof(int), hipMemcpyHostToDevice, stream));
