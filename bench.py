@@ -106,7 +106,12 @@ def curve_error(fa, fb):
         return None
     peak = max(float(np.max(fb["reaction_force"][:n])), 1e-12)
     d = np.abs(fa["reaction_force"][:n] - fb["reaction_force"][:n]) / peak
+    # up to the peak force: before the lattice buckles, where the equilibrium
+    # path is unique; past it two solvers that round differently may settle
+    # on different post-buckling branches
+    kp = int(np.argmax(fb["reaction_force"][:n]))
     return {"frames": int(n), "max_rel_force": float(d.max()), "worst_frame": int(d.argmax()),
+            "peak_frame": kp, "max_rel_force_to_peak": float(d[:kp + 1].max()),
             "loading_work_rel": float(abs(fa["loading_work"][n - 1] - fb["loading_work"][n - 1])
                                       / max(abs(fb["loading_work"][n - 1]), 1e-12)),
             "permanent_set_abs": float(abs(fa["permanent_set"][n - 1] - fb["permanent_set"][n - 1]))}
@@ -156,6 +161,7 @@ def main():
     ap.add_argument("--line-search", type=int, default=0, choices=[0, 1])
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--save-frames", default="", help="write each leg's frames to <prefix>_<precision>.npy")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this solver has no multi-GPU path (pass --gpus 1)", file=sys.stderr)
@@ -202,11 +208,15 @@ def main():
     }
     inc_s = sim.increment_s
     sim.close()
+    if args.save_frames:
+        np.save(f"{args.save_frames}_{args.precision}.npy", fr)
     if not args.no_compare:
         other = mmc.MMC_FP64 if prec == mmc.MMC_MIXED else mmc.MMC_MIXED
         sim2, fr2, _, K2, dt2, _ = run_leg(cfg, other, args.warmup, args.steps, log)
         sim2.close()
         lbl = "fp64" if other == mmc.MMC_FP64 else "mixed"
+        if args.save_frames:
+            np.save(f"{args.save_frames}_{lbl}.npy", fr2)
         out[f"{lbl}_leg"] = {"value": round(K2 / dt2, 4), "ms_per_step": round(dt2 / K2 * 1e3, 3),
                              "lbfgs_iterations": int(fr2["iterations"][warm:].sum()),
                              "all_converged": bool(np.all(fr2["converged"] == 1))}

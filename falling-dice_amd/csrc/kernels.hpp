@@ -101,18 +101,19 @@ __device__ __forceinline__ void block_reduce(double (&v)[K], unsigned maxmask) {
   __syncthreads();
 }
 // Sum (or max) of P block partials laid out [K][P], evaluated identically by
-// every block that calls it (all threads participate; result broadcast).
-template <int K>
+// every block of NT threads that calls it (all threads participate; result
+// broadcast).
+template <int K, int NT = kBlock>
 __device__ __forceinline__ void reduce_partials(const double* part, int P, double (&out)[K], unsigned maxmask) {
   double v[K];
 #pragma unroll
   for (int k = 0; k < K; k++) {
     bool mx = (maxmask >> k) & 1;
     double a = mx ? -INFINITY : 0.0;
-    for (int i = threadIdx.x; i < P; i += kBlock) a = mx ? fmax(a, part[k * P + i]) : a + part[k * P + i];
+    for (int i = threadIdx.x; i < P; i += NT) a = mx ? fmax(a, part[k * P + i]) : a + part[k * P + i];
     v[k] = a;
   }
-  block_reduce<K>(v, maxmask);
+  block_reduce<K, NT>(v, maxmask);
   __shared__ double bc[K];
   if (threadIdx.x == 0)
 #pragma unroll

@@ -263,11 +263,15 @@ __device__ __forceinline__ bool adjacent(const Lat<T>& L, int u, int v) {
 template <class T>
 __device__ __forceinline__ bool excluded(const Lat<T>& L, int a0, int a1, int b0, int b1) {
   int ea[2] = {a0, a1}, eb[2] = {b0, b1};
+  // shared nodes first (no memory traffic), then the adjacency lists
+  for (int i = 0; i < 2; i++)
+    for (int j = 0; j < 2; j++)
+      if (ea[i] >= 0 && ea[i] == eb[j]) return true;
   for (int i = 0; i < 2; i++) {
     if (ea[i] < 0) continue;
     for (int j = 0; j < 2; j++) {
       if (eb[j] < 0) continue;
-      if (ea[i] == eb[j] || adjacent(L, ea[i], eb[j])) return true;
+      if (adjacent(L, ea[i], eb[j])) return true;
     }
   }
   return false;
@@ -333,11 +337,11 @@ __global__ __launch_bounds__(kBlock) void k_pairs(Lat<T> L, Scalars S, const dou
             double orr;
             int o0, o1;
             shape_geom(L, S, x, o, oa, ob, orr, o0, o1);
-            if (excluded(L, s0, s1, o0, o1)) continue;
             const bool low = s < o, ocap = o >= N;
             closest cp = low ? shape_closest(scap, sa, sb, ocap, oa, ob) : shape_closest(ocap, oa, ob, scap, sa, sb);
             double dist = norm(cp.pb - cp.pa);
             if (!(dist < sr + orr + S.margin + S.skin)) continue;
+            if (excluded(L, s0, s1, o0, o1)) continue;
             if (n < slab) plist[base + n] = o;
             n++;
           }
@@ -965,58 +969,6 @@ __device__ __forceinline__ void pair_dots_body(int N, size_t n6, const double* _
       for (int r = 0; r < 16 && c0 + r < K; r++) part[(c0 + r) * P + blockIdx.x] = v[r];
   }
 }
-// Element-parallel form for the device loop: one thread per vector element
-// (the 6N DOF planes are contiguous, so every vector is a flat array of n6),
-// ceil(n6 / kBlock) blocks; same partial layout [4*nold + 5][P].
-__device__ __forceinline__ void pair_dots_elem(size_t n6, const double* __restrict__ z, const double* __restrict__ zc,
-                                               const double* __restrict__ g, const double* __restrict__ gc,
-                                               double* __restrict__ Sh, double* __restrict__ Yh, const PairArgs& A,
-                                               double* __restrict__ part) {
-  const size_t e = size_t(blockIdx.x) * kBlock + threadIdx.x;
-  const int P = gridDim.x;
-  double sv = 0.0, yv = 0.0, gv = 0.0;
-  if (e < n6) {
-    gv = gc[e];
-    sv = zc[e] - z[e];
-    yv = gv - g[e];
-    Sh[n6 * A.t + e] = sv;
-    Yh[n6 * A.t + e] = yv;
-  }
-  constexpr int KM = 4 * kMaxHist + 5;
-  double acc[KM];
-#pragma unroll
-  for (int r = 0; r < KM; r++) acc[r] = 0.0;
-  if (e < n6) {
-#pragma unroll
-    for (int jj = 0; jj < kMaxHist; jj++) {
-      if (jj < A.nold) {
-        const double a = Sh[n6 * A.slot[jj] + e], b = Yh[n6 * A.slot[jj] + e];
-        acc[4 * jj + 0] = a * yv;
-        acc[4 * jj + 1] = b * yv;
-        acc[4 * jj + 2] = a * gv;
-        acc[4 * jj + 3] = b * gv;
-      }
-    }
-  }
-  const double w[5] = {sv * yv, sv * sv, yv * yv, sv * gv, yv * gv};
-#pragma unroll
-  for (int jj = 0; jj <= kMaxHist; jj++)
-    if (jj == A.nold)
-#pragma unroll
-      for (int c = 0; c < 5; c++) acc[4 * jj + c] = w[c];
-  const int K = 4 * A.nold + 5;
-#pragma unroll
-  for (int c0 = 0; c0 < KM; c0 += 16) {
-    if (c0 >= K) break;
-    double v[16];
-#pragma unroll
-    for (int r = 0; r < 16; r++) v[r] = (c0 + r < KM) ? acc[c0 + r] : 0.0;
-    block_reduce<16>(v, 0u);
-    if (threadIdx.x == 0)
-      for (int r = 0; r < 16 && c0 + r < K; r++) part[(c0 + r) * P + blockIdx.x] = v[r];
-  }
-}
-
 __global__ __launch_bounds__(kBlock) void k_pair_dots(int N, size_t n6, const double* __restrict__ z,
                                                       const double* __restrict__ zc, const double* __restrict__ g,
                                                       const double* __restrict__ gc, double* __restrict__ Sh,
@@ -1039,36 +991,90 @@ __global__ __launch_bounds__(kBlock) void k_sum_rows(const double* __restrict__ 
 // Device-resident minimiser, MC/src/sim.cpp:174-208.  A control block holds
 // the loop state; every kernel of one L-BFGS iteration is predicated on it, so
 // the iteration can be captured once as a hipGraph and replayed without host
-// round trips.  Phases of one iteration:
-//   DIRECTION  p = -H g (compact form), g.p; the descent check     k_dir, k_state
-//              runs in the state kernel of the same replay
-//   STEEPEST   p = -g when g.p >= 0 or after a failed line search  (same, mode 1)
-//   SEARCH     one Armijo trial per replay: state, beam, contact,
-//              gather, final (decision: accept, halve, or fail)
-//   PAIR       s, y and their inner products; history update,
-//              convergence test of the new iterate, next coefficients
+// round trips.  One replay runs five kernels:
+//   k_dir_g     p = -H g (compact form) and g.p         phase DIRECTION, or
+//               p = -g                                   phase STEEPEST
+//   k_state_g   descent check of a fresh direction (g.p >= 0 -> steepest),
+//               then the trial state z + alpha p         phases DIRECTION,
+//   k_elem_g    beams and contact shapes of the trial    STEEPEST or SEARCH
+//   k_gather_g  gradient, and the trial's curvature pair and its products
+//   k_decide_g  Armijo decision: accept (pair update, convergence test, next
+//               coefficients, phase DIRECTION), halve (phase SEARCH: the next
+//               replay is a backtrack), steepest retry, or failure
 // The single-block kernels own the control block.  One wide kernel writes to
 // it: block 0 of the state kernel records the descent-check outcome in two
 // fields (srch_eval, srch_dg) that the state kernel itself never reads; every
 // other field a wide kernel reads is written only by earlier kernels (kernel
 // boundaries order the two).
 // ---------------------------------------------------------------------------
-enum { PH_DIRECTION = 0, PH_STEEPEST = 2, PH_SEARCH = 4, PH_PAIR = 5 };
+enum { PH_DIRECTION = 0, PH_STEEPEST = 2, PH_SEARCH = 4 };
 enum { ST_RUNNING = 0, ST_CONVERGED = 1, ST_FAILED = 2, ST_ITERCAP = 3, ST_HOLD = 4 };
 
+constexpr int kMaxSlots = kMaxHist + 1;  // history slots of the device loop: m + 1 (one spare)
 struct Ctl {
   int status, phase, par, it, iterations, backtracks, evals, retry;
-  int bt, count, head, m, max_it, max_ls, rel_escape, ls_mode;
+  int bt, count, spare, m, max_it, max_ls, rel_escape, ls_mode;
   int wolfe, srch_eval;  // srch_eval: the state kernel of this replay evaluates a trial
   double srch_dg;        // g.p of a fresh direction, reduced by the state kernel
   double step, dg, top, armijo, min_step, tol_rms, tol_max, rel_tol, pen_tol, ts, skin_half, wolfe_eps;
   double cur[kRec], cand[kRec];
   DirCoef coef;
-  PairArgs pair;
-  double SY[kMaxHist][kMaxHist];  // by slot: s_i . y_j (i not newer than j)
-  double YY[kMaxHist][kMaxHist];  // by slot: y_i . y_j
-  double u[kMaxHist], v[kMaxHist];  // s_j . g and y_j . g of the current gradient
+  int order[kMaxHist];              // history slots, oldest first (count of them)
+  double SY[kMaxSlots][kMaxSlots];  // by slot: s_i . y_j (i not newer than j)
+  double YY[kMaxSlots][kMaxSlots];  // by slot: y_i . y_j
+  double u[kMaxSlots], v[kMaxSlots];  // s_j . g and y_j . g of the current gradient
 };
+
+// Device-loop history bookkeeping.  The loop keeps m + 1 slots: `order` lists
+// the count retained pairs oldest first, `spare` is a free slot.  Every trial
+// writes its (s, y) into the spare slot and computes its products with the
+// pairs that survive an acceptance (all of them, or all but the oldest when
+// the history is full), so that the products are ready when the trial is
+// accepted; a rejected trial's slot is simply overwritten by the next one.
+__host__ __device__ inline PairArgs pair_args_dev(const Ctl* c) {
+  PairArgs A{};
+  A.t = c->spare;
+  A.nold = 0;
+  for (int j = (c->count == c->m ? 1 : 0); j < c->count; j++) A.slot[A.nold++] = c->order[j];
+  return A;
+}
+
+// MC/src/sim.cpp:205-206 on the device loop's slots: accept the new pair if
+// s.y > 1e-12 sqrt(s.s y.y), else clear the history.  d holds the products in
+// the layout of pair_dots_body.
+__device__ inline void history_update_dev(Ctl* c, const PairArgs& A, const double* d) {
+  const double* nw = d + 4 * A.nold;
+  const double sy = nw[0], ss = nw[1], yy = nw[2];
+  if (!(sy > 1e-12 * sqrt(fmax(1e-30, ss * yy)))) {
+    c->count = 0;
+    return;
+  }
+  const int t = A.t;
+  for (int jj = 0; jj < A.nold; jj++) {
+    const int j = A.slot[jj];
+    c->SY[j][t] = d[4 * jj + 0];
+    c->YY[j][t] = c->YY[t][j] = d[4 * jj + 1];
+    c->u[j] = d[4 * jj + 2];
+    c->v[j] = d[4 * jj + 3];
+  }
+  c->SY[t][t] = sy;
+  c->YY[t][t] = yy;
+  c->u[t] = nw[3];
+  c->v[t] = nw[4];
+  const int evicted = c->count == c->m ? c->order[0] : -1;
+  for (int jj = 0; jj < A.nold; jj++) c->order[jj] = A.slot[jj];
+  c->order[A.nold] = t;
+  c->count = A.nold + 1;
+  if (evicted >= 0) {
+    c->spare = evicted;
+  } else {
+    unsigned used = 0;
+    for (int j = 0; j < c->count; j++) used |= 1u << c->order[j];
+    int f = 0;
+    while ((used >> f) & 1u) f++;
+    c->spare = f;
+  }
+}
 
 // Coefficients of H g = gamma g + sum_j a_j s_j + c_j y_j for the retained
 // history (oldest first from slot head): Byrd, Nocedal & Schnabel 1994, eq. 2.6:
@@ -1153,14 +1159,12 @@ __host__ __device__ inline PairArgs pair_slots(int m, int count, int head) {
   return A;
 }
 
-// compact_coefficients for one wavefront (all 64 lanes call it; lane l holds
-// history row l, oldest first).  The two triangular solves run by columns:
+// compact_coefficients for one wavefront on the device loop's slots (all 64
+// lanes call it; lane l holds history row l, oldest first).  The two triangular solves run by columns:
 // each step broadcasts one solved component and every lane updates its own
 // residual, so the dependent chain is k steps instead of k(k+1)/2.
-__device__ void compact_coefficients_wave(int m, int count, int head, const double (*SY)[kMaxHist],
-                                          const double (*YY)[kMaxHist], const double* u, const double* v,
-                                          DirCoef& C) {
-  const int k = count, l = threadIdx.x & (kWave - 1);
+__device__ void compact_coefficients_wave(const Ctl& c, DirCoef& C) {
+  const int k = c.count, l = threadIdx.x & (kWave - 1);
   if (k == 0) {
     if (l == 0) {
       C.k = 0;
@@ -1169,33 +1173,31 @@ __device__ void compact_coefficients_wave(int m, int count, int head, const doub
     return;
   }
   const bool on = l < k;
-  const int sl = on ? (head + l) % m : 0;
-  const int last = (head + k - 1) % m;
-  const double sy = SY[last][last], yy = YY[last][last];
+  const int sl = on ? c.order[l] : 0;
+  const int last = c.order[k - 1];
+  const double sy = c.SY[last][last], yy = c.YY[last][last];
   const double gamma = (sy > 1e-20 && yy > 1e-20) ? fmin(1e4, fmax(1e-7, sy / yy)) : 0.10;
-  const double dl = on ? SY[sl][sl] : 1.0;
+  const double dl = on ? c.SY[sl][sl] : 1.0;
   // w = R^{-1} u, R the upper triangle of S'Y (R_ij = s_i . y_j, i <= j)
-  double r = on ? u[sl] : 0.0, w = 0.0;
+  double r = on ? c.u[sl] : 0.0, w = 0.0;
   for (int i = k - 1; i >= 0; i--) {
     const double wi = __shfl(r / dl, i, kWave);
-    const int si = (head + i) % m;
     if (l == i) w = wi;
-    if (l < i) r -= SY[sl][si] * wi;
+    if (l < i) r -= c.SY[sl][c.order[i]] * wi;
   }
   // t = (D + gamma Y'Y) w - gamma v
-  double t = on ? SY[sl][sl] * w - gamma * v[sl] : 0.0;
+  double t = on ? c.SY[sl][sl] * w - gamma * c.v[sl] : 0.0;
   for (int j = 0; j < k; j++) {
     const double wj = __shfl(w, j, kWave);
-    if (on) t += gamma * YY[sl][(head + j) % m] * wj;
+    if (on) t += gamma * c.YY[sl][c.order[j]] * wj;
   }
   // a = R^{-T} t
   double a = 0.0;
   r = t;
   for (int i = 0; i < k; i++) {
     const double ai = __shfl(r / dl, i, kWave);
-    const int si = (head + i) % m;
     if (l == i) a = ai;
-    if (on && l > i) r -= SY[si][sl] * ai;
+    if (on && l > i) r -= c.SY[c.order[i]][sl] * ai;
   }
   if (on) {
     C.slot[l] = sl;
@@ -1222,9 +1224,6 @@ __device__ __forceinline__ void ctl_store(Ctl* C, const Ctl& sc) {
   for (int e = threadIdx.x; e < W; e += kBlock) reinterpret_cast<double*>(C)[e] = reinterpret_cast<const double*>(&sc)[e];
 }
 
-__device__ __forceinline__ bool ctl_gate(const Ctl* C, int phase) {
-  return C->status == ST_RUNNING && C->phase == phase;
-}
 // A replay evaluates a trial right after a fresh direction (phase DIRECTION or
 // STEEPEST: the descent check runs inside the state kernel) or on a backtrack
 // (phase SEARCH).
@@ -1314,34 +1313,49 @@ __global__ __launch_bounds__(kBlock) void k_elem_g(const Ctl* __restrict__ C, La
   }
 }
 
+// Gradient assembly of the trial (gather_body, one thread per node), then
+// (history on) the trial's curvature pair s = zc - z, y = gc - g written into
+// the spare history slot with all its products (partial rows 3.., layout of
+// pair_dots_body), computed by the thread that assembled the node's gradient.
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_gather_g(Lat<T> L, Scalars S, const Ctl* __restrict__ C,
                                                      const double* __restrict__ x, const double* __restrict__ rec,
                                                      const double* __restrict__ cf_sph,
                                                      const double* __restrict__ cf_cap,
                                                      const double* __restrict__ spart, int Ps,
+                                                     const double* __restrict__ z0, const double* __restrict__ z1,
                                                      double* __restrict__ g0, double* __restrict__ g1,
-                                                     const double* __restrict__ p, double* __restrict__ part) {
+                                                     const double* __restrict__ p, double* __restrict__ Sh,
+                                                     double* __restrict__ Yh, size_t n6, double* __restrict__ part) {
   if (!ctl_trial(C)) return;
   S.top = C->top;
-  gather_body<T>(L, S, x, rec, cf_sph, cf_cap, spart, Ps, C->par ? g0 : g1, p, part);
+  const int par = C->par;
+  double* gnew = par ? g0 : g1;
+  gather_body<T>(L, S, x, rec, cf_sph, cf_cap, spart, Ps, gnew, p, part);
+  if (C->m == 0) return;
+  const PairArgs A = pair_args_dev(C);
+  pair_dots_body(L.N, n6, par ? z1 : z0, par ? z0 : z1, par ? g1 : g0, gnew, Sh, Yh, A, part + 3 * size_t(gridDim.x));
 }
 
-// Candidate record and the Armijo decision, MC/src/sim.cpp:178-180,192-204.
-__global__ __launch_bounds__(kBlock) void k_final_g(Ctl* __restrict__ C, int N, double centering, double n_free,
-                                                    const double* __restrict__ spart, int Ps,
-                                                    const double* __restrict__ cpart, int Pc,
-                                                    const double* __restrict__ bpart, int Pb,
-                                                    const double* __restrict__ gpart, int Pg,
-                                                    int* __restrict__ flags) {
+// Candidate record and the Armijo decision (MC/src/sim.cpp:178-180,192-204);
+// on acceptance the curvature-pair update, the swap of current and candidate
+// buffers, the next convergence test and the next coefficients
+// (MC/src/sim.cpp:181-183,205-207).
+__global__ __launch_bounds__(kBlock) void k_decide_g(Ctl* __restrict__ C, int N, double centering, double n_free,
+                                                     const double* __restrict__ spart, int Ps,
+                                                     const double* __restrict__ cpart, int Pc,
+                                                     const double* __restrict__ bpart, int Pb,
+                                                     const double* __restrict__ gpart, int Pg,
+                                                     int* __restrict__ flags) {
   if (!ctl_trial_phase(C)) return;
   __shared__ Ctl sc;
+  __shared__ double d[4 * kMaxHist + 5];
+  __shared__ int accepted, need_coef;
   ctl_load(C, sc);
   if (sc.phase != PH_SEARCH) {
     if (!sc.srch_eval) {  // not a descent direction: steepest descent next
       if (threadIdx.x == 0) {
         sc.count = 0;
-        sc.head = 0;
         sc.phase = PH_STEEPEST;
       }
       ctl_store(C, sc);
@@ -1360,6 +1374,8 @@ __global__ __launch_bounds__(kBlock) void k_final_g(Ctl* __restrict__ C, int N, 
     Ctl* c = &sc;
     const double* r = c->cand;
     c->evals++;
+    accepted = 0;
+    need_coef = 0;
     if (r[11] >= c->skin_half) {  // contact list stale at this trial: rebuild, then redo it
       c->status = ST_HOLD;
     } else {
@@ -1373,8 +1389,7 @@ __global__ __launch_bounds__(kBlock) void k_final_g(Ctl* __restrict__ C, int N, 
       if (!ok && c->wolfe && r[9] != 0.0 && fabs(dE) <= c->wolfe_eps * fabs(c->cur[0]))
         ok = r[14] <= (2.0 * c->armijo - 1.0) * c->dg;
       if (ok) {
-        c->phase = PH_PAIR;
-        if (c->m > 0) c->pair = pair_slots(c->m, c->count, c->head);
+        accepted = 1;
       } else {
         c->step *= 0.5;
         c->bt++;
@@ -1382,7 +1397,6 @@ __global__ __launch_bounds__(kBlock) void k_final_g(Ctl* __restrict__ C, int N, 
           if (c->retry == 0) {  // steepest-descent retry
             c->retry = 1;
             c->count = 0;
-            c->head = 0;
             c->phase = PH_STEEPEST;
           } else {
             c->status = ST_FAILED;
@@ -1391,67 +1405,46 @@ __global__ __launch_bounds__(kBlock) void k_final_g(Ctl* __restrict__ C, int N, 
       }
     }
   }
-  ctl_store(C, sc);
-}
-
-__global__ __launch_bounds__(kBlock) void k_pair_g(const Ctl* __restrict__ C, int N, size_t n6,
-                                                   double* __restrict__ z0, double* __restrict__ z1,
-                                                   double* __restrict__ g0, double* __restrict__ g1,
-                                                   double* __restrict__ Sh, double* __restrict__ Yh,
-                                                   double* __restrict__ part) {
-  if (!ctl_gate(C, PH_PAIR) || C->m == 0) return;
-  const int par = C->par;
-  pair_dots_elem(n6, par ? z1 : z0, par ? z0 : z1, par ? g1 : g0, par ? g0 : g1, Sh, Yh, C->pair, part);
-}
-
-// history update, swap of current/candidate buffers, next iteration's
-// convergence test and coefficients (MC/src/sim.cpp:181-183,205-207).  The
-// pair products are reduced one wave per row; the small history matrices are
-// staged in LDS for the scalar algebra.
-__global__ __launch_bounds__(kBlock) void k_update_g(Ctl* __restrict__ C, const double* __restrict__ part, int P) {
-  if (!ctl_gate(C, PH_PAIR)) return;
-  __shared__ Ctl sc;
-  __shared__ double d[4 * kMaxHist + 5];
-  __shared__ int need_coef;
-  ctl_load(C, sc);
-  const int m = sc.m;
-  if (m > 0) {
-    // one wave per row (rows strided over the block's waves), fixed order
-    const int K = 4 * sc.pair.nold + 5, lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
-    for (int r = wv; r < K; r += kBlock / kWave) {
-      double a = 0.0;
-      for (int i = lane; i < P; i += kWave) a += part[size_t(r) * P + i];
-      a = wave_sum(a);
-      if (lane == 0) d[r] = a;
-    }
-  }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    Ctl* c = &sc;
-    if (m > 0) history_update(m, c->count, c->head, c->SY, c->YY, c->u, c->v, c->pair, d);
-    c->par ^= 1;
-    for (int k = 0; k < kRec; k++) c->cur[k] = c->cand[k];
-    c->it++;
-    c->retry = 0;
-    need_coef = 0;
-    if (c->it >= c->max_it) {
-      c->status = ST_ITERCAP;
-    } else {
-      c->iterations = c->it + 1;
-      const double* r = c->cur;
-      const double rel = r[6] / fmax(1e-6, c->ts * fmax(1.0, r[4]));
-      const bool ok = (r[6] <= c->tol_rms && r[7] <= c->tol_max) || (c->rel_escape && rel < c->rel_tol);
-      if (ok && r[5] <= c->pen_tol) {
-        c->status = ST_CONVERGED;
-      } else {
-        need_coef = 1;
-        c->phase = PH_DIRECTION;
+  if (accepted) {
+    const PairArgs A = pair_args_dev(&sc);
+    const int m = sc.m;
+    if (m > 0) {
+      // pair products: rows 3.. of the gather partials, one wave per row
+      const int K = 4 * A.nold + 5, lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
+      for (int r = wv; r < K; r += kBlock / kWave) {
+        double a = 0.0;
+        for (int i = lane; i < Pg; i += kWave) a += gpart[size_t(3 + r) * Pg + i];
+        a = wave_sum(a);
+        if (lane == 0) d[r] = a;
       }
     }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      Ctl* c = &sc;
+      if (m > 0) history_update_dev(c, A, d);
+      c->par ^= 1;
+      for (int k = 0; k < kRec; k++) c->cur[k] = c->cand[k];
+      c->it++;
+      c->retry = 0;
+      if (c->it >= c->max_it) {
+        c->status = ST_ITERCAP;
+      } else {
+        c->iterations = c->it + 1;
+        const double* r = c->cur;
+        const double rel = r[6] / fmax(1e-6, c->ts * fmax(1.0, r[4]));
+        const bool ok = (r[6] <= c->tol_rms && r[7] <= c->tol_max) || (c->rel_escape && rel < c->rel_tol);
+        if (ok && r[5] <= c->pen_tol) {
+          c->status = ST_CONVERGED;
+        } else {
+          need_coef = 1;
+          c->phase = PH_DIRECTION;
+        }
+      }
+    }
+    __syncthreads();
+    if (need_coef && threadIdx.x < kWave) compact_coefficients_wave(sc, sc.coef);
   }
-  __syncthreads();
-  if (need_coef && threadIdx.x < kWave)
-    compact_coefficients_wave(m, sc.count, sc.head, sc.SY, sc.YY, sc.u, sc.v, sc.coef);
   ctl_store(C, sc);
 }
 
@@ -1824,7 +1817,7 @@ void mmc_ctx::setup(const mmc_lattice* L, const mmc_config* c, int prec) {
   spart = dalloc<double>(9 * size_t(Ps));
   cpart = dalloc<double>(4 * size_t(Pc));
   bpart = dalloc<double>(4 * size_t(Pb));
-  gpart = dalloc<double>(3 * size_t(Ps));
+  gpart = dalloc<double>((3 + 4 * size_t(kMaxHist) + 5) * size_t(Ps));  // gradient rows, then pair rows
   Pe = blocks(6 * N);
   vpart = dalloc<double>((4 * size_t(kMaxHist) + 8) * size_t(std::max({Ps, Pe, (N + kDirNodes - 1) / kDirNodes})));
   stpart = dalloc<double>(8 * size_t(blocks(std::max(N, B))));
@@ -1838,7 +1831,7 @@ void mmc_ctx::setup(const mmc_lattice* L, const mmc_config* c, int prec) {
   gc = dalloc<double>(n6);
   zt = dalloc<double>(n6);
   zin = dalloc<double>(n6);
-  const size_t hm = size_t(std::max(1, cfg.history));
+  const size_t hm = size_t(cfg.history + 1);  // the device loop keeps one spare slot
   Sh = dalloc<double>(n6 * hm);
   Yh = dalloc<double>(n6 * hm);
   HIPCALL(hipHostMalloc((void**)&host, kHost * sizeof(double), hipHostMallocDefault));
@@ -2141,11 +2134,10 @@ void mmc_ctx::capture_iteration(Lat<T> Lt) {
                                           vpart, Pd, spart, ctl);
   k_elem_g<T><<<Pb + Pc, kBlock, 0, stream>>>(ctl, Lt, M, S, x, (const double*)R, rec, ubuf[0], ubuf[1], bpart, Pb,
                                               pcount, lref, cf_sph, cf_cap, cbuf[0], cbuf[1], cpart, Pc);
-  k_gather_g<T><<<Ps, kBlock, 0, stream>>>(Lt, S, ctl, x, rec, cf_sph, cf_cap, spart, Ps, gbuf[0], gbuf[1], p, gpart);
-  k_final_g<<<1, kBlock, 0, stream>>>(ctl, N, cfg.centering_stiffness, n_free, spart, Ps, cpart, Pc, bpart, Pb,
-                                      gpart, Ps, flags);
-  k_pair_g<<<Pe, kBlock, 0, stream>>>(ctl, N, n6, zbuf[0], zbuf[1], gbuf[0], gbuf[1], Sh, Yh, vpart);
-  k_update_g<<<1, kBlock, 0, stream>>>(ctl, vpart, Pe);
+  k_gather_g<T><<<Ps, kBlock, 0, stream>>>(Lt, S, ctl, x, rec, cf_sph, cf_cap, spart, Ps, zbuf[0], zbuf[1], gbuf[0],
+                                           gbuf[1], p, Sh, Yh, n6, gpart);
+  k_decide_g<<<1, kBlock, 0, stream>>>(ctl, N, cfg.centering_stiffness, n_free, spart, Ps, cpart, Pc, bpart, Pb,
+                                       gpart, Ps, flags);
   }
   HIPCALL(hipStreamEndCapture(stream, &graph));
   HIPCALL(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0));
