@@ -67,6 +67,27 @@ struct Scalars {
   int entry_cap;
 };
 
+// One lattice of a batch (a single lattice is a batch of one).  All lattices
+// share one concatenated SoA; an instance owns contiguous node, beam and shape
+// ranges (its shapes are its spheres, then its capsules), and contiguous block
+// ranges in every block table, so that every block partial belongs to exactly
+// one instance.
+struct Inst {
+  Scalars S;                  // per-instance constants (S.top is unused: the platen is in the control block)
+  Material M;
+  double n_free, mt, mr;      // free DOFs; direction clamps in z units
+  int n0, N, b0, B, s0, NS;   // node, beam, shape ranges
+  int pn0, pn1;               // blocks of kBlock nodes
+  int pb0, pb1;               // blocks of kBlock beams
+  int pc0, pc1;               // blocks of kBlock shapes
+  int pd0, pd1;               // blocks of kWave nodes (element-parallel vector kernels)
+};
+// One entry of a block table: the block covers items [first, first + count)
+// of instance inst.
+struct Blk {
+  int inst, first, count, pad;
+};
+
 // ---- block reduction helpers (wave64 butterfly + LDS across waves) ----
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -111,6 +132,30 @@ __device__ __forceinline__ void reduce_partials(const double* part, int P, doubl
     bool mx = (maxmask >> k) & 1;
     double a = mx ? -INFINITY : 0.0;
     for (int i = threadIdx.x; i < P; i += NT) a = mx ? fmax(a, part[k * P + i]) : a + part[k * P + i];
+    v[k] = a;
+  }
+  block_reduce<K, NT>(v, maxmask);
+  __shared__ double bc[K];
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int k = 0; k < K; k++) bc[k] = v[k];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < K; k++) out[k] = bc[k];
+  __syncthreads();
+}
+
+// Sum (or max) of the partial columns [c0, c1) of K rows with row stride P,
+// evaluated identically by every block of NT threads that calls it.
+template <int K, int NT = kBlock>
+__device__ __forceinline__ void reduce_columns(const double* part, int P, int c0, int c1, double (&out)[K],
+                                               unsigned maxmask) {
+  double v[K];
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    bool mx = (maxmask >> k) & 1;
+    double a = mx ? -INFINITY : 0.0;
+    for (int i = c0 + threadIdx.x; i < c1; i += NT) a = mx ? fmax(a, part[k * P + i]) : a + part[k * P + i];
     v[k] = a;
   }
   block_reduce<K, NT>(v, maxmask);

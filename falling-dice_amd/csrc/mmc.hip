@@ -34,8 +34,9 @@ using namespace mmc;
 
 namespace {
 
-__device__ __forceinline__ unsigned cell_hash(int i, int j, int k, unsigned mask) {
-  return ((unsigned)i * 73856093u ^ (unsigned)j * 19349663u ^ (unsigned)k * 83492791u) & mask;
+__device__ __forceinline__ unsigned cell_hash(int i, int j, int k, int inst, unsigned mask) {
+  return ((unsigned)i * 73856093u ^ (unsigned)j * 19349663u ^ (unsigned)k * 83492791u ^ (unsigned)inst * 2654435761u) &
+         mask;
 }
 __device__ __forceinline__ float round_down(double v) {
   float f = (float)v;
@@ -75,10 +76,10 @@ __device__ __forceinline__ void state_body(Lat<T> L, Scalars S, const double* __
                                                   const double* __restrict__ p, double step, double* __restrict__ zc,
                                                   double* __restrict__ x, double* __restrict__ R,
                                                   const double* __restrict__ xref, TermBufs nt,
-                                                  double* __restrict__ part) {
-  const int N = L.N, i = blockIdx.x * kBlock + threadIdx.x, P = gridDim.x;
+                                                  double* __restrict__ part, int i, bool on) {
+  const int N = L.N, P = gridDim.x;
   double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-  if (i < N) {
+  if (on) {
     const unsigned lk = L.lock[i];
     double q[6];
 #pragma unroll
@@ -131,12 +132,16 @@ __device__ __forceinline__ void state_body(Lat<T> L, Scalars S, const double* __
     for (int k = 0; k < 9; k++) part[k * P + blockIdx.x] = v[k];
 }
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_state(Lat<T> L, Scalars S, const double* __restrict__ z,
+__global__ __launch_bounds__(kBlock) void k_state(Lat<T> L, const Inst* __restrict__ I, const Blk* __restrict__ tn,
+                                                  const double* __restrict__ tops, const double* __restrict__ z,
                                                   const double* __restrict__ p, double step, double* __restrict__ zc,
                                                   double* __restrict__ x, double* __restrict__ R,
                                                   const double* __restrict__ xref, TermBufs nt,
                                                   double* __restrict__ part) {
-  state_body<T>(L, S, z, p, step, zc, x, R, xref, nt, part);
+  const Blk bk = tn[blockIdx.x];
+  Scalars S = I[bk.inst].S;
+  S.top = tops[bk.inst];
+  state_body<T>(L, S, z, p, step, zc, x, R, xref, nt, part, bk.first + threadIdx.x, int(threadIdx.x) < bk.count);
 }
 
 
@@ -155,17 +160,19 @@ __global__ __launch_bounds__(kBlock) void k_state(Lat<T> L, Scalars S, const dou
 // is sorted, so the partner order, and therefore every contact sum, does not
 // depend on the order of the atomic bucket fill.
 // ---------------------------------------------------------------------------
+// Shape s of instance in: its spheres (one per node) come first, then its
+// capsules (one per beam).
 template <class T>
-__device__ __forceinline__ void shape_geom(const Lat<T>& L, const Scalars& S, const double* x, int s, v3<double>& a,
+__device__ __forceinline__ void shape_geom(const Lat<T>& L, const Inst& in, const double* x, int s, v3<double>& a,
                                            v3<double>& b, double& r, int& n0, int& n1) {
-  const int N = L.N;
-  if (s < N) {
-    a = b = node_x<T>(x, N, s);
-    r = S.jr;
-    n0 = s;
+  const int N = L.N, ls = s - in.s0;
+  if (ls < in.N) {
+    n0 = in.n0 + ls;
+    a = b = node_x<T>(x, N, n0);
+    r = in.S.jr;
     n1 = -1;
   } else {
-    int k = s - N;
+    const int k = in.b0 + ls - in.N;
     n0 = L.ba[k];
     n1 = L.bb[k];
     a = node_x<T>(x, N, n0);
@@ -177,15 +184,18 @@ __device__ __forceinline__ void shape_geom(const Lat<T>& L, const Scalars& S, co
 constexpr int kMaxCellsPerShape = 512;
 
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_count(Lat<T> L, Scalars S, const double* __restrict__ x,
-                                                  float* __restrict__ box, unsigned* __restrict__ cnt,
-                                                  int* __restrict__ flags) {
-  const int NS = L.N + L.B, s = blockIdx.x * kBlock + threadIdx.x;
-  if (s >= NS) return;
+__global__ __launch_bounds__(kBlock) void k_count(Lat<T> L, const Inst* __restrict__ I, const Blk* __restrict__ tc,
+                                                  const double* __restrict__ x, float* __restrict__ box,
+                                                  unsigned* __restrict__ cnt, int* __restrict__ iflags) {
+  const Blk bk = tc[blockIdx.x];
+  if (int(threadIdx.x) >= bk.count) return;
+  const int s = bk.first + threadIdx.x;
+  const Inst& in = I[bk.inst];
+  const Scalars& S = in.S;
   v3<double> a, b;
   double r;
   int n0, n1;
-  shape_geom(L, S, x, s, a, b, r, n0, n1);
+  shape_geom(L, in, x, s, a, b, r, n0, n1);
   double pad = r + S.margin + 0.5 * S.skin;
   float lo[3] = {round_down(fmin(a.x, b.x) - pad), round_down(fmin(a.y, b.y) - pad), round_down(fmin(a.z, b.z) - pad)};
   float hi[3] = {round_up(fmax(a.x, b.x) + pad), round_up(fmax(a.y, b.y) + pad), round_up(fmax(a.z, b.z) + pad)};
@@ -204,18 +214,18 @@ __global__ __launch_bounds__(kBlock) void k_count(Lat<T> L, Scalars S, const dou
     // Left out of the grid.  A non-finite pose already makes the energy
     // non-finite; an absurdly large box is reported (flag 1) and the
     // evaluation is marked non-finite so that the line search rejects it.
-    if (ok) atomicOr(flags, 1);
+    if (ok) atomicOr(iflags + bk.inst, 1);
     reinterpret_cast<float4*>(box)[2 * s] = make_float4(INFINITY, INFINITY, INFINITY, 0.f);
     return;
   }
   for (int j = c0[1]; j <= c1[1]; j++)
     for (int k = c0[2]; k <= c1[2]; k++)
-      for (int i = c0[0]; i <= c1[0]; i++) atomicAdd(&cnt[cell_hash(i, j, k, S.table_mask)], 1u);
+      for (int i = c0[0]; i <= c1[0]; i++) atomicAdd(&cnt[cell_hash(i, j, k, bk.inst, S.table_mask)], 1u);
 }
 
 // Boxes are packed as two float4 (lo, hi) per shape: one cache line per lookup.
-__device__ __forceinline__ bool load_cells(const float* box, int NS, int s, double inv, float lo[3], float hi[3],
-                                           int c0[3], int c1[3]) {
+__device__ __forceinline__ bool load_cells(const float* box, int s, double inv, float lo[3], float hi[3], int c0[3],
+                                           int c1[3]) {
   const float4 l = reinterpret_cast<const float4*>(box)[2 * s], h = reinterpret_cast<const float4*>(box)[2 * s + 1];
   lo[0] = l.x;
   lo[1] = l.y;
@@ -232,18 +242,21 @@ __device__ __forceinline__ bool load_cells(const float* box, int NS, int s, doub
   return true;
 }
 
-__global__ __launch_bounds__(kBlock) void k_fill(int NS, Scalars S, const float* __restrict__ box,
-                                                 unsigned* __restrict__ cnt, const unsigned* __restrict__ start,
-                                                 int* __restrict__ entries, int* __restrict__ flags) {
-  const int s = blockIdx.x * kBlock + threadIdx.x;
-  if (s >= NS) return;
+__global__ __launch_bounds__(kBlock) void k_fill(const Inst* __restrict__ I, const Blk* __restrict__ tc,
+                                                 const float* __restrict__ box, unsigned* __restrict__ cnt,
+                                                 const unsigned* __restrict__ start, int* __restrict__ entries,
+                                                 int* __restrict__ flags) {
+  const Blk bk = tc[blockIdx.x];
+  if (int(threadIdx.x) >= bk.count) return;
+  const int s = bk.first + threadIdx.x;
+  const Scalars& S = I[bk.inst].S;
   float lo[3], hi[3];
   int c0[3], c1[3];
-  if (!load_cells(box, NS, s, S.inv_cell, lo, hi, c0, c1)) return;
+  if (!load_cells(box, s, S.inv_cell, lo, hi, c0, c1)) return;
   for (int j = c0[1]; j <= c1[1]; j++)
     for (int k = c0[2]; k <= c1[2]; k++)
       for (int i = c0[0]; i <= c1[0]; i++) {
-        unsigned h = cell_hash(i, j, k, S.table_mask);
+        unsigned h = cell_hash(i, j, k, bk.inst, S.table_mask);
         unsigned slot = atomicSub(&cnt[h], 1u) - 1u;  // also returns cnt[] to zero
         unsigned idx = start[h] + slot;
         if (idx < (unsigned)S.entry_cap)
@@ -299,31 +312,36 @@ __device__ __forceinline__ closest shape_closest(bool capA, v3<double> a0, v3<do
 // (cells in fixed order, sorted buckets); pcount gets the true count, and a
 // count above the slab raises flag 8 (the host widens the slab and rebuilds).
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_pairs(Lat<T> L, Scalars S, const double* __restrict__ x,
-                                                  const float* __restrict__ box, const unsigned* __restrict__ start,
-                                                  const int* __restrict__ entries, unsigned* __restrict__ pcount,
-                                                  int* __restrict__ plist, int* __restrict__ flags) {
-  const int N = L.N, NS = N + L.B, s = blockIdx.x * kBlock + threadIdx.x;
-  if (s >= NS) return;
+__global__ __launch_bounds__(kBlock) void k_pairs(Lat<T> L, const Inst* __restrict__ I, const Blk* __restrict__ tc,
+                                                  int slab_in, const int* __restrict__ sinst,
+                                                  const double* __restrict__ x, const float* __restrict__ box,
+                                                  const unsigned* __restrict__ start, const int* __restrict__ entries,
+                                                  unsigned* __restrict__ pcount, int* __restrict__ plist,
+                                                  int* __restrict__ flags) {
+  const Blk bk = tc[blockIdx.x];
+  if (int(threadIdx.x) >= bk.count) return;
+  const int s = bk.first + threadIdx.x;
+  const Inst& in = I[bk.inst];
+  const Scalars& S = in.S;
   float lo[3], hi[3];
   int c0[3], c1[3];
-  const unsigned slab = unsigned(S.slab);
+  const unsigned slab = unsigned(slab_in);
   const size_t base = size_t(s) * slab;
   unsigned n = 0;
-  if (load_cells(box, NS, s, S.inv_cell, lo, hi, c0, c1)) {
+  if (load_cells(box, s, S.inv_cell, lo, hi, c0, c1)) {
     v3<double> sa, sb;
     double sr;
     int s0, s1;
-    shape_geom(L, S, x, s, sa, sb, sr, s0, s1);
-    const bool scap = s >= N;
+    shape_geom(L, in, x, s, sa, sb, sr, s0, s1);
+    const bool scap = s - in.s0 >= in.N;
     for (int cj = c0[1]; cj <= c1[1]; cj++)
       for (int ck = c0[2]; ck <= c1[2]; ck++)
         for (int ci = c0[0]; ci <= c1[0]; ci++) {
-          unsigned h = cell_hash(ci, cj, ck, S.table_mask);
+          unsigned h = cell_hash(ci, cj, ck, bk.inst, S.table_mask);
           unsigned e0 = start[h], e1 = min(start[h + 1], (unsigned)S.entry_cap);
           for (unsigned e = e0; e < e1; e++) {
             int o = entries[e];
-            if (o == s) continue;
+            if (o == s || sinst[o] != bk.inst) continue;
             const float4 ol = reinterpret_cast<const float4*>(box)[2 * o];
             const float4 oh = reinterpret_cast<const float4*>(box)[2 * o + 1];
             const float olo[3] = {ol.x, ol.y, ol.z}, ohi[3] = {oh.x, oh.y, oh.z};
@@ -336,8 +354,8 @@ __global__ __launch_bounds__(kBlock) void k_pairs(Lat<T> L, Scalars S, const dou
             v3<double> oa, ob;
             double orr;
             int o0, o1;
-            shape_geom(L, S, x, o, oa, ob, orr, o0, o1);
-            const bool low = s < o, ocap = o >= N;
+            shape_geom(L, in, x, o, oa, ob, orr, o0, o1);
+            const bool low = s < o, ocap = o - in.s0 >= in.N;
             closest cp = low ? shape_closest(scap, sa, sb, ocap, oa, ob) : shape_closest(ocap, oa, ob, scap, sa, sb);
             double dist = norm(cp.pb - cp.pa);
             if (!(dist < sr + orr + S.margin + S.skin)) continue;
@@ -373,34 +391,38 @@ __global__ __launch_bounds__(kBlock) void k_pairs(Lat<T> L, Scalars S, const dou
 // accumulates its own force; each pair's energy is counted by its lower shape.
 // partials [4][P]: E_contact, max_pen, contacts, sum of per-shape energy changes
 template <class T>
-__device__ __forceinline__ void contact_body(Lat<T> L, Scalars S, const double* __restrict__ x,
+__device__ __forceinline__ void contact_body(Lat<T> L, const Inst& in, const double* __restrict__ x,
                                                     const unsigned* __restrict__ pcount, const int* __restrict__ plist,
-                                                    double* __restrict__ cf_sph, double* __restrict__ cf_cap,
-                                                    TermBufs ct, double* __restrict__ part, int blk, int P) {
-  const int N = L.N, B = L.B, NS = N + B, s = blk * kBlock + threadIdx.x;
+                                                    int slab, double* __restrict__ cf_sph, double* __restrict__ cf_cap,
+                                                    TermBufs ct, double* __restrict__ part, int s, bool on, int blk,
+                                                    int P) {
+  const int N = L.N, B = L.B;
+  const Scalars& S = in.S;
   double v[4] = {0, 0, 0, 0};
-  if (s < NS) {
+  if (on) {
     v3<double> f0{0, 0, 0}, f1{0, 0, 0};
-    const size_t e0 = size_t(s) * S.slab, e1 = e0 + min(pcount[s], unsigned(S.slab));
+    const size_t e0 = size_t(s) * slab, e1 = e0 + min(pcount[s], unsigned(slab));
+    const bool scap = s - in.s0 >= in.N;
     if (e1 > e0) {
       v3<double> sa, sb;
       double sr;
       int s0, s1;
-      shape_geom(L, S, x, s, sa, sb, sr, s0, s1);
-      const bool scap = s >= N;
+      shape_geom(L, in, x, s, sa, sb, sr, s0, s1);
       for (size_t e = e0; e < e1; e++) {
         const int o = plist[e];
         v3<double> oa, ob;
         double orr;
         int o0, o1;
-        shape_geom(L, S, x, o, oa, ob, orr, o0, o1);
-        const bool low = s < o, ocap = o >= N;
+        shape_geom(L, in, x, o, oa, ob, orr, o0, o1);
+        const bool low = s < o, ocap = o - in.s0 >= in.N;
         closest cp = low ? shape_closest(scap, sa, sb, ocap, oa, ob) : shape_closest(ocap, oa, ob, scap, sa, sb);
         v3<double> delta = cp.pb - cp.pa;
         double dist = norm(delta);
         double target = (low ? sr + orr : orr + sr) + S.margin;
         if (dist >= target) continue;
-        int ia = low ? s : o, ib = low ? o : s;
+        // instance-local shape indices (the fallback normal of coincident
+        // shapes depends on them)
+        int ia = (low ? s : o) - in.s0, ib = (low ? o : s) - in.s0;
         v3<double> n = dist > 1e-12 ? over(delta, dist) : unit(v3<double>{0.37 + 0.001 * ia, 0.73, -0.41 + 0.001 * ib});
         double pen = target - dist, mag = S.kc * pen;
         v3<double> fv = low ? n * mag : n * (-mag);
@@ -418,8 +440,8 @@ __device__ __forceinline__ void contact_body(Lat<T> L, Scalars S, const double* 
         }
       }
     }
-    if (s >= N) {
-      int k = s - N;
+    if (scap) {
+      const int k = in.b0 + (s - in.s0 - in.N);
       cf_cap[k] = f0.x;
       cf_cap[B + k] = f0.y;
       cf_cap[2 * B + k] = f0.z;
@@ -427,9 +449,10 @@ __device__ __forceinline__ void contact_body(Lat<T> L, Scalars S, const double* 
       cf_cap[4 * B + k] = f1.y;
       cf_cap[5 * B + k] = f1.z;
     } else {
-      cf_sph[s] = f0.x;
-      cf_sph[N + s] = f0.y;
-      cf_sph[2 * N + s] = f0.z;
+      const int i = in.n0 + (s - in.s0);
+      cf_sph[i] = f0.x;
+      cf_sph[N + i] = f0.y;
+      cf_sph[2 * N + i] = f0.z;
     }
     if (ct.out) ct.out[s] = v[0];
     if (ct.ref) v[3] = v[0] - ct.ref[s];
@@ -440,11 +463,14 @@ __device__ __forceinline__ void contact_body(Lat<T> L, Scalars S, const double* 
     for (int k = 0; k < 4; k++) part[k * P + blk] = v[k];
 }
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_contact(Lat<T> L, Scalars S, const double* __restrict__ x,
-                                                    const unsigned* __restrict__ pcount, const int* __restrict__ plist,
+__global__ __launch_bounds__(kBlock) void k_contact(Lat<T> L, const Inst* __restrict__ I, const Blk* __restrict__ tc,
+                                                    const double* __restrict__ x, const unsigned* __restrict__ pcount,
+                                                    const int* __restrict__ plist, int slab,
                                                     double* __restrict__ cf_sph, double* __restrict__ cf_cap,
                                                     TermBufs ct, double* __restrict__ part) {
-  contact_body<T>(L, S, x, pcount, plist, cf_sph, cf_cap, ct, part, blockIdx.x, gridDim.x);
+  const Blk bk = tc[blockIdx.x];
+  contact_body<T>(L, I[bk.inst], x, pcount, plist, slab, cf_sph, cf_cap, ct, part, bk.first + threadIdx.x,
+                  int(threadIdx.x) < bk.count, blockIdx.x, gridDim.x);
 }
 
 
@@ -457,10 +483,10 @@ template <class T>
 __device__ __forceinline__ void beam_body(Lat<T> L, Material M, const double* __restrict__ x,
                                                  const double* __restrict__ R, double* __restrict__ rec,
                                                  double* __restrict__ trial, TermBufs bt,
-                                                 double* __restrict__ part, int blk, int P) {
-  const int N = L.N, B = L.B, k = blk * kBlock + threadIdx.x;
+                                                 double* __restrict__ part, int k, bool on, int blk, int P) {
+  const int N = L.N, B = L.B;
   double v[4] = {0, 0, 0, 0};
-  if (k < B) {
+  if (on) {
     const int ia = L.ba[k], ib = L.bb[k];
     v3<double> d = node_x<T>(x, N, ib) - node_x<T>(x, N, ia);
     double Ln = norm(d);
@@ -559,10 +585,13 @@ __device__ __forceinline__ void beam_body(Lat<T> L, Material M, const double* __
     for (int c = 0; c < 4; c++) part[c * P + blk] = v[c];
 }
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_beam(Lat<T> L, Material M, const double* __restrict__ x,
-                                                 const double* __restrict__ R, double* __restrict__ rec,
-                                                 double* __restrict__ trial, TermBufs bt, double* __restrict__ part) {
-  beam_body<T>(L, M, x, R, rec, trial, bt, part, blockIdx.x, gridDim.x);
+__global__ __launch_bounds__(kBlock) void k_beam(Lat<T> L, const Inst* __restrict__ I, const Blk* __restrict__ tb,
+                                                 const double* __restrict__ x, const double* __restrict__ R,
+                                                 double* __restrict__ rec, double* __restrict__ trial, TermBufs bt,
+                                                 double* __restrict__ part) {
+  const Blk bk = tb[blockIdx.x];
+  beam_body<T>(L, I[bk.inst].M, x, R, rec, trial, bt, part, bk.first + threadIdx.x, int(threadIdx.x) < bk.count,
+               blockIdx.x, gridDim.x);
 }
 
 
@@ -572,22 +601,24 @@ __global__ __launch_bounds__(kBlock) void k_beam(Lat<T> L, Material M, const dou
 // search direction's directional derivative at this point, when p is given).
 // ---------------------------------------------------------------------------
 template <class T>
-__device__ __forceinline__ void gather_body(Lat<T> L, Scalars S, const double* __restrict__ x,
+__device__ __forceinline__ void gather_body(Lat<T> L, const Inst& in, double top, const double* __restrict__ x,
                                                    const double* __restrict__ rec, const double* __restrict__ cf_sph,
                                                    const double* __restrict__ cf_cap,
                                                    const double* __restrict__ spart, int Ps,
                                                    double* __restrict__ g, const double* __restrict__ p,
-                                                   double* __restrict__ part) {
-  const int N = L.N, B = L.B, i = blockIdx.x * kBlock + threadIdx.x, P = gridDim.x;
+                                                   double* __restrict__ part, int i, bool on) {
+  const int N = L.N, B = L.B, P = gridDim.x;
+  const Scalars& S = in.S;
+  const int Ni = in.N;  // this lattice's node count (N is the plane stride)
   double cen[2];
   {
     double t[2];
-    reduce_partials<2>(spart + 5 * Ps, Ps, t, 0u);
-    cen[0] = t[0] * (1.0 / N);
-    cen[1] = t[1] * (1.0 / N);
+    reduce_columns<2>(spart + 5 * Ps, Ps, in.pn0, in.pn1, t, 0u);
+    cen[0] = t[0] * (1.0 / Ni);
+    cen[1] = t[1] * (1.0 / Ni);
   }
   double v[3] = {0, 0, 0};
-  if (i < N) {
+  if (on) {
     v3<double> gx{0, 0, 0}, gr{0, 0, 0};
     for (int e = L.inc_off[i]; e < L.inc_off[i + 1]; e++) {
       const int code = L.inc[e], k = code >> 1, side = code & 1;
@@ -600,12 +631,12 @@ __device__ __forceinline__ void gather_body(Lat<T> L, Scalars S, const double* _
     }
     gx = gx + v3<double>{cf_sph[i], cf_sph[N + i], cf_sph[2 * N + i]};
     gx.y += L.mass[i] * S.gravity;
-    gx.x += S.centering * cen[0] / N;
-    gx.z += S.centering * cen[1] / N;
+    gx.x += S.centering * cen[0] / Ni;
+    gx.z += S.centering * cen[1] / Ni;
     const double y = x[N + i];
     double pb = S.jr - y;
     if (pb > 0) gx.y -= S.kc * pb;
-    double pt = y + S.jr - S.top;
+    double pt = y + S.jr - top;
     if (pt > 0) gx.y += S.kc * pt;
     double gv[6] = {S.ts * gx.x, S.ts * gx.y, S.ts * gx.z, gr.x, gr.y, gr.z};
     const unsigned lk = L.lock[i];
@@ -628,13 +659,16 @@ __device__ __forceinline__ void gather_body(Lat<T> L, Scalars S, const double* _
   }
 }
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_gather(Lat<T> L, Scalars S, const double* __restrict__ x,
+__global__ __launch_bounds__(kBlock) void k_gather(Lat<T> L, const Inst* __restrict__ I, const Blk* __restrict__ tn,
+                                                   const double* __restrict__ tops, const double* __restrict__ x,
                                                    const double* __restrict__ rec, const double* __restrict__ cf_sph,
                                                    const double* __restrict__ cf_cap,
                                                    const double* __restrict__ spart, int Ps,
                                                    double* __restrict__ g, const double* __restrict__ p,
                                                    double* __restrict__ part) {
-  gather_body<T>(L, S, x, rec, cf_sph, cf_cap, spart, Ps, g, p, part);
+  const Blk bk = tn[blockIdx.x];
+  gather_body<T>(L, I[bk.inst], tops[bk.inst], x, rec, cf_sph, cf_cap, spart, Ps, g, p, part, bk.first + threadIdx.x,
+                 int(threadIdx.x) < bk.count);
 }
 
 
@@ -645,13 +679,16 @@ __global__ __launch_bounds__(kBlock) void k_gather(Lat<T> L, Scalars S, const do
 // 13 centering energy, 14 g.p
 constexpr int kRec = 16;
 
-__device__ __forceinline__ void final_body(int N, double centering, double n_free,
-                                                  const double* __restrict__ spart, int Ps,
+// Reduces the partial columns of instance in (node blocks of spart and gpart,
+// shape blocks of cpart, beam blocks of bpart) into its record.
+__device__ __forceinline__ void final_body(const Inst& in, const double* __restrict__ spart, int Ps,
                                                   const double* __restrict__ cpart, int Pc,
                                                   const double* __restrict__ bpart, int Pb,
                                                   const double* __restrict__ gpart, int Pg,
-                                                  int* __restrict__ flags, const double* __restrict__ refrec,
+                                                  int* __restrict__ iflag, const double* __restrict__ refrec,
                                                   double* __restrict__ out) {
+  const int N = in.N;
+  const double centering = in.S.centering, n_free = in.n_free;
   // 20 partial rows in one pass: every thread accumulates its share of every
   // row (independent loads), then one block reduction (fixed order)
   constexpr int R = 20;
@@ -659,19 +696,19 @@ __device__ __forceinline__ void final_body(int N, double centering, double n_fre
   const unsigned maxmask = (1u << 3) | (1u << 7) | (1u << 10) | (1u << 18);
 #pragma unroll
   for (int r = 0; r < R; r++) acc[r] = ((maxmask >> r) & 1u) ? -INFINITY : 0.0;
-  for (int i = threadIdx.x; i < Ps; i += kBlock) {
+  for (int i = in.pn0 + threadIdx.x; i < in.pn1; i += kBlock) {
 #pragma unroll
     for (int r = 0; r < 9; r++) acc[r] = (r == 3 || r == 7) ? fmax(acc[r], spart[r * Ps + i]) : acc[r] + spart[r * Ps + i];
   }
-  for (int i = threadIdx.x; i < Pc; i += kBlock) {
+  for (int i = in.pc0 + threadIdx.x; i < in.pc1; i += kBlock) {
 #pragma unroll
     for (int r = 0; r < 4; r++) acc[9 + r] = r == 1 ? fmax(acc[9 + r], cpart[r * Pc + i]) : acc[9 + r] + cpart[r * Pc + i];
   }
-  for (int i = threadIdx.x; i < Pb; i += kBlock) {
+  for (int i = in.pb0 + threadIdx.x; i < in.pb1; i += kBlock) {
 #pragma unroll
     for (int r = 0; r < 4; r++) acc[13 + r] += bpart[r * Pb + i];
   }
-  for (int i = threadIdx.x; i < Pg; i += kBlock) {
+  for (int i = in.pn0 + threadIdx.x; i < in.pn1; i += kBlock) {
     acc[17] += gpart[i];
     acc[18] = fmax(acc[18], gpart[Pg + i]);
     acc[19] += gpart[2 * Pg + i];
@@ -687,8 +724,8 @@ __device__ __forceinline__ void final_body(int N, double centering, double n_fre
   const double* b = rows + 13;
   const double* g = rows + 17;
   if (threadIdx.x == 0) {
-    const int fl = flags[0];
-    flags[0] = 0;
+    const int fl = iflag[0];
+    iflag[0] = 0;
     double cx = s[5] * (1.0 / N), cz = s[6] * (1.0 / N);
     double ecen = 0.5 * centering * (cx * cx + cz * cz);
     double energy = b[0] + s[0] + ecen + c[0] + s[1];
@@ -711,14 +748,16 @@ __device__ __forceinline__ void final_body(int N, double centering, double n_fre
     out[15] = 0.0;
   }
 }
-__global__ __launch_bounds__(kBlock) void k_final(int N, double centering, double n_free,
-                                                  const double* __restrict__ spart, int Ps,
+// one block per instance; records at out + inst * kRec
+__global__ __launch_bounds__(kBlock) void k_final(const Inst* __restrict__ I, const double* __restrict__ spart, int Ps,
                                                   const double* __restrict__ cpart, int Pc,
                                                   const double* __restrict__ bpart, int Pb,
                                                   const double* __restrict__ gpart, int Pg,
-                                                  int* __restrict__ flags, const double* __restrict__ refrec,
+                                                  int* __restrict__ iflags, const double* __restrict__ refrec,
                                                   double* __restrict__ out) {
-  final_body(N, centering, n_free, spart, Ps, cpart, Pc, bpart, Pb, gpart, Pg, flags, refrec, out);
+  const int b = blockIdx.x;
+  final_body(I[b], spart, Ps, cpart, Pc, bpart, Pb, gpart, Pg, iflags + b, refrec ? refrec + b * kRec : nullptr,
+             out + b * kRec);
 }
 
 
@@ -726,11 +765,15 @@ __global__ __launch_bounds__(kBlock) void k_final(int N, double centering, doubl
 // beamPotential, MC/src/sim.cpp:468-509 (objectivity check)
 // ---------------------------------------------------------------------------
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_beam_potential(Lat<T> L, Material M, const double* __restrict__ x,
+__global__ __launch_bounds__(kBlock) void k_beam_potential(Lat<T> L, const Inst* __restrict__ I,
+                                                           const Blk* __restrict__ tb, const double* __restrict__ x,
                                                            const double* __restrict__ R, double* __restrict__ part) {
-  const int N = L.N, B = L.B, k = blockIdx.x * kBlock + threadIdx.x;
+  const int N = L.N, B = L.B;
+  const Blk bk = tb[blockIdx.x];
+  const int k = bk.first + threadIdx.x;
+  const Material M = I[bk.inst].M;
   double v[1] = {0};
-  if (k < B) {
+  if (int(threadIdx.x) < bk.count) {
     const int ia = L.ba[k], ib = L.bb[k];
     v3<double> d = node_x<T>(x, N, ib) - node_x<T>(x, N, ia);
     double Ln = fmax(1e-12, norm(d));
@@ -857,11 +900,11 @@ __device__ __forceinline__ void direction_elem(int N, size_t n6, const double* _
                                                const double* __restrict__ Sh, const double* __restrict__ Yh,
                                                const DirCoef& C, double* __restrict__ p,
                                                const uint8_t* __restrict__ lock, double mt, double mr,
-                                               double* __restrict__ part) {
+                                               double* __restrict__ part, int first, int count) {
   __shared__ double qs[6][kDirNodes];
   const int d = threadIdx.x / kDirNodes, l = threadIdx.x % kDirNodes;
-  const int i = blockIdx.x * kDirNodes + l;
-  const bool on = i < N;
+  const int i = first + l;
+  const bool on = l < count;
   const size_t e = size_t(d) * N + i;
   const bool locked = on && ((lock[i] >> d) & 1u);
   double gv = 0.0, q = 0.0;
@@ -904,10 +947,11 @@ __global__ __launch_bounds__(kBlock) void k_direction(int N, size_t n6, const do
 __device__ __forceinline__ void pair_dots_body(int N, size_t n6, const double* __restrict__ z,
                                                       const double* __restrict__ zc, const double* __restrict__ g,
                                                       const double* __restrict__ gc, double* __restrict__ Sh,
-                                                      double* __restrict__ Yh, const PairArgs& A, double* __restrict__ part) {
-  const int i = blockIdx.x * kBlock + threadIdx.x, P = gridDim.x;
+                                                      double* __restrict__ Yh, const PairArgs& A, double* __restrict__ part,
+                                                      int i, bool on) {
+  const int P = gridDim.x;
   double sv[6] = {0, 0, 0, 0, 0, 0}, yv[6] = {0, 0, 0, 0, 0, 0}, gv[6] = {0, 0, 0, 0, 0, 0};
-  if (i < N) {
+  if (on) {
     double* st = Sh + n6 * A.t;
     double* yt = Yh + n6 * A.t;
 #pragma unroll
@@ -925,7 +969,7 @@ __device__ __forceinline__ void pair_dots_body(int N, size_t n6, const double* _
   double acc[KM];
 #pragma unroll
   for (int r = 0; r < KM; r++) acc[r] = 0.0;
-  if (i < N) {
+  if (on) {
 #pragma unroll
     for (int jj = 0; jj < kMaxHist; jj++) {
       if (jj < A.nold) {
@@ -973,7 +1017,8 @@ __global__ __launch_bounds__(kBlock) void k_pair_dots(int N, size_t n6, const do
                                                       const double* __restrict__ zc, const double* __restrict__ g,
                                                       const double* __restrict__ gc, double* __restrict__ Sh,
                                                       double* __restrict__ Yh, PairArgs A, double* __restrict__ part) {
-  pair_dots_body(N, n6, z, zc, g, gc, Sh, Yh, A, part);
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  pair_dots_body(N, n6, z, zc, g, gc, Sh, Yh, A, part, i, i < N);
 }
 
 
@@ -1235,52 +1280,63 @@ __device__ __forceinline__ bool ctl_trial(const Ctl* C) { return ctl_trial_phase
 
 // L-BFGS direction from the control block's coefficients (phase DIRECTION)
 // or -g (phase STEEPEST)
-__global__ __launch_bounds__(kDirThreads) void k_dir_g(const Ctl* __restrict__ C, int N, size_t n6,
-                                                  const double* __restrict__ g0, const double* __restrict__ g1,
-                                                  const double* __restrict__ Sh, const double* __restrict__ Yh,
-                                                  double* __restrict__ p, const uint8_t* __restrict__ lock, double mt,
-                                                  double mr, double* __restrict__ part) {
+__global__ __launch_bounds__(kDirThreads) void k_dir_g(const Ctl* __restrict__ Cs, const Inst* __restrict__ I,
+                                                       const Blk* __restrict__ td, int N, size_t n6,
+                                                       const double* __restrict__ g0, const double* __restrict__ g1,
+                                                       const double* __restrict__ Sh, const double* __restrict__ Yh,
+                                                       double* __restrict__ p, const uint8_t* __restrict__ lock,
+                                                       double* __restrict__ part) {
+  const Blk bk = td[blockIdx.x];
+  const Ctl* C = Cs + bk.inst;
   if (C->status != ST_RUNNING || (C->phase != PH_DIRECTION && C->phase != PH_STEEPEST)) return;
+  const Inst& in = I[bk.inst];
   const double* g = C->par ? g1 : g0;
   if (C->phase == PH_STEEPEST) {
     DirCoef D{};
     D.cg = 1.0;
-    direction_elem(N, n6, g, Sh, Yh, D, p, lock, mt, mr, part);
+    direction_elem(N, n6, g, Sh, Yh, D, p, lock, in.mt, in.mr, part, bk.first, bk.count);
   } else {
-    direction_elem(N, n6, g, Sh, Yh, C->coef, p, lock, mt, mr, part);
+    direction_elem(N, n6, g, Sh, Yh, C->coef, p, lock, in.mt, in.mr, part, bk.first, bk.count);
   }
 }
 
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_state_g(Lat<T> L, Scalars S, const Ctl* C,
-                                                    double* __restrict__ z0, double* __restrict__ z1,
+__global__ __launch_bounds__(kBlock) void k_state_g(Lat<T> L, const Inst* __restrict__ I, const Blk* __restrict__ tn,
+                                                    const Ctl* Cs, double* __restrict__ z0, double* __restrict__ z1,
                                                     const double* __restrict__ p, double* __restrict__ x,
                                                     double* __restrict__ R, const double* __restrict__ xref,
                                                     double* __restrict__ nb0, double* __restrict__ nb1,
                                                     const double* __restrict__ dpart, int Pd,
                                                     double* __restrict__ part, Ctl* Cw) {
+  const Blk bk = tn[blockIdx.x];
+  const Ctl* C = Cs + bk.inst;
   if (!ctl_trial_phase(C)) return;
+  const Inst& in = I[bk.inst];
+  const bool lead = int(blockIdx.x) == in.pn0 && threadIdx.x == 0;  // the instance's first block
   double step = C->step;
   if (C->phase != PH_SEARCH) {
     // descent check of the fresh direction, MC/src/sim.cpp:191: every block
-    // reduces the same partials in the same order; block 0 records the outcome
-    // for the later kernels of this replay (fields this kernel never reads)
+    // of the instance reduces the same partials in the same order; its first
+    // block records the outcome for the later kernels of this replay (fields
+    // this kernel never reads)
     double dg[1];
-    reduce_partials<1>(dpart, Pd, dg, 0u);
+    reduce_columns<1>(dpart, Pd, in.pd0, in.pd1, dg, 0u);
     const bool eval = !(C->phase == PH_DIRECTION && !(dg[0] < 0));
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-      Cw->srch_eval = eval;
-      Cw->srch_dg = dg[0];
+    if (lead) {
+      Cw[bk.inst].srch_eval = eval;
+      Cw[bk.inst].srch_dg = dg[0];
     }
     if (!eval) return;
     step = 1.0;
-  } else if (blockIdx.x == 0 && threadIdx.x == 0) {
-    Cw->srch_eval = 1;
+  } else if (lead) {
+    Cw[bk.inst].srch_eval = 1;
   }
+  Scalars S = in.S;
   S.top = C->top;
   const int par = C->par;
   TermBufs nt{par ? nb1 : nb0, par ? nb0 : nb1};
-  state_body<T>(L, S, par ? z1 : z0, p, step, par ? z0 : z1, x, R, xref, nt, part);
+  state_body<T>(L, S, par ? z1 : z0, p, step, par ? z0 : z1, x, R, xref, nt, part, bk.first + threadIdx.x,
+                int(threadIdx.x) < bk.count);
 }
 // Location of the Verlet pair list, read by the graph kernels at run time so
 // that a list reallocated with a wider slab needs no new graph.
@@ -1293,7 +1349,8 @@ struct ListRef {
 // evaluate beams, blocks [Pb, Pb + Pc) contact shapes (independent work; one
 // launch gap fewer per trial)
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_elem_g(const Ctl* __restrict__ C, Lat<T> L, Material M, Scalars S,
+__global__ __launch_bounds__(kBlock) void k_elem_g(const Ctl* __restrict__ Cs, Lat<T> L, const Inst* __restrict__ I,
+                                                   const Blk* __restrict__ tb, const Blk* __restrict__ tc,
                                                    const double* __restrict__ x, const double* __restrict__ R,
                                                    double* __restrict__ rec, double* __restrict__ ub0,
                                                    double* __restrict__ ub1, double* __restrict__ bpart, int Pb,
@@ -1301,15 +1358,20 @@ __global__ __launch_bounds__(kBlock) void k_elem_g(const Ctl* __restrict__ C, La
                                                    const ListRef* __restrict__ lr, double* __restrict__ cf_sph,
                                                    double* __restrict__ cf_cap, double* __restrict__ cb0,
                                                    double* __restrict__ cb1, double* __restrict__ cpart, int Pc) {
+  const bool beams = int(blockIdx.x) < Pb;
+  const int blk = beams ? blockIdx.x : blockIdx.x - Pb;
+  const Blk bk = beams ? tb[blk] : tc[blk];
+  const Ctl* C = Cs + bk.inst;
   if (!ctl_trial(C)) return;
   const int par = C->par;
-  if (int(blockIdx.x) < Pb) {
+  const int item = bk.first + threadIdx.x;
+  const bool on = int(threadIdx.x) < bk.count;
+  if (beams) {
     TermBufs bt{par ? ub1 : ub0, par ? ub0 : ub1};
-    beam_body<T>(L, M, x, R, rec, nullptr, bt, bpart, blockIdx.x, Pb);
+    beam_body<T>(L, I[bk.inst].M, x, R, rec, nullptr, bt, bpart, item, on, blk, Pb);
   } else {
     TermBufs ct{par ? cb1 : cb0, par ? cb0 : cb1};
-    S.slab = lr->slab;
-    contact_body<T>(L, S, x, pcount, lr->plist, cf_sph, cf_cap, ct, cpart, blockIdx.x - Pb, Pc);
+    contact_body<T>(L, I[bk.inst], x, pcount, lr->plist, lr->slab, cf_sph, cf_cap, ct, cpart, item, on, blk, Pc);
   }
 }
 
@@ -1318,8 +1380,9 @@ __global__ __launch_bounds__(kBlock) void k_elem_g(const Ctl* __restrict__ C, La
 // the spare history slot with all its products (partial rows 3.., layout of
 // pair_dots_body), computed by the thread that assembled the node's gradient.
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_gather_g(Lat<T> L, Scalars S, const Ctl* __restrict__ C,
-                                                     const double* __restrict__ x, const double* __restrict__ rec,
+__global__ __launch_bounds__(kBlock) void k_gather_g(Lat<T> L, const Inst* __restrict__ I, const Blk* __restrict__ tn,
+                                                     const Ctl* __restrict__ Cs, const double* __restrict__ x,
+                                                     const double* __restrict__ rec,
                                                      const double* __restrict__ cf_sph,
                                                      const double* __restrict__ cf_cap,
                                                      const double* __restrict__ spart, int Ps,
@@ -1327,26 +1390,33 @@ __global__ __launch_bounds__(kBlock) void k_gather_g(Lat<T> L, Scalars S, const 
                                                      double* __restrict__ g0, double* __restrict__ g1,
                                                      const double* __restrict__ p, double* __restrict__ Sh,
                                                      double* __restrict__ Yh, size_t n6, double* __restrict__ part) {
+  const Blk bk = tn[blockIdx.x];
+  const Ctl* C = Cs + bk.inst;
   if (!ctl_trial(C)) return;
-  S.top = C->top;
   const int par = C->par;
   double* gnew = par ? g0 : g1;
-  gather_body<T>(L, S, x, rec, cf_sph, cf_cap, spart, Ps, gnew, p, part);
+  const int i = bk.first + threadIdx.x;
+  const bool on = int(threadIdx.x) < bk.count;
+  gather_body<T>(L, I[bk.inst], C->top, x, rec, cf_sph, cf_cap, spart, Ps, gnew, p, part, i, on);
   if (C->m == 0) return;
   const PairArgs A = pair_args_dev(C);
-  pair_dots_body(L.N, n6, par ? z1 : z0, par ? z0 : z1, par ? g1 : g0, gnew, Sh, Yh, A, part + 3 * size_t(gridDim.x));
+  pair_dots_body(L.N, n6, par ? z1 : z0, par ? z0 : z1, par ? g1 : g0, gnew, Sh, Yh, A, part + 3 * size_t(gridDim.x),
+                 i, on);
 }
 
 // Candidate record and the Armijo decision (MC/src/sim.cpp:178-180,192-204);
 // on acceptance the curvature-pair update, the swap of current and candidate
 // buffers, the next convergence test and the next coefficients
 // (MC/src/sim.cpp:181-183,205-207).
-__global__ __launch_bounds__(kBlock) void k_decide_g(Ctl* __restrict__ C, int N, double centering, double n_free,
+// one block per instance
+__global__ __launch_bounds__(kBlock) void k_decide_g(Ctl* __restrict__ Cs, const Inst* __restrict__ I,
                                                      const double* __restrict__ spart, int Ps,
                                                      const double* __restrict__ cpart, int Pc,
                                                      const double* __restrict__ bpart, int Pb,
                                                      const double* __restrict__ gpart, int Pg,
-                                                     int* __restrict__ flags) {
+                                                     int* __restrict__ iflags) {
+  Ctl* C = Cs + blockIdx.x;
+  const Inst& in = I[blockIdx.x];
   if (!ctl_trial_phase(C)) return;
   __shared__ Ctl sc;
   __shared__ double d[4 * kMaxHist + 5];
@@ -1369,7 +1439,7 @@ __global__ __launch_bounds__(kBlock) void k_decide_g(Ctl* __restrict__ C, int N,
     }
     __syncthreads();
   }
-  final_body(N, centering, n_free, spart, Ps, cpart, Pc, bpart, Pb, gpart, Pg, flags, sc.cur, sc.cand);
+  final_body(in, spart, Ps, cpart, Pc, bpart, Pb, gpart, Pg, iflags + blockIdx.x, sc.cur, sc.cand);
   if (threadIdx.x == 0) {
     Ctl* c = &sc;
     const double* r = c->cand;
@@ -1414,7 +1484,7 @@ __global__ __launch_bounds__(kBlock) void k_decide_g(Ctl* __restrict__ C, int N,
       const int K = 4 * A.nold + 5, lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
       for (int r = wv; r < K; r += kBlock / kWave) {
         double a = 0.0;
-        for (int i = lane; i < Pg; i += kWave) a += gpart[size_t(3 + r) * Pg + i];
+        for (int i = in.pn0 + lane; i < in.pn1; i += kWave) a += gpart[size_t(3 + r) * Pg + i];
         a = wave_sum(a);
         if (lane == 0) d[r] = a;
       }
@@ -1453,30 +1523,35 @@ __global__ __launch_bounds__(kBlock) void k_fill_zero(double* __restrict__ a, lo
   if (i < n) a[i] = 0.0;
 }
 
-// commit, MC/src/sim.cpp:212: base pose <- state(z); plastic <- trial.next
-template <class T>
-__global__ __launch_bounds__(kBlock) void k_commit_nodes(int N, const double* __restrict__ x, const T* __restrict__ R,
+// commit, MC/src/sim.cpp:212: base pose <- state(z); plastic <- trial.next,
+// for the instances whose mask entry is set
+__global__ __launch_bounds__(kBlock) void k_commit_nodes(const Blk* __restrict__ tn, const int* __restrict__ mask, int N,
+                                                         const double* __restrict__ x, const double* __restrict__ R,
                                                          double* __restrict__ bpos, double* __restrict__ bquat) {
-  const int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= N) return;
+  const Blk bk = tn[blockIdx.x];
+  if (!mask[bk.inst] || int(threadIdx.x) >= bk.count) return;
+  const int i = bk.first + threadIdx.x;
   for (int d = 0; d < 3; d++) bpos[d * N + i] = x[d * N + i];
-  for (int d = 0; d < 4; d++) bquat[d * N + i] = double(R[d * N + i]);
+  for (int d = 0; d < 4; d++) bquat[d * N + i] = R[d * N + i];
 }
-__global__ __launch_bounds__(kBlock) void k_commit_beams(int B, const double* __restrict__ trial,
+__global__ __launch_bounds__(kBlock) void k_commit_beams(const Blk* __restrict__ tb, const int* __restrict__ mask, int B,
+                                                         const double* __restrict__ trial,
                                                          double* __restrict__ plastic) {
-  const int k = blockIdx.x * kBlock + threadIdx.x;
-  if (k >= B) return;
+  const Blk bk = tb[blockIdx.x];
+  if (!mask[bk.inst] || int(threadIdx.x) >= bk.count) return;
+  const int k = bk.first + threadIdx.x;
   for (int c = 0; c < 7; c++) plastic[c * B + k] = trial[c * B + k];
 }
 
 // diagnostics reductions, MC/src/sim.cpp:339-370.
-// partials [8][P]: sum|epsp|, max|epsp|, sum|rotp|, max|rotp|, #plastic beams,
-// #hinges, min y, max y (min stored negated so that all maxima share one op)
-__global__ __launch_bounds__(kBlock) void k_stats(int N, int B, const double* __restrict__ plastic,
-                                                  const double* __restrict__ bpos, double* __restrict__ part) {
-  const int i = blockIdx.x * kBlock + threadIdx.x, P = gridDim.x;
-  double v[8] = {0, 0, 0, 0, 0, 0, -INFINITY, -INFINITY};
-  if (i < B) {
+// beam partials [6][Pb]: sum|epsp|, max|epsp|, sum|rotp|, max|rotp|, #plastic
+// beams, #hinges; node partials [2][Pn]: min y (negated), max y
+__global__ __launch_bounds__(kBlock) void k_stats_beams(const Blk* __restrict__ tb, int B,
+                                                        const double* __restrict__ plastic, double* __restrict__ part) {
+  const Blk bk = tb[blockIdx.x];
+  const int i = bk.first + threadIdx.x, P = gridDim.x;
+  double v[6] = {0, 0, 0, 0, 0, 0};
+  if (int(threadIdx.x) < bk.count) {
     double pa = fabs(plastic[i]);
     double pr = sqrt(plastic[2 * B + i] * plastic[2 * B + i] + plastic[3 * B + i] * plastic[3 * B + i] +
                      plastic[4 * B + i] * plastic[4 * B + i]);
@@ -1487,20 +1562,48 @@ __global__ __launch_bounds__(kBlock) void k_stats(int N, int B, const double* __
     v[4] = pa > 1e-5 ? 1 : 0;
     v[5] = pr > 1e-4 ? 1 : 0;
   }
-  if (i < N) {
-    v[6] = -bpos[N + i];
-    v[7] = bpos[N + i];
-  }
-  block_reduce<8>(v, (1u << 1) | (1u << 3) | (1u << 6) | (1u << 7));
+  block_reduce<6>(v, (1u << 1) | (1u << 3));
   if (threadIdx.x == 0)
-    for (int k = 0; k < 8; k++) part[k * P + blockIdx.x] = v[k];
+    for (int k = 0; k < 6; k++) part[k * P + blockIdx.x] = v[k];
 }
-__global__ __launch_bounds__(kBlock) void k_stats_final(const double* __restrict__ part, int P,
-                                                        double* __restrict__ out) {
-  double s[8];
-  reduce_partials<8>(part, P, s, (1u << 1) | (1u << 3) | (1u << 6) | (1u << 7));
+__global__ __launch_bounds__(kBlock) void k_stats_nodes(const Blk* __restrict__ tn, int N,
+                                                        const double* __restrict__ bpos, double* __restrict__ part) {
+  const Blk bk = tn[blockIdx.x];
+  const int i = bk.first + threadIdx.x, P = gridDim.x;
+  double v[2] = {-INFINITY, -INFINITY};
+  if (int(threadIdx.x) < bk.count) {
+    v[0] = -bpos[N + i];
+    v[1] = bpos[N + i];
+  }
+  block_reduce<2>(v, 3u);
   if (threadIdx.x == 0)
-    for (int k = 0; k < 8; k++) out[k] = s[k];
+    for (int k = 0; k < 2; k++) part[k * P + blockIdx.x] = v[k];
+}
+// one block per instance: out[8 * inst + k] in the order of mmc_state_stats
+__global__ __launch_bounds__(kBlock) void k_stats_final(const Inst* __restrict__ I, const double* __restrict__ bpartial,
+                                                        int Pb, const double* __restrict__ npartial, int Pn,
+                                                        double* __restrict__ out) {
+  const Inst& in = I[blockIdx.x];
+  double b[6], n[2];
+  reduce_columns<6>(bpartial, Pb, in.pb0, in.pb1, b, (1u << 1) | (1u << 3));
+  reduce_columns<2>(npartial, Pn, in.pn0, in.pn1, n, 3u);
+  if (threadIdx.x == 0) {
+    double* o = out + 8 * blockIdx.x;
+    for (int k = 0; k < 6; k++) o[k] = b[k];
+    o[6] = n[0];
+    o[7] = n[1];
+  }
+}
+
+// z of every instance from its current buffer (after the device loop)
+__global__ __launch_bounds__(kBlock) void k_take_current(const Blk* __restrict__ tn, const Ctl* __restrict__ Cs, int N,
+                                                         const double* __restrict__ z0, const double* __restrict__ z1,
+                                                         double* __restrict__ z) {
+  const Blk bk = tn[blockIdx.x];
+  if (int(threadIdx.x) >= bk.count) return;
+  const int i = bk.first + threadIdx.x;
+  const double* src = Cs[bk.inst].par ? z1 : z0;
+  for (int d = 0; d < 6; d++) z[size_t(d) * N + i] = src[size_t(d) * N + i];
 }
 
 }  // namespace
@@ -1543,15 +1646,18 @@ constexpr int kScal = 128;
 constexpr int kHost = 256;
 static_assert(4 * kMaxHist + 5 <= kScal && 128 + 4 * kMaxHist + 5 <= kHost, "scalar buffers too small");
 
+// A context holds n_inst independent lattices (one for the single-lattice
+// API) in one concatenated SoA with per-instance constants, block tables and
+// control blocks; every launch covers all of them.
 struct mmc_ctx {
   std::string err;
   int precision = MMC_FP64;
-  int N = 0, B = 0, NS = 0;
-  mmc_config cfg{};
-  double jr = 0, n_free = 0;
+  int n_inst = 0;
+  int N = 0, B = 0, NS = 0;  // totals over the instances
+  std::vector<mmc_config> icfg;  // per instance (translation_scale = cell_size)
+  std::vector<Inst> hinst;
+  mmc_config cfg{};  // instance 0 (single-lattice API)
   hipStream_t stream = nullptr;
-  Material M{};
-  Scalars S{};
   // device buffers
   double *bpos = nullptr, *bquat = nullptr, *mass = nullptr;
   uint8_t* lock = nullptr;
@@ -1559,32 +1665,44 @@ struct mmc_ctx {
   double *L0 = nullptr, *rad = nullptr, *area = nullptr, *inert = nullptr, *e0 = nullptr, *plastic = nullptr;
   int *inc_off = nullptr, *inc = nullptr, *adj_off = nullptr, *adj = nullptr;
   double *x = nullptr;
-  void* R = nullptr;  // double or float planes
+  void* R = nullptr;  // FP64 planes
   double *rec = nullptr, *trial = nullptr, *cf_sph = nullptr, *cf_cap = nullptr;
   float* box = nullptr;
   unsigned *cnt = nullptr, *start = nullptr;
   int* entries = nullptr;
-  int* flags = nullptr;
+  int entry_cap = 0;
+  unsigned table = 0;
+  int* flags = nullptr;  // [0]: list-build bits (4 entry overflow, 8 slab overflow); [4 + b]: instance b's bits
+  // instances and block tables
+  Inst* inst_d = nullptr;
+  Blk *tn = nullptr, *tb = nullptr, *tc = nullptr, *td = nullptr;
+  int* sinst = nullptr;       // instance of every shape
+  double* tops_d = nullptr;   // platen of every instance (host-driven evaluations)
+  double* htops = nullptr;    // pinned staging of tops_d
+  int* mask_d = nullptr;      // commit mask
+  int* hmask = nullptr;       // pinned staging of mask_d
   // Verlet contact list
   double* xref = nullptr;
   unsigned* pcount = nullptr;
-  int* plist = nullptr;  // [NS][S.slab]
+  int* plist = nullptr;  // [NS][slab]
+  int slab = 8;
   bool list_valid = false;
   int64_t rebuilds = 0;
   void* scan_tmp = nullptr;
   size_t scan_bytes = 0;
-  unsigned table = 0;
   double *spart = nullptr, *cpart = nullptr, *bpart = nullptr, *gpart = nullptr, *vpart = nullptr;
-  double* stpart = nullptr;
-  int Ps = 0, Pc = 0, Pb = 0, Pe = 0;  // blocks over nodes, shapes, beams, 6N vector elements
-  double* record = nullptr;  // kRec per slot, slots: 0 current, 1 candidate, 2 scratch
+  double *stb = nullptr, *stn = nullptr, *stout = nullptr, *hstat = nullptr;
+  int Ps = 0, Pc = 0, Pb = 0, Pd = 0;  // blocks of the node, shape, beam and 64-node tables
+  double* record = nullptr;  // [3][n_inst][kRec]: slot 0 current, 1 candidate, 2 final
+  double* hrec = nullptr;    // pinned readback of one record slot
   double* scal = nullptr;    // misc scalars (dots)
   double *z = nullptr, *g = nullptr, *p = nullptr, *zc = nullptr, *gc = nullptr, *zt = nullptr, *zin = nullptr;
-  double *Sh = nullptr, *Yh = nullptr;  // [m][6N]
+  double *Sh = nullptr, *Yh = nullptr;  // [m + 1][6N]
+  int hist_slots = 1;
   double* host = nullptr;  // pinned readback
-  // device-resident minimiser: control block, its pinned staging copy, the
-  // (current, candidate) buffer pairs selected by Ctl::par, and the graph of
-  // one iteration
+  // device-resident minimiser: control blocks (one per instance), their pinned
+  // staging copies, the (current, candidate) buffer pairs selected by each
+  // instance's Ctl::par, and the graph of kGraphIterations replays
   Ctl* ctl = nullptr;
   Ctl* hctl = nullptr;
   double *zbuf[2] = {nullptr, nullptr}, *gbuf[2] = {nullptr, nullptr};
@@ -1631,44 +1749,61 @@ struct mmc_ctx {
   Lat<float> lat32() const {
     return Lat<float>{N, B, bpos, bquat, mass, lock, ba, bb, L0, rad, area, inert, e0, plastic, inc_off, inc, adj_off, adj};
   }
+  int* iflags() const { return flags + 4; }
+  double* rec_slot(int s) const { return record + size_t(s) * n_inst * kRec; }
   ~mmc_ctx();
-  void setup(const mmc_lattice* L, const mmc_config* c, int prec);
+  void setup(const mmc_lattice* const* L, const mmc_config* c, int n, int prec);
   void ensure_entries(size_t need);
+  void upload_inst() {
+    if (inst_d) HIPCALL(hipMemcpy(inst_d, hinst.data(), hinst.size() * sizeof(Inst), hipMemcpyHostToDevice));
+  }
   template <class T>
   void build_list(Lat<T> L);
+  void rebuild() {
+    if (precision == MMC_MIXED)
+      build_list<float>(lat32());
+    else
+      build_list<double>(lat64());
+  }
   template <class T>
   void capture_iteration(Lat<T> L);
-  void minimize_graph(double top, mmc_opt_stats& st);
+  void minimize_graph(const double* tops, const int32_t* active, mmc_opt_stats* st);
   void minimize_host(double top, mmc_opt_stats& st);
-  void evaluate(const double* zsrc, const double* psrc, double step, double* zdst, double* gdst, double top,
+  void final_evaluation(const double* tops, mmc_opt_stats* st, mmc_objective_out* fin);
+  void evaluate(const double* zsrc, const double* psrc, double step, double* zdst, double* gdst, const double* tops,
                 bool trials, double* rec_out);
   template <class T>
   void evaluate_t(Lat<T> L, const double* zsrc, const double* psrc, double step, double* zdst, double* gdst,
-                  double top, bool trials, double* rec_out);
+                  bool trials, double* rec_out);
+  void set_tops(const double* tops) {
+    for (int b = 0; b < n_inst; b++) htops[b] = tops[b];
+    HIPCALL(hipMemcpyAsync(tops_d, htops, n_inst * sizeof(double), hipMemcpyHostToDevice, stream));
+  }
   void read(void* dst, const void* src, size_t bytes) {
     HIPCALL(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream));
     HIPCALL(hipStreamSynchronize(stream));
     harvest();
   }
   void zero(double* a, long n) { k_fill_zero<<<blocks(n), kBlock, 0, stream>>>(a, n); }
+  void need_single(const char* what) const {
+    if (n_inst != 1) throw Fail(MMC_ERR_ARG, std::string(what) + " needs a single-lattice context");
+  }
 };
 
 mmc_ctx::~mmc_ctx() {
   void* ptrs[] = {bpos, bquat, mass, lock, ba, bb, L0, rad, area, inert, e0, plastic, inc_off, inc, adj_off, adj, x, R,
                   rec, trial, cf_sph, cf_cap, box, cnt, start, entries, flags, scan_tmp, spart, cpart, bpart, gpart,
-                  vpart, stpart, record, scal, z, g, p, zc, gc, zt, zin, Sh, Yh, xref, pcount, plist};
+                  vpart, stb, stn, stout, record, scal, z, g, p, zc, gc, zt, zin, Sh, Yh, xref, pcount, plist,
+                  inst_d, tn, tb, tc, td, sinst, tops_d, mask_d, ctl, lref};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
-  if (host) (void)hipHostFree(host);
-  if (hctl) (void)hipHostFree(hctl);
-  if (ctl) (void)hipFree(ctl);
-  if (lref) (void)hipFree(lref);
+  void* pinned[] = {host, hctl, htops, hmask, hrec, hstat};
+  for (void* q : pinned)
+    if (q) (void)hipHostFree(q);
   for (int b = 0; b < 2; b++) {
-    if (zbuf[b]) (void)hipFree(zbuf[b]);
-    if (gbuf[b]) (void)hipFree(gbuf[b]);
-    if (nbuf[b]) (void)hipFree(nbuf[b]);
-    if (ubuf[b]) (void)hipFree(ubuf[b]);
-    if (cbuf[b]) (void)hipFree(cbuf[b]);
+    void* bufs[] = {zbuf[b], gbuf[b], nbuf[b], ubuf[b], cbuf[b]};
+    for (void* q : bufs)
+      if (q) (void)hipFree(q);
   }
   if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
   if (graph) (void)hipGraphDestroy(graph);
@@ -1677,105 +1812,180 @@ mmc_ctx::~mmc_ctx() {
   if (stream) (void)hipStreamDestroy(stream);
 }
 
-void mmc_ctx::setup(const mmc_lattice* L, const mmc_config* c, int prec) {
-  cfg = *c;
-  cfg.translation_scale = cfg.cell_size;  // MC/src/sim.cpp:49
+namespace {
+// Splits every instance's item range [first, first + count) into blocks of
+// `per` items; range(b) gives instance b's items.  Returns the table and, for
+// each instance, its block range.
+template <class F>
+std::vector<Blk> block_table(int n, int per, F range, std::vector<int>& b0, std::vector<int>& b1) {
+  std::vector<Blk> t;
+  b0.assign(n, 0);
+  b1.assign(n, 0);
+  for (int b = 0; b < n; b++) {
+    int first, count;
+    range(b, first, count);
+    b0[b] = int(t.size());
+    for (int o = 0; o < count; o += per) t.push_back(Blk{b, first + o, std::min(per, count - o), 0});
+    b1[b] = int(t.size());
+  }
+  if (t.empty()) t.push_back(Blk{0, 0, 0, 0});
+  return t;
+}
+}  // namespace
+
+void mmc_ctx::setup(const mmc_lattice* const* Ls, const mmc_config* cs, int n, int prec) {
+  if (n <= 0) throw Fail(MMC_ERR_ARG, "a context needs at least one lattice");
   precision = prec;
-  N = L->n_nodes;
-  B = L->n_beams;
+  n_inst = n;
+  icfg.assign(cs, cs + n);
+  int max_hist = 0;
+  long long tn_ = 0, tb_ = 0;
+  for (int b = 0; b < n; b++) {
+    mmc_config& c = icfg[b];
+    c.translation_scale = c.cell_size;  // MC/src/sim.cpp:49
+    if (Ls[b]->n_nodes <= 0 || Ls[b]->n_beams <= 0) throw Fail(MMC_ERR_ARG, "lattice has no nodes or beams");
+    if (c.history < 0 || c.history > kMaxHist) throw Fail(MMC_ERR_ARG, "history must be in [0, 16]");
+    if (c.line_search != 0 && c.line_search != 1) throw Fail(MMC_ERR_ARG, "line_search must be 0 or 1");
+    max_hist = std::max(max_hist, c.history);
+    tn_ += Ls[b]->n_nodes;
+    tb_ += Ls[b]->n_beams;
+  }
+  if (tn_ + tb_ > 0x3fffffffLL) throw Fail(MMC_ERR_CAPACITY, "batch exceeds 2^30 nodes + beams");
+  cfg = icfg[0];
+  N = int(tn_);
+  B = int(tb_);
   NS = N + B;
-  if (N <= 0 || B <= 0) throw Fail(MMC_ERR_ARG, "lattice has no nodes or beams");
-  if (cfg.history < 0 || cfg.history > kMaxHist) throw Fail(MMC_ERR_ARG, "history must be in [0, 16]");
-  if (cfg.line_search != 0 && cfg.line_search != 1) throw Fail(MMC_ERR_ARG, "line_search must be 0 or 1");
-  jr = cfg.joint_scale * cfg.max_radius;
   HIPCALL(hipStreamCreate(&stream));
-  const mmc_material& m = cfg.material;
-  M = Material{m.young, m.young / (2 * (1 + m.nu)), m.yield, m.hardening, m.bend_hardening, m.torsion_hardening,
-               cfg.shear_correction, m.density};
-  // ---- host-side SoA staging ----
+  // ---- instances: ranges and constants ----
+  hinst.assign(n, Inst{});
+  {
+    int n0 = 0, b0 = 0;
+    for (int b = 0; b < n; b++) {
+      Inst& in = hinst[b];
+      const mmc_config& c = icfg[b];
+      in.n0 = n0;
+      in.N = Ls[b]->n_nodes;
+      in.b0 = b0;
+      in.B = Ls[b]->n_beams;
+      in.s0 = n0 + b0;  // shapes of instance b: its spheres, then its capsules
+      in.NS = in.N + in.B;
+      n0 += in.N;
+      b0 += in.B;
+      const double jr = c.joint_scale * c.max_radius;
+      Scalars& S = in.S;
+      S.ts = c.translation_scale;
+      S.jr = jr;
+      S.gravity = c.gravity;
+      S.kc = c.contact_stiffness;
+      S.margin = c.contact_margin;
+      S.centering = c.centering_stiffness;
+      S.cell = std::max(2.8 * jr, 0.72 * c.cell_size);  // MC/src/sim.cpp:151
+      S.inv_cell = 1.0 / S.cell;
+      S.skin = 0.5 * S.cell;
+      const mmc_material& m = c.material;
+      in.M = Material{m.young, m.young / (2 * (1 + m.nu)), m.yield, m.hardening, m.bend_hardening,
+                      m.torsion_hardening, c.shear_correction, m.density};
+      in.mt = c.max_translation_step / c.translation_scale;
+      in.mr = c.max_rotation_step;
+    }
+  }
+  // ---- host-side SoA staging of the concatenated lattice ----
   std::vector<double> h(7 * size_t(std::max(N, B)));
-  auto up = [&](double* dst, const double* src, size_t n) {
-    HIPCALL(hipMemcpy(dst, src, n * sizeof(double), hipMemcpyHostToDevice));
+  auto up = [&](double* dst, const double* src, size_t cnt_) {
+    HIPCALL(hipMemcpy(dst, src, cnt_ * sizeof(double), hipMemcpyHostToDevice));
   };
-  auto soa = [&](const double* aos, int n, int k) {
-    for (int i = 0; i < n; i++)
-      for (int d = 0; d < k; d++) h[size_t(d) * n + i] = aos[size_t(k) * i + d];
+  // AoS field of k components of every instance -> SoA planes of the total
+  auto soa = [&](int total, int k, bool nodes, auto get) {
+    for (int b = 0; b < n; b++) {
+      const Inst& in = hinst[b];
+      const int off = nodes ? in.n0 : in.b0, cntb = nodes ? in.N : in.B;
+      for (int i = 0; i < cntb; i++)
+        for (int d = 0; d < k; d++) h[size_t(d) * total + off + i] = get(b, i, d);
+    }
   };
   bpos = dalloc<double>(3 * size_t(N));
-  soa(L->x, N, 3);
+  soa(N, 3, true, [&](int b, int i, int d) { return Ls[b]->x[3 * size_t(i) + d]; });
   up(bpos, h.data(), 3 * size_t(N));
   bquat = dalloc<double>(4 * size_t(N));
-  if (L->q) {
-    soa(L->q, N, 4);
-  } else {
-    for (int i = 0; i < N; i++) {
-      h[i] = 1;
-      h[N + i] = h[2 * N + i] = h[3 * N + i] = 0;
-    }
-  }
+  soa(N, 4, true, [&](int b, int i, int d) { return Ls[b]->q ? Ls[b]->q[4 * size_t(i) + d] : (d == 0 ? 1.0 : 0.0); });
   up(bquat, h.data(), 4 * size_t(N));
   mass = dalloc<double>(N);
-  up(mass, L->mass, N);
+  soa(N, 1, true, [&](int b, int i, int) { return Ls[b]->mass[i]; });
+  up(mass, h.data(), N);
   std::vector<uint8_t> lk(N, 0);
-  n_free = 0;
-  for (int i = 0; i < N; i++)
-    for (int d = 0; d < 6; d++) {
-      if (L->locked && L->locked[6 * i + d]) lk[i] |= uint8_t(1u << d);
-      else n_free += 1;
+  std::vector<int> ha(B), hb(B);
+  for (int b = 0; b < n; b++) {
+    Inst& in = hinst[b];
+    const mmc_lattice* L = Ls[b];
+    in.n_free = 0;
+    for (int i = 0; i < in.N; i++)
+      for (int d = 0; d < 6; d++) {
+        if (L->locked && L->locked[6 * size_t(i) + d])
+          lk[in.n0 + i] |= uint8_t(1u << d);
+        else
+          in.n_free += 1;
+      }
+    for (int k = 0; k < in.B; k++) {
+      if (L->beam_a[k] < 0 || L->beam_a[k] >= in.N || L->beam_b[k] < 0 || L->beam_b[k] >= in.N)
+        throw Fail(MMC_ERR_ARG, "beam endpoint out of range");
+      ha[in.b0 + k] = in.n0 + L->beam_a[k];
+      hb[in.b0 + k] = in.n0 + L->beam_b[k];
     }
+  }
   lock = dalloc<uint8_t>(N);
   HIPCALL(hipMemcpy(lock, lk.data(), N, hipMemcpyHostToDevice));
-  for (int k = 0; k < B; k++)
-    if (L->beam_a[k] < 0 || L->beam_a[k] >= N || L->beam_b[k] < 0 || L->beam_b[k] >= N)
-      throw Fail(MMC_ERR_ARG, "beam endpoint out of range");
   ba = dalloc<int>(B);
   bb = dalloc<int>(B);
-  HIPCALL(hipMemcpy(ba, L->beam_a, B * sizeof(int), hipMemcpyHostToDevice));
-  HIPCALL(hipMemcpy(bb, L->beam_b, B * sizeof(int), hipMemcpyHostToDevice));
+  HIPCALL(hipMemcpy(ba, ha.data(), B * sizeof(int), hipMemcpyHostToDevice));
+  HIPCALL(hipMemcpy(bb, hb.data(), B * sizeof(int), hipMemcpyHostToDevice));
   L0 = dalloc<double>(B);
-  up(L0, L->L0, B);
+  soa(B, 1, false, [&](int b, int i, int) { return Ls[b]->L0[i]; });
+  up(L0, h.data(), B);
   rad = dalloc<double>(B);
-  up(rad, L->radius, B);
+  soa(B, 1, false, [&](int b, int i, int) { return Ls[b]->radius[i]; });
+  up(rad, h.data(), B);
   area = dalloc<double>(B);
-  up(area, L->area, B);
+  soa(B, 1, false, [&](int b, int i, int) { return Ls[b]->area[i]; });
+  up(area, h.data(), B);
   inert = dalloc<double>(B);
-  up(inert, L->inertia, B);
+  soa(B, 1, false, [&](int b, int i, int) { return Ls[b]->inertia[i]; });
+  up(inert, h.data(), B);
   e0 = dalloc<double>(3 * size_t(B));
-  soa(L->e0, B, 3);
+  soa(B, 3, false, [&](int b, int i, int d) { return Ls[b]->e0[3 * size_t(i) + d]; });
   up(e0, h.data(), 3 * size_t(B));
   plastic = dalloc<double>(7 * size_t(B));
-  if (L->plastic) {
-    soa(L->plastic, B, 7);
-  } else {
-    std::fill(h.begin(), h.begin() + 7 * size_t(B), 0.0);
-  }
+  soa(B, 7, false, [&](int b, int i, int d) { return Ls[b]->plastic ? Ls[b]->plastic[7 * size_t(i) + d] : 0.0; });
   up(plastic, h.data(), 7 * size_t(B));
   // ---- CSR beam incidence, ordered by beam index ----
   std::vector<int> off(N + 1, 0), list(2 * size_t(B));
   for (int k = 0; k < B; k++) {
-    off[L->beam_a[k] + 1]++;
-    off[L->beam_b[k] + 1]++;
+    off[ha[k] + 1]++;
+    off[hb[k] + 1]++;
   }
   for (int i = 0; i < N; i++) off[i + 1] += off[i];
   {
     std::vector<int> cur(off.begin(), off.end() - 1);
     for (int k = 0; k < B; k++) {
-      list[cur[L->beam_a[k]]++] = 2 * k;
-      list[cur[L->beam_b[k]]++] = 2 * k + 1;
+      list[cur[ha[k]]++] = 2 * k;
+      list[cur[hb[k]]++] = 2 * k + 1;
     }
   }
   inc_off = dalloc<int>(N + 1);
   inc = dalloc<int>(2 * size_t(B));
   HIPCALL(hipMemcpy(inc_off, off.data(), (N + 1) * sizeof(int), hipMemcpyHostToDevice));
   HIPCALL(hipMemcpy(inc, list.data(), 2 * size_t(B) * sizeof(int), hipMemcpyHostToDevice));
-  // ---- exclusion adjacency from the connected pair set ----
+  // ---- exclusion adjacency from the connected pair sets ----
   std::vector<std::vector<int>> nb(N);
-  for (int64_t c2 = 0; c2 < L->n_connected; c2++) {
-    uint64_t key = L->connected[c2];
-    int u = int(uint32_t(key >> 32)), v = int(uint32_t(key & 0xffffffffu));
-    if (u < 0 || u >= N || v < 0 || v >= N) throw Fail(MMC_ERR_ARG, "connected pair out of range");
-    nb[u].push_back(v);
-    nb[v].push_back(u);
+  for (int b = 0; b < n; b++) {
+    const mmc_lattice* L = Ls[b];
+    const int o = hinst[b].n0, Nb = hinst[b].N;
+    for (int64_t c2 = 0; c2 < L->n_connected; c2++) {
+      uint64_t key = L->connected[c2];
+      int u = int(uint32_t(key >> 32)), v = int(uint32_t(key & 0xffffffffu));
+      if (u < 0 || u >= Nb || v < 0 || v >= Nb) throw Fail(MMC_ERR_ARG, "connected pair out of range");
+      nb[o + u].push_back(o + v);
+      nb[o + v].push_back(o + u);
+    }
   }
   std::vector<int> aoff(N + 1, 0), alist;
   for (int i = 0; i < N; i++) {
@@ -1788,6 +1998,35 @@ void mmc_ctx::setup(const mmc_lattice* L, const mmc_config* c, int prec) {
   adj = dalloc<int>(alist.size());
   HIPCALL(hipMemcpy(adj_off, aoff.data(), (N + 1) * sizeof(int), hipMemcpyHostToDevice));
   if (!alist.empty()) HIPCALL(hipMemcpy(adj, alist.data(), alist.size() * sizeof(int), hipMemcpyHostToDevice));
+  // ---- block tables ----
+  {
+    std::vector<int> r0, r1;
+    auto put = [&](Blk*& dst, const std::vector<Blk>& t) {
+      dst = dalloc<Blk>(t.size());
+      HIPCALL(hipMemcpy(dst, t.data(), t.size() * sizeof(Blk), hipMemcpyHostToDevice));
+    };
+    auto t = block_table(n, kBlock, [&](int b, int& f, int& c) { f = hinst[b].n0, c = hinst[b].N; }, r0, r1);
+    Ps = int(t.size());
+    for (int b = 0; b < n; b++) hinst[b].pn0 = r0[b], hinst[b].pn1 = r1[b];
+    put(tn, t);
+    t = block_table(n, kBlock, [&](int b, int& f, int& c) { f = hinst[b].b0, c = hinst[b].B; }, r0, r1);
+    Pb = int(t.size());
+    for (int b = 0; b < n; b++) hinst[b].pb0 = r0[b], hinst[b].pb1 = r1[b];
+    put(tb, t);
+    t = block_table(n, kBlock, [&](int b, int& f, int& c) { f = hinst[b].s0, c = hinst[b].NS; }, r0, r1);
+    Pc = int(t.size());
+    for (int b = 0; b < n; b++) hinst[b].pc0 = r0[b], hinst[b].pc1 = r1[b];
+    put(tc, t);
+    t = block_table(n, kDirNodes, [&](int b, int& f, int& c) { f = hinst[b].n0, c = hinst[b].N; }, r0, r1);
+    Pd = int(t.size());
+    for (int b = 0; b < n; b++) hinst[b].pd0 = r0[b], hinst[b].pd1 = r1[b];
+    put(td, t);
+    std::vector<int> si(NS);
+    for (int b = 0; b < n; b++)
+      for (int s = 0; s < hinst[b].NS; s++) si[hinst[b].s0 + s] = b;
+    sinst = dalloc<int>(NS);
+    HIPCALL(hipMemcpy(sinst, si.data(), NS * sizeof(int), hipMemcpyHostToDevice));
+  }
   // ---- evaluation buffers ----
   x = dalloc<double>(3 * size_t(N));
   R = dalloc<double>(4 * size_t(N));
@@ -1801,27 +2040,26 @@ void mmc_ctx::setup(const mmc_lattice* L, const mmc_config* c, int prec) {
   cnt = dalloc<unsigned>(table + 1);
   start = dalloc<unsigned>(table + 1);
   HIPCALL(hipMemset(cnt, 0, (table + 1) * sizeof(unsigned)));
-  flags = dalloc<int>(4);
-  HIPCALL(hipMemset(flags, 0, 4 * sizeof(int)));
+  flags = dalloc<int>(4 + size_t(n));
+  HIPCALL(hipMemset(flags, 0, (4 + size_t(n)) * sizeof(int)));
   xref = dalloc<double>(3 * size_t(N));
   pcount = dalloc<unsigned>(size_t(NS));
   HIPCALL(hipMemset(pcount, 0, size_t(NS) * sizeof(unsigned)));
   HIPCALL(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, cnt, start, table + 1, stream));
   scan_tmp = dalloc<char>(scan_bytes);
+  for (Inst& in : hinst) in.S.table_mask = table - 1;
   ensure_entries(size_t(NS) * 16);
-  S.slab = 8;
-  plist = dalloc<int>(size_t(NS) * S.slab);
-  Ps = blocks(N);
-  Pc = blocks(NS);
-  Pb = blocks(B);
+  slab = 8;
+  plist = dalloc<int>(size_t(NS) * slab);
   spart = dalloc<double>(9 * size_t(Ps));
   cpart = dalloc<double>(4 * size_t(Pc));
   bpart = dalloc<double>(4 * size_t(Pb));
   gpart = dalloc<double>((3 + 4 * size_t(kMaxHist) + 5) * size_t(Ps));  // gradient rows, then pair rows
-  Pe = blocks(6 * N);
-  vpart = dalloc<double>((4 * size_t(kMaxHist) + 8) * size_t(std::max({Ps, Pe, (N + kDirNodes - 1) / kDirNodes})));
-  stpart = dalloc<double>(8 * size_t(blocks(std::max(N, B))));
-  record = dalloc<double>(kRec * 4);
+  vpart = dalloc<double>((4 * size_t(kMaxHist) + 8) * size_t(std::max(Ps, Pd)));
+  stb = dalloc<double>(6 * size_t(Pb));
+  stn = dalloc<double>(2 * size_t(Ps));
+  stout = dalloc<double>(8 * size_t(n));
+  record = dalloc<double>(3 * size_t(n) * kRec);
   scal = dalloc<double>(kScal);
   size_t n6 = 6 * size_t(N);
   z = dalloc<double>(n6);
@@ -1831,13 +2069,21 @@ void mmc_ctx::setup(const mmc_lattice* L, const mmc_config* c, int prec) {
   gc = dalloc<double>(n6);
   zt = dalloc<double>(n6);
   zin = dalloc<double>(n6);
-  const size_t hm = size_t(cfg.history + 1);  // the device loop keeps one spare slot
-  Sh = dalloc<double>(n6 * hm);
-  Yh = dalloc<double>(n6 * hm);
+  hist_slots = max_hist + 1;  // the device loop keeps one spare slot
+  Sh = dalloc<double>(n6 * hist_slots);
+  Yh = dalloc<double>(n6 * hist_slots);
   HIPCALL(hipHostMalloc((void**)&host, kHost * sizeof(double), hipHostMallocDefault));
-  HIPCALL(hipHostMalloc((void**)&hctl, sizeof(Ctl), hipHostMallocDefault));
-  ctl = dalloc<Ctl>(1);
+  HIPCALL(hipHostMalloc((void**)&hctl, size_t(n) * sizeof(Ctl), hipHostMallocDefault));
+  HIPCALL(hipHostMalloc((void**)&htops, size_t(n) * sizeof(double), hipHostMallocDefault));
+  HIPCALL(hipHostMalloc((void**)&hmask, size_t(n) * sizeof(int), hipHostMallocDefault));
+  HIPCALL(hipHostMalloc((void**)&hrec, size_t(n) * kRec * sizeof(double), hipHostMallocDefault));
+  HIPCALL(hipHostMalloc((void**)&hstat, size_t(n) * 8 * sizeof(double), hipHostMallocDefault));
+  ctl = dalloc<Ctl>(n);
   lref = dalloc<ListRef>(1);
+  tops_d = dalloc<double>(n);
+  mask_d = dalloc<int>(n);
+  inst_d = dalloc<Inst>(n);
+  upload_inst();  // the instances are complete here (table mask, entry capacity)
   for (int b = 0; b < 2; b++) {
     zbuf[b] = dalloc<double>(n6);
     gbuf[b] = dalloc<double>(n6);
@@ -1845,43 +2091,35 @@ void mmc_ctx::setup(const mmc_lattice* L, const mmc_config* c, int prec) {
     ubuf[b] = dalloc<double>(size_t(B));
     cbuf[b] = dalloc<double>(size_t(NS));
   }
-  S.ts = cfg.translation_scale;
-  S.jr = jr;
-  S.gravity = cfg.gravity;
-  S.kc = cfg.contact_stiffness;
-  S.margin = cfg.contact_margin;
-  S.centering = cfg.centering_stiffness;
-  S.cell = std::max(2.8 * jr, 0.72 * cfg.cell_size);  // MC/src/sim.cpp:151
-  S.inv_cell = 1.0 / S.cell;
-  S.skin = 0.5 * S.cell;
-  S.table_mask = table - 1;
   HIPCALL(hipStreamSynchronize(stream));
 }
 
 void mmc_ctx::ensure_entries(size_t need) {
-  if (entries && size_t(S.entry_cap) >= need) return;
+  if (entries && size_t(entry_cap) >= need) return;
   if (entries) HIPCALL(hipFree(entries));
   size_t cap = std::max<size_t>(need, 1024);
   if (cap > size_t(0x7fffffff)) throw Fail(MMC_ERR_CAPACITY, "contact grid entry capacity exceeds 2^31");
   entries = dalloc<int>(cap);
-  S.entry_cap = int(cap);
+  entry_cap = int(cap);
+  for (Inst& in : hinst) in.S.entry_cap = entry_cap;
+  upload_inst();
 }
 
-// Contact list rebuild at the current positions x (rare: only when some node
-// has moved skin/2 since the last build).  Two host synchronisations size the
-// grid entries and the pair list exactly.
+// Contact list rebuild at the current positions x of every instance (rare:
+// only when some node has moved skin/2 since the last build).  Two host
+// synchronisations size the grid entries and the pair list exactly.
 template <class T>
 void mmc_ctx::build_list(Lat<T> Lt) {
-  k_count<T><<<Pc, kBlock, 0, stream>>>(Lt, S, x, box, cnt, flags);
+  k_count<T><<<Pc, kBlock, 0, stream>>>(Lt, inst_d, tc, x, box, cnt, iflags());
   HIPCALL(hipcub::DeviceScan::ExclusiveSum(scan_tmp, scan_bytes, cnt, start, table + 1, stream));
   unsigned total = 0;
   read(&total, start + table, sizeof(unsigned));
   ensure_entries(size_t(total) + 1024);
   tick(MMC_K_FILL);
-  k_fill<<<Pc, kBlock, 0, stream>>>(NS, S, box, cnt, start, entries, flags);
+  k_fill<<<Pc, kBlock, 0, stream>>>(inst_d, tc, box, cnt, start, entries, flags);
   tock(MMC_K_FILL);
   for (;;) {
-    k_pairs<T><<<Pc, kBlock, 0, stream>>>(Lt, S, x, box, start, entries, pcount, plist, flags);
+    k_pairs<T><<<Pc, kBlock, 0, stream>>>(Lt, inst_d, tc, slab, sinst, x, box, start, entries, pcount, plist, flags);
     int fl = 0;
     read(&fl, flags, sizeof(int));
     if (!(fl & 8)) break;
@@ -1890,9 +2128,9 @@ void mmc_ctx::build_list(Lat<T> Lt) {
     HIPCALL(hipMemcpy(flags, &cleared, sizeof(int), hipMemcpyHostToDevice));
     HIPCALL(hipFree(plist));
     plist = nullptr;
-    if (S.slab >= (1 << 16)) throw Fail(MMC_ERR_CAPACITY, "contact partner slab exceeds 65536 per shape");
-    S.slab *= 2;
-    plist = dalloc<int>(size_t(NS) * S.slab);
+    if (slab >= (1 << 16)) throw Fail(MMC_ERR_CAPACITY, "contact partner slab exceeds 65536 per shape");
+    slab *= 2;
+    plist = dalloc<int>(size_t(NS) * slab);
   }
   HIPCALL(hipMemcpyAsync(xref, x, 3 * size_t(N) * sizeof(double), hipMemcpyDeviceToDevice, stream));
   HIPCALL(hipGetLastError());
@@ -1900,7 +2138,7 @@ void mmc_ctx::build_list(Lat<T> Lt) {
   // any later replay; staged through pinned memory)
   ListRef* hl = reinterpret_cast<ListRef*>(host + kHost - 4);
   hl->plist = plist;
-  hl->slab = S.slab;
+  hl->slab = slab;
   hl->pad = 0;
   HIPCALL(hipMemcpyAsync(lref, hl, sizeof(ListRef), hipMemcpyHostToDevice, stream));
   HIPCALL(hipStreamSynchronize(stream));
@@ -1908,16 +2146,17 @@ void mmc_ctx::build_list(Lat<T> Lt) {
   rebuilds++;
 }
 
+// One objective evaluation of every instance at its platen (tops_d), records
+// into rec_out[inst * kRec] and the pinned hrec.
 template <class T>
 void mmc_ctx::evaluate_t(Lat<T> Lt, const double* zsrc, const double* psrc, double step, double* zdst, double* gdst,
-                         double top, bool trials, double* rec_out) {
-  S.top = top;
+                         bool trials, double* rec_out) {
   tick(MMC_K_STATE);
   const bool terms = terms_out0;
   terms_out0 = false;
-  k_state<T><<<Ps, kBlock, 0, stream>>>(Lt, S, zsrc, psrc, step, psrc ? zdst : nullptr, x, (double*)R,
-                                        list_valid ? xref : nullptr, TermBufs{nullptr, terms ? nbuf[0] : nullptr},
-                                        spart);
+  k_state<T><<<Ps, kBlock, 0, stream>>>(Lt, inst_d, tn, tops_d, zsrc, psrc, step, psrc ? zdst : nullptr, x,
+                                        (double*)R, list_valid ? xref : nullptr,
+                                        TermBufs{nullptr, terms ? nbuf[0] : nullptr}, spart);
   tock(MMC_K_STATE);
   bool built = false;
   if (!list_valid) {
@@ -1925,28 +2164,31 @@ void mmc_ctx::evaluate_t(Lat<T> Lt, const double* zsrc, const double* psrc, doub
     built = true;
   }
   tick(MMC_K_BEAM);
-  k_beam<T><<<Pb, kBlock, 0, stream>>>(Lt, M, x, (const double*)R, rec, trials ? trial : nullptr,
+  k_beam<T><<<Pb, kBlock, 0, stream>>>(Lt, inst_d, tb, x, (const double*)R, rec, trials ? trial : nullptr,
                                        TermBufs{nullptr, terms ? ubuf[0] : nullptr}, bpart);
   tock(MMC_K_BEAM);
   for (;;) {
     tick(MMC_K_CONTACT);
-    k_contact<T><<<Pc, kBlock, 0, stream>>>(Lt, S, x, pcount, plist, cf_sph, cf_cap,
+    k_contact<T><<<Pc, kBlock, 0, stream>>>(Lt, inst_d, tc, x, pcount, plist, slab, cf_sph, cf_cap,
                                             TermBufs{nullptr, terms ? cbuf[0] : nullptr}, cpart);
     tock(MMC_K_CONTACT);
     tick(MMC_K_GATHER);
-    k_gather<T><<<Ps, kBlock, 0, stream>>>(Lt, S, x, rec, cf_sph, cf_cap, spart, Ps, gdst, nullptr, gpart);
+    k_gather<T><<<Ps, kBlock, 0, stream>>>(Lt, inst_d, tn, tops_d, x, rec, cf_sph, cf_cap, spart, Ps, gdst, nullptr,
+                                           gpart);
     tock(MMC_K_GATHER);
     tick(MMC_K_FINAL);
-    k_final<<<1, kBlock, 0, stream>>>(N, cfg.centering_stiffness, n_free, spart, Ps, cpart, Pc, bpart, Pb, gpart, Ps,
-                                      flags, nullptr, rec_out);
+    k_final<<<n_inst, kBlock, 0, stream>>>(inst_d, spart, Ps, cpart, Pc, bpart, Pb, gpart, Ps, iflags(), nullptr,
+                                           rec_out);
     tock(MMC_K_FINAL);
     HIPCALL(hipGetLastError());
-    HIPCALL(hipMemcpyAsync(host, rec_out, kRec * sizeof(double), hipMemcpyDeviceToHost, stream));
+    HIPCALL(hipMemcpyAsync(hrec, rec_out, size_t(n_inst) * kRec * sizeof(double), hipMemcpyDeviceToHost, stream));
     HIPCALL(hipStreamSynchronize(stream));
     harvest();
     // a stale list (some node moved skin/2 or more since the build) cannot be
     // trusted: rebuild at these positions and redo the contact part
-    if (built || !(host[11] >= 0.5 * S.skin)) break;
+    bool stale = false;
+    for (int b = 0; b < n_inst; b++) stale = stale || hrec[b * kRec + 11] >= 0.5 * hinst[b].S.skin;
+    if (built || !stale) break;
     build_list<T>(Lt);
     built = true;
   }
@@ -1954,12 +2196,13 @@ void mmc_ctx::evaluate_t(Lat<T> Lt, const double* zsrc, const double* psrc, doub
   total_evaluations++;
 }
 
-void mmc_ctx::evaluate(const double* zsrc, const double* psrc, double step, double* zdst, double* gdst, double top,
-                       bool trials, double* rec_out) {
+void mmc_ctx::evaluate(const double* zsrc, const double* psrc, double step, double* zdst, double* gdst,
+                       const double* tops, bool trials, double* rec_out) {
+  set_tops(tops);
   if (precision == MMC_MIXED)
-    evaluate_t<float>(lat32(), zsrc, psrc, step, zdst, gdst, top, trials, rec_out);
+    evaluate_t<float>(lat32(), zsrc, psrc, step, zdst, gdst, trials, rec_out);
   else
-    evaluate_t<double>(lat64(), zsrc, psrc, step, zdst, gdst, top, trials, rec_out);
+    evaluate_t<double>(lat64(), zsrc, psrc, step, zdst, gdst, trials, rec_out);
 }
 
 namespace {
@@ -2000,6 +2243,32 @@ void aos_from_soa(const double* soa, double* aos, int n, int k) {
 void soa_from_aos(const double* aos, double* soa, int n, int k) {
   for (int i = 0; i < n; i++)
     for (int d = 0; d < k; d++) soa[size_t(d) * n + i] = aos[size_t(k) * i + d];
+}
+int create_ctx(const mmc_lattice* const* L, const mmc_config* cfgs, int32_t n, int32_t precision, mmc_ctx** out) {
+  if (!L || !cfgs || !out || n <= 0 || (precision != MMC_FP64 && precision != MMC_MIXED)) {
+    g_create_error = "mmc_create: invalid argument";
+    return MMC_ERR_ARG;
+  }
+  for (int b = 0; b < n; b++)
+    if (!L[b]) {
+      g_create_error = "mmc_create: NULL lattice";
+      return MMC_ERR_ARG;
+    }
+  *out = nullptr;
+  mmc_ctx* c = new mmc_ctx();
+  try {
+    c->setup(L, cfgs, n, precision);
+  } catch (const Fail& e) {
+    g_create_error = e.what();
+    delete c;
+    return e.code;
+  } catch (const std::exception& e) {
+    g_create_error = e.what();
+    delete c;
+    return MMC_ERR_HIP;
+  }
+  *out = c;
+  return MMC_OK;
 }
 }  // namespace
 
@@ -2051,47 +2320,44 @@ void mmc_config_defaults(mmc_config* c) {
 }
 
 int mmc_create(const mmc_lattice* L, const mmc_config* cfg, int32_t precision, mmc_ctx** out) {
-  if (!L || !cfg || !out || (precision != MMC_FP64 && precision != MMC_MIXED)) {
-    g_create_error = "mmc_create: invalid argument";
+  return create_ctx(&L, cfg, 1, precision, out);
+}
+
+int mmc_create_batch(const mmc_lattice* lattices, const mmc_config* cfgs, int32_t n, int32_t precision,
+                     mmc_ctx** out) {
+  if (!lattices || n <= 0) {
+    g_create_error = "mmc_create_batch: invalid argument";
     return MMC_ERR_ARG;
   }
-  *out = nullptr;
-  mmc_ctx* c = new mmc_ctx();
-  try {
-    c->setup(L, cfg, precision);
-  } catch (const Fail& e) {
-    g_create_error = e.what();
-    delete c;
-    return e.code;
-  } catch (const std::exception& e) {
-    g_create_error = e.what();
-    delete c;
-    return MMC_ERR_HIP;
-  }
-  *out = c;
-  return MMC_OK;
+  std::vector<const mmc_lattice*> L(n);
+  for (int b = 0; b < n; b++) L[b] = lattices + b;
+  return create_ctx(L.data(), cfgs, n, precision, out);
 }
 
 void mmc_destroy(mmc_ctx* c) { delete c; }
 
 const char* mmc_last_error(const mmc_ctx* c) { return c ? c->err.c_str() : g_create_error.c_str(); }
 
+int mmc_instances(const mmc_ctx* c) { return c ? c->n_inst : 0; }
+
 int mmc_objective(mmc_ctx* c, const double* zh, double top, int32_t trials, mmc_objective_out* out, double* grad) {
   return guarded(c, [&] {
+    c->need_single("mmc_objective");
     if (!zh) throw Fail(MMC_ERR_ARG, "z is NULL");
     const int N = c->N;
     std::vector<double> s(6 * size_t(N));
     soa_from_aos(zh, s.data(), N, 6);
     HIPCALL(hipMemcpyAsync(c->zin, s.data(), s.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    c->evaluate(c->zin, nullptr, 0.0, nullptr, c->gc, top, trials != 0, c->record + 2 * kRec);
-    fill_out(c->host, out);
+    c->evaluate(c->zin, nullptr, 0.0, nullptr, c->gc, &top, trials != 0, c->rec_slot(2));
+    fill_out(c->hrec, out);
+    const bool finite = c->hrec[9] != 0.0;
     if (trials) {
       HIPCALL(hipMemcpyAsync(c->zt, c->zin, s.size() * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
       c->have_trial = true;
     }
     if (grad) {
       c->read(s.data(), c->gc, s.size() * sizeof(double));
-      if (c->host[9] == 0.0)
+      if (!finite)
         std::fill(grad, grad + s.size(), 0.0);
       else
         aos_from_soa(s.data(), grad, N, 6);
@@ -2116,7 +2382,8 @@ bool use_host_loop() {
 }
 }  // namespace
 
-// kGraphIterations L-BFGS iterations as one graph of predicated kernels (see Ctl).
+// kGraphIterations replays of the phase machine as one graph (see Ctl); every
+// launch covers all instances, each block gated on its own instance's phase.
 constexpr int kGraphIterations = 4;
 template <class T>
 void mmc_ctx::capture_iteration(Lat<T> Lt) {
@@ -2124,113 +2391,134 @@ void mmc_ctx::capture_iteration(Lat<T> Lt) {
   if (graph) HIPCALL(hipGraphDestroy(graph));
   graph_exec = nullptr;
   graph = nullptr;
-  const int Pd = (N + kDirNodes - 1) / kDirNodes;
   const size_t n6 = 6 * size_t(N);
-  const double mt = cfg.max_translation_step / cfg.translation_scale, mr = cfg.max_rotation_step;
   HIPCALL(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
   for (int u = 0; u < kGraphIterations; u++) {
-  k_dir_g<<<Pd, kDirThreads, 0, stream>>>(ctl, N, n6, gbuf[0], gbuf[1], Sh, Yh, p, lock, mt, mr, vpart);
-  k_state_g<T><<<Ps, kBlock, 0, stream>>>(Lt, S, ctl, zbuf[0], zbuf[1], p, x, (double*)R, xref, nbuf[0], nbuf[1],
-                                          vpart, Pd, spart, ctl);
-  k_elem_g<T><<<Pb + Pc, kBlock, 0, stream>>>(ctl, Lt, M, S, x, (const double*)R, rec, ubuf[0], ubuf[1], bpart, Pb,
-                                              pcount, lref, cf_sph, cf_cap, cbuf[0], cbuf[1], cpart, Pc);
-  k_gather_g<T><<<Ps, kBlock, 0, stream>>>(Lt, S, ctl, x, rec, cf_sph, cf_cap, spart, Ps, zbuf[0], zbuf[1], gbuf[0],
-                                           gbuf[1], p, Sh, Yh, n6, gpart);
-  k_decide_g<<<1, kBlock, 0, stream>>>(ctl, N, cfg.centering_stiffness, n_free, spart, Ps, cpart, Pc, bpart, Pb,
-                                       gpart, Ps, flags);
+    k_dir_g<<<Pd, kDirThreads, 0, stream>>>(ctl, inst_d, td, N, n6, gbuf[0], gbuf[1], Sh, Yh, p, lock, vpart);
+    k_state_g<T><<<Ps, kBlock, 0, stream>>>(Lt, inst_d, tn, ctl, zbuf[0], zbuf[1], p, x, (double*)R, xref, nbuf[0],
+                                            nbuf[1], vpart, Pd, spart, ctl);
+    k_elem_g<T><<<Pb + Pc, kBlock, 0, stream>>>(ctl, Lt, inst_d, tb, tc, x, (const double*)R, rec, ubuf[0], ubuf[1],
+                                                bpart, Pb, pcount, lref, cf_sph, cf_cap, cbuf[0], cbuf[1], cpart, Pc);
+    k_gather_g<T><<<Ps, kBlock, 0, stream>>>(Lt, inst_d, tn, ctl, x, rec, cf_sph, cf_cap, spart, Ps, zbuf[0],
+                                             zbuf[1], gbuf[0], gbuf[1], p, Sh, Yh, n6, gpart);
+    k_decide_g<<<n_inst, kBlock, 0, stream>>>(ctl, inst_d, spart, Ps, cpart, Pc, bpart, Pb, gpart, Ps, iflags());
   }
   HIPCALL(hipStreamEndCapture(stream, &graph));
   HIPCALL(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0));
-
 }
 
-// Simulator::minimize, MC/src/sim.cpp:174-211, driven from the device: the host
-// replays the captured iteration in batches and reads the 16-word status head
-// of the control block after each batch.  It steps in only to rebuild a stale
-// contact list (status HOLD) and for the initial and final evaluations.
-void mmc_ctx::minimize_graph(double top, mmc_opt_stats& st) {
-  const int m = cfg.history;
+namespace {
+__global__ __launch_bounds__(kBlock) void k_release_hold(Ctl* __restrict__ Cs, int n) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b < n && Cs[b].status == ST_HOLD) Cs[b].status = ST_RUNNING;
+}
+}  // namespace
+
+// Simulator::minimize, MC/src/sim.cpp:174-211, for every active instance,
+// driven from the device: the host replays the captured graph in batches and
+// reads the 16-word status heads of the control blocks after each batch.  It
+// steps in only to rebuild a stale contact list (status HOLD) and for the
+// initial evaluation.  Inactive instances (active[b] == 0) are left alone.
+void mmc_ctx::minimize_graph(const double* tops, const int32_t* active, mmc_opt_stats* st) {
+  const int n = n_inst;
   zero(zbuf[0], long(6 * size_t(N)));
   terms_out0 = true;
-  evaluate(zbuf[0], nullptr, 0.0, nullptr, gbuf[0], top, false, record);
-  Ctl& h = *hctl;
-  std::memset(&h, 0, sizeof(Ctl));
-  std::memcpy(h.cur, host, kRec * sizeof(double));
-  h.m = m;
-  h.max_it = cfg.max_iterations;
-  h.max_ls = cfg.max_line_search;
-  h.rel_escape = cfg.relative_escape;
-  h.top = top;
-  h.armijo = cfg.armijo;
-  h.min_step = cfg.min_line_step;
-  h.tol_rms = cfg.gradient_rms_tolerance;
-  h.tol_max = cfg.gradient_max_tolerance;
-  h.rel_tol = cfg.relative_tolerance;
-  h.pen_tol = cfg.penetration_tolerance;
-  h.ts = cfg.translation_scale;
-  h.skin_half = 0.5 * S.skin;
-  h.ls_mode = cfg.line_search;
-  h.wolfe = precision == MMC_MIXED;
-  h.wolfe_eps = 1e-6;
-  h.phase = PH_DIRECTION;
-  h.coef.cg = 0.10;
-  if (cfg.max_iterations <= 0) {
-    h.status = ST_ITERCAP;
-  } else {
-    h.iterations = 1;
-    h.status = host_converged(cfg, h.cur) ? ST_CONVERGED : ST_RUNNING;
+  evaluate(zbuf[0], nullptr, 0.0, nullptr, gbuf[0], tops, false, rec_slot(0));
+  std::memset(hctl, 0, size_t(n) * sizeof(Ctl));
+  bool any = false;
+  for (int b = 0; b < n; b++) {
+    Ctl& h = hctl[b];
+    const mmc_config& cf = icfg[b];
+    std::memcpy(h.cur, hrec + b * kRec, kRec * sizeof(double));
+    h.m = cf.history;
+    h.max_it = cf.max_iterations;
+    h.max_ls = cf.max_line_search;
+    h.rel_escape = cf.relative_escape;
+    h.top = tops[b];
+    h.armijo = cf.armijo;
+    h.min_step = cf.min_line_step;
+    h.tol_rms = cf.gradient_rms_tolerance;
+    h.tol_max = cf.gradient_max_tolerance;
+    h.rel_tol = cf.relative_tolerance;
+    h.pen_tol = cf.penetration_tolerance;
+    h.ts = cf.translation_scale;
+    h.skin_half = 0.5 * hinst[b].S.skin;
+    h.ls_mode = cf.line_search;
+    h.wolfe = precision == MMC_MIXED;
+    h.wolfe_eps = 1e-6;
+    h.phase = PH_DIRECTION;
+    h.coef.cg = 0.10;
+    if (active && !active[b]) {
+      h.status = ST_CONVERGED;  // not part of this solve
+    } else if (cf.max_iterations <= 0) {
+      h.status = ST_ITERCAP;
+    } else {
+      h.iterations = 1;
+      h.status = host_converged(cf, h.cur) ? ST_CONVERGED : ST_RUNNING;
+    }
+    any = any || h.status == ST_RUNNING;
   }
-  HIPCALL(hipMemcpyAsync(ctl, &h, sizeof(Ctl), hipMemcpyHostToDevice, stream));
-  const bool mixed = precision == MMC_MIXED;
-  auto recapture = [&] {
-    if (mixed)
+  HIPCALL(hipMemcpyAsync(ctl, hctl, size_t(n) * sizeof(Ctl), hipMemcpyHostToDevice, stream));
+  if (!graph_exec) {
+    if (precision == MMC_MIXED)
       capture_iteration<float>(lat32());
     else
       capture_iteration<double>(lat64());
-  };
-  if (!graph_exec) recapture();
-  // every replay advances the phase machine by at least one step; bound the
-  // total so that a logic error cannot spin forever
-  const int64_t guard = int64_t(cfg.max_iterations + 1) * (cfg.max_line_search + 3) * 2 + 64;
+  }
+  // every replay advances each running phase machine by at least one step;
+  // bound the total so that a logic error cannot spin forever
+  int max_it = 0, max_ls = 0;
+  for (const mmc_config& cf : icfg) max_it = std::max(max_it, cf.max_iterations), max_ls = std::max(max_ls, cf.max_line_search);
+  const int64_t guard = int64_t(max_it + 1) * (max_ls + 3) * 2 + 64;
   int64_t replays = 0;
   int batch = 2;
-  HIPCALL(hipMemcpyAsync(&h, ctl, 16 * sizeof(int), hipMemcpyDeviceToHost, stream));
-  HIPCALL(hipStreamSynchronize(stream));
-  while (h.status == ST_RUNNING || h.status == ST_HOLD) {
-    if (h.status == ST_HOLD) {
-      // the trial's positions are still in x: rebuild there and redo the trial
-      if (mixed)
-        build_list<float>(lat32());
-      else
-        build_list<double>(lat64());
-      int* running = reinterpret_cast<int*>(host + kHost - 1);
-      *running = ST_RUNNING;
-      HIPCALL(hipMemcpyAsync(&ctl->status, running, sizeof(int), hipMemcpyHostToDevice, stream));
+  auto heads = [&] {
+    HIPCALL(hipMemcpy2DAsync(hctl, sizeof(Ctl), ctl, sizeof(Ctl), 16 * sizeof(int), n, hipMemcpyDeviceToHost,
+                             stream));
+    HIPCALL(hipStreamSynchronize(stream));
+    int running = 0, hold = 0;
+    for (int b = 0; b < n; b++) {
+      running += hctl[b].status == ST_RUNNING;
+      hold += hctl[b].status == ST_HOLD;
+    }
+    return std::make_pair(running, hold);
+  };
+  auto state = any ? heads() : std::make_pair(0, 0);
+  while (state.first + state.second > 0) {
+    if (state.second > 0) {
+      // the held trials' positions are still in x: rebuild there and redo them
+      rebuild();
+      k_release_hold<<<(n + kBlock - 1) / kBlock, kBlock, 0, stream>>>(ctl, n);
     }
     for (int r = 0; r < batch; r++) HIPCALL(hipGraphLaunch(graph_exec, stream));
     replays += int64_t(batch) * kGraphIterations;
     graph_replays += batch;
     batch = std::min(8, batch * 2);
-    HIPCALL(hipMemcpyAsync(&h, ctl, 16 * sizeof(int), hipMemcpyDeviceToHost, stream));
-    HIPCALL(hipStreamSynchronize(stream));
+    state = heads();
     if (replays > guard) throw Fail(MMC_ERR_STATE, "device minimiser exceeded its replay bound");
   }
-  HIPCALL(hipMemcpyAsync(&h, ctl, sizeof(Ctl), hipMemcpyDeviceToHost, stream));
+  HIPCALL(hipMemcpyAsync(hctl, ctl, size_t(n) * sizeof(Ctl), hipMemcpyDeviceToHost, stream));
   HIPCALL(hipStreamSynchronize(stream));
-  st.iterations = h.iterations;
-  st.backtracks = h.backtracks;
-  st.converged = h.status == ST_CONVERGED;
-  evaluations += h.evals;
-  total_evaluations += h.evals;
-  // the current iterate; Armijo steps decrease the energy strictly, so the
-  // reference's best-iterate fallback (MC/src/sim.cpp:209) selects it (the
+  for (int b = 0; b < n; b++) {
+    const Ctl& h = hctl[b];
+    st[b] = mmc_opt_stats{};
+    st[b].iterations = h.iterations;
+    st[b].backtracks = h.backtracks;
+    st[b].converged = h.status == ST_CONVERGED;
+    st[b].evaluations = h.evals;
+    evaluations += h.evals;
+    total_evaluations += h.evals;
+  }
+  // the current iterates; Armijo steps decrease the energy strictly, so the
+  // reference's best-iterate fallback (MC/src/sim.cpp:209) selects them (the
   // approximate-Wolfe acceptances of the mixed path admit changes within the
   // energy's rounding noise, where "best" carries no information)
-  HIPCALL(hipMemcpyAsync(z, zbuf[h.par], 6 * size_t(N) * sizeof(double), hipMemcpyDeviceToDevice, stream));
+  k_take_current<<<Ps, kBlock, 0, stream>>>(tn, ctl, N, zbuf[0], zbuf[1], z);
+  HIPCALL(hipGetLastError());
 }
 
 // The same loop driven from the host (one synchronisation per trial): kept as
-// a cross-check of the device phase machine (MMC_HOST_LOOP=1).
+// a cross-check of the device phase machine (MMC_HOST_LOOP=1, one lattice).
 void mmc_ctx::minimize_host(double top, mmc_opt_stats& st) {
   const int P = Ps;
   const size_t n6 = 6 * size_t(N);
@@ -2238,8 +2526,8 @@ void mmc_ctx::minimize_host(double top, mmc_opt_stats& st) {
   const double mt = cf.max_translation_step / cf.translation_scale, mr = cf.max_rotation_step;
   zero(z, long(n6));
   double cur[kRec], cand[kRec];
-  evaluate(z, nullptr, 0.0, nullptr, g, top, false, record);
-  std::memcpy(cur, host, sizeof cur);
+  evaluate(z, nullptr, 0.0, nullptr, g, &top, false, rec_slot(0));
+  std::memcpy(cur, hrec, sizeof cur);
   const int m = cf.history;
   int count = 0, head = 0;
   static thread_local double SY[kMaxHist][kMaxHist], YY[kMaxHist][kMaxHist], uu[kMaxHist], vv[kMaxHist];
@@ -2256,8 +2544,8 @@ void mmc_ctx::minimize_host(double top, mmc_opt_stats& st) {
   auto line_search = [&](double dg) {
     double step = 1.0;
     for (int bt = 0; bt < cf.max_line_search; bt++) {
-      evaluate(z, p, step, zc, gc, top, false, record + kRec);
-      std::memcpy(cand, host, sizeof cand);
+      evaluate(z, p, step, zc, gc, &top, false, rec_slot(1));
+      std::memcpy(cand, hrec, sizeof cand);
       st.backtracks++;
       if (cand[9] != 0.0 && cand[0] <= cur[0] + cf.armijo * step * dg) return true;
       step *= 0.5;
@@ -2301,6 +2589,23 @@ void mmc_ctx::minimize_host(double top, mmc_opt_stats& st) {
   }
 }
 
+// The reference's final objective(z, top, true) (MC/src/sim.cpp:210) at every
+// instance's iterate z: trial states for the commit, the reported record and
+// the convergence verdict on it.
+void mmc_ctx::final_evaluation(const double* tops, mmc_opt_stats* st, mmc_objective_out* fin) {
+  evaluate(z, nullptr, 0.0, nullptr, gc, tops, true, rec_slot(2));
+  HIPCALL(hipMemcpyAsync(zt, z, 6 * size_t(N) * sizeof(double), hipMemcpyDeviceToDevice, stream));
+  have_trial = true;
+  for (int b = 0; b < n_inst; b++) {
+    const double* f2 = hrec + b * kRec;
+    st[b].rms = f2[6];
+    st[b].max = f2[7];
+    st[b].energy = f2[0];
+    st[b].converged = host_converged(icfg[b], f2) ? 1 : 0;
+    if (fin) fill_out(f2, fin + b);
+  }
+}
+
 extern "C" {
 
 // Simulator::minimize, MC/src/sim.cpp:174-211.  Every vector stays on the
@@ -2308,27 +2613,19 @@ extern "C" {
 // iterate, which is left in z.
 int mmc_minimize(mmc_ctx* c, double top, mmc_opt_stats* stats, mmc_objective_out* fin, double* z_out) {
   return guarded(c, [&] {
+    c->need_single("mmc_minimize");
     const size_t n6 = 6 * size_t(c->N);
     c->evaluations = 0;
     mmc_opt_stats st{};
     if (use_host_loop()) {
       if (c->cfg.line_search != 0) throw Fail(MMC_ERR_ARG, "the host-driven loop supports line_search 0 only");
       c->minimize_host(top, st);
+    } else {
+      c->minimize_graph(&top, nullptr, &st);
     }
-    else
-      c->minimize_graph(top, st);
-    c->evaluate(c->z, nullptr, 0.0, nullptr, c->gc, top, true, c->record + 2 * kRec);
-    double f2[kRec];
-    std::memcpy(f2, c->host, sizeof f2);
-    HIPCALL(hipMemcpyAsync(c->zt, c->z, n6 * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-    c->have_trial = true;
-    st.rms = f2[6];
-    st.max = f2[7];
-    st.energy = f2[0];
-    st.converged = host_converged(c->cfg, f2) ? 1 : 0;
+    c->final_evaluation(&top, &st, fin);
     st.evaluations = c->evaluations;
     if (stats) *stats = st;
-    fill_out(f2, fin);
     if (z_out) {
       std::vector<double> s(n6);
       c->read(s.data(), c->z, n6 * sizeof(double));
@@ -2338,23 +2635,42 @@ int mmc_minimize(mmc_ctx* c, double top, mmc_opt_stats* stats, mmc_objective_out
   });
 }
 
-int mmc_commit(mmc_ctx* c) {
+// mmc_minimize for every active instance of a batch context at once.
+int mmc_minimize_batch(mmc_ctx* c, const double* tops, const int32_t* active, mmc_opt_stats* stats,
+                       mmc_objective_out* fin) {
+  return guarded(c, [&] {
+    if (!tops || !stats) throw Fail(MMC_ERR_ARG, "NULL argument");
+    c->evaluations = 0;
+    c->minimize_graph(tops, active, stats);
+    c->final_evaluation(tops, stats, fin);
+    HIPCALL(hipStreamSynchronize(c->stream));
+  });
+}
+
+// commit of the instances whose mask entry is set (all when mask is NULL)
+int mmc_commit_batch(mmc_ctx* c, const int32_t* mask) {
   return guarded(c, [&] {
     if (!c->have_trial) throw Fail(MMC_ERR_STATE, "missing constitutive trial states");
+    for (int b = 0; b < c->n_inst; b++) c->hmask[b] = mask ? (mask[b] != 0) : 1;
+    HIPCALL(hipMemcpyAsync(c->mask_d, c->hmask, c->n_inst * sizeof(int), hipMemcpyHostToDevice, c->stream));
     // recompute state(z) of the trial evaluation in FP64 (for either precision
     // policy the committed pose is FP64), then copy it into the base pose
-    k_state<double><<<c->Ps, kBlock, 0, c->stream>>>(c->lat64(), c->S, c->zt, nullptr, 0.0, nullptr, c->x, c->zin,
-                                                     nullptr, TermBufs{nullptr, nullptr}, c->spart);
-    k_commit_nodes<double><<<c->Ps, kBlock, 0, c->stream>>>(c->N, c->x, c->zin, c->bpos, c->bquat);
-    k_commit_beams<<<c->Pb, kBlock, 0, c->stream>>>(c->B, c->trial, c->plastic);
+    k_state<double><<<c->Ps, kBlock, 0, c->stream>>>(c->lat64(), c->inst_d, c->tn, c->tops_d, c->zt, nullptr, 0.0,
+                                                     nullptr, c->x, c->zin, nullptr, TermBufs{nullptr, nullptr},
+                                                     c->spart);
+    k_commit_nodes<<<c->Ps, kBlock, 0, c->stream>>>(c->tn, c->mask_d, c->N, c->x, c->zin, c->bpos, c->bquat);
+    k_commit_beams<<<c->Pb, kBlock, 0, c->stream>>>(c->tb, c->mask_d, c->B, c->trial, c->plastic);
     HIPCALL(hipGetLastError());
     HIPCALL(hipStreamSynchronize(c->stream));
     c->have_trial = false;
   });
 }
 
+int mmc_commit(mmc_ctx* c) { return mmc_commit_batch(c, nullptr); }
+
 int mmc_read_state(mmc_ctx* c, double* xh, double* qh, double* ph) {
   return guarded(c, [&] {
+    c->need_single("mmc_read_state");
     std::vector<double> s(7 * size_t(std::max(c->N, c->B)));
     if (xh) {
       c->read(s.data(), c->bpos, 3 * size_t(c->N) * sizeof(double));
@@ -2373,6 +2689,7 @@ int mmc_read_state(mmc_ctx* c, double* xh, double* qh, double* ph) {
 
 int mmc_read_trial(mmc_ctx* c, double* t) {
   return guarded(c, [&] {
+    c->need_single("mmc_read_trial");
     if (!t) throw Fail(MMC_ERR_ARG, "trial is NULL");
     std::vector<double> s(size_t(MMC_TRIAL_STRIDE) * c->B);
     c->read(s.data(), c->trial, s.size() * sizeof(double));
@@ -2380,23 +2697,34 @@ int mmc_read_trial(mmc_ctx* c, double* t) {
   });
 }
 
-int mmc_read_stats(mmc_ctx* c, mmc_state_stats* o) {
+// state statistics of every instance (out[n_inst])
+int mmc_read_stats_batch(mmc_ctx* c, mmc_state_stats* o) {
   return guarded(c, [&] {
-    int P = blocks(std::max(c->N, c->B));
-    double* part = c->stpart;
-    k_stats<<<P, kBlock, 0, c->stream>>>(c->N, c->B, c->plastic, c->bpos, part);
-    k_stats_final<<<1, kBlock, 0, c->stream>>>(part, P, c->scal);
-    c->read(c->host + 40, c->scal, 8 * sizeof(double));
-    const double* r = c->host + 40;
-    o->sum_axial = r[0];
-    o->max_axial = r[1];
-    o->sum_rot = r[2];
-    o->max_rot = r[3];
-    o->plastic_beams = int32_t(r[4]);
-    o->plastic_hinges = int32_t(r[5]);
-    o->y_min = -r[6];
-    o->y_max = r[7];
+    if (!o) throw Fail(MMC_ERR_ARG, "NULL argument");
+    k_stats_beams<<<c->Pb, kBlock, 0, c->stream>>>(c->tb, c->B, c->plastic, c->stb);
+    k_stats_nodes<<<c->Ps, kBlock, 0, c->stream>>>(c->tn, c->N, c->bpos, c->stn);
+    k_stats_final<<<c->n_inst, kBlock, 0, c->stream>>>(c->inst_d, c->stb, c->Pb, c->stn, c->Ps, c->stout);
+    c->read(c->hstat, c->stout, size_t(c->n_inst) * 8 * sizeof(double));
+    for (int b = 0; b < c->n_inst; b++) {
+      const double* r = c->hstat + 8 * b;
+      o[b].sum_axial = r[0];
+      o[b].max_axial = r[1];
+      o[b].sum_rot = r[2];
+      o[b].max_rot = r[3];
+      o[b].plastic_beams = int32_t(r[4]);
+      o[b].plastic_hinges = int32_t(r[5]);
+      o[b].y_min = -r[6];
+      o[b].y_max = r[7];
+    }
   });
+}
+
+int mmc_read_stats(mmc_ctx* c, mmc_state_stats* o) {
+  if (c && c->n_inst != 1) {
+    c->err = "mmc_read_stats needs a single-lattice context";
+    return MMC_ERR_ARG;
+  }
+  return mmc_read_stats_batch(c, o);
 }
 
 int mmc_kernel_timer(mmc_ctx* c, int32_t kernel) {
@@ -2436,20 +2764,22 @@ int mmc_time_kernel(mmc_ctx* c, int32_t kernel, int32_t launches, double* mean_m
       for (int i = 0; i < launches; i++) {
         switch (kernel) {
           case MMC_K_BEAM:
-            k_beam<T><<<c->Pb, kBlock, 0, c->stream>>>(Lt, c->M, c->x, R, c->rec, nullptr, TermBufs{nullptr, nullptr},
-                                                       c->bpart);
+            k_beam<T><<<c->Pb, kBlock, 0, c->stream>>>(Lt, c->inst_d, c->tb, c->x, R, c->rec, nullptr,
+                                                       TermBufs{nullptr, nullptr}, c->bpart);
             break;
           case MMC_K_CONTACT:
-            k_contact<T><<<c->Pc, kBlock, 0, c->stream>>>(Lt, c->S, c->x, c->pcount, c->plist, c->cf_sph, c->cf_cap,
-                                                          TermBufs{nullptr, nullptr}, c->cpart);
+            k_contact<T><<<c->Pc, kBlock, 0, c->stream>>>(Lt, c->inst_d, c->tc, c->x, c->pcount, c->plist, c->slab,
+                                                          c->cf_sph, c->cf_cap, TermBufs{nullptr, nullptr},
+                                                          c->cpart);
             break;
           case MMC_K_GATHER:
-            k_gather<T><<<c->Ps, kBlock, 0, c->stream>>>(Lt, c->S, c->x, c->rec, c->cf_sph, c->cf_cap, c->spart,
-                                                          c->Ps, c->gc, nullptr, c->gpart);
+            k_gather<T><<<c->Ps, kBlock, 0, c->stream>>>(Lt, c->inst_d, c->tn, c->tops_d, c->x, c->rec, c->cf_sph,
+                                                         c->cf_cap, c->spart, c->Ps, c->gc, nullptr, c->gpart);
             break;
           case MMC_K_STATE:
-            k_state<T><<<c->Ps, kBlock, 0, c->stream>>>(Lt, c->S, c->z, nullptr, 0.0, nullptr, c->x, R, nullptr,
-                                                          TermBufs{nullptr, nullptr}, c->spart);
+            k_state<T><<<c->Ps, kBlock, 0, c->stream>>>(Lt, c->inst_d, c->tn, c->tops_d, c->z, nullptr, 0.0,
+                                                        nullptr, c->x, R, nullptr, TermBufs{nullptr, nullptr},
+                                                        c->spart);
             break;
           default:
             throw Fail(MMC_ERR_ARG, "kernel id cannot be timed in isolation");
@@ -2471,6 +2801,7 @@ int mmc_time_kernel(mmc_ctx* c, int32_t kernel, int32_t launches, double* mean_m
 
 int mmc_beam_potential(mmc_ctx* c, const double* xh, const double* qh, double* out) {
   return guarded(c, [&] {
+    c->need_single("mmc_beam_potential");
     if (!xh || !qh || !out) throw Fail(MMC_ERR_ARG, "NULL argument");
     const int N = c->N;
     std::vector<double> s(4 * size_t(N));
@@ -2479,7 +2810,8 @@ int mmc_beam_potential(mmc_ctx* c, const double* xh, const double* qh, double* o
     HIPCALL(hipStreamSynchronize(c->stream));
     soa_from_aos(qh, s.data(), N, 4);
     HIPCALL(hipMemcpyAsync(c->zin, s.data(), 4 * size_t(N) * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    k_beam_potential<double><<<c->Pb, kBlock, 0, c->stream>>>(c->lat64(), c->M, c->x, c->zin, c->bpart);
+    k_beam_potential<double><<<c->Pb, kBlock, 0, c->stream>>>(c->lat64(), c->inst_d, c->tb, c->x, c->zin,
+                                                              c->bpart);
     k_sum1<<<1, kBlock, 0, c->stream>>>(c->bpart, c->Pb, c->scal);
     c->read(c->host + 48, c->scal, sizeof(double));
     *out = c->host[48];

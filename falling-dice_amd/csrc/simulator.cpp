@@ -351,20 +351,24 @@ std::vector<Snapshot> Simulator::capture(double top, const std::vector<Vec3>* pr
 }
 
 // MC/src/sim.cpp:312-372; plastic and height reductions come from the device.
-FrameDiagnostics Simulator::diagnostics(double time, double top, bool loading, const mmc_objective_out& o,
-                                        const mmc_opt_stats& s) const {
+namespace {
+// MC/src/sim.cpp:312-372 from the final objective record, the optimiser
+// statistics and the device reductions over the committed state.
+FrameDiagnostics make_diagnostics(const HostLattice& lat, double time, double top, bool loading,
+                                  const mmc_objective_out& o, const mmc_opt_stats& s, const mmc_state_stats& st,
+                                  double reactionOffset, double loadingWork, double recoveredWork) {
   FrameDiagnostics d;
-  const double height = lat_.height;
+  const double height = lat.height;
   d.time = time;
   d.platenY = top;
-  d.displacement = lat_.top0 - top;
+  d.displacement = lat.top0 - top;
   d.engineeringStrain = d.displacement / std::max(1e-12, height);
-  d.reactionForce = std::max(0.0, o.reaction - reactionOffset_);
-  d.engineeringStress = d.reactionForce / std::max(1e-12, lat_.area_ref);
-  d.loadingWork = loadingWork_;
-  d.recoveredWork = recoveredWork_;
-  d.dissipatedWork = std::max(0.0, loadingWork_ - recoveredWork_);
-  d.specificEnergyAbsorption = d.dissipatedWork / std::max(1e-12, lat_.total_mass);
+  d.reactionForce = std::max(0.0, o.reaction - reactionOffset);
+  d.engineeringStress = d.reactionForce / std::max(1e-12, lat.area_ref);
+  d.loadingWork = loadingWork;
+  d.recoveredWork = recoveredWork;
+  d.dissipatedWork = std::max(0.0, loadingWork - recoveredWork);
+  d.specificEnergyAbsorption = d.dissipatedWork / std::max(1e-12, lat.total_mass);
   d.elasticEnergy = o.elastic;
   d.plasticDissipation = o.plastic;
   d.contactEnergy = o.contact;
@@ -375,10 +379,8 @@ FrameDiagnostics Simulator::diagnostics(double time, double top, bool loading, c
   d.iterations = s.iterations;
   d.converged = s.converged != 0;
   d.loading = loading;
-  mmc_state_stats st{};
-  check(mmc_read_stats(ctx_, &st));
-  const double B = double(lat_.n_beams);
-  if (lat_.n_beams > 0) {
+  const double B = double(lat.n_beams);
+  if (lat.n_beams > 0) {
     d.meanPlasticStrain = st.sum_axial / B;
     d.maxPlasticStrain = st.max_axial;
     d.meanPlasticRotation = st.sum_rot / B;
@@ -389,6 +391,14 @@ FrameDiagnostics Simulator::diagnostics(double time, double top, bool loading, c
   const double hcur = std::max(0.0, st.y_max - st.y_min);
   d.permanentSet = std::max(0.0, 1.0 - hcur / std::max(1e-12, height));
   return d;
+}
+}  // namespace
+
+FrameDiagnostics Simulator::diagnostics(double time, double top, bool loading, const mmc_objective_out& o,
+                                        const mmc_opt_stats& s) const {
+  mmc_state_stats st{};
+  check(mmc_read_stats(ctx_, &st));
+  return make_diagnostics(lat_, time, top, loading, o, s, st, reactionOffset_, loadingWork_, recoveredWork_);
 }
 
 // MC/src/sim.cpp:379-388
@@ -580,6 +590,156 @@ bool Simulator::gradientCheck(double* relativeError) const {
   return checked >= 24 && rel < 0.075;
 }
 
+// ---------------------------------------------------------------------------
+// BatchSimulator
+// ---------------------------------------------------------------------------
+BatchSimulator::BatchSimulator(std::vector<SimulationConfig> cfgs, int precision) : cfgs_(std::move(cfgs)) {
+  if (cfgs_.empty()) throw std::runtime_error("a batch needs at least one lattice");
+  const size_t n = cfgs_.size();
+  c_.resize(n);
+  lats_.resize(n);
+  std::vector<mmc_lattice> views(n);
+  for (size_t b = 0; b < n; b++) {
+    cfgs_[b].translationScale = cfgs_[b].cellSize;
+    c_[b] = to_c(cfgs_[b]);
+    lats_[b] = build_lattice(c_[b]);
+    views[b] = lats_[b].view();
+  }
+  int rc = mmc_create_batch(views.data(), c_.data(), int32_t(n), precision, &ctx_);
+  if (rc != MMC_OK) throw std::runtime_error(mmc_last_error(nullptr));
+}
+
+BatchSimulator::~BatchSimulator() { mmc_destroy(ctx_); }
+
+void BatchSimulator::check(int rc) const {
+  if (rc != MMC_OK) throw std::runtime_error(mmc_last_error(ctx_));
+}
+
+namespace {
+// One lattice's schedule (MC/src/sim.cpp:374-467) as an explicit state
+// machine: the recursion of advanceTo becomes a stack of pending targets.
+struct Schedule {
+  struct Target {
+    double top;
+    bool loading;
+    int refinement;
+  };
+  double loadingWork = 0, recoveredWork = 0, prevDisp = 0, prevForce = 0, reactionOffset = 0;
+  double time = 0, currentTop = 0;
+  bool first = true, done = false;
+  int cursor = 0;
+  std::vector<Target> stack;  // back() is solved next
+};
+}  // namespace
+
+std::vector<BatchSimulator::Outcome> BatchSimulator::run() {
+  const int n = instances();
+  const size_t nz = size_t(n);
+  std::vector<Outcome> out(nz);
+  std::vector<Schedule> sch(nz);
+  for (int b = 0; b < n; b++) sch[size_t(b)].currentTop = lats_[size_t(b)].top0;
+  std::vector<double> tops(nz);
+  std::vector<int32_t> active(nz), mask(nz);
+  std::vector<mmc_opt_stats> stats(nz);
+  std::vector<mmc_objective_out> fin(nz);
+  std::vector<mmc_state_stats> st(nz);
+  for (;;) {
+    int any = 0;
+    for (int b = 0; b < n; b++) {
+      Schedule& s = sch[size_t(b)];
+      const SimulationConfig& cf = cfgs_[size_t(b)];
+      const HostLattice& L = lats_[size_t(b)];
+      active[size_t(b)] = 0;
+      if (s.done) continue;
+      if (s.stack.empty()) {  // next scheduled increment, MC/src/sim.cpp:456-465
+        const int Ls = cf.loadSteps, Us = cf.unloadSteps;
+        if (s.cursor > Ls + Us) {
+          s.done = true;
+          continue;
+        }
+        const int k = s.cursor++;
+        const double low = L.top0 - cf.maxStrain * L.height;
+        auto topY = [&](double f, bool loading) {
+          const double t = std::max(0.0, std::min(1.0, f));
+          return loading ? L.top0 * (1.0 - t) + low * t : low * (1.0 - t) + L.top0 * t;
+        };
+        if (k == 0)
+          s.stack.push_back({L.top0, true, 0});
+        else if (k <= Ls)
+          s.stack.push_back({topY(double(k) / std::max(1, Ls), true), true, 0});
+        else
+          s.stack.push_back({topY(double(k - Ls) / std::max(1, Us), false), false, 0});
+      }
+      tops[size_t(b)] = s.stack.back().top;
+      active[size_t(b)] = 1;
+      any++;
+    }
+    if (!any) break;
+    rounds_++;
+    check(mmc_minimize_batch(ctx_, tops.data(), active.data(), stats.data(), fin.data()));
+    int commits = 0;
+    for (int b = 0; b < n; b++) {
+      mask[size_t(b)] = 0;
+      if (!active[size_t(b)]) continue;
+      Schedule& s = sch[size_t(b)];
+      const SimulationConfig& cf = cfgs_[size_t(b)];
+      const Schedule::Target t = s.stack.back();
+      const mmc_opt_stats& so = stats[size_t(b)];
+      const mmc_objective_out& fo = fin[size_t(b)];
+      const bool acceptable = so.converged && fo.max_pen <= cf.penetrationTolerance && std::isfinite(fo.energy);
+      if (acceptable) {
+        mask[size_t(b)] = 1;
+        commits++;
+      } else if (t.refinement < cf.maxAdaptiveRefinements && std::abs(t.top - s.currentTop) > 2.5e-5) {
+        // MC/src/sim.cpp:403-407: the midpoint first, then the target again
+        s.stack.pop_back();
+        s.stack.push_back({t.top, t.loading, t.refinement + 1});
+        s.stack.push_back({0.5 * (s.currentTop + t.top), t.loading, t.refinement + 1});
+      } else {
+        std::ostringstream message;
+        message << "quasi-static equilibrium failed at platen=" << t.top << " m, residual=" << so.rms
+                << ", penetration=" << fo.max_pen * 1000.0 << " mm after " << so.iterations << " iterations";
+        out[size_t(b)].ok = false;
+        out[size_t(b)].error = message.str();
+        s.done = true;
+      }
+    }
+    if (!commits) continue;
+    check(mmc_commit_batch(ctx_, mask.data()));
+    check(mmc_read_stats_batch(ctx_, st.data()));
+    for (int b = 0; b < n; b++) {
+      if (!mask[size_t(b)]) continue;
+      Schedule& s = sch[size_t(b)];
+      const SimulationConfig& cf = cfgs_[size_t(b)];
+      const HostLattice& L = lats_[size_t(b)];
+      const Schedule::Target t = s.stack.back();
+      s.stack.pop_back();
+      const mmc_objective_out& fo = fin[size_t(b)];
+      // work accounting, MC/src/sim.cpp:420-433
+      if (s.first) s.reactionOffset = fo.reaction;
+      const double correctedForce = std::max(0.0, fo.reaction - s.reactionOffset);
+      const double displacement = L.top0 - t.top;
+      if (!s.first) {
+        const double dd = displacement - s.prevDisp;
+        const double work = 0.5 * (s.prevForce + correctedForce) * dd;
+        if (dd >= 0.0)
+          s.loadingWork += std::max(0.0, work);
+        else
+          s.recoveredWork += std::max(0.0, -work);
+      }
+      s.prevDisp = displacement;
+      s.prevForce = correctedForce;
+      s.currentTop = t.top;
+      out[size_t(b)].result.diagnostics.push_back(make_diagnostics(L, s.time, t.top, t.loading, fo, stats[size_t(b)],
+                                                                   st[size_t(b)], s.reactionOffset, s.loadingWork,
+                                                                   s.recoveredWork));
+      s.first = false;
+      s.time += cf.stepTime;
+    }
+  }
+  return out;
+}
+
 }  // namespace meta
 
 // ===========================================================================
@@ -587,6 +747,10 @@ bool Simulator::gradientCheck(double* relativeError) const {
 // ===========================================================================
 struct mmc_sim {
   std::unique_ptr<meta::Simulator> sim;
+  std::string err;
+};
+struct mmc_batch {
+  std::unique_ptr<meta::BatchSimulator> sim;
   std::string err;
 };
 
@@ -737,6 +901,61 @@ int mmc_sim_gradient_check(mmc_sim* s, double* error, int32_t* pass) {
 }
 
 mmc_ctx* mmc_sim_context(mmc_sim* s) { return (s && s->sim) ? s->sim->context() : nullptr; }
+
+int mmc_batch_create(const mmc_config* cfgs, int32_t n, int32_t precision, mmc_batch** out) {
+  if (!cfgs || n <= 0 || !out) return MMC_ERR_ARG;
+  *out = nullptr;
+  auto* s = new mmc_batch();
+  try {
+    std::vector<meta::SimulationConfig> c(static_cast<size_t>(n));
+    for (int b = 0; b < n; b++) c[size_t(b)] = meta::from_c(cfgs[b]);
+    s->sim = std::make_unique<meta::BatchSimulator>(std::move(c), precision);
+  } catch (const std::exception& e) {
+    s->err = e.what();
+    g_sim_create_error = e.what();
+    *out = s;
+    return MMC_ERR_ARG;
+  }
+  *out = s;
+  return MMC_OK;
+}
+
+void mmc_batch_destroy(mmc_batch* s) { delete s; }
+
+const char* mmc_batch_last_error(const mmc_batch* s) { return s ? s->err.c_str() : g_sim_create_error.c_str(); }
+
+int mmc_batch_info(const mmc_batch* s, int32_t i, int32_t* nn, int32_t* nb, double* height, double* mass,
+                   double* area) {
+  if (!s || !s->sim || i < 0 || i >= s->sim->instances()) return MMC_ERR_ARG;
+  const meta::HostLattice& L = s->sim->lattice(i);
+  if (nn) *nn = L.n_nodes;
+  if (nb) *nb = L.n_beams;
+  if (height) *height = L.height;
+  if (mass) *mass = L.total_mass;
+  if (area) *area = L.area_ref;
+  return MMC_OK;
+}
+
+int mmc_batch_run(mmc_batch* s, mmc_frame* frames, int32_t cap, int32_t* n_frames, int32_t* status) {
+  if (!s || !s->sim) return MMC_ERR_ARG;
+  try {
+    auto res = s->sim->run();
+    for (size_t b = 0; b < res.size(); b++) {
+      const auto& d = res[b].result.diagnostics;
+      const int m = std::min<int>(cap, int(d.size()));
+      for (int i = 0; i < m && frames; i++) frames[b * size_t(cap) + size_t(i)] = to_frame(d[size_t(i)]);
+      if (n_frames) n_frames[b] = int32_t(d.size());
+      if (status) status[b] = res[b].ok ? MMC_OK : MMC_ERR_EQUILIBRIUM;
+      if (!res[b].ok && s->err.empty()) s->err = "lattice " + std::to_string(b) + ": " + res[b].error;
+    }
+    return MMC_OK;
+  } catch (const std::exception& e) {
+    s->err = e.what();
+    return MMC_ERR_HIP;
+  }
+}
+
+mmc_ctx* mmc_batch_context(mmc_batch* s) { return (s && s->sim) ? s->sim->context() : nullptr; }
 
 int mmc_lattice_build(const mmc_config* cfg, mmc_lattice* view, void** handle) {
   if (!cfg || !view || !handle) return MMC_ERR_ARG;

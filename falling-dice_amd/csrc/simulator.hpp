@@ -152,4 +152,37 @@ class Simulator {
                                const mmc_opt_stats& s) const;
 };
 
+// Many independent lattices solved together (no reference counterpart): one
+// batched device context, and per lattice the schedule of Simulator::run
+// (tare, load, unload, adaptive bisection, work accounting, diagnostics).
+// Every round solves the next target of every unfinished lattice in one
+// device minimisation; a lattice whose equilibrium fails stops with the
+// reference's error text while the others continue.
+class BatchSimulator {
+ public:
+  BatchSimulator(std::vector<SimulationConfig> cfgs, int precision);
+  ~BatchSimulator();
+  BatchSimulator(const BatchSimulator&) = delete;
+  BatchSimulator& operator=(const BatchSimulator&) = delete;
+
+  struct Outcome {
+    SimulationResult result;  // diagnostics only (no snapshots)
+    bool ok = true;
+    std::string error;
+  };
+  std::vector<Outcome> run();
+  int instances() const { return int(lats_.size()); }
+  const HostLattice& lattice(int i) const { return lats_[size_t(i)]; }
+  mmc_ctx* context() const { return ctx_; }
+  int64_t rounds() const { return rounds_; }
+
+ private:
+  std::vector<SimulationConfig> cfgs_;
+  std::vector<mmc_config> c_;
+  std::vector<HostLattice> lats_;
+  mmc_ctx* ctx_ = nullptr;
+  int64_t rounds_ = 0;
+  void check(int rc) const;
+};
+
 }  // namespace meta

@@ -22,6 +22,7 @@ HERE = Path(__file__).resolve().parent
 LIB_PATH = HERE / "libmmc.so"
 
 MMC_FP64, MMC_MIXED = 0, 1
+MMC_OK, MMC_ERR_ARG, MMC_ERR_HIP, MMC_ERR_STATE, MMC_ERR_EQUILIBRIUM, MMC_ERR_CAPACITY = 0, 1, 2, 3, 4, 5
 MMC_LATTICE_BCC, MMC_LATTICE_CUBIC = 0, 1
 TRIAL_STRIDE = 17
 
@@ -241,6 +242,15 @@ def lib():
         L.mmc_time_kernel.argtypes = [P, C.c_int32, C.c_int32, dp]
         L.mmc_evaluations.argtypes = [P]
         L.mmc_evaluations.restype = C.c_int64
+        L.mmc_batch_create.argtypes = [C.POINTER(Config), C.c_int32, C.c_int32, C.POINTER(P)]
+        L.mmc_batch_destroy.argtypes = [P]
+        L.mmc_batch_destroy.restype = None
+        L.mmc_batch_last_error.argtypes = [P]
+        L.mmc_batch_last_error.restype = C.c_char_p
+        L.mmc_batch_info.argtypes = [P, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32), dp, dp, dp]
+        L.mmc_batch_run.argtypes = [P, C.POINTER(Frame), C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
+        L.mmc_batch_context.argtypes = [P]
+        L.mmc_batch_context.restype = P
         L.mmc_lattice_build.argtypes = [C.POINTER(Config), C.POINTER(Lattice), C.POINTER(P)]
         L.mmc_lattice_free.argtypes = [P]
         L.mmc_lattice_free.restype = None
@@ -473,3 +483,54 @@ KERNELS = {"state": 0, "fill": 2, "contact": 4, "beam": 5, "gather": 6, "final":
 
 def device_name() -> str:
     return os.environ.get("HIP_VISIBLE_DEVICES", "0")
+
+
+class BatchSimulator:
+    """Many lattices solved together (mmc_batch_*): each with the schedule of
+    Simulator.run under its own config."""
+
+    def __init__(self, cfgs, precision: int = MMC_FP64):
+        L = lib()
+        n = len(cfgs)
+        arr = (Config * n)(*cfgs)
+        h = C.c_void_p()
+        rc = L.mmc_batch_create(arr, n, precision, C.byref(h))
+        if rc != 0:
+            msg = L.mmc_batch_last_error(h).decode() if h else L.mmc_batch_last_error(None).decode()
+            if h:
+                L.mmc_batch_destroy(h)
+            raise MMCError(msg)
+        self.h, self.n, self.cfgs = h, n, list(cfgs)
+        self.n_nodes, self.n_beams = [], []
+        for i in range(n):
+            nn, nb = C.c_int32(), C.c_int32()
+            hh, m, a = C.c_double(), C.c_double(), C.c_double()
+            L.mmc_batch_info(h, i, C.byref(nn), C.byref(nb), C.byref(hh), C.byref(m), C.byref(a))
+            self.n_nodes.append(nn.value)
+            self.n_beams.append(nb.value)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mmc_batch_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def context(self) -> int:
+        return lib().mmc_batch_context(self.h)
+
+    def evaluations(self) -> int:
+        return int(lib().mmc_evaluations(self.context()))
+
+    def run(self):
+        """Returns (frames per lattice, status per lattice, first error text)."""
+        cap = 4 * max(c.load_steps + c.unload_steps + 1 for c in self.cfgs) + 256
+        frames = (Frame * (cap * self.n))()
+        nf = (C.c_int32 * self.n)()
+        st = (C.c_int32 * self.n)()
+        rc = lib().mmc_batch_run(self.h, frames, cap, nf, st)
+        if rc != 0:
+            raise MMCError(lib().mmc_batch_last_error(self.h).decode())
+        raw = np.frombuffer(bytes(frames), dtype=FRAME_DTYPE)
+        out = [raw[b * cap: b * cap + min(cap, nf[b])].copy() for b in range(self.n)]
+        return out, list(st), lib().mmc_batch_last_error(self.h).decode()

@@ -205,6 +205,27 @@ int mmc_time_kernel(mmc_ctx* ctx, int32_t kernel, int32_t launches, double* mean
  * against the committed plastic state.  x is 3N, q is 4N (w,x,y,z). */
 int mmc_beam_potential(mmc_ctx* ctx, const double* x, const double* q, double* out);
 
+/* ==== batched contexts (no reference counterpart) ===========================
+ * Many independent lattices in one context: one concatenated SoA, every launch
+ * covering all of them, each instance with its own control block, platen,
+ * material and schedule constants.  Per-instance results are those of the
+ * single-lattice entry points above on that lattice alone. */
+
+/* n lattices with one config each (cfgs[i].history etc. may differ). */
+int mmc_create_batch(const mmc_lattice* lattices, const mmc_config* cfgs, int32_t n, int32_t precision,
+                     mmc_ctx** out);
+/* Number of lattices held by ctx (1 for mmc_create). */
+int mmc_instances(const mmc_ctx* ctx);
+/* mmc_minimize for every instance b with active[b] != 0 (all when active is
+ * NULL) at platen tops[b], from z = 0.  stats and final_out have n entries;
+ * inactive instances' entries describe their unchanged state. */
+int mmc_minimize_batch(mmc_ctx* ctx, const double* tops, const int32_t* active, mmc_opt_stats* stats,
+                       mmc_objective_out* final_out);
+/* mmc_commit for the instances with mask[b] != 0 (all when mask is NULL). */
+int mmc_commit_batch(mmc_ctx* ctx, const int32_t* mask);
+/* mmc_read_stats of every instance (out has n entries). */
+int mmc_read_stats_batch(mmc_ctx* ctx, mmc_state_stats* out);
+
 /* ==== host lattice construction (no device work) ========================== */
 /* Builds the lattice the facade would build for cfg (MC/src/sim.cpp:49-135:
  * build + distributeMass, or the simple cubic family) without touching a GPU.
@@ -243,6 +264,25 @@ int mmc_sim_rigid_rotation_test(mmc_sim* sim, double* error, int32_t* pass);
 int mmc_sim_gradient_check(mmc_sim* sim, double* error, int32_t* pass);
 /* The device context under the facade (for objective-level callers). */
 mmc_ctx* mmc_sim_context(mmc_sim* sim);
+
+/* ==== batched Simulator facade (no reference counterpart) ==================
+ * n lattices, each with the schedule of mmc_sim_run (its own config: lattice,
+ * material, radii, schedule), solved together on one device context. */
+typedef struct mmc_batch mmc_batch;
+int mmc_batch_create(const mmc_config* cfgs, int32_t n, int32_t precision, mmc_batch** out);
+void mmc_batch_destroy(mmc_batch* batch);
+/* First failure message (a lattice whose equilibrium failed), or the creation
+ * error when batch is NULL.  Never NULL. */
+const char* mmc_batch_last_error(const mmc_batch* batch);
+/* Lattice i: nodes, beams, initial height, mass, reference area. */
+int mmc_batch_info(const mmc_batch* batch, int32_t i, int32_t* n_nodes, int32_t* n_beams, double* height,
+                   double* mass, double* area);
+/* Runs every lattice's schedule.  Lattice b's frames go to frames[b * cap ..]
+ * (at most cap), n_frames[b] gets its frame count and status[b] MMC_OK or
+ * MMC_ERR_EQUILIBRIUM (that lattice stopped at a failed equilibrium, like
+ * mmc_sim_run; the others continue).  Returns MMC_OK unless the device fails. */
+int mmc_batch_run(mmc_batch* batch, mmc_frame* frames, int32_t cap, int32_t* n_frames, int32_t* status);
+mmc_ctx* mmc_batch_context(mmc_batch* batch);
 
 #ifdef __cplusplus
 }
