@@ -117,6 +117,104 @@ def curve_error(fa, fb):
             "permanent_set_abs": float(abs(fa["permanent_set"][n - 1] - fb["permanent_set"][n - 1]))}
 
 
+def splitmix64(v):
+    v = (v + 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
+    v = ((v ^ (v >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
+    v = ((v ^ (v >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
+    return v ^ (v >> 31)
+
+
+def batch_workload(kind, count, load, unload):
+    """Seeded lattice sets for the batched solver.
+      uneven: cubes of n^3 cells, n uniform in 4..16, linear radius grading with
+              endpoints uniform in 0.3..0.7 mm, 30% strain
+      sweep:  6^3 cells, E log-uniform 1..3 GPa, yield uniform 30..80 MPa,
+              hardening uniform 2..10% of E, 40% strain
+    Each lattice draws from its own stream seeded with splitmix64(seed ^ i)."""
+    import mmc
+    seed = 0x494E445553545259
+    cfgs = []
+    for i in range(count):
+        rng = np.random.default_rng(splitmix64(seed ^ i))
+        if kind == "uneven":
+            n = int(rng.integers(4, 17))
+            r0, r1 = rng.uniform(3e-4, 7e-4, 2)
+            c = mmc.cubic_config(n, n, n, r_bot=float(r0), r_top=float(r1), strain=0.30, load=load, unload=unload,
+                                 max_iterations=20000)
+        else:
+            E = float(np.exp(rng.uniform(np.log(1e9), np.log(3e9))))
+            c = mmc.cubic_config(6, 6, 6, strain=0.40, load=load, unload=unload, max_iterations=20000)
+            c = c.replace(young=E, yield_=float(rng.uniform(30e6, 80e6)), hardening=float(rng.uniform(0.02, 0.10)) * E)
+        cfgs.append(c)
+    return cfgs
+
+
+def run_batch(args, log):
+    """Batched workloads: lattices x load-steps per second over whole schedules."""
+    import mmc
+    kind = args.workload
+    count = args.lattices or (256 if kind == "uneven" else 1024)
+    cfgs = batch_workload(kind, count, args.load_steps, args.unload_steps)
+    prec = mmc.MMC_FP64 if args.precision == "fp64" else mmc.MMC_MIXED
+    t0 = time.perf_counter()
+    sim = mmc.BatchSimulator(cfgs, prec)
+    setup = time.perf_counter() - t0
+    log(f"batch of {count}: {sum(sim.n_nodes)} nodes, {sum(sim.n_beams)} beams (setup {setup:.1f} s)")
+    t0 = time.perf_counter()
+    frames, status, err = sim.run(verbose=True)
+    dt = time.perf_counter() - t0
+    steps = 1 + args.load_steps + args.unload_steps
+    ok = [b for b in range(count) if status[b] == 0]
+    out = {
+        "metric": "lattices x load-steps/sec (batched)", "value": round(len(ok) * steps / dt, 3),
+        "unit": "lattice-load-steps/s", "n_gpus": 1, "higher_is_better": True,
+        "designs_per_s": round(len(ok) / dt, 4), "seconds": round(dt, 2), "setup_s": round(setup, 2),
+        "dtype": "f64" if prec == 0 else "f32-element/f64-reduction", "data": "synthetic (seeded lattices)",
+        "config": {"workload": kind, "lattices": count, "nodes": int(sum(sim.n_nodes)),
+                   "beams": int(sum(sim.n_beams)), "increments": f"{args.load_steps}+{args.unload_steps} + tare"},
+        "completed": len(ok), "failed": count - len(ok), "first_failure": err or None,
+        "evaluations": sim.evaluations(),
+    }
+    if args.save_frames:
+        np.save(f"{args.save_frames}_{kind}_{args.precision}.npy", np.concatenate(frames), allow_pickle=False)
+    sim.close()
+    rng = np.random.default_rng(7)
+    if args.single_sample:
+        # the plain FP64 single-lattice GPU solver, one lattice after another
+        pick = sorted(rng.choice(ok, size=min(args.single_sample, len(ok)), replace=False).tolist())
+        t0 = time.perf_counter()
+        for b in pick:
+            mmc.Simulator(cfgs[b], mmc.MMC_FP64).run()
+        ts = time.perf_counter() - t0
+        single = len(pick) * steps / ts
+        out["fp64_single_lattice"] = {"lattices": pick, "seconds": round(ts, 2),
+                                      "value": round(single, 3), "unit": "lattice-load-steps/s"}
+        out["speedup_vs_fp64_single_lattice"] = round(out["value"] / single, 2)
+        log(f"single-lattice FP64 sample done ({ts:.1f} s)")
+    if args.oracle_sample:
+        # per-lattice dissipated-work error against the FP64 oracle on a seeded
+        # sample, and the oracle's own rate on it (one host core)
+        sys.path.insert(0, str(ROOT / "oracle"))
+        import py_oracle
+        pick = sorted(rng.choice(ok, size=min(args.oracle_sample, len(ok)), replace=False).tolist())
+        errs = []
+        t0 = time.perf_counter()
+        for b in pick:
+            fo = py_oracle.Solver(cfgs[b], "orc").run()
+            fg = frames[b]
+            if len(fo) == len(fg):
+                wo, wg = fo["dissipated_work"][-1], fg["dissipated_work"][-1]
+                errs.append(abs(wg - wo) / max(abs(wo), 1e-12))
+        tc = time.perf_counter() - t0
+        out["oracle_sample"] = {"lattices": pick, "max_rel_dissipated_work": max(errs) if errs else None,
+                                "compared": len(errs)}
+        out["cpu_baseline"] = {"value": round(len(pick) * steps / tc, 4), "unit": "lattice-load-steps/s",
+                               "cores": 1, "kind": "port",
+                               "sample": f"{len(pick)} lattices of the set, full schedules ({tc:.1f} s, one core)"}
+        log(f"oracle sample done ({tc:.1f} s)")
+    print(json.dumps(out))
+
+
 def cpu_baseline(cfg, budget_s):
     sys.path.insert(0, str(ROOT / "oracle"))
     import py_oracle
@@ -162,6 +260,12 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--save-frames", default="", help="write each leg's frames to <prefix>_<precision>.npy")
+    ap.add_argument("--workload", choices=["single", "uneven", "sweep"], default="single",
+                    help="single: the 32x32x16 lattice; uneven / sweep: batched lattice sets")
+    ap.add_argument("--lattices", type=int, default=0, help="batch size (default 256 uneven, 1024 sweep)")
+    ap.add_argument("--oracle-sample", type=int, default=0, help="batched: lattices checked against the oracle")
+    ap.add_argument("--single-sample", type=int, default=0,
+                    help="batched: lattices also run one by one on the FP64 single-lattice solver")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this solver has no multi-GPU path (pass --gpus 1)", file=sys.stderr)
@@ -171,6 +275,9 @@ def main():
     def log(msg):
         print(msg, file=sys.stderr, flush=True)
 
+    if args.workload != "single":
+        run_batch(args, log)
+        return
     prec = mmc.MMC_FP64 if args.precision == "fp64" else mmc.MMC_MIXED
     cfg, name = workload(args)
     sim, fr, warm, K, dt, evals = run_leg(cfg, prec, args.warmup, args.steps, log)

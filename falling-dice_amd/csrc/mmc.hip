@@ -2,20 +2,22 @@
 // equilibrium solver (the hot path of Simulator::objective/minimize/commit,
 // projects/metamaterial-compression/industry-grade/src/sim.cpp:136-212).
 //
-// One objective evaluation is five kernels on one stream:
-//   k_state   node DOFs -> pose (x, R), gravity, platen contact, centroid sums
-//   k_count / scan / k_fill / k_sort
-//             uniform-grid spatial hash of joint spheres and strut capsules
-//   k_narrow  per-shape narrowphase; each shape accumulates its own contact
-//             force over its candidate partners in a fixed order
-//   k_beam    per-beam axial/shear/bending/torsion energy, gradient record and
-//             trial plastic state
-//   k_gather  per-node deterministic assembly over the CSR incidence list,
-//             gradient pack, lock mask and convergence norms
-//   k_final   one block: fixed-order sum of every block partial -> record
-// L-BFGS vector algebra (two-loop recursion, clamp, pair update) runs in
-// fused thread-per-node kernels over the 6-plane DOF vectors; its scalars
-// travel between kernels as block partials, never through the host.
+// A context holds one or many independent lattices in one concatenated SoA;
+// every launch covers all of them through block tables (one lattice per
+// block).  One objective evaluation:
+//   k_state    node DOFs -> pose (x, R), gravity, platen contact, centroid sums
+//   k_beam     per-beam axial/shear/bending/torsion energy, gradient record and
+//              trial plastic state
+//   k_contact  per-shape narrowphase over the Verlet partner list; each shape
+//              accumulates its own contact force in a fixed order
+//   k_gather   per-node deterministic assembly over the CSR incidence list,
+//              gradient pack, lock mask and convergence norms
+//   k_final    one block per lattice: fixed-order sum of its block partials
+// The Verlet list is rebuilt (k_count / scan / k_fill / k_pairs on a uniform
+// spatial hash) only when some node has moved half the skin.  The L-BFGS loop
+// runs on the device as a phase machine of five predicated kernels per trial
+// (k_dir_g, k_state_g, k_elem_g, k_gather_g, k_decide_g), replayed from a
+// hipGraph; the host-driven loop (minimize_host) is its cross-check.
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>

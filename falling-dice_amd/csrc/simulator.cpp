@@ -4,6 +4,7 @@
 #include "simulator.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <functional>
@@ -632,8 +633,9 @@ struct Schedule {
 };
 }  // namespace
 
-std::vector<BatchSimulator::Outcome> BatchSimulator::run() {
+std::vector<BatchSimulator::Outcome> BatchSimulator::run(bool verbose) {
   const int n = instances();
+  auto last = std::chrono::steady_clock::now();
   const size_t nz = size_t(n);
   std::vector<Outcome> out(nz);
   std::vector<Schedule> sch(nz);
@@ -676,6 +678,13 @@ std::vector<BatchSimulator::Outcome> BatchSimulator::run() {
     }
     if (!any) break;
     rounds_++;
+    if (verbose && std::chrono::steady_clock::now() - last > std::chrono::seconds(10)) {
+      last = std::chrono::steady_clock::now();
+      size_t frames = 0;
+      for (const Outcome& o : out) frames += o.result.diagnostics.size();
+      std::cerr << "[batch] round " << rounds_ << ": " << any << " of " << n << " lattices running, " << frames
+                << " frames" << std::endl;
+    }
     check(mmc_minimize_batch(ctx_, tops.data(), active.data(), stats.data(), fin.data()));
     int commits = 0;
     for (int b = 0; b < n; b++) {
@@ -936,10 +945,11 @@ int mmc_batch_info(const mmc_batch* s, int32_t i, int32_t* nn, int32_t* nb, doub
   return MMC_OK;
 }
 
-int mmc_batch_run(mmc_batch* s, mmc_frame* frames, int32_t cap, int32_t* n_frames, int32_t* status) {
+int mmc_batch_run(mmc_batch* s, mmc_frame* frames, int32_t cap, int32_t* n_frames, int32_t* status,
+                  int32_t verbose) {
   if (!s || !s->sim) return MMC_ERR_ARG;
   try {
-    auto res = s->sim->run();
+    auto res = s->sim->run(verbose != 0);
     for (size_t b = 0; b < res.size(); b++) {
       const auto& d = res[b].result.diagnostics;
       const int m = std::min<int>(cap, int(d.size()));

@@ -170,7 +170,9 @@ class BatchSimulator {
     bool ok = true;
     std::string error;
   };
-  std::vector<Outcome> run();
+  // verbose: one progress line on stderr about every 10 s (rounds, lattices
+  // still running, frames so far)
+  std::vector<Outcome> run(bool verbose = false);
   int instances() const { return int(lats_.size()); }
   const HostLattice& lattice(int i) const { return lats_[size_t(i)]; }
   mmc_ctx* context() const { return ctx_; }

@@ -248,7 +248,8 @@ def lib():
         L.mmc_batch_last_error.argtypes = [P]
         L.mmc_batch_last_error.restype = C.c_char_p
         L.mmc_batch_info.argtypes = [P, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32), dp, dp, dp]
-        L.mmc_batch_run.argtypes = [P, C.POINTER(Frame), C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
+        L.mmc_batch_run.argtypes = [P, C.POINTER(Frame), C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                    C.c_int32]
         L.mmc_batch_context.argtypes = [P]
         L.mmc_batch_context.restype = P
         L.mmc_lattice_build.argtypes = [C.POINTER(Config), C.POINTER(Lattice), C.POINTER(P)]
@@ -522,13 +523,13 @@ class BatchSimulator:
     def evaluations(self) -> int:
         return int(lib().mmc_evaluations(self.context()))
 
-    def run(self):
+    def run(self, verbose=False):
         """Returns (frames per lattice, status per lattice, first error text)."""
         cap = 4 * max(c.load_steps + c.unload_steps + 1 for c in self.cfgs) + 256
         frames = (Frame * (cap * self.n))()
         nf = (C.c_int32 * self.n)()
         st = (C.c_int32 * self.n)()
-        rc = lib().mmc_batch_run(self.h, frames, cap, nf, st)
+        rc = lib().mmc_batch_run(self.h, frames, cap, nf, st, int(verbose))
         if rc != 0:
             raise MMCError(lib().mmc_batch_last_error(self.h).decode())
         raw = np.frombuffer(bytes(frames), dtype=FRAME_DTYPE)

@@ -280,8 +280,10 @@ int mmc_batch_info(const mmc_batch* batch, int32_t i, int32_t* n_nodes, int32_t*
 /* Runs every lattice's schedule.  Lattice b's frames go to frames[b * cap ..]
  * (at most cap), n_frames[b] gets its frame count and status[b] MMC_OK or
  * MMC_ERR_EQUILIBRIUM (that lattice stopped at a failed equilibrium, like
- * mmc_sim_run; the others continue).  Returns MMC_OK unless the device fails. */
-int mmc_batch_run(mmc_batch* batch, mmc_frame* frames, int32_t cap, int32_t* n_frames, int32_t* status);
+ * mmc_sim_run; the others continue).  With verbose != 0 a progress line goes
+ * to stderr about every 10 s.  Returns MMC_OK unless the device fails. */
+int mmc_batch_run(mmc_batch* batch, mmc_frame* frames, int32_t cap, int32_t* n_frames, int32_t* status,
+                  int32_t verbose);
 mmc_ctx* mmc_batch_context(mmc_batch* batch);
 
 #ifdef __cplusplus
