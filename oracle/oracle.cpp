@@ -1352,6 +1352,21 @@ double orc_objective(orc* h, const double* z, double top, int32_t trials, mmc_ob
   return e.energy;
 }
 
+// The line search's energy change from z0 to z1 in both forms: out[0] the
+// per-term sum (line_search 1), out[1] the difference of the totals.
+void orc_energy_change(orc* h, const double* z0, const double* z1, double top, double* out) {
+  const size_t n = h->nodes.size() * 6;
+  const int keep = h->cfg.line_search;
+  h->cfg.line_search = 1;
+  Eval a = h->objective(std::vector<double>(z0, z0 + n), top, false);
+  Eval b = h->objective(std::vector<double>(z1, z1 + n), top, false);
+  h->cfg.line_search = keep;
+  double acc = 0;
+  for (size_t i = 0; i < b.terms.size(); i++) acc += b.terms[i] - a.terms[i];
+  out[0] = acc + (b.ecen - a.ecen);
+  out[1] = b.energy - a.energy;
+}
+
 int orc_minimize(orc* h, double top, double* z, mmc_opt_stats* st, mmc_objective_out* fin) {
   std::vector<double> zz(z, z + h->nodes.size() * 6);
   Eval f;

@@ -38,6 +38,9 @@ int orc_minimize(orc* h, double top, double* z, mmc_opt_stats* st, mmc_objective
 int orc_commit(orc* h);
 int orc_run(orc* h, mmc_frame* frames, int32_t cap, int32_t* n_frames, int32_t verbose);
 double orc_beam_potential(orc* h, const double* x, const double* q);
+/* Energy change z0 -> z1 at platen top: out[0] summed per term (the
+ * line_search 1 form), out[1] as the difference of total energies. */
+void orc_energy_change(orc* h, const double* z0, const double* z1, double top, double* out);
 int orc_rigid_rotation_test(orc* h, double* err);
 int orc_gradient_check(orc* h, double* err);
 void orc_read_state(orc* h, double* x, double* q, double* plastic);

@@ -48,6 +48,8 @@ def _load(prefix: str):
     f("commit", [P])
     f("run", [P, C.POINTER(mmc.Frame), C.c_int32, i32p, C.c_int32])
     f("beam_potential", [P, dp, dp], C.c_double)
+    if prefix == "orc":
+        f("energy_change", [P, dp, dp, C.c_double, dp], None)
     f("rigid_rotation_test", [P, dp])
     f("gradient_check", [P, dp])
     f("read_state", [P, dp, dp, dp], None)
@@ -127,6 +129,13 @@ class Solver:
         if trials:
             return out, g, t.reshape(self.n_beams, mmc.TRIAL_STRIDE)
         return out, g
+
+    def energy_change(self, z0, z1, top):
+        """(per-term sum, total difference) of the energy change z0 -> z1."""
+        out = np.zeros(2)
+        self._fn("energy_change")(self.h, _dp(np.ascontiguousarray(z0, dtype=np.float64)),
+                                  _dp(np.ascontiguousarray(z1, dtype=np.float64)), top, _dp(out))
+        return out[0], out[1]
 
     def minimize(self, top, z=None):
         z = np.zeros(6 * self.n_nodes) if z is None else np.array(z, dtype=np.float64)

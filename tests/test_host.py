@@ -55,6 +55,19 @@ def test_defaults_match_header_defaults(mmc):
 
 def test_last_error_never_null(mmc):
     assert mmc.lib().mmc_last_error(None) is not None
+    assert mmc.lib().mmc_batch_last_error(None) is not None
+
+
+def test_batch_rejects_bad_arguments_before_device_work(mmc):
+    """Argument errors of the batched entry points come back as MMC_ERR_ARG
+    without any device call (the host checks run first)."""
+    lib = mmc.lib()
+    h = C.c_void_p()
+    assert lib.mmc_batch_create(None, 1, 0, C.byref(h)) == mmc.MMC_ERR_ARG
+    cfgs = (mmc.Config * 1)(mmc.cubic_config(2, 2, 2))
+    assert lib.mmc_batch_create(cfgs, 0, 0, C.byref(h)) == mmc.MMC_ERR_ARG
+    assert lib.mmc_batch_info(None, 0, None, None, None, None, None) == mmc.MMC_ERR_ARG
+    assert lib.mmc_batch_run(None, None, 0, None, None, 0) == mmc.MMC_ERR_ARG
 
 
 @pytest.mark.parametrize("name", ["bcc_quick", "bcc_222", "cubic_8x8x4", "cubic_jitter", "cubic_seeded"])

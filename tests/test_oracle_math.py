@@ -257,3 +257,31 @@ def test_contact_all_pairs(mmc, po):
     count += int((bottom > 0).sum())
     assert count > 10 and out.contacts == count
     assert out.contact == pytest.approx(energy, rel=1e-11)
+
+
+def test_per_term_energy_change_is_the_energy_difference(po):
+    """line_search 1 sums the energy change per beam, node and contact shape.
+    Pinned two ways: for a finite step it equals the difference of the total
+    energies to round-off; for a tiny translational step (where the gradient
+    is the exact derivative) it follows the first-order change g.dz far below
+    the round-off floor of the totals."""
+    import mmc
+    from conftest import seeded_z
+    cfg = mmc.cubic_config(4, 4, 3)
+    o = po.Solver(cfg, "orc")
+    lat = o.lattice()
+    y = lat.x.reshape(-1, 3)[:, 1]
+    jr = cfg.joint_scale * cfg.max_radius
+    top = o.top0 - 0.03 * o.height
+    z0 = seeded_z(o.n_nodes, 5, squash=0.6, y=y, jr=jr, ts=cfg.cell_size)
+    dz = np.random.default_rng(9).normal(0, 1e-3, (o.n_nodes, 6))
+    e0, g0 = o.objective(z0, top)[:2]
+    per, tot = o.energy_change(z0, z0 + dz.reshape(-1), top)
+    assert abs(per - tot) <= 1e-12 * abs(e0.energy)
+    dz[:, 3:] = 0.0  # translations: the gradient there is exact
+    dz = dz.reshape(-1)
+    h = 1e-9
+    per_h, tot_h = o.energy_change(z0, z0 + h * dz, top)
+    lin = h * float(g0 @ dz)
+    assert abs(per_h - lin) <= 2e-4 * abs(lin)  # measured 3.4e-5
+    assert abs(per_h - lin) * 5 < abs(tot_h - lin)  # total difference: 7e-4
