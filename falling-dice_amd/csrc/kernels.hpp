@@ -23,6 +23,8 @@ namespace mmc {
 
 constexpr int kBlock = 256;
 constexpr int kWave = 64;
+// element-parallel blocks over nodes: 6 waves (one per DOF) of 64 nodes
+constexpr int kElemNodes = kWave, kElemThreads = 6 * kElemNodes;
 
 struct Material {
   double E, G, sy, H, hb_frac, ht_frac, shear, density;
@@ -93,6 +95,13 @@ __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
   return v;
+}
+// v of lane `lane` (wave-uniform) in every lane: two scalar lane reads, no LDS
+__device__ __forceinline__ double read_lane(double v, int lane) {
+  const long long b = __double_as_longlong(v);
+  const unsigned lo = unsigned(__builtin_amdgcn_readlane(int(b), lane));
+  const unsigned hi = unsigned(__builtin_amdgcn_readlane(int(b >> 32), lane));
+  return __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo));
 }
 __device__ __forceinline__ double wave_max(double v) {
 #pragma unroll

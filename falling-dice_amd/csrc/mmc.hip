@@ -597,80 +597,193 @@ __global__ __launch_bounds__(kBlock) void k_beam(Lat<T> L, const Inst* __restric
 }
 
 
+constexpr int kMaxHist = 16;  // longest L-BFGS history of the device loop
+
 // ---------------------------------------------------------------------------
-// k_gather: per-node assembly + pack, MC/src/sim.cpp:139-143,164,168-172.
-// partials [3][P]: sum of squares and max |g| over unlocked DOFs, g.p (the
+// k_gather: per-node assembly + pack, MC/src/sim.cpp:139-143,164,168-172, in
+// element-parallel form: blocks of 6 waves over 64 nodes (the 64-node table),
+// wave d assembles DOF d of its block's nodes.  partials [3][P] over the
+// 64-node blocks: sum of squares and max |g| over unlocked DOFs, g.p (the
 // search direction's directional derivative at this point, when p is given).
 // ---------------------------------------------------------------------------
+constexpr int kIncUnroll = 8;  // incident beams whose loads are issued together
+constexpr int kGRows = 3 + 4 * kMaxHist + 5;  // gradient rows, then pair products
+
+// Gradient component d of node i (before the lock), with the per-component
+// arithmetic of the reference's assembly in a fixed order: incident beams in
+// CSR order (chord force with the side's sign, then the capsule contact force
+// of that side; rotational gradient of the side), the sphere contact force,
+// gravity, centering, the two platens.  Split in two: gather_load issues the
+// loads of the first kIncUnroll incident beams (indices, then values, every
+// load before the first use; an index past the node's list re-reads its last
+// entry and its value is not used), gather_sum adds them up.
+struct IncVals {
+  int e0, e1;
+  int code[kIncUnroll];
+  double val[kIncUnroll], cap[kIncUnroll];
+};
 template <class T>
-__device__ __forceinline__ void gather_body(Lat<T> L, const Inst& in, double top, const double* __restrict__ x,
-                                                   const double* __restrict__ rec, const double* __restrict__ cf_sph,
-                                                   const double* __restrict__ cf_cap,
-                                                   const double* __restrict__ spart, int Ps,
-                                                   double* __restrict__ g, const double* __restrict__ p,
-                                                   double* __restrict__ part, int i, bool on) {
-  const int N = L.N, B = L.B, P = gridDim.x;
-  const Scalars& S = in.S;
-  const int Ni = in.N;  // this lattice's node count (N is the plane stride)
-  double cen[2];
-  {
-    double t[2];
-    reduce_columns<2>(spart + 5 * Ps, Ps, in.pn0, in.pn1, t, 0u);
-    cen[0] = t[0] * (1.0 / Ni);
-    cen[1] = t[1] * (1.0 / Ni);
-  }
-  double v[3] = {0, 0, 0};
-  if (on) {
-    v3<double> gx{0, 0, 0}, gr{0, 0, 0};
-    for (int e = L.inc_off[i]; e < L.inc_off[i + 1]; e++) {
-      const int code = L.inc[e], k = code >> 1, side = code & 1;
-      v3<double> f{rec[k], rec[B + k], rec[2 * B + k]};
-      gx = side ? gx + f : gx - f;
-      const int o = 3 + 3 * side;
-      gr = gr + v3<double>{rec[o * B + k], rec[(o + 1) * B + k], rec[(o + 2) * B + k]};
-      const int c = 3 * side;
-      gx = gx + v3<double>{cf_cap[c * B + k], cf_cap[(c + 1) * B + k], cf_cap[(c + 2) * B + k]};
-    }
-    gx = gx + v3<double>{cf_sph[i], cf_sph[N + i], cf_sph[2 * N + i]};
-    gx.y += L.mass[i] * S.gravity;
-    gx.x += S.centering * cen[0] / Ni;
-    gx.z += S.centering * cen[1] / Ni;
-    const double y = x[N + i];
-    double pb = S.jr - y;
-    if (pb > 0) gx.y -= S.kc * pb;
-    double pt = y + S.jr - top;
-    if (pt > 0) gx.y += S.kc * pt;
-    double gv[6] = {S.ts * gx.x, S.ts * gx.y, S.ts * gx.z, gr.x, gr.y, gr.z};
-    const unsigned lk = L.lock[i];
+__device__ __forceinline__ void gather_load(Lat<T> L, const double* __restrict__ rec,
+                                            const double* __restrict__ cf_cap, int d, int i, IncVals& v) {
+  const int B = L.B;
+  v.e0 = L.inc_off[i];
+  v.e1 = L.inc_off[i + 1];
+  const int elast = max(v.e1 - 1, 0);
 #pragma unroll
-    for (int d = 0; d < 6; d++) {
-      double gd = ((lk >> d) & 1u) ? 0.0 : gv[d];
-      g[d * N + i] = gd;
-      if (!((lk >> d) & 1u)) {
-        v[0] += gd * gd;
-        v[1] = fmax(v[1], fabs(gd));
-      }
-      if (p) v[2] += gd * p[d * N + i];
+  for (int u = 0; u < kIncUnroll; u++) v.code[u] = L.inc[min(v.e0 + u, elast)];
+#pragma unroll
+  for (int u = 0; u < kIncUnroll; u++) {
+    const int k = v.code[u] >> 1, side = v.code[u] & 1;
+    if (d < 3) {
+      v.val[u] = rec[size_t(d) * B + k];
+      v.cap[u] = cf_cap[size_t(3 * side + d) * B + k];
+    } else {
+      v.val[u] = rec[size_t(3 + 3 * side + (d - 3)) * B + k];
+      v.cap[u] = 0.0;
     }
-  }
-  block_reduce<3>(v, 1u << 1);
-  if (threadIdx.x == 0) {
-    part[blockIdx.x] = v[0];
-    part[P + blockIdx.x] = v[1];
-    part[2 * P + blockIdx.x] = v[2];
   }
 }
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_gather(Lat<T> L, const Inst* __restrict__ I, const Blk* __restrict__ tn,
-                                                   const double* __restrict__ tops, const double* __restrict__ x,
-                                                   const double* __restrict__ rec, const double* __restrict__ cf_sph,
-                                                   const double* __restrict__ cf_cap,
-                                                   const double* __restrict__ spart, int Ps,
-                                                   double* __restrict__ g, const double* __restrict__ p,
-                                                   double* __restrict__ part) {
-  const Blk bk = tn[blockIdx.x];
-  gather_body<T>(L, I[bk.inst], tops[bk.inst], x, rec, cf_sph, cf_cap, spart, Ps, g, p, part, bk.first + threadIdx.x,
-                 int(threadIdx.x) < bk.count);
+__device__ __forceinline__ double gather_sum(Lat<T> L, const Inst& in, double top, const double* __restrict__ x,
+                                             const double* __restrict__ rec, const double* __restrict__ cf_sph,
+                                             const double* __restrict__ cf_cap, double cen0, double cen1, int d,
+                                             int i, const IncVals& v) {
+  const int N = L.N, B = L.B;
+  const Scalars& S = in.S;
+  double gs = 0.0;
+#pragma unroll
+  for (int u = 0; u < kIncUnroll; u++) {
+    if (v.e0 + u < v.e1) {
+      if (d < 3) {
+        gs = (v.code[u] & 1) ? gs + v.val[u] : gs - v.val[u];
+        gs = gs + v.cap[u];
+      } else {
+        gs = gs + v.val[u];
+      }
+    }
+  }
+  for (int e = v.e0 + kIncUnroll; e < v.e1; e++) {
+    const int c = L.inc[e], k = c >> 1, side = c & 1;
+    if (d < 3) {
+      const double f = rec[size_t(d) * B + k];
+      gs = side ? gs + f : gs - f;
+      gs = gs + cf_cap[size_t(3 * side + d) * B + k];
+    } else {
+      gs = gs + rec[size_t(3 + 3 * side + (d - 3)) * B + k];
+    }
+  }
+  if (d >= 3) return gs;
+  gs = gs + cf_sph[size_t(d) * N + i];
+  if (d == 1) {
+    gs += L.mass[i] * S.gravity;
+    const double y = x[N + i];
+    const double pb = S.jr - y;
+    if (pb > 0) gs -= S.kc * pb;
+    const double pt = y + S.jr - top;
+    if (pt > 0) gs += S.kc * pt;
+  } else {
+    gs += S.centering * (d == 0 ? cen0 : cen1) / in.N;
+  }
+  return S.ts * gs;
+}
+
+// Node centroid (x, z) of the instance from the state kernel's partial rows
+// 5 and 6 (every block reduces them in the same order).
+template <int NT>
+__device__ __forceinline__ void centroid(const Inst& in, const double* __restrict__ spart, int Ps, double& c0,
+                                         double& c1) {
+  double t[2];
+  reduce_columns<2, NT>(spart + 5 * Ps, Ps, in.pn0, in.pn1, t, 0u);
+  c0 = t[0] * (1.0 / in.N);
+  c1 = t[1] * (1.0 / in.N);
+}
+
+// Block totals of NR per-thread rows (row 1 a maximum, the others sums) over
+// the NW waves of an element-parallel block; thread r < NR returns row r's
+// total.  Within a wave the first 64 rows are reduced by a butterfly that
+// halves the rows at every step (each lane keeps the half whose row bit
+// matches its lane bit and sends the other half across), so that lane l ends
+// with row l after 63 exchanges instead of 6 per row; rows past 64 are summed
+// one by one.  Fixed order throughout.
+// one step of that butterfly: rows [0, 2H) of a, exchanged with lane ^ H
+template <int H, int NA>
+__device__ __forceinline__ void halve_rows(double (&a)[NA], int lane) {
+  const bool up = lane & H;
+#pragma unroll
+  for (int r = 0; r < H; r++) {
+    const double send = up ? a[r] : a[r + H];
+    const double keep = up ? a[r + H] : a[r];
+    a[r] = keep + __shfl_xor(send, H, kWave);
+  }
+}
+template <int NW, int NA>
+__device__ __forceinline__ double block_rows(double (&a)[NA], int NR, double (*sh)[NA]) {
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  const double mx = wave_max(a[1]);
+  if constexpr (NA < kWave) {
+#pragma unroll
+    for (int r = 0; r < NA; r++) {
+      if (r < NR) {
+        const double s = r == 1 ? mx : wave_sum(a[r]);
+        if (lane == 0) sh[w][r] = s;
+      }
+    }
+  } else {
+    a[1] = 0.0;
+    halve_rows<32>(a, lane);
+    halve_rows<16>(a, lane);
+    halve_rows<8>(a, lane);
+    halve_rows<4>(a, lane);
+    halve_rows<2>(a, lane);
+    halve_rows<1>(a, lane);
+    sh[w][lane] = lane == 1 ? mx : a[0];
+#pragma unroll
+    for (int r = kWave; r < NA; r++) {
+      if (r < NR) {
+        const double s = wave_sum(a[r]);
+        if (lane == 0) sh[w][r] = s;
+      }
+    }
+  }
+  __syncthreads();
+  double tot = 0.0;
+  if (int(threadIdx.x) < NR) {
+    const int r = threadIdx.x;
+    tot = sh[0][r];
+#pragma unroll
+    for (int k = 1; k < NW; k++) tot = r == 1 ? fmax(tot, sh[k][r]) : tot + sh[k][r];
+  }
+  return tot;
+}
+
+template <class T>
+__global__ __launch_bounds__(kElemThreads) void k_gather(Lat<T> L, const Inst* __restrict__ I,
+                                                         const Blk* __restrict__ td, const double* __restrict__ tops,
+                                                         const double* __restrict__ x, const double* __restrict__ rec,
+                                                         const double* __restrict__ cf_sph,
+                                                         const double* __restrict__ cf_cap,
+                                                         const double* __restrict__ spart, int Ps,
+                                                         double* __restrict__ g, const double* __restrict__ p,
+                                                         double* __restrict__ part) {
+  const Blk bk = td[blockIdx.x];
+  const Inst& in = I[bk.inst];
+  const int d = threadIdx.x / kWave, l = threadIdx.x % kWave, i = bk.first + l;
+  const bool on = l < bk.count;
+  const int ic = on ? i : bk.first;
+  const size_t e = size_t(d) * L.N + ic;
+  const bool locked = (L.lock[ic] >> d) & 1u;
+  const double pv = p ? p[e] : 0.0;
+  IncVals iv;
+  gather_load<T>(L, rec, cf_cap, d, ic, iv);
+  double c0, c1;
+  centroid<kElemThreads>(in, spart, Ps, c0, c1);
+  const double gv = gather_sum<T>(L, in, tops[bk.inst], x, rec, cf_sph, cf_cap, c0, c1, d, ic, iv);
+  const double gd = locked ? 0.0 : gv;
+  if (on) g[e] = gd;
+  double a[4] = {on && !locked ? gd * gd : 0.0, on && !locked ? fabs(gd) : 0.0, on ? gd * pv : 0.0, 0.0};
+  __shared__ double sh[kElemThreads / kWave][4];
+  const double t = block_rows<kElemThreads / kWave>(a, 3, sh);
+  if (threadIdx.x < 3) part[size_t(threadIdx.x) * gridDim.x + blockIdx.x] = t;
 }
 
 
@@ -681,73 +794,138 @@ __global__ __launch_bounds__(kBlock) void k_gather(Lat<T> L, const Inst* __restr
 // 13 centering energy, 14 g.p
 constexpr int kRec = 16;
 
-// Reduces the partial columns of instance in (node blocks of spart and gpart,
-// shape blocks of cpart, beam blocks of bpart) into its record.
-__device__ __forceinline__ void final_body(const Inst& in, const double* __restrict__ spart, int Ps,
-                                                  const double* __restrict__ cpart, int Pc,
-                                                  const double* __restrict__ bpart, int Pb,
-                                                  const double* __restrict__ gpart, int Pg,
-                                                  int* __restrict__ iflag, const double* __restrict__ refrec,
-                                                  double* __restrict__ out) {
+// The record of instance in from its 20 reduced partial rows: 9 of the state
+// kernel, 4 of the contact shapes, 4 of the beams, 3 of the gradient (fl: the
+// instance's flag word).
+__device__ __forceinline__ void record_from_rows(const Inst& in, const double* __restrict__ rows, int fl,
+                                                 const double* __restrict__ refrec, double* __restrict__ out) {
   const int N = in.N;
   const double centering = in.S.centering, n_free = in.n_free;
-  // 20 partial rows in one pass: every thread accumulates its share of every
-  // row (independent loads), then one block reduction (fixed order)
-  constexpr int R = 20;
-  double acc[R];
-  const unsigned maxmask = (1u << 3) | (1u << 7) | (1u << 10) | (1u << 18);
-#pragma unroll
-  for (int r = 0; r < R; r++) acc[r] = ((maxmask >> r) & 1u) ? -INFINITY : 0.0;
-  for (int i = in.pn0 + threadIdx.x; i < in.pn1; i += kBlock) {
-#pragma unroll
-    for (int r = 0; r < 9; r++) acc[r] = (r == 3 || r == 7) ? fmax(acc[r], spart[r * Ps + i]) : acc[r] + spart[r * Ps + i];
-  }
-  for (int i = in.pc0 + threadIdx.x; i < in.pc1; i += kBlock) {
-#pragma unroll
-    for (int r = 0; r < 4; r++) acc[9 + r] = r == 1 ? fmax(acc[9 + r], cpart[r * Pc + i]) : acc[9 + r] + cpart[r * Pc + i];
-  }
-  for (int i = in.pb0 + threadIdx.x; i < in.pb1; i += kBlock) {
-#pragma unroll
-    for (int r = 0; r < 4; r++) acc[13 + r] += bpart[r * Pb + i];
-  }
-  for (int i = in.pn0 + threadIdx.x; i < in.pn1; i += kBlock) {
-    acc[17] += gpart[i];
-    acc[18] = fmax(acc[18], gpart[Pg + i]);
-    acc[19] += gpart[2 * Pg + i];
-  }
-  block_reduce<R>(acc, maxmask);
-  __shared__ double rows[R];
-  if (threadIdx.x == 0)
-#pragma unroll
-    for (int r = 0; r < R; r++) rows[r] = acc[r];
-  __syncthreads();
   const double* s = rows;
   const double* c = rows + 9;
   const double* b = rows + 13;
   const double* g = rows + 17;
+  double cx = s[5] * (1.0 / N), cz = s[6] * (1.0 / N);
+  double ecen = 0.5 * centering * (cx * cx + cz * cz);
+  double energy = b[0] + s[0] + ecen + c[0] + s[1];
+  bool fin = isfinite(energy) && !(fl & 1);
+  out[0] = fin ? energy : INFINITY;
+  out[1] = b[1];
+  out[2] = b[2];
+  out[3] = c[0] + s[1];
+  out[4] = s[2];
+  out[5] = fmax(s[3], c[1]);
+  out[6] = fin ? (n_free > 0 ? sqrt(g[0] / n_free) : 0.0) : 0.0;
+  out[7] = fin ? g[1] : 0.0;
+  out[8] = s[4] + c[2];
+  out[9] = fin ? 1.0 : 0.0;
+  out[10] = (double)fl;
+  out[11] = s[7];
+  out[12] = refrec ? (s[8] + c[3] + b[3]) + (ecen - refrec[13]) : 0.0;
+  out[13] = ecen;
+  out[14] = g[2];
+  out[15] = 0.0;
+}
+
+// Row pass of the record reductions: 20 record rows (9 of the state kernel,
+// 4 of the contact shapes, 4 of the beams, 3 of the gradient) and, with
+// history on, the 4 nold + 5 pair-product rows of the gather kernel.  Every
+// row is folded in one fixed order, whichever block and wave folds it: lane l
+// folds columns l, l + 64, ... in ascending order, then one thread folds the
+// 64 lane values in lane order.  The loads of the first kDecCols columns of
+// every row of a wave are issued together (one memory latency for instances
+// of up to kDecCols blocks per table).
+constexpr int kDecRows = 20 + 4 * kMaxHist + 5;
+constexpr int kDecWaves = kBlock / kWave;
+constexpr int kDecColRounds = 5, kDecCols = kDecColRounds * kWave;
+
+// Row j of an instance's partials: 0..8 state kernel, 9..12 contact shapes,
+// 13..16 beams, 17.. gradient rows (3, then the pair products); mx: folded by
+// maximum
+struct DecRow {
+  const double* p;
+  int n;
+  bool mx;
+};
+struct DecParts {
+  const double *spart, *cpart, *bpart, *gpart;
+  int Ps, Pc, Pb, Pg;
+};
+__device__ __forceinline__ bool row_max(int j) { return j == 3 || j == 7 || j == 10 || j == 18; }
+__device__ __forceinline__ DecRow decide_row(int j, const Inst& in, const DecParts& P) {
+  if (j < 9) return {P.spart + size_t(j) * P.Ps + in.pn0, in.pn1 - in.pn0, row_max(j)};
+  if (j < 13) return {P.cpart + size_t(j - 9) * P.Pc + in.pc0, in.pc1 - in.pc0, row_max(j)};
+  if (j < 17) return {P.bpart + size_t(j - 13) * P.Pb + in.pb0, in.pb1 - in.pb0, false};
+  return {P.gpart + size_t(j - 17) * P.Pg + in.pd0, in.pd1 - in.pd0, row_max(j)};
+}
+// Folds rows jbase + w + q js (w the wave, q < NQ) that lie in [0, R) into
+// out[j] (returns after a block barrier).  The first loads carry no branches
+// (an index past a row's end reads the row's last column, or the instance's
+// first state partial for a row outside [0, R) or an empty one, and the value
+// is replaced by the fold's identity), so that all of them are in flight
+// together.
+template <int NQ>
+__device__ __forceinline__ void rows_pass(const Inst& in, const DecParts& P, int R, int jbase, int js,
+                                          double (*lanes)[kWave + 1], double* out) {
+  const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
+  double x[NQ][kDecColRounds];
+#pragma unroll
+  for (int q = 0; q < NQ; q++) {
+    const int j = jbase + wv + q * js;
+    const DecRow rw = decide_row(j < R ? j : 0, in, P);
+    const bool any = j < R && rw.n > 0;
+    const double* base = any ? rw.p : P.spart + in.pn0;
+    const int last = any ? rw.n - 1 : 0;
+#pragma unroll
+    for (int u = 0; u < kDecColRounds; u++) x[q][u] = base[min(lane + u * kWave, last)];
+  }
+#pragma unroll
+  for (int q = 0; q < NQ; q++) {
+    const int j = jbase + wv + q * js;
+    const DecRow rw = decide_row(j < R ? j : 0, in, P);
+    const double id = rw.mx ? -INFINITY : 0.0;
+    double a = id;
+#pragma unroll
+    for (int u = 0; u < kDecColRounds; u++) {
+      const double e = (j < R && lane + u * kWave < rw.n) ? x[q][u] : id;
+      a = rw.mx ? fmax(a, e) : a + e;
+    }
+    if (j < R) {
+      for (int col = lane + kDecCols; col < rw.n; col += kWave) a = rw.mx ? fmax(a, rw.p[col]) : a + rw.p[col];
+      lanes[wv * NQ + q][lane] = a;
+    }
+  }
+  __syncthreads();
+  if (int(threadIdx.x) < kDecWaves * NQ) {
+    const int w = threadIdx.x / NQ, q = threadIdx.x % NQ, j = jbase + w + q * js;
+    if (j < R) {
+      const bool mx = row_max(j);
+      double a = lanes[threadIdx.x][0];
+      for (int k = 1; k < kWave; k++) a = mx ? fmax(a, lanes[threadIdx.x][k]) : a + lanes[threadIdx.x][k];
+      out[j] = a;
+    }
+  }
+  __syncthreads();
+}
+
+// Reduces the partial columns of instance in (node blocks of spart and gpart,
+// shape blocks of cpart, beam blocks of bpart) into its record, in the order
+// of the device loop's decision kernel.
+__device__ __forceinline__ void final_body(const Inst& in, const double* __restrict__ spart, int Ps,
+                                           const double* __restrict__ cpart, int Pc,
+                                           const double* __restrict__ bpart, int Pb,
+                                           const double* __restrict__ gpart, int Pg,
+                                           int* __restrict__ iflag, const double* __restrict__ refrec,
+                                           double* __restrict__ out) {
+  constexpr int NQ = 20 / kDecWaves;
+  __shared__ double lanes[kDecWaves * NQ][kWave + 1];
+  __shared__ double rows[20];
+  const DecParts P{spart, cpart, bpart, gpart, Ps, Pc, Pb, Pg};
+  rows_pass<NQ>(in, P, 20, 0, kDecWaves, lanes, rows);
   if (threadIdx.x == 0) {
     const int fl = iflag[0];
     iflag[0] = 0;
-    double cx = s[5] * (1.0 / N), cz = s[6] * (1.0 / N);
-    double ecen = 0.5 * centering * (cx * cx + cz * cz);
-    double energy = b[0] + s[0] + ecen + c[0] + s[1];
-    bool fin = isfinite(energy) && !(fl & 1);
-    out[0] = fin ? energy : INFINITY;
-    out[1] = b[1];
-    out[2] = b[2];
-    out[3] = c[0] + s[1];
-    out[4] = s[2];
-    out[5] = fmax(s[3], c[1]);
-    out[6] = fin ? (n_free > 0 ? sqrt(g[0] / n_free) : 0.0) : 0.0;
-    out[7] = fin ? g[1] : 0.0;
-    out[8] = s[4] + c[2];
-    out[9] = fin ? 1.0 : 0.0;
-    out[10] = (double)fl;
-    out[11] = s[7];
-    out[12] = refrec ? (s[8] + c[3] + b[3]) + (ecen - refrec[13]) : 0.0;
-    out[13] = ecen;
-    out[14] = g[2];
-    out[15] = 0.0;
+    record_from_rows(in, rows, fl, refrec, out);
   }
 }
 // one block per instance; records at out + inst * kRec
@@ -819,7 +997,6 @@ __global__ __launch_bounds__(kBlock) void k_sum1(const double* __restrict__ part
 // system solved from the pair inner products.  Two vector passes per iteration
 // (pair + dots, direction) replace the recursion's 2m dependent passes.
 // ---------------------------------------------------------------------------
-constexpr int kMaxHist = 16;
 struct DirCoef {
   double cg;
   int k;
@@ -897,7 +1074,7 @@ __device__ __forceinline__ void direction_body(int N, size_t n6, const double* _
 // Element-parallel form for the device loop: blocks of 6 waves, wave d holds
 // DOF plane d of 64 nodes; the per-node clamp reads the node's other planes
 // from LDS.  Same arithmetic per element as direction_body.
-constexpr int kDirNodes = kWave, kDirThreads = 6 * kDirNodes;
+constexpr int kDirNodes = kElemNodes, kDirThreads = kElemThreads;
 __device__ __forceinline__ void direction_elem(int N, size_t n6, const double* __restrict__ g,
                                                const double* __restrict__ Sh, const double* __restrict__ Yh,
                                                const DirCoef& C, double* __restrict__ p,
@@ -1077,49 +1254,48 @@ struct Ctl {
 // writes its (s, y) into the spare slot and computes its products with the
 // pairs that survive an acceptance (all of them, or all but the oldest when
 // the history is full), so that the products are ready when the trial is
-// accepted; a rejected trial's slot is simply overwritten by the next one.
-__host__ __device__ inline PairArgs pair_args_dev(const Ctl* c) {
-  PairArgs A{};
-  A.t = c->spare;
-  A.nold = 0;
-  for (int j = (c->count == c->m ? 1 : 0); j < c->count; j++) A.slot[A.nold++] = c->order[j];
-  return A;
-}
+// accepted (the surviving pairs are order[off..count), off = 1 when the
+// history is full); a rejected trial's slot is simply overwritten by the next
+// one.
 
 // MC/src/sim.cpp:205-206 on the device loop's slots: accept the new pair if
 // s.y > 1e-12 sqrt(s.s y.y), else clear the history.  d holds the products in
-// the layout of pair_dots_body.
-__device__ inline void history_update_dev(Ctl* c, const PairArgs& A, const double* d) {
-  const double* nw = d + 4 * A.nold;
+// the layout of pair_dots_body.  One wavefront (all 64 lanes call it, c in
+// LDS): lane jj writes the products of retained pair jj (the retained pairs
+// are order[off..count), off = 1 when the history is full and the oldest pair
+// is evicted); the new pair goes into the spare slot, and the next spare is
+// the evicted slot or the lowest free one.
+__device__ inline void history_update_wave(Ctl& c, const double* d) {
+  const int l = threadIdx.x & (kWave - 1);
+  const int off = c.count == c.m ? 1 : 0, nold = c.count - off, t = c.spare;
+  const double* nw = d + 4 * nold;
   const double sy = nw[0], ss = nw[1], yy = nw[2];
   if (!(sy > 1e-12 * sqrt(fmax(1e-30, ss * yy)))) {
-    c->count = 0;
+    if (l == 0) c.count = 0;
     return;
   }
-  const int t = A.t;
-  for (int jj = 0; jj < A.nold; jj++) {
-    const int j = A.slot[jj];
-    c->SY[j][t] = d[4 * jj + 0];
-    c->YY[j][t] = c->YY[t][j] = d[4 * jj + 1];
-    c->u[j] = d[4 * jj + 2];
-    c->v[j] = d[4 * jj + 3];
+  // every read of the old order precedes the first write (one wavefront:
+  // its LDS accesses complete in program order)
+  const int evicted = off ? c.order[0] : -1;
+  const int j = l < nold ? c.order[l + off] : 0;
+  if (l < nold) {
+    c.SY[j][t] = d[4 * l + 0];
+    c.YY[j][t] = c.YY[t][j] = d[4 * l + 1];
+    c.u[j] = d[4 * l + 2];
+    c.v[j] = d[4 * l + 3];
+    c.order[l] = j;
   }
-  c->SY[t][t] = sy;
-  c->YY[t][t] = yy;
-  c->u[t] = nw[3];
-  c->v[t] = nw[4];
-  const int evicted = c->count == c->m ? c->order[0] : -1;
-  for (int jj = 0; jj < A.nold; jj++) c->order[jj] = A.slot[jj];
-  c->order[A.nold] = t;
-  c->count = A.nold + 1;
-  if (evicted >= 0) {
-    c->spare = evicted;
-  } else {
-    unsigned used = 0;
-    for (int j = 0; j < c->count; j++) used |= 1u << c->order[j];
-    int f = 0;
-    while ((used >> f) & 1u) f++;
-    c->spare = f;
+  unsigned used = (l < nold ? 1u << j : 0u) | (l == 0 ? 1u << t : 0u);
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) used |= unsigned(__shfl_xor(int(used), o, kWave));
+  if (l == 0) {
+    c.SY[t][t] = sy;
+    c.YY[t][t] = yy;
+    c.u[t] = nw[3];
+    c.v[t] = nw[4];
+    c.order[nold] = t;
+    c.count = nold + 1;
+    c.spare = evicted >= 0 ? evicted : __builtin_ctz(~used);
   }
 }
 
@@ -1209,7 +1385,9 @@ __host__ __device__ inline PairArgs pair_slots(int m, int count, int head) {
 // compact_coefficients for one wavefront on the device loop's slots (all 64
 // lanes call it; lane l holds history row l, oldest first).  The two triangular solves run by columns:
 // each step broadcasts one solved component and every lane updates its own
-// residual, so the dependent chain is k steps instead of k(k+1)/2.
+// residual, so the dependent chain is k steps instead of k(k+1)/2.  Each lane
+// reads its rows of S'Y and Y'Y (and its column of S'Y) into registers first,
+// so a step is a division, a lane read and a multiply-add.
 __device__ void compact_coefficients_wave(const Ctl& c, DirCoef& C) {
   const int k = c.count, l = threadIdx.x & (kWave - 1);
   if (k == 0) {
@@ -1225,26 +1403,44 @@ __device__ void compact_coefficients_wave(const Ctl& c, DirCoef& C) {
   const double sy = c.SY[last][last], yy = c.YY[last][last];
   const double gamma = (sy > 1e-20 && yy > 1e-20) ? fmin(1e4, fmax(1e-7, sy / yy)) : 0.10;
   const double dl = on ? c.SY[sl][sl] : 1.0;
+  // order[] entries past count are stale but valid slot numbers
+  double syr[kMaxHist], syc[kMaxHist], yyr[kMaxHist];
+#pragma unroll
+  for (int i = 0; i < kMaxHist; i++) {
+    const int oi = c.order[i];
+    syr[i] = c.SY[sl][oi];
+    syc[i] = c.SY[oi][sl];
+    yyr[i] = c.YY[sl][oi];
+  }
   // w = R^{-1} u, R the upper triangle of S'Y (R_ij = s_i . y_j, i <= j)
   double r = on ? c.u[sl] : 0.0, w = 0.0;
-  for (int i = k - 1; i >= 0; i--) {
-    const double wi = __shfl(r / dl, i, kWave);
-    if (l == i) w = wi;
-    if (l < i) r -= c.SY[sl][c.order[i]] * wi;
+#pragma unroll
+  for (int i = kMaxHist - 1; i >= 0; i--) {
+    if (i < k) {
+      const double wi = read_lane(r / dl, i);
+      if (l == i) w = wi;
+      if (l < i) r -= syr[i] * wi;
+    }
   }
   // t = (D + gamma Y'Y) w - gamma v
   double t = on ? c.SY[sl][sl] * w - gamma * c.v[sl] : 0.0;
-  for (int j = 0; j < k; j++) {
-    const double wj = __shfl(w, j, kWave);
-    if (on) t += gamma * c.YY[sl][c.order[j]] * wj;
+#pragma unroll
+  for (int j = 0; j < kMaxHist; j++) {
+    if (j < k) {
+      const double wj = read_lane(w, j);
+      if (on) t += gamma * yyr[j] * wj;
+    }
   }
   // a = R^{-T} t
   double a = 0.0;
   r = t;
-  for (int i = 0; i < k; i++) {
-    const double ai = __shfl(r / dl, i, kWave);
-    if (l == i) a = ai;
-    if (on && l > i) r -= c.SY[c.order[i]][sl] * ai;
+#pragma unroll
+  for (int i = 0; i < kMaxHist; i++) {
+    if (i < k) {
+      const double ai = read_lane(r / dl, i);
+      if (l == i) a = ai;
+      if (on && l > i) r -= syc[i] * ai;
+    }
   }
   if (on) {
     C.slot[l] = sl;
@@ -1257,18 +1453,18 @@ __device__ void compact_coefficients_wave(const Ctl& c, DirCoef& C) {
   }
 }
 
-// The single-block kernels copy the control block to LDS with one coalesced
-// round trip, run their scalar logic there, and copy it back.
+// The decision kernel copies the control block to LDS (every load of the
+// copy issued before the first store, so that the copy costs one memory
+// latency), runs its scalar logic there, and copies it back.
 static_assert(sizeof(Ctl) % 8 == 0, "Ctl is copied as 8-byte words");
-__device__ __forceinline__ void ctl_load(const Ctl* C, Ctl& sc) {
-  const int W = int(sizeof(Ctl) / 8);
-  for (int e = threadIdx.x; e < W; e += kBlock) reinterpret_cast<double*>(&sc)[e] = reinterpret_cast<const double*>(C)[e];
-  __syncthreads();
-}
+constexpr int kCtlWords = int(sizeof(Ctl) / 8), kCtlRounds = (kCtlWords + kBlock - 1) / kBlock;
 __device__ __forceinline__ void ctl_store(Ctl* C, const Ctl& sc) {
   __syncthreads();
-  const int W = int(sizeof(Ctl) / 8);
-  for (int e = threadIdx.x; e < W; e += kBlock) reinterpret_cast<double*>(C)[e] = reinterpret_cast<const double*>(&sc)[e];
+#pragma unroll
+  for (int q = 0; q < kCtlRounds; q++) {
+    const int e = threadIdx.x + q * kBlock;
+    if (e < kCtlWords) reinterpret_cast<double*>(C)[e] = reinterpret_cast<const double*>(&sc)[e];
+  }
 }
 
 // A replay evaluates a trial right after a fresh direction (phase DIRECTION or
@@ -1377,73 +1573,174 @@ __global__ __launch_bounds__(kBlock) void k_elem_g(const Ctl* __restrict__ Cs, L
   }
 }
 
-// Gradient assembly of the trial (gather_body, one thread per node), then
+// Gradient assembly of the trial (element-parallel, as k_gather), then
 // (history on) the trial's curvature pair s = zc - z, y = gc - g written into
-// the spare history slot with all its products (partial rows 3.., layout of
-// pair_dots_body), computed by the thread that assembled the node's gradient.
+// the spare history slot with all its products (partial rows 3..: for each
+// retained pair j, oldest first: s_j.y, y_j.y, s_j.gc, y_j.gc; then s.y, s.s,
+// y.y, s.gc, y.gc), computed by the thread that assembled the element.  The
+// history elements are loaded before the assembly so that their latency
+// overlaps it.
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_gather_g(Lat<T> L, const Inst* __restrict__ I, const Blk* __restrict__ tn,
-                                                     const Ctl* __restrict__ Cs, const double* __restrict__ x,
-                                                     const double* __restrict__ rec,
-                                                     const double* __restrict__ cf_sph,
-                                                     const double* __restrict__ cf_cap,
-                                                     const double* __restrict__ spart, int Ps,
-                                                     const double* __restrict__ z0, const double* __restrict__ z1,
-                                                     double* __restrict__ g0, double* __restrict__ g1,
-                                                     const double* __restrict__ p, double* __restrict__ Sh,
-                                                     double* __restrict__ Yh, size_t n6, double* __restrict__ part) {
-  const Blk bk = tn[blockIdx.x];
+__global__ __launch_bounds__(kElemThreads) void k_gather_g(Lat<T> L, const Inst* __restrict__ I,
+                                                           const Blk* __restrict__ td, const Ctl* __restrict__ Cs,
+                                                           const double* __restrict__ x,
+                                                           const double* __restrict__ rec,
+                                                           const double* __restrict__ cf_sph,
+                                                           const double* __restrict__ cf_cap,
+                                                           const double* __restrict__ spart, int Ps,
+                                                           const double* __restrict__ z0, const double* __restrict__ z1,
+                                                           double* __restrict__ g0, double* __restrict__ g1,
+                                                           const double* __restrict__ p, double* __restrict__ Sh,
+                                                           double* __restrict__ Yh, size_t n6,
+                                                           double* __restrict__ part) {
+  const Blk bk = td[blockIdx.x];
   const Ctl* C = Cs + bk.inst;
   if (!ctl_trial(C)) return;
-  const int par = C->par;
+  const Inst& in = I[bk.inst];
+  const int d = threadIdx.x / kWave, l = threadIdx.x % kWave, i = bk.first + l;
+  const bool on = l < bk.count;
+  const int ic = on ? i : bk.first;
+  const size_t e = size_t(d) * L.N + ic;
+  const int par = C->par, m = C->m, count = C->count, spare = C->spare;
+  const int off = count == m ? 1 : 0, nold = m > 0 ? count - off : 0;
   double* gnew = par ? g0 : g1;
-  const int i = bk.first + threadIdx.x;
-  const bool on = int(threadIdx.x) < bk.count;
-  gather_body<T>(L, I[bk.inst], C->top, x, rec, cf_sph, cf_cap, spart, Ps, gnew, p, part, i, on);
-  if (C->m == 0) return;
-  const PairArgs A = pair_args_dev(C);
-  pair_dots_body(L.N, n6, par ? z1 : z0, par ? z0 : z1, par ? g1 : g0, gnew, Sh, Yh, A, part + 3 * size_t(gridDim.x),
-                 i, on);
+  const double* gcur = par ? g1 : g0;
+  const double* zcur = par ? z1 : z0;
+  const double* zcand = par ? z0 : z1;
+  double hs[kMaxHist], hy[kMaxHist];
+#pragma unroll
+  for (int jj = 0; jj < kMaxHist; jj++) {
+    hs[jj] = 0.0;
+    hy[jj] = 0.0;
+    if (jj < nold) {
+      const size_t o = size_t(C->order[jj + off]) * n6 + e;
+      hs[jj] = Sh[o];
+      hy[jj] = Yh[o];
+    }
+  }
+  const double zcv = zcand[e], zv = zcur[e], gov = gcur[e], pv = p[e];
+  const bool locked = (L.lock[ic] >> d) & 1u;
+  IncVals iv;
+  gather_load<T>(L, rec, cf_cap, d, ic, iv);
+  double c0, c1;
+  centroid<kElemThreads>(in, spart, Ps, c0, c1);
+  const double gv = gather_sum<T>(L, in, C->top, x, rec, cf_sph, cf_cap, c0, c1, d, ic, iv);
+  const double gd = locked ? 0.0 : gv;
+  if (on) gnew[e] = gd;
+  double a[kGRows];
+#pragma unroll
+  for (int r = 0; r < kGRows; r++) a[r] = 0.0;
+  a[0] = on && !locked ? gd * gd : 0.0;
+  a[1] = on && !locked ? fabs(gd) : 0.0;
+  a[2] = on ? gd * pv : 0.0;
+  int NR = 3;
+  if (m > 0) {
+    const double sv = on ? zcv - zv : 0.0, yv = on ? gd - gov : 0.0, gc = on ? gd : 0.0;
+    if (on) {
+      Sh[size_t(spare) * n6 + e] = sv;
+      Yh[size_t(spare) * n6 + e] = yv;
+    }
+#pragma unroll
+    for (int jj = 0; jj < kMaxHist; jj++) {
+      if (jj < nold) {
+        a[3 + 4 * jj + 0] = hs[jj] * yv;
+        a[3 + 4 * jj + 1] = hy[jj] * yv;
+        a[3 + 4 * jj + 2] = hs[jj] * gc;
+        a[3 + 4 * jj + 3] = hy[jj] * gc;
+      }
+    }
+    const double w[5] = {sv * yv, sv * sv, yv * yv, sv * gc, yv * gc};
+#pragma unroll
+    for (int jj = 0; jj <= kMaxHist; jj++)
+      if (jj == nold)
+#pragma unroll
+        for (int c = 0; c < 5; c++) a[3 + 4 * jj + c] = w[c];
+    NR = 3 + 4 * nold + 5;
+  }
+  __shared__ double sh[kElemThreads / kWave][kGRows];
+  const double t = block_rows<kElemThreads / kWave>(a, NR, sh);
+  if (int(threadIdx.x) < NR) part[size_t(threadIdx.x) * gridDim.x + blockIdx.x] = t;
 }
 
 // Candidate record and the Armijo decision (MC/src/sim.cpp:178-180,192-204);
 // on acceptance the curvature-pair update, the swap of current and candidate
 // buffers, the next convergence test and the next coefficients
 // (MC/src/sim.cpp:181-183,205-207).
-// one block per instance
+//
+// kDecSplit blocks per instance: each folds its share of the partial rows
+// (the record's and, with history on, the pair products) into drows, so that
+// the row loads are spread over several compute units; the block that
+// arrives last at the instance's counter (device-scope fences on both sides of
+// the count) reads the folded rows and the control block and takes the
+// decision, and resets the counter for the next trial.
+constexpr int kDecSplit = 8;
+constexpr int kDecNQ = (kDecRows + kDecSplit * kDecWaves - 1) / (kDecSplit * kDecWaves);
 __global__ __launch_bounds__(kBlock) void k_decide_g(Ctl* __restrict__ Cs, const Inst* __restrict__ I,
                                                      const double* __restrict__ spart, int Ps,
                                                      const double* __restrict__ cpart, int Pc,
                                                      const double* __restrict__ bpart, int Pb,
                                                      const double* __restrict__ gpart, int Pg,
-                                                     int* __restrict__ iflags) {
-  Ctl* C = Cs + blockIdx.x;
-  const Inst& in = I[blockIdx.x];
+                                                     int* __restrict__ iflags, double* drows, unsigned* dcount) {
+  const int b = blockIdx.x / kDecSplit, part = blockIdx.x % kDecSplit;
+  Ctl* C = Cs + b;
+  const Inst& in = I[b];
   if (!ctl_trial_phase(C)) return;
-  __shared__ Ctl sc;
-  __shared__ double d[4 * kMaxHist + 5];
-  __shared__ int accepted, need_coef;
-  ctl_load(C, sc);
-  if (sc.phase != PH_SEARCH) {
-    if (!sc.srch_eval) {  // not a descent direction: steepest descent next
-      if (threadIdx.x == 0) {
-        sc.count = 0;
-        sc.phase = PH_STEEPEST;
-      }
-      ctl_store(C, sc);
-      return;
+  if (C->phase != PH_SEARCH && !C->srch_eval) {
+    // not a descent direction (no trial was evaluated): steepest descent next
+    // (the other blocks of the instance take this branch too, before or after
+    // this write)
+    if (part == 0 && threadIdx.x == 0) {
+      C->count = 0;
+      C->phase = PH_STEEPEST;
     }
-    if (threadIdx.x == 0) {
-      sc.dg = sc.srch_dg;
-      sc.step = 1.0;
-      sc.bt = 0;
-      sc.phase = PH_SEARCH;
-    }
-    __syncthreads();
+    return;
   }
-  final_body(in, spart, Ps, cpart, Pc, bpart, Pb, gpart, Pg, iflags + blockIdx.x, sc.cur, sc.cand);
+  __shared__ double lanes[kDecWaves * kDecNQ][kWave + 1];
+  __shared__ int last;
+  {
+    const int m = C->m, count = C->count;
+    const int R = 20 + (m > 0 ? 4 * (count - (count == m ? 1 : 0)) + 5 : 0);
+    const DecParts P{spart, cpart, bpart, gpart, Ps, Pc, Pb, Pg};
+    rows_pass<kDecNQ>(in, P, R, part * kDecWaves, kDecSplit * kDecWaves, lanes, drows + size_t(b) * kDecRows);
+  }
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    last = atomicAdd(dcount + b, 1u) == unsigned(kDecSplit - 1);
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  // the control block and the folded rows in one round of loads
+  __shared__ Ctl sc;
+  __shared__ double rows[kDecRows];
+  __shared__ int accepted, need_coef;
+  double cr[kCtlRounds];
+#pragma unroll
+  for (int q = 0; q < kCtlRounds; q++)
+    cr[q] = reinterpret_cast<const double*>(C)[min(int(threadIdx.x) + q * kBlock, kCtlWords - 1)];
+  const double rv = drows[size_t(b) * kDecRows + min(int(threadIdx.x), kDecRows - 1)];
+  const int fl = iflags[b];
+  const int m = C->m;
+#pragma unroll
+  for (int q = 0; q < kCtlRounds; q++) {
+    const int e = threadIdx.x + q * kBlock;
+    if (e < kCtlWords) reinterpret_cast<double*>(&sc)[e] = cr[q];
+  }
+  if (int(threadIdx.x) < kDecRows) rows[threadIdx.x] = rv;
+  __syncthreads();
   if (threadIdx.x == 0) {
     Ctl* c = &sc;
+    if (c->phase != PH_SEARCH) {
+      c->dg = c->srch_dg;
+      c->step = 1.0;
+      c->bt = 0;
+      c->phase = PH_SEARCH;
+    }
+    iflags[b] = 0;
+    dcount[b] = 0;
+    record_from_rows(in, rows, fl, c->cur, c->cand);
     const double* r = c->cand;
     c->evals++;
     accepted = 0;
@@ -1479,22 +1776,11 @@ __global__ __launch_bounds__(kBlock) void k_decide_g(Ctl* __restrict__ Cs, const
   }
   __syncthreads();
   if (accepted) {
-    const PairArgs A = pair_args_dev(&sc);
-    const int m = sc.m;
-    if (m > 0) {
-      // pair products: rows 3.. of the gather partials, one wave per row
-      const int K = 4 * A.nold + 5, lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
-      for (int r = wv; r < K; r += kBlock / kWave) {
-        double a = 0.0;
-        for (int i = in.pn0 + lane; i < in.pn1; i += kWave) a += gpart[size_t(3 + r) * Pg + i];
-        a = wave_sum(a);
-        if (lane == 0) d[r] = a;
-      }
-    }
+    // pair update from the products in rows 20.. (one wavefront)
+    if (m > 0 && threadIdx.x < kWave) history_update_wave(sc, rows + 20);
     __syncthreads();
     if (threadIdx.x == 0) {
       Ctl* c = &sc;
-      if (m > 0) history_update_dev(c, A, d);
       c->par ^= 1;
       for (int k = 0; k < kRec; k++) c->cur[k] = c->cand[k];
       c->it++;
@@ -1707,6 +1993,8 @@ struct mmc_ctx {
   // instance's Ctl::par, and the graph of kGraphIterations replays
   Ctl* ctl = nullptr;
   Ctl* hctl = nullptr;
+  double* drows = nullptr;     // [n_inst][kDecRows] rows folded by the decision kernel's blocks
+  unsigned* dcount = nullptr;  // [n_inst] blocks of the decision kernel done with their rows
   double *zbuf[2] = {nullptr, nullptr}, *gbuf[2] = {nullptr, nullptr};
   // per-item energies (nodes, beams, shapes) at the current iterate and at the
   // candidate, for the per-term energy change of the line search
@@ -1796,7 +2084,7 @@ mmc_ctx::~mmc_ctx() {
   void* ptrs[] = {bpos, bquat, mass, lock, ba, bb, L0, rad, area, inert, e0, plastic, inc_off, inc, adj_off, adj, x, R,
                   rec, trial, cf_sph, cf_cap, box, cnt, start, entries, flags, scan_tmp, spart, cpart, bpart, gpart,
                   vpart, stb, stn, stout, record, scal, z, g, p, zc, gc, zt, zin, Sh, Yh, xref, pcount, plist,
-                  inst_d, tn, tb, tc, td, sinst, tops_d, mask_d, ctl, lref};
+                  inst_d, tn, tb, tc, td, sinst, tops_d, mask_d, ctl, lref, drows, dcount};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   void* pinned[] = {host, hctl, htops, hmask, hrec, hstat};
@@ -2056,7 +2344,7 @@ void mmc_ctx::setup(const mmc_lattice* const* Ls, const mmc_config* cs, int n, i
   spart = dalloc<double>(9 * size_t(Ps));
   cpart = dalloc<double>(4 * size_t(Pc));
   bpart = dalloc<double>(4 * size_t(Pb));
-  gpart = dalloc<double>((3 + 4 * size_t(kMaxHist) + 5) * size_t(Ps));  // gradient rows, then pair rows
+  gpart = dalloc<double>(size_t(kGRows) * size_t(Pd));  // gradient rows, then pair rows
   vpart = dalloc<double>((4 * size_t(kMaxHist) + 8) * size_t(std::max(Ps, Pd)));
   stb = dalloc<double>(6 * size_t(Pb));
   stn = dalloc<double>(2 * size_t(Ps));
@@ -2081,6 +2369,9 @@ void mmc_ctx::setup(const mmc_lattice* const* Ls, const mmc_config* cs, int n, i
   HIPCALL(hipHostMalloc((void**)&hrec, size_t(n) * kRec * sizeof(double), hipHostMallocDefault));
   HIPCALL(hipHostMalloc((void**)&hstat, size_t(n) * 8 * sizeof(double), hipHostMallocDefault));
   ctl = dalloc<Ctl>(n);
+  drows = dalloc<double>(size_t(n) * kDecRows);
+  dcount = dalloc<unsigned>(n);
+  HIPCALL(hipMemset(dcount, 0, size_t(n) * sizeof(unsigned)));
   lref = dalloc<ListRef>(1);
   tops_d = dalloc<double>(n);
   mask_d = dalloc<int>(n);
@@ -2175,11 +2466,11 @@ void mmc_ctx::evaluate_t(Lat<T> Lt, const double* zsrc, const double* psrc, doub
                                             TermBufs{nullptr, terms ? cbuf[0] : nullptr}, cpart);
     tock(MMC_K_CONTACT);
     tick(MMC_K_GATHER);
-    k_gather<T><<<Ps, kBlock, 0, stream>>>(Lt, inst_d, tn, tops_d, x, rec, cf_sph, cf_cap, spart, Ps, gdst, nullptr,
-                                           gpart);
+    k_gather<T><<<Pd, kElemThreads, 0, stream>>>(Lt, inst_d, td, tops_d, x, rec, cf_sph, cf_cap, spart, Ps, gdst,
+                                                 nullptr, gpart);
     tock(MMC_K_GATHER);
     tick(MMC_K_FINAL);
-    k_final<<<n_inst, kBlock, 0, stream>>>(inst_d, spart, Ps, cpart, Pc, bpart, Pb, gpart, Ps, iflags(), nullptr,
+    k_final<<<n_inst, kBlock, 0, stream>>>(inst_d, spart, Ps, cpart, Pc, bpart, Pb, gpart, Pd, iflags(), nullptr,
                                            rec_out);
     tock(MMC_K_FINAL);
     HIPCALL(hipGetLastError());
@@ -2401,9 +2692,10 @@ void mmc_ctx::capture_iteration(Lat<T> Lt) {
                                             nbuf[1], vpart, Pd, spart, ctl);
     k_elem_g<T><<<Pb + Pc, kBlock, 0, stream>>>(ctl, Lt, inst_d, tb, tc, x, (const double*)R, rec, ubuf[0], ubuf[1],
                                                 bpart, Pb, pcount, lref, cf_sph, cf_cap, cbuf[0], cbuf[1], cpart, Pc);
-    k_gather_g<T><<<Ps, kBlock, 0, stream>>>(Lt, inst_d, tn, ctl, x, rec, cf_sph, cf_cap, spart, Ps, zbuf[0],
-                                             zbuf[1], gbuf[0], gbuf[1], p, Sh, Yh, n6, gpart);
-    k_decide_g<<<n_inst, kBlock, 0, stream>>>(ctl, inst_d, spart, Ps, cpart, Pc, bpart, Pb, gpart, Ps, iflags());
+    k_gather_g<T><<<Pd, kElemThreads, 0, stream>>>(Lt, inst_d, td, ctl, x, rec, cf_sph, cf_cap, spart, Ps, zbuf[0],
+                                                   zbuf[1], gbuf[0], gbuf[1], p, Sh, Yh, n6, gpart);
+    k_decide_g<<<n_inst * kDecSplit, kBlock, 0, stream>>>(ctl, inst_d, spart, Ps, cpart, Pc, bpart, Pb, gpart, Pd,
+                                                           iflags(), drows, dcount);
   }
   HIPCALL(hipStreamEndCapture(stream, &graph));
   HIPCALL(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0));
@@ -2775,8 +3067,9 @@ int mmc_time_kernel(mmc_ctx* c, int32_t kernel, int32_t launches, double* mean_m
                                                           c->cpart);
             break;
           case MMC_K_GATHER:
-            k_gather<T><<<c->Ps, kBlock, 0, c->stream>>>(Lt, c->inst_d, c->tn, c->tops_d, c->x, c->rec, c->cf_sph,
-                                                         c->cf_cap, c->spart, c->Ps, c->gc, nullptr, c->gpart);
+            k_gather<T><<<c->Pd, kElemThreads, 0, c->stream>>>(Lt, c->inst_d, c->td, c->tops_d, c->x, c->rec,
+                                                               c->cf_sph, c->cf_cap, c->spart, c->Ps, c->gc, nullptr,
+                                                               c->gpart);
             break;
           case MMC_K_STATE:
             k_state<T><<<c->Ps, kBlock, 0, c->stream>>>(Lt, c->inst_d, c->tn, c->tops_d, c->z, nullptr, 0.0,
