@@ -51,6 +51,18 @@ BEAM_TRAFFIC_BYTES = {(32, 32, 16): 13.08e6}
 
 def workload(args):
     import mmc
+    if args.workload == "large":
+        # contact-heavy lattice: 64x64x32 cells, node positions jittered by a
+        # seeded Gaussian of 2% of the cell, per-beam radii uniform 0.3..0.7 mm,
+        # platen to 50% strain in 100 + 100 increments
+        cell = 0.005
+        cfg = mmc.cubic_config(args.nx, args.ny, args.nz, cell=cell, r_bot=3e-4, r_top=7e-4, strain=0.5,
+                               load=100, unload=100, grad_tol=args.grad_tol, max_iterations=args.max_iterations,
+                               sigma=0.02 * cell, radius_mode=1)
+        name = (f"cubic {args.nx}x{args.ny}x{args.nz} cells, cell 5 mm, node jitter N(0, (0.1 mm)^2), "
+                f"per-beam r U(0.3, 0.7) mm, E 2.0 GPa, yield 50 MPa, H 5%E, 50% strain, 100+100 increments, "
+                f"L-BFGS m=10, grad tol {args.grad_tol:g}, max {args.max_iterations} iterations")
+        return cfg, name
     cfg = mmc.cubic_config(args.nx, args.ny, args.nz, load=args.load_steps, unload=args.unload_steps,
                            grad_tol=args.grad_tol, strain=args.strain, max_iterations=args.max_iterations)
     cfg = cfg.replace(line_search=args.line_search)
@@ -100,17 +112,36 @@ def run_leg(cfg, prec, warmup, steps, log):
     return sim, fr, warm, K, dt, evals
 
 
+def first_instability(*legs, jump=4.0):
+    """First frame at which any leg's L-BFGS iteration count exceeds `jump`
+    times the largest count of its earlier frames (the tare, frame 0, aside).
+    Such a jump marks the loss of stability of the equilibrium path (the
+    Hessian turns near-singular where the lattice starts to buckle); from
+    there on the solution branch is selected by round-off, so two solvers
+    that round differently need not follow the same branch."""
+    first = None
+    for f in legs:
+        it = f["iterations"]
+        for k in range(2, len(it)):
+            if it[k] > jump * max(1, int(it[1:k].max())):
+                first = k if first is None else min(first, k)
+                break
+    return first
+
+
 def curve_error(fa, fb):
     n = min(len(fa), len(fb))
     if n == 0:
         return None
     peak = max(float(np.max(fb["reaction_force"][:n])), 1e-12)
     d = np.abs(fa["reaction_force"][:n] - fb["reaction_force"][:n]) / peak
-    # up to the peak force: before the lattice buckles, where the equilibrium
-    # path is unique; past it two solvers that round differently may settle
-    # on different post-buckling branches
     kp = int(np.argmax(fb["reaction_force"][:n]))
+    # frames before the first instability, where the equilibrium path is unique
+    ku = first_instability(fa[:n], fb[:n])
+    ku = n if ku is None else ku
     return {"frames": int(n), "max_rel_force": float(d.max()), "worst_frame": int(d.argmax()),
+            "unique_path_frames": int(ku), "max_rel_force_unique_path": float(d[:ku].max()),
+            "unique_path_strain": float(fb["engineering_strain"][ku - 1]),
             "peak_frame": kp, "max_rel_force_to_peak": float(d[:kp + 1].max()),
             "loading_work_rel": float(abs(fa["loading_work"][n - 1] - fb["loading_work"][n - 1])
                                       / max(abs(fb["loading_work"][n - 1]), 1e-12)),
@@ -243,6 +274,37 @@ def cpu_baseline(cfg, budget_s):
             "steps": steps, "seconds": round(dt, 2)}
 
 
+def cpu_baseline_window(cfg, budget_s, evals_window, steps_window):
+    """CPU rate for a workload whose increments take the oracle minutes each:
+    the oracle's L-BFGS loop on load increment 1, capped so that it runs for
+    about budget_s, gives seconds per objective evaluation (line-search trials
+    and the L-BFGS algebra included); the timed GPU window's evaluation count
+    turns that into the oracle's time for the same increments."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import py_oracle
+    probe = py_oracle.Solver(cfg, "orc")
+    z = np.zeros(6 * probe.n_nodes)
+    t0 = time.perf_counter()
+    probe.objective(z, probe.top0)
+    t_eval = time.perf_counter() - t0
+    probe.close()
+    cap = max(2, int(budget_s / max(t_eval, 1e-6)))
+    o = py_oracle.Solver(cfg.replace(max_iterations=cap), "orc")
+    top = o.top0 - cfg.max_strain * o.height / cfg.load_steps
+    t0 = time.perf_counter()
+    st, _, _ = o.minimize(top, np.zeros(6 * o.n_nodes))
+    dt = time.perf_counter() - t0
+    ev = max(1, int(st.evaluations))
+    per_eval = dt / ev
+    proj = per_eval * evals_window
+    return {"value": round(steps_window / proj, 6), "unit": "load-steps/s", "cores": 1, "kind": "port",
+            "sample": f"platen of load increment 1 from the undeformed lattice, first {st.iterations} L-BFGS "
+                      f"iterations ({ev} objective evaluations, "
+                      f"{dt:.1f} s on one host core, {per_eval * 1e3:.1f} ms per evaluation), scaled to the "
+                      f"{evals_window} evaluations of the timed increments",
+            "seconds_per_evaluation": round(per_eval, 5), "projected_seconds": round(proj, 1)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -264,8 +326,9 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--save-frames", default="", help="write each leg's frames to <prefix>_<precision>.npy")
-    ap.add_argument("--workload", choices=["single", "uneven", "sweep"], default="single",
-                    help="single: the 32x32x16 lattice; uneven / sweep: batched lattice sets")
+    ap.add_argument("--workload", choices=["single", "large", "uneven", "sweep"], default="single",
+                    help="single: the 32x32x16 lattice; large: the 64x64x32 contact-heavy lattice (a timed "
+                         "window of --steps increments, default 10); uneven / sweep: batched lattice sets")
     ap.add_argument("--lattices", type=int, default=0, help="batch size (default 256 uneven, 1024 sweep)")
     ap.add_argument("--oracle-sample", type=int, default=0, help="batched: lattices checked against the oracle")
     ap.add_argument("--single-sample", type=int, default=0,
@@ -279,9 +342,15 @@ def main():
     def log(msg):
         print(msg, file=sys.stderr, flush=True)
 
-    if args.workload != "single":
+    if args.workload in ("uneven", "sweep"):
         run_batch(args, log)
         return
+    if args.workload == "large":
+        if (args.nx, args.ny, args.nz) == (32, 32, 16):
+            args.nx, args.ny, args.nz = 64, 64, 32
+        args.load_steps, args.unload_steps = 100, 100
+        if args.steps == 0:
+            args.steps = 10
     prec = mmc.MMC_FP64 if args.precision == "fp64" else mmc.MMC_MIXED
     cfg, name = workload(args)
     sim, fr, warm, K, dt, evals = run_leg(cfg, prec, args.warmup, args.steps, log)
@@ -342,7 +411,10 @@ def main():
         if golden.exists() and full:
             fo = np.load(golden)
             out["force_error_vs_oracle"] = {"mixed": curve_error(mixed_fr, fo), "fp64": curve_error(fp64_fr, fo)}
-    if not args.no_cpu:
+    if not args.no_cpu and args.workload == "large":
+        log("cpu baseline ...")
+        out["cpu_baseline"] = cpu_baseline_window(cfg, args.cpu_budget, evals, K)
+    elif not args.no_cpu:
         log("cpu baseline ...")
         cb = cpu_baseline(cfg, args.cpu_budget)
         # the same increments on the GPU (headline leg), for a like-for-like ratio

@@ -15,9 +15,9 @@
 //   k_final    one block per lattice: fixed-order sum of its block partials
 // The Verlet list is rebuilt (k_count / scan / k_fill / k_pairs on a uniform
 // spatial hash) only when some node has moved half the skin.  The L-BFGS loop
-// runs on the device as a phase machine of five predicated kernels per trial
-// (k_dir_g, k_state_g, k_elem_g, k_gather_g, k_decide_g), replayed from a
-// hipGraph; the host-driven loop (minimize_host) is its cross-check.
+// runs on the device as a phase machine of four predicated kernels per trial
+// (k_dstate_g, k_elem_g, k_gather_g, k_decide_g), replayed from a hipGraph;
+// the host-driven loop (minimize_host) is its cross-check.
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
@@ -73,12 +73,73 @@ __device__ __forceinline__ quat<T> node_R(const T* R, int N, int i) {
 // sum_x, sum_z, max displacement from xref (the positions of the last contact
 // list build), sum of per-node energy changes against nt.ref
 // ---------------------------------------------------------------------------
+// The per-node inputs of state_node (base pose, mass, list reference
+// position), loaded ahead of the DOF values.
+struct NodeBase {
+  v3<double> p;
+  quat<double> q;
+  double mass;
+  v3<double> ref;
+};
+template <class T>
+__device__ __forceinline__ NodeBase node_base(Lat<T> L, const double* __restrict__ xref, int i) {
+  const int N = L.N;
+  NodeBase nb;
+  nb.p = v3<double>{L.bpos[i], L.bpos[N + i], L.bpos[2 * N + i]};
+  nb.q = quat<double>{L.bquat[i], L.bquat[N + i], L.bquat[2 * N + i], L.bquat[3 * N + i]};
+  nb.mass = L.mass[i];
+  nb.ref = xref ? v3<double>{xref[i], xref[N + i], xref[2 * N + i]} : v3<double>{0, 0, 0};
+  return nb;
+}
+// Node i at DOF values q: pose written to x, R; its gravity and platen terms,
+// centroid sums and displacement from xref in v[0..8] (the rows above).
+template <class T>
+__device__ __forceinline__ void state_node(Lat<T> L, const Scalars& S, const double (&q)[6], const NodeBase& nb,
+                                           double* __restrict__ x, double* __restrict__ R, bool has_ref,
+                                           TermBufs nt, int i, double (&v)[9]) {
+  const int N = L.N;
+  v3<double> pos{nb.p.x + q[0] * S.ts, nb.p.y + q[1] * S.ts, nb.p.z + q[2] * S.ts};
+  x[i] = pos.x;
+  x[N + i] = pos.y;
+  x[2 * N + i] = pos.z;
+  quat<double> r = qnormalize(qmul(qexp(v3<double>{q[3], q[4], q[5]}), nb.q));
+  R[i] = r.w;
+  R[N + i] = r.x;
+  R[2 * N + i] = r.y;
+  R[3 * N + i] = r.z;
+  const double m = nb.mass;
+  v[0] = m * S.gravity * pos.y;
+  double pb = S.jr - pos.y;
+  if (pb > 0) {
+    v[1] += 0.5 * S.kc * pb * pb;
+    v[3] = fmax(v[3], pb);
+    v[4] += 1;
+  }
+  double pt = pos.y + S.jr - S.top;
+  if (pt > 0) {
+    v[1] += 0.5 * S.kc * pt * pt;
+    v[2] += S.kc * pt;
+    v[3] = fmax(v[3], pt);
+    v[4] += 1;
+  }
+  v[5] = pos.x;
+  v[6] = pos.z;
+  if (has_ref) {
+    v3<double> d{pos.x - nb.ref.x, pos.y - nb.ref.y, pos.z - nb.ref.z};
+    double dn = norm(d);
+    v[7] = isfinite(dn) ? dn : INFINITY;
+  }
+  const double en = v[0] + v[1];
+  if (nt.out) nt.out[i] = en;
+  if (nt.ref) v[8] = en - nt.ref[i];
+}
+
 template <class T>
 __device__ __forceinline__ void state_body(Lat<T> L, Scalars S, const double* __restrict__ z,
-                                                  const double* __restrict__ p, double step, double* __restrict__ zc,
-                                                  double* __restrict__ x, double* __restrict__ R,
-                                                  const double* __restrict__ xref, TermBufs nt,
-                                                  double* __restrict__ part, int i, bool on) {
+                                           const double* __restrict__ p, double step, double* __restrict__ zc,
+                                           double* __restrict__ x, double* __restrict__ R,
+                                           const double* __restrict__ xref, TermBufs nt,
+                                           double* __restrict__ part, int i, bool on) {
   const int N = L.N, P = gridDim.x;
   double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   if (on) {
@@ -92,41 +153,7 @@ __device__ __forceinline__ void state_body(Lat<T> L, Scalars S, const double* __
       q[d] = zz;
       if (zc) zc[d * N + i] = zz;
     }
-    v3<double> pos{L.bpos[i] + q[0] * S.ts, L.bpos[N + i] + q[1] * S.ts, L.bpos[2 * N + i] + q[2] * S.ts};
-    x[i] = pos.x;
-    x[N + i] = pos.y;
-    x[2 * N + i] = pos.z;
-    quat<double> b{L.bquat[i], L.bquat[N + i], L.bquat[2 * N + i], L.bquat[3 * N + i]};
-    quat<double> r = qnormalize(qmul(qexp(v3<double>{q[3], q[4], q[5]}), b));
-    R[i] = r.w;
-    R[N + i] = r.x;
-    R[2 * N + i] = r.y;
-    R[3 * N + i] = r.z;
-    const double m = L.mass[i];
-    v[0] = m * S.gravity * pos.y;
-    double pb = S.jr - pos.y;
-    if (pb > 0) {
-      v[1] += 0.5 * S.kc * pb * pb;
-      v[3] = fmax(v[3], pb);
-      v[4] += 1;
-    }
-    double pt = pos.y + S.jr - S.top;
-    if (pt > 0) {
-      v[1] += 0.5 * S.kc * pt * pt;
-      v[2] += S.kc * pt;
-      v[3] = fmax(v[3], pt);
-      v[4] += 1;
-    }
-    v[5] = pos.x;
-    v[6] = pos.z;
-    if (xref) {
-      v3<double> d{pos.x - xref[i], pos.y - xref[N + i], pos.z - xref[2 * N + i]};
-      double dn = norm(d);
-      v[7] = isfinite(dn) ? dn : INFINITY;
-    }
-    const double en = v[0] + v[1];
-    if (nt.out) nt.out[i] = en;
-    if (nt.ref) v[8] = en - nt.ref[i];
+    state_node<T>(L, S, q, node_base(L, xref, i), x, R, xref != nullptr, nt, i, v);
   }
   block_reduce<9>(v, (1u << 3) | (1u << 7));
   if (threadIdx.x == 0)
@@ -835,7 +862,6 @@ __device__ __forceinline__ void record_from_rows(const Inst& in, const double* _
 // 64 lane values in lane order.  The loads of the first kDecCols columns of
 // every row of a wave are issued together (one memory latency for instances
 // of up to kDecCols blocks per table).
-constexpr int kDecRows = 20 + 4 * kMaxHist + 5;
 constexpr int kDecWaves = kBlock / kWave;
 constexpr int kDecColRounds = 5, kDecCols = kDecColRounds * kWave;
 
@@ -859,22 +885,22 @@ __device__ __forceinline__ DecRow decide_row(int j, const Inst& in, const DecPar
   return {P.gpart + size_t(j - 17) * P.Pg + in.pd0, in.pd1 - in.pd0, row_max(j)};
 }
 // Folds rows jbase + w + q js (w the wave, q < NQ) that lie in [0, R) into
-// out[j] (returns after a block barrier).  The first loads carry no branches
-// (an index past a row's end reads the row's last column, or the instance's
-// first state partial for a row outside [0, R) or an empty one, and the value
-// is replaced by the fold's identity), so that all of them are in flight
+// out[j] (returns after a block barrier); row(j) describes row j.  The first
+// loads carry no branches (an index past a row's end reads the row's last
+// column, or dummy for a row outside [0, R) or an empty one, and the value is
+// replaced by the fold's identity), so that all of them are in flight
 // together.
-template <int NQ>
-__device__ __forceinline__ void rows_pass(const Inst& in, const DecParts& P, int R, int jbase, int js,
+template <int NQ, class Row>
+__device__ __forceinline__ void rows_pass(Row row, const double* dummy, int R, int jbase, int js,
                                           double (*lanes)[kWave + 1], double* out) {
   const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
   double x[NQ][kDecColRounds];
 #pragma unroll
   for (int q = 0; q < NQ; q++) {
     const int j = jbase + wv + q * js;
-    const DecRow rw = decide_row(j < R ? j : 0, in, P);
+    const DecRow rw = row(j < R ? j : 0);
     const bool any = j < R && rw.n > 0;
-    const double* base = any ? rw.p : P.spart + in.pn0;
+    const double* base = any ? rw.p : dummy;
     const int last = any ? rw.n - 1 : 0;
 #pragma unroll
     for (int u = 0; u < kDecColRounds; u++) x[q][u] = base[min(lane + u * kWave, last)];
@@ -882,7 +908,7 @@ __device__ __forceinline__ void rows_pass(const Inst& in, const DecParts& P, int
 #pragma unroll
   for (int q = 0; q < NQ; q++) {
     const int j = jbase + wv + q * js;
-    const DecRow rw = decide_row(j < R ? j : 0, in, P);
+    const DecRow rw = row(j < R ? j : 0);
     const double id = rw.mx ? -INFINITY : 0.0;
     double a = id;
 #pragma unroll
@@ -921,7 +947,7 @@ __device__ __forceinline__ void final_body(const Inst& in, const double* __restr
   __shared__ double lanes[kDecWaves * NQ][kWave + 1];
   __shared__ double rows[20];
   const DecParts P{spart, cpart, bpart, gpart, Ps, Pc, Pb, Pg};
-  rows_pass<NQ>(in, P, 20, 0, kDecWaves, lanes, rows);
+  rows_pass<NQ>([&](int j) { return decide_row(j, in, P); }, spart + in.pn0, 20, 0, kDecWaves, lanes, rows);
   if (threadIdx.x == 0) {
     const int fl = iflag[0];
     iflag[0] = 0;
@@ -1073,30 +1099,36 @@ __device__ __forceinline__ void direction_body(int N, size_t n6, const double* _
 }
 // Element-parallel form for the device loop: blocks of 6 waves, wave d holds
 // DOF plane d of 64 nodes; the per-node clamp reads the node's other planes
-// from LDS.  Same arithmetic per element as direction_body.
-constexpr int kDirNodes = kElemNodes, kDirThreads = kElemThreads;
-__device__ __forceinline__ void direction_elem(int N, size_t n6, const double* __restrict__ g,
-                                               const double* __restrict__ Sh, const double* __restrict__ Yh,
-                                               const DirCoef& C, double* __restrict__ p,
-                                               const uint8_t* __restrict__ lock, double mt, double mr,
-                                               double* __restrict__ part, int first, int count) {
+// from LDS.  Same arithmetic per element as direction_body.  Writes p and
+// returns this element's direction (0 past the block's nodes) and g.p term.
+constexpr int kDirNodes = kElemNodes;
+__device__ __forceinline__ double direction_elem(size_t e, bool on, bool locked, double gv, double cg, int k,
+                                                 const DirCoef& C, const double* __restrict__ Sh,
+                                                 const double* __restrict__ Yh, size_t n6, double mt, double mr,
+                                                 double* __restrict__ p, double& dv) {
   __shared__ double qs[6][kDirNodes];
   const int d = threadIdx.x / kDirNodes, l = threadIdx.x % kDirNodes;
-  const int i = first + l;
-  const bool on = l < count;
-  const size_t e = size_t(d) * N + i;
-  const bool locked = on && ((lock[i] >> d) & 1u);
-  double gv = 0.0, q = 0.0;
+  double q = 0.0;
   if (on) {
-    gv = g[e];
-    q = C.cg * gv;
-    for (int j = 0; j < C.k; j++) q += C.cs[j] * Sh[n6 * C.slot[j] + e] + C.cy[j] * Yh[n6 * C.slot[j] + e];
+    // every history load issued before the first use
+    double hs[kMaxHist], hy[kMaxHist];
+#pragma unroll
+    for (int j = 0; j < kMaxHist; j++) {
+      if (j < k) {
+        hs[j] = Sh[n6 * C.slot[j] + e];
+        hy[j] = Yh[n6 * C.slot[j] + e];
+      }
+    }
+    q = cg * gv;
+#pragma unroll
+    for (int j = 0; j < kMaxHist; j++)
+      if (j < k) q += C.cs[j] * hs[j] + C.cy[j] * hy[j];
     q = -q;
     if (locked) q = 0.0;
   }
   qs[d][l] = q;
   __syncthreads();
-  double dv = 0.0;
+  dv = 0.0;
   if (on) {
     const int b = d < 3 ? 0 : 3;
     const double a0 = qs[b][l], a1 = qs[b + 1][l], a2 = qs[b + 2][l];
@@ -1106,9 +1138,7 @@ __device__ __forceinline__ void direction_elem(int N, size_t n6, const double* _
     p[e] = q;
     dv = gv * q;
   }
-  double t[1] = {dv};
-  block_reduce<1, kDirThreads>(t, 0u);
-  if (threadIdx.x == 0) part[blockIdx.x] = t[0];
+  return q;
 }
 
 __global__ __launch_bounds__(kBlock) void k_direction(int N, size_t n6, const double* __restrict__ g,
@@ -1215,21 +1245,23 @@ __global__ __launch_bounds__(kBlock) void k_sum_rows(const double* __restrict__ 
 // Device-resident minimiser, MC/src/sim.cpp:174-208.  A control block holds
 // the loop state; every kernel of one L-BFGS iteration is predicated on it, so
 // the iteration can be captured once as a hipGraph and replayed without host
-// round trips.  One replay runs five kernels:
-//   k_dir_g     p = -H g (compact form) and g.p         phase DIRECTION, or
-//               p = -g                                   phase STEEPEST
-//   k_state_g   descent check of a fresh direction (g.p >= 0 -> steepest),
-//               then the trial state z + alpha p         phases DIRECTION,
-//   k_elem_g    beams and contact shapes of the trial    STEEPEST or SEARCH
-//   k_gather_g  gradient, and the trial's curvature pair and its products
-//   k_decide_g  Armijo decision: accept (pair update, convergence test, next
+// round trips.  One replay (one line-search trial) runs four kernels:
+//   k_dstate_g  p = -H g (compact form) and g.p          phase DIRECTION, or
+//               p = -g                                   phase STEEPEST,
+//               then the trial state z + alpha p         (alpha = 1), or the
+//               stored p at the current alpha            phase SEARCH
+//   k_elem_g    beams and contact shapes of the trial
+//   k_gather_g  gradient, the trial's curvature pair and its products
+//   k_decide_g  a fresh direction that is not a descent direction (g.p >= 0)
+//               discards the trial (steepest descent next); otherwise
+//               Armijo: accept (pair update, convergence test, next
 //               coefficients, phase DIRECTION), halve (phase SEARCH: the next
 //               replay is a backtrack), steepest retry, or failure
-// The single-block kernels own the control block.  One wide kernel writes to
-// it: block 0 of the state kernel records the descent-check outcome in two
-// fields (srch_eval, srch_dg) that the state kernel itself never reads; every
-// other field a wide kernel reads is written only by earlier kernels (kernel
-// boundaries order the two).
+// A fresh direction's trial is evaluated before its descent check is known
+// (the check needs g.p over the whole lattice); the rare non-descent
+// direction costs one discarded trial and saves a launch on every other one.
+// Only the deciding block writes the control block; the wide kernels read it,
+// and kernel boundaries order the two.
 // ---------------------------------------------------------------------------
 enum { PH_DIRECTION = 0, PH_STEEPEST = 2, PH_SEARCH = 4 };
 enum { ST_RUNNING = 0, ST_CONVERGED = 1, ST_FAILED = 2, ST_ITERCAP = 3, ST_HOLD = 4 };
@@ -1238,8 +1270,7 @@ constexpr int kMaxSlots = kMaxHist + 1;  // history slots of the device loop: m 
 struct Ctl {
   int status, phase, par, it, iterations, backtracks, evals, retry;
   int bt, count, spare, m, max_it, max_ls, rel_escape, ls_mode;
-  int wolfe, srch_eval;  // srch_eval: the state kernel of this replay evaluates a trial
-  double srch_dg;        // g.p of a fresh direction, reduced by the state kernel
+  int wolfe, pad;
   double step, dg, top, armijo, min_step, tol_rms, tol_max, rel_tol, pen_tol, ts, skin_half, wolfe_eps;
   double cur[kRec], cand[kRec];
   DirCoef coef;
@@ -1453,88 +1484,80 @@ __device__ void compact_coefficients_wave(const Ctl& c, DirCoef& C) {
   }
 }
 
-// The decision kernel copies the control block to LDS (every load of the
-// copy issued before the first store, so that the copy costs one memory
-// latency), runs its scalar logic there, and copies it back.
-static_assert(sizeof(Ctl) % 8 == 0, "Ctl is copied as 8-byte words");
-constexpr int kCtlWords = int(sizeof(Ctl) / 8), kCtlRounds = (kCtlWords + kBlock - 1) / kBlock;
-__device__ __forceinline__ void ctl_store(Ctl* C, const Ctl& sc) {
-  __syncthreads();
-#pragma unroll
-  for (int q = 0; q < kCtlRounds; q++) {
-    const int e = threadIdx.x + q * kBlock;
-    if (e < kCtlWords) reinterpret_cast<double*>(C)[e] = reinterpret_cast<const double*>(&sc)[e];
-  }
-}
-
 // A replay evaluates a trial right after a fresh direction (phase DIRECTION or
-// STEEPEST: the descent check runs inside the state kernel) or on a backtrack
-// (phase SEARCH).
+// STEEPEST) or on a backtrack (phase SEARCH).
 __device__ __forceinline__ bool ctl_trial_phase(const Ctl* C) {
   return C->status == ST_RUNNING &&
          (C->phase == PH_SEARCH || C->phase == PH_DIRECTION || C->phase == PH_STEEPEST);
 }
-__device__ __forceinline__ bool ctl_trial(const Ctl* C) { return ctl_trial_phase(C) && C->srch_eval; }
 
-// L-BFGS direction from the control block's coefficients (phase DIRECTION)
-// or -g (phase STEEPEST)
-__global__ __launch_bounds__(kDirThreads) void k_dir_g(const Ctl* __restrict__ Cs, const Inst* __restrict__ I,
-                                                       const Blk* __restrict__ td, int N, size_t n6,
-                                                       const double* __restrict__ g0, const double* __restrict__ g1,
-                                                       const double* __restrict__ Sh, const double* __restrict__ Yh,
-                                                       double* __restrict__ p, const uint8_t* __restrict__ lock,
-                                                       double* __restrict__ part) {
-  const Blk bk = td[blockIdx.x];
-  const Ctl* C = Cs + bk.inst;
-  if (C->status != ST_RUNNING || (C->phase != PH_DIRECTION && C->phase != PH_STEEPEST)) return;
-  const Inst& in = I[bk.inst];
-  const double* g = C->par ? g1 : g0;
-  if (C->phase == PH_STEEPEST) {
-    DirCoef D{};
-    D.cg = 1.0;
-    direction_elem(N, n6, g, Sh, Yh, D, p, lock, in.mt, in.mr, part, bk.first, bk.count);
-  } else {
-    direction_elem(N, n6, g, Sh, Yh, C->coef, p, lock, in.mt, in.mr, part, bk.first, bk.count);
-  }
-}
-
+// Direction and trial state of one trial, element-parallel (blocks of 6
+// waves over 64 nodes, wave d on DOF plane d).  A fresh direction (phase
+// DIRECTION: the control block's compact-form coefficients; STEEPEST: -g) is
+// written to p, its g.p terms go to partial row 9, and the trial is z + p;
+// phase SEARCH takes the stored p at the control block's step.  Wave 0 then
+// evaluates the pose of the block's nodes (state_node: partial rows 0..8 as
+// k_state).  partials [10][Pd].
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_state_g(Lat<T> L, const Inst* __restrict__ I, const Blk* __restrict__ tn,
-                                                    const Ctl* Cs, double* __restrict__ z0, double* __restrict__ z1,
-                                                    const double* __restrict__ p, double* __restrict__ x,
-                                                    double* __restrict__ R, const double* __restrict__ xref,
-                                                    double* __restrict__ nb0, double* __restrict__ nb1,
-                                                    const double* __restrict__ dpart, int Pd,
-                                                    double* __restrict__ part, Ctl* Cw) {
-  const Blk bk = tn[blockIdx.x];
+__global__ __launch_bounds__(kElemThreads) void k_dstate_g(Lat<T> L, const Inst* __restrict__ I,
+                                                           const Blk* __restrict__ td, const Ctl* __restrict__ Cs,
+                                                           double* __restrict__ z0, double* __restrict__ z1,
+                                                           const double* __restrict__ g0, const double* __restrict__ g1,
+                                                           const double* __restrict__ Sh, const double* __restrict__ Yh,
+                                                           size_t n6, double* __restrict__ p, double* __restrict__ x,
+                                                           double* __restrict__ R, const double* __restrict__ xref,
+                                                           double* __restrict__ nb0, double* __restrict__ nb1,
+                                                           double* __restrict__ part) {
+  const Blk bk = td[blockIdx.x];
   const Ctl* C = Cs + bk.inst;
   if (!ctl_trial_phase(C)) return;
   const Inst& in = I[bk.inst];
-  const bool lead = int(blockIdx.x) == in.pn0 && threadIdx.x == 0;  // the instance's first block
-  double step = C->step;
-  if (C->phase != PH_SEARCH) {
-    // descent check of the fresh direction, MC/src/sim.cpp:191: every block
-    // of the instance reduces the same partials in the same order; its first
-    // block records the outcome for the later kernels of this replay (fields
-    // this kernel never reads)
-    double dg[1];
-    reduce_columns<1>(dpart, Pd, in.pd0, in.pd1, dg, 0u);
-    const bool eval = !(C->phase == PH_DIRECTION && !(dg[0] < 0));
-    if (lead) {
-      Cw[bk.inst].srch_eval = eval;
-      Cw[bk.inst].srch_dg = dg[0];
-    }
-    if (!eval) return;
+  const int N = L.N;
+  const int d = threadIdx.x / kWave, l = threadIdx.x % kWave, i = bk.first + l;
+  const bool on = l < bk.count;
+  const int ic = on ? i : bk.first;
+  const size_t e = size_t(d) * N + ic;
+  const int par = C->par, phase = C->phase;
+  const bool locked = (L.lock[ic] >> d) & 1u;
+  const double zv = (par ? z1 : z0)[e];
+  // wave 0 evaluates the block's node poses below: its inputs are loaded now,
+  // so that their latency overlaps the direction
+  NodeBase nbase{};
+  if (d == 0) nbase = node_base(L, xref, ic);
+  double q, dv = 0.0, step;
+  if (phase == PH_SEARCH) {
+    step = C->step;
+    q = p[e];
+  } else {
     step = 1.0;
-  } else if (lead) {
-    Cw[bk.inst].srch_eval = 1;
+    const double gv = (par ? g1 : g0)[e];
+    const bool steep = phase == PH_STEEPEST;
+    q = direction_elem(e, on, on && locked, gv, steep ? 1.0 : C->coef.cg, steep ? 0 : C->coef.k, C->coef, Sh, Yh,
+                       n6, in.mt, in.mr, p, dv);
   }
-  Scalars S = in.S;
-  S.top = C->top;
-  const int par = C->par;
-  TermBufs nt{par ? nb1 : nb0, par ? nb0 : nb1};
-  state_body<T>(L, S, par ? z1 : z0, p, step, par ? z0 : z1, x, R, xref, nt, part, bk.first + threadIdx.x,
-                int(threadIdx.x) < bk.count);
+  // trial DOF value, as state_body
+  double zz = zv + step * q;
+  if (locked) zz = 0.0;
+  __shared__ double zs[6][kWave];
+  zs[d][l] = zz;
+  if (on) (par ? z0 : z1)[e] = zz;  // the candidate buffer
+  __syncthreads();
+  double v[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  if (d == 0 && on) {
+    Scalars S = in.S;
+    S.top = C->top;
+    const double qn[6] = {zs[0][l], zs[1][l], zs[2][l], zs[3][l], zs[4][l], zs[5][l]};
+    TermBufs nt{par ? nb1 : nb0, par ? nb0 : nb1};
+    double sv[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    state_node<T>(L, S, qn, nbase, x, R, xref != nullptr, nt, i, sv);
+#pragma unroll
+    for (int k = 0; k < 9; k++) v[k] = sv[k];
+  }
+  v[9] = dv;
+  block_reduce<10, kElemThreads>(v, (1u << 3) | (1u << 7));
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int k = 0; k < 10; k++) part[size_t(k) * gridDim.x + blockIdx.x] = v[k];
 }
 // Location of the Verlet pair list, read by the graph kernels at run time so
 // that a list reallocated with a wider slab needs no new graph.
@@ -1560,7 +1583,7 @@ __global__ __launch_bounds__(kBlock) void k_elem_g(const Ctl* __restrict__ Cs, L
   const int blk = beams ? blockIdx.x : blockIdx.x - Pb;
   const Blk bk = beams ? tb[blk] : tc[blk];
   const Ctl* C = Cs + bk.inst;
-  if (!ctl_trial(C)) return;
+  if (!ctl_trial_phase(C)) return;
   const int par = C->par;
   const int item = bk.first + threadIdx.x;
   const bool on = int(threadIdx.x) < bk.count;
@@ -1573,202 +1596,121 @@ __global__ __launch_bounds__(kBlock) void k_elem_g(const Ctl* __restrict__ Cs, L
   }
 }
 
-// Gradient assembly of the trial (element-parallel, as k_gather), then
-// (history on) the trial's curvature pair s = zc - z, y = gc - g written into
-// the spare history slot with all its products (partial rows 3..: for each
-// retained pair j, oldest first: s_j.y, y_j.y, s_j.gc, y_j.gc; then s.y, s.s,
-// y.y, s.gc, y.gc), computed by the thread that assembled the element.  The
-// history elements are loaded before the assembly so that their latency
-// overlaps it.
-template <class T>
-__global__ __launch_bounds__(kElemThreads) void k_gather_g(Lat<T> L, const Inst* __restrict__ I,
-                                                           const Blk* __restrict__ td, const Ctl* __restrict__ Cs,
-                                                           const double* __restrict__ x,
-                                                           const double* __restrict__ rec,
-                                                           const double* __restrict__ cf_sph,
-                                                           const double* __restrict__ cf_cap,
-                                                           const double* __restrict__ spart, int Ps,
-                                                           const double* __restrict__ z0, const double* __restrict__ z1,
-                                                           double* __restrict__ g0, double* __restrict__ g1,
-                                                           const double* __restrict__ p, double* __restrict__ Sh,
-                                                           double* __restrict__ Yh, size_t n6,
-                                                           double* __restrict__ part) {
-  const Blk bk = td[blockIdx.x];
-  const Ctl* C = Cs + bk.inst;
-  if (!ctl_trial(C)) return;
-  const Inst& in = I[bk.inst];
-  const int d = threadIdx.x / kWave, l = threadIdx.x % kWave, i = bk.first + l;
-  const bool on = l < bk.count;
-  const int ic = on ? i : bk.first;
-  const size_t e = size_t(d) * L.N + ic;
-  const int par = C->par, m = C->m, count = C->count, spare = C->spare;
-  const int off = count == m ? 1 : 0, nold = m > 0 ? count - off : 0;
-  double* gnew = par ? g0 : g1;
-  const double* gcur = par ? g1 : g0;
-  const double* zcur = par ? z1 : z0;
-  const double* zcand = par ? z0 : z1;
-  double hs[kMaxHist], hy[kMaxHist];
-#pragma unroll
-  for (int jj = 0; jj < kMaxHist; jj++) {
-    hs[jj] = 0.0;
-    hy[jj] = 0.0;
-    if (jj < nold) {
-      const size_t o = size_t(C->order[jj + off]) * n6 + e;
-      hs[jj] = Sh[o];
-      hy[jj] = Yh[o];
-    }
-  }
-  const double zcv = zcand[e], zv = zcur[e], gov = gcur[e], pv = p[e];
-  const bool locked = (L.lock[ic] >> d) & 1u;
-  IncVals iv;
-  gather_load<T>(L, rec, cf_cap, d, ic, iv);
-  double c0, c1;
-  centroid<kElemThreads>(in, spart, Ps, c0, c1);
-  const double gv = gather_sum<T>(L, in, C->top, x, rec, cf_sph, cf_cap, c0, c1, d, ic, iv);
-  const double gd = locked ? 0.0 : gv;
-  if (on) gnew[e] = gd;
-  double a[kGRows];
-#pragma unroll
-  for (int r = 0; r < kGRows; r++) a[r] = 0.0;
-  a[0] = on && !locked ? gd * gd : 0.0;
-  a[1] = on && !locked ? fabs(gd) : 0.0;
-  a[2] = on ? gd * pv : 0.0;
-  int NR = 3;
-  if (m > 0) {
-    const double sv = on ? zcv - zv : 0.0, yv = on ? gd - gov : 0.0, gc = on ? gd : 0.0;
-    if (on) {
-      Sh[size_t(spare) * n6 + e] = sv;
-      Yh[size_t(spare) * n6 + e] = yv;
-    }
-#pragma unroll
-    for (int jj = 0; jj < kMaxHist; jj++) {
-      if (jj < nold) {
-        a[3 + 4 * jj + 0] = hs[jj] * yv;
-        a[3 + 4 * jj + 1] = hy[jj] * yv;
-        a[3 + 4 * jj + 2] = hs[jj] * gc;
-        a[3 + 4 * jj + 3] = hy[jj] * gc;
-      }
-    }
-    const double w[5] = {sv * yv, sv * sv, yv * yv, sv * gc, yv * gc};
-#pragma unroll
-    for (int jj = 0; jj <= kMaxHist; jj++)
-      if (jj == nold)
-#pragma unroll
-        for (int c = 0; c < 5; c++) a[3 + 4 * jj + c] = w[c];
-    NR = 3 + 4 * nold + 5;
-  }
-  __shared__ double sh[kElemThreads / kWave][kGRows];
-  const double t = block_rows<kElemThreads / kWave>(a, NR, sh);
-  if (int(threadIdx.x) < NR) part[size_t(threadIdx.x) * gridDim.x + blockIdx.x] = t;
+// ---------------------------------------------------------------------------
+// Rows of the device loop's decision: 0..8 the state rows of k_dstate_g,
+// 9..12 contact shapes, 13..16 beams, 17..19 gradient (sum g^2, max |g|, g.p),
+// 20 the fresh direction's g.p (k_dstate_g row 9), 21.. the curvature-pair
+// products (gather rows 3..).
+// ---------------------------------------------------------------------------
+constexpr int kGDecRows = 21 + 4 * kMaxHist + 5;
+// An instance's block ranges in the device loop's block tables.  The loop's
+// tables list only the instances still running (compacted as a batch drains);
+// each instance keeps its own blocks in their own order, so its partials, and
+// every fold of them, do not depend on which other instances are listed.
+struct GRng {
+  int pd0, pd1, pb0, pb1, pc0, pc1, pad0, pad1;
+};
+struct GraphParts {
+  const double *spart, *cpart, *bpart, *gpart;  // [10][Pd], [4][Pc], [4][Pb], [kGRows][Pd]
+  int Pd, Pc, Pb;
+};
+__device__ __forceinline__ DecRow graph_row(int j, const GRng& g, const GraphParts& P) {
+  if (j < 9) return {P.spart + size_t(j) * P.Pd + g.pd0, g.pd1 - g.pd0, row_max(j)};
+  if (j < 13) return {P.cpart + size_t(j - 9) * P.Pc + g.pc0, g.pc1 - g.pc0, row_max(j)};
+  if (j < 17) return {P.bpart + size_t(j - 13) * P.Pb + g.pb0, g.pb1 - g.pb0, false};
+  if (j == 20) return {P.spart + size_t(9) * P.Pd + g.pd0, g.pd1 - g.pd0, false};
+  return {P.gpart + size_t(j < 20 ? j - 17 : j - 18) * P.Pd + g.pd0, g.pd1 - g.pd0, row_max(j)};
 }
 
-// Candidate record and the Armijo decision (MC/src/sim.cpp:178-180,192-204);
-// on acceptance the curvature-pair update, the swap of current and candidate
-// buffers, the next convergence test and the next coefficients
-// (MC/src/sim.cpp:181-183,205-207).
-//
-// kDecSplit blocks per instance: each folds its share of the partial rows
-// (the record's and, with history on, the pair products) into drows, so that
-// the row loads are spread over several compute units; the block that
-// arrives last at the instance's counter (device-scope fences on both sides of
-// the count) reads the folded rows and the control block and takes the
-// decision, and resets the counter for the next trial.
-constexpr int kDecSplit = 8;
-constexpr int kDecNQ = (kDecRows + kDecSplit * kDecWaves - 1) / (kDecSplit * kDecWaves);
-__global__ __launch_bounds__(kBlock) void k_decide_g(Ctl* __restrict__ Cs, const Inst* __restrict__ I,
-                                                     const double* __restrict__ spart, int Ps,
-                                                     const double* __restrict__ cpart, int Pc,
-                                                     const double* __restrict__ bpart, int Pb,
-                                                     const double* __restrict__ gpart, int Pg,
-                                                     int* __restrict__ iflags, double* drows, unsigned* dcount) {
-  const int b = blockIdx.x / kDecSplit, part = blockIdx.x % kDecSplit;
-  Ctl* C = Cs + b;
-  const Inst& in = I[b];
-  if (!ctl_trial_phase(C)) return;
-  if (C->phase != PH_SEARCH && !C->srch_eval) {
-    // not a descent direction (no trial was evaluated): steepest descent next
-    // (the other blocks of the instance take this branch too, before or after
-    // this write)
-    if (part == 0 && threadIdx.x == 0) {
-      C->count = 0;
-      C->phase = PH_STEEPEST;
-    }
-    return;
-  }
-  __shared__ double lanes[kDecWaves * kDecNQ][kWave + 1];
-  __shared__ int last;
-  {
-    const int m = C->m, count = C->count;
-    const int R = 20 + (m > 0 ? 4 * (count - (count == m ? 1 : 0)) + 5 : 0);
-    const DecParts P{spart, cpart, bpart, gpart, Ps, Pc, Pb, Pg};
-    rows_pass<kDecNQ>(in, P, R, part * kDecWaves, kDecSplit * kDecWaves, lanes, drows + size_t(b) * kDecRows);
-  }
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __threadfence();
-    last = atomicAdd(dcount + b, 1u) == unsigned(kDecSplit - 1);
-  }
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-  // the control block and the folded rows in one round of loads
-  __shared__ Ctl sc;
-  __shared__ double rows[kDecRows];
-  __shared__ int accepted, need_coef;
-  double cr[kCtlRounds];
+// The deciding block copies the control block to LDS (every load of the copy
+// issued before the first store, so that the copy costs one memory latency),
+// runs its scalar logic there, and copies it back.
+static_assert(sizeof(Ctl) % 8 == 0, "Ctl is copied as 8-byte words");
+constexpr int kCtlWords = int(sizeof(Ctl) / 8);
+template <int NT>
+__device__ __forceinline__ void ctl_load(Ctl& sc, const Ctl* C) {
+  constexpr int RD = (kCtlWords + NT - 1) / NT;
+  double cr[RD];
 #pragma unroll
-  for (int q = 0; q < kCtlRounds; q++)
-    cr[q] = reinterpret_cast<const double*>(C)[min(int(threadIdx.x) + q * kBlock, kCtlWords - 1)];
-  const double rv = drows[size_t(b) * kDecRows + min(int(threadIdx.x), kDecRows - 1)];
-  const int fl = iflags[b];
-  const int m = C->m;
+  for (int q = 0; q < RD; q++)
+    cr[q] = reinterpret_cast<const double*>(C)[min(int(threadIdx.x) + q * NT, kCtlWords - 1)];
 #pragma unroll
-  for (int q = 0; q < kCtlRounds; q++) {
-    const int e = threadIdx.x + q * kBlock;
+  for (int q = 0; q < RD; q++) {
+    const int e = threadIdx.x + q * NT;
     if (e < kCtlWords) reinterpret_cast<double*>(&sc)[e] = cr[q];
   }
-  if (int(threadIdx.x) < kDecRows) rows[threadIdx.x] = rv;
+}
+template <int NT>
+__device__ __forceinline__ void ctl_store(Ctl* C, const Ctl& sc) {
+  constexpr int RD = (kCtlWords + NT - 1) / NT;
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < RD; q++) {
+    const int e = threadIdx.x + q * NT;
+    if (e < kCtlWords) reinterpret_cast<double*>(C)[e] = reinterpret_cast<const double*>(&sc)[e];
+  }
+}
+
+// The decision on the folded rows (MC/src/sim.cpp:178-183,191-207), by one
+// block of NT threads: the descent check of a fresh direction (:191; a
+// non-descent direction discards the trial and steepest descent follows),
+// the candidate record, Armijo (the reference's total-energy test or the
+// per-term change; the mixed policy adds the approximate-Wolfe test of Hager
+// & Zhang, SIAM J. Optim. 16, 2005, for changes below its energy noise), and
+// on acceptance the curvature-pair update, the swap of current and candidate
+// buffers, the next convergence test and the next coefficients.
+template <int NT>
+__device__ void decide_block(Ctl* __restrict__ C, const Inst& in, const double* rows, int* __restrict__ iflag) {
+  __shared__ Ctl sc;
+  __shared__ int accepted, need_coef;
+  ctl_load<NT>(sc, C);
+  const int fl = *iflag;
   __syncthreads();
   if (threadIdx.x == 0) {
     Ctl* c = &sc;
-    if (c->phase != PH_SEARCH) {
-      c->dg = c->srch_dg;
-      c->step = 1.0;
-      c->bt = 0;
-      c->phase = PH_SEARCH;
-    }
-    iflags[b] = 0;
-    dcount[b] = 0;
-    record_from_rows(in, rows, fl, c->cur, c->cand);
-    const double* r = c->cand;
+    *iflag = 0;
     c->evals++;
     accepted = 0;
     need_coef = 0;
-    if (r[11] >= c->skin_half) {  // contact list stale at this trial: rebuild, then redo it
-      c->status = ST_HOLD;
-    } else {
-      c->backtracks++;
-      // Armijo on the energy change: total energies (the reference) or the
-      // per-term sum; the mixed policy adds the approximate-Wolfe test of Hager
-      // & Zhang (SIAM J. Optim. 16, 2005) for changes below its energy noise
-      const double dE = c->ls_mode ? r[12] : r[0] - c->cur[0];
-      const double bound = c->armijo * c->step * c->dg;
-      bool ok = r[9] != 0.0 && dE <= bound;
-      if (!ok && c->wolfe && r[9] != 0.0 && fabs(dE) <= c->wolfe_eps * fabs(c->cur[0]))
-        ok = r[14] <= (2.0 * c->armijo - 1.0) * c->dg;
-      if (ok) {
-        accepted = 1;
+    bool discard = false;
+    if (c->phase != PH_SEARCH) {
+      const double dg = rows[20];
+      if (c->phase == PH_DIRECTION && !(dg < 0)) {
+        c->count = 0;
+        c->phase = PH_STEEPEST;
+        discard = true;
       } else {
-        c->step *= 0.5;
-        c->bt++;
-        if (c->bt >= c->max_ls || c->step < c->min_step) {
-          if (c->retry == 0) {  // steepest-descent retry
-            c->retry = 1;
-            c->count = 0;
-            c->phase = PH_STEEPEST;
-          } else {
-            c->status = ST_FAILED;
+        c->dg = dg;
+        c->step = 1.0;
+        c->bt = 0;
+        c->phase = PH_SEARCH;
+      }
+    }
+    if (!discard) {
+      record_from_rows(in, rows, fl, c->cur, c->cand);
+      const double* r = c->cand;
+      if (r[11] >= c->skin_half) {  // contact list stale at this trial: rebuild, then redo it
+        c->status = ST_HOLD;
+      } else {
+        c->backtracks++;
+        const double dE = c->ls_mode ? r[12] : r[0] - c->cur[0];
+        const double bound = c->armijo * c->step * c->dg;
+        bool ok = r[9] != 0.0 && dE <= bound;
+        if (!ok && c->wolfe && r[9] != 0.0 && fabs(dE) <= c->wolfe_eps * fabs(c->cur[0]))
+          ok = r[14] <= (2.0 * c->armijo - 1.0) * c->dg;
+        if (ok) {
+          accepted = 1;
+        } else {
+          c->step *= 0.5;
+          c->bt++;
+          if (c->bt >= c->max_ls || c->step < c->min_step) {
+            if (c->retry == 0) {  // steepest-descent retry
+              c->retry = 1;
+              c->count = 0;
+              c->phase = PH_STEEPEST;
+            } else {
+              c->status = ST_FAILED;
+            }
           }
         }
       }
@@ -1776,8 +1718,8 @@ __global__ __launch_bounds__(kBlock) void k_decide_g(Ctl* __restrict__ Cs, const
   }
   __syncthreads();
   if (accepted) {
-    // pair update from the products in rows 20.. (one wavefront)
-    if (m > 0 && threadIdx.x < kWave) history_update_wave(sc, rows + 20);
+    // pair update from the products in rows 21.. (one wavefront)
+    if (sc.m > 0 && threadIdx.x < kWave) history_update_wave(sc, rows + 21);
     __syncthreads();
     if (threadIdx.x == 0) {
       Ctl* c = &sc;
@@ -1803,7 +1745,156 @@ __global__ __launch_bounds__(kBlock) void k_decide_g(Ctl* __restrict__ Cs, const
     __syncthreads();
     if (need_coef && threadIdx.x < kWave) compact_coefficients_wave(sc, sc.coef);
   }
-  ctl_store(C, sc);
+  ctl_store<NT>(C, sc);
+}
+
+// Gradient assembly of the trial (element-parallel, as k_gather), then
+// (history on) the trial's curvature pair s = zc - z, y = gc - g written into
+// the spare history slot with all its products (partial rows 3..: for each
+// retained pair j, oldest first: s_j.y, y_j.y, s_j.gc, y_j.gc; then s.y, s.s,
+// y.y, s.gc, y.gc), computed by the thread that assembled the element.  The
+// history elements are loaded before the assembly so that their latency
+// overlaps it.
+// MH: the history capacity the kernel is built for (the arrays below live in
+// registers; the common m <= 10 gets the smaller build, with half the
+// register footprint of the m <= 16 one).
+template <class T, int MH>
+__global__ __launch_bounds__(kElemThreads) void k_gather_g(Lat<T> L, const Inst* __restrict__ I,
+                                                           const Blk* __restrict__ td, const GRng* __restrict__ G,
+                                                           const Ctl* __restrict__ Cs, const double* __restrict__ x,
+                                                           const double* __restrict__ rec,
+                                                           const double* __restrict__ cf_sph,
+                                                           const double* __restrict__ cf_cap,
+                                                           const double* __restrict__ dspart,
+                                                           const double* __restrict__ z0, const double* __restrict__ z1,
+                                                           double* __restrict__ g0, double* __restrict__ g1,
+                                                           const double* __restrict__ p, double* __restrict__ Sh,
+                                                           double* __restrict__ Yh, size_t n6,
+                                                           double* __restrict__ part) {
+  const Blk bk = td[blockIdx.x];
+  const Ctl* C = Cs + bk.inst;
+  if (!ctl_trial_phase(C)) return;
+  const Inst& in = I[bk.inst];
+  const int Pd = gridDim.x;
+  const int d = threadIdx.x / kWave, l = threadIdx.x % kWave, i = bk.first + l;
+  const bool on = l < bk.count;
+  const int ic = on ? i : bk.first;
+  const size_t e = size_t(d) * L.N + ic;
+  const int par = C->par, m = C->m, count = C->count, spare = C->spare;
+  const int off = count == m ? 1 : 0, nold = m > 0 ? count - off : 0;
+  double* gnew = par ? g0 : g1;
+  const double* gcur = par ? g1 : g0;
+  const double* zcur = par ? z1 : z0;
+  const double* zcand = par ? z0 : z1;
+  constexpr int NA = 3 + 4 * MH + 5;  // gradient rows, then pair products
+  // rows past NA stay zero: the row butterfly of block_rows takes 64 rows
+  constexpr int NP = NA <= kWave ? kWave : NA;
+  double hs[MH], hy[MH];
+#pragma unroll
+  for (int jj = 0; jj < MH; jj++) {
+    hs[jj] = 0.0;
+    hy[jj] = 0.0;
+    if (jj < nold) {
+      const size_t o = size_t(C->order[jj + off]) * n6 + e;
+      hs[jj] = Sh[o];
+      hy[jj] = Yh[o];
+    }
+  }
+  const double zcv = zcand[e], zv = zcur[e], gov = gcur[e], pv = p[e];
+  const bool locked = (L.lock[ic] >> d) & 1u;
+  IncVals iv;
+  gather_load<T>(L, rec, cf_cap, d, ic, iv);
+  double c0, c1;
+  {
+    double t[2];
+    reduce_columns<2, kElemThreads>(dspart + 5 * size_t(Pd), Pd, G[bk.inst].pd0, G[bk.inst].pd1, t, 0u);
+    c0 = t[0] * (1.0 / in.N);
+    c1 = t[1] * (1.0 / in.N);
+  }
+  const double gv = gather_sum<T>(L, in, C->top, x, rec, cf_sph, cf_cap, c0, c1, d, ic, iv);
+  const double gd = locked ? 0.0 : gv;
+  if (on) gnew[e] = gd;
+  double a[NP];
+#pragma unroll
+  for (int r = 0; r < NP; r++) a[r] = 0.0;
+  a[0] = on && !locked ? gd * gd : 0.0;
+  a[1] = on && !locked ? fabs(gd) : 0.0;
+  a[2] = on ? gd * pv : 0.0;
+  int NR = 3;
+  if (m > 0) {
+    const double sv = on ? zcv - zv : 0.0, yv = on ? gd - gov : 0.0, gc = on ? gd : 0.0;
+    if (on) {
+      Sh[size_t(spare) * n6 + e] = sv;
+      Yh[size_t(spare) * n6 + e] = yv;
+    }
+#pragma unroll
+    for (int jj = 0; jj < MH; jj++) {
+      if (jj < nold) {
+        a[3 + 4 * jj + 0] = hs[jj] * yv;
+        a[3 + 4 * jj + 1] = hy[jj] * yv;
+        a[3 + 4 * jj + 2] = hs[jj] * gc;
+        a[3 + 4 * jj + 3] = hy[jj] * gc;
+      }
+    }
+    const double w[5] = {sv * yv, sv * sv, yv * yv, sv * gc, yv * gc};
+#pragma unroll
+    for (int jj = 0; jj <= MH; jj++)
+      if (jj == nold)
+#pragma unroll
+        for (int c = 0; c < 5; c++) a[3 + 4 * jj + c] = w[c];
+    NR = 3 + 4 * nold + 5;
+  }
+  __shared__ double sh[kElemThreads / kWave][NP];
+  const double t = block_rows<kElemThreads / kWave>(a, NR, sh);
+  if (int(threadIdx.x) < NR) part[size_t(threadIdx.x) * Pd + blockIdx.x] = t;
+
+}
+
+// The decision of one trial: kDecSplit blocks per instance, each folding its
+// share of the decision rows (rows_pass) into drows, so that the row loads are
+// spread over several compute units; the block that arrives last at the
+// instance's counter reads the folded rows and decides (decide_block), and
+// resets the counter for the next trial.  Arrival: a block barrier completes
+// the block's row stores, one thread releases them (device-scope release
+// fence: L2 write-back, for the other XCDs) and counts the arrival; only the
+// last arriver acquires (invalidates its cached lines) before reading.
+constexpr int kDecSplit = 8;
+constexpr int kDecNQ = (kGDecRows + kDecSplit * kDecWaves - 1) / (kDecSplit * kDecWaves);
+__global__ __launch_bounds__(kBlock) void k_decide_g(Ctl* __restrict__ Cs, const Inst* __restrict__ I,
+                                                     const int* __restrict__ act, const GRng* __restrict__ G,
+                                                     const double* __restrict__ dspart, int Pd,
+                                                     const double* __restrict__ cpart, int Pc,
+                                                     const double* __restrict__ bpart, int Pb,
+                                                     const double* __restrict__ gpart, int* __restrict__ iflags,
+                                                     double* drows, unsigned* dcount) {
+  const int b = act[blockIdx.x / kDecSplit], part = blockIdx.x % kDecSplit;
+  Ctl* C = Cs + b;
+  const Inst& in = I[b];
+  if (!ctl_trial_phase(C)) return;
+  __shared__ double lanes[kDecWaves * kDecNQ][kWave + 1];
+  __shared__ int last;
+  {
+    const int m = C->m, count = C->count;
+    const int R = 21 + (m > 0 ? 4 * (count - (count == m ? 1 : 0)) + 5 : 0);
+    const GraphParts P{dspart, cpart, bpart, gpart, Pd, Pc, Pb};
+    const GRng g = G[b];
+    rows_pass<kDecNQ>([&](int j) { return graph_row(j, g, P); }, dspart + g.pd0, R, part * kDecWaves,
+                      kDecSplit * kDecWaves, lanes, drows + size_t(b) * kGDecRows);
+  }
+  if (threadIdx.x == 0) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    last = atomicAdd(dcount + b, 1u) == unsigned(kDecSplit - 1);
+    if (last) {
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      dcount[b] = 0;
+    }
+  }
+  __syncthreads();
+  if (!last) return;
+  __shared__ double rows[kGDecRows];
+  if (int(threadIdx.x) < kGDecRows) rows[threadIdx.x] = drows[size_t(b) * kGDecRows + threadIdx.x];
+  __syncthreads();
+  decide_block<kBlock>(C, in, rows, iflags + b);
 }
 
 __global__ __launch_bounds__(kBlock) void k_fill_zero(double* __restrict__ a, long n) {
@@ -1987,13 +2078,23 @@ struct mmc_ctx {
   double *z = nullptr, *g = nullptr, *p = nullptr, *zc = nullptr, *gc = nullptr, *zt = nullptr, *zin = nullptr;
   double *Sh = nullptr, *Yh = nullptr;  // [m + 1][6N]
   int hist_slots = 1;
+  int max_hist = 0;  // longest history over the instances
   double* host = nullptr;  // pinned readback
   // device-resident minimiser: control blocks (one per instance), their pinned
   // staging copies, the (current, candidate) buffer pairs selected by each
   // instance's Ctl::par, and the graph of kGraphIterations replays
   Ctl* ctl = nullptr;
   Ctl* hctl = nullptr;
-  double* drows = nullptr;     // [n_inst][kDecRows] rows folded by the decision kernel's blocks
+  // block tables of the captured device loop: the instances of gset only
+  std::vector<Blk> htd, htb, htc;  // host copies of the full tables
+  Blk *gtd = nullptr, *gtb = nullptr, *gtc = nullptr;
+  GRng* grng = nullptr;  // [n_inst] ranges in those tables
+  int* gact = nullptr;   // [n_inst] instances of the decision grid
+  int gPd = 0, gPb = 0, gPc = 0;
+  std::vector<int> gset;
+  int64_t captures = 0;
+  double* dspart = nullptr;    // [10][Pd] partials of the trial state kernel
+  double* drows = nullptr;     // [n_inst][kGDecRows] rows folded by the decision kernel's blocks
   unsigned* dcount = nullptr;  // [n_inst] blocks of the decision kernel done with their rows
   double *zbuf[2] = {nullptr, nullptr}, *gbuf[2] = {nullptr, nullptr};
   // per-item energies (nodes, beams, shapes) at the current iterate and at the
@@ -2057,6 +2158,7 @@ struct mmc_ctx {
   }
   template <class T>
   void capture_iteration(Lat<T> L);
+  void capture(const std::vector<int>& set);
   void minimize_graph(const double* tops, const int32_t* active, mmc_opt_stats* st);
   void minimize_host(double top, mmc_opt_stats& st);
   void final_evaluation(const double* tops, mmc_opt_stats* st, mmc_objective_out* fin);
@@ -2084,7 +2186,7 @@ mmc_ctx::~mmc_ctx() {
   void* ptrs[] = {bpos, bquat, mass, lock, ba, bb, L0, rad, area, inert, e0, plastic, inc_off, inc, adj_off, adj, x, R,
                   rec, trial, cf_sph, cf_cap, box, cnt, start, entries, flags, scan_tmp, spart, cpart, bpart, gpart,
                   vpart, stb, stn, stout, record, scal, z, g, p, zc, gc, zt, zin, Sh, Yh, xref, pcount, plist,
-                  inst_d, tn, tb, tc, td, sinst, tops_d, mask_d, ctl, lref, drows, dcount};
+                  inst_d, tn, tb, tc, td, sinst, tops_d, mask_d, ctl, lref, dspart, drows, dcount, gtd, gtb, gtc, grng, gact};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   void* pinned[] = {host, hctl, htops, hmask, hrec, hstat};
@@ -2128,7 +2230,7 @@ void mmc_ctx::setup(const mmc_lattice* const* Ls, const mmc_config* cs, int n, i
   precision = prec;
   n_inst = n;
   icfg.assign(cs, cs + n);
-  int max_hist = 0;
+  max_hist = 0;
   long long tn_ = 0, tb_ = 0;
   for (int b = 0; b < n; b++) {
     mmc_config& c = icfg[b];
@@ -2303,14 +2405,22 @@ void mmc_ctx::setup(const mmc_lattice* const* Ls, const mmc_config* cs, int n, i
     Pb = int(t.size());
     for (int b = 0; b < n; b++) hinst[b].pb0 = r0[b], hinst[b].pb1 = r1[b];
     put(tb, t);
+    htb = t;
     t = block_table(n, kBlock, [&](int b, int& f, int& c) { f = hinst[b].s0, c = hinst[b].NS; }, r0, r1);
     Pc = int(t.size());
     for (int b = 0; b < n; b++) hinst[b].pc0 = r0[b], hinst[b].pc1 = r1[b];
     put(tc, t);
+    htc = t;
     t = block_table(n, kDirNodes, [&](int b, int& f, int& c) { f = hinst[b].n0, c = hinst[b].N; }, r0, r1);
     Pd = int(t.size());
     for (int b = 0; b < n; b++) hinst[b].pd0 = r0[b], hinst[b].pd1 = r1[b];
     put(td, t);
+    htd = t;
+    gtd = dalloc<Blk>(htd.size());
+    gtb = dalloc<Blk>(htb.size());
+    gtc = dalloc<Blk>(htc.size());
+    grng = dalloc<GRng>(n);
+    gact = dalloc<int>(n);
     std::vector<int> si(NS);
     for (int b = 0; b < n; b++)
       for (int s = 0; s < hinst[b].NS; s++) si[hinst[b].s0 + s] = b;
@@ -2369,7 +2479,8 @@ void mmc_ctx::setup(const mmc_lattice* const* Ls, const mmc_config* cs, int n, i
   HIPCALL(hipHostMalloc((void**)&hrec, size_t(n) * kRec * sizeof(double), hipHostMallocDefault));
   HIPCALL(hipHostMalloc((void**)&hstat, size_t(n) * 8 * sizeof(double), hipHostMallocDefault));
   ctl = dalloc<Ctl>(n);
-  drows = dalloc<double>(size_t(n) * kDecRows);
+  dspart = dalloc<double>(10 * size_t(Pd));
+  drows = dalloc<double>(size_t(n) * kGDecRows);
   dcount = dalloc<unsigned>(n);
   HIPCALL(hipMemset(dcount, 0, size_t(n) * sizeof(unsigned)));
   lref = dalloc<ListRef>(1);
@@ -2685,20 +2796,63 @@ void mmc_ctx::capture_iteration(Lat<T> Lt) {
   graph_exec = nullptr;
   graph = nullptr;
   const size_t n6 = 6 * size_t(N);
+  const int na = int(gset.size());
   HIPCALL(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
   for (int u = 0; u < kGraphIterations; u++) {
-    k_dir_g<<<Pd, kDirThreads, 0, stream>>>(ctl, inst_d, td, N, n6, gbuf[0], gbuf[1], Sh, Yh, p, lock, vpart);
-    k_state_g<T><<<Ps, kBlock, 0, stream>>>(Lt, inst_d, tn, ctl, zbuf[0], zbuf[1], p, x, (double*)R, xref, nbuf[0],
-                                            nbuf[1], vpart, Pd, spart, ctl);
-    k_elem_g<T><<<Pb + Pc, kBlock, 0, stream>>>(ctl, Lt, inst_d, tb, tc, x, (const double*)R, rec, ubuf[0], ubuf[1],
-                                                bpart, Pb, pcount, lref, cf_sph, cf_cap, cbuf[0], cbuf[1], cpart, Pc);
-    k_gather_g<T><<<Pd, kElemThreads, 0, stream>>>(Lt, inst_d, td, ctl, x, rec, cf_sph, cf_cap, spart, Ps, zbuf[0],
-                                                   zbuf[1], gbuf[0], gbuf[1], p, Sh, Yh, n6, gpart);
-    k_decide_g<<<n_inst * kDecSplit, kBlock, 0, stream>>>(ctl, inst_d, spart, Ps, cpart, Pc, bpart, Pb, gpart, Pd,
-                                                           iflags(), drows, dcount);
+    k_dstate_g<T><<<gPd, kElemThreads, 0, stream>>>(Lt, inst_d, gtd, ctl, zbuf[0], zbuf[1], gbuf[0], gbuf[1], Sh,
+                                                    Yh, n6, p, x, (double*)R, xref, nbuf[0], nbuf[1], dspart);
+    k_elem_g<T><<<gPb + gPc, kBlock, 0, stream>>>(ctl, Lt, inst_d, gtb, gtc, x, (const double*)R, rec, ubuf[0],
+                                                  ubuf[1], bpart, gPb, pcount, lref, cf_sph, cf_cap, cbuf[0], cbuf[1],
+                                                  cpart, gPc);
+    if (max_hist <= 10)
+      k_gather_g<T, 10><<<gPd, kElemThreads, 0, stream>>>(Lt, inst_d, gtd, grng, ctl, x, rec, cf_sph, cf_cap, dspart,
+                                                          zbuf[0], zbuf[1], gbuf[0], gbuf[1], p, Sh, Yh, n6, gpart);
+    else
+      k_gather_g<T, kMaxHist><<<gPd, kElemThreads, 0, stream>>>(Lt, inst_d, gtd, grng, ctl, x, rec, cf_sph, cf_cap,
+                                                                dspart, zbuf[0], zbuf[1], gbuf[0], gbuf[1], p, Sh, Yh,
+                                                                n6, gpart);
+    k_decide_g<<<na * kDecSplit, kBlock, 0, stream>>>(ctl, inst_d, gact, grng, dspart, gPd, cpart, gPc, bpart, gPb,
+                                                       gpart, iflags(), drows, dcount);
   }
   HIPCALL(hipStreamEndCapture(stream, &graph));
   HIPCALL(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0));
+}
+
+// Block tables of the device loop for the instances of `set` (ascending):
+// each instance's blocks of the full tables, in order, and its ranges there;
+// then the loop is captured over them.
+void mmc_ctx::capture(const std::vector<int>& set) {
+  HIPCALL(hipStreamSynchronize(stream));
+  std::vector<Blk> d, b, c;
+  std::vector<GRng> r(size_t(n_inst), GRng{0, 0, 0, 0, 0, 0, 0, 0});
+  for (int i : set) {
+    const Inst& in = hinst[size_t(i)];
+    GRng& g = r[size_t(i)];
+    g.pd0 = int(d.size());
+    d.insert(d.end(), htd.begin() + in.pd0, htd.begin() + in.pd1);
+    g.pd1 = int(d.size());
+    g.pb0 = int(b.size());
+    b.insert(b.end(), htb.begin() + in.pb0, htb.begin() + in.pb1);
+    g.pb1 = int(b.size());
+    g.pc0 = int(c.size());
+    c.insert(c.end(), htc.begin() + in.pc0, htc.begin() + in.pc1);
+    g.pc1 = int(c.size());
+  }
+  gPd = int(d.size());
+  gPb = int(b.size());
+  gPc = int(c.size());
+  // the stream is idle here (the caller synchronised it): plain copies
+  if (gPd) HIPCALL(hipMemcpy(gtd, d.data(), d.size() * sizeof(Blk), hipMemcpyHostToDevice));
+  if (gPb) HIPCALL(hipMemcpy(gtb, b.data(), b.size() * sizeof(Blk), hipMemcpyHostToDevice));
+  if (gPc) HIPCALL(hipMemcpy(gtc, c.data(), c.size() * sizeof(Blk), hipMemcpyHostToDevice));
+  HIPCALL(hipMemcpy(grng, r.data(), r.size() * sizeof(GRng), hipMemcpyHostToDevice));
+  HIPCALL(hipMemcpy(gact, set.data(), set.size() * sizeof(int), hipMemcpyHostToDevice));
+  gset = set;
+  captures++;
+  if (precision == MMC_MIXED)
+    capture_iteration<float>(lat32());
+  else
+    capture_iteration<double>(lat64());
 }
 
 namespace {
@@ -2753,11 +2907,19 @@ void mmc_ctx::minimize_graph(const double* tops, const int32_t* active, mmc_opt_
     any = any || h.status == ST_RUNNING;
   }
   HIPCALL(hipMemcpyAsync(ctl, hctl, size_t(n) * sizeof(Ctl), hipMemcpyHostToDevice, stream));
-  if (!graph_exec) {
-    if (precision == MMC_MIXED)
-      capture_iteration<float>(lat32());
-    else
-      capture_iteration<double>(lat64());
+  // the loop's graph covers the running instances only; as a batch drains it
+  // is captured again over the remaining ones (whenever they are at most
+  // half of those captured), so that the replays of the last few lattices do
+  // not dispatch every other lattice's blocks
+  auto running_set = [&] {
+    std::vector<int> set;
+    for (int b = 0; b < n; b++)
+      if (hctl[b].status == ST_RUNNING || hctl[b].status == ST_HOLD) set.push_back(b);
+    return set;
+  };
+  {
+    std::vector<int> set = running_set();
+    if (!set.empty() && (set != gset || !graph_exec)) capture(set);
   }
   // every replay advances each running phase machine by at least one step;
   // bound the total so that a logic error cannot spin forever
@@ -2790,6 +2952,8 @@ void mmc_ctx::minimize_graph(const double* tops, const int32_t* active, mmc_opt_
     batch = std::min(8, batch * 2);
     state = heads();
     if (replays > guard) throw Fail(MMC_ERR_STATE, "device minimiser exceeded its replay bound");
+    const int left = state.first + state.second;
+    if (left > 0 && 2 * size_t(left) <= gset.size()) capture(running_set());
   }
   HIPCALL(hipMemcpyAsync(hctl, ctl, size_t(n) * sizeof(Ctl), hipMemcpyDeviceToHost, stream));
   HIPCALL(hipStreamSynchronize(stream));
