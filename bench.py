@@ -19,11 +19,13 @@ frame (and against the oracle's FP64 curve when tests/golden holds it for this
 workload).
 
 The JSON line also carries:
-  roofline      the beam element kernel's algorithmic bytes per launch divided
-                by its mean launch duration, measured with HIP events on the
-                solver stream at the state the timed region ended in (peak:
-                MI355X HBM 8 TB/s); `kernel_share` lists the per-iteration
-                kernels' mean durations measured the same way
+  roofline      the device loop's dominant kernel (k_gather_g: gradient
+                assembly and curvature-pair products of a trial): algorithmic
+                bytes per launch divided by its mean launch duration, measured
+                with HIP events on the solver stream at the state the timed
+                region ended in (peak: MI355X HBM 8 TB/s); `loop_kernel_us`
+                lists the four loop kernels' mean durations measured the same
+                way, `beam_kernel` the objective's beam kernel
   cpu_baseline  the FP64 CPU oracle (single-threaded restatement of the
                 reference solver) on the host, on a bounded sample of the same
                 workload (the first load increments after the tare)
@@ -43,10 +45,11 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT / "falling-dice_amd"))
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per k_beam launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
-# passes, calibrated against device copies of known size (command and figures:
-# profiles/k_beam_cubic32x32x16.md); keyed by lattice cells
+# HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE passes,
+# calibrated against device copies of known size (commands and figures in
+# profiles/); keyed by lattice cells
 BEAM_TRAFFIC_BYTES = {(32, 32, 16): 13.08e6}
+GATHER_TRAFFIC_BYTES = {(32, 32, 16): 37.75e6}
 
 
 def workload(args):
@@ -79,6 +82,18 @@ def beam_kernel_bytes(N, B, precision):
     read and the 9-double gradient record written (72); per node the pose
     read once (x 24 B, R 32 B; the pose is FP64 under both policies)."""
     return B * (8 + 32 + 24 + 56 + 72) + N * (24 + 32)
+
+
+def gather_loop_bytes(N, B, m):
+    """Compulsory HBM bytes of one k_gather_g launch (the device loop's
+    gradient assembly with the trial's curvature pair), at a full history:
+    per DOF element the m - 1 retained pairs s_j, y_j, the candidate and
+    current z, the current gradient and the direction read, the gradient and
+    the new pair written ((2m + 2) reads + 3 writes, 8 B each); per node the
+    incidence offsets (4), sphere contact force (24), mass (8), height (8)
+    and lock byte (1); per beam its two incidence entries (8), its 9-double
+    record (72) and the capsule contact forces on both ends (48)."""
+    return 6 * N * ((2 * m + 2) * 8 + 24) + N * (4 + 24 + 8 + 8 + 1) + B * (8 + 72 + 48)
 
 
 def run_leg(cfg, prec, warmup, steps, log):
@@ -357,19 +372,25 @@ def main():
     N, B = sim.n_nodes, sim.n_beams
     total = 1 + args.load_steps + args.unload_steps
     # kernel durations with HIP events on the solver stream, at the state the
-    # timed region ended in (launches inside graph replays carry no per-launch
-    # events, so each kernel is replayed in isolation on the same buffers)
+    # timed region ended in: the four kernels of a device-loop trial (launches
+    # inside graph replays carry no per-launch events, so each is launched on
+    # its own, on the loop's scratch buffers) and the objective's beam kernel
     launches = 200
+    loop_us = {k: round(sim.time_kernel(k, launches) * 1e3, 3) for k in ("dstate_g", "elem_g", "gather_g", "decide_g")}
     beam_ms = sim.time_kernel("beam", launches)
-    share = {k: round(sim.time_kernel(k, 100) * 1e3, 3) for k in ("state", "contact", "gather")}
-    share["beam"] = round(beam_ms * 1e3, 3)
-    bytes_launch = beam_kernel_bytes(N, B, prec)
-    ach = bytes_launch / (beam_ms * 1e-3) / 1e9
+    gather_ms = loop_us["gather_g"] * 1e-3
+    g_bytes = gather_loop_bytes(N, B, cfg.history)
+    ach = g_bytes / (gather_ms * 1e-3) / 1e9
+    b_bytes = beam_kernel_bytes(N, B, prec)
+    b_ach = b_bytes / (beam_ms * 1e-3) / 1e9
     roof = {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(ach / HBM_PEAK_GBS, 4),
-            "traffic": BEAM_TRAFFIC_BYTES.get((args.nx, args.ny, args.nz)), "kernel": "k_beam",
-            "bytes_per_launch": bytes_launch, "mean_launch_us": round(beam_ms * 1e3, 3),
-            "launches_timed": launches, "kernel_us": share}
+            "traffic": GATHER_TRAFFIC_BYTES.get((args.nx, args.ny, args.nz)), "kernel": "k_gather_g",
+            "bytes_per_launch": g_bytes, "mean_launch_us": round(gather_ms * 1e3, 3), "launches_timed": launches,
+            "loop_kernel_us": loop_us,
+            "beam_kernel": {"kernel": "k_beam", "bytes_per_launch": b_bytes, "mean_launch_us": round(beam_ms * 1e3, 3),
+                            "achieved": round(b_ach, 2), "frac": round(b_ach / HBM_PEAK_GBS, 4),
+                            "traffic": BEAM_TRAFFIC_BYTES.get((args.nx, args.ny, args.nz))}}
     value = K / dt
     out = {
         "metric": "load-steps/sec, mixed vs FP64 on 1 GPU, at <=1e-5 rel platen-force error",

@@ -2093,6 +2093,7 @@ struct mmc_ctx {
   int gPd = 0, gPb = 0, gPc = 0;
   std::vector<int> gset;
   int64_t captures = 0;
+  Ctl* tctl = nullptr;  // scratch control blocks of time_loop_kernel
   double* dspart = nullptr;    // [10][Pd] partials of the trial state kernel
   double* drows = nullptr;     // [n_inst][kGDecRows] rows folded by the decision kernel's blocks
   unsigned* dcount = nullptr;  // [n_inst] blocks of the decision kernel done with their rows
@@ -2159,6 +2160,7 @@ struct mmc_ctx {
   template <class T>
   void capture_iteration(Lat<T> L);
   void capture(const std::vector<int>& set);
+  double time_loop_kernel(int kernel, int launches);
   void minimize_graph(const double* tops, const int32_t* active, mmc_opt_stats* st);
   void minimize_host(double top, mmc_opt_stats& st);
   void final_evaluation(const double* tops, mmc_opt_stats* st, mmc_objective_out* fin);
@@ -2186,7 +2188,8 @@ mmc_ctx::~mmc_ctx() {
   void* ptrs[] = {bpos, bquat, mass, lock, ba, bb, L0, rad, area, inert, e0, plastic, inc_off, inc, adj_off, adj, x, R,
                   rec, trial, cf_sph, cf_cap, box, cnt, start, entries, flags, scan_tmp, spart, cpart, bpart, gpart,
                   vpart, stb, stn, stout, record, scal, z, g, p, zc, gc, zt, zin, Sh, Yh, xref, pcount, plist,
-                  inst_d, tn, tb, tc, td, sinst, tops_d, mask_d, ctl, lref, dspart, drows, dcount, gtd, gtb, gtc, grng, gact};
+                  inst_d, tn, tb, tc, td, sinst, tops_d, mask_d, ctl, lref, dspart, drows, dcount, gtd, gtb, gtc,
+                  grng, gact, tctl};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   void* pinned[] = {host, hctl, htops, hmask, hrec, hstat};
@@ -2855,6 +2858,90 @@ void mmc_ctx::capture(const std::vector<int>& set) {
     capture_iteration<double>(lat64());
 }
 
+// One device-loop kernel timed on its own: the kernels run against a scratch
+// copy of the control blocks (tctl) set to a trial of every captured
+// instance, and write only buffers that the loop treats as scratch between
+// iterations (the candidate z and gradient, the direction, the spare history
+// slot, poses, records and partials).  The trial kernels only read the
+// control blocks: `launches` back-to-back launches between two events.  The
+// decision kernel rewrites them, so each of its launches follows a restore
+// kernel; the restores alone are timed the same way and subtracted.
+namespace {
+__global__ __launch_bounds__(kBlock) void k_copy_words(double* __restrict__ dst, const double* __restrict__ src,
+                                                       long n) {
+  const long i = long(blockIdx.x) * kBlock + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+}  // namespace
+double mmc_ctx::time_loop_kernel(int kernel, int launches) {
+  if (gset.empty()) throw Fail(MMC_ERR_STATE, "the device loop has not run yet (no mmc_minimize)");
+  HIPCALL(hipStreamSynchronize(stream));
+  std::vector<Ctl> h(static_cast<size_t>(n_inst));
+  HIPCALL(hipMemcpy(h.data(), ctl, h.size() * sizeof(Ctl), hipMemcpyDeviceToHost));
+  for (Ctl& c : h) {
+    c.status = ST_RUNNING;
+    c.phase = kernel == MMC_K_DSTATE_G ? PH_DIRECTION : PH_SEARCH;
+    c.step = 1.0;
+    c.bt = 0;
+    c.dg = -1.0;
+    if (kernel == MMC_K_GATHER_G && c.m > 0) c.count = c.m;  // full history: m - 1 retained pairs
+  }
+  if (!tctl) tctl = dalloc<Ctl>(2 * size_t(n_inst));  // [0, n): scratch, [n, 2n): its initial state
+  Ctl* tinit = tctl + n_inst;
+  HIPCALL(hipMemcpy(tctl, h.data(), h.size() * sizeof(Ctl), hipMemcpyHostToDevice));
+  HIPCALL(hipMemcpy(tinit, h.data(), h.size() * sizeof(Ctl), hipMemcpyHostToDevice));
+  const long words = long(n_inst) * long(sizeof(Ctl) / 8);
+  const int wblocks = int((words + kBlock - 1) / kBlock);
+  const size_t n6 = 6 * size_t(N);
+  const int na = int(gset.size());
+  auto loop = [&](auto Lt, auto* tag, bool restore, bool work) {
+    using T = std::remove_pointer_t<decltype(tag)>;
+    HIPCALL(hipEventRecord(ev0, stream));
+    for (int i = 0; i < launches; i++) {
+      if (restore) k_copy_words<<<wblocks, kBlock, 0, stream>>>((double*)tctl, (const double*)tinit, words);
+      if (!work) continue;
+      switch (kernel) {
+        case MMC_K_DSTATE_G:
+          k_dstate_g<T><<<gPd, kElemThreads, 0, stream>>>(Lt, inst_d, gtd, tctl, zbuf[0], zbuf[1], gbuf[0],
+                                                          gbuf[1], Sh, Yh, n6, p, x, (double*)R, xref, nbuf[0],
+                                                          nbuf[1], dspart);
+          break;
+        case MMC_K_ELEM_G:
+          k_elem_g<T><<<gPb + gPc, kBlock, 0, stream>>>(tctl, Lt, inst_d, gtb, gtc, x, (const double*)R, rec,
+                                                        ubuf[0], ubuf[1], bpart, gPb, pcount, lref, cf_sph, cf_cap,
+                                                        cbuf[0], cbuf[1], cpart, gPc);
+          break;
+        case MMC_K_GATHER_G:
+          if (max_hist <= 10)
+            k_gather_g<T, 10><<<gPd, kElemThreads, 0, stream>>>(Lt, inst_d, gtd, grng, tctl, x, rec, cf_sph,
+                                                                cf_cap, dspart, zbuf[0], zbuf[1], gbuf[0], gbuf[1],
+                                                                p, Sh, Yh, n6, gpart);
+          else
+            k_gather_g<T, kMaxHist><<<gPd, kElemThreads, 0, stream>>>(Lt, inst_d, gtd, grng, tctl, x, rec, cf_sph,
+                                                                      cf_cap, dspart, zbuf[0], zbuf[1], gbuf[0],
+                                                                      gbuf[1], p, Sh, Yh, n6, gpart);
+          break;
+        default:
+          k_decide_g<<<na * kDecSplit, kBlock, 0, stream>>>(tctl, inst_d, gact, grng, dspart, gPd, cpart, gPc,
+                                                             bpart, gPb, gpart, iflags(), drows, dcount);
+      }
+    }
+    HIPCALL(hipEventRecord(ev1, stream));
+    HIPCALL(hipEventSynchronize(ev1));
+    HIPCALL(hipGetLastError());
+    float ms = 0;
+    HIPCALL(hipEventElapsedTime(&ms, ev0, ev1));
+    return double(ms);
+  };
+  auto run = [&](bool restore, bool work) {
+    return precision == MMC_MIXED ? loop(lat32(), (float*)nullptr, restore, work)
+                                  : loop(lat64(), (double*)nullptr, restore, work);
+  };
+  if (kernel != MMC_K_DECIDE_G) return run(false, true) / launches;
+  const double both = run(true, true), restores = run(true, false);
+  return std::max(0.0, both - restores) / launches;
+}
+
 namespace {
 __global__ __launch_bounds__(kBlock) void k_release_hold(Ctl* __restrict__ Cs, int n) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
@@ -3214,6 +3301,10 @@ int mmc_time_kernel(mmc_ctx* c, int32_t kernel, int32_t launches, double* mean_m
     if (!c->ev0) {
       HIPCALL(hipEventCreate(&c->ev0));
       HIPCALL(hipEventCreate(&c->ev1));
+    }
+    if (kernel >= MMC_K_DSTATE_G && kernel <= MMC_K_DECIDE_G) {
+      *mean_ms = c->time_loop_kernel(kernel, launches);
+      return;
     }
     auto run = [&](auto Lt, auto* tag) {
       using T = std::remove_pointer_t<decltype(tag)>;

@@ -479,7 +479,8 @@ class Simulator:
         return ms.value
 
 
-KERNELS = {"state": 0, "fill": 2, "contact": 4, "beam": 5, "gather": 6, "final": 7, "direction": 8, "pair": 9}
+KERNELS = {"state": 0, "fill": 2, "contact": 4, "beam": 5, "gather": 6, "final": 7, "direction": 8, "pair": 9,
+           "dstate_g": 10, "elem_g": 11, "gather_g": 12, "decide_g": 13}
 
 
 def device_name() -> str:

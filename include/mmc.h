@@ -190,7 +190,9 @@ int mmc_read_stats(mmc_ctx* ctx, mmc_state_stats* out);
  * mmc_kernel_timer(ctx, id) starts (and clears) it, id < 0 stops it;
  * mmc_kernel_timer_read returns the launch count and the mean duration. */
 enum { MMC_K_STATE = 0, MMC_K_FILL = 2, MMC_K_CONTACT = 4, MMC_K_BEAM = 5, MMC_K_GATHER = 6,
-       MMC_K_FINAL = 7, MMC_K_DIRECTION = 8, MMC_K_PAIR = 9 };
+       MMC_K_FINAL = 7, MMC_K_DIRECTION = 8, MMC_K_PAIR = 9,
+       /* the four kernels of one device-loop trial (mmc_time_kernel only) */
+       MMC_K_DSTATE_G = 10, MMC_K_ELEM_G = 11, MMC_K_GATHER_G = 12, MMC_K_DECIDE_G = 13 };
 int mmc_kernel_timer(mmc_ctx* ctx, int32_t kernel);
 int mmc_kernel_timer_read(mmc_ctx* ctx, int64_t* launches, double* mean_ms);
 /* Objective evaluations performed by the context since creation. */
@@ -198,7 +200,12 @@ int64_t mmc_evaluations(const mmc_ctx* ctx);
 /* Mean duration of one objective kernel (MMC_K_STATE, _CONTACT, _BEAM,
  * _GATHER) over `launches` back-to-back launches on the context stream at the
  * state of the last evaluation, timed with HIP events (no reference
- * counterpart). */
+ * counterpart).  MMC_K_DSTATE_G, _ELEM_G, _GATHER_G and _DECIDE_G time one
+ * kernel of the device loop (after an mmc_minimize*) on a scratch copy of the
+ * control blocks set to a trial (a fresh L-BFGS direction for DSTATE_G, a
+ * backtrack for the others, with a full history for GATHER_G), one event pair
+ * per launch; they write only the loop's scratch buffers, so the solver state
+ * is left as it was. */
 int mmc_time_kernel(mmc_ctx* ctx, int32_t kernel, int32_t launches, double* mean_ms);
 
 /* Simulator::beamPotential(x, R), MC/src/sim.cpp:468-509, evaluated on the device
