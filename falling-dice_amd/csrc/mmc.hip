@@ -21,6 +21,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -419,7 +420,7 @@ __global__ __launch_bounds__(kBlock) void k_pairs(Lat<T> L, const Inst* __restri
 // k_contact: MC/src/sim.cpp:159-165 over the Verlet list.  Each shape
 // accumulates its own force; each pair's energy is counted by its lower shape.
 // partials [4][P]: E_contact, max_pen, contacts, sum of per-shape energy changes
-template <class T>
+template <class T, int NT = kBlock>
 __device__ __forceinline__ void contact_body(Lat<T> L, const Inst& in, const double* __restrict__ x,
                                                     const unsigned* __restrict__ pcount, const int* __restrict__ plist,
                                                     int slab, double* __restrict__ cf_sph, double* __restrict__ cf_cap,
@@ -486,7 +487,7 @@ __device__ __forceinline__ void contact_body(Lat<T> L, const Inst& in, const dou
     if (ct.out) ct.out[s] = v[0];
     if (ct.ref) v[3] = v[0] - ct.ref[s];
   }
-  block_reduce<4>(v, 1u << 1);
+  block_reduce<4, NT>(v, 1u << 1);
   if (threadIdx.x == 0)
 #pragma unroll
     for (int k = 0; k < 4; k++) part[k * P + blk] = v[k];
@@ -508,7 +509,7 @@ __global__ __launch_bounds__(kBlock) void k_contact(Lat<T> L, const Inst* __rest
 // rec [9][B]: f (force on b, -f on a), rotational gradient of a, of b.
 // partials [4][P]: energy, elastic, plastic, sum of per-beam energy changes.
 // ---------------------------------------------------------------------------
-template <class T>
+template <class T, int NT = kBlock>
 __device__ __forceinline__ void beam_body(Lat<T> L, Material M, const double* __restrict__ x,
                                                  const double* __restrict__ R, double* __restrict__ rec,
                                                  double* __restrict__ trial, TermBufs bt,
@@ -608,7 +609,7 @@ __device__ __forceinline__ void beam_body(Lat<T> L, Material M, const double* __
     if (bt.out) bt.out[k] = v[0];
     if (bt.ref) v[3] = v[0] - bt.ref[k];
   }
-  block_reduce<4>(v, 0u);
+  block_reduce<4, NT>(v, 0u);
   if (threadIdx.x == 0)
 #pragma unroll
     for (int c = 0; c < 4; c++) part[c * P + blk] = v[c];
@@ -712,6 +713,76 @@ __device__ __forceinline__ double gather_sum(Lat<T> L, const Inst& in, double to
     gs += S.centering * (d == 0 ? cen0 : cen1) / in.N;
   }
   return S.ts * gs;
+}
+
+// The six gradient components of node i at once (gather_sum's arithmetic per
+// component, in the same order): the incidence list is walked once and every
+// record load of the six components is issued before the first use.
+template <class T>
+__device__ __forceinline__ void gather_node(Lat<T> L, const Inst& in, double top, const double* __restrict__ x,
+                                            const double* __restrict__ rec, const double* __restrict__ cf_sph,
+                                            const double* __restrict__ cf_cap, double cen0, double cen1, int i,
+                                            double (&g)[6]) {
+  const int N = L.N, B = L.B;
+  const Scalars& S = in.S;
+  const int e0 = L.inc_off[i], e1 = L.inc_off[i + 1];
+  const int elast = max(e1 - 1, 0);
+  int code[kIncUnroll];
+#pragma unroll
+  for (int u = 0; u < kIncUnroll; u++) code[u] = L.inc[min(e0 + u, elast)];
+  double val[6][kIncUnroll], cap[3][kIncUnroll];
+#pragma unroll
+  for (int u = 0; u < kIncUnroll; u++) {
+    const int k = code[u] >> 1, side = code[u] & 1;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      val[d][u] = rec[size_t(d) * B + k];
+      cap[d][u] = cf_cap[size_t(3 * side + d) * B + k];
+      val[3 + d][u] = rec[size_t(3 + 3 * side + d) * B + k];
+    }
+  }
+  const double sph[3] = {cf_sph[i], cf_sph[N + i], cf_sph[2 * N + i]};
+  const double mass = L.mass[i], y = x[N + i];
+#pragma unroll
+  for (int d = 0; d < 6; d++) {
+    double gs = 0.0;
+#pragma unroll
+    for (int u = 0; u < kIncUnroll; u++) {
+      if (e0 + u < e1) {
+        if (d < 3) {
+          gs = (code[u] & 1) ? gs + val[d][u] : gs - val[d][u];
+          gs = gs + cap[d][u];
+        } else {
+          gs = gs + val[d][u];
+        }
+      }
+    }
+    for (int e = e0 + kIncUnroll; e < e1; e++) {
+      const int c = L.inc[e], k = c >> 1, side = c & 1;
+      if (d < 3) {
+        const double f = rec[size_t(d) * B + k];
+        gs = side ? gs + f : gs - f;
+        gs = gs + cf_cap[size_t(3 * side + d) * B + k];
+      } else {
+        gs = gs + rec[size_t(3 + 3 * side + (d - 3)) * B + k];
+      }
+    }
+    if (d >= 3) {
+      g[d] = gs;
+      continue;
+    }
+    gs = gs + sph[d];
+    if (d == 1) {
+      gs += mass * S.gravity;
+      const double pb = S.jr - y;
+      if (pb > 0) gs -= S.kc * pb;
+      const double pt = y + S.jr - top;
+      if (pt > 0) gs += S.kc * pt;
+    } else {
+      gs += S.centering * (d == 0 ? cen0 : cen1) / in.N;
+    }
+    g[d] = S.ts * gs;
+  }
 }
 
 // Node centroid (x, z) of the instance from the state kernel's partial rows
@@ -1419,6 +1490,7 @@ __host__ __device__ inline PairArgs pair_slots(int m, int count, int head) {
 // residual, so the dependent chain is k steps instead of k(k+1)/2.  Each lane
 // reads its rows of S'Y and Y'Y (and its column of S'Y) into registers first,
 // so a step is a division, a lane read and a multiply-add.
+template <int MH = kMaxHist>
 __device__ void compact_coefficients_wave(const Ctl& c, DirCoef& C) {
   const int k = c.count, l = threadIdx.x & (kWave - 1);
   if (k == 0) {
@@ -1435,9 +1507,9 @@ __device__ void compact_coefficients_wave(const Ctl& c, DirCoef& C) {
   const double gamma = (sy > 1e-20 && yy > 1e-20) ? fmin(1e4, fmax(1e-7, sy / yy)) : 0.10;
   const double dl = on ? c.SY[sl][sl] : 1.0;
   // order[] entries past count are stale but valid slot numbers
-  double syr[kMaxHist], syc[kMaxHist], yyr[kMaxHist];
+  double syr[MH], syc[MH], yyr[MH];
 #pragma unroll
-  for (int i = 0; i < kMaxHist; i++) {
+  for (int i = 0; i < MH; i++) {
     const int oi = c.order[i];
     syr[i] = c.SY[sl][oi];
     syc[i] = c.SY[oi][sl];
@@ -1446,7 +1518,7 @@ __device__ void compact_coefficients_wave(const Ctl& c, DirCoef& C) {
   // w = R^{-1} u, R the upper triangle of S'Y (R_ij = s_i . y_j, i <= j)
   double r = on ? c.u[sl] : 0.0, w = 0.0;
 #pragma unroll
-  for (int i = kMaxHist - 1; i >= 0; i--) {
+  for (int i = MH - 1; i >= 0; i--) {
     if (i < k) {
       const double wi = read_lane(r / dl, i);
       if (l == i) w = wi;
@@ -1456,7 +1528,7 @@ __device__ void compact_coefficients_wave(const Ctl& c, DirCoef& C) {
   // t = (D + gamma Y'Y) w - gamma v
   double t = on ? c.SY[sl][sl] * w - gamma * c.v[sl] : 0.0;
 #pragma unroll
-  for (int j = 0; j < kMaxHist; j++) {
+  for (int j = 0; j < MH; j++) {
     if (j < k) {
       const double wj = read_lane(w, j);
       if (on) t += gamma * yyr[j] * wj;
@@ -1466,7 +1538,7 @@ __device__ void compact_coefficients_wave(const Ctl& c, DirCoef& C) {
   double a = 0.0;
   r = t;
 #pragma unroll
-  for (int i = 0; i < kMaxHist; i++) {
+  for (int i = 0; i < MH; i++) {
     if (i < k) {
       const double ai = read_lane(r / dl, i);
       if (l == i) a = ai;
@@ -1659,11 +1731,12 @@ __device__ __forceinline__ void ctl_store(Ctl* C, const Ctl& sc) {
 // & Zhang, SIAM J. Optim. 16, 2005, for changes below its energy noise), and
 // on acceptance the curvature-pair update, the swap of current and candidate
 // buffers, the next convergence test and the next coefficients.
-template <int NT>
-__device__ void decide_block(Ctl* __restrict__ C, const Inst& in, const double* rows, int* __restrict__ iflag) {
-  __shared__ Ctl sc;
+// decide_core runs it on a control block in LDS (every thread of the block
+// calls it; returns after a barrier); decide_block copies the control block
+// in from global memory and back.
+template <int NT, int MH = kMaxHist>
+__device__ void decide_core(Ctl& sc, const Inst& in, const double* rows, int* __restrict__ iflag) {
   __shared__ int accepted, need_coef;
-  ctl_load<NT>(sc, C);
   const int fl = *iflag;
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -1743,8 +1816,15 @@ __device__ void decide_block(Ctl* __restrict__ C, const Inst& in, const double* 
       }
     }
     __syncthreads();
-    if (need_coef && threadIdx.x < kWave) compact_coefficients_wave(sc, sc.coef);
+    if (need_coef && threadIdx.x < kWave) compact_coefficients_wave<MH>(sc, sc.coef);
   }
+  __syncthreads();
+}
+template <int NT>
+__device__ void decide_block(Ctl* __restrict__ C, const Inst& in, const double* rows, int* __restrict__ iflag) {
+  __shared__ Ctl sc;
+  ctl_load<NT>(sc, C);
+  decide_core<NT>(sc, in, rows, iflag);
   ctl_store<NT>(C, sc);
 }
 
@@ -1895,6 +1975,274 @@ __global__ __launch_bounds__(kBlock) void k_decide_g(Ctl* __restrict__ Cs, const
   if (int(threadIdx.x) < kGDecRows) rows[threadIdx.x] = drows[size_t(b) * kGDecRows + threadIdx.x];
   __syncthreads();
   decide_block<kBlock>(C, in, rows, iflags + b);
+}
+
+// ---------------------------------------------------------------------------
+// Block solver: the whole minimisation of one lattice in one workgroup
+// (MC/src/sim.cpp:174-211, the same phase machine as the graph-replayed
+// loop), for lattices small enough that one workgroup holds their work.  One
+// launch takes the running instances, one workgroup each, and every
+// workgroup loops over trials until its instance has converged, failed, hit
+// its iteration cap or found its contact list stale (HOLD: the host rebuilds
+// the list and launches again), or until max_trials (a bound the phase
+// machine never reaches; it only guarantees the exit).  The control block
+// lives in LDS for the whole launch; the only synchronisation is the block
+// barrier, so lattices of any mix of sizes run side by side without host
+// round trips or graph replays.  A trial, as the graph loop's four kernels:
+//   nodes    thread per node: fresh direction (its six DOFs, locks, clamps)
+//            or the stored one, trial DOFs, pose, gravity, platens
+//   beams    thread per beam (beam_body); contact shapes thread per shape
+//            (contact_body)
+//   gather   thread per node, its six DOFs in turn: gradient assembly, the
+//            curvature pair into the spare slot and its products
+//   decide   decide_core on the block's folded rows
+// Partial sums are kept per thread across the passes over the instance's
+// items and folded by the block in a fixed order: results are reproducible,
+// and a lattice's results do not depend on the other lattices of the launch.
+// ---------------------------------------------------------------------------
+constexpr int kSolveThreads = 512, kSolveWaves = kSolveThreads / kWave;
+// largest lattice (nodes) the block solver takes: one pass of the workgroup
+// over the nodes
+constexpr int kBlockEngineNodes = kSolveThreads;
+constexpr int kSolveBudget = 64;  // trials per workgroup and launch
+
+template <class T, int MH>
+__global__ __launch_bounds__(kSolveThreads) void k_solve_block(
+    Lat<T> L, const Inst* __restrict__ I, Ctl* __restrict__ Cs, const int* __restrict__ act, int max_trials,
+    double* __restrict__ z0, double* __restrict__ z1, double* __restrict__ g0, double* __restrict__ g1,
+    double* __restrict__ Sh, double* __restrict__ Yh, size_t n6, double* __restrict__ p, double* __restrict__ x,
+    double* __restrict__ R, const double* __restrict__ xref, double* __restrict__ nb0, double* __restrict__ nb1,
+    double* __restrict__ ub0, double* __restrict__ ub1, double* __restrict__ cb0, double* __restrict__ cb1,
+    const unsigned* __restrict__ pcount, const ListRef* __restrict__ lr, double* __restrict__ rec,
+    double* __restrict__ cf_sph, double* __restrict__ cf_cap, int* __restrict__ iflags) {
+  const int b = act[blockIdx.x];
+  const Inst& in = I[b];
+  const int N = L.N, tid = threadIdx.x;
+  constexpr int NA = 3 + 4 * MH + 5;  // gradient rows, then pair products
+  constexpr int NP = NA <= kWave ? kWave : NA;
+  __shared__ Ctl sc;
+  __shared__ double rows[kGDecRows];
+  __shared__ double part4[4];
+  __shared__ double shr[kSolveWaves][NP];
+  __shared__ double qsh[6][kSolveThreads];  // raw directions, one column per thread
+  ctl_load<kSolveThreads>(sc, Cs + b);
+  __syncthreads();
+  for (int t = 0; t < max_trials && sc.status == ST_RUNNING; t++) {
+    const int par = sc.par, phase = sc.phase;
+    const bool fresh = phase != PH_SEARCH, steep = phase == PH_STEEPEST;
+    const double step = fresh ? 1.0 : sc.step;
+    double* zcur = par ? z1 : z0;
+    double* zcand = par ? z0 : z1;
+    const double* gcur = par ? g1 : g0;
+    double* gnew = par ? g0 : g1;
+    // ---- nodes: direction and trial state ----
+    {
+      const int kk = steep ? 0 : sc.coef.k;
+      const double cg = steep ? 1.0 : sc.coef.cg;
+      Scalars S = in.S;
+      S.top = sc.top;
+      const TermBufs nt{par ? nb1 : nb0, par ? nb0 : nb1};
+      double acc[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+      for (int base = 0; base < in.N; base += kSolveThreads) {
+        const int li = base + tid;
+        if (li >= in.N) continue;
+        const int i = in.n0 + li;
+        const unsigned lk = L.lock[i];
+        const NodeBase nbase = node_base(L, xref, i);
+        double q[6];
+        if (fresh) {
+          // one DOF at a time (its history loads in flight together; more at
+          // once spill), the raw direction parked in this thread's LDS column
+#pragma unroll 1
+          for (int d = 0; d < 6; d++) {
+            const size_t e = size_t(d) * N + i;
+            double hs[MH], hy[MH];
+#pragma unroll
+            for (int j = 0; j < MH; j++) {
+              if (j < kk) {
+                hs[j] = Sh[n6 * sc.coef.slot[j] + e];
+                hy[j] = Yh[n6 * sc.coef.slot[j] + e];
+              }
+            }
+            double v = cg * gcur[e];
+#pragma unroll
+            for (int j = 0; j < MH; j++)
+              if (j < kk) v += sc.coef.cs[j] * hs[j] + sc.coef.cy[j] * hy[j];
+            v = -v;
+            qsh[d][tid] = ((lk >> d) & 1u) ? 0.0 : v;
+          }
+#pragma unroll
+          for (int d = 0; d < 6; d++) q[d] = qsh[d][tid];
+#pragma unroll
+          for (int h = 0; h < 2; h++) {  // per-node clamps: translation, rotation
+            const double len = sqrt(q[3 * h] * q[3 * h] + q[3 * h + 1] * q[3 * h + 1] + q[3 * h + 2] * q[3 * h + 2]);
+            const double lim = h == 0 ? in.mt : in.mr;
+#pragma unroll
+            for (int c = 3 * h; c < 3 * h + 3; c++) {
+              if (len > lim) q[c] *= lim / len;
+              if ((lk >> c) & 1u) q[c] = 0.0;
+            }
+          }
+#pragma unroll
+          for (int d = 0; d < 6; d++) {
+            const size_t e = size_t(d) * N + i;
+            p[e] = q[d];
+            acc[9] += gcur[e] * q[d];
+          }
+        } else {
+#pragma unroll
+          for (int d = 0; d < 6; d++) q[d] = p[size_t(d) * N + i];
+        }
+        double zz[6];
+#pragma unroll
+        for (int d = 0; d < 6; d++) {
+          const size_t e = size_t(d) * N + i;
+          zz[d] = zcur[e] + step * q[d];
+          if ((lk >> d) & 1u) zz[d] = 0.0;
+          zcand[e] = zz[d];
+        }
+        double sv[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        state_node<T>(L, S, zz, nbase, x, R, xref != nullptr, nt, i, sv);
+#pragma unroll
+        for (int k = 0; k < 9; k++) acc[k] = (k == 3 || k == 7) ? fmax(acc[k], sv[k]) : acc[k] + sv[k];
+      }
+      block_reduce<10, kSolveThreads>(acc, (1u << 3) | (1u << 7));
+      if (tid == 0) {
+#pragma unroll
+        for (int k = 0; k < 9; k++) rows[k] = acc[k];
+        rows[20] = acc[9];
+      }
+    }
+    // ---- beams, then contact shapes ----
+    {
+      const TermBufs bt{par ? ub1 : ub0, par ? ub0 : ub1};
+      double acc[4] = {0, 0, 0, 0};
+      for (int base = 0; base < in.B; base += kSolveThreads) {
+        const int li = base + tid;
+        const bool on = li < in.B;
+        beam_body<T, kSolveThreads>(L, in.M, x, R, rec, nullptr, bt, part4, in.b0 + (on ? li : 0), on, 0, 1);
+        if (tid == 0)
+#pragma unroll
+          for (int c = 0; c < 4; c++) acc[c] += part4[c];
+      }
+      const TermBufs ct{par ? cb1 : cb0, par ? cb0 : cb1};
+      const int* plist = lr->plist;
+      const int slab = lr->slab;
+      double accc[4] = {0, 0, 0, 0};
+      for (int base = 0; base < in.NS; base += kSolveThreads) {
+        const int li = base + tid;
+        const bool on = li < in.NS;
+        contact_body<T, kSolveThreads>(L, in, x, pcount, plist, slab, cf_sph, cf_cap, ct, part4,
+                                       in.s0 + (on ? li : 0), on, 0, 1);
+        if (tid == 0) {
+          accc[0] += part4[0];
+          accc[1] = fmax(accc[1], part4[1]);
+          accc[2] += part4[2];
+          accc[3] += part4[3];
+        }
+      }
+      if (tid == 0)
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+          rows[9 + c] = accc[c];
+          rows[13 + c] = acc[c];
+        }
+    }
+    __syncthreads();
+    // ---- gather: gradient, curvature pair and its products ----
+    {
+      const int m = sc.m, count = sc.count, spare = sc.spare;
+      const int off = count == m ? 1 : 0, nold = m > 0 ? count - off : 0;
+      const double c0 = rows[5] * (1.0 / in.N), c1 = rows[6] * (1.0 / in.N);
+      int slots[MH];
+#pragma unroll
+      for (int jj = 0; jj < MH; jj++) slots[jj] = jj < nold ? sc.order[jj + off] : 0;
+      double a[NP];
+#pragma unroll
+      for (int r = 0; r < NP; r++) a[r] = 0.0;
+      // pass 1: gradient assembly (gather_node: all six components from one
+      // walk of the incidence list), norms, g.p, and the pair s, y written
+      for (int base = 0; base < in.N; base += kSolveThreads) {
+        const int li = base + tid;
+        if (li >= in.N) continue;
+        const int i = in.n0 + li;
+        const unsigned lk = L.lock[i];
+        double gval[6];
+        gather_node<T>(L, in, sc.top, x, rec, cf_sph, cf_cap, c0, c1, i, gval);
+#pragma unroll
+        for (int d = 0; d < 6; d++) {
+          const size_t e = size_t(d) * N + i;
+          const bool locked = (lk >> d) & 1u;
+          const double gd = locked ? 0.0 : gval[d];
+          gnew[e] = gd;
+          if (!locked) {
+            a[0] += gd * gd;
+            a[1] = fmax(a[1], fabs(gd));
+          }
+          a[2] += gd * p[e];
+          if (m > 0) {
+            Sh[size_t(spare) * n6 + e] = zcand[e] - zcur[e];
+            Yh[size_t(spare) * n6 + e] = gd - gcur[e];
+          }
+        }
+      }
+      // pass 2: the pair's products with the retained pairs and itself (each
+      // thread re-reads only what it wrote in pass 1)
+      if (m > 0) {
+        for (int base = 0; base < in.N; base += kSolveThreads) {
+          const int li = base + tid;
+          if (li >= in.N) continue;
+          const int i = in.n0 + li;
+#pragma unroll 1
+          for (int d = 0; d < 6; d++) {
+            const size_t e = size_t(d) * N + i;
+            double hs[MH], hy[MH];
+#pragma unroll
+            for (int jj = 0; jj < MH; jj++) {
+              hs[jj] = 0.0;
+              hy[jj] = 0.0;
+              if (jj < nold) {
+                hs[jj] = Sh[size_t(slots[jj]) * n6 + e];
+                hy[jj] = Yh[size_t(slots[jj]) * n6 + e];
+              }
+            }
+            const double gd = gnew[e], sv = Sh[size_t(spare) * n6 + e], yv = Yh[size_t(spare) * n6 + e];
+#pragma unroll
+            for (int jj = 0; jj < MH; jj++) {
+              if (jj < nold) {
+                a[3 + 4 * jj + 0] += hs[jj] * yv;
+                a[3 + 4 * jj + 1] += hy[jj] * yv;
+                a[3 + 4 * jj + 2] += hs[jj] * gd;
+                a[3 + 4 * jj + 3] += hy[jj] * gd;
+              }
+            }
+            const double w[5] = {sv * yv, sv * sv, yv * yv, sv * gd, yv * gd};
+#pragma unroll
+            for (int jj = 0; jj <= MH; jj++)
+              if (jj == nold)
+#pragma unroll
+                for (int c = 0; c < 5; c++) a[3 + 4 * jj + c] += w[c];
+          }
+        }
+      }
+      const int NR = m > 0 ? 3 + 4 * nold + 5 : 3;
+      const double tt = block_rows<kSolveWaves>(a, NR, shr);
+      if (tid < NR) rows[tid < 3 ? 17 + tid : 18 + tid] = tt;
+    }
+    __syncthreads();
+    decide_core<kSolveThreads, MH>(sc, in, rows, iflags + b);
+  }
+  ctl_store<kSolveThreads>(Cs + b, sc);
+}
+
+// list reference positions of the nodes of a node block table: xref = x
+__global__ __launch_bounds__(kBlock) void k_copy_xref(const Blk* __restrict__ tn, int N, const double* __restrict__ x,
+                                                      double* __restrict__ xref) {
+  const Blk bk = tn[blockIdx.x];
+  if (int(threadIdx.x) >= bk.count) return;
+  const int i = bk.first + threadIdx.x;
+#pragma unroll
+  for (int d = 0; d < 3; d++) xref[size_t(d) * N + i] = x[size_t(d) * N + i];
 }
 
 __global__ __launch_bounds__(kBlock) void k_fill_zero(double* __restrict__ a, long n) {
@@ -2079,6 +2427,11 @@ struct mmc_ctx {
   double *Sh = nullptr, *Yh = nullptr;  // [m + 1][6N]
   int hist_slots = 1;
   int max_hist = 0;  // longest history over the instances
+  // device loop engine: the graph-replayed loop, or (MMC_DEVICE_LOOP=block,
+  // lattices of up to kBlockEngineNodes nodes) the workgroup-per-lattice solver
+  bool block_engine = false;
+  template <class T>
+  void solve_blocks(Lat<T> L, const std::vector<int>& set, int max_trials);
   double* host = nullptr;  // pinned readback
   // device-resident minimiser: control blocks (one per instance), their pinned
   // staging copies, the (current, candidate) buffer pairs selected by each
@@ -2086,13 +2439,19 @@ struct mmc_ctx {
   Ctl* ctl = nullptr;
   Ctl* hctl = nullptr;
   // block tables of the captured device loop: the instances of gset only
-  std::vector<Blk> htd, htb, htc;  // host copies of the full tables
+  std::vector<Blk> htd, htb, htc, htn;  // host copies of the full tables
+  Blk *rtc = nullptr, *rtn = nullptr;    // shape and node tables of a partial list rebuild
   Blk *gtd = nullptr, *gtb = nullptr, *gtc = nullptr;
   GRng* grng = nullptr;  // [n_inst] ranges in those tables
   int* gact = nullptr;   // [n_inst] instances of the decision grid
   int gPd = 0, gPb = 0, gPc = 0;
   std::vector<int> gset;
   int64_t captures = 0;
+  // host-clock seconds: initial evaluation, graph captures, replay loop,
+  // minimize_graph in all, final evaluation (reported at destruction when
+  // MMC_LOOP_TIMING is set)
+  double loop_s[5] = {0, 0, 0, 0, 0};
+  int64_t loop_polls = 0;
   Ctl* tctl = nullptr;  // scratch control blocks of time_loop_kernel
   double* dspart = nullptr;    // [10][Pd] partials of the trial state kernel
   double* drows = nullptr;     // [n_inst][kGDecRows] rows folded by the decision kernel's blocks
@@ -2149,14 +2508,17 @@ struct mmc_ctx {
   void upload_inst() {
     if (inst_d) HIPCALL(hipMemcpy(inst_d, hinst.data(), hinst.size() * sizeof(Inst), hipMemcpyHostToDevice));
   }
+  // the contact lists of every instance (tables tc, tn), or of some
+  // instances only (their blocks of those tables in ctab, ntab)
   template <class T>
-  void build_list(Lat<T> L);
+  void build_list(Lat<T> L, const Blk* ctab = nullptr, int cblocks = 0, const Blk* ntab = nullptr, int nblocks = 0);
   void rebuild() {
     if (precision == MMC_MIXED)
       build_list<float>(lat32());
     else
       build_list<double>(lat64());
   }
+  void rebuild_held(const std::vector<int>& held);
   template <class T>
   void capture_iteration(Lat<T> L);
   void capture(const std::vector<int>& set);
@@ -2185,11 +2547,17 @@ struct mmc_ctx {
 };
 
 mmc_ctx::~mmc_ctx() {
+  if (std::getenv("MMC_LOOP_TIMING"))
+    std::fprintf(stderr,
+                 "[loop] %d instances: initial evaluations %.3f s, captures %lld (%.3f s), replay loop %.3f s "
+                 "(%lld graph launches, %lld polls), minimize %.3f s, final evaluations %.3f s\n",
+                 n_inst, loop_s[0], (long long)captures, loop_s[1], loop_s[2], (long long)graph_replays,
+                 (long long)loop_polls, loop_s[3], loop_s[4]);
   void* ptrs[] = {bpos, bquat, mass, lock, ba, bb, L0, rad, area, inert, e0, plastic, inc_off, inc, adj_off, adj, x, R,
                   rec, trial, cf_sph, cf_cap, box, cnt, start, entries, flags, scan_tmp, spart, cpart, bpart, gpart,
                   vpart, stb, stn, stout, record, scal, z, g, p, zc, gc, zt, zin, Sh, Yh, xref, pcount, plist,
                   inst_d, tn, tb, tc, td, sinst, tops_d, mask_d, ctl, lref, dspart, drows, dcount, gtd, gtb, gtc,
-                  grng, gact, tctl};
+                  grng, gact, tctl, rtc, rtn};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   void* pinned[] = {host, hctl, htops, hmask, hrec, hstat};
@@ -2404,6 +2772,8 @@ void mmc_ctx::setup(const mmc_lattice* const* Ls, const mmc_config* cs, int n, i
     Ps = int(t.size());
     for (int b = 0; b < n; b++) hinst[b].pn0 = r0[b], hinst[b].pn1 = r1[b];
     put(tn, t);
+    htn = t;
+    rtn = dalloc<Blk>(t.size());
     t = block_table(n, kBlock, [&](int b, int& f, int& c) { f = hinst[b].b0, c = hinst[b].B; }, r0, r1);
     Pb = int(t.size());
     for (int b = 0; b < n; b++) hinst[b].pb0 = r0[b], hinst[b].pb1 = r1[b];
@@ -2414,6 +2784,7 @@ void mmc_ctx::setup(const mmc_lattice* const* Ls, const mmc_config* cs, int n, i
     for (int b = 0; b < n; b++) hinst[b].pc0 = r0[b], hinst[b].pc1 = r1[b];
     put(tc, t);
     htc = t;
+    rtc = dalloc<Blk>(t.size());
     t = block_table(n, kDirNodes, [&](int b, int& f, int& c) { f = hinst[b].n0, c = hinst[b].N; }, r0, r1);
     Pd = int(t.size());
     for (int b = 0; b < n; b++) hinst[b].pd0 = r0[b], hinst[b].pd1 = r1[b];
@@ -2473,6 +2844,14 @@ void mmc_ctx::setup(const mmc_lattice* const* Ls, const mmc_config* cs, int n, i
   zt = dalloc<double>(n6);
   zin = dalloc<double>(n6);
   hist_slots = max_hist + 1;  // the device loop keeps one spare slot
+  {
+    int max_n = 0;
+    for (const Inst& in : hinst) max_n = std::max(max_n, in.N);
+    // the graph-replayed loop unless MMC_DEVICE_LOOP=block selects the block
+    // solver (only for lattices it takes in one pass over the nodes)
+    const char* eng = std::getenv("MMC_DEVICE_LOOP");
+    block_engine = eng && std::strcmp(eng, "block") == 0 && max_n <= kBlockEngineNodes;
+  }
   Sh = dalloc<double>(n6 * hist_slots);
   Yh = dalloc<double>(n6 * hist_slots);
   HIPCALL(hipHostMalloc((void**)&host, kHost * sizeof(double), hipHostMallocDefault));
@@ -2516,21 +2895,27 @@ void mmc_ctx::ensure_entries(size_t need) {
 // only when some node has moved skin/2 since the last build).  Two host
 // synchronisations size the grid entries and the pair list exactly.
 template <class T>
-void mmc_ctx::build_list(Lat<T> Lt) {
-  k_count<T><<<Pc, kBlock, 0, stream>>>(Lt, inst_d, tc, x, box, cnt, iflags());
+void mmc_ctx::build_list(Lat<T> Lt, const Blk* ctab, int cblocks, const Blk* ntab, int nblocks) {
+  const bool part = ctab != nullptr;
+  if (!part) {
+    ctab = tc;
+    cblocks = Pc;
+  }
+  k_count<T><<<cblocks, kBlock, 0, stream>>>(Lt, inst_d, ctab, x, box, cnt, iflags());
   HIPCALL(hipcub::DeviceScan::ExclusiveSum(scan_tmp, scan_bytes, cnt, start, table + 1, stream));
   unsigned total = 0;
   read(&total, start + table, sizeof(unsigned));
   ensure_entries(size_t(total) + 1024);
   tick(MMC_K_FILL);
-  k_fill<<<Pc, kBlock, 0, stream>>>(inst_d, tc, box, cnt, start, entries, flags);
+  k_fill<<<cblocks, kBlock, 0, stream>>>(inst_d, ctab, box, cnt, start, entries, flags);
   tock(MMC_K_FILL);
-  for (;;) {
-    k_pairs<T><<<Pc, kBlock, 0, stream>>>(Lt, inst_d, tc, slab, sinst, x, box, start, entries, pcount, plist, flags);
-    int fl = 0;
-    read(&fl, flags, sizeof(int));
-    if (!(fl & 8)) break;
-    // some shape has more partners than slots: widen the slab and enumerate again
+  k_pairs<T><<<cblocks, kBlock, 0, stream>>>(Lt, inst_d, ctab, slab, sinst, x, box, start, entries, pcount, plist,
+                                             flags);
+  int fl = 0;
+  read(&fl, flags, sizeof(int));
+  if (fl & 8) {
+    // some shape has more partners than slots: widen the slab and enumerate
+    // every instance again (the wider slab holds no list yet)
     int cleared = fl & ~8;
     HIPCALL(hipMemcpy(flags, &cleared, sizeof(int), hipMemcpyHostToDevice));
     HIPCALL(hipFree(plist));
@@ -2538,8 +2923,13 @@ void mmc_ctx::build_list(Lat<T> Lt) {
     if (slab >= (1 << 16)) throw Fail(MMC_ERR_CAPACITY, "contact partner slab exceeds 65536 per shape");
     slab *= 2;
     plist = dalloc<int>(size_t(NS) * slab);
+    build_list<T>(Lt);
+    return;
   }
-  HIPCALL(hipMemcpyAsync(xref, x, 3 * size_t(N) * sizeof(double), hipMemcpyDeviceToDevice, stream));
+  if (part)
+    k_copy_xref<<<nblocks, kBlock, 0, stream>>>(ntab, N, x, xref);
+  else
+    HIPCALL(hipMemcpyAsync(xref, x, 3 * size_t(N) * sizeof(double), hipMemcpyDeviceToDevice, stream));
   HIPCALL(hipGetLastError());
   // publish the list location for the graph kernels (stream-ordered before
   // any later replay; staged through pinned memory)
@@ -2821,6 +3211,48 @@ void mmc_ctx::capture_iteration(Lat<T> Lt) {
   HIPCALL(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0));
 }
 
+// The contact lists of the instances of `held` only (their trial positions are
+// in x): the other instances' lists and reference positions are left as they
+// are, and stay exact for them.
+void mmc_ctx::rebuild_held(const std::vector<int>& held) {
+  if (held.size() == size_t(n_inst)) {
+    rebuild();
+    return;
+  }
+  std::vector<Blk> c, nb;
+  for (int i : held) {
+    const Inst& in = hinst[size_t(i)];
+    c.insert(c.end(), htc.begin() + in.pc0, htc.begin() + in.pc1);
+    nb.insert(nb.end(), htn.begin() + in.pn0, htn.begin() + in.pn1);
+  }
+  HIPCALL(hipStreamSynchronize(stream));
+  HIPCALL(hipMemcpy(rtc, c.data(), c.size() * sizeof(Blk), hipMemcpyHostToDevice));
+  HIPCALL(hipMemcpy(rtn, nb.data(), nb.size() * sizeof(Blk), hipMemcpyHostToDevice));
+  if (precision == MMC_MIXED)
+    build_list<float>(lat32(), rtc, int(c.size()), rtn, int(nb.size()));
+  else
+    build_list<double>(lat64(), rtc, int(c.size()), rtn, int(nb.size()));
+}
+
+// One launch of the block solver over the instances of `set`.
+template <class T>
+void mmc_ctx::solve_blocks(Lat<T> Lt, const std::vector<int>& set, int max_trials) {
+  HIPCALL(hipMemcpyAsync(gact, set.data(), set.size() * sizeof(int), hipMemcpyHostToDevice, stream));
+  const size_t n6 = 6 * size_t(N);
+  const int nb = int(set.size());
+  if (max_hist <= 10)
+    k_solve_block<T, 10><<<nb, kSolveThreads, 0, stream>>>(
+        Lt, inst_d, ctl, gact, max_trials, zbuf[0], zbuf[1], gbuf[0], gbuf[1], Sh, Yh, n6, p, x, (double*)R, xref,
+        nbuf[0], nbuf[1], ubuf[0], ubuf[1], cbuf[0], cbuf[1], pcount, lref, rec, cf_sph, cf_cap, iflags());
+  else
+    k_solve_block<T, kMaxHist><<<nb, kSolveThreads, 0, stream>>>(
+        Lt, inst_d, ctl, gact, max_trials, zbuf[0], zbuf[1], gbuf[0], gbuf[1], Sh, Yh, n6, p, x, (double*)R, xref,
+        nbuf[0], nbuf[1], ubuf[0], ubuf[1], cbuf[0], cbuf[1], pcount, lref, rec, cf_sph, cf_cap, iflags());
+  HIPCALL(hipGetLastError());
+  // the set is staged from host memory that the caller may reuse
+  HIPCALL(hipStreamSynchronize(stream));
+}
+
 // Block tables of the device loop for the instances of `set` (ascending):
 // each instance's blocks of the full tables, in order, and its ranges there;
 // then the loop is captured over them.
@@ -2956,9 +3388,13 @@ __global__ __launch_bounds__(kBlock) void k_release_hold(Ctl* __restrict__ Cs, i
 // initial evaluation.  Inactive instances (active[b] == 0) are left alone.
 void mmc_ctx::minimize_graph(const double* tops, const int32_t* active, mmc_opt_stats* st) {
   const int n = n_inst;
+  auto clock = [] { return std::chrono::steady_clock::now(); };
+  auto secs = [](auto a, auto b) { return std::chrono::duration<double>(b - a).count(); };
+  const auto t_init = clock();
   zero(zbuf[0], long(6 * size_t(N)));
   terms_out0 = true;
   evaluate(zbuf[0], nullptr, 0.0, nullptr, gbuf[0], tops, false, rec_slot(0));
+  loop_s[0] += secs(t_init, clock());
   std::memset(hctl, 0, size_t(n) * sizeof(Ctl));
   bool any = false;
   for (int b = 0; b < n; b++) {
@@ -2994,6 +3430,44 @@ void mmc_ctx::minimize_graph(const double* tops, const int32_t* active, mmc_opt_
     any = any || h.status == ST_RUNNING;
   }
   HIPCALL(hipMemcpyAsync(ctl, hctl, size_t(n) * sizeof(Ctl), hipMemcpyHostToDevice, stream));
+  if (block_engine) {
+    // launches of kSolveBudget trials per workgroup: a lattice held for a
+    // list rebuild waits at most one budget for the others; the total is
+    // bounded as the graph loop's replays are
+    int max_it = 0, max_ls = 0;
+    for (const mmc_config& cf : icfg) max_it = std::max(max_it, cf.max_iterations), max_ls = std::max(max_ls, cf.max_line_search);
+    const int64_t guard = int64_t(max_it + 1) * (max_ls + 3) * 2 + 64;
+    int64_t trials = 0;
+    for (;;) {
+      std::vector<int> set;
+      int hold = 0;
+      for (int b = 0; b < n; b++) {
+        if (hctl[b].status == ST_RUNNING || hctl[b].status == ST_HOLD) set.push_back(b);
+        hold += hctl[b].status == ST_HOLD;
+      }
+      if (set.empty()) break;
+      if (hold > 0) {
+        // the held trials' positions are still in x: rebuild there and redo them
+        std::vector<int> held;
+        for (int b = 0; b < n; b++)
+          if (hctl[b].status == ST_HOLD) held.push_back(b);
+        rebuild_held(held);
+        k_release_hold<<<(n + kBlock - 1) / kBlock, kBlock, 0, stream>>>(ctl, n);
+      }
+      if (trials > guard) throw Fail(MMC_ERR_STATE, "device minimiser exceeded its trial bound");
+      trials += kSolveBudget;
+      const auto t_run = clock();
+      if (precision == MMC_MIXED)
+        solve_blocks<float>(lat32(), set, kSolveBudget);
+      else
+        solve_blocks<double>(lat64(), set, kSolveBudget);
+      HIPCALL(hipMemcpy2DAsync(hctl, sizeof(Ctl), ctl, sizeof(Ctl), 16 * sizeof(int), n, hipMemcpyDeviceToHost,
+                               stream));
+      HIPCALL(hipStreamSynchronize(stream));
+      loop_s[2] += secs(t_run, clock());
+      loop_polls++;
+    }
+  } else {
   // the loop's graph covers the running instances only; as a batch drains it
   // is captured again over the remaining ones (whenever they are at most
   // half of those captured), so that the replays of the last few lattices do
@@ -3006,8 +3480,11 @@ void mmc_ctx::minimize_graph(const double* tops, const int32_t* active, mmc_opt_
   };
   {
     std::vector<int> set = running_set();
+    const auto t_cap = clock();
     if (!set.empty() && (set != gset || !graph_exec)) capture(set);
+    loop_s[1] += secs(t_cap, clock());
   }
+  const auto t_loop = clock();
   // every replay advances each running phase machine by at least one step;
   // bound the total so that a logic error cannot spin forever
   int max_it = 0, max_ls = 0;
@@ -3030,7 +3507,10 @@ void mmc_ctx::minimize_graph(const double* tops, const int32_t* active, mmc_opt_
   while (state.first + state.second > 0) {
     if (state.second > 0) {
       // the held trials' positions are still in x: rebuild there and redo them
-      rebuild();
+      std::vector<int> held;
+      for (int b = 0; b < n; b++)
+        if (hctl[b].status == ST_HOLD) held.push_back(b);
+      rebuild_held(held);
       k_release_hold<<<(n + kBlock - 1) / kBlock, kBlock, 0, stream>>>(ctl, n);
     }
     for (int r = 0; r < batch; r++) HIPCALL(hipGraphLaunch(graph_exec, stream));
@@ -3039,8 +3519,15 @@ void mmc_ctx::minimize_graph(const double* tops, const int32_t* active, mmc_opt_
     batch = std::min(8, batch * 2);
     state = heads();
     if (replays > guard) throw Fail(MMC_ERR_STATE, "device minimiser exceeded its replay bound");
+    loop_polls++;
     const int left = state.first + state.second;
-    if (left > 0 && 2 * size_t(left) <= gset.size()) capture(running_set());
+    if (left > 0 && 2 * size_t(left) <= gset.size()) {
+      const auto t_cap = clock();
+      capture(running_set());
+      loop_s[1] += secs(t_cap, clock());
+    }
+  }
+  loop_s[2] += secs(t_loop, clock());
   }
   HIPCALL(hipMemcpyAsync(hctl, ctl, size_t(n) * sizeof(Ctl), hipMemcpyDeviceToHost, stream));
   HIPCALL(hipStreamSynchronize(stream));
@@ -3186,9 +3673,13 @@ int mmc_minimize_batch(mmc_ctx* c, const double* tops, const int32_t* active, mm
   return guarded(c, [&] {
     if (!tops || !stats) throw Fail(MMC_ERR_ARG, "NULL argument");
     c->evaluations = 0;
+    const auto t0 = std::chrono::steady_clock::now();
     c->minimize_graph(tops, active, stats);
+    const auto t1 = std::chrono::steady_clock::now();
     c->final_evaluation(tops, stats, fin);
     HIPCALL(hipStreamSynchronize(c->stream));
+    c->loop_s[3] += std::chrono::duration<double>(t1 - t0).count();
+    c->loop_s[4] += std::chrono::duration<double>(std::chrono::steady_clock::now() - t1).count();
   });
 }
 

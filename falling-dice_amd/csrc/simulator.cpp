@@ -645,6 +645,7 @@ std::vector<BatchSimulator::Outcome> BatchSimulator::run(bool verbose) {
   std::vector<mmc_opt_stats> stats(nz);
   std::vector<mmc_objective_out> fin(nz);
   std::vector<mmc_state_stats> st(nz);
+  const auto t_run = std::chrono::steady_clock::now();
   for (;;) {
     int any = 0;
     for (int b = 0; b < n; b++) {
@@ -685,7 +686,9 @@ std::vector<BatchSimulator::Outcome> BatchSimulator::run(bool verbose) {
       std::cerr << "[batch] round " << rounds_ << ": " << any << " of " << n << " lattices running, " << frames
                 << " frames" << std::endl;
     }
+    auto t_min = std::chrono::steady_clock::now();
     check(mmc_minimize_batch(ctx_, tops.data(), active.data(), stats.data(), fin.data()));
+    phase_s_[0] += std::chrono::duration<double>(std::chrono::steady_clock::now() - t_min).count();
     int commits = 0;
     for (int b = 0; b < n; b++) {
       mask[size_t(b)] = 0;
@@ -714,8 +717,10 @@ std::vector<BatchSimulator::Outcome> BatchSimulator::run(bool verbose) {
       }
     }
     if (!commits) continue;
+    auto t_commit = std::chrono::steady_clock::now();
     check(mmc_commit_batch(ctx_, mask.data()));
     check(mmc_read_stats_batch(ctx_, st.data()));
+    phase_s_[1] += std::chrono::duration<double>(std::chrono::steady_clock::now() - t_commit).count();
     for (int b = 0; b < n; b++) {
       if (!mask[size_t(b)]) continue;
       Schedule& s = sch[size_t(b)];
@@ -746,6 +751,10 @@ std::vector<BatchSimulator::Outcome> BatchSimulator::run(bool verbose) {
       s.time += cf.stepTime;
     }
   }
+  phase_s_[2] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_run).count();
+  if (verbose)
+    std::cerr << "[batch] " << rounds_ << " rounds, " << phase_s_[2] << " s: minimize " << phase_s_[0]
+              << " s, commit + statistics " << phase_s_[1] << " s" << std::endl;
   return out;
 }
 

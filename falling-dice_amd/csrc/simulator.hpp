@@ -184,6 +184,7 @@ class BatchSimulator {
   std::vector<HostLattice> lats_;
   mmc_ctx* ctx_ = nullptr;
   int64_t rounds_ = 0;
+  double phase_s_[3] = {0, 0, 0};  // seconds in minimize, in commit + statistics, in run
   void check(int rc) const;
 };
 

@@ -35,6 +35,16 @@ def po():
     return py_oracle
 
 
+@pytest.fixture(params=["block", "graph"])
+def loop_engine(request, monkeypatch):
+    """Device-loop engine of the contexts the test creates (MMC_DEVICE_LOOP is
+    read at context creation): the graph-replayed four-kernel loop ("graph",
+    the default) or the workgroup-per-lattice solver ("block", opt-in, for
+    lattices of up to 512 nodes)."""
+    monkeypatch.setenv("MMC_DEVICE_LOOP", request.param)
+    return request.param
+
+
 @pytest.fixture(scope="session")
 def gpu(mmc):
     """The GPU library; fails loudly (no fallback) when it is not built."""

@@ -3,7 +3,10 @@
 A batch concatenates its lattices in one device context; every lattice keeps
 its own block partition, reduction order and control block, so each lattice's
 frames must equal, bit for bit, those of mmc_sim_run on that lattice alone
-(which the parity tests pin against the oracle)."""
+(which the parity tests pin against the oracle).  Both device-loop engines:
+the workgroup-per-lattice solver runs each lattice in its own workgroup, the
+graph-replayed loop runs every lattice's blocks in every replay (and re-captures
+over the running ones as the batch drains)."""
 import numpy as np
 import pytest
 
@@ -26,7 +29,7 @@ def _same(fa, fb):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_batch_equals_single_runs(gpu, precision):
+def test_batch_equals_single_runs(gpu, precision, loop_engine):
     mmc = gpu
     prec = mmc.MMC_FP64 if precision == "fp64" else mmc.MMC_MIXED
     cfgs = _configs(mmc)
@@ -38,7 +41,7 @@ def test_batch_equals_single_runs(gpu, precision):
         _same(fb, fs)
 
 
-def test_batch_isolates_a_failed_lattice(gpu):
+def test_batch_isolates_a_failed_lattice(gpu, loop_engine):
     """A lattice whose equilibrium fails stops with the facade's error text;
     the others finish unchanged."""
     mmc = gpu

@@ -108,7 +108,7 @@ def test_objective_locked_and_inf(gpu, po):
     assert rg.finite == 0 and np.isinf(rg.energy) and np.all(gb == 0)
 
 
-def test_minimize_commit_parity(gpu, po):
+def test_minimize_commit_parity(gpu, po, loop_engine):
     """Both solvers stop anywhere inside the gradient tolerance (1e-6): the
     equilibria agree to that band (force ~1e-6, positions ~1e-7 relative), not
     to round-off."""
@@ -132,7 +132,7 @@ def test_minimize_commit_parity(gpu, po):
         assert np.max(np.abs(pg - po_)) <= 2e-4 * max(1e-3, np.max(np.abs(po_)))
 
 
-def test_minimize_deterministic(gpu, po):
+def test_minimize_deterministic(gpu, po, loop_engine):
     mmc = gpu
     cfg = cases(mmc)["bcc_quick"].replace(relative_escape=0, gradient_rms_tolerance=1e-5,
                                            gradient_max_tolerance=1e-4, max_iterations=400)
@@ -197,7 +197,7 @@ def _frames_error(fg, fo):
     }
 
 
-def test_run_config1_parity(gpu, po):
+def test_run_config1_parity(gpu, po, loop_engine):
     """Full 60 + 60 step schedule of the 8x8x4 cubic lattice (plain FP64 GPU
     path) against the oracle's force-displacement curve."""
     mmc = gpu
@@ -218,7 +218,7 @@ def test_run_config1_parity(gpu, po):
     assert err["loading_work"] <= 2e-5 and err["dissipated"] <= 5e-5
 
 
-def test_device_loop_matches_host_loop(gpu, po, monkeypatch):
+def test_device_loop_matches_host_loop(gpu, po, loop_engine, monkeypatch):
     """The graph-replayed phase machine and the host-driven loop implement the
     same algorithm: same convergence, equilibria inside the tolerance band.
     The two sum in different orders, so their iterates part within the band
@@ -251,7 +251,7 @@ def test_device_loop_matches_host_loop(gpu, po, monkeypatch):
 # (FP32 shear/rotation element arithmetic, FP64 axial strain, positions and
 # reductions, approximate-Wolfe acceptance below the FP32 energy noise)
 # ---------------------------------------------------------------------------
-def test_per_term_line_search_matches_oracle(gpu, po):
+def test_per_term_line_search_matches_oracle(gpu, po, loop_engine):
     """line_search = 1 on both sides, at a gradient tolerance (1e-7) below the
     total-energy Armijo floor of the 8x8x4 lattice: both converge, and agree
     to the tighter band."""
@@ -296,7 +296,7 @@ def test_objective_parity_mixed(gpu, po, case, state):
     assert abs(rg.reaction - ro.reaction) <= 1e-12 * max(ro.reaction, 1.0)
 
 
-def test_mixed_minimize_matches_fp64(gpu, po):
+def test_mixed_minimize_matches_fp64(gpu, po, loop_engine):
     """Mixed-precision equilibria against the FP64 oracle at gradient
     tolerance 1e-7: platen force within 1e-5 relative."""
     mmc = gpu
@@ -314,7 +314,7 @@ def test_mixed_minimize_matches_fp64(gpu, po):
         ctx.commit()
 
 
-def test_run_config1_mixed(gpu, po):
+def test_run_config1_mixed(gpu, po, loop_engine):
     """Full 60 + 60 schedule of the 8x8x4 lattice with the mixed policy against
     the oracle's FP64 curve (same band as the FP64 path at tolerance 1e-6)."""
     mmc = gpu
