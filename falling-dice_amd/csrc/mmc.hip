@@ -338,9 +338,17 @@ __device__ __forceinline__ closest shape_closest(bool capA, v3<double> a0, v3<do
 }
 
 // Verlet-list build: per shape, partners (either side) closer than target +
-// skin, written to the shape's slab of S.slab slots in the enumeration order
-// (cells in fixed order, sorted buckets); pcount gets the true count, and a
-// count above the slab raises flag 8 (the host widens the slab and rebuilds).
+// skin, written to the shape's slab of S.slab slots; pcount gets the true
+// count, and a count above the slab raises flag 8 (the host widens the slab
+// and rebuilds).  One wavefront per shape: lane c takes cell c of the shape's
+// box (64 cells at a time) and the lanes walk their buckets in step, one
+// entry each per round; a round's partners are appended in lane order.  Every
+// overlapping pair is kept only in the cell that holds the low corner of the
+// two boxes' overlap (once per partner); lane 0 then sorts the list and drops
+// the duplicates that cells hashing to one bucket produce, so the list, and
+// every contact sum over it, does not depend on the order of the bucket fill.
+constexpr int kPairWaves = kBlock / kWave;                 // shapes per workgroup
+constexpr int kPairSub = kBlock / kPairWaves;              // workgroups per table block
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_pairs(Lat<T> L, const Inst* __restrict__ I, const Blk* __restrict__ tc,
                                                   int slab_in, const int* __restrict__ sinst,
@@ -348,9 +356,11 @@ __global__ __launch_bounds__(kBlock) void k_pairs(Lat<T> L, const Inst* __restri
                                                   const unsigned* __restrict__ start, const int* __restrict__ entries,
                                                   unsigned* __restrict__ pcount, int* __restrict__ plist,
                                                   int* __restrict__ flags) {
-  const Blk bk = tc[blockIdx.x];
-  if (int(threadIdx.x) >= bk.count) return;
-  const int s = bk.first + threadIdx.x;
+  const Blk bk = tc[blockIdx.x / kPairSub];
+  const int lane = threadIdx.x % kWave;
+  const int li = (blockIdx.x % kPairSub) * kPairWaves + int(threadIdx.x) / kWave;
+  if (li >= bk.count) return;  // the whole wavefront
+  const int s = bk.first + li;
   const Inst& in = I[bk.inst];
   const Scalars& S = in.S;
   float lo[3], hi[3];
@@ -364,37 +374,59 @@ __global__ __launch_bounds__(kBlock) void k_pairs(Lat<T> L, const Inst* __restri
     int s0, s1;
     shape_geom(L, in, x, s, sa, sb, sr, s0, s1);
     const bool scap = s - in.s0 >= in.N;
-    for (int cj = c0[1]; cj <= c1[1]; cj++)
-      for (int ck = c0[2]; ck <= c1[2]; ck++)
-        for (int ci = c0[0]; ci <= c1[0]; ci++) {
-          unsigned h = cell_hash(ci, cj, ck, bk.inst, S.table_mask);
-          unsigned e0 = start[h], e1 = min(start[h + 1], (unsigned)S.entry_cap);
-          for (unsigned e = e0; e < e1; e++) {
-            int o = entries[e];
-            if (o == s || sinst[o] != bk.inst) continue;
+    const int nx = c1[0] - c0[0] + 1, ny = c1[1] - c0[1] + 1, ncell = nx * ny * (c1[2] - c0[2] + 1);
+    for (int cb = 0; cb < ncell; cb += kWave) {
+      const int k = cb + lane;
+      int ci = 0, cj = 0, ck = 0;
+      unsigned e0 = 0, e1 = 0;
+      if (k < ncell) {
+        ci = c0[0] + k % nx;
+        cj = c0[1] + (k / nx) % ny;
+        ck = c0[2] + k / (nx * ny);
+        const unsigned h = cell_hash(ci, cj, ck, bk.inst, S.table_mask);
+        e0 = start[h];
+        e1 = max(e0, min(start[h + 1], (unsigned)S.entry_cap));
+      }
+      const unsigned len = e1 - e0;
+      unsigned rounds = len;
+#pragma unroll
+      for (int o = kWave / 2; o > 0; o >>= 1) rounds = max(rounds, unsigned(__shfl_xor(int(rounds), o, kWave)));
+      for (unsigned r = 0; r < rounds; r++) {
+        bool found = false;
+        int o = -1;
+        if (r < len) {
+          o = entries[e0 + r];
+          if (o != s && sinst[o] == bk.inst) {
             const float4 ol = reinterpret_cast<const float4*>(box)[2 * o];
             const float4 oh = reinterpret_cast<const float4*>(box)[2 * o + 1];
             const float olo[3] = {ol.x, ol.y, ol.z}, ohi[3] = {oh.x, oh.y, oh.z};
-            if (!(olo[0] <= hi[0] && lo[0] <= ohi[0] && olo[1] <= hi[1] && lo[1] <= ohi[1] && olo[2] <= hi[2] &&
-                  lo[2] <= ohi[2]))
-              continue;
-            if (cell_of(fmaxf(lo[0], olo[0]), S.inv_cell) != ci || cell_of(fmaxf(lo[1], olo[1]), S.inv_cell) != cj ||
-                cell_of(fmaxf(lo[2], olo[2]), S.inv_cell) != ck)
-              continue;
-            v3<double> oa, ob;
-            double orr;
-            int o0, o1;
-            shape_geom(L, in, x, o, oa, ob, orr, o0, o1);
-            const bool low = s < o, ocap = o - in.s0 >= in.N;
-            closest cp = low ? shape_closest(scap, sa, sb, ocap, oa, ob) : shape_closest(ocap, oa, ob, scap, sa, sb);
-            double dist = norm(cp.pb - cp.pa);
-            if (!(dist < sr + orr + S.margin + S.skin)) continue;
-            if (excluded(L, s0, s1, o0, o1)) continue;
-            if (n < slab) plist[base + n] = o;
-            n++;
+            if (olo[0] <= hi[0] && lo[0] <= ohi[0] && olo[1] <= hi[1] && lo[1] <= ohi[1] && olo[2] <= hi[2] &&
+                lo[2] <= ohi[2] && cell_of(fmaxf(lo[0], olo[0]), S.inv_cell) == ci &&
+                cell_of(fmaxf(lo[1], olo[1]), S.inv_cell) == cj && cell_of(fmaxf(lo[2], olo[2]), S.inv_cell) == ck) {
+              v3<double> oa, ob;
+              double orr;
+              int o0, o1;
+              shape_geom(L, in, x, o, oa, ob, orr, o0, o1);
+              const bool low = s < o, ocap = o - in.s0 >= in.N;
+              closest cp =
+                  low ? shape_closest(scap, sa, sb, ocap, oa, ob) : shape_closest(ocap, oa, ob, scap, sa, sb);
+              const double dist = norm(cp.pb - cp.pa);
+              found = dist < sr + orr + S.margin + S.skin && !excluded(L, s0, s1, o0, o1);
+            }
           }
         }
+        const unsigned long long m = __ballot(found);
+        if (found) {
+          const unsigned pos = n + unsigned(__popcll(m & ((1ull << lane) - 1ull)));
+          if (pos < slab) plist[base + pos] = o;
+        }
+        n += unsigned(__popcll(m));
+      }
+    }
   }
+  // the other lanes' slab writes are read below by lane 0 of the same wavefront
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  if (lane != 0) return;
   pcount[s] = n;
   if (n > slab) {
     atomicOr(flags, 8);
@@ -2450,7 +2482,7 @@ struct mmc_ctx {
   // host-clock seconds: initial evaluation, graph captures, replay loop,
   // minimize_graph in all, final evaluation (reported at destruction when
   // MMC_LOOP_TIMING is set)
-  double loop_s[5] = {0, 0, 0, 0, 0};
+  double loop_s[6] = {0, 0, 0, 0, 0, 0};  // [5]: contact list builds
   int64_t loop_polls = 0;
   Ctl* tctl = nullptr;  // scratch control blocks of time_loop_kernel
   double* dspart = nullptr;    // [10][Pd] partials of the trial state kernel
@@ -2550,9 +2582,10 @@ mmc_ctx::~mmc_ctx() {
   if (std::getenv("MMC_LOOP_TIMING"))
     std::fprintf(stderr,
                  "[loop] %d instances: initial evaluations %.3f s, captures %lld (%.3f s), replay loop %.3f s "
-                 "(%lld graph launches, %lld polls), minimize %.3f s, final evaluations %.3f s\n",
+                 "(%lld graph launches, %lld polls), minimize %.3f s, final evaluations %.3f s, "
+                 "%lld contact list builds %.3f s\n",
                  n_inst, loop_s[0], (long long)captures, loop_s[1], loop_s[2], (long long)graph_replays,
-                 (long long)loop_polls, loop_s[3], loop_s[4]);
+                 (long long)loop_polls, loop_s[3], loop_s[4], (long long)rebuilds, loop_s[5]);
   void* ptrs[] = {bpos, bquat, mass, lock, ba, bb, L0, rad, area, inert, e0, plastic, inc_off, inc, adj_off, adj, x, R,
                   rec, trial, cf_sph, cf_cap, box, cnt, start, entries, flags, scan_tmp, spart, cpart, bpart, gpart,
                   vpart, stb, stn, stout, record, scal, z, g, p, zc, gc, zt, zin, Sh, Yh, xref, pcount, plist,
@@ -2896,6 +2929,7 @@ void mmc_ctx::ensure_entries(size_t need) {
 // synchronisations size the grid entries and the pair list exactly.
 template <class T>
 void mmc_ctx::build_list(Lat<T> Lt, const Blk* ctab, int cblocks, const Blk* ntab, int nblocks) {
+  const auto t_build = std::chrono::steady_clock::now();
   const bool part = ctab != nullptr;
   if (!part) {
     ctab = tc;
@@ -2909,8 +2943,8 @@ void mmc_ctx::build_list(Lat<T> Lt, const Blk* ctab, int cblocks, const Blk* nta
   tick(MMC_K_FILL);
   k_fill<<<cblocks, kBlock, 0, stream>>>(inst_d, ctab, box, cnt, start, entries, flags);
   tock(MMC_K_FILL);
-  k_pairs<T><<<cblocks, kBlock, 0, stream>>>(Lt, inst_d, ctab, slab, sinst, x, box, start, entries, pcount, plist,
-                                             flags);
+  k_pairs<T><<<cblocks * kPairSub, kBlock, 0, stream>>>(Lt, inst_d, ctab, slab, sinst, x, box, start, entries,
+                                                        pcount, plist, flags);
   int fl = 0;
   read(&fl, flags, sizeof(int));
   if (fl & 8) {
@@ -2941,6 +2975,7 @@ void mmc_ctx::build_list(Lat<T> Lt, const Blk* ctab, int cblocks, const Blk* nta
   HIPCALL(hipStreamSynchronize(stream));
   list_valid = true;
   rebuilds++;
+  loop_s[5] += std::chrono::duration<double>(std::chrono::steady_clock::now() - t_build).count();
 }
 
 // One objective evaluation of every instance at its platen (tops_d), records
