@@ -381,13 +381,14 @@ def main():
     gather_ms = loop_us["gather_g"] * 1e-3
     g_bytes = gather_loop_bytes(N, B, cfg.history)
     ach = g_bytes / (gather_ms * 1e-3) / 1e9
+    loop_shape = sim.loop_shape()
     b_bytes = beam_kernel_bytes(N, B, prec)
     b_ach = b_bytes / (beam_ms * 1e-3) / 1e9
     roof = {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(ach / HBM_PEAK_GBS, 4),
             "traffic": GATHER_TRAFFIC_BYTES.get((args.nx, args.ny, args.nz)), "kernel": "k_gather_g",
             "bytes_per_launch": g_bytes, "mean_launch_us": round(gather_ms * 1e3, 3), "launches_timed": launches,
-            "loop_kernel_us": loop_us,
+            "loop_kernel_us": loop_us, "loop_shape": loop_shape,
             "beam_kernel": {"kernel": "k_beam", "bytes_per_launch": b_bytes, "mean_launch_us": round(beam_ms * 1e3, 3),
                             "achieved": round(b_ach, 2), "frac": round(b_ach / HBM_PEAK_GBS, 4),
                             "traffic": BEAM_TRAFFIC_BYTES.get((args.nx, args.ny, args.nz))}}

@@ -340,15 +340,17 @@ __device__ __forceinline__ closest shape_closest(bool capA, v3<double> a0, v3<do
 // Verlet-list build: per shape, partners (either side) closer than target +
 // skin, written to the shape's slab of S.slab slots; pcount gets the true
 // count, and a count above the slab raises flag 8 (the host widens the slab
-// and rebuilds).  One wavefront per shape: lane c takes cell c of the shape's
-// box (64 cells at a time) and the lanes walk their buckets in step, one
-// entry each per round; a round's partners are appended in lane order.  Every
-// overlapping pair is kept only in the cell that holds the low corner of the
-// two boxes' overlap (once per partner); lane 0 then sorts the list and drops
+// and rebuilds).  kPairLanes lanes per shape (four shapes per wavefront): lane
+// c of a shape's segment takes cells c, c + kPairLanes, ... of the shape's
+// box, and the lanes walk their buckets in step, one entry each per round; a
+// round's partners are appended in lane order.  Every overlapping pair is
+// kept only in the cell that holds the low corner of the two boxes' overlap
+// (once per partner); the segment's first lane then sorts the list and drops
 // the duplicates that cells hashing to one bucket produce, so the list, and
 // every contact sum over it, does not depend on the order of the bucket fill.
-constexpr int kPairWaves = kBlock / kWave;                 // shapes per workgroup
-constexpr int kPairSub = kBlock / kPairWaves;              // workgroups per table block
+constexpr int kPairLanes = 16;                            // lanes per shape
+constexpr int kPairShapes = kBlock / kPairLanes;          // shapes per workgroup
+constexpr int kPairSub = kBlock / kPairShapes;            // workgroups per table block
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_pairs(Lat<T> L, const Inst* __restrict__ I, const Blk* __restrict__ tc,
                                                   int slab_in, const int* __restrict__ sinst,
@@ -357,76 +359,84 @@ __global__ __launch_bounds__(kBlock) void k_pairs(Lat<T> L, const Inst* __restri
                                                   unsigned* __restrict__ pcount, int* __restrict__ plist,
                                                   int* __restrict__ flags) {
   const Blk bk = tc[blockIdx.x / kPairSub];
-  const int lane = threadIdx.x % kWave;
-  const int li = (blockIdx.x % kPairSub) * kPairWaves + int(threadIdx.x) / kWave;
-  if (li >= bk.count) return;  // the whole wavefront
-  const int s = bk.first + li;
+  const int lane = threadIdx.x % kWave, sl = lane % kPairLanes, seg = lane / kPairLanes;
+  const int li = (blockIdx.x % kPairSub) * kPairShapes + int(threadIdx.x) / kPairLanes;
+  // a whole wavefront returns together (its four shapes lie past the block)
+  if ((blockIdx.x % kPairSub) * kPairShapes + int(threadIdx.x) / kWave * (kWave / kPairLanes) >= bk.count) return;
+  const bool real = li < bk.count;
+  const int s = bk.first + (real ? li : 0);
   const Inst& in = I[bk.inst];
   const Scalars& S = in.S;
   float lo[3], hi[3];
   int c0[3], c1[3];
   const unsigned slab = unsigned(slab_in);
   const size_t base = size_t(s) * slab;
+  const unsigned long long segmask = ((1ull << kPairLanes) - 1ull) << (seg * kPairLanes);
   unsigned n = 0;
-  if (load_cells(box, s, S.inv_cell, lo, hi, c0, c1)) {
-    v3<double> sa, sb;
-    double sr;
-    int s0, s1;
+  const bool boxed = real && load_cells(box, s, S.inv_cell, lo, hi, c0, c1);
+  v3<double> sa{0, 0, 0}, sb{0, 0, 0};
+  double sr = 0;
+  int s0 = 0, s1 = 0, nx = 1, ny = 1, ncell = 0;
+  if (boxed) {
     shape_geom(L, in, x, s, sa, sb, sr, s0, s1);
-    const bool scap = s - in.s0 >= in.N;
-    const int nx = c1[0] - c0[0] + 1, ny = c1[1] - c0[1] + 1, ncell = nx * ny * (c1[2] - c0[2] + 1);
-    for (int cb = 0; cb < ncell; cb += kWave) {
-      const int k = cb + lane;
-      int ci = 0, cj = 0, ck = 0;
-      unsigned e0 = 0, e1 = 0;
-      if (k < ncell) {
-        ci = c0[0] + k % nx;
-        cj = c0[1] + (k / nx) % ny;
-        ck = c0[2] + k / (nx * ny);
-        const unsigned h = cell_hash(ci, cj, ck, bk.inst, S.table_mask);
-        e0 = start[h];
-        e1 = max(e0, min(start[h + 1], (unsigned)S.entry_cap));
-      }
-      const unsigned len = e1 - e0;
-      unsigned rounds = len;
+    nx = c1[0] - c0[0] + 1;
+    ny = c1[1] - c0[1] + 1;
+    ncell = nx * ny * (c1[2] - c0[2] + 1);
+  }
+  const bool scap = s - in.s0 >= in.N;
+  int cmax = ncell;  // wavefront-uniform bounds for the loops below
 #pragma unroll
-      for (int o = kWave / 2; o > 0; o >>= 1) rounds = max(rounds, unsigned(__shfl_xor(int(rounds), o, kWave)));
-      for (unsigned r = 0; r < rounds; r++) {
-        bool found = false;
-        int o = -1;
-        if (r < len) {
-          o = entries[e0 + r];
-          if (o != s && sinst[o] == bk.inst) {
-            const float4 ol = reinterpret_cast<const float4*>(box)[2 * o];
-            const float4 oh = reinterpret_cast<const float4*>(box)[2 * o + 1];
-            const float olo[3] = {ol.x, ol.y, ol.z}, ohi[3] = {oh.x, oh.y, oh.z};
-            if (olo[0] <= hi[0] && lo[0] <= ohi[0] && olo[1] <= hi[1] && lo[1] <= ohi[1] && olo[2] <= hi[2] &&
-                lo[2] <= ohi[2] && cell_of(fmaxf(lo[0], olo[0]), S.inv_cell) == ci &&
-                cell_of(fmaxf(lo[1], olo[1]), S.inv_cell) == cj && cell_of(fmaxf(lo[2], olo[2]), S.inv_cell) == ck) {
-              v3<double> oa, ob;
-              double orr;
-              int o0, o1;
-              shape_geom(L, in, x, o, oa, ob, orr, o0, o1);
-              const bool low = s < o, ocap = o - in.s0 >= in.N;
-              closest cp =
-                  low ? shape_closest(scap, sa, sb, ocap, oa, ob) : shape_closest(ocap, oa, ob, scap, sa, sb);
-              const double dist = norm(cp.pb - cp.pa);
-              found = dist < sr + orr + S.margin + S.skin && !excluded(L, s0, s1, o0, o1);
-            }
+  for (int o = kWave / 2; o > 0; o >>= 1) cmax = max(cmax, __shfl_xor(cmax, o, kWave));
+  for (int cb = 0; cb < cmax; cb += kPairLanes) {
+    const int k = cb + sl;
+    int ci = 0, cj = 0, ck = 0;
+    unsigned e0 = 0, e1 = 0;
+    if (k < ncell) {
+      ci = c0[0] + k % nx;
+      cj = c0[1] + (k / nx) % ny;
+      ck = c0[2] + k / (nx * ny);
+      const unsigned h = cell_hash(ci, cj, ck, bk.inst, S.table_mask);
+      e0 = start[h];
+      e1 = max(e0, min(start[h + 1], (unsigned)S.entry_cap));
+    }
+    const unsigned len = e1 - e0;
+    unsigned rounds = len;
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) rounds = max(rounds, unsigned(__shfl_xor(int(rounds), o, kWave)));
+    for (unsigned r = 0; r < rounds; r++) {
+      bool found = false;
+      int o = -1;
+      if (r < len) {
+        o = entries[e0 + r];
+        if (o != s && sinst[o] == bk.inst) {
+          const float4 ol = reinterpret_cast<const float4*>(box)[2 * o];
+          const float4 oh = reinterpret_cast<const float4*>(box)[2 * o + 1];
+          const float olo[3] = {ol.x, ol.y, ol.z}, ohi[3] = {oh.x, oh.y, oh.z};
+          if (olo[0] <= hi[0] && lo[0] <= ohi[0] && olo[1] <= hi[1] && lo[1] <= ohi[1] && olo[2] <= hi[2] &&
+              lo[2] <= ohi[2] && cell_of(fmaxf(lo[0], olo[0]), S.inv_cell) == ci &&
+              cell_of(fmaxf(lo[1], olo[1]), S.inv_cell) == cj && cell_of(fmaxf(lo[2], olo[2]), S.inv_cell) == ck) {
+            v3<double> oa, ob;
+            double orr;
+            int o0, o1;
+            shape_geom(L, in, x, o, oa, ob, orr, o0, o1);
+            const bool low = s < o, ocap = o - in.s0 >= in.N;
+            closest cp = low ? shape_closest(scap, sa, sb, ocap, oa, ob) : shape_closest(ocap, oa, ob, scap, sa, sb);
+            const double dist = norm(cp.pb - cp.pa);
+            found = dist < sr + orr + S.margin + S.skin && !excluded(L, s0, s1, o0, o1);
           }
         }
-        const unsigned long long m = __ballot(found);
-        if (found) {
-          const unsigned pos = n + unsigned(__popcll(m & ((1ull << lane) - 1ull)));
-          if (pos < slab) plist[base + pos] = o;
-        }
-        n += unsigned(__popcll(m));
       }
+      const unsigned long long m = __ballot(found) & segmask;
+      if (found) {
+        const unsigned pos = n + unsigned(__popcll(m & ((1ull << lane) - 1ull)));
+        if (pos < slab) plist[base + pos] = o;
+      }
+      n += unsigned(__popcll(m));
     }
   }
-  // the other lanes' slab writes are read below by lane 0 of the same wavefront
+  // the segment's slab writes are read below by its first lane
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-  if (lane != 0) return;
+  if (sl != 0 || !real) return;
   pcount[s] = n;
   if (n > slab) {
     atomicOr(flags, 8);
@@ -519,10 +529,16 @@ __device__ __forceinline__ void contact_body(Lat<T> L, const Inst& in, const dou
     if (ct.out) ct.out[s] = v[0];
     if (ct.ref) v[3] = v[0] - ct.ref[s];
   }
-  block_reduce<4, NT>(v, 1u << 1);
-  if (threadIdx.x == 0)
+  // one partial column per wavefront (64 consecutive shapes of the instance):
+  // the fold over the columns does not depend on the workgroup size
+  constexpr int W = NT / kWave;
+  v[0] = wave_sum(v[0]);
+  v[1] = wave_max(v[1]);
+  v[2] = wave_sum(v[2]);
+  v[3] = wave_sum(v[3]);
+  if ((threadIdx.x & (kWave - 1)) == 0)
 #pragma unroll
-    for (int k = 0; k < 4; k++) part[k * P + blk] = v[k];
+    for (int k = 0; k < 4; k++) part[size_t(k) * P * W + size_t(blk) * W + threadIdx.x / kWave] = v[k];
 }
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_contact(Lat<T> L, const Inst* __restrict__ I, const Blk* __restrict__ tc,
@@ -641,10 +657,14 @@ __device__ __forceinline__ void beam_body(Lat<T> L, Material M, const double* __
     if (bt.out) bt.out[k] = v[0];
     if (bt.ref) v[3] = v[0] - bt.ref[k];
   }
-  block_reduce<4, NT>(v, 0u);
-  if (threadIdx.x == 0)
+  // one partial column per wavefront (64 consecutive beams of the instance):
+  // the fold over the columns does not depend on the workgroup size
+  constexpr int W = NT / kWave;
 #pragma unroll
-    for (int c = 0; c < 4; c++) part[c * P + blk] = v[c];
+  for (int c = 0; c < 4; c++) v[c] = wave_sum(v[c]);
+  if ((threadIdx.x & (kWave - 1)) == 0)
+#pragma unroll
+    for (int c = 0; c < 4; c++) part[size_t(c) * P * W + size_t(blk) * W + threadIdx.x / kWave] = v[c];
 }
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_beam(Lat<T> L, const Inst* __restrict__ I, const Blk* __restrict__ tb,
@@ -981,10 +1001,14 @@ struct DecParts {
   int Ps, Pc, Pb, Pg;
 };
 __device__ __forceinline__ bool row_max(int j) { return j == 3 || j == 7 || j == 10 || j == 18; }
+// (the beam and contact rows hold one column per wavefront of the kBlock-item
+// blocks: Pc and Pb count blocks)
+constexpr int kBW = kBlock / kWave;
 __device__ __forceinline__ DecRow decide_row(int j, const Inst& in, const DecParts& P) {
   if (j < 9) return {P.spart + size_t(j) * P.Ps + in.pn0, in.pn1 - in.pn0, row_max(j)};
-  if (j < 13) return {P.cpart + size_t(j - 9) * P.Pc + in.pc0, in.pc1 - in.pc0, row_max(j)};
-  if (j < 17) return {P.bpart + size_t(j - 13) * P.Pb + in.pb0, in.pb1 - in.pb0, false};
+  if (j < 13)
+    return {P.cpart + size_t(j - 9) * P.Pc * kBW + in.pc0 * kBW, (in.pc1 - in.pc0) * kBW, row_max(j)};
+  if (j < 17) return {P.bpart + size_t(j - 13) * P.Pb * kBW + in.pb0 * kBW, (in.pb1 - in.pb0) * kBW, false};
   return {P.gpart + size_t(j - 17) * P.Pg + in.pd0, in.pd1 - in.pd0, row_max(j)};
 }
 // Folds rows jbase + w + q js (w the wave, q < NQ) that lie in [0, R) into
@@ -1020,6 +1044,7 @@ __device__ __forceinline__ void rows_pass(Row row, const double* dummy, int R, i
       a = rw.mx ? fmax(a, e) : a + e;
     }
     if (j < R) {
+#pragma unroll 4
       for (int col = lane + kDecCols; col < rw.n; col += kWave) a = rw.mx ? fmax(a, rw.p[col]) : a + rw.p[col];
       lanes[wv * NQ + q][lane] = a;
     }
@@ -1672,9 +1697,10 @@ struct ListRef {
 
 // beams and contact shapes of one trial in a single launch: blocks [0, Pb)
 // evaluate beams, blocks [Pb, Pb + Pc) contact shapes (independent work; one
-// launch gap fewer per trial)
-template <class T>
-__global__ __launch_bounds__(kBlock) void k_elem_g(const Ctl* __restrict__ Cs, Lat<T> L, const Inst* __restrict__ I,
+// launch gap fewer per trial).  E: items (threads) per block, chosen by the
+// loop's tuner; the partials have one column per wavefront whatever E is.
+template <class T, int E>
+__global__ __launch_bounds__(E) void k_elem_g(const Ctl* __restrict__ Cs, Lat<T> L, const Inst* __restrict__ I,
                                                    const Blk* __restrict__ tb, const Blk* __restrict__ tc,
                                                    const double* __restrict__ x, const double* __restrict__ R,
                                                    double* __restrict__ rec, double* __restrict__ ub0,
@@ -1693,10 +1719,10 @@ __global__ __launch_bounds__(kBlock) void k_elem_g(const Ctl* __restrict__ Cs, L
   const bool on = int(threadIdx.x) < bk.count;
   if (beams) {
     TermBufs bt{par ? ub1 : ub0, par ? ub0 : ub1};
-    beam_body<T>(L, I[bk.inst].M, x, R, rec, nullptr, bt, bpart, item, on, blk, Pb);
+    beam_body<T, E>(L, I[bk.inst].M, x, R, rec, nullptr, bt, bpart, item, on, blk, Pb);
   } else {
     TermBufs ct{par ? cb1 : cb0, par ? cb0 : cb1};
-    contact_body<T>(L, I[bk.inst], x, pcount, lr->plist, lr->slab, cf_sph, cf_cap, ct, cpart, item, on, blk, Pc);
+    contact_body<T, E>(L, I[bk.inst], x, pcount, lr->plist, lr->slab, cf_sph, cf_cap, ct, cpart, item, on, blk, Pc);
   }
 }
 
@@ -1712,11 +1738,14 @@ constexpr int kGDecRows = 21 + 4 * kMaxHist + 5;
 // each instance keeps its own blocks in their own order, so its partials, and
 // every fold of them, do not depend on which other instances are listed.
 struct GRng {
-  int pd0, pd1, pb0, pb1, pc0, pc1, pad0, pad1;
+  int pd0, pd1;  // blocks of the 64-node table
+  int pb0, pb1;  // partial columns (wavefronts) of the beams
+  int pc0, pc1;  // partial columns (wavefronts) of the contact shapes
+  int pad0, pad1;
 };
 struct GraphParts {
   const double *spart, *cpart, *bpart, *gpart;  // [10][Pd], [4][Pc], [4][Pb], [kGRows][Pd]
-  int Pd, Pc, Pb;
+  int Pd, Pc, Pb;                               // Pc, Pb: wavefront columns
 };
 __device__ __forceinline__ DecRow graph_row(int j, const GRng& g, const GraphParts& P) {
   if (j < 9) return {P.spart + size_t(j) * P.Pd + g.pd0, g.pd1 - g.pd0, row_max(j)};
@@ -1962,7 +1991,7 @@ __global__ __launch_bounds__(kElemThreads) void k_gather_g(Lat<T> L, const Inst*
 
 }
 
-// The decision of one trial: kDecSplit blocks per instance, each folding its
+// The decision of one trial: D blocks per instance, each folding its
 // share of the decision rows (rows_pass) into drows, so that the row loads are
 // spread over several compute units; the block that arrives last at the
 // instance's counter reads the folded rows and decides (decide_block), and
@@ -1970,8 +1999,9 @@ __global__ __launch_bounds__(kElemThreads) void k_gather_g(Lat<T> L, const Inst*
 // the block's row stores, one thread releases them (device-scope release
 // fence: L2 write-back, for the other XCDs) and counts the arrival; only the
 // last arriver acquires (invalidates its cached lines) before reading.
-constexpr int kDecSplit = 8;
-constexpr int kDecNQ = (kGDecRows + kDecSplit * kDecWaves - 1) / (kDecSplit * kDecWaves);
+// D: blocks per instance, chosen by the loop's tuner (any D gives the same
+// rows: each row is folded by one wavefront in one fixed order).
+template <int D>
 __global__ __launch_bounds__(kBlock) void k_decide_g(Ctl* __restrict__ Cs, const Inst* __restrict__ I,
                                                      const int* __restrict__ act, const GRng* __restrict__ G,
                                                      const double* __restrict__ dspart, int Pd,
@@ -1979,23 +2009,24 @@ __global__ __launch_bounds__(kBlock) void k_decide_g(Ctl* __restrict__ Cs, const
                                                      const double* __restrict__ bpart, int Pb,
                                                      const double* __restrict__ gpart, int* __restrict__ iflags,
                                                      double* drows, unsigned* dcount) {
-  const int b = act[blockIdx.x / kDecSplit], part = blockIdx.x % kDecSplit;
+  constexpr int NQ = (kGDecRows + D * kDecWaves - 1) / (D * kDecWaves);
+  const int b = act[blockIdx.x / D], part = blockIdx.x % D;
   Ctl* C = Cs + b;
   const Inst& in = I[b];
   if (!ctl_trial_phase(C)) return;
-  __shared__ double lanes[kDecWaves * kDecNQ][kWave + 1];
+  __shared__ double lanes[kDecWaves * NQ][kWave + 1];
   __shared__ int last;
   {
     const int m = C->m, count = C->count;
     const int R = 21 + (m > 0 ? 4 * (count - (count == m ? 1 : 0)) + 5 : 0);
     const GraphParts P{dspart, cpart, bpart, gpart, Pd, Pc, Pb};
     const GRng g = G[b];
-    rows_pass<kDecNQ>([&](int j) { return graph_row(j, g, P); }, dspart + g.pd0, R, part * kDecWaves,
-                      kDecSplit * kDecWaves, lanes, drows + size_t(b) * kGDecRows);
+    rows_pass<NQ>([&](int j) { return graph_row(j, g, P); }, dspart + g.pd0, R, part * kDecWaves, D * kDecWaves,
+                  lanes, drows + size_t(b) * kGDecRows);
   }
   if (threadIdx.x == 0) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    last = atomicAdd(dcount + b, 1u) == unsigned(kDecSplit - 1);
+    last = atomicAdd(dcount + b, 1u) == unsigned(D - 1);
     if (last) {
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
       dcount[b] = 0;
@@ -2054,7 +2085,7 @@ __global__ __launch_bounds__(kSolveThreads) void k_solve_block(
   constexpr int NP = NA <= kWave ? kWave : NA;
   __shared__ Ctl sc;
   __shared__ double rows[kGDecRows];
-  __shared__ double part4[4];
+  __shared__ double partw[4][kSolveWaves];  // one column per wavefront
   __shared__ double shr[kSolveWaves][NP];
   __shared__ double qsh[6][kSolveThreads];  // raw directions, one column per thread
   ctl_load<kSolveThreads>(sc, Cs + b);
@@ -2152,10 +2183,15 @@ __global__ __launch_bounds__(kSolveThreads) void k_solve_block(
       for (int base = 0; base < in.B; base += kSolveThreads) {
         const int li = base + tid;
         const bool on = li < in.B;
-        beam_body<T, kSolveThreads>(L, in.M, x, R, rec, nullptr, bt, part4, in.b0 + (on ? li : 0), on, 0, 1);
+        beam_body<T, kSolveThreads>(L, in.M, x, R, rec, nullptr, bt, &partw[0][0], in.b0 + (on ? li : 0), on, 0,
+                                    1);
+        __syncthreads();
         if (tid == 0)
 #pragma unroll
-          for (int c = 0; c < 4; c++) acc[c] += part4[c];
+          for (int c = 0; c < 4; c++)
+#pragma unroll
+            for (int w = 0; w < kSolveWaves; w++) acc[c] += partw[c][w];
+        __syncthreads();
       }
       const TermBufs ct{par ? cb1 : cb0, par ? cb0 : cb1};
       const int* plist = lr->plist;
@@ -2164,14 +2200,18 @@ __global__ __launch_bounds__(kSolveThreads) void k_solve_block(
       for (int base = 0; base < in.NS; base += kSolveThreads) {
         const int li = base + tid;
         const bool on = li < in.NS;
-        contact_body<T, kSolveThreads>(L, in, x, pcount, plist, slab, cf_sph, cf_cap, ct, part4,
+        contact_body<T, kSolveThreads>(L, in, x, pcount, plist, slab, cf_sph, cf_cap, ct, &partw[0][0],
                                        in.s0 + (on ? li : 0), on, 0, 1);
-        if (tid == 0) {
-          accc[0] += part4[0];
-          accc[1] = fmax(accc[1], part4[1]);
-          accc[2] += part4[2];
-          accc[3] += part4[3];
-        }
+        __syncthreads();
+        if (tid == 0)
+#pragma unroll
+          for (int w = 0; w < kSolveWaves; w++) {
+            accc[0] += partw[0][w];
+            accc[1] = fmax(accc[1], partw[1][w]);
+            accc[2] += partw[2][w];
+            accc[3] += partw[3][w];
+          }
+        __syncthreads();
       }
       if (tid == 0)
 #pragma unroll
@@ -2479,10 +2519,24 @@ struct mmc_ctx {
   int gPd = 0, gPb = 0, gPc = 0;
   std::vector<int> gset;
   int64_t captures = 0;
+  // launch shapes of the loop, chosen by tune() at the first capture (neither
+  // changes a result): items per block of the element kernel, blocks per
+  // instance of the decision kernel; and the timings the choice was made on
+  int elem_items = kBlock, dec_split = 8;
+  bool tuned = false;
+  double tune_elem_us[3] = {0, 0, 0}, tune_dec_us[4] = {0, 0, 0, 0};
+  template <class T>
+  void launch_elem(Lat<T> Lt, const Ctl* C);
+  void launch_decide(Ctl* C);
+  void graph_tables(const std::vector<int>& set);
+  void tune();
   // host-clock seconds: initial evaluation, graph captures, replay loop,
   // minimize_graph in all, final evaluation (reported at destruction when
   // MMC_LOOP_TIMING is set)
-  double loop_s[6] = {0, 0, 0, 0, 0, 0};  // [5]: contact list builds
+  // [5]: contact list builds, [6]: of them partial, [7..9]: count + scan,
+  // fill + pairs, reference positions + publication
+  double loop_s[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  int64_t partial_builds = 0;
   int64_t loop_polls = 0;
   Ctl* tctl = nullptr;  // scratch control blocks of time_loop_kernel
   double* dspart = nullptr;    // [10][Pd] partials of the trial state kernel
@@ -2583,9 +2637,11 @@ mmc_ctx::~mmc_ctx() {
     std::fprintf(stderr,
                  "[loop] %d instances: initial evaluations %.3f s, captures %lld (%.3f s), replay loop %.3f s "
                  "(%lld graph launches, %lld polls), minimize %.3f s, final evaluations %.3f s, "
-                 "%lld contact list builds %.3f s\n",
+                 "%lld contact list builds %.3f s (%lld partial, %.3f s; count + scan %.3f, fill + pairs %.3f, "
+                 "rest %.3f); element blocks of %d, decision %d blocks per lattice\n",
                  n_inst, loop_s[0], (long long)captures, loop_s[1], loop_s[2], (long long)graph_replays,
-                 (long long)loop_polls, loop_s[3], loop_s[4], (long long)rebuilds, loop_s[5]);
+                 (long long)loop_polls, loop_s[3], loop_s[4], (long long)rebuilds, loop_s[5],
+                 (long long)partial_builds, loop_s[6], loop_s[7], loop_s[8], loop_s[9], elem_items, dec_split);
   void* ptrs[] = {bpos, bquat, mass, lock, ba, bb, L0, rad, area, inert, e0, plastic, inc_off, inc, adj_off, adj, x, R,
                   rec, trial, cf_sph, cf_cap, box, cnt, start, entries, flags, scan_tmp, spart, cpart, bpart, gpart,
                   vpart, stb, stn, stout, record, scal, z, g, p, zc, gc, zt, zin, Sh, Yh, xref, pcount, plist,
@@ -2824,8 +2880,8 @@ void mmc_ctx::setup(const mmc_lattice* const* Ls, const mmc_config* cs, int n, i
     put(td, t);
     htd = t;
     gtd = dalloc<Blk>(htd.size());
-    gtb = dalloc<Blk>(htb.size());
-    gtc = dalloc<Blk>(htc.size());
+    gtb = dalloc<Blk>(htb.size() * kBW);  // the loop's element blocks may hold 64 items
+    gtc = dalloc<Blk>(htc.size() * kBW);
     grng = dalloc<GRng>(n);
     gact = dalloc<int>(n);
     std::vector<int> si(NS);
@@ -2859,8 +2915,8 @@ void mmc_ctx::setup(const mmc_lattice* const* Ls, const mmc_config* cs, int n, i
   slab = 8;
   plist = dalloc<int>(size_t(NS) * slab);
   spart = dalloc<double>(9 * size_t(Ps));
-  cpart = dalloc<double>(4 * size_t(Pc));
-  bpart = dalloc<double>(4 * size_t(Pb));
+  cpart = dalloc<double>(4 * size_t(kBW) * size_t(Pc));  // one column per wavefront
+  bpart = dalloc<double>(4 * size_t(kBW) * size_t(Pb));
   gpart = dalloc<double>(size_t(kGRows) * size_t(Pd));  // gradient rows, then pair rows
   vpart = dalloc<double>((4 * size_t(kMaxHist) + 8) * size_t(std::max(Ps, Pd)));
   stb = dalloc<double>(6 * size_t(Pb));
@@ -2939,6 +2995,8 @@ void mmc_ctx::build_list(Lat<T> Lt, const Blk* ctab, int cblocks, const Blk* nta
   HIPCALL(hipcub::DeviceScan::ExclusiveSum(scan_tmp, scan_bytes, cnt, start, table + 1, stream));
   unsigned total = 0;
   read(&total, start + table, sizeof(unsigned));
+  const auto t_scan = std::chrono::steady_clock::now();
+  loop_s[7] += std::chrono::duration<double>(t_scan - t_build).count();
   ensure_entries(size_t(total) + 1024);
   tick(MMC_K_FILL);
   k_fill<<<cblocks, kBlock, 0, stream>>>(inst_d, ctab, box, cnt, start, entries, flags);
@@ -2947,6 +3005,8 @@ void mmc_ctx::build_list(Lat<T> Lt, const Blk* ctab, int cblocks, const Blk* nta
                                                         pcount, plist, flags);
   int fl = 0;
   read(&fl, flags, sizeof(int));
+  const auto t_pairs = std::chrono::steady_clock::now();
+  loop_s[8] += std::chrono::duration<double>(t_pairs - t_scan).count();
   if (fl & 8) {
     // some shape has more partners than slots: widen the slab and enumerate
     // every instance again (the wider slab holds no list yet)
@@ -2975,7 +3035,14 @@ void mmc_ctx::build_list(Lat<T> Lt, const Blk* ctab, int cblocks, const Blk* nta
   HIPCALL(hipStreamSynchronize(stream));
   list_valid = true;
   rebuilds++;
-  loop_s[5] += std::chrono::duration<double>(std::chrono::steady_clock::now() - t_build).count();
+  const auto t_end = std::chrono::steady_clock::now();
+  const double dt = std::chrono::duration<double>(t_end - t_build).count();
+  loop_s[5] += dt;
+  loop_s[9] += std::chrono::duration<double>(t_end - t_pairs).count();
+  if (part) {
+    loop_s[6] += dt;
+    partial_builds++;
+  }
 }
 
 // One objective evaluation of every instance at its platen (tops_d), records
@@ -3017,11 +3084,13 @@ void mmc_ctx::evaluate_t(Lat<T> Lt, const double* zsrc, const double* psrc, doub
     HIPCALL(hipStreamSynchronize(stream));
     harvest();
     // a stale list (some node moved skin/2 or more since the build) cannot be
-    // trusted: rebuild at these positions and redo the contact part
-    bool stale = false;
-    for (int b = 0; b < n_inst; b++) stale = stale || hrec[b * kRec + 11] >= 0.5 * hinst[b].S.skin;
-    if (built || !stale) break;
-    build_list<T>(Lt);
+    // trusted: rebuild at these positions (the stale instances' lists only)
+    // and redo the contact part
+    std::vector<int> stale;
+    for (int b = 0; b < n_inst; b++)
+      if (hrec[b * kRec + 11] >= 0.5 * hinst[b].S.skin) stale.push_back(b);
+    if (built || stale.empty()) break;
+    rebuild_held(stale);
     built = true;
   }
   evaluations++;
@@ -3224,14 +3293,11 @@ void mmc_ctx::capture_iteration(Lat<T> Lt) {
   graph_exec = nullptr;
   graph = nullptr;
   const size_t n6 = 6 * size_t(N);
-  const int na = int(gset.size());
   HIPCALL(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
   for (int u = 0; u < kGraphIterations; u++) {
     k_dstate_g<T><<<gPd, kElemThreads, 0, stream>>>(Lt, inst_d, gtd, ctl, zbuf[0], zbuf[1], gbuf[0], gbuf[1], Sh,
                                                     Yh, n6, p, x, (double*)R, xref, nbuf[0], nbuf[1], dspart);
-    k_elem_g<T><<<gPb + gPc, kBlock, 0, stream>>>(ctl, Lt, inst_d, gtb, gtc, x, (const double*)R, rec, ubuf[0],
-                                                  ubuf[1], bpart, gPb, pcount, lref, cf_sph, cf_cap, cbuf[0], cbuf[1],
-                                                  cpart, gPc);
+    launch_elem<T>(Lt, ctl);
     if (max_hist <= 10)
       k_gather_g<T, 10><<<gPd, kElemThreads, 0, stream>>>(Lt, inst_d, gtd, grng, ctl, x, rec, cf_sph, cf_cap, dspart,
                                                           zbuf[0], zbuf[1], gbuf[0], gbuf[1], p, Sh, Yh, n6, gpart);
@@ -3239,8 +3305,7 @@ void mmc_ctx::capture_iteration(Lat<T> Lt) {
       k_gather_g<T, kMaxHist><<<gPd, kElemThreads, 0, stream>>>(Lt, inst_d, gtd, grng, ctl, x, rec, cf_sph, cf_cap,
                                                                 dspart, zbuf[0], zbuf[1], gbuf[0], gbuf[1], p, Sh, Yh,
                                                                 n6, gpart);
-    k_decide_g<<<na * kDecSplit, kBlock, 0, stream>>>(ctl, inst_d, gact, grng, dspart, gPd, cpart, gPc, bpart, gPb,
-                                                       gpart, iflags(), drows, dcount);
+    launch_decide(ctl);
   }
   HIPCALL(hipStreamEndCapture(stream, &graph));
   HIPCALL(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0));
@@ -3289,25 +3354,34 @@ void mmc_ctx::solve_blocks(Lat<T> Lt, const std::vector<int>& set, int max_trial
 }
 
 // Block tables of the device loop for the instances of `set` (ascending):
-// each instance's blocks of the full tables, in order, and its ranges there;
-// then the loop is captured over them.
-void mmc_ctx::capture(const std::vector<int>& set) {
-  HIPCALL(hipStreamSynchronize(stream));
+// each instance's blocks of the full 64-node table, in order, and its beams
+// and contact shapes in blocks of elem_items from the instance's first item;
+// ranges of the beam and shape partials in wavefront columns.
+void mmc_ctx::graph_tables(const std::vector<int>& set) {
   std::vector<Blk> d, b, c;
   std::vector<GRng> r(size_t(n_inst), GRng{0, 0, 0, 0, 0, 0, 0, 0});
+  const int E = elem_items, W = E / kWave;
+  auto split = [&](std::vector<Blk>& t, int inst, int first, int count) {
+    for (int o = 0; o < count; o += E) t.push_back(Blk{inst, first + o, std::min(E, count - o), 0});
+  };
   for (int i : set) {
     const Inst& in = hinst[size_t(i)];
     GRng& g = r[size_t(i)];
     g.pd0 = int(d.size());
     d.insert(d.end(), htd.begin() + in.pd0, htd.begin() + in.pd1);
     g.pd1 = int(d.size());
-    g.pb0 = int(b.size());
-    b.insert(b.end(), htb.begin() + in.pb0, htb.begin() + in.pb1);
-    g.pb1 = int(b.size());
-    g.pc0 = int(c.size());
-    c.insert(c.end(), htc.begin() + in.pc0, htc.begin() + in.pc1);
-    g.pc1 = int(c.size());
+    g.pb0 = int(b.size()) * W;
+    split(b, i, in.b0, in.B);
+    g.pb1 = int(b.size()) * W;
+    g.pc0 = int(c.size()) * W;
+    split(c, i, in.s0, in.NS);
+    g.pc1 = int(c.size()) * W;
   }
+  // partial columns and table entries within what setup allocated (the
+  // kBlock tables and one column per wavefront of them)
+  if (b.size() * W > size_t(kBW) * htb.size() || c.size() * W > size_t(kBW) * htc.size() ||
+      b.size() > size_t(kBW) * htb.size() || c.size() > size_t(kBW) * htc.size())
+    throw Fail(MMC_ERR_STATE, "device loop tables exceed their buffers");
   gPd = int(d.size());
   gPb = int(b.size());
   gPc = int(c.size());
@@ -3318,6 +3392,85 @@ void mmc_ctx::capture(const std::vector<int>& set) {
   HIPCALL(hipMemcpy(grng, r.data(), r.size() * sizeof(GRng), hipMemcpyHostToDevice));
   HIPCALL(hipMemcpy(gact, set.data(), set.size() * sizeof(int), hipMemcpyHostToDevice));
   gset = set;
+}
+
+template <class T>
+void mmc_ctx::launch_elem(Lat<T> Lt, const Ctl* C) {
+  const int nb = gPb + gPc;
+  switch (elem_items) {
+    case 64:
+      k_elem_g<T, 64><<<nb, 64, 0, stream>>>(C, Lt, inst_d, gtb, gtc, x, (const double*)R, rec, ubuf[0], ubuf[1],
+                                             bpart, gPb, pcount, lref, cf_sph, cf_cap, cbuf[0], cbuf[1], cpart, gPc);
+      break;
+    case 128:
+      k_elem_g<T, 128><<<nb, 128, 0, stream>>>(C, Lt, inst_d, gtb, gtc, x, (const double*)R, rec, ubuf[0], ubuf[1],
+                                               bpart, gPb, pcount, lref, cf_sph, cf_cap, cbuf[0], cbuf[1], cpart, gPc);
+      break;
+    default:
+      k_elem_g<T, kBlock><<<nb, kBlock, 0, stream>>>(C, Lt, inst_d, gtb, gtc, x, (const double*)R, rec, ubuf[0],
+                                                     ubuf[1], bpart, gPb, pcount, lref, cf_sph, cf_cap, cbuf[0],
+                                                     cbuf[1], cpart, gPc);
+  }
+}
+
+void mmc_ctx::launch_decide(Ctl* C) {
+  const int na = int(gset.size()), W = elem_items / kWave;
+  const int cw = gPc * W, bw = gPb * W;  // wavefront columns of the partials
+  switch (dec_split) {
+    case 2:
+      k_decide_g<2><<<na * 2, kBlock, 0, stream>>>(C, inst_d, gact, grng, dspart, gPd, cpart, cw, bpart, bw, gpart,
+                                                   iflags(), drows, dcount);
+      break;
+    case 4:
+      k_decide_g<4><<<na * 4, kBlock, 0, stream>>>(C, inst_d, gact, grng, dspart, gPd, cpart, cw, bpart, bw, gpart,
+                                                   iflags(), drows, dcount);
+      break;
+    case 16:
+      k_decide_g<16><<<na * 16, kBlock, 0, stream>>>(C, inst_d, gact, grng, dspart, gPd, cpart, cw, bpart, bw,
+                                                     gpart, iflags(), drows, dcount);
+      break;
+    default:
+      k_decide_g<8><<<na * 8, kBlock, 0, stream>>>(C, inst_d, gact, grng, dspart, gPd, cpart, cw, bpart, bw, gpart,
+                                                   iflags(), drows, dcount);
+  }
+}
+
+// Launch-shape tuner of the device loop, run at a context's first capture on
+// the state of its first evaluation: the element kernel at 64, 128 and 256
+// items per block and the decision kernel at 2, 4, 8 and 16 blocks per
+// instance, each timed with HIP events over back-to-back launches
+// (time_loop_kernel); the fastest of each is kept.  Neither choice changes a
+// result (one partial column per wavefront; one fixed fold order per row), so
+// the tuning is free to differ between contexts.  MMC_TUNE=0 keeps 256 and 8.
+void mmc_ctx::tune() {
+  tuned = true;
+  const char* env = std::getenv("MMC_TUNE");
+  if (env && std::strcmp(env, "0") == 0) return;
+  const std::vector<int> set = gset;
+  constexpr int kLaunches = 20;
+  const int items[3] = {64, 128, kBlock}, splits[4] = {2, 4, 8, 16};
+  int best = 0;
+  for (int k = 0; k < 3; k++) {
+    elem_items = items[k];
+    graph_tables(set);
+    tune_elem_us[k] = 1e3 * time_loop_kernel(MMC_K_ELEM_G, kLaunches);
+    if (tune_elem_us[k] < tune_elem_us[best]) best = k;
+  }
+  elem_items = items[best];
+  graph_tables(set);
+  best = 0;
+  for (int k = 0; k < 4; k++) {
+    dec_split = splits[k];
+    tune_dec_us[k] = 1e3 * time_loop_kernel(MMC_K_DECIDE_G, kLaunches);
+    if (tune_dec_us[k] < tune_dec_us[best]) best = k;
+  }
+  dec_split = splits[best];
+}
+
+void mmc_ctx::capture(const std::vector<int>& set) {
+  HIPCALL(hipStreamSynchronize(stream));
+  graph_tables(set);
+  if (!tuned) tune();
   captures++;
   if (precision == MMC_MIXED)
     capture_iteration<float>(lat32());
@@ -3342,6 +3495,10 @@ __global__ __launch_bounds__(kBlock) void k_copy_words(double* __restrict__ dst,
 }  // namespace
 double mmc_ctx::time_loop_kernel(int kernel, int launches) {
   if (gset.empty()) throw Fail(MMC_ERR_STATE, "the device loop has not run yet (no mmc_minimize)");
+  if (!ev0) {
+    HIPCALL(hipEventCreate(&ev0));
+    HIPCALL(hipEventCreate(&ev1));
+  }
   HIPCALL(hipStreamSynchronize(stream));
   std::vector<Ctl> h(static_cast<size_t>(n_inst));
   HIPCALL(hipMemcpy(h.data(), ctl, h.size() * sizeof(Ctl), hipMemcpyDeviceToHost));
@@ -3351,7 +3508,10 @@ double mmc_ctx::time_loop_kernel(int kernel, int launches) {
     c.step = 1.0;
     c.bt = 0;
     c.dg = -1.0;
-    if (kernel == MMC_K_GATHER_G && c.m > 0) c.count = c.m;  // full history: m - 1 retained pairs
+    // full history (m - 1 retained pairs): the most rows a gather or a
+    // decision has; the decision then folds rows the gather may not have
+    // written since setup (scratch values, only timed)
+    if ((kernel == MMC_K_GATHER_G || kernel == MMC_K_DECIDE_G) && c.m > 0) c.count = c.m;
   }
   if (!tctl) tctl = dalloc<Ctl>(2 * size_t(n_inst));  // [0, n): scratch, [n, 2n): its initial state
   Ctl* tinit = tctl + n_inst;
@@ -3360,7 +3520,6 @@ double mmc_ctx::time_loop_kernel(int kernel, int launches) {
   const long words = long(n_inst) * long(sizeof(Ctl) / 8);
   const int wblocks = int((words + kBlock - 1) / kBlock);
   const size_t n6 = 6 * size_t(N);
-  const int na = int(gset.size());
   auto loop = [&](auto Lt, auto* tag, bool restore, bool work) {
     using T = std::remove_pointer_t<decltype(tag)>;
     HIPCALL(hipEventRecord(ev0, stream));
@@ -3374,9 +3533,7 @@ double mmc_ctx::time_loop_kernel(int kernel, int launches) {
                                                           nbuf[1], dspart);
           break;
         case MMC_K_ELEM_G:
-          k_elem_g<T><<<gPb + gPc, kBlock, 0, stream>>>(tctl, Lt, inst_d, gtb, gtc, x, (const double*)R, rec,
-                                                        ubuf[0], ubuf[1], bpart, gPb, pcount, lref, cf_sph, cf_cap,
-                                                        cbuf[0], cbuf[1], cpart, gPc);
+          launch_elem<T>(Lt, tctl);
           break;
         case MMC_K_GATHER_G:
           if (max_hist <= 10)
@@ -3389,8 +3546,7 @@ double mmc_ctx::time_loop_kernel(int kernel, int launches) {
                                                                       gbuf[1], p, Sh, Yh, n6, gpart);
           break;
         default:
-          k_decide_g<<<na * kDecSplit, kBlock, 0, stream>>>(tctl, inst_d, gact, grng, dspart, gPd, cpart, gPc,
-                                                             bpart, gPb, gpart, iflags(), drows, dcount);
+          launch_decide(tctl);
       }
     }
     HIPCALL(hipEventRecord(ev1, stream));
@@ -3872,6 +4028,17 @@ int mmc_time_kernel(mmc_ctx* c, int32_t kernel, int32_t launches, double* mean_m
       run(c->lat32(), (float*)nullptr);
     else
       run(c->lat64(), (double*)nullptr);
+  });
+}
+
+int mmc_loop_shape(mmc_ctx* c, int32_t* elem_items, int32_t* decide_blocks, double* elem_us, double* decide_us) {
+  return guarded(c, [&] {
+    if (elem_items) *elem_items = c->elem_items;
+    if (decide_blocks) *decide_blocks = c->dec_split;
+    if (elem_us)
+      for (int k = 0; k < 3; k++) elem_us[k] = c->tune_elem_us[k];
+    if (decide_us)
+      for (int k = 0; k < 4; k++) decide_us[k] = c->tune_dec_us[k];
   });
 }
 
