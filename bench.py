@@ -247,17 +247,24 @@ def run_batch(args, log):
         sys.path.insert(0, str(ROOT / "oracle"))
         import py_oracle
         pick = sorted(rng.choice(ok, size=min(args.oracle_sample, len(ok)), replace=False).tolist())
-        errs = []
+        errs, failed = [], {}
         t0 = time.perf_counter()
         for b in pick:
-            fo = py_oracle.Solver(cfgs[b], "orc").run()
+            try:
+                fo = py_oracle.Solver(cfgs[b], "orc").run()
+            except RuntimeError as e:
+                # the oracle's own schedule can end in an equilibrium failure
+                # where the device run (other summation order) went through:
+                # past buckling the two take different branches
+                failed[int(b)] = str(e)
+                continue
             fg = frames[b]
             if len(fo) == len(fg):
                 wo, wg = fo["dissipated_work"][-1], fg["dissipated_work"][-1]
                 errs.append(abs(wg - wo) / max(abs(wo), 1e-12))
         tc = time.perf_counter() - t0
         out["oracle_sample"] = {"lattices": pick, "max_rel_dissipated_work": max(errs) if errs else None,
-                                "compared": len(errs)}
+                                "compared": len(errs), "oracle_failed": failed or None}
         out["cpu_baseline"] = {"value": round(len(pick) * steps / tc, 4), "unit": "lattice-load-steps/s",
                                "cores": 1, "kind": "port",
                                "sample": f"{len(pick)} lattices of the set, full schedules ({tc:.1f} s, one core)"}

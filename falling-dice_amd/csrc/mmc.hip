@@ -3441,10 +3441,22 @@ void mmc_ctx::launch_decide(Ctl* C) {
 // instance, each timed with HIP events over back-to-back launches
 // (time_loop_kernel); the fastest of each is kept.  Neither choice changes a
 // result (one partial column per wavefront; one fixed fold order per row), so
-// the tuning is free to differ between contexts.  MMC_TUNE=0 keeps 256 and 8.
+// the tuning is free to differ between contexts.  MMC_TUNE=0 keeps 256 and 8;
+// MMC_ELEM_ITEMS and MMC_DECIDE_BLOCKS fix either shape (tests).
 void mmc_ctx::tune() {
   tuned = true;
   const char* env = std::getenv("MMC_TUNE");
+  const char* fe = std::getenv("MMC_ELEM_ITEMS");
+  const char* fd = std::getenv("MMC_DECIDE_BLOCKS");
+  if (fe || fd) {
+    const int e = fe ? std::atoi(fe) : kBlock, d = fd ? std::atoi(fd) : 8;
+    if ((e != 64 && e != 128 && e != kBlock) || (d != 2 && d != 4 && d != 8 && d != 16))
+      throw Fail(MMC_ERR_ARG, "MMC_ELEM_ITEMS must be 64, 128 or 256 and MMC_DECIDE_BLOCKS 2, 4, 8 or 16");
+    elem_items = e;
+    dec_split = d;
+    graph_tables(gset);
+    return;
+  }
   if (env && std::strcmp(env, "0") == 0) return;
   const std::vector<int> set = gset;
   constexpr int kLaunches = 20;
