@@ -1017,11 +1017,13 @@ __device__ __forceinline__ DecRow decide_row(int j, const Inst& in, const DecPar
 // column, or dummy for a row outside [0, R) or an empty one, and the value is
 // replaced by the fold's identity), so that all of them are in flight
 // together.
-template <int NQ, class Row>
+// RD: columns per lane loaded in the first round (RD x 64 columns per row).
+template <int NQ, int RD = kDecColRounds, class Row>
 __device__ __forceinline__ void rows_pass(Row row, const double* dummy, int R, int jbase, int js,
                                           double (*lanes)[kWave + 1], double* out) {
+  constexpr int kCols = RD * kWave;
   const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
-  double x[NQ][kDecColRounds];
+  double x[NQ][RD];
 #pragma unroll
   for (int q = 0; q < NQ; q++) {
     const int j = jbase + wv + q * js;
@@ -1030,7 +1032,7 @@ __device__ __forceinline__ void rows_pass(Row row, const double* dummy, int R, i
     const double* base = any ? rw.p : dummy;
     const int last = any ? rw.n - 1 : 0;
 #pragma unroll
-    for (int u = 0; u < kDecColRounds; u++) x[q][u] = base[min(lane + u * kWave, last)];
+    for (int u = 0; u < RD; u++) x[q][u] = base[min(lane + u * kWave, last)];
   }
 #pragma unroll
   for (int q = 0; q < NQ; q++) {
@@ -1039,13 +1041,13 @@ __device__ __forceinline__ void rows_pass(Row row, const double* dummy, int R, i
     const double id = rw.mx ? -INFINITY : 0.0;
     double a = id;
 #pragma unroll
-    for (int u = 0; u < kDecColRounds; u++) {
+    for (int u = 0; u < RD; u++) {
       const double e = (j < R && lane + u * kWave < rw.n) ? x[q][u] : id;
       a = rw.mx ? fmax(a, e) : a + e;
     }
     if (j < R) {
 #pragma unroll 4
-      for (int col = lane + kDecCols; col < rw.n; col += kWave) a = rw.mx ? fmax(a, rw.p[col]) : a + rw.p[col];
+      for (int col = lane + kCols; col < rw.n; col += kWave) a = rw.mx ? fmax(a, rw.p[col]) : a + rw.p[col];
       lanes[wv * NQ + q][lane] = a;
     }
   }
@@ -2021,7 +2023,10 @@ __global__ __launch_bounds__(kBlock) void k_decide_g(Ctl* __restrict__ Cs, const
     const int R = 21 + (m > 0 ? 4 * (count - (count == m ? 1 : 0)) + 5 : 0);
     const GraphParts P{dspart, cpart, bpart, gpart, Pd, Pc, Pb};
     const GRng g = G[b];
-    rows_pass<NQ>([&](int j) { return graph_row(j, g, P); }, dspart + g.pd0, R, part * kDecWaves, D * kDecWaves,
+    // first-round columns per lane: as many as 32 registers per row pair hold
+    // (the beam and contact rows have one column per 64 items)
+    constexpr int RD = NQ <= 2 ? 16 : (NQ <= 4 ? 8 : kDecColRounds);
+    rows_pass<NQ, RD>([&](int j) { return graph_row(j, g, P); }, dspart + g.pd0, R, part * kDecWaves, D * kDecWaves,
                   lanes, drows + size_t(b) * kGDecRows);
   }
   if (threadIdx.x == 0) {
