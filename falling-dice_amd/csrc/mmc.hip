@@ -982,11 +982,12 @@ __device__ __forceinline__ void record_from_rows(const Inst& in, const double* _
 // history on, the 4 nold + 5 pair-product rows of the gather kernel.  Every
 // row is folded in one fixed order, whichever block and wave folds it: lane l
 // folds columns l, l + 64, ... in ascending order, then one thread folds the
-// 64 lane values in lane order.  The loads of the first kDecCols columns of
-// every row of a wave are issued together (one memory latency for instances
-// of up to kDecCols blocks per table).
+// 64 lane values in lane order.  The loads of the first RD * 64 columns of
+// every row of a wave (RD = kDecColRounds unless a pass names its own) are
+// issued together (one memory latency for instances of up to that many
+// columns per row).
 constexpr int kDecWaves = kBlock / kWave;
-constexpr int kDecColRounds = 5, kDecCols = kDecColRounds * kWave;
+constexpr int kDecColRounds = 5;
 
 // Row j of an instance's partials: 0..8 state kernel, 9..12 contact shapes,
 // 13..16 beams, 17.. gradient rows (3, then the pair products); mx: folded by
@@ -1697,10 +1698,50 @@ struct ListRef {
   int slab, pad;
 };
 
+// An instance's block ranges in the device loop's block tables.  The loop's
+// tables list only the instances still running (compacted as a batch drains);
+// each instance keeps its own blocks in their own order, so its partials, and
+// every fold of them, do not depend on which other instances are listed.
+struct GRng {
+  int pd0, pd1;  // blocks of the 64-node table
+  int pb0, pb1;  // partial columns (wavefronts) of the beams
+  int pc0, pc1;  // partial columns (wavefronts) of the contact shapes
+  int pad0, pad1;
+};
+
+// Node centroid (x, z) of one instance from the state kernel's partial rows 5
+// and 6, by one wavefront, in exactly the order of reduce_columns<2, NT>
+// (virtual thread t = 64 w + lane sums columns c0 + t, c0 + t + NT, ...;
+// the lanes of virtual wave w by wave_sum, the wave totals in ascending w).
+template <int NT>
+__device__ __forceinline__ void centroid_wave(const double* __restrict__ part, int P, int c0, int c1, int n,
+                                              double* __restrict__ out) {
+  const int lane = threadIdx.x & (kWave - 1);
+  double tot[2];
+#pragma unroll
+  for (int k = 0; k < 2; k++) {
+    double a = 0.0;
+    for (int w = 0; w < NT / kWave; w++) {
+      double s = 0.0;
+      for (int i = c0 + w * kWave + lane; i < c1; i += NT) s += part[size_t(k) * P + i];
+      s = wave_sum(s);
+      a = w == 0 ? s : a + s;
+    }
+    tot[k] = a;
+  }
+  if (lane == 0) {
+    out[0] = tot[0] * (1.0 / n);
+    out[1] = tot[1] * (1.0 / n);
+  }
+}
+
 // beams and contact shapes of one trial in a single launch: blocks [0, Pb)
 // evaluate beams, blocks [Pb, Pb + Pc) contact shapes (independent work; one
-// launch gap fewer per trial).  E: items (threads) per block, chosen by the
-// loop's tuner; the partials have one column per wavefront whatever E is.
+// launch gap fewer per trial); blocks [Pb + Pc, Pb + Pc + |act|) one running
+// instance each, the node centroid of its trial for k_gather_g (once per
+// instance instead of once per gather block).  E: items (threads) per block,
+// chosen by the loop's tuner; the partials have one column per wavefront
+// whatever E is.
 template <class T, int E>
 __global__ __launch_bounds__(E) void k_elem_g(const Ctl* __restrict__ Cs, Lat<T> L, const Inst* __restrict__ I,
                                                    const Blk* __restrict__ tb, const Blk* __restrict__ tc,
@@ -1710,7 +1751,16 @@ __global__ __launch_bounds__(E) void k_elem_g(const Ctl* __restrict__ Cs, Lat<T>
                                                    const unsigned* __restrict__ pcount,
                                                    const ListRef* __restrict__ lr, double* __restrict__ cf_sph,
                                                    double* __restrict__ cf_cap, double* __restrict__ cb0,
-                                                   double* __restrict__ cb1, double* __restrict__ cpart, int Pc) {
+                                                   double* __restrict__ cb1, double* __restrict__ cpart, int Pc,
+                                                   const int* __restrict__ act, const GRng* __restrict__ G,
+                                                   const double* __restrict__ dspart, int Pd,
+                                                   double* __restrict__ cent) {
+  if (int(blockIdx.x) >= Pb + Pc) {
+    const int k = act[blockIdx.x - Pb - Pc];
+    if (threadIdx.x < kWave && ctl_trial_phase(Cs + k))
+      centroid_wave<kElemThreads>(dspart + 5 * size_t(Pd), Pd, G[k].pd0, G[k].pd1, I[k].N, cent + 2 * k);
+    return;
+  }
   const bool beams = int(blockIdx.x) < Pb;
   const int blk = beams ? blockIdx.x : blockIdx.x - Pb;
   const Blk bk = beams ? tb[blk] : tc[blk];
@@ -1735,16 +1785,6 @@ __global__ __launch_bounds__(E) void k_elem_g(const Ctl* __restrict__ Cs, Lat<T>
 // products (gather rows 3..).
 // ---------------------------------------------------------------------------
 constexpr int kGDecRows = 21 + 4 * kMaxHist + 5;
-// An instance's block ranges in the device loop's block tables.  The loop's
-// tables list only the instances still running (compacted as a batch drains);
-// each instance keeps its own blocks in their own order, so its partials, and
-// every fold of them, do not depend on which other instances are listed.
-struct GRng {
-  int pd0, pd1;  // blocks of the 64-node table
-  int pb0, pb1;  // partial columns (wavefronts) of the beams
-  int pc0, pc1;  // partial columns (wavefronts) of the contact shapes
-  int pad0, pad1;
-};
 struct GraphParts {
   const double *spart, *cpart, *bpart, *gpart;  // [10][Pd], [4][Pc], [4][Pb], [kGRows][Pd]
   int Pd, Pc, Pb;                               // Pc, Pb: wavefront columns
@@ -1891,7 +1931,8 @@ __device__ void decide_block(Ctl* __restrict__ C, const Inst& in, const double* 
   ctl_store<NT>(C, sc);
 }
 
-// Gradient assembly of the trial (element-parallel, as k_gather), then
+// Gradient assembly of the trial (element-parallel, as k_gather; the node
+// centroid of the centering term comes from k_elem_g's cent), then
 // (history on) the trial's curvature pair s = zc - z, y = gc - g written into
 // the spare history slot with all its products (partial rows 3..: for each
 // retained pair j, oldest first: s_j.y, y_j.y, s_j.gc, y_j.gc; then s.y, s.s,
@@ -1903,12 +1944,12 @@ __device__ void decide_block(Ctl* __restrict__ C, const Inst& in, const double* 
 // register footprint of the m <= 16 one).
 template <class T, int MH>
 __global__ __launch_bounds__(kElemThreads) void k_gather_g(Lat<T> L, const Inst* __restrict__ I,
-                                                           const Blk* __restrict__ td, const GRng* __restrict__ G,
+                                                           const Blk* __restrict__ td,
                                                            const Ctl* __restrict__ Cs, const double* __restrict__ x,
                                                            const double* __restrict__ rec,
                                                            const double* __restrict__ cf_sph,
                                                            const double* __restrict__ cf_cap,
-                                                           const double* __restrict__ dspart,
+                                                           const double* __restrict__ cent,
                                                            const double* __restrict__ z0, const double* __restrict__ z1,
                                                            double* __restrict__ g0, double* __restrict__ g1,
                                                            const double* __restrict__ p, double* __restrict__ Sh,
@@ -1947,13 +1988,7 @@ __global__ __launch_bounds__(kElemThreads) void k_gather_g(Lat<T> L, const Inst*
   const bool locked = (L.lock[ic] >> d) & 1u;
   IncVals iv;
   gather_load<T>(L, rec, cf_cap, d, ic, iv);
-  double c0, c1;
-  {
-    double t[2];
-    reduce_columns<2, kElemThreads>(dspart + 5 * size_t(Pd), Pd, G[bk.inst].pd0, G[bk.inst].pd1, t, 0u);
-    c0 = t[0] * (1.0 / in.N);
-    c1 = t[1] * (1.0 / in.N);
-  }
+  const double c0 = cent[2 * bk.inst], c1 = cent[2 * bk.inst + 1];
   const double gv = gather_sum<T>(L, in, C->top, x, rec, cf_sph, cf_cap, c0, c1, d, ic, iv);
   const double gd = locked ? 0.0 : gv;
   if (on) gnew[e] = gd;
@@ -2545,6 +2580,7 @@ struct mmc_ctx {
   int64_t loop_polls = 0;
   Ctl* tctl = nullptr;  // scratch control blocks of time_loop_kernel
   double* dspart = nullptr;    // [10][Pd] partials of the trial state kernel
+  double* cent = nullptr;      // [n_inst][2] node centroid (x, z) of the trial (k_elem_g)
   double* drows = nullptr;     // [n_inst][kGDecRows] rows folded by the decision kernel's blocks
   unsigned* dcount = nullptr;  // [n_inst] blocks of the decision kernel done with their rows
   double *zbuf[2] = {nullptr, nullptr}, *gbuf[2] = {nullptr, nullptr};
@@ -2650,7 +2686,7 @@ mmc_ctx::~mmc_ctx() {
   void* ptrs[] = {bpos, bquat, mass, lock, ba, bb, L0, rad, area, inert, e0, plastic, inc_off, inc, adj_off, adj, x, R,
                   rec, trial, cf_sph, cf_cap, box, cnt, start, entries, flags, scan_tmp, spart, cpart, bpart, gpart,
                   vpart, stb, stn, stout, record, scal, z, g, p, zc, gc, zt, zin, Sh, Yh, xref, pcount, plist,
-                  inst_d, tn, tb, tc, td, sinst, tops_d, mask_d, ctl, lref, dspart, drows, dcount, gtd, gtb, gtc,
+                  inst_d, tn, tb, tc, td, sinst, tops_d, mask_d, ctl, lref, dspart, cent, drows, dcount, gtd, gtb, gtc,
                   grng, gact, tctl, rtc, rtn};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
@@ -2956,6 +2992,8 @@ void mmc_ctx::setup(const mmc_lattice* const* Ls, const mmc_config* cs, int n, i
   HIPCALL(hipHostMalloc((void**)&hstat, size_t(n) * 8 * sizeof(double), hipHostMallocDefault));
   ctl = dalloc<Ctl>(n);
   dspart = dalloc<double>(10 * size_t(Pd));
+  cent = dalloc<double>(2 * size_t(n));
+  HIPCALL(hipMemset(cent, 0, 2 * size_t(n) * sizeof(double)));
   drows = dalloc<double>(size_t(n) * kGDecRows);
   dcount = dalloc<unsigned>(n);
   HIPCALL(hipMemset(dcount, 0, size_t(n) * sizeof(unsigned)));
@@ -3304,11 +3342,11 @@ void mmc_ctx::capture_iteration(Lat<T> Lt) {
                                                     Yh, n6, p, x, (double*)R, xref, nbuf[0], nbuf[1], dspart);
     launch_elem<T>(Lt, ctl);
     if (max_hist <= 10)
-      k_gather_g<T, 10><<<gPd, kElemThreads, 0, stream>>>(Lt, inst_d, gtd, grng, ctl, x, rec, cf_sph, cf_cap, dspart,
+      k_gather_g<T, 10><<<gPd, kElemThreads, 0, stream>>>(Lt, inst_d, gtd, ctl, x, rec, cf_sph, cf_cap, cent,
                                                           zbuf[0], zbuf[1], gbuf[0], gbuf[1], p, Sh, Yh, n6, gpart);
     else
-      k_gather_g<T, kMaxHist><<<gPd, kElemThreads, 0, stream>>>(Lt, inst_d, gtd, grng, ctl, x, rec, cf_sph, cf_cap,
-                                                                dspart, zbuf[0], zbuf[1], gbuf[0], gbuf[1], p, Sh, Yh,
+      k_gather_g<T, kMaxHist><<<gPd, kElemThreads, 0, stream>>>(Lt, inst_d, gtd, ctl, x, rec, cf_sph, cf_cap,
+                                                                cent, zbuf[0], zbuf[1], gbuf[0], gbuf[1], p, Sh, Yh,
                                                                 n6, gpart);
     launch_decide(ctl);
   }
@@ -3401,20 +3439,20 @@ void mmc_ctx::graph_tables(const std::vector<int>& set) {
 
 template <class T>
 void mmc_ctx::launch_elem(Lat<T> Lt, const Ctl* C) {
-  const int nb = gPb + gPc;
+  const int nb = gPb + gPc + int(gset.size());
   switch (elem_items) {
     case 64:
       k_elem_g<T, 64><<<nb, 64, 0, stream>>>(C, Lt, inst_d, gtb, gtc, x, (const double*)R, rec, ubuf[0], ubuf[1],
-                                             bpart, gPb, pcount, lref, cf_sph, cf_cap, cbuf[0], cbuf[1], cpart, gPc);
+                                             bpart, gPb, pcount, lref, cf_sph, cf_cap, cbuf[0], cbuf[1], cpart, gPc, gact, grng, dspart, gPd, cent);
       break;
     case 128:
       k_elem_g<T, 128><<<nb, 128, 0, stream>>>(C, Lt, inst_d, gtb, gtc, x, (const double*)R, rec, ubuf[0], ubuf[1],
-                                               bpart, gPb, pcount, lref, cf_sph, cf_cap, cbuf[0], cbuf[1], cpart, gPc);
+                                               bpart, gPb, pcount, lref, cf_sph, cf_cap, cbuf[0], cbuf[1], cpart, gPc, gact, grng, dspart, gPd, cent);
       break;
     default:
       k_elem_g<T, kBlock><<<nb, kBlock, 0, stream>>>(C, Lt, inst_d, gtb, gtc, x, (const double*)R, rec, ubuf[0],
                                                      ubuf[1], bpart, gPb, pcount, lref, cf_sph, cf_cap, cbuf[0],
-                                                     cbuf[1], cpart, gPc);
+                                                     cbuf[1], cpart, gPc, gact, grng, dspart, gPd, cent);
   }
 }
 
@@ -3554,12 +3592,12 @@ double mmc_ctx::time_loop_kernel(int kernel, int launches) {
           break;
         case MMC_K_GATHER_G:
           if (max_hist <= 10)
-            k_gather_g<T, 10><<<gPd, kElemThreads, 0, stream>>>(Lt, inst_d, gtd, grng, tctl, x, rec, cf_sph,
-                                                                cf_cap, dspart, zbuf[0], zbuf[1], gbuf[0], gbuf[1],
+            k_gather_g<T, 10><<<gPd, kElemThreads, 0, stream>>>(Lt, inst_d, gtd, tctl, x, rec, cf_sph,
+                                                                cf_cap, cent, zbuf[0], zbuf[1], gbuf[0], gbuf[1],
                                                                 p, Sh, Yh, n6, gpart);
           else
-            k_gather_g<T, kMaxHist><<<gPd, kElemThreads, 0, stream>>>(Lt, inst_d, gtd, grng, tctl, x, rec, cf_sph,
-                                                                      cf_cap, dspart, zbuf[0], zbuf[1], gbuf[0],
+            k_gather_g<T, kMaxHist><<<gPd, kElemThreads, 0, stream>>>(Lt, inst_d, gtd, tctl, x, rec, cf_sph,
+                                                                      cf_cap, cent, zbuf[0], zbuf[1], gbuf[0],
                                                                       gbuf[1], p, Sh, Yh, n6, gpart);
           break;
         default:
