@@ -461,13 +461,14 @@ __global__ __launch_bounds__(kBlock) void k_pairs(Lat<T> L, const Inst* __restri
 
 // k_contact: MC/src/sim.cpp:159-165 over the Verlet list.  Each shape
 // accumulates its own force; each pair's energy is counted by its lower shape.
-// partials [4][P]: E_contact, max_pen, contacts, sum of per-shape energy changes
-template <class T, int NT = kBlock>
-__device__ __forceinline__ void contact_body(Lat<T> L, const Inst& in, const double* __restrict__ x,
-                                                    const unsigned* __restrict__ pcount, const int* __restrict__ plist,
-                                                    int slab, double* __restrict__ cf_sph, double* __restrict__ cf_cap,
-                                                    TermBufs ct, double* __restrict__ part, int s, bool on, int blk,
-                                                    int P) {
+// partials [4][ncols]: E_contact, max_pen, contacts, sum of per-shape energy
+// changes; one column per wavefront (64 consecutive shapes), column col.
+template <class T>
+__device__ __forceinline__ void contact_walk(Lat<T> L, const Inst& in, const double* __restrict__ x,
+                                             const unsigned* __restrict__ pcount, const int* __restrict__ plist,
+                                             int slab, double* __restrict__ cf_sph, double* __restrict__ cf_cap,
+                                             TermBufs ct, double* __restrict__ part, int s, bool on, size_t col,
+                                             size_t ncols) {
   const int N = L.N, B = L.B;
   const Scalars& S = in.S;
   double v[4] = {0, 0, 0, 0};
@@ -529,17 +530,171 @@ __device__ __forceinline__ void contact_body(Lat<T> L, const Inst& in, const dou
     if (ct.out) ct.out[s] = v[0];
     if (ct.ref) v[3] = v[0] - ct.ref[s];
   }
-  // one partial column per wavefront (64 consecutive shapes of the instance):
-  // the fold over the columns does not depend on the workgroup size
-  constexpr int W = NT / kWave;
   v[0] = wave_sum(v[0]);
   v[1] = wave_max(v[1]);
   v[2] = wave_sum(v[2]);
   v[3] = wave_sum(v[3]);
   if ((threadIdx.x & (kWave - 1)) == 0)
 #pragma unroll
-    for (int k = 0; k < 4; k++) part[size_t(k) * P * W + size_t(blk) * W + threadIdx.x / kWave] = v[k];
+    for (int k = 0; k < 4; k++) part[size_t(k) * ncols + col] = v[k];
 }
+// contact_walk over blocks of NT shapes, P blocks: one partial column per
+// wavefront (64 consecutive shapes of the instance), so that the fold over
+// the columns does not depend on the workgroup size
+template <class T, int NT = kBlock>
+__device__ __forceinline__ void contact_body(Lat<T> L, const Inst& in, const double* __restrict__ x,
+                                             const unsigned* __restrict__ pcount, const int* __restrict__ plist,
+                                             int slab, double* __restrict__ cf_sph, double* __restrict__ cf_cap,
+                                             TermBufs ct, double* __restrict__ part, int s, bool on, int blk, int P) {
+  constexpr int W = NT / kWave;
+  contact_walk<T>(L, in, x, pcount, plist, slab, cf_sph, cf_cap, ct, part, s, on,
+                  size_t(blk) * W + threadIdx.x / kWave, size_t(P) * W);
+}
+// Contact shapes of a chunk of 64 whose Verlet lists are long: one block,
+// kContactLanes lanes per shape.  Lane c of a shape's group evaluates partner slots c,
+// c + kContactLanes, ... with contact_body's arithmetic; after each round the
+// group adds the round's pairs up in ascending slot order from the lanes'
+// factors (fv, u, pen, shuffled), so that every shape's force and energy sums
+// are contact_walk's sequential walk, operation for operation.  The per-shape
+// totals are folded over the chunk's 64 shapes by one wavefront as in
+// contact_walk (the chunk's partial column blk).  A wavefront's time is the
+// longest partner list of its 16 shapes over kContactLanes instead of the
+// longest of 64.  Chunks with at most kContactSerial partners per shape stay
+// with contact_walk (one lane per shape), where lane groups would cost four
+// times the instructions for little to share.
+constexpr int kContactLanes = kBlock / kWave;
+constexpr int kContactSerial = 8;
+template <class T>
+__device__ __forceinline__ void contact_group(Lat<T> L, const Inst& in, const double* __restrict__ x,
+                                              const unsigned* __restrict__ pcount, const int* __restrict__ plist,
+                                              int slab, double* __restrict__ cf_sph, double* __restrict__ cf_cap,
+                                              TermBufs ct, double* __restrict__ part, int first, int count, int blk,
+                                              int P) {
+  static_assert(kBlock == kWave * kContactLanes, "a block holds 64 shapes");
+  const int N = L.N, B = L.B;
+  const Scalars& S = in.S;
+  const int g = threadIdx.x / kContactLanes, c = threadIdx.x % kContactLanes;
+  const int gbase = (threadIdx.x & (kWave - 1)) & ~(kContactLanes - 1);
+  const bool on = g < count;
+  const int s = first + (on ? g : 0);
+  const size_t e0 = size_t(s) * slab;
+  const int n = on ? int(min(pcount[s], unsigned(slab))) : 0;
+  const bool scap = s - in.s0 >= in.N;
+  double v[4] = {0, 0, 0, 0};
+  v3<double> f0{0, 0, 0}, f1{0, 0, 0};
+  v3<double> sa{0, 0, 0}, sb{0, 0, 0};
+  double sr = 0;
+  int s0 = 0, s1 = 0;
+  if (n > c) shape_geom(L, in, x, s, sa, sb, sr, s0, s1);
+  // rounds of kContactLanes slots; the trip count is the wavefront's longest
+  // list, so that the shuffles below see every lane
+  for (int k = 0; __ballot(k < n) != 0; k += kContactLanes) {
+    const int j = k + c;
+    int flag = 0;  // 1: in contact, 2: the pair's lower shape
+    v3<double> fv{0, 0, 0};
+    double u = 0, pen = 0;
+    if (j < n) {
+      const int o = plist[e0 + j];
+      v3<double> oa, ob;
+      double orr;
+      int o0, o1;
+      shape_geom(L, in, x, o, oa, ob, orr, o0, o1);
+      const bool low = s < o, ocap = o - in.s0 >= in.N;
+      closest cp = low ? shape_closest(scap, sa, sb, ocap, oa, ob) : shape_closest(ocap, oa, ob, scap, sa, sb);
+      v3<double> delta = cp.pb - cp.pa;
+      double dist = norm(delta);
+      double target = (low ? sr + orr : orr + sr) + S.margin;
+      if (dist < target) {
+        int ia = (low ? s : o) - in.s0, ib = (low ? o : s) - in.s0;
+        v3<double> nv =
+            dist > 1e-12 ? over(delta, dist) : unit(v3<double>{0.37 + 0.001 * ia, 0.73, -0.41 + 0.001 * ib});
+        pen = target - dist;
+        double mag = S.kc * pen;
+        fv = low ? nv * mag : nv * (-mag);
+        u = low ? cp.s : cp.t;
+        flag = low ? 3 : 1;
+      }
+    }
+    if (__ballot(flag != 0) == 0) continue;
+#pragma unroll
+    for (int q = 0; q < kContactLanes; q++) {
+      const int src = gbase + q;
+      const int fq = __shfl(flag, src);
+      const v3<double> fvq{__shfl(fv.x, src), __shfl(fv.y, src), __shfl(fv.z, src)};
+      const double uq = __shfl(u, src), penq = __shfl(pen, src);
+      if (fq & 1) {
+        if (scap) {
+          f0 = f0 + fvq * (1 - uq);
+          f1 = f1 + fvq * uq;
+        } else {
+          f0 = f0 + fvq;
+        }
+        if (fq & 2) {
+          v[0] += 0.5 * S.kc * penq * penq;
+          v[1] = fmax(v[1], penq);
+          v[2] += 1;
+        }
+      }
+    }
+  }
+  __shared__ double cv[4][kWave];
+  if (c == 0) {
+    if (on) {
+      if (scap) {
+        const int kb = in.b0 + (s - in.s0 - in.N);
+        cf_cap[kb] = f0.x;
+        cf_cap[B + kb] = f0.y;
+        cf_cap[2 * B + kb] = f0.z;
+        cf_cap[3 * B + kb] = f1.x;
+        cf_cap[4 * B + kb] = f1.y;
+        cf_cap[5 * B + kb] = f1.z;
+      } else {
+        const int i = in.n0 + (s - in.s0);
+        cf_sph[i] = f0.x;
+        cf_sph[N + i] = f0.y;
+        cf_sph[2 * N + i] = f0.z;
+      }
+      if (ct.out) ct.out[s] = v[0];
+      if (ct.ref) v[3] = v[0] - ct.ref[s];
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++) cv[r][g] = v[r];
+  }
+  __syncthreads();
+  if (threadIdx.x < kWave) {
+    double w[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) w[r] = cv[r][threadIdx.x];
+    w[0] = wave_sum(w[0]);
+    w[1] = wave_max(w[1]);
+    w[2] = wave_sum(w[2]);
+    w[3] = wave_sum(w[3]);
+    if (threadIdx.x == 0)
+#pragma unroll
+      for (int r = 0; r < 4; r++) part[size_t(r) * P + blk] = w[r];
+  }
+  __syncthreads();  // cv is free for the block's next chunk
+}
+// Chunks of the device loop's contact table (64 shapes each) whose longest
+// Verlet list exceeds kContactSerial: flag[j] = 1 and j appended to list
+// (count reset by the caller).  One wavefront per chunk.  Run after every list
+// build and table change; the flags only route work (both routes give the
+// same sums), so a stale flag costs time, not results.
+__global__ __launch_bounds__(kBlock) void k_classify(const Blk* __restrict__ tc, int nchunks,
+                                                     const unsigned* __restrict__ pcount, int slab,
+                                                     int* __restrict__ flag, int* __restrict__ list,
+                                                     unsigned* __restrict__ count) {
+  const int j = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave, l = threadIdx.x & (kWave - 1);
+  if (j >= nchunks) return;
+  const Blk bk = tc[j];
+  const double n = l < bk.count ? double(min(pcount[bk.first + l], unsigned(slab))) : 0.0;
+  const bool heavy = wave_max(n) > kContactSerial;
+  if (l == 0) {
+    flag[j] = heavy ? 1 : 0;
+    if (heavy) list[atomicAdd(count, 1u)] = j;
+  }
+}
+
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_contact(Lat<T> L, const Inst* __restrict__ I, const Blk* __restrict__ tc,
                                                     const double* __restrict__ x, const unsigned* __restrict__ pcount,
@@ -1735,15 +1890,16 @@ __device__ __forceinline__ void centroid_wave(const double* __restrict__ part, i
   }
 }
 
-// beams and contact shapes of one trial in a single launch: blocks [0, Pb)
-// evaluate beams, blocks [Pb, Pb + Pc) contact shapes (independent work; one
-// launch gap fewer per trial); blocks [Pb + Pc, Pb + Pc + |act|) one running
+// beams and contact shapes of one trial in a single launch (independent work,
+// one launch gap fewer per trial): blocks [0, Pb) evaluate 256 beams each;
+// the next ceil(Pc / 4) blocks four chunks of 64 contact shapes each, one per
+// wavefront (contact_walk); the next H blocks the chunks flagged by
+// k_classify (long lists, contact_group); the last |act| blocks one running
 // instance each, the node centroid of its trial for k_gather_g (once per
-// instance instead of once per gather block).  E: items (threads) per block,
-// chosen by the loop's tuner; the partials have one column per wavefront
-// whatever E is.
-template <class T, int E>
-__global__ __launch_bounds__(E) void k_elem_g(const Ctl* __restrict__ Cs, Lat<T> L, const Inst* __restrict__ I,
+// instance instead of once per gather block).  The partials have one column
+// per 64 items.
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_elem_g(const Ctl* __restrict__ Cs, Lat<T> L, const Inst* __restrict__ I,
                                                    const Blk* __restrict__ tb, const Blk* __restrict__ tc,
                                                    const double* __restrict__ x, const double* __restrict__ R,
                                                    double* __restrict__ rec, double* __restrict__ ub0,
@@ -1752,29 +1908,50 @@ __global__ __launch_bounds__(E) void k_elem_g(const Ctl* __restrict__ Cs, Lat<T>
                                                    const ListRef* __restrict__ lr, double* __restrict__ cf_sph,
                                                    double* __restrict__ cf_cap, double* __restrict__ cb0,
                                                    double* __restrict__ cb1, double* __restrict__ cpart, int Pc,
+                                                   const int* __restrict__ cflag, const int* __restrict__ hlist,
+                                                   const unsigned* __restrict__ hcount, int H,
                                                    const int* __restrict__ act, const GRng* __restrict__ G,
                                                    const double* __restrict__ dspart, int Pd,
                                                    double* __restrict__ cent) {
-  if (int(blockIdx.x) >= Pb + Pc) {
-    const int k = act[blockIdx.x - Pb - Pc];
+  const int Pcb = (Pc + kBW - 1) / kBW;  // blocks of four contact chunks
+  const int b = blockIdx.x;
+  if (b < Pb) {
+    const Blk bk = tb[b];
+    const Ctl* C = Cs + bk.inst;
+    if (!ctl_trial_phase(C)) return;
+    const int par = C->par;
+    TermBufs bt{par ? ub1 : ub0, par ? ub0 : ub1};
+    beam_body<T>(L, I[bk.inst].M, x, R, rec, nullptr, bt, bpart, bk.first + threadIdx.x,
+                 int(threadIdx.x) < bk.count, b, Pb);
+  } else if (b < Pb + Pcb) {
+    // wavefront w walks chunk 4 (b - Pb) + w, one lane per shape, unless the
+    // chunk is flagged for the lane groups below (no barriers on this path)
+    const int j = (b - Pb) * kBW + int(threadIdx.x) / kWave, l = threadIdx.x & (kWave - 1);
+    if (j >= Pc || cflag[j]) return;
+    const Blk bk = tc[j];
+    const Ctl* C = Cs + bk.inst;
+    if (!ctl_trial_phase(C)) return;
+    const int par = C->par;
+    TermBufs ct{par ? cb1 : cb0, par ? cb0 : cb1};
+    contact_walk<T>(L, I[bk.inst], x, pcount, lr->plist, lr->slab, cf_sph, cf_cap, ct, cpart, bk.first + l,
+                    l < bk.count, size_t(j), size_t(Pc));
+  } else if (b < Pb + Pcb + H) {
+    // H blocks share the flagged chunks (block-uniform loop)
+    const unsigned nh = *hcount;
+    for (unsigned q = b - Pb - Pcb; q < nh; q += H) {
+      const int j = hlist[q];
+      const Blk bk = tc[j];
+      const Ctl* C = Cs + bk.inst;
+      if (!ctl_trial_phase(C)) continue;
+      const int par = C->par;
+      TermBufs ct{par ? cb1 : cb0, par ? cb0 : cb1};
+      contact_group<T>(L, I[bk.inst], x, pcount, lr->plist, lr->slab, cf_sph, cf_cap, ct, cpart, bk.first,
+                       bk.count, j, Pc);
+    }
+  } else {
+    const int k = act[b - Pb - Pcb - H];
     if (threadIdx.x < kWave && ctl_trial_phase(Cs + k))
       centroid_wave<kElemThreads>(dspart + 5 * size_t(Pd), Pd, G[k].pd0, G[k].pd1, I[k].N, cent + 2 * k);
-    return;
-  }
-  const bool beams = int(blockIdx.x) < Pb;
-  const int blk = beams ? blockIdx.x : blockIdx.x - Pb;
-  const Blk bk = beams ? tb[blk] : tc[blk];
-  const Ctl* C = Cs + bk.inst;
-  if (!ctl_trial_phase(C)) return;
-  const int par = C->par;
-  const int item = bk.first + threadIdx.x;
-  const bool on = int(threadIdx.x) < bk.count;
-  if (beams) {
-    TermBufs bt{par ? ub1 : ub0, par ? ub0 : ub1};
-    beam_body<T, E>(L, I[bk.inst].M, x, R, rec, nullptr, bt, bpart, item, on, blk, Pb);
-  } else {
-    TermBufs ct{par ? cb1 : cb0, par ? cb0 : cb1};
-    contact_body<T, E>(L, I[bk.inst], x, pcount, lr->plist, lr->slab, cf_sph, cf_cap, ct, cpart, item, on, blk, Pc);
   }
 }
 
@@ -2562,9 +2739,9 @@ struct mmc_ctx {
   // launch shapes of the loop, chosen by tune() at the first capture (neither
   // changes a result): items per block of the element kernel, blocks per
   // instance of the decision kernel; and the timings the choice was made on
-  int elem_items = kBlock, dec_split = 8;
+  int dec_split = 8;
   bool tuned = false;
-  double tune_elem_us[3] = {0, 0, 0}, tune_dec_us[4] = {0, 0, 0, 0};
+  double tune_dec_us[4] = {0, 0, 0, 0};
   template <class T>
   void launch_elem(Lat<T> Lt, const Ctl* C);
   void launch_decide(Ctl* C);
@@ -2581,6 +2758,11 @@ struct mmc_ctx {
   Ctl* tctl = nullptr;  // scratch control blocks of time_loop_kernel
   double* dspart = nullptr;    // [10][Pd] partials of the trial state kernel
   double* cent = nullptr;      // [n_inst][2] node centroid (x, z) of the trial (k_elem_g)
+  // contact chunks of the loop's table with long lists (k_classify)
+  int* cflag = nullptr;
+  int* hlist = nullptr;
+  unsigned* hcount = nullptr;
+  void classify();
   double* drows = nullptr;     // [n_inst][kGDecRows] rows folded by the decision kernel's blocks
   unsigned* dcount = nullptr;  // [n_inst] blocks of the decision kernel done with their rows
   double *zbuf[2] = {nullptr, nullptr}, *gbuf[2] = {nullptr, nullptr};
@@ -2679,14 +2861,14 @@ mmc_ctx::~mmc_ctx() {
                  "[loop] %d instances: initial evaluations %.3f s, captures %lld (%.3f s), replay loop %.3f s "
                  "(%lld graph launches, %lld polls), minimize %.3f s, final evaluations %.3f s, "
                  "%lld contact list builds %.3f s (%lld partial, %.3f s; count + scan %.3f, fill + pairs %.3f, "
-                 "rest %.3f); element blocks of %d, decision %d blocks per lattice\n",
+                 "rest %.3f); decision %d blocks per lattice\n",
                  n_inst, loop_s[0], (long long)captures, loop_s[1], loop_s[2], (long long)graph_replays,
                  (long long)loop_polls, loop_s[3], loop_s[4], (long long)rebuilds, loop_s[5],
-                 (long long)partial_builds, loop_s[6], loop_s[7], loop_s[8], loop_s[9], elem_items, dec_split);
+                 (long long)partial_builds, loop_s[6], loop_s[7], loop_s[8], loop_s[9], dec_split);
   void* ptrs[] = {bpos, bquat, mass, lock, ba, bb, L0, rad, area, inert, e0, plastic, inc_off, inc, adj_off, adj, x, R,
                   rec, trial, cf_sph, cf_cap, box, cnt, start, entries, flags, scan_tmp, spart, cpart, bpart, gpart,
                   vpart, stb, stn, stout, record, scal, z, g, p, zc, gc, zt, zin, Sh, Yh, xref, pcount, plist,
-                  inst_d, tn, tb, tc, td, sinst, tops_d, mask_d, ctl, lref, dspart, cent, drows, dcount, gtd, gtb, gtc,
+                  inst_d, tn, tb, tc, td, sinst, tops_d, mask_d, ctl, lref, dspart, cent, cflag, hlist, hcount, drows, dcount, gtd, gtb, gtc,
                   grng, gact, tctl, rtc, rtn};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
@@ -2993,6 +3175,11 @@ void mmc_ctx::setup(const mmc_lattice* const* Ls, const mmc_config* cs, int n, i
   ctl = dalloc<Ctl>(n);
   dspart = dalloc<double>(10 * size_t(Pd));
   cent = dalloc<double>(2 * size_t(n));
+  cflag = dalloc<int>(htc.size() * kBW);
+  hlist = dalloc<int>(htc.size() * kBW);
+  hcount = dalloc<unsigned>(1);
+  HIPCALL(hipMemset(cflag, 0, htc.size() * kBW * sizeof(int)));
+  HIPCALL(hipMemset(hcount, 0, sizeof(unsigned)));
   HIPCALL(hipMemset(cent, 0, 2 * size_t(n) * sizeof(double)));
   drows = dalloc<double>(size_t(n) * kGDecRows);
   dcount = dalloc<unsigned>(n);
@@ -3075,6 +3262,7 @@ void mmc_ctx::build_list(Lat<T> Lt, const Blk* ctab, int cblocks, const Blk* nta
   hl->slab = slab;
   hl->pad = 0;
   HIPCALL(hipMemcpyAsync(lref, hl, sizeof(ListRef), hipMemcpyHostToDevice, stream));
+  classify();
   HIPCALL(hipStreamSynchronize(stream));
   list_valid = true;
   rebuilds++;
@@ -3397,14 +3585,13 @@ void mmc_ctx::solve_blocks(Lat<T> Lt, const std::vector<int>& set, int max_trial
 }
 
 // Block tables of the device loop for the instances of `set` (ascending):
-// each instance's blocks of the full 64-node table, in order, and its beams
-// and contact shapes in blocks of elem_items from the instance's first item;
-// ranges of the beam and shape partials in wavefront columns.
+// each instance's blocks of the full 64-node table, in order, its beams in
+// blocks of 256 and its contact shapes in blocks of 64 from the instance's
+// first item; ranges of the beam and shape partials in columns of 64 items.
 void mmc_ctx::graph_tables(const std::vector<int>& set) {
   std::vector<Blk> d, b, c;
   std::vector<GRng> r(size_t(n_inst), GRng{0, 0, 0, 0, 0, 0, 0, 0});
-  const int E = elem_items, W = E / kWave;
-  auto split = [&](std::vector<Blk>& t, int inst, int first, int count) {
+  auto split = [&](std::vector<Blk>& t, int inst, int first, int count, int E) {
     for (int o = 0; o < count; o += E) t.push_back(Blk{inst, first + o, std::min(E, count - o), 0});
   };
   for (int i : set) {
@@ -3413,17 +3600,16 @@ void mmc_ctx::graph_tables(const std::vector<int>& set) {
     g.pd0 = int(d.size());
     d.insert(d.end(), htd.begin() + in.pd0, htd.begin() + in.pd1);
     g.pd1 = int(d.size());
-    g.pb0 = int(b.size()) * W;
-    split(b, i, in.b0, in.B);
-    g.pb1 = int(b.size()) * W;
-    g.pc0 = int(c.size()) * W;
-    split(c, i, in.s0, in.NS);
-    g.pc1 = int(c.size()) * W;
+    g.pb0 = int(b.size()) * kBW;
+    split(b, i, in.b0, in.B, kBlock);
+    g.pb1 = int(b.size()) * kBW;
+    g.pc0 = int(c.size());
+    split(c, i, in.s0, in.NS, kWave);
+    g.pc1 = int(c.size());
   }
-  // partial columns and table entries within what setup allocated (the
-  // kBlock tables and one column per wavefront of them)
-  if (b.size() * W > size_t(kBW) * htb.size() || c.size() * W > size_t(kBW) * htc.size() ||
-      b.size() > size_t(kBW) * htb.size() || c.size() > size_t(kBW) * htc.size())
+  // partial columns and table entries within what setup allocated (kBW
+  // entries and columns per block of the kBlock tables)
+  if (b.size() > htb.size() || c.size() > size_t(kBW) * htc.size())
     throw Fail(MMC_ERR_STATE, "device loop tables exceed their buffers");
   gPd = int(d.size());
   gPb = int(b.size());
@@ -3435,30 +3621,35 @@ void mmc_ctx::graph_tables(const std::vector<int>& set) {
   HIPCALL(hipMemcpy(grng, r.data(), r.size() * sizeof(GRng), hipMemcpyHostToDevice));
   HIPCALL(hipMemcpy(gact, set.data(), set.size() * sizeof(int), hipMemcpyHostToDevice));
   gset = set;
+  if (list_valid) classify();
 }
+
+// helper blocks of k_elem_g for the flagged contact chunks
+constexpr int kHelperBlocks = 256;
 
 template <class T>
 void mmc_ctx::launch_elem(Lat<T> Lt, const Ctl* C) {
-  const int nb = gPb + gPc + int(gset.size());
-  switch (elem_items) {
-    case 64:
-      k_elem_g<T, 64><<<nb, 64, 0, stream>>>(C, Lt, inst_d, gtb, gtc, x, (const double*)R, rec, ubuf[0], ubuf[1],
-                                             bpart, gPb, pcount, lref, cf_sph, cf_cap, cbuf[0], cbuf[1], cpart, gPc, gact, grng, dspart, gPd, cent);
-      break;
-    case 128:
-      k_elem_g<T, 128><<<nb, 128, 0, stream>>>(C, Lt, inst_d, gtb, gtc, x, (const double*)R, rec, ubuf[0], ubuf[1],
-                                               bpart, gPb, pcount, lref, cf_sph, cf_cap, cbuf[0], cbuf[1], cpart, gPc, gact, grng, dspart, gPd, cent);
-      break;
-    default:
-      k_elem_g<T, kBlock><<<nb, kBlock, 0, stream>>>(C, Lt, inst_d, gtb, gtc, x, (const double*)R, rec, ubuf[0],
-                                                     ubuf[1], bpart, gPb, pcount, lref, cf_sph, cf_cap, cbuf[0],
-                                                     cbuf[1], cpart, gPc, gact, grng, dspart, gPd, cent);
-  }
+  const int H = std::min(gPc, kHelperBlocks);
+  const int nb = gPb + (gPc + kBW - 1) / kBW + H + int(gset.size());
+  k_elem_g<T><<<nb, kBlock, 0, stream>>>(C, Lt, inst_d, gtb, gtc, x, (const double*)R, rec, ubuf[0], ubuf[1], bpart,
+                                         gPb, pcount, lref, cf_sph, cf_cap, cbuf[0], cbuf[1], cpart, gPc, cflag, hlist,
+                                         hcount, H, gact, grng, dspart, gPd, cent);
+}
+
+// Flags the chunks of the loop's contact table whose lists are long
+// (k_classify); stream-ordered after the list build or table change that
+// calls it, before any later replay.
+void mmc_ctx::classify() {
+  if (gPc == 0) return;
+  HIPCALL(hipMemsetAsync(hcount, 0, sizeof(unsigned), stream));
+  const int per = kBlock / kWave;
+  k_classify<<<(gPc + per - 1) / per, kBlock, 0, stream>>>(gtc, gPc, pcount, slab, cflag, hlist, hcount);
+  HIPCALL(hipGetLastError());
 }
 
 void mmc_ctx::launch_decide(Ctl* C) {
-  const int na = int(gset.size()), W = elem_items / kWave;
-  const int cw = gPc * W, bw = gPb * W;  // wavefront columns of the partials
+  const int na = int(gset.size());
+  const int cw = gPc, bw = gPb * kBW;  // columns of the partials (one per 64 items)
   switch (dec_split) {
     case 2:
       k_decide_g<2><<<na * 2, kBlock, 0, stream>>>(C, inst_d, gact, grng, dspart, gPd, cpart, cw, bpart, bw, gpart,
@@ -3479,41 +3670,25 @@ void mmc_ctx::launch_decide(Ctl* C) {
 }
 
 // Launch-shape tuner of the device loop, run at a context's first capture on
-// the state of its first evaluation: the element kernel at 64, 128 and 256
-// items per block and the decision kernel at 2, 4, 8 and 16 blocks per
-// instance, each timed with HIP events over back-to-back launches
-// (time_loop_kernel); the fastest of each is kept.  Neither choice changes a
-// result (one partial column per wavefront; one fixed fold order per row), so
-// the tuning is free to differ between contexts.  MMC_TUNE=0 keeps 256 and 8;
-// MMC_ELEM_ITEMS and MMC_DECIDE_BLOCKS fix either shape (tests).
+// the state of its first evaluation: the decision kernel at 2, 4, 8 and 16
+// blocks per instance, each timed with HIP events over back-to-back launches
+// (time_loop_kernel); the fastest is kept.  The choice changes no result (one
+// fixed fold order per row), so the tuning is free to differ between
+// contexts.  MMC_TUNE=0 keeps 8; MMC_DECIDE_BLOCKS fixes the split (tests).
 void mmc_ctx::tune() {
   tuned = true;
   const char* env = std::getenv("MMC_TUNE");
-  const char* fe = std::getenv("MMC_ELEM_ITEMS");
   const char* fd = std::getenv("MMC_DECIDE_BLOCKS");
-  if (fe || fd) {
-    const int e = fe ? std::atoi(fe) : kBlock, d = fd ? std::atoi(fd) : 8;
-    if ((e != 64 && e != 128 && e != kBlock) || (d != 2 && d != 4 && d != 8 && d != 16))
-      throw Fail(MMC_ERR_ARG, "MMC_ELEM_ITEMS must be 64, 128 or 256 and MMC_DECIDE_BLOCKS 2, 4, 8 or 16");
-    elem_items = e;
+  if (fd) {
+    const int d = std::atoi(fd);
+    if (d != 2 && d != 4 && d != 8 && d != 16) throw Fail(MMC_ERR_ARG, "MMC_DECIDE_BLOCKS must be 2, 4, 8 or 16");
     dec_split = d;
-    graph_tables(gset);
     return;
   }
   if (env && std::strcmp(env, "0") == 0) return;
-  const std::vector<int> set = gset;
   constexpr int kLaunches = 20;
-  const int items[3] = {64, 128, kBlock}, splits[4] = {2, 4, 8, 16};
+  const int splits[4] = {2, 4, 8, 16};
   int best = 0;
-  for (int k = 0; k < 3; k++) {
-    elem_items = items[k];
-    graph_tables(set);
-    tune_elem_us[k] = 1e3 * time_loop_kernel(MMC_K_ELEM_G, kLaunches);
-    if (tune_elem_us[k] < tune_elem_us[best]) best = k;
-  }
-  elem_items = items[best];
-  graph_tables(set);
-  best = 0;
   for (int k = 0; k < 4; k++) {
     dec_split = splits[k];
     tune_dec_us[k] = 1e3 * time_loop_kernel(MMC_K_DECIDE_G, kLaunches);
@@ -4086,12 +4261,9 @@ int mmc_time_kernel(mmc_ctx* c, int32_t kernel, int32_t launches, double* mean_m
   });
 }
 
-int mmc_loop_shape(mmc_ctx* c, int32_t* elem_items, int32_t* decide_blocks, double* elem_us, double* decide_us) {
+int mmc_loop_shape(mmc_ctx* c, int32_t* decide_blocks, double* decide_us) {
   return guarded(c, [&] {
-    if (elem_items) *elem_items = c->elem_items;
     if (decide_blocks) *decide_blocks = c->dec_split;
-    if (elem_us)
-      for (int k = 0; k < 3; k++) elem_us[k] = c->tune_elem_us[k];
     if (decide_us)
       for (int k = 0; k < 4; k++) decide_us[k] = c->tune_dec_us[k];
   });

@@ -240,7 +240,7 @@ def lib():
         L.mmc_kernel_timer.argtypes = [P, C.c_int32]
         L.mmc_kernel_timer_read.argtypes = [P, C.POINTER(C.c_int64), dp]
         L.mmc_time_kernel.argtypes = [P, C.c_int32, C.c_int32, dp]
-        L.mmc_loop_shape.argtypes = [P, C.POINTER(C.c_int32), C.POINTER(C.c_int32), dp, dp]
+        L.mmc_loop_shape.argtypes = [P, C.POINTER(C.c_int32), dp]
         L.mmc_evaluations.argtypes = [P]
         L.mmc_evaluations.restype = C.c_int64
         L.mmc_batch_create.argtypes = [C.POINTER(Config), C.c_int32, C.c_int32, C.POINTER(P)]
@@ -472,12 +472,11 @@ class Simulator:
         return int(lib().mmc_evaluations(self.context()))
 
     def loop_shape(self) -> dict:
-        """Launch shapes chosen by the device loop's tuner, and its timings (us)."""
-        e, d = C.c_int32(), C.c_int32()
-        eu, du = (C.c_double * 3)(), (C.c_double * 4)()
-        lib().mmc_loop_shape(self.context(), C.byref(e), C.byref(d), eu, du)
-        return {"elem_items": e.value, "decide_blocks": d.value,
-                "elem_us": {str(k): round(v, 3) for k, v in zip((64, 128, 256), eu)},
+        """Decision split chosen by the device loop's tuner, and its timings (us)."""
+        d = C.c_int32()
+        du = (C.c_double * 4)()
+        lib().mmc_loop_shape(self.context(), C.byref(d), du)
+        return {"decide_blocks": d.value,
                 "decide_us": {str(k): round(v, 3) for k, v in zip((2, 4, 8, 16), du)}}
 
     def time_kernel(self, kernel: str, launches: int = 200) -> float:

@@ -207,13 +207,12 @@ int64_t mmc_evaluations(const mmc_ctx* ctx);
  * they write only the loop's scratch buffers, so the solver state is left as
  * it was. */
 int mmc_time_kernel(mmc_ctx* ctx, int32_t kernel, int32_t launches, double* mean_ms);
-/* Launch shapes the device loop's tuner chose at the context's first
- * minimisation (no reference counterpart): items per block of the element
- * kernel (64, 128 or 256) and blocks per lattice of the decision kernel (2, 4,
- * 8 or 16), with the mean launch times (us) it measured for each candidate
- * (elem_us[3], decide_us[4]; zeros before the first minimisation or with
- * MMC_TUNE=0).  Neither shape changes a result.  Any pointer may be NULL. */
-int mmc_loop_shape(mmc_ctx* ctx, int32_t* elem_items, int32_t* decide_blocks, double* elem_us, double* decide_us);
+/* Launch shape the device loop's tuner chose at the context's first
+ * minimisation (no reference counterpart): blocks per lattice of the decision
+ * kernel (2, 4, 8 or 16), with the mean launch times (us) it measured for each
+ * candidate (decide_us[4]; zeros before the first minimisation or with
+ * MMC_TUNE=0).  The split changes no result.  Either pointer may be NULL. */
+int mmc_loop_shape(mmc_ctx* ctx, int32_t* decide_blocks, double* decide_us);
 
 /* Simulator::beamPotential(x, R), MC/src/sim.cpp:468-509, evaluated on the device
  * against the committed plastic state.  x is 3N, q is 4N (w,x,y,z). */

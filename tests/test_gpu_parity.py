@@ -329,26 +329,24 @@ def test_run_config1_mixed(gpu, po, loop_engine):
     assert err["loading_work"] <= 2e-5 and err["dissipated"] <= 5e-5
 
 
-@pytest.mark.parametrize("shape", [("64", "2"), ("128", "4"), ("256", "16")])
-def test_launch_shapes_leave_results_unchanged(gpu, shape, monkeypatch):
-    """The loop's tuner picks the element kernel's block size and the decision
-    kernel's split per context; neither may change a result: the 8x8x4 curve
-    (mixed policy, 12 + 4 increments) with each fixed shape equals, bit for
-    bit, the curve at the untuned shapes (256 items, 8 blocks)."""
+@pytest.mark.parametrize("split", ["2", "4", "16"])
+def test_launch_shapes_leave_results_unchanged(gpu, split, monkeypatch):
+    """The loop's tuner picks the decision kernel's split per context; it may
+    not change a result: the 8x8x4 curve (mixed policy, 12 + 4 increments)
+    with each fixed split equals, bit for bit, the curve at the untuned split
+    (8 blocks)."""
     mmc = gpu
     cfg = mmc.cubic_config(8, 8, 4, load=12, unload=4)
     monkeypatch.setenv("MMC_DEVICE_LOOP", "graph")
     monkeypatch.setenv("MMC_TUNE", "0")
     ref = mmc.Simulator(cfg, mmc.MMC_MIXED).run()
     monkeypatch.delenv("MMC_TUNE")
-    monkeypatch.setenv("MMC_ELEM_ITEMS", shape[0])
-    monkeypatch.setenv("MMC_DECIDE_BLOCKS", shape[1])
+    monkeypatch.setenv("MMC_DECIDE_BLOCKS", split)
     sim = mmc.Simulator(cfg, mmc.MMC_MIXED)
     got = sim.run()
-    ctx = sim.context()
-    e, d = mmc.C.c_int32(), mmc.C.c_int32()
-    mmc.lib().mmc_loop_shape(ctx, mmc.C.byref(e), mmc.C.byref(d), None, None)
-    assert (e.value, d.value) == (int(shape[0]), int(shape[1]))
+    d = mmc.C.c_int32()
+    mmc.lib().mmc_loop_shape(sim.context(), mmc.C.byref(d), None)
+    assert d.value == int(split)
     assert len(got) == len(ref)
     for name in ref.dtype.names:
         assert np.array_equal(got[name], ref[name]), name
