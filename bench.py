@@ -49,7 +49,7 @@ HBM_PEAK_GBS = 8000.0
 # calibrated against device copies of known size (commands and figures in
 # profiles/); keyed by lattice cells
 BEAM_TRAFFIC_BYTES = {(32, 32, 16): 13.08e6}
-GATHER_TRAFFIC_BYTES = {(32, 32, 16): 37.75e6}
+GATHER_TRAFFIC_BYTES = {(32, 32, 16): 37.70e6}
 
 
 def workload(args):
