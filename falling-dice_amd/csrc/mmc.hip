@@ -563,7 +563,7 @@ __device__ __forceinline__ void contact_body(Lat<T> L, const Inst& in, const dou
 // with contact_walk (one lane per shape), where lane groups would cost four
 // times the instructions for little to share.
 constexpr int kContactLanes = kBlock / kWave;
-constexpr int kContactSerial = 8;
+constexpr int kContactSerial = 8;  // default; MMC_CONTACT_SERIAL overrides (tests)
 template <class T>
 __device__ __forceinline__ void contact_group(Lat<T> L, const Inst& in, const double* __restrict__ x,
                                               const unsigned* __restrict__ pcount, const int* __restrict__ plist,
@@ -676,19 +676,19 @@ __device__ __forceinline__ void contact_group(Lat<T> L, const Inst& in, const do
   __syncthreads();  // cv is free for the block's next chunk
 }
 // Chunks of the device loop's contact table (64 shapes each) whose longest
-// Verlet list exceeds kContactSerial: flag[j] = 1 and j appended to list
+// Verlet list exceeds `serial` partners: flag[j] = 1 and j appended to list
 // (count reset by the caller).  One wavefront per chunk.  Run after every list
 // build and table change; the flags only route work (both routes give the
 // same sums), so a stale flag costs time, not results.
 __global__ __launch_bounds__(kBlock) void k_classify(const Blk* __restrict__ tc, int nchunks,
-                                                     const unsigned* __restrict__ pcount, int slab,
+                                                     const unsigned* __restrict__ pcount, int slab, int serial,
                                                      int* __restrict__ flag, int* __restrict__ list,
                                                      unsigned* __restrict__ count) {
   const int j = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave, l = threadIdx.x & (kWave - 1);
   if (j >= nchunks) return;
   const Blk bk = tc[j];
   const double n = l < bk.count ? double(min(pcount[bk.first + l], unsigned(slab))) : 0.0;
-  const bool heavy = wave_max(n) > kContactSerial;
+  const bool heavy = wave_max(n) > double(serial);
   if (l == 0) {
     flag[j] = heavy ? 1 : 0;
     if (heavy) list[atomicAdd(count, 1u)] = j;
@@ -2762,6 +2762,7 @@ struct mmc_ctx {
   int* cflag = nullptr;
   int* hlist = nullptr;
   unsigned* hcount = nullptr;
+  int contact_serial = kContactSerial;  // -1: every chunk to the lane groups
   void classify();
   double* drows = nullptr;     // [n_inst][kGDecRows] rows folded by the decision kernel's blocks
   unsigned* dcount = nullptr;  // [n_inst] blocks of the decision kernel done with their rows
@@ -3178,6 +3179,7 @@ void mmc_ctx::setup(const mmc_lattice* const* Ls, const mmc_config* cs, int n, i
   cflag = dalloc<int>(htc.size() * kBW);
   hlist = dalloc<int>(htc.size() * kBW);
   hcount = dalloc<unsigned>(1);
+  if (const char* cs = std::getenv("MMC_CONTACT_SERIAL")) contact_serial = std::atoi(cs);
   HIPCALL(hipMemset(cflag, 0, htc.size() * kBW * sizeof(int)));
   HIPCALL(hipMemset(hcount, 0, sizeof(unsigned)));
   HIPCALL(hipMemset(cent, 0, 2 * size_t(n) * sizeof(double)));
@@ -3643,7 +3645,8 @@ void mmc_ctx::classify() {
   if (gPc == 0) return;
   HIPCALL(hipMemsetAsync(hcount, 0, sizeof(unsigned), stream));
   const int per = kBlock / kWave;
-  k_classify<<<(gPc + per - 1) / per, kBlock, 0, stream>>>(gtc, gPc, pcount, slab, cflag, hlist, hcount);
+  k_classify<<<(gPc + per - 1) / per, kBlock, 0, stream>>>(gtc, gPc, pcount, slab, contact_serial, cflag, hlist,
+                                                           hcount);
   HIPCALL(hipGetLastError());
 }
 

@@ -350,3 +350,23 @@ def test_launch_shapes_leave_results_unchanged(gpu, split, monkeypatch):
     assert len(got) == len(ref)
     for name in ref.dtype.names:
         assert np.array_equal(got[name], ref[name]), name
+
+
+@pytest.mark.parametrize("serial", ["-1", "0", "1000000"])
+def test_contact_routing_leaves_results_unchanged(gpu, serial, monkeypatch):
+    """The graph loop evaluates a chunk of 64 contact shapes either one lane
+    per shape or, when its longest Verlet list is long, four lanes per shape
+    whose pair contributions are added in ascending slot order.  Routing every
+    chunk to the lane groups (-1), every chunk with a partner (0) or none
+    (1000000) must give the default curve bit for bit: 8x8x4 at 30% strain
+    (platen and self-contact), mixed policy, 12 + 4 increments."""
+    mmc = gpu
+    cfg = mmc.cubic_config(8, 8, 4, load=12, unload=4)
+    monkeypatch.setenv("MMC_DEVICE_LOOP", "graph")
+    ref = mmc.Simulator(cfg, mmc.MMC_MIXED).run()
+    assert ref["active_contacts"].max() > 0
+    monkeypatch.setenv("MMC_CONTACT_SERIAL", serial)
+    got = mmc.Simulator(cfg, mmc.MMC_MIXED).run()
+    assert len(got) == len(ref)
+    for name in ref.dtype.names:
+        assert np.array_equal(got[name], ref[name]), name
