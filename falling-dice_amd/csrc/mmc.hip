@@ -2621,6 +2621,27 @@ __global__ __launch_bounds__(kBlock) void k_take_current(const Blk* __restrict__
   const double* src = Cs[bk.inst].par ? z1 : z0;
   for (int d = 0; d < 6; d++) z[size_t(d) * N + i] = src[size_t(d) * N + i];
 }
+// z = 0 on the nodes of the masked instances
+__global__ __launch_bounds__(kBlock) void k_zero_masked(const Blk* __restrict__ tn, const int* __restrict__ mask,
+                                                        int N, double* __restrict__ z) {
+  const Blk bk = tn[blockIdx.x];
+  if (!mask[bk.inst] || int(threadIdx.x) >= bk.count) return;
+  const int i = bk.first + threadIdx.x;
+  for (int d = 0; d < 6; d++) z[size_t(d) * N + i] = 0.0;
+}
+// a masked instance's loop state at the start of a solve: iterate z = 0 in
+// slot 0, its gradient g (evaluated there) in slot 0
+__global__ __launch_bounds__(kBlock) void k_seed_masked(const Blk* __restrict__ tn, const int* __restrict__ mask,
+                                                        int N, const double* __restrict__ g,
+                                                        double* __restrict__ z0, double* __restrict__ g0) {
+  const Blk bk = tn[blockIdx.x];
+  if (!mask[bk.inst] || int(threadIdx.x) >= bk.count) return;
+  const int i = bk.first + threadIdx.x;
+  for (int d = 0; d < 6; d++) {
+    z0[size_t(d) * N + i] = 0.0;
+    g0[size_t(d) * N + i] = g[size_t(d) * N + i];
+  }
+}
 
 }  // namespace
 
@@ -2696,6 +2717,8 @@ struct mmc_ctx {
   double* tops_d = nullptr;   // platen of every instance (host-driven evaluations)
   double* htops = nullptr;    // pinned staging of tops_d
   int* mask_d = nullptr;      // commit mask
+  int* smask_d = nullptr;     // lattices started by batch_start
+  std::vector<char> in_flight;  // started by batch_start, not yet finished
   int* hmask = nullptr;       // pinned staging of mask_d
   // Verlet contact list
   double* xref = nullptr;
@@ -2834,6 +2857,14 @@ struct mmc_ctx {
   void capture(const std::vector<int>& set);
   double time_loop_kernel(int kernel, int launches);
   void minimize_graph(const double* tops, const int32_t* active, mmc_opt_stats* st);
+  void init_ctl(int b, double top, bool run);
+  // per-lattice progression of a batch: each lattice starts a solve when the
+  // caller says so and is finished when its solve ends, the others go on
+  std::vector<double> atop;  // every instance's current platen target
+  void batch_iterates(const int32_t* zmask);
+  void batch_start(const double* tops, const int32_t* mask);
+  void batch_advance(int min_done, int32_t* running);
+  void batch_finish(const int32_t* mask, mmc_opt_stats* st, mmc_objective_out* fin);
   void minimize_host(double top, mmc_opt_stats& st);
   void final_evaluation(const double* tops, mmc_opt_stats* st, mmc_objective_out* fin);
   void evaluate(const double* zsrc, const double* psrc, double step, double* zdst, double* gdst, const double* tops,
@@ -2869,7 +2900,7 @@ mmc_ctx::~mmc_ctx() {
   void* ptrs[] = {bpos, bquat, mass, lock, ba, bb, L0, rad, area, inert, e0, plastic, inc_off, inc, adj_off, adj, x, R,
                   rec, trial, cf_sph, cf_cap, box, cnt, start, entries, flags, scan_tmp, spart, cpart, bpart, gpart,
                   vpart, stb, stn, stout, record, scal, z, g, p, zc, gc, zt, zin, Sh, Yh, xref, pcount, plist,
-                  inst_d, tn, tb, tc, td, sinst, tops_d, mask_d, ctl, lref, dspart, cent, cflag, hlist, hcount, drows, dcount, gtd, gtb, gtc,
+                  inst_d, tn, tb, tc, td, sinst, tops_d, mask_d, smask_d, ctl, lref, dspart, cent, cflag, hlist, hcount, drows, dcount, gtd, gtb, gtc,
                   grng, gact, tctl, rtc, rtn};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
@@ -3189,6 +3220,7 @@ void mmc_ctx::setup(const mmc_lattice* const* Ls, const mmc_config* cs, int n, i
   lref = dalloc<ListRef>(1);
   tops_d = dalloc<double>(n);
   mask_d = dalloc<int>(n);
+  smask_d = dalloc<int>(n);
   inst_d = dalloc<Inst>(n);
   upload_inst();  // the instances are complete here (table mask, entry capacity)
   for (int b = 0; b < 2; b++) {
@@ -3810,6 +3842,42 @@ __global__ __launch_bounds__(kBlock) void k_release_hold(Ctl* __restrict__ Cs, i
 // reads the 16-word status heads of the control blocks after each batch.  It
 // steps in only to rebuild a stale contact list (status HOLD) and for the
 // initial evaluation.  Inactive instances (active[b] == 0) are left alone.
+// Control block of instance b at the start of a solve at platen top, its
+// current record from the evaluation at z = 0 (hrec); run = false marks it as
+// not part of the solve.
+void mmc_ctx::init_ctl(int b, double top, bool run) {
+  Ctl& h = hctl[b];
+  std::memset(&h, 0, sizeof(Ctl));
+  const mmc_config& cf = icfg[b];
+  std::memcpy(h.cur, hrec + b * kRec, kRec * sizeof(double));
+  h.m = cf.history;
+  h.max_it = cf.max_iterations;
+  h.max_ls = cf.max_line_search;
+  h.rel_escape = cf.relative_escape;
+  h.top = top;
+  h.armijo = cf.armijo;
+  h.min_step = cf.min_line_step;
+  h.tol_rms = cf.gradient_rms_tolerance;
+  h.tol_max = cf.gradient_max_tolerance;
+  h.rel_tol = cf.relative_tolerance;
+  h.pen_tol = cf.penetration_tolerance;
+  h.ts = cf.translation_scale;
+  h.skin_half = 0.5 * hinst[b].S.skin;
+  h.ls_mode = cf.line_search;
+  h.wolfe = precision == MMC_MIXED;
+  h.wolfe_eps = 1e-6;
+  h.phase = PH_DIRECTION;
+  h.coef.cg = 0.10;
+  if (!run) {
+    h.status = ST_CONVERGED;  // not part of this solve
+  } else if (cf.max_iterations <= 0) {
+    h.status = ST_ITERCAP;
+  } else {
+    h.iterations = 1;
+    h.status = host_converged(cf, h.cur) ? ST_CONVERGED : ST_RUNNING;
+  }
+}
+
 void mmc_ctx::minimize_graph(const double* tops, const int32_t* active, mmc_opt_stats* st) {
   const int n = n_inst;
   auto clock = [] { return std::chrono::steady_clock::now(); };
@@ -3819,39 +3887,10 @@ void mmc_ctx::minimize_graph(const double* tops, const int32_t* active, mmc_opt_
   terms_out0 = true;
   evaluate(zbuf[0], nullptr, 0.0, nullptr, gbuf[0], tops, false, rec_slot(0));
   loop_s[0] += secs(t_init, clock());
-  std::memset(hctl, 0, size_t(n) * sizeof(Ctl));
   bool any = false;
   for (int b = 0; b < n; b++) {
-    Ctl& h = hctl[b];
-    const mmc_config& cf = icfg[b];
-    std::memcpy(h.cur, hrec + b * kRec, kRec * sizeof(double));
-    h.m = cf.history;
-    h.max_it = cf.max_iterations;
-    h.max_ls = cf.max_line_search;
-    h.rel_escape = cf.relative_escape;
-    h.top = tops[b];
-    h.armijo = cf.armijo;
-    h.min_step = cf.min_line_step;
-    h.tol_rms = cf.gradient_rms_tolerance;
-    h.tol_max = cf.gradient_max_tolerance;
-    h.rel_tol = cf.relative_tolerance;
-    h.pen_tol = cf.penetration_tolerance;
-    h.ts = cf.translation_scale;
-    h.skin_half = 0.5 * hinst[b].S.skin;
-    h.ls_mode = cf.line_search;
-    h.wolfe = precision == MMC_MIXED;
-    h.wolfe_eps = 1e-6;
-    h.phase = PH_DIRECTION;
-    h.coef.cg = 0.10;
-    if (active && !active[b]) {
-      h.status = ST_CONVERGED;  // not part of this solve
-    } else if (cf.max_iterations <= 0) {
-      h.status = ST_ITERCAP;
-    } else {
-      h.iterations = 1;
-      h.status = host_converged(cf, h.cur) ? ST_CONVERGED : ST_RUNNING;
-    }
-    any = any || h.status == ST_RUNNING;
+    init_ctl(b, tops[b], !active || active[b]);
+    any = any || hctl[b].status == ST_RUNNING;
   }
   HIPCALL(hipMemcpyAsync(ctl, hctl, size_t(n) * sizeof(Ctl), hipMemcpyHostToDevice, stream));
   if (block_engine) {
@@ -3971,6 +4010,182 @@ void mmc_ctx::minimize_graph(const double* tops, const int32_t* active, mmc_opt_
   // energy's rounding noise, where "best" carries no information)
   k_take_current<<<Ps, kBlock, 0, stream>>>(tn, ctl, N, zbuf[0], zbuf[1], z);
   HIPCALL(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// Per-lattice progression of a batch (no reference counterpart).  The rounds
+// of minimize_graph start every lattice's solve together and end it when the
+// slowest has finished; here each lattice starts (batch_start) when its
+// driver has its next target and is finished (batch_finish) when its own
+// solve has ended, while the others go on in the same graph replays.  A
+// lattice's own sequence of operations is the round's: its iterate set to 0,
+// the evaluation there, the phase machine, the final evaluation at its
+// iterate, the commit; the other lattices' evaluations in between only
+// rewrite what every trial recomputes (poses, records, contact forces), so
+// its results are those of a round, bit for bit.  Line search 0 only (the
+// per-term line search keeps per-item energies of the current iterate that
+// an evaluation would rewrite).
+// z for an evaluation: the in-flight lattices' current iterates, 0 for the
+// others (not started, or finished and committed) and for those in zmask
+void mmc_ctx::batch_iterates(const int32_t* zmask) {
+  const int n = n_inst;
+  for (int b = 0; b < n; b++) hmask[b] = !in_flight[size_t(b)] || (zmask && zmask[b]);
+  HIPCALL(hipMemcpyAsync(mask_d, hmask, size_t(n) * sizeof(int), hipMemcpyHostToDevice, stream));
+  k_take_current<<<Ps, kBlock, 0, stream>>>(tn, ctl, N, zbuf[0], zbuf[1], z);
+  k_zero_masked<<<Ps, kBlock, 0, stream>>>(tn, mask_d, N, z);
+  HIPCALL(hipGetLastError());
+}
+
+void mmc_ctx::batch_start(const double* tops, const int32_t* mask) {
+  if (block_engine) throw Fail(MMC_ERR_STATE, "per-lattice progression runs on the graph loop (MMC_DEVICE_LOOP=block)");
+  const int n = n_inst;
+  const bool first = atop.empty();
+  if (first) {
+    atop.assign(size_t(n), 0.0);
+    in_flight.assign(size_t(n), 0);
+  }
+  std::vector<int> sm(size_t(n), 0);
+  int started = 0;
+  for (int b = 0; b < n; b++) {
+    if (!mask[b]) continue;
+    if (in_flight[size_t(b)]) throw Fail(MMC_ERR_STATE, "batch_start of a lattice whose solve is in flight");
+    if (icfg[b].line_search != 0) throw Fail(MMC_ERR_ARG, "per-lattice progression supports line_search 0 only");
+    atop[size_t(b)] = tops[b];
+    sm[size_t(b)] = 1;
+    started++;
+  }
+  if (!started && !first) return;
+  const auto t0 = std::chrono::steady_clock::now();
+  batch_iterates(mask);
+  terms_out0 = false;
+  evaluate(z, nullptr, 0.0, nullptr, gc, atop.data(), false, rec_slot(0));
+  HIPCALL(hipMemcpyAsync(smask_d, sm.data(), size_t(n) * sizeof(int), hipMemcpyHostToDevice, stream));
+  k_seed_masked<<<Ps, kBlock, 0, stream>>>(tn, smask_d, N, gc, zbuf[0], gbuf[0]);
+  HIPCALL(hipGetLastError());
+  if (first) {
+    // every control block, the others marked as not solving
+    for (int b = 0; b < n; b++) init_ctl(b, atop[size_t(b)], sm[size_t(b)] != 0);
+    HIPCALL(hipMemcpyAsync(ctl, hctl, size_t(n) * sizeof(Ctl), hipMemcpyHostToDevice, stream));
+  } else {
+    for (int b = 0; b < n; b++)
+      if (sm[size_t(b)]) {
+        init_ctl(b, atop[size_t(b)], true);
+        HIPCALL(hipMemcpyAsync(ctl + b, hctl + b, sizeof(Ctl), hipMemcpyHostToDevice, stream));
+      }
+  }
+  HIPCALL(hipStreamSynchronize(stream));
+  for (int b = 0; b < n; b++)
+    if (sm[size_t(b)]) in_flight[size_t(b)] = 1;
+  loop_s[0] += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// Graph replays until at least min_done of the in-flight lattices (or all of
+// them) have ended their solves; running[b] = 1 for the in-flight lattices
+// still solving.
+void mmc_ctx::batch_advance(int min_done, int32_t* running) {
+  const int n = n_inst;
+  if (in_flight.empty()) in_flight.assign(size_t(n), 0);
+  auto heads = [&] {
+    HIPCALL(hipMemcpy2DAsync(hctl, sizeof(Ctl), ctl, sizeof(Ctl), 16 * sizeof(int), n, hipMemcpyDeviceToHost,
+                             stream));
+    HIPCALL(hipStreamSynchronize(stream));
+  };
+  auto live = [&](int b) { return hctl[b].status == ST_RUNNING || hctl[b].status == ST_HOLD; };
+  int flying = 0;
+  for (int b = 0; b < n; b++) flying += in_flight[size_t(b)] != 0;
+  const int need = std::max(1, std::min(min_done, flying));
+  auto ended = [&] {
+    int e = 0;
+    for (int b = 0; b < n; b++) e += in_flight[size_t(b)] && !live(b);
+    return e;
+  };
+  auto running_set = [&] {
+    std::vector<int> set;
+    for (int b = 0; b < n; b++)
+      if (in_flight[size_t(b)] && live(b)) set.push_back(b);
+    return set;
+  };
+  heads();
+  if (flying > 0 && ended() < need) {
+    const auto t_loop = std::chrono::steady_clock::now();
+    {
+      const std::vector<int> set = running_set();
+      const bool covered =
+          graph_exec != nullptr && std::includes(gset.begin(), gset.end(), set.begin(), set.end());
+      if (!set.empty() && (!covered || 2 * set.size() <= gset.size())) {
+        const auto t_cap = std::chrono::steady_clock::now();
+        capture(set);
+        loop_s[1] += std::chrono::duration<double>(std::chrono::steady_clock::now() - t_cap).count();
+      }
+    }
+    int max_it = 0, max_ls = 0;
+    for (const mmc_config& cf : icfg) max_it = std::max(max_it, cf.max_iterations), max_ls = std::max(max_ls, cf.max_line_search);
+    const int64_t guard = int64_t(max_it + 1) * (max_ls + 3) * 2 + 64;
+    int64_t replays = 0;
+    int batch = 2;
+    for (;;) {
+      std::vector<int> held;
+      for (int b = 0; b < n; b++)
+        if (in_flight[size_t(b)] && hctl[b].status == ST_HOLD) held.push_back(b);
+      if (!held.empty()) {
+        // the held trials' positions are still in x: rebuild there and redo them
+        rebuild_held(held);
+        k_release_hold<<<(n + kBlock - 1) / kBlock, kBlock, 0, stream>>>(ctl, n);
+      }
+      if (running_set().empty()) break;
+      for (int r = 0; r < batch; r++) HIPCALL(hipGraphLaunch(graph_exec, stream));
+      replays += int64_t(batch) * kGraphIterations;
+      graph_replays += batch;
+      batch = std::min(8, batch * 2);
+      heads();
+      loop_polls++;
+      if (ended() >= need) break;
+      if (replays > guard) throw Fail(MMC_ERR_STATE, "device minimiser exceeded its replay bound");
+      const std::vector<int> set = running_set();
+      if (!set.empty() && 2 * set.size() <= gset.size()) {
+        const auto t_cap = std::chrono::steady_clock::now();
+        capture(set);
+        loop_s[1] += std::chrono::duration<double>(std::chrono::steady_clock::now() - t_cap).count();
+      }
+    }
+    loop_s[2] += std::chrono::duration<double>(std::chrono::steady_clock::now() - t_loop).count();
+  }
+  for (int b = 0; b < n; b++) running[b] = in_flight[size_t(b)] && live(b);
+}
+
+// The final evaluation of the masked lattices (in flight, solves ended) at
+// their iterates, with trial states for mmc_commit_batch; statistics and
+// outputs into stats[b], fin[b] for the masked b only.
+void mmc_ctx::batch_finish(const int32_t* mask, mmc_opt_stats* so, mmc_objective_out* fo) {
+  const int n = n_inst;
+  if (in_flight.empty()) in_flight.assign(size_t(n), 0);
+  HIPCALL(hipMemcpyAsync(hctl, ctl, size_t(n) * sizeof(Ctl), hipMemcpyDeviceToHost, stream));
+  HIPCALL(hipStreamSynchronize(stream));
+  std::vector<mmc_opt_stats> s(static_cast<size_t>(n));
+  std::vector<mmc_objective_out> f(static_cast<size_t>(n));
+  for (int b = 0; b < n; b++) {
+    if (!mask[b]) continue;
+    if (!in_flight[size_t(b)] || hctl[b].status == ST_RUNNING || hctl[b].status == ST_HOLD)
+      throw Fail(MMC_ERR_STATE, "batch_finish of a lattice whose solve has not ended");
+    const Ctl& h = hctl[b];
+    s[size_t(b)].iterations = h.iterations;
+    s[size_t(b)].backtracks = h.backtracks;
+    s[size_t(b)].converged = h.status == ST_CONVERGED;
+    s[size_t(b)].evaluations = h.evals;
+    evaluations += h.evals;
+    total_evaluations += h.evals;
+  }
+  const auto t0 = std::chrono::steady_clock::now();
+  batch_iterates(nullptr);
+  final_evaluation(atop.data(), s.data(), f.data());
+  HIPCALL(hipStreamSynchronize(stream));
+  loop_s[4] += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  for (int b = 0; b < n; b++) {
+    if (!mask[b]) continue;
+    so[b] = s[size_t(b)];
+    if (fo) fo[b] = f[size_t(b)];
+    in_flight[size_t(b)] = 0;
+  }
 }
 
 // The same loop driven from the host (one synchronisation per trial): kept as
@@ -4104,6 +4319,27 @@ int mmc_minimize_batch(mmc_ctx* c, const double* tops, const int32_t* active, mm
     HIPCALL(hipStreamSynchronize(c->stream));
     c->loop_s[3] += std::chrono::duration<double>(t1 - t0).count();
     c->loop_s[4] += std::chrono::duration<double>(std::chrono::steady_clock::now() - t1).count();
+  });
+}
+
+int mmc_batch_start(mmc_ctx* c, const double* tops, const int32_t* mask) {
+  return guarded(c, [&] {
+    if (!tops || !mask) throw Fail(MMC_ERR_ARG, "NULL argument");
+    c->batch_start(tops, mask);
+  });
+}
+
+int mmc_batch_advance(mmc_ctx* c, int32_t min_done, int32_t* running) {
+  return guarded(c, [&] {
+    if (!running) throw Fail(MMC_ERR_ARG, "NULL argument");
+    c->batch_advance(min_done, running);
+  });
+}
+
+int mmc_batch_finish(mmc_ctx* c, const int32_t* mask, mmc_opt_stats* stats, mmc_objective_out* fin) {
+  return guarded(c, [&] {
+    if (!mask || !stats) throw Fail(MMC_ERR_ARG, "NULL argument");
+    c->batch_finish(mask, stats, fin);
   });
 }
 

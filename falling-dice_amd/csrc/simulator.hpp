@@ -186,6 +186,8 @@ class BatchSimulator {
   int64_t rounds_ = 0;
   double phase_s_[3] = {0, 0, 0};  // seconds in minimize, in commit + statistics, in run
   void check(int rc) const;
+  std::vector<Outcome> run_rounds(bool verbose);
+  std::vector<Outcome> run_per_lattice(bool verbose);
 };
 
 }  // namespace meta

@@ -238,6 +238,21 @@ int mmc_minimize_batch(mmc_ctx* ctx, const double* tops, const int32_t* active, 
 int mmc_commit_batch(mmc_ctx* ctx, const int32_t* mask);
 /* mmc_read_stats of every instance (out has n entries). */
 int mmc_read_stats_batch(mmc_ctx* ctx, mmc_state_stats* out);
+/* Per-lattice progression of a batch (no reference counterpart; the batched
+ * driver's alternative to rounds of mmc_minimize_batch, so that no lattice
+ * waits for the slowest one of a round).  mmc_batch_start begins the solve of
+ * every lattice b with mask[b] != 0 at platen tops[b] from z = 0 (a lattice
+ * whose solve is in flight is an error); the solves in flight go on.
+ * mmc_batch_advance runs the device loop until at least min_done of the
+ * lattices in flight (or all of them) have ended their solves, and sets
+ * running[b] (n entries) for those still solving.  mmc_batch_finish evaluates
+ * the lattices with mask[b] != 0, whose solves must have ended, at their
+ * iterates (stats[b], final_out[b]; trial states for mmc_commit_batch) and
+ * takes them out of flight.  Each lattice's results equal those of
+ * mmc_minimize_batch.  Needs line_search 0 and the graph-replayed loop. */
+int mmc_batch_start(mmc_ctx* ctx, const double* tops, const int32_t* mask);
+int mmc_batch_advance(mmc_ctx* ctx, int32_t min_done, int32_t* running);
+int mmc_batch_finish(mmc_ctx* ctx, const int32_t* mask, mmc_opt_stats* stats, mmc_objective_out* final_out);
 
 /* ==== host lattice construction (no device work) ========================== */
 /* Builds the lattice the facade would build for cfg (MC/src/sim.cpp:49-135:

@@ -6,7 +6,12 @@ frames must equal, bit for bit, those of mmc_sim_run on that lattice alone
 (which the parity tests pin against the oracle).  Both device-loop engines:
 the workgroup-per-lattice solver runs each lattice in its own workgroup, the
 graph-replayed loop runs every lattice's blocks in every replay (and re-captures
-over the running ones as the batch drains)."""
+over the running ones as the batch drains).  The driver runs rounds of
+mmc_minimize_batch (every lattice waits for the slowest of the round) or, with
+MMC_BATCH_PER_LATTICE=1 and the graph loop, advances every lattice through its
+own schedule (mmc_batch_start / advance / finish: a lattice starts its next
+solve while others are still in theirs); both must give the single runs'
+frames."""
 import numpy as np
 import pytest
 
@@ -31,6 +36,17 @@ def _same(fa, fb):
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_batch_equals_single_runs(gpu, precision, loop_engine):
     mmc = gpu
+    _batch_vs_single(mmc, precision)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batch_per_lattice_equals_single_runs(gpu, precision, monkeypatch):
+    monkeypatch.setenv("MMC_DEVICE_LOOP", "graph")
+    monkeypatch.setenv("MMC_BATCH_PER_LATTICE", "1")
+    _batch_vs_single(gpu, precision)
+
+
+def _batch_vs_single(mmc, precision):
     prec = mmc.MMC_FP64 if precision == "fp64" else mmc.MMC_MIXED
     cfgs = _configs(mmc)
     frames, status, err = mmc.BatchSimulator(cfgs, prec).run()
@@ -53,3 +69,9 @@ def test_batch_isolates_a_failed_lattice(gpu, loop_engine):
         mmc.Simulator(bad).run()
     assert str(e.value) in err
     _same(frames[1], mmc.Simulator(cfgs[1]).run())
+
+
+def test_batch_per_lattice_isolates_a_failed_lattice(gpu, monkeypatch):
+    monkeypatch.setenv("MMC_DEVICE_LOOP", "graph")
+    monkeypatch.setenv("MMC_BATCH_PER_LATTICE", "1")
+    test_batch_isolates_a_failed_lattice(gpu, "graph")
