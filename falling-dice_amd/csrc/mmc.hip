@@ -3892,6 +3892,8 @@ void mmc_ctx::minimize_graph(const double* tops, const int32_t* active, mmc_opt_
     init_ctl(b, tops[b], !active || active[b]);
     any = any || hctl[b].status == ST_RUNNING;
   }
+  // every lattice's solve starts here: none is in flight for mmc_batch_*
+  std::fill(in_flight.begin(), in_flight.end(), 0);
   HIPCALL(hipMemcpyAsync(ctl, hctl, size_t(n) * sizeof(Ctl), hipMemcpyHostToDevice, stream));
   if (block_engine) {
     // launches of kSolveBudget trials per workgroup: a lattice held for a
