@@ -588,17 +588,30 @@ __device__ __forceinline__ void contact_group(Lat<T> L, const Inst& in, const do
   if (n > c) shape_geom(L, in, x, s, sa, sb, sr, s0, s1);
   // rounds of kContactLanes slots; the trip count is the wavefront's longest
   // list, so that the shuffles below see every lane
+  // the partner of the lane's next slot and its pose are loaded while the
+  // current one is evaluated (one dependent load chain per round overlaps the
+  // previous round's arithmetic)
+  int onext = c < n ? plist[e0 + c] : -1;
+  v3<double> anext{0, 0, 0}, bnext{0, 0, 0};
+  double rnext = 0;
+  {
+    int o0, o1;
+    if (onext >= 0) shape_geom(L, in, x, onext, anext, bnext, rnext, o0, o1);
+  }
   for (int k = 0; __ballot(k < n) != 0; k += kContactLanes) {
     const int j = k + c;
+    const int o = onext;
+    const v3<double> oa = anext, ob = bnext;
+    const double orr = rnext;
+    onext = j + kContactLanes < n ? plist[e0 + j + kContactLanes] : -1;
+    if (onext >= 0) {
+      int o0, o1;
+      shape_geom(L, in, x, onext, anext, bnext, rnext, o0, o1);
+    }
     int flag = 0;  // 1: in contact, 2: the pair's lower shape
     v3<double> fv{0, 0, 0};
     double u = 0, pen = 0;
     if (j < n) {
-      const int o = plist[e0 + j];
-      v3<double> oa, ob;
-      double orr;
-      int o0, o1;
-      shape_geom(L, in, x, o, oa, ob, orr, o0, o1);
       const bool low = s < o, ocap = o - in.s0 >= in.N;
       closest cp = low ? shape_closest(scap, sa, sb, ocap, oa, ob) : shape_closest(ocap, oa, ob, scap, sa, sb);
       v3<double> delta = cp.pb - cp.pa;
