@@ -3563,7 +3563,11 @@ bool use_host_loop() {
 
 // kGraphIterations replays of the phase machine as one graph (see Ctl); every
 // launch covers all instances, each block gated on its own instance's phase.
-constexpr int kGraphIterations = 4;
+constexpr int kGraphIterations = 8;
+// graph launches between two reads of the status heads: trials grow from 8
+// to 32 per read as a solve goes on
+constexpr int kPollLaunches0 = 8 / kGraphIterations > 0 ? 8 / kGraphIterations : 1;
+constexpr int kPollLaunchesMax = 32 / kGraphIterations > 0 ? 32 / kGraphIterations : 1;
 template <class T>
 void mmc_ctx::capture_iteration(Lat<T> Lt) {
   if (graph_exec) HIPCALL(hipGraphExecDestroy(graph_exec));
@@ -3969,7 +3973,7 @@ void mmc_ctx::minimize_graph(const double* tops, const int32_t* active, mmc_opt_
   for (const mmc_config& cf : icfg) max_it = std::max(max_it, cf.max_iterations), max_ls = std::max(max_ls, cf.max_line_search);
   const int64_t guard = int64_t(max_it + 1) * (max_ls + 3) * 2 + 64;
   int64_t replays = 0;
-  int batch = 2;
+  int batch = kPollLaunches0;
   auto heads = [&] {
     HIPCALL(hipMemcpy2DAsync(hctl, sizeof(Ctl), ctl, sizeof(Ctl), 16 * sizeof(int), n, hipMemcpyDeviceToHost,
                              stream));
@@ -3994,7 +3998,7 @@ void mmc_ctx::minimize_graph(const double* tops, const int32_t* active, mmc_opt_
     for (int r = 0; r < batch; r++) HIPCALL(hipGraphLaunch(graph_exec, stream));
     replays += int64_t(batch) * kGraphIterations;
     graph_replays += batch;
-    batch = std::min(8, batch * 2);
+    batch = std::min(kPollLaunchesMax, batch * 2);
     state = heads();
     if (replays > guard) throw Fail(MMC_ERR_STATE, "device minimiser exceeded its replay bound");
     loop_polls++;
@@ -4137,7 +4141,7 @@ void mmc_ctx::batch_advance(int min_done, int32_t* running) {
     for (const mmc_config& cf : icfg) max_it = std::max(max_it, cf.max_iterations), max_ls = std::max(max_ls, cf.max_line_search);
     const int64_t guard = int64_t(max_it + 1) * (max_ls + 3) * 2 + 64;
     int64_t replays = 0;
-    int batch = 2;
+    int batch = kPollLaunches0;
     for (;;) {
       std::vector<int> held;
       for (int b = 0; b < n; b++)
@@ -4151,7 +4155,7 @@ void mmc_ctx::batch_advance(int min_done, int32_t* running) {
       for (int r = 0; r < batch; r++) HIPCALL(hipGraphLaunch(graph_exec, stream));
       replays += int64_t(batch) * kGraphIterations;
       graph_replays += batch;
-      batch = std::min(8, batch * 2);
+      batch = std::min(kPollLaunchesMax, batch * 2);
       heads();
       loop_polls++;
       if (ended() >= need) break;
