@@ -3724,7 +3724,7 @@ void mmc_ctx::launch_decide(Ctl* C) {
 // Launch-shape tuner of the device loop, run at a context's first capture on
 // the state of its first evaluation: the decision kernel at 2, 4, 8 and 16
 // blocks per instance, each timed with HIP events over back-to-back launches
-// (time_loop_kernel); the fastest is kept.  The choice changes no result (one
+// (time_loop_kernel); the most blocks within 3% of the fastest are kept.  The choice changes no result (one
 // fixed fold order per row), so the tuning is free to differ between
 // contexts.  MMC_TUNE=0 keeps 8; MMC_DECIDE_BLOCKS fixes the split (tests).
 void mmc_ctx::tune() {
@@ -3746,6 +3746,14 @@ void mmc_ctx::tune() {
     tune_dec_us[k] = 1e3 * time_loop_kernel(MMC_K_DECIDE_G, kLaunches);
     if (tune_dec_us[k] < tune_dec_us[best]) best = k;
   }
+  // the most blocks within 3% of the fastest: the timed launches find the
+  // partial rows in cache, while in the loop the other kernels have just
+  // written them, and more blocks spread those colder loads
+  for (int k = 3; k > best; k--)
+    if (tune_dec_us[k] <= 1.03 * tune_dec_us[best]) {
+      best = k;
+      break;
+    }
   dec_split = splits[best];
 }
 
