@@ -2965,7 +2965,7 @@ void mmc_ctx::setup(const mmc_lattice* const* Ls, const mmc_config* cs, int n, i
     c.translation_scale = c.cell_size;  // MC/src/sim.cpp:49
     if (Ls[b]->n_nodes <= 0 || Ls[b]->n_beams <= 0) throw Fail(MMC_ERR_ARG, "lattice has no nodes or beams");
     if (c.history < 0 || c.history > kMaxHist) throw Fail(MMC_ERR_ARG, "history must be in [0, 16]");
-    if (c.line_search != 0 && c.line_search != 1) throw Fail(MMC_ERR_ARG, "line_search must be 0 or 1");
+    if (c.line_search < 0 || c.line_search > 2) throw Fail(MMC_ERR_ARG, "line_search must be 0, 1 or 2");
     max_hist = std::max(max_hist, c.history);
     tn_ += Ls[b]->n_nodes;
     tb_ += Ls[b]->n_beams;
@@ -3888,8 +3888,8 @@ void mmc_ctx::init_ctl(int b, double top, bool run) {
   h.pen_tol = cf.penetration_tolerance;
   h.ts = cf.translation_scale;
   h.skin_half = 0.5 * hinst[b].S.skin;
-  h.ls_mode = cf.line_search;
-  h.wolfe = precision == MMC_MIXED;
+  h.ls_mode = cf.line_search != 0;
+  h.wolfe = precision == MMC_MIXED || cf.line_search == 2;
   h.wolfe_eps = 1e-6;
   h.phase = PH_DIRECTION;
   h.coef.cg = 0.10;

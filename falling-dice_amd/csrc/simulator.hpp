@@ -74,7 +74,7 @@ struct SimulationConfig {
   double relativeTolerance = 1.5e-3;
   double positionSigma = 0;
   int radiusMode = 0;
-  int lineSearch = 0;  // 0 total-energy Armijo (the reference's), 1 per-term energy change
+  int lineSearch = 0;  // 0 total-energy Armijo (the reference's), 1 per-term energy change; 2 per-term with the approximate-Wolfe slope test below 1e-6 |E|
   int precision = MMC_FP64;
   bool captureFrames = true;  // per-frame snapshots for the META3 cache
   bool verbose = true;        // per-step "[equilibrium]" line on stdout

@@ -188,6 +188,66 @@ def cubic_config(nx: int, ny: int, nz: int, *, cell=0.005, r_bot=0.0004, r_top=0
     return c
 
 
+def specimen_config() -> Config:
+    """The reference's 2x4x2 graded BCC specimen (MC/docs/VALIDATION.md:9-11)
+    under its coarse convergence schedule: 10 + 7 increments to 50% strain,
+    500 iterations, contact stiffness 1e6 (MC/src/main.cpp:213-221)."""
+    return default_config().replace(cells_x=2, cells_y=4, cells_z=2, max_strain=0.5, load_steps=10,
+                                    unload_steps=7, max_iterations=500, contact_stiffness=1e6)
+
+
+def tight(cfg: Config, tol: float = 1e-7) -> Config:
+    """The parity form of a workload: gradient tolerance tol (rms and max)
+    with the per-term Armijo test and its approximate-Wolfe completion below the
+    energy's resolution (line_search 2), relative escape off."""
+    return cfg.replace(line_search=2, gradient_rms_tolerance=tol, gradient_max_tolerance=tol,
+                       relative_escape=0)
+
+
+def splitmix64(v: int) -> int:
+    v = (v + 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
+    v = ((v ^ (v >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
+    v = ((v ^ (v >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
+    return v ^ (v >> 31)
+
+
+def batch_workload(kind: str, count: int, load: int = 60, unload: int = 60, indices=None):
+    """Seeded lattice sets for the batched solver.
+      uneven: cubes of n^3 cells, n uniform in 4..16, linear radius grading with
+              endpoints uniform in 0.3..0.7 mm, 30% strain
+      sweep:  6^3 cells, E log-uniform 1..3 GPa, yield uniform 30..80 MPa,
+              hardening uniform 2..10% of E, 40% strain
+    Lattice i draws from its own stream seeded with splitmix64(seed ^ i), so a
+    subset (indices) has the same lattices as the whole set."""
+    import numpy as np
+    seed = 0x494E445553545259
+    cfgs = []
+    for i in (range(count) if indices is None else indices):
+        rng = np.random.default_rng(splitmix64(seed ^ i))
+        if kind == "uneven":
+            n = int(rng.integers(4, 17))
+            r0, r1 = rng.uniform(3e-4, 7e-4, 2)
+            c = cubic_config(n, n, n, r_bot=float(r0), r_top=float(r1), strain=0.30, load=load, unload=unload,
+                             max_iterations=20000)
+        elif kind == "sweep":
+            E = float(np.exp(rng.uniform(np.log(1e9), np.log(3e9))))
+            c = cubic_config(6, 6, 6, strain=0.40, load=load, unload=unload, max_iterations=20000)
+            c = c.replace(young=E, yield_=float(rng.uniform(30e6, 80e6)), hardening=float(rng.uniform(0.02, 0.10)) * E)
+        else:
+            raise ValueError(kind)
+        cfgs.append(c)
+    return cfgs
+
+
+def large_config(nx=64, ny=64, nz=32, *, grad_tol=1e-6, max_iterations=20000, load=100, unload=100) -> Config:
+    """Contact-heavy workload: cubic cells of 5 mm, node positions jittered by a
+    seeded Gaussian of 2% of the cell, per-beam radii uniform 0.3..0.7 mm,
+    platen to 50% strain in 100 + 100 increments."""
+    cell = 0.005
+    return cubic_config(nx, ny, nz, cell=cell, r_bot=3e-4, r_top=7e-4, strain=0.5, load=load, unload=unload,
+                        grad_tol=grad_tol, max_iterations=max_iterations, sigma=0.02 * cell, radius_mode=1)
+
+
 # ---------------------------------------------------------------------------
 # GPU library
 # ---------------------------------------------------------------------------

@@ -80,7 +80,11 @@ typedef struct mmc_config {
                                  reference, MC/src/sim.cpp:179); 1 sum of per-beam,
                                  per-node and per-contact-shape energy differences (same
                                  test in exact arithmetic, resolves decreases far below
-                                 the round-off of the total)                            */
+                                 the round-off of the total); 2 the per-term test of 1,
+                                 and when the change is within 1e-6 |E| the approximate-
+                                 Wolfe slope test g(z + a p).p <= (2 armijo - 1) g(z).p
+                                 of Hager & Zhang (SIAM J. Optim. 16, 2005): reaches
+                                 gradient tolerances below the energy's resolution      */
 } mmc_config;
 
 /* Fills *cfg with the reference defaults of MC/include/sim.hpp:16-26 (BCC lattice,

@@ -300,6 +300,9 @@ struct Stats {
   bool converged = false;
 };
 
+// thrown by run() when the frame observer asks to stop: not an equilibrium failure
+struct RunStopped {};
+
 }  // namespace
 
 struct orc {
@@ -318,6 +321,9 @@ struct orc {
   bool have_last = false;
   std::string err;
   int evaluations = 0;
+  // optional per-frame observer of run(): returns nonzero to end the run there
+  int (*frame_cb)(const mmc_frame*, int32_t, void*) = nullptr;
+  void* frame_cb_user = nullptr;
 
   double jr() const { return cfg.joint_scale * cfg.max_radius; }
   void add_connected(uint64_t k) {
@@ -860,12 +866,20 @@ struct orc {
         cand = objective(zc, top, false);
         st.backtracks++;
         bool accept;
-        if (cfg.line_search == 1) {
+        if (cfg.line_search == 1 || cfg.line_search == 2) {
           // Armijo on the energy change summed per term: free of the
           // cancellation of two nearly equal totals
           double acc = 0;
           for (size_t i = 0; i < cand.terms.size(); i++) acc += cand.terms[i] - cur.terms[i];
-          accept = std::isfinite(cand.energy) && acc + (cand.ecen - cur.ecen) <= cfg.armijo * step * dg;
+          const double dE = acc + (cand.ecen - cur.ecen);
+          accept = std::isfinite(cand.energy) && dE <= cfg.armijo * step * dg;
+          // line_search 2: the approximate-Wolfe test of Hager & Zhang (SIAM
+          // J. Optim. 16, 2005, eq. 4.1, epsilon 1e-6 |E|) when the change is
+          // within the energy's resolution: the slope along p at the trial,
+          // phi'(step) = g(zc).p, against (2 armijo - 1) phi'(0)
+          if (!accept && cfg.line_search == 2 && std::isfinite(cand.energy) &&
+              std::abs(dE) <= 1e-6 * std::abs(cur.energy))
+            accept = vdot(cand.grad, p) <= (2.0 * cfg.armijo - 1.0) * dg;
         } else {
           accept = std::isfinite(cand.energy) && cand.energy <= cur.energy + cfg.armijo * step * dg;
         }
@@ -1061,6 +1075,7 @@ struct orc {
       prevForce = F;
       current = target;
       out.push_back(diagnostics(time, target, loading, fin, st));
+      if (frame_cb && frame_cb(&out.back(), int32_t(out.size()), frame_cb_user)) throw RunStopped{};
       if (verbose) {
         const mmc_frame& d = out.back();
         std::printf("[oracle] %3d %s strain=%.4f force=%.3f N residual=%.2e pen=%.3f mm iter=%d%s\n", accepted,
@@ -1419,6 +1434,7 @@ int orc_run(orc* h, mmc_frame* frames, int32_t cap, int32_t* n, int32_t verbose)
   int rc = 0;
   try {
     h->run(out, verbose != 0);
+  } catch (const RunStopped&) {
   } catch (const std::exception& e) {
     h->err = e.what();
     rc = MMC_ERR_EQUILIBRIUM;
@@ -1427,6 +1443,11 @@ int orc_run(orc* h, mmc_frame* frames, int32_t cap, int32_t* n, int32_t verbose)
   if (frames) std::memcpy(frames, out.data(), size_t(m) * sizeof(mmc_frame));
   if (n) *n = int32_t(out.size());
   return rc;
+}
+
+void orc_set_frame_callback(orc* h, int (*cb)(const mmc_frame*, int32_t, void*), void* user) {
+  h->frame_cb = cb;
+  h->frame_cb_user = user;
 }
 
 double orc_beam_potential(orc* h, const double* x, const double* q) {

@@ -37,6 +37,9 @@ double orc_objective(orc* h, const double* z, double top, int32_t trials, mmc_ob
 int orc_minimize(orc* h, double top, double* z, mmc_opt_stats* st, mmc_objective_out* fin);
 int orc_commit(orc* h);
 int orc_run(orc* h, mmc_frame* frames, int32_t cap, int32_t* n_frames, int32_t verbose);
+/* Observer called by orc_run after every accepted frame with the frame and the
+ * number of frames so far; a nonzero return ends the run there (rc 0). */
+void orc_set_frame_callback(orc* h, int (*cb)(const mmc_frame* frame, int32_t n_frames, void* user), void* user);
 double orc_beam_potential(orc* h, const double* x, const double* q);
 /* Energy change z0 -> z1 at platen top: out[0] summed per term (the
  * line_search 1 form), out[1] as the difference of total energies. */

@@ -20,6 +20,7 @@ ORACLE_LIB = HERE / "liboracle.so"
 REF_LIB = HERE / "_ref" / "libmmcref.so"
 
 _libs: dict = {}
+FRAME_CB = C.CFUNCTYPE(C.c_int, C.POINTER(mmc.Frame), C.c_int32, C.c_void_p)
 
 
 def _load(prefix: str):
@@ -50,6 +51,7 @@ def _load(prefix: str):
     f("beam_potential", [P, dp, dp], C.c_double)
     if prefix == "orc":
         f("energy_change", [P, dp, dp, C.c_double, dp], None)
+        f("set_frame_callback", [P, FRAME_CB, P], None)
     f("rigid_rotation_test", [P, dp])
     f("gradient_check", [P, dp])
     f("read_state", [P, dp, dp, dp], None)
@@ -148,11 +150,29 @@ class Solver:
         if rc:
             raise RuntimeError(self._fn("last_error")(self.h).decode())
 
-    def run(self, verbose=False, cap=None):
+    def run(self, verbose=False, cap=None, on_frame=None, limit=None):
+        """The reference schedule (tare, loading, unloading).  on_frame(frames)
+        is called with the frames so far after every accepted frame (oracle
+        only); the run ends after `limit` frames when that is given."""
         cap = cap or 4 * (self.cfg.load_steps + self.cfg.unload_steps + 1) + 4096
         frames = (mmc.Frame * cap)()
         n = C.c_int32()
+        cb = None
+        if on_frame is not None or limit is not None:
+            if self.prefix != "orc":
+                raise ValueError("frame observers are an oracle feature")
+            acc = []
+
+            def observe(frame, count, _user):
+                acc.append(np.frombuffer(C.string_at(frame, C.sizeof(mmc.Frame)), dtype=mmc.FRAME_DTYPE).copy())
+                if on_frame is not None:
+                    on_frame(np.concatenate(acc))
+                return 1 if (limit is not None and count >= limit) else 0
+            cb = FRAME_CB(observe)
+            self._fn("set_frame_callback")(self.h, cb, None)
         rc = self._fn("run")(self.h, frames, cap, C.byref(n), int(verbose))
+        if cb is not None:
+            self._fn("set_frame_callback")(self.h, FRAME_CB(), None)
         arr = np.frombuffer(bytes(frames)[: min(n.value, cap) * C.sizeof(mmc.Frame)],
                             dtype=mmc.FRAME_DTYPE).copy()
         if rc:

@@ -5,18 +5,34 @@ solver, pinned against the reference by tests/test_oracle_pin.py), kept so that
 GPU tests and the benchmark can compare whole curves without re-running it.
 Run from the repository root, one curve per call (each is single-threaded):
 
-  python tests/golden/make_curves.py cubic_8x8x4_tight
-  python tests/golden/make_curves.py cubic_32x32x16
-  python tests/golden/make_curves.py cubic_32x32x16_tight
+  python tests/golden/make_curves.py <curve> [max frames]
+
+Each accepted frame is written as it comes (<curve>_oracle_frames.npy), with
+<curve>_oracle_status.json holding the frame count, the oracle's error text if
+its schedule ended in an equilibrium failure, and the seconds spent.
 
 Curves:
-  cubic_8x8x4_tight      8x8x4 cubic workload, 60 + 60 increments, gradient
-                         tolerance 1e-7 (rms and max), per-term Armijo
-                         (line_search 1), relative escape off
-  cubic_32x32x16         32x32x16 cubic workload, 60 + 60 increments, gradient
-                         tolerance 1e-6, the reference's Armijo test
-  cubic_32x32x16_tight   32x32x16 at tolerance 1e-7, per-term Armijo
+  cubic_8x8x4_tight      8x8x4 cubic workload, 60 + 60 increments to 30%,
+                         gradient tolerance 1e-7 (rms and max), per-term Armijo
+                         and its approximate-Wolfe completion (line_search 2),
+                         relative escape off
+  cubic_32x32x16         32x32x16 cubic workload, 60 + 60 increments to 30%,
+                         gradient tolerance 1e-6, the reference's Armijo test,
+                         20,000 iterations per solve
+  cubic_32x32x16_tight   the same in the parity form (tolerance 1e-7,
+                         line_search 2)
+  large_16x16x8_tight    the contact-heavy family (2% node jitter, per-beam
+                         radii 0.3..0.7 mm, 50% strain, 100 + 100) on 16x16x8
+                         cells, parity form at gradient tolerance 3e-8 (the
+                         lattice softens past 13% strain, where 1e-7 leaves a
+                         band of ~1e-6 of the peak force)
+  bcc_222_tight          the reference's 2x4x2 graded BCC specimen under its
+                         coarse schedule (10 + 7 increments to 50%), parity
+                         form, 20,000 iterations per solve
+  uneven_<i>_tight       lattice i of the 256-lattice uneven set, tight
+  sweep_<i>_tight        design i of the 1,024-design sweep, tight
 """
+import json
 import sys
 import time
 from pathlib import Path
@@ -32,25 +48,52 @@ import py_oracle as po  # noqa: E402
 OUT = Path(__file__).resolve().parent
 
 
-def tight(cfg):
-    return cfg.replace(line_search=1, gradient_rms_tolerance=1e-7, gradient_max_tolerance=1e-7, relative_escape=0)
-
-
-CURVES = {
-    "cubic_8x8x4_tight": lambda: tight(mmc.cubic_config(8, 8, 4)),
-    "cubic_32x32x16": lambda: mmc.cubic_config(32, 32, 16),
-    "cubic_32x32x16_tight": lambda: tight(mmc.cubic_config(32, 32, 16)),
-}
+def curve_config(name: str) -> mmc.Config:
+    fixed = {
+        "cubic_8x8x4_tight": lambda: mmc.tight(mmc.cubic_config(8, 8, 4)),
+        "cubic_32x32x16": lambda: mmc.cubic_config(32, 32, 16, max_iterations=20000),
+        "cubic_32x32x16_tight": lambda: mmc.tight(mmc.cubic_config(32, 32, 16, max_iterations=20000)),
+        "large_16x16x8_tight": lambda: mmc.tight(mmc.large_config(16, 16, 8), tol=3e-8),
+        "bcc_222_tight": lambda: mmc.tight(mmc.specimen_config().replace(max_iterations=20000)),
+    }
+    if name in fixed:
+        return fixed[name]()
+    kind, idx, *rest = name.split("_")
+    cfg = mmc.batch_workload(kind, 0, indices=[int(idx)])[0]
+    return mmc.tight(cfg) if rest == ["tight"] else cfg
 
 
 def main():
     name = sys.argv[1]
-    cfg = CURVES[name]()
+    limit = int(sys.argv[2]) if len(sys.argv) > 2 else None
+    cfg = curve_config(name)
     t0 = time.time()
-    frames = po.Solver(cfg, "orc").run()
-    np.save(OUT / f"{name}_oracle_frames.npy", frames)
-    print(name, "frames", len(frames), "peak", frames["reaction_force"].max(),
-          "iterations", int(frames["iterations"].sum()), f"{time.time() - t0:.0f} s")
+    status_path = OUT / f"{name}_oracle_status.json"
+    frames_path = OUT / f"{name}_oracle_frames.npy"
+
+    def save(frames, error=None, done=False):
+        np.save(frames_path, frames, allow_pickle=False)
+        status_path.write_text(json.dumps({
+            "curve": name, "frames": int(len(frames)), "complete": done, "error": error,
+            "frame_limit": limit, "seconds": round(time.time() - t0, 1)}, indent=1) + "\n")
+
+    last = []
+
+    def on_frame(frames):
+        last[:] = [frames]
+        save(frames)
+        print(f"{name}: frame {len(frames)} strain {frames['engineering_strain'][-1]:.4f} "
+              f"force {frames['reaction_force'][-1]:.6g} N, {int(frames['iterations'][-1])} iterations, "
+              f"{time.time() - t0:.0f} s", flush=True)
+
+    try:
+        frames = po.Solver(cfg, "orc").run(on_frame=on_frame, limit=limit)
+        save(frames, done=True)
+    except RuntimeError as e:
+        frames = last[0] if last else np.zeros(0, dtype=mmc.FRAME_DTYPE)
+        save(frames, error=str(e), done=True)
+    print(name, "frames", len(frames), "iterations", int(frames["iterations"].sum()),
+          f"{time.time() - t0:.0f} s")
 
 
 if __name__ == "__main__":

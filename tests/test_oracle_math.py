@@ -285,3 +285,35 @@ def test_per_term_energy_change_is_the_energy_difference(po):
     lin = h * float(g0 @ dz)
     assert abs(per_h - lin) <= 2e-4 * abs(lin)  # measured 3.4e-5
     assert abs(per_h - lin) * 5 < abs(tot_h - lin)  # total difference: 7e-4
+
+
+def test_approximate_wolfe_line_search_reaches_the_same_equilibrium(mmc, po):
+    """line_search 2 completes the per-term Armijo test with the approximate-
+    Wolfe slope test of Hager & Zhang below 1e-6 |E|.  Pinned against the
+    per-term test alone on increments both converge (the equilibrium is the
+    same: platen force to 1e-7 relative, energy to 1e-10), and by what it adds:
+    at a gradient tolerance of 3e-9, below the per-term test's resolution on
+    this lattice, line_search 2 still converges while line_search 1 stops at
+    a line-search failure."""
+    base = mmc.cubic_config(4, 4, 3)
+    top = None
+    res = {}
+    for ls in (1, 2):
+        cfg = base.replace(line_search=ls, gradient_rms_tolerance=1e-7, gradient_max_tolerance=1e-7,
+                           relative_escape=0)
+        o = po.Solver(cfg, "orc")
+        top = o.top0 - 0.05 * o.height
+        st, fin, _ = o.minimize(top)
+        assert st.converged
+        res[ls] = fin
+    assert abs(res[2].reaction - res[1].reaction) <= 1e-7 * res[1].reaction
+    assert abs(res[2].energy - res[1].energy) <= 1e-10 * abs(res[1].energy)
+    deep = {}
+    for ls in (1, 2):
+        cfg = base.replace(line_search=ls, gradient_rms_tolerance=3e-9, gradient_max_tolerance=3e-9,
+                           relative_escape=0, max_iterations=20000)
+        o = po.Solver(cfg, "orc")
+        st, fin, _ = o.minimize(top)
+        deep[ls] = st
+    assert deep[2].converged and deep[2].max <= 3e-9
+    assert not deep[1].converged
