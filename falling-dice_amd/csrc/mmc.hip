@@ -26,6 +26,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -2699,6 +2700,35 @@ static_assert(4 * kMaxHist + 5 <= kScal && 128 + 4 * kMaxHist + 5 <= kHost, "sca
 // A context holds n_inst independent lattices (one for the single-lattice
 // API) in one concatenated SoA with per-instance constants, block tables and
 // control blocks; every launch covers all of them.
+// ---------------------------------------------------------------------------
+// Static scratch of the allocation-free entry points (mmc_beam_potential_noalloc):
+// the block partials of the beam-potential reduction, one double per 256-beam
+// block, so up to kStaticScratchDoubles blocks (16.7 M beams) per call.  Each
+// GPU holds its own copy for the life of the process.  Host code never uses the
+// array's name as a device address (it is a host-side stand-in): the address of
+// the device copy is taken once per device with hipGetSymbolAddress, while that
+// device is current, and kept in g_scratch_ptr.  Calls that overlap on one GPU
+// share the array and race; a call overwrites every partial it later reads, so
+// nothing needs clearing between calls.
+// ---------------------------------------------------------------------------
+constexpr int kStaticScratchDoubles = 1 << 16;  // 512 KiB
+static __device__ __align__(256) double scratch[kStaticScratchDoubles];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// Takes the device copy's address for `device` (current) once; throws on failure.
+void static_scratch_init(int device) {
+  if (device < 0 || device >= kMaxDevices) throw Fail(MMC_ERR_CAPACITY, "device ordinal beyond 63");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[device]) return;
+  void* ptr = nullptr;
+  hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr)
+    throw HipFail(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+  g_scratch_ptr[device] = static_cast<double*>(ptr);
+}
+
 struct mmc_ctx {
   std::string err;
   int precision = MMC_FP64;
@@ -2708,6 +2738,7 @@ struct mmc_ctx {
   std::vector<Inst> hinst;
   mmc_config cfg{};  // instance 0 (single-lattice API)
   hipStream_t stream = nullptr;
+  int device = 0;  // the device current at creation; every launch goes there
   // device buffers
   double *bpos = nullptr, *bquat = nullptr, *mass = nullptr;
   uint8_t* lock = nullptr;
@@ -2975,6 +3006,8 @@ void mmc_ctx::setup(const mmc_lattice* const* Ls, const mmc_config* cs, int n, i
   N = int(tn_);
   B = int(tb_);
   NS = N + B;
+  HIPCALL(hipGetDevice(&device));
+  static_scratch_init(device);
   HIPCALL(hipStreamCreate(&stream));
   // ---- instances: ranges and constants ----
   hinst.assign(n, Inst{});
@@ -4551,6 +4584,28 @@ int mmc_beam_potential(mmc_ctx* c, const double* xh, const double* qh, double* o
     k_sum1<<<1, kBlock, 0, c->stream>>>(c->bpart, c->Pb, c->scal);
     c->read(c->host + 48, c->scal, sizeof(double));
     *out = c->host[48];
+  });
+}
+
+int mmc_beam_potential_noalloc(mmc_ctx* c, const double* x, const double* q, double* energy, void* stream) {
+  return guarded(c, [&] {
+    c->need_single("mmc_beam_potential_noalloc");
+    if (!x || !q || !energy) throw Fail(MMC_ERR_ARG, "NULL argument");
+    if (c->Pb > kStaticScratchDoubles)
+      throw Fail(MMC_ERR_CAPACITY, "mmc_beam_potential_noalloc: " + std::to_string(c->B) +
+                                       " beams need " + std::to_string(c->Pb) + " partials; the static scratch holds " +
+                                       std::to_string(kStaticScratchDoubles) + " (16,777,216 beams)");
+    int dev = -1;
+    HIPCALL(hipGetDevice(&dev));
+    if (dev != c->device)
+      throw Fail(MMC_ERR_ARG, "mmc_beam_potential_noalloc: device " + std::to_string(dev) +
+                                  " is current, the context lives on device " + std::to_string(c->device));
+    double* part = g_scratch_ptr[c->device];  // taken at mmc_create on this device
+    if (!part) throw Fail(MMC_ERR_STATE, "static scratch not initialised on this device");
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c->stream;
+    k_beam_potential<double><<<c->Pb, kBlock, 0, st>>>(c->lat64(), c->inst_d, c->tb, x, q, part);
+    k_sum1<<<1, kBlock, 0, st>>>(part, c->Pb, energy);
+    HIPCALL(hipGetLastError());
   });
 }
 

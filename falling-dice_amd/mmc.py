@@ -279,6 +279,7 @@ def lib():
         L.mmc_read_trial.argtypes = [P, dp]
         L.mmc_read_stats.argtypes = [P, C.POINTER(StateStats)]
         L.mmc_beam_potential.argtypes = [P, dp, dp, dp]
+        L.mmc_beam_potential_noalloc.argtypes = [P, P, P, P, P]
         L.mmc_config_defaults.argtypes = [C.POINTER(Config)]
         L.mmc_config_defaults.restype = None
         L.mmc_sim_create.argtypes = [C.POINTER(Config), C.c_int32, C.POINTER(P)]

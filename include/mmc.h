@@ -222,6 +222,19 @@ int mmc_loop_shape(mmc_ctx* ctx, int32_t* decide_blocks, double* decide_us);
  * against the committed plastic state.  x is 3N, q is 4N (w,x,y,z). */
 int mmc_beam_potential(mmc_ctx* ctx, const double* x, const double* q, double* out);
 
+/* The same beam potential for callers that cannot allocate device memory (code
+ * under HIP graph capture, serving loops, harnesses that forbid hipMalloc): no
+ * counterpart in the reference.  x (3N) and q (4N) are DEVICE arrays in planes
+ * (x[d*N + i]; q[0..3][N] = w, x, y, z) and energy is one DEVICE double; the
+ * two kernels are queued on `stream` (a hipStream_t; NULL: the context's own
+ * stream) and the call returns without waiting.  The reduction's block
+ * partials live in a static device array of 65,536 doubles (512 KiB per GPU,
+ * held until the process ends), so contexts of up to 16,777,216 beams;
+ * larger ones get MMC_ERR_CAPACITY.  No hipMalloc/hipFree or other allocating
+ * call is made.  Calls that overlap on one GPU share that array and race.
+ * The device current at mmc_create must be current. */
+int mmc_beam_potential_noalloc(mmc_ctx* ctx, const double* x, const double* q, double* energy, void* stream);
+
 /* ==== batched contexts (no reference counterpart) ===========================
  * Many independent lattices in one context: one concatenated SoA, every launch
  * covering all of them, each instance with its own control block, platen,
