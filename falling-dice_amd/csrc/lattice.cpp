@@ -293,7 +293,7 @@ mmc_lattice HostLattice::view() const {
   v.n_nodes = n_nodes;
   v.n_beams = n_beams;
   v.x = x.data();
-  v.q = nullptr;
+  v.q = q.empty() ? nullptr : q.data();
   v.mass = mass.data();
   v.locked = locked.data();
   v.beam_a = a.data();
@@ -303,7 +303,7 @@ mmc_lattice HostLattice::view() const {
   v.area = area.data();
   v.inertia = inertia.data();
   v.e0 = e0.data();
-  v.plastic = nullptr;
+  v.plastic = plastic.empty() ? nullptr : plastic.data();
   v.n_connected = int64_t(connected.size());
   v.connected = connected.data();
   v.node_layer = node_layer.data();

@@ -26,8 +26,13 @@ struct HostLattice {
   std::vector<double> L0, radius, area, inertia, e0;  // e0: 3B
   std::vector<uint64_t> connected;  // node pairs excluded from self-contact (insertion order)
   double width = 0, height = 0, depth = 0, area_ref = 0, total_mass = 0, top0 = 0, joint_radius = 0;
+  // committed state after construction (all empty for a new lattice): base
+  // orientations (4N, w,x,y,z; empty = identity), plastic states (7B; empty =
+  // zero) and the initial positions when x holds a committed base pose (3N;
+  // empty = x)
+  std::vector<double> q, plastic, x0;
 
-  // A view for mmc_create; identity orientations and zero plastic state.
+  // A view for mmc_create.
   mmc_lattice view() const;
 };
 

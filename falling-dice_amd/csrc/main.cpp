@@ -169,7 +169,9 @@ struct Args {
   std::filesystem::path output = "output", cache;
   int threads = 0;
   bool quick = false, noUnload = false, skipStl = false;
+  bool strictValidation = false;  // exit 1 when VALIDATION.txt holds a FAIL (additive)
   SimulationConfig sim;
+  SimulationExtensions ext;  // the additive options
 };
 
 void usage() {
@@ -177,8 +179,9 @@ void usage() {
                "gfx950 build: sim, validate, convergence (all = sim + validate; render/hero/stl not built)\n"
                "additive options: --lattice bcc|cubic --cell-size M --min-radius M --max-radius M --nu V\n"
                "  --density V --seed N --history N --grad-tol V --grad-rms-tol V --grad-max-tol V\n"
-               "  --no-relative-escape --position-sigma M --radius-mode 0|1 --line-search 0|1 --imperfection M\n"
-               "  --member-imperfection M --joint-scale V --precision fp64|mixed\n";
+               "  --no-relative-escape --position-sigma M --radius-mode 0|1 --line-search 0|1|2 --imperfection M\n"
+               "  --member-imperfection M --joint-scale V --precision fp64|mixed\n"
+               "  --strict-validation (exit 1 when VALIDATION.txt holds a FAIL)\n";
 }
 
 // MC/src/main.cpp:165-181 plus the additive options
@@ -213,11 +216,12 @@ Args parse_args(int argc, char** argv) {
     else if (o == "--no-unload") a.noUnload = true;
     else if (o == "--no-denoise") {
     } else if (o == "--skip-stl") a.skipStl = true;
+    else if (o == "--strict-validation") a.strictValidation = true;
     // ---- additive ----
     else if (o == "--lattice") {
       std::string v = value();
-      if (v == "bcc") a.sim.lattice = MMC_LATTICE_BCC;
-      else if (v == "cubic") a.sim.lattice = MMC_LATTICE_CUBIC;
+      if (v == "bcc") a.ext.lattice = MMC_LATTICE_BCC;
+      else if (v == "cubic") a.ext.lattice = MMC_LATTICE_CUBIC;
       else throw std::runtime_error("unknown lattice " + v);
     } else if (o == "--cell-size") a.sim.cellSize = std::stod(value());
     else if (o == "--min-radius") a.sim.minRadius = std::stod(value());
@@ -229,17 +233,17 @@ Args parse_args(int argc, char** argv) {
     else if (o == "--grad-tol") a.sim.gradientRmsTolerance = a.sim.gradientMaxTolerance = std::stod(value());
     else if (o == "--grad-rms-tol") a.sim.gradientRmsTolerance = std::stod(value());
     else if (o == "--grad-max-tol") a.sim.gradientMaxTolerance = std::stod(value());
-    else if (o == "--no-relative-escape") a.sim.relativeEscape = false;
-    else if (o == "--position-sigma") a.sim.positionSigma = std::stod(value());
-    else if (o == "--radius-mode") a.sim.radiusMode = std::stoi(value());
-    else if (o == "--line-search") a.sim.lineSearch = std::stoi(value());
+    else if (o == "--no-relative-escape") a.ext.relativeEscape = false;
+    else if (o == "--position-sigma") a.ext.positionSigma = std::stod(value());
+    else if (o == "--radius-mode") a.ext.radiusMode = std::stoi(value());
+    else if (o == "--line-search") a.ext.lineSearch = std::stoi(value());
     else if (o == "--imperfection") a.sim.imperfection = std::stod(value());
     else if (o == "--member-imperfection") a.sim.memberImperfection = std::stod(value());
     else if (o == "--joint-scale") a.sim.jointScale = std::stod(value());
     else if (o == "--precision") {
       std::string v = value();
-      if (v == "fp64") a.sim.precision = MMC_FP64;
-      else if (v == "mixed") a.sim.precision = MMC_MIXED;
+      if (v == "fp64") a.ext.precision = MMC_FP64;
+      else if (v == "mixed") a.ext.precision = MMC_MIXED;
       else throw std::runtime_error("unknown precision " + v);
     } else if (o == "--help") {
       usage();
@@ -283,7 +287,9 @@ bool finite_result(const SimulationResult& r) {
 }
 
 // MC/src/main.cpp:200-208
-void write_validation(const std::filesystem::path& path, const Simulator& sim, const SimulationResult* result) {
+// Returns false when any check of the report is FAIL (the reference's exit
+// code ignores that; --strict-validation turns it into exit code 1).
+bool write_validation(const std::filesystem::path& path, const Simulator& sim, const SimulationResult* result) {
   double objectivity = std::numeric_limits<double>::infinity(), gradient = objectivity;
   bool obj = sim.rigidRotationTest(&objectivity), grad = sim.gradientCheck(&gradient);
   std::ofstream out(path);
@@ -315,7 +321,9 @@ void write_validation(const std::filesystem::path& path, const Simulator& sim, c
         << "max_contact_penetration_mm=" << s.maxPen * 1000 << " " << (penetration ? "PASS" : "WARN") << '\n'
         << "energy_accounting=" << (energy ? "PASS" : "FAIL") << '\n'
         << "overall_numerical_validation=" << ((obj && grad && finite && energy) ? "PASS" : "FAIL") << '\n';
+    return obj && grad && finite && energy;
   }
+  return obj && grad;
 }
 
 // MC/src/main.cpp:210-228
@@ -341,6 +349,7 @@ void run_convergence(const Args& args) {
     c.contactStiffness = 1.0e6;
     c.frames = c.loadSteps + c.unloadSteps + 1;
     std::cout << "\n[convergence] running " << k.name << "\n";
+    setNextSimulationExtensions(args.ext);
     Simulator sim(c);
     SimulationResult r = sim.run();
     Summary s = summarize(r);
@@ -368,16 +377,20 @@ int main(int argc, char** argv) {
     SimulationResult result;
     std::unique_ptr<Simulator> sim;
     if (args.mode == "sim" || args.mode == "all") {
+      setNextSimulationExtensions(args.ext);
       sim = std::make_unique<Simulator>(args.sim);
       result = sim->run();
       save_cache(args.cache, result, args.sim.fps);
       write_metrics(args.output / "compression_metrics.csv", result);
       write_summary(args.output / "summary.json", result, args.sim, sim->mass());
       if (!args.skipStl) std::cerr << "note: STL export is not part of this build (as with --skip-stl)\n";
-      write_validation(args.output / "VALIDATION.txt", *sim, &result);
+      if (!write_validation(args.output / "VALIDATION.txt", *sim, &result) && args.strictValidation)
+        throw std::runtime_error("validation failed (VALIDATION.txt)");
     } else if (args.mode == "validate") {
+      setNextSimulationExtensions(args.ext);
       sim = std::make_unique<Simulator>(args.sim);
-      write_validation(args.output / "VALIDATION.txt", *sim, nullptr);
+      const bool ok = write_validation(args.output / "VALIDATION.txt", *sim, nullptr);
+      if (!ok && args.strictValidation) throw std::runtime_error("validation failed (VALIDATION.txt)");
       return 0;
     } else {
       throw std::runtime_error("unknown mode " + args.mode);

@@ -12,8 +12,10 @@
 #include <iomanip>
 #include <iostream>
 #include <limits>
+#include <mutex>
 #include <sstream>
 #include <stdexcept>
+#include <unordered_map>
 
 namespace meta {
 namespace {
@@ -97,7 +99,7 @@ uint64_t splitmix(uint64_t v) {
 
 }  // namespace
 
-mmc_config to_c(const SimulationConfig& s) {
+mmc_config to_c(const SimulationConfig& s, const SimulationExtensions& e) {
   mmc_config c;
   mmc_config_defaults(&c);
   c.cells_x = s.cellsX;
@@ -139,16 +141,16 @@ mmc_config to_c(const SimulationConfig& s) {
   c.min_line_step = s.minLineStep;
   c.penetration_tolerance = s.penetrationTolerance;
   c.seed = s.seed;
-  c.lattice = s.lattice;
-  c.relative_escape = s.relativeEscape ? 1 : 0;
-  c.relative_tolerance = s.relativeTolerance;
-  c.position_sigma = s.positionSigma;
-  c.radius_mode = s.radiusMode;
-  c.line_search = s.lineSearch;
+  c.lattice = e.lattice;
+  c.relative_escape = e.relativeEscape ? 1 : 0;
+  c.relative_tolerance = e.relativeTolerance;
+  c.position_sigma = e.positionSigma;
+  c.radius_mode = e.radiusMode;
+  c.line_search = e.lineSearch;
   return c;
 }
 
-SimulationConfig from_c(const mmc_config& c) {
+void from_c(const mmc_config& c, SimulationConfig* out, SimulationExtensions* ext) {
   SimulationConfig s;
   s.cellsX = c.cells_x;
   s.cellsY = c.cells_y;
@@ -194,37 +196,201 @@ SimulationConfig from_c(const mmc_config& c) {
   s.minLineStep = c.min_line_step;
   s.penetrationTolerance = c.penetration_tolerance;
   s.seed = c.seed;
-  s.lattice = c.lattice;
-  s.relativeEscape = c.relative_escape != 0;
-  s.relativeTolerance = c.relative_tolerance;
-  s.positionSigma = c.position_sigma;
-  s.radiusMode = c.radius_mode;
-  s.lineSearch = c.line_search;
-  return s;
+  if (out) *out = s;
+  if (ext) {
+    ext->lattice = c.lattice;
+    ext->relativeEscape = c.relative_escape != 0;
+    ext->relativeTolerance = c.relative_tolerance;
+    ext->positionSigma = c.position_sigma;
+    ext->radiusMode = c.radius_mode;
+    ext->lineSearch = c.line_search;
+  }
 }
 
-// MC/src/sim.cpp:49
+// ---------------------------------------------------------------------------
+// Extensions handed to the next Simulator of a thread, and per instance
+// ---------------------------------------------------------------------------
+namespace {
+thread_local std::unique_ptr<SimulationExtensions> t_next_ext;
+std::mutex g_ext_mu;
+std::unordered_map<const Simulator*, SimulationExtensions> g_ext;
+
+SimulationExtensions take_next_extensions() {
+  SimulationExtensions e;
+  if (t_next_ext) {
+    e = *t_next_ext;
+    t_next_ext.reset();
+  }
+  return e;
+}
+}  // namespace
+
+void setNextSimulationExtensions(const SimulationExtensions& ext) {
+  t_next_ext = std::make_unique<SimulationExtensions>(ext);
+}
+
+SimulationExtensions simulationExtensions(const Simulator& sim) {
+  std::lock_guard<std::mutex> lock(g_ext_mu);
+  auto it = g_ext.find(&sim);
+  return it == g_ext.end() ? SimulationExtensions{} : it->second;
+}
+
+// ---------------------------------------------------------------------------
+// Simulator: the reference's members, the hot path through a Driver
+// ---------------------------------------------------------------------------
+// MC/src/sim.cpp:49: the lattice is built on the host (build, distributeMass)
+// into the reference's members; the device context exists while run() or a
+// check needs it.
 Simulator::Simulator(SimulationConfig cfg) : cfg_(std::move(cfg)) {
+  const SimulationExtensions ext = take_next_extensions();
   cfg_.translationScale = cfg_.cellSize;
-  c_ = to_c(cfg_);
-  lat_ = build_lattice(c_);
-  mmc_lattice v = lat_.view();
-  int rc = mmc_create(&v, &c_, cfg_.precision, &ctx_);
-  if (rc != MMC_OK) throw std::runtime_error(mmc_last_error(nullptr));
+  const HostLattice L = build_lattice(to_c(cfg_, ext));
+  {
+    std::lock_guard<std::mutex> lock(g_ext_mu);
+    g_ext[this] = ext;
+  }
+  const size_t N = size_t(L.n_nodes), B = size_t(L.n_beams);
+  nodes_.resize(N);
+  baseX_.resize(N);
+  baseR_.assign(N, Quat{});
+  for (size_t i = 0; i < N; i++) {
+    Node& n = nodes_[i];
+    n.x0 = n.x = baseX_[i] = Vec3{L.x[3 * i], L.x[3 * i + 1], L.x[3 * i + 2]};
+    n.R = Quat{};
+    n.mass = L.mass[i];
+    n.layer = L.node_layer[i];
+  }
+  beams_.resize(B);
+  for (size_t k = 0; k < B; k++) {
+    Beam& e = beams_[k];
+    e.a = L.a[k];
+    e.b = L.b[k];
+    e.layer = L.layer[k];
+    e.family = L.family[k];
+    e.L0 = L.L0[k];
+    e.radius = L.radius[k];
+    e.area = L.area[k];
+    e.I = L.inertia[k];
+    e.J = 2 * e.I;
+    e.e0 = Vec3{L.e0[3 * k], L.e0[3 * k + 1], L.e0[3 * k + 2]};
+  }
+  connected_.insert(L.connected.begin(), L.connected.end());
+  locked_ = L.locked;
+  width_ = L.width;
+  height_ = L.height;
+  depth_ = L.depth;
+  area_ = L.area_ref;
+  mass_ = L.total_mass;
+  top0_ = L.top0;
 }
 
-Simulator::~Simulator() { mmc_destroy(ctx_); }
+// The members as the device takes them: base pose and plastic state as
+// committed, the initial positions for the objectivity test, the exclusion
+// set in the construction's insertion order.
+HostLattice Simulator::committedLattice() const {
+  const SimulationExtensions ext = simulationExtensions(*this);
+  HostLattice L = build_lattice(to_c(cfg_, ext));  // connectivity, masses, locks, envelope
+  const size_t N = nodes_.size(), B = beams_.size();
+  if (size_t(L.n_nodes) != N || size_t(L.n_beams) != B) throw std::runtime_error("lattice size mismatch");
+  L.x0 = L.x;
+  L.q.resize(4 * N);
+  for (size_t i = 0; i < N; i++) {
+    const Vec3& x = baseX_[i];
+    const Quat& q = baseR_[i];
+    L.x[3 * i] = x.x, L.x[3 * i + 1] = x.y, L.x[3 * i + 2] = x.z;
+    L.q[4 * i] = q.w, L.q[4 * i + 1] = q.x, L.q[4 * i + 2] = q.y, L.q[4 * i + 3] = q.z;
+  }
+  L.plastic.resize(7 * B);
+  for (size_t k = 0; k < B; k++) {
+    const PlasticState& p = beams_[k].plastic;
+    double* o = &L.plastic[7 * k];
+    o[0] = p.epsp, o[1] = p.alpha, o[2] = p.rotp.x, o[3] = p.rotp.y, o[4] = p.rotp.z, o[5] = p.alphaB,
+    o[6] = p.alphaT;
+  }
+  return L;
+}
 
-void Simulator::check(int rc) const {
-  if (rc != MMC_OK) throw std::runtime_error(mmc_last_error(ctx_));
+// MC/src/sim.cpp:374-467 on the device; the committed state of every accepted
+// increment is copied back into the members (also when an equilibrium failure
+// ends the schedule, as the reference's own commits leave them).
+SimulationResult Simulator::run() {
+  const SimulationExtensions ext = simulationExtensions(*this);
+  Driver d(to_c(cfg_, ext), ext.precision, committedLattice());
+  d.setVerbose(ext.verbose);
+  d.setCaptureFrames(ext.captureFrames);
+  auto copy_back = [&] {
+    const std::vector<Node>& n = d.nodes();
+    const std::vector<Beam>& b = d.beams();
+    for (size_t i = 0; i < nodes_.size(); i++) {
+      nodes_[i].x = baseX_[i] = n[i].x;
+      nodes_[i].R = baseR_[i] = n[i].R;
+    }
+    for (size_t k = 0; k < beams_.size(); k++) beams_[k].plastic = b[k].plastic;
+    loadingWork_ = d.loadingWork();
+    recoveredWork_ = d.recoveredWork();
+    prevDisp_ = d.prevDisp();
+    prevForce_ = d.prevForce();
+    reactionOffset_ = d.reactionOffset();
+  };
+  SimulationResult r;
+  try {
+    r = d.run();
+  } catch (...) {
+    copy_back();
+    throw;
+  }
+  copy_back();
+  return r;
 }
 
 void Simulator::exportSTL(const std::filesystem::path&, int) const {
   throw std::runtime_error("STL export is not part of the gfx950 equilibrium solver");
 }
 
+// MC/src/sim.cpp:511-536 and :538-577, on the committed state, through a
+// device context made for the check.
+bool Simulator::rigidRotationTest(double* error) const {
+  const SimulationExtensions ext = simulationExtensions(*this);
+  Driver d(to_c(cfg_, ext), ext.precision, committedLattice());
+  return d.rigidRotationTest(error);
+}
+
+bool Simulator::gradientCheck(double* relativeError) const {
+  const SimulationExtensions ext = simulationExtensions(*this);
+  Driver d(to_c(cfg_, ext), ext.precision, committedLattice());
+  return d.gradientCheck(relativeError);
+}
+
+// ---------------------------------------------------------------------------
+// Driver
+// ---------------------------------------------------------------------------
+Driver::Driver(const mmc_config& c, int precision) : c_(c) {
+  c_.translation_scale = c_.cell_size;  // MC/src/sim.cpp:49
+  lat_ = build_lattice(c_);
+  create(precision);
+}
+
+Driver::Driver(const mmc_config& c, int precision, HostLattice lat) : c_(c), lat_(std::move(lat)) {
+  c_.translation_scale = c_.cell_size;
+  create(precision);
+}
+
+void Driver::create(int precision) {
+  from_c(c_, &cfg_, &ext_);
+  ext_.precision = precision;
+  mmc_lattice v = lat_.view();
+  int rc = mmc_create(&v, &c_, precision, &ctx_);
+  if (rc != MMC_OK) throw std::runtime_error(mmc_last_error(nullptr));
+}
+
+Driver::~Driver() { mmc_destroy(ctx_); }
+
+void Driver::check(int rc) const {
+  if (rc != MMC_OK) throw std::runtime_error(mmc_last_error(ctx_));
+}
+
 // Host mirror of the committed lattice state (read from the device on demand).
-void Simulator::sync() const {
+void Driver::sync() const {
   if (synced_) return;
   const int N = lat_.n_nodes, B = lat_.n_beams;
   std::vector<double> x(3 * size_t(N)), q(4 * size_t(N)), p(7 * size_t(B));
@@ -233,7 +399,8 @@ void Simulator::sync() const {
   beams_.resize(size_t(B));
   for (int i = 0; i < N; i++) {
     Node& n = nodes_[size_t(i)];
-    n.x0 = Vec3{lat_.x[3 * size_t(i)], lat_.x[3 * size_t(i) + 1], lat_.x[3 * size_t(i) + 2]};
+    const std::vector<double>& x0 = lat_.x0.empty() ? lat_.x : lat_.x0;
+    n.x0 = Vec3{x0[3 * size_t(i)], x0[3 * size_t(i) + 1], x0[3 * size_t(i) + 2]};
     n.x = Vec3{x[3 * size_t(i)], x[3 * size_t(i) + 1], x[3 * size_t(i) + 2]};
     n.R = Quat{q[4 * size_t(i)], q[4 * size_t(i) + 1], q[4 * size_t(i) + 2], q[4 * size_t(i) + 3]};
     n.mass = lat_.mass[size_t(i)];
@@ -258,24 +425,24 @@ void Simulator::sync() const {
   synced_ = true;
 }
 
-const std::vector<Node>& Simulator::nodes() const {
+const std::vector<Node>& Driver::nodes() const {
   sync();
   return nodes_;
 }
-const std::vector<Beam>& Simulator::beams() const {
+const std::vector<Beam>& Driver::beams() const {
   sync();
   return beams_;
 }
 
 // MC/src/sim.cpp:213
-double Simulator::topY(double s, bool loading) const {
+double Driver::topY(double s, bool loading) const {
   double low = lat_.top0 - cfg_.maxStrain * lat_.height;
   double t = std::max(0.0, std::min(1.0, s));
   return loading ? lat_.top0 * (1.0 - t) + low * t : low * (1.0 - t) + lat_.top0 * t;
 }
 
 // MC/src/sim.cpp:215-310
-std::vector<Snapshot> Simulator::capture(double top, const std::vector<Vec3>* prevX,
+std::vector<Snapshot> Driver::capture(double top, const std::vector<Vec3>* prevX,
                                          const std::vector<Quat>* prevR) const {
   sync();
   std::vector<Snapshot> out;
@@ -396,7 +563,7 @@ FrameDiagnostics make_diagnostics(const HostLattice& lat, double time, double to
 }
 }  // namespace
 
-FrameDiagnostics Simulator::diagnostics(double time, double top, bool loading, const mmc_objective_out& o,
+FrameDiagnostics Driver::diagnostics(double time, double top, bool loading, const mmc_objective_out& o,
                                         const mmc_opt_stats& s) const {
   mmc_state_stats st{};
   check(mmc_read_stats(ctx_, &st));
@@ -404,7 +571,7 @@ FrameDiagnostics Simulator::diagnostics(double time, double top, bool loading, c
 }
 
 // MC/src/sim.cpp:379-388
-void Simulator::begin() {
+void Driver::begin() {
   loadingWork_ = recoveredWork_ = prevDisp_ = prevForce_ = reactionOffset_ = 0.0;
   time_ = 0.0;
   currentTop_ = lat_.top0;
@@ -414,10 +581,10 @@ void Simulator::begin() {
 }
 
 // One equilibrium increment with adaptive bisection, MC/src/sim.cpp:390-454.
-void Simulator::advanceTo(double targetTop, bool loading, int refinement, SimulationResult& result) {
+void Driver::advanceTo(double targetTop, bool loading, int refinement, SimulationResult& result) {
   std::vector<Vec3> previousX;
   std::vector<Quat> previousR;
-  if (cfg_.captureFrames) {
+  if (ext_.captureFrames) {
     sync();
     previousX.resize(nodes_.size());
     previousR.resize(nodes_.size());
@@ -458,11 +625,11 @@ void Simulator::advanceTo(double targetTop, bool loading, int refinement, Simula
   prevDisp_ = displacement;
   prevForce_ = correctedForce;
   currentTop_ = targetTop;
-  if (cfg_.captureFrames)
+  if (ext_.captureFrames)
     result.frames.push_back(capture(targetTop, first_ ? nullptr : &previousX, first_ ? nullptr : &previousR));
   result.diagnostics.push_back(diagnostics(time_, targetTop, loading, fin, stats));
   const FrameDiagnostics& d = result.diagnostics.back();
-  if (cfg_.verbose)
+  if (ext_.verbose)
     std::cout << "[equilibrium] " << std::setw(3) << acceptedSteps_ << " " << (loading ? "load  " : "unload")
               << " strain=" << std::fixed << std::setprecision(4) << d.engineeringStrain
               << " force=" << std::setprecision(3) << d.reactionForce << " N"
@@ -476,7 +643,7 @@ void Simulator::advanceTo(double targetTop, bool loading, int refinement, Simula
 
 // Next scheduled increment, MC/src/sim.cpp:456-465.  Returns false once the
 // schedule is complete.
-bool Simulator::advance(SimulationResult& result) {
+bool Driver::advance(SimulationResult& result) {
   const int L = cfg_.loadSteps, U = cfg_.unloadSteps;
   if (cursor_ > L + U) return false;
   const int k = cursor_++;
@@ -490,7 +657,7 @@ bool Simulator::advance(SimulationResult& result) {
 }
 
 // MC/src/sim.cpp:374-467
-SimulationResult Simulator::run() {
+SimulationResult Driver::run() {
   SimulationResult result;
   result.frames.reserve(size_t(cfg_.loadSteps + cfg_.unloadSteps + 24));
   result.diagnostics.reserve(size_t(cfg_.loadSteps + cfg_.unloadSteps + 24));
@@ -501,13 +668,14 @@ SimulationResult Simulator::run() {
 }
 
 // MC/src/sim.cpp:511-536; the potential is evaluated on the device.
-bool Simulator::rigidRotationTest(double* error) const {
+bool Driver::rigidRotationTest(double* error) const {
   const int N = lat_.n_nodes;
   const double jr = cfg_.jointScale * cfg_.maxRadius, height = lat_.height;
   std::vector<Vec3> x(static_cast<size_t>(N));
   std::vector<Quat> R(static_cast<size_t>(N));
   for (int i = 0; i < N; ++i) {
-    Vec3 x0{lat_.x[3 * size_t(i)], lat_.x[3 * size_t(i) + 1], lat_.x[3 * size_t(i) + 2]};
+    const std::vector<double>& xi = lat_.x0.empty() ? lat_.x : lat_.x0;  // Node::x0
+    Vec3 x0{xi[3 * size_t(i)], xi[3 * size_t(i) + 1], xi[3 * size_t(i) + 2]};
     double yf = std::max(0.0, std::min(1.0, (x0.y - jr) / std::max(1e-12, height)));
     x[size_t(i)] = x0;
     x[size_t(i)].y -= 0.018 * height * yf;
@@ -545,7 +713,7 @@ bool Simulator::rigidRotationTest(double* error) const {
 }
 
 // MC/src/sim.cpp:538-577; every objective evaluation runs on the device.
-bool Simulator::gradientCheck(double* relativeError) const {
+bool Driver::gradientCheck(double* relativeError) const {
   const int N = lat_.n_nodes;
   std::vector<double> z(6 * size_t(N), 0.0);
   // the reference RNG stream (MC/include/core.hpp:46) seeded with seed ^ "GRADCHEK"
@@ -595,15 +763,15 @@ bool Simulator::gradientCheck(double* relativeError) const {
 // ---------------------------------------------------------------------------
 // BatchSimulator
 // ---------------------------------------------------------------------------
-BatchSimulator::BatchSimulator(std::vector<SimulationConfig> cfgs, int precision) : cfgs_(std::move(cfgs)) {
-  if (cfgs_.empty()) throw std::runtime_error("a batch needs at least one lattice");
-  const size_t n = cfgs_.size();
-  c_.resize(n);
+BatchSimulator::BatchSimulator(std::vector<mmc_config> cfgs, int precision) : c_(std::move(cfgs)) {
+  if (c_.empty()) throw std::runtime_error("a batch needs at least one lattice");
+  const size_t n = c_.size();
+  cfgs_.resize(n);
   lats_.resize(n);
   std::vector<mmc_lattice> views(n);
   for (size_t b = 0; b < n; b++) {
-    cfgs_[b].translationScale = cfgs_[b].cellSize;
-    c_[b] = to_c(cfgs_[b]);
+    c_[b].translation_scale = c_[b].cell_size;  // MC/src/sim.cpp:49
+    from_c(c_[b], &cfgs_[b], nullptr);
     lats_[b] = build_lattice(c_[b]);
     views[b] = lats_[b].view();
   }
@@ -709,19 +877,19 @@ void record(Schedule& s, const SimulationConfig& cf, const HostLattice& L, const
 // iterations against a median of 14,253), and its trials are cheaper in a
 // round's tail, where the graph covers only the last few lattices, than
 // interleaved with every other lattice's solves.
-bool per_lattice(const std::vector<SimulationConfig>& cfgs) {
+bool per_lattice(const std::vector<mmc_config>& cfgs) {
   const char* r = std::getenv("MMC_BATCH_PER_LATTICE");
   if (!r || std::string(r) != "1") return false;
   const char* e = std::getenv("MMC_DEVICE_LOOP");
   if (e && std::string(e) == "block") return false;
-  for (const SimulationConfig& c : cfgs)
-    if (c.lineSearch != 0) return false;
+  for (const mmc_config& c : cfgs)
+    if (c.line_search != 0) return false;
   return true;
 }
 }  // namespace
 
 std::vector<BatchSimulator::Outcome> BatchSimulator::run(bool verbose) {
-  return per_lattice(cfgs_) ? run_per_lattice(verbose) : run_rounds(verbose);
+  return per_lattice(c_) ? run_per_lattice(verbose) : run_rounds(verbose);
 }
 
 // Rounds: every lattice with a target solves in one mmc_minimize_batch; the
@@ -883,7 +1051,7 @@ std::vector<BatchSimulator::Outcome> BatchSimulator::run_per_lattice(bool verbos
 // C entry points of the facade
 // ===========================================================================
 struct mmc_sim {
-  std::unique_ptr<meta::Simulator> sim;
+  std::unique_ptr<meta::Driver> sim;
   std::string err;
 };
 struct mmc_batch {
@@ -945,11 +1113,9 @@ int mmc_sim_create(const mmc_config* cfg, int32_t precision, mmc_sim** out) {
   *out = nullptr;
   auto* s = new mmc_sim();
   try {
-    meta::SimulationConfig c = meta::from_c(*cfg);
-    c.precision = precision;
-    c.captureFrames = false;
-    c.verbose = false;
-    s->sim = std::make_unique<meta::Simulator>(c);
+    s->sim = std::make_unique<meta::Driver>(*cfg, precision);
+    s->sim->setCaptureFrames(false);
+    s->sim->setVerbose(false);
   } catch (const std::exception& e) {
     s->err = e.what();
     g_sim_create_error = e.what();
@@ -1044,9 +1210,7 @@ int mmc_batch_create(const mmc_config* cfgs, int32_t n, int32_t precision, mmc_b
   *out = nullptr;
   auto* s = new mmc_batch();
   try {
-    std::vector<meta::SimulationConfig> c(static_cast<size_t>(n));
-    for (int b = 0; b < n; b++) c[size_t(b)] = meta::from_c(cfgs[b]);
-    s->sim = std::make_unique<meta::BatchSimulator>(std::move(c), precision);
+    s->sim = std::make_unique<meta::BatchSimulator>(std::vector<mmc_config>(cfgs, cfgs + n), precision);
   } catch (const std::exception& e) {
     s->err = e.what();
     g_sim_create_error = e.what();

@@ -1,14 +1,28 @@
-// simulator.hpp — C++20 facade with the public API of the reference's
-// meta::Simulator (projects/metamaterial-compression/industry-grade/include/
-// sim.hpp:6-54, MC/ below).  Construction, the load/unload schedule with
-// adaptive bisection, diagnostics, snapshot capture and the verification checks
-// run on the host; the equilibrium hot path (objective, minimize, commit) runs
-// on the GPU through the C-ABI of include/mmc.h.
+// simulator.hpp — the reference's solver API (projects/metamaterial-compression/
+// industry-grade/include/sim.hpp, MC/ below) over the gfx950 hot path.
+//
+// meta::Simulator and every struct it exposes keep the reference's exact
+// layout (same members, types, order and defaults), so MC/src/main.cpp and
+// MC/src/render.cpp compiled unchanged against the reference's own headers link
+// against libmmc.so in place of MC/src/sim.cpp: the constructor, run(),
+// exportSTL(), rigidRotationTest() and gradientCheck() are defined here, and
+// the reference header's inline accessors and implicit destructor work on the
+// same members.  The equilibrium hot path (objective, minimize, commit) runs on
+// the GPU through the C-ABI of include/mmc.h; run() drives it through a
+// Driver (below) and copies the committed state back into the members.
+//
+// What the reference's SimulationConfig cannot carry (the cubic lattice
+// family, the precision policy, the parity-form line search, ...) is a
+// SimulationExtensions value handed to the next Simulator constructed on the
+// calling thread (setNextSimulationExtensions); without one every Simulator
+// behaves as the reference's.
 #pragma once
 #include <cstdint>
 #include <filesystem>
+#include <limits>
 #include <memory>
 #include <string>
+#include <unordered_set>
 #include <vector>
 
 #include "../../include/mmc.h"
@@ -16,6 +30,7 @@
 
 namespace meta {
 
+// MC/include/core.hpp:38,40 (layout: three and four doubles)
 struct Vec3 {
   double x = 0, y = 0, z = 0;
 };
@@ -46,8 +61,7 @@ struct SimulationResult {
   std::vector<FrameDiagnostics> diagnostics;
 };
 
-// MC/include/sim.hpp:16-26, same names and defaults; the fields after the
-// marker are additive (no reference counterpart).
+// MC/include/sim.hpp:16-26: same names, order and defaults
 struct MaterialConfig {
   std::string name = "FFF TPU 95A provisional isotropic baseline";
   double density = 1180, young = 26e6, nu = .485, yield = 5.2e6, hardening = 1.4e6, bendHardening = .06,
@@ -68,21 +82,9 @@ struct SimulationConfig {
   int maxAdaptiveRefinements = 7;
   int fps = 24, frames = 83, threads = 0;
   uint64_t seed = 0x494e445553545259ULL;
-  // --- additive ---
-  int lattice = MMC_LATTICE_BCC;
-  bool relativeEscape = true;
-  double relativeTolerance = 1.5e-3;
-  double positionSigma = 0;
-  int radiusMode = 0;
-  int lineSearch = 0;  // 0 total-energy Armijo (the reference's), 1 per-term energy change; 2 per-term with the approximate-Wolfe slope test below 1e-6 |E|
-  int precision = MMC_FP64;
-  bool captureFrames = true;  // per-frame snapshots for the META3 cache
-  bool verbose = true;        // per-step "[equilibrium]" line on stdout
 };
 
-mmc_config to_c(const SimulationConfig& c);
-SimulationConfig from_c(const mmc_config& c);
-
+// MC/include/sim.hpp:27-33
 struct Node {
   Vec3 x0, x;
   Quat R;
@@ -100,13 +102,102 @@ struct Beam {
   Vec3 e0{1, 0, 0};
   PlasticState plastic;
 };
+struct TrialBeam {
+  PlasticState next;
+  double strain = 0, stress = 0;
+  Vec3 moment{0, 0, 0};
+  double elastic = 0, dissipation = 0;
+  bool axialPlastic = false, bendPlastic = false, torsionPlastic = false;
+};
+struct ObjectiveDiag {
+  double elastic = 0, plastic = 0, contact = 0, reaction = 0, maxPen = 0;
+  int contacts = 0;
+};
+struct Objective {
+  double energy = 0;
+  std::vector<double> grad;
+  ObjectiveDiag d;
+  std::vector<TrialBeam> trial;
+};
+struct OptStats {
+  int iterations = 0, backtracks = 0;
+  double rms = std::numeric_limits<double>::infinity(), max = std::numeric_limits<double>::infinity(),
+         energy = std::numeric_limits<double>::infinity();
+  bool converged = false;
+};
 
+// MC/include/sim.hpp:35-54: public API and data members of the reference.
 class Simulator {
  public:
   explicit Simulator(SimulationConfig cfg);
-  ~Simulator();
-  Simulator(const Simulator&) = delete;
-  Simulator& operator=(const Simulator&) = delete;
+  SimulationResult run();
+  // Manufacturing export is outside this build (the reference's STL path,
+  // MC/src/sim.cpp:579-888): throws std::runtime_error.
+  void exportSTL(const std::filesystem::path& path, int radialSegments = 12) const;
+  bool rigidRotationTest(double* error = nullptr) const;
+  bool gradientCheck(double* relativeError = nullptr) const;
+  const std::vector<Node>& nodes() const { return nodes_; }
+  const std::vector<Beam>& beams() const { return beams_; }
+  double initialHeight() const { return height_; }
+  double mass() const { return mass_; }
+  double area() const { return area_; }
+
+ private:
+  SimulationConfig cfg_;
+  std::vector<Node> nodes_;
+  std::vector<Beam> beams_;
+  std::unordered_set<uint64_t> connected_;
+  std::vector<uint8_t> locked_;
+  std::vector<Vec3> baseX_;
+  std::vector<Quat> baseR_;
+  double width_ = 0, height_ = 0, depth_ = 0, area_ = 0, mass_ = 0, top0_ = 0;
+  double loadingWork_ = 0, recoveredWork_ = 0, prevDisp_ = 0, prevForce_ = 0, reactionOffset_ = 0;
+
+  // the committed lattice as the device takes it (base pose, plastic state)
+  HostLattice committedLattice() const;
+};
+
+// ---------------------------------------------------------------------------
+// Additive configuration (no counterpart in the reference's SimulationConfig)
+// ---------------------------------------------------------------------------
+struct SimulationExtensions {
+  int lattice = MMC_LATTICE_BCC;     // MMC_LATTICE_BCC (the reference's) or MMC_LATTICE_CUBIC
+  bool relativeEscape = true;        // the relative-residual escape of MC/src/sim.cpp:177
+  double relativeTolerance = 1.5e-3;
+  double positionSigma = 0;          // cubic: Gaussian node jitter (m); 0 = corner-node imperfection
+  int radiusMode = 0;                // cubic: 0 linear grading along y, 1 per-beam uniform
+  int lineSearch = 0;                // mmc_config::line_search (0 = the reference's Armijo test)
+  int precision = MMC_FP64;          // MMC_FP64 or MMC_MIXED
+  bool captureFrames = true;         // per-frame snapshots in SimulationResult::frames
+  bool verbose = true;               // per-step "[equilibrium]" line on stdout
+};
+// Applies to the next Simulator constructed on the calling thread (once).
+void setNextSimulationExtensions(const SimulationExtensions& ext);
+// The extensions a Simulator was constructed with (the reference's behaviour
+// when none were set).  The entry of a destroyed Simulator stays until another
+// Simulator is constructed at the same address (a few dozen bytes each).
+SimulationExtensions simulationExtensions(const Simulator& sim);
+
+mmc_config to_c(const SimulationConfig& c, const SimulationExtensions& e);
+void from_c(const mmc_config& c, SimulationConfig* s, SimulationExtensions* e);
+
+// ---------------------------------------------------------------------------
+// Driver: one lattice's device context and the reference schedule around it
+// (MC/src/sim.cpp:374-467) — tare, load, unload, adaptive bisection, work
+// accounting, diagnostics and snapshot capture — stepwise (begin/advance) or
+// whole (run).  Simulator::run, the mmc_sim_* C entry points and the CLI use
+// it; it has no counterpart in the reference (whose Simulator does this on
+// its own members).
+// ---------------------------------------------------------------------------
+class Driver {
+ public:
+  // A lattice built from the configuration (mmc_lattice_build's construction).
+  Driver(const mmc_config& c, int precision);
+  // A given lattice (Simulator::run hands over its committed state).
+  Driver(const mmc_config& c, int precision, HostLattice lat);
+  ~Driver();
+  Driver(const Driver&) = delete;
+  Driver& operator=(const Driver&) = delete;
 
   SimulationResult run();
   // Stepwise schedule (run() is begin() followed by advance() until it returns
@@ -114,23 +205,28 @@ class Simulator {
   // unloadSteps unloading increments (MC/src/sim.cpp:456-465).
   void begin();
   bool advance(SimulationResult& result);
-  // Manufacturing export is outside this build (the reference's STL path,
-  // MC/src/sim.cpp:579-888); throws.
-  void exportSTL(const std::filesystem::path& path, int radialSegments = 12) const;
   bool rigidRotationTest(double* error = nullptr) const;
   bool gradientCheck(double* relativeError = nullptr) const;
+  // committed state mirrored from the device (read on demand)
   const std::vector<Node>& nodes() const;
   const std::vector<Beam>& beams() const;
   double initialHeight() const { return lat_.height; }
   double mass() const { return lat_.total_mass; }
   double area() const { return lat_.area_ref; }
-  const SimulationConfig& config() const { return cfg_; }
-  void setVerbose(bool v) { cfg_.verbose = v; }
+  void setVerbose(bool v) { ext_.verbose = v; }
+  void setCaptureFrames(bool v) { ext_.captureFrames = v; }
   mmc_ctx* context() const { return ctx_; }
   const HostLattice& lattice() const { return lat_; }
+  // work accounting of the schedule so far (MC/src/sim.cpp:420-433)
+  double loadingWork() const { return loadingWork_; }
+  double recoveredWork() const { return recoveredWork_; }
+  double prevDisp() const { return prevDisp_; }
+  double prevForce() const { return prevForce_; }
+  double reactionOffset() const { return reactionOffset_; }
 
  private:
   SimulationConfig cfg_;
+  SimulationExtensions ext_;
   mmc_config c_;
   HostLattice lat_;
   mmc_ctx* ctx_ = nullptr;
@@ -142,8 +238,8 @@ class Simulator {
   bool first_ = true;
   int acceptedSteps_ = 0, cursor_ = 0;
 
+  void create(int precision);
   void advanceTo(double targetTop, bool loading, int refinement, SimulationResult& result);
-
   void check(int rc) const;
   void sync() const;
   double topY(double s, bool loading) const;
@@ -160,7 +256,7 @@ class Simulator {
 // reference's error text while the others continue.
 class BatchSimulator {
  public:
-  BatchSimulator(std::vector<SimulationConfig> cfgs, int precision);
+  BatchSimulator(std::vector<mmc_config> cfgs, int precision);
   ~BatchSimulator();
   BatchSimulator(const BatchSimulator&) = delete;
   BatchSimulator& operator=(const BatchSimulator&) = delete;

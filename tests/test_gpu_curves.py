@@ -89,29 +89,33 @@ def test_published_specimen_run_matches_reference(gpu):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_published_specimen_equilibria_match_oracle(gpu, precision):
-    """The same specimen and schedule in the parity form (tolerance 1e-7,
-    line_search 2, 20,000 iterations per solve) against the oracle's curve,
-    which is the reference's algorithm at that tolerance (the oracle equals the
-    reference bit for bit on this specimen, tests/test_oracle_pin.py): the
-    tare and the first four load increments, to 20% strain.  The soft, nearly
-    incompressible BCC material (E 26 MPa, nu 0.485) needs 6,000-9,000
-    iterations per increment at this tolerance there, and from 22.5% strain
-    the oracle needs bisection and up to 15,000 iterations per solve while the
-    GPU's solves, summing in other orders, exhaust the 20,000-iteration cap
-    through all seven bisections: the window ends before that."""
+def test_published_specimen_equilibria_match_oracle(gpu, po, precision):
+    """The same specimen in the parity form against the oracle, which is the
+    reference's algorithm (the oracle equals the reference bit for bit on this
+    specimen, tests/test_oracle_pin.py): the tare and the first two load
+    increments of its schedule (0, 5% and 10% strain), each solved from the
+    committed state by both to gradient tolerance 1e-8 and committed (the
+    oracle's side: tests/golden/bcc_222_equilibria_oracle.npz).  The
+    soft, nearly incompressible BCC material (E 26 MPa, nu 0.485) is 77 times
+    more compliant than the cubic workloads, so the same absolute gradient
+    tolerance leaves a band 77 times wider: at 1e-7 the curves agree to 3.7e-6
+    of the force; 1e-8 pins the 1e-6."""
     mmc = gpu
-    fo = _fixture("bcc_222_tight")
-    n = 5
-    assert len(fo) >= n
-    fo = fo[:n]
+    g = np.load(GOLDEN / "bcc_222_equilibria_oracle.npz")  # tests/golden/make_specimen_equilibria.py
+    cfg = mmc.tight(mmc.specimen_config().replace(max_iterations=40000), tol=1e-8)
+    lat = po.Solver(cfg, "orc").lattice()
     prec = mmc.MMC_FP64 if precision == "fp64" else mmc.MMC_MIXED
-    fg = _advance_frames(mmc.Simulator(mmc.tight(mmc.specimen_config().replace(max_iterations=20000)), prec), n)
-    assert np.array_equal(fg["engineering_strain"], fo["engineering_strain"])
-    assert np.all(fg["converged"] == 1)
-    force, lw = _window_errors(fg, fo, n)
-    print(precision, "bcc 2x4x2 parity form,", n, "frames: force", force, "loading work", lw)
-    assert force <= BOUND[precision] and lw <= BOUND[precision]
+    ctx = mmc.Context(lat, cfg, prec)
+    assert bool(np.all(g["converged"]))
+    for k, top in enumerate(g["top"]):
+        sg, fg, _ = ctx.minimize(float(top))
+        err = abs(fg.reaction - g["reaction"][k]) / max(g["reaction"][k], 1.0)
+        print(precision, "bcc 2x4x2 strain", g["strain"][k], "iterations", g["iterations"][k], sg.iterations,
+              "force", g["reaction"][k], "error", err)
+        assert sg.converged
+        assert err <= BOUND[precision]
+        assert abs(fg.energy - g["energy"][k]) <= BOUND[precision] * abs(g["energy"][k])
+        ctx.commit()
 
 
 def _advance_frames(sim, n):
@@ -164,7 +168,7 @@ def _batched(mmc, names, precision):
     cfgs = []
     for name in names:
         kind, idx, _ = name.split("_")
-        cfgs.append(mmc.tight(mmc.batch_workload(kind, 0, indices=[int(idx)])[0], tol=3e-8))
+        cfgs.append(mmc.tight(mmc.batch_workload(kind, 0, indices=[int(idx)])[0], tol=1e-8))
     prec = mmc.MMC_FP64 if precision == "fp64" else mmc.MMC_MIXED
     frames, status, err = mmc.BatchSimulator(cfgs, prec).run()
     assert status == [0] * len(cfgs), err
