@@ -507,6 +507,14 @@ class Simulator:
     def context(self) -> int:
         return lib().mmc_sim_context(self.h)
 
+    def read_state(self):
+        """Committed base pose (x 3N, q 4N) and plastic states (7B), node-major."""
+        x, q, p = np.zeros(3 * self.n_nodes), np.zeros(4 * self.n_nodes), np.zeros(7 * self.n_beams)
+        rc = lib().mmc_read_state(self.context(), _dp(x), _dp(q), _dp(p))
+        if rc != 0:
+            raise MMCError(lib().mmc_last_error(self.context()).decode())
+        return x, q, p
+
     def begin(self):
         self._check(lib().mmc_sim_begin(self.h))
 
