@@ -50,6 +50,7 @@ struct Lat {
   const int* inc;         // beam*2 + side
   const int* adj_off;     // [N+1]
   const int* adj;         // sorted neighbour lists (contact exclusion)
+  const int* eslot;       // [2B] slot of each beam end (beam*2 + side) in the node-ordered records
 };
 
 // Per-evaluation scalars handed between kernels.

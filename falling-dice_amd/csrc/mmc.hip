@@ -460,6 +460,29 @@ __global__ __launch_bounds__(kBlock) void k_pairs(Lat<T> L, const Inst* __restri
   pcount[s] = u;
 }
 
+// Node-ordered gradient records of the device loop.  Each beam end owns one
+// slot; the slots of the 64 nodes of a gather block are interleaved (slot u of
+// lane l at base + 64 u + l, u in CSR order of the node's incident beams), so
+// that the gather reads one coalesced column per incident beam, with no index
+// chain.  rec holds per slot the end's share of the chord force (-f on end a,
+// f on end b) and the end's rotational gradient ([6][S]); cap the capsule
+// contact force on the end ([3][S]).  Slots past a node's degree stay zero.
+struct EllOut {
+  double* rec;
+  double* cap;
+  long S;
+};
+template <class T>
+__device__ __forceinline__ void ell_cap(Lat<T> L, EllOut eo, int k, v3<double> f0, v3<double> f1) {
+  const long sa = L.eslot[2 * k], sb = L.eslot[2 * k + 1];
+  eo.cap[sa] = f0.x;
+  eo.cap[eo.S + sa] = f0.y;
+  eo.cap[2 * eo.S + sa] = f0.z;
+  eo.cap[sb] = f1.x;
+  eo.cap[eo.S + sb] = f1.y;
+  eo.cap[2 * eo.S + sb] = f1.z;
+}
+
 // k_contact: MC/src/sim.cpp:159-165 over the Verlet list.  Each shape
 // accumulates its own force; each pair's energy is counted by its lower shape.
 // partials [4][ncols]: E_contact, max_pen, contacts, sum of per-shape energy
@@ -469,7 +492,7 @@ __device__ __forceinline__ void contact_walk(Lat<T> L, const Inst& in, const dou
                                              const unsigned* __restrict__ pcount, const int* __restrict__ plist,
                                              int slab, double* __restrict__ cf_sph, double* __restrict__ cf_cap,
                                              TermBufs ct, double* __restrict__ part, int s, bool on, size_t col,
-                                             size_t ncols) {
+                                             size_t ncols, EllOut eo = EllOut{nullptr, nullptr, 0}) {
   const int N = L.N, B = L.B;
   const Scalars& S = in.S;
   double v[4] = {0, 0, 0, 0};
@@ -516,12 +539,15 @@ __device__ __forceinline__ void contact_walk(Lat<T> L, const Inst& in, const dou
     }
     if (scap) {
       const int k = in.b0 + (s - in.s0 - in.N);
-      cf_cap[k] = f0.x;
-      cf_cap[B + k] = f0.y;
-      cf_cap[2 * B + k] = f0.z;
-      cf_cap[3 * B + k] = f1.x;
-      cf_cap[4 * B + k] = f1.y;
-      cf_cap[5 * B + k] = f1.z;
+      if (cf_cap) {
+        cf_cap[k] = f0.x;
+        cf_cap[B + k] = f0.y;
+        cf_cap[2 * B + k] = f0.z;
+        cf_cap[3 * B + k] = f1.x;
+        cf_cap[4 * B + k] = f1.y;
+        cf_cap[5 * B + k] = f1.z;
+      }
+      if (eo.cap) ell_cap(L, eo, k, f0, f1);
     } else {
       const int i = in.n0 + (s - in.s0);
       cf_sph[i] = f0.x;
@@ -570,7 +596,7 @@ __device__ __forceinline__ void contact_group(Lat<T> L, const Inst& in, const do
                                               const unsigned* __restrict__ pcount, const int* __restrict__ plist,
                                               int slab, double* __restrict__ cf_sph, double* __restrict__ cf_cap,
                                               TermBufs ct, double* __restrict__ part, int first, int count, int blk,
-                                              int P) {
+                                              int P, EllOut eo = EllOut{nullptr, nullptr, 0}) {
   static_assert(kBlock == kWave * kContactLanes, "a block holds 64 shapes");
   const int N = L.N, B = L.B;
   const Scalars& S = in.S;
@@ -656,12 +682,15 @@ __device__ __forceinline__ void contact_group(Lat<T> L, const Inst& in, const do
     if (on) {
       if (scap) {
         const int kb = in.b0 + (s - in.s0 - in.N);
-        cf_cap[kb] = f0.x;
-        cf_cap[B + kb] = f0.y;
-        cf_cap[2 * B + kb] = f0.z;
-        cf_cap[3 * B + kb] = f1.x;
-        cf_cap[4 * B + kb] = f1.y;
-        cf_cap[5 * B + kb] = f1.z;
+        if (cf_cap) {
+          cf_cap[kb] = f0.x;
+          cf_cap[B + kb] = f0.y;
+          cf_cap[2 * B + kb] = f0.z;
+          cf_cap[3 * B + kb] = f1.x;
+          cf_cap[4 * B + kb] = f1.y;
+          cf_cap[5 * B + kb] = f1.z;
+        }
+        if (eo.cap) ell_cap(L, eo, kb, f0, f1);
       } else {
         const int i = in.n0 + (s - in.s0);
         cf_sph[i] = f0.x;
@@ -730,7 +759,8 @@ template <class T, int NT = kBlock>
 __device__ __forceinline__ void beam_body(Lat<T> L, Material M, const double* __restrict__ x,
                                                  const double* __restrict__ R, double* __restrict__ rec,
                                                  double* __restrict__ trial, TermBufs bt,
-                                                 double* __restrict__ part, int k, bool on, int blk, int P) {
+                                                 double* __restrict__ part, int k, bool on, int blk, int P,
+                                                 EllOut eo = EllOut{nullptr, nullptr, 0}) {
   const int N = L.N, B = L.B;
   double v[4] = {0, 0, 0, 0};
   if (on) {
@@ -818,8 +848,19 @@ __device__ __forceinline__ void beam_body(Lat<T> L, Material M, const double* __
         tr[16] = to.yielded;
       }
     }
+    if (rec)
 #pragma unroll
-    for (int c = 0; c < 9; c++) rec[c * B + k] = out[c];
+      for (int c = 0; c < 9; c++) rec[c * B + k] = out[c];
+    if (eo.rec) {
+      const long sa = L.eslot[2 * k], sb = L.eslot[2 * k + 1];
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        eo.rec[c * eo.S + sa] = -out[c];
+        eo.rec[c * eo.S + sb] = out[c];
+        eo.rec[(3 + c) * eo.S + sa] = out[3 + c];
+        eo.rec[(3 + c) * eo.S + sb] = out[6 + c];
+      }
+    }
     if (trial)
 #pragma unroll
       for (int c = 0; c < MMC_TRIAL_STRIDE; c++) trial[c * B + k] = tr[c];
@@ -934,6 +975,59 @@ __device__ __forceinline__ double gather_sum(Lat<T> L, const Inst& in, double to
     gs += S.centering * (d == 0 ? cen0 : cen1) / in.N;
   }
   return S.ts * gs;
+}
+
+// Gradient component d of node i from the node-ordered records (EllOut): the
+// same terms in the same order as gather_load + gather_sum (incident beams in
+// CSR order: the end's chord-force share, then its capsule contact force; the
+// rotational gradient of the end), so the sums are bit for bit those of the
+// CSR walk.  The md slots of the node's block (md: the block's largest degree)
+// are loaded before the first use, without waiting for the node's degree.
+constexpr int kEllUnroll = 8;
+template <class T>
+__device__ __forceinline__ double gather_ell(Lat<T> L, const Inst& in, double top, const double* __restrict__ x,
+                                             const double* __restrict__ erec, const double* __restrict__ ecap,
+                                             long S, const double* __restrict__ cf_sph, double cen0, double cen1,
+                                             int d, int i, int deg, long sb, int md) {
+  const int N = L.N;
+  const Scalars& Sc = in.S;
+  double r[kEllUnroll], c[kEllUnroll];
+  const double* rr = erec + d * S + sb;
+  const double* cc = ecap + (d < 3 ? d : 0) * S + sb;
+#pragma unroll
+  for (int u = 0; u < kEllUnroll; u++) {
+    r[u] = 0.0;
+    c[u] = 0.0;
+    if (u < md) {
+      r[u] = rr[long(u) * kWave];
+      if (d < 3) c[u] = cc[long(u) * kWave];
+    }
+  }
+  double gs = 0.0;
+#pragma unroll
+  for (int u = 0; u < kEllUnroll; u++) {
+    if (u < deg) {
+      gs = gs + r[u];
+      if (d < 3) gs = gs + c[u];
+    }
+  }
+  for (int u = kEllUnroll; u < deg; u++) {
+    gs = gs + rr[long(u) * kWave];
+    if (d < 3) gs = gs + cc[long(u) * kWave];
+  }
+  if (d >= 3) return gs;
+  gs = gs + cf_sph[size_t(d) * N + i];
+  if (d == 1) {
+    gs += L.mass[i] * Sc.gravity;
+    const double y = x[N + i];
+    const double pb = Sc.jr - y;
+    if (pb > 0) gs -= Sc.kc * pb;
+    const double pt = y + Sc.jr - top;
+    if (pt > 0) gs += Sc.kc * pt;
+  } else {
+    gs += Sc.centering * (d == 0 ? cen0 : cen1) / in.N;
+  }
+  return Sc.ts * gs;
 }
 
 // The six gradient components of node i at once (gather_sum's arithmetic per
@@ -1926,7 +2020,7 @@ __global__ __launch_bounds__(kBlock) void k_elem_g(const Ctl* __restrict__ Cs, L
                                                    const unsigned* __restrict__ hcount, int H,
                                                    const int* __restrict__ act, const GRng* __restrict__ G,
                                                    const double* __restrict__ dspart, int Pd,
-                                                   double* __restrict__ cent) {
+                                                   double* __restrict__ cent, EllOut eo) {
   const int Pcb = (Pc + kBW - 1) / kBW;  // blocks of four contact chunks
   const int b = blockIdx.x;
   if (b < Pb) {
@@ -1935,8 +2029,8 @@ __global__ __launch_bounds__(kBlock) void k_elem_g(const Ctl* __restrict__ Cs, L
     if (!ctl_trial_phase(C)) return;
     const int par = C->par;
     TermBufs bt{par ? ub1 : ub0, par ? ub0 : ub1};
-    beam_body<T>(L, I[bk.inst].M, x, R, rec, nullptr, bt, bpart, bk.first + threadIdx.x,
-                 int(threadIdx.x) < bk.count, b, Pb);
+    beam_body<T>(L, I[bk.inst].M, x, R, nullptr, nullptr, bt, bpart, bk.first + threadIdx.x,
+                 int(threadIdx.x) < bk.count, b, Pb, eo);
   } else if (b < Pb + Pcb) {
     // wavefront w walks chunk 4 (b - Pb) + w, one lane per shape, unless the
     // chunk is flagged for the lane groups below (no barriers on this path)
@@ -1947,8 +2041,8 @@ __global__ __launch_bounds__(kBlock) void k_elem_g(const Ctl* __restrict__ Cs, L
     if (!ctl_trial_phase(C)) return;
     const int par = C->par;
     TermBufs ct{par ? cb1 : cb0, par ? cb0 : cb1};
-    contact_walk<T>(L, I[bk.inst], x, pcount, lr->plist, lr->slab, cf_sph, cf_cap, ct, cpart, bk.first + l,
-                    l < bk.count, size_t(j), size_t(Pc));
+    contact_walk<T>(L, I[bk.inst], x, pcount, lr->plist, lr->slab, cf_sph, nullptr, ct, cpart, bk.first + l,
+                    l < bk.count, size_t(j), size_t(Pc), eo);
   } else if (b < Pb + Pcb + H) {
     // H blocks share the flagged chunks (block-uniform loop)
     const unsigned nh = *hcount;
@@ -1959,8 +2053,8 @@ __global__ __launch_bounds__(kBlock) void k_elem_g(const Ctl* __restrict__ Cs, L
       if (!ctl_trial_phase(C)) continue;
       const int par = C->par;
       TermBufs ct{par ? cb1 : cb0, par ? cb0 : cb1};
-      contact_group<T>(L, I[bk.inst], x, pcount, lr->plist, lr->slab, cf_sph, cf_cap, ct, cpart, bk.first,
-                       bk.count, j, Pc);
+      contact_group<T>(L, I[bk.inst], x, pcount, lr->plist, lr->slab, cf_sph, nullptr, ct, cpart, bk.first,
+                       bk.count, j, Pc, eo);
     }
   } else {
     const int k = act[b - Pb - Pcb - H];
@@ -2137,9 +2231,9 @@ template <class T, int MH>
 __global__ __launch_bounds__(kElemThreads) void k_gather_g(Lat<T> L, const Inst* __restrict__ I,
                                                            const Blk* __restrict__ td,
                                                            const Ctl* __restrict__ Cs, const double* __restrict__ x,
-                                                           const double* __restrict__ rec,
+                                                           const double* __restrict__ erec,
                                                            const double* __restrict__ cf_sph,
-                                                           const double* __restrict__ cf_cap,
+                                                           const double* __restrict__ ecap, long eS,
                                                            const double* __restrict__ cent,
                                                            const double* __restrict__ z0, const double* __restrict__ z1,
                                                            double* __restrict__ g0, double* __restrict__ g1,
@@ -2177,10 +2271,10 @@ __global__ __launch_bounds__(kElemThreads) void k_gather_g(Lat<T> L, const Inst*
   }
   const double zcv = zcand[e], zv = zcur[e], gov = gcur[e], pv = p[e];
   const bool locked = (L.lock[ic] >> d) & 1u;
-  IncVals iv;
-  gather_load<T>(L, rec, cf_cap, d, ic, iv);
+  const int deg = L.inc_off[ic + 1] - L.inc_off[ic];
   const double c0 = cent[2 * bk.inst], c1 = cent[2 * bk.inst + 1];
-  const double gv = gather_sum<T>(L, in, C->top, x, rec, cf_sph, cf_cap, c0, c1, d, ic, iv);
+  const double gv = gather_ell<T>(L, in, C->top, x, erec, ecap, eS, cf_sph, c0, c1, d, ic, deg,
+                                  long(bk.pad >> 5) + l, bk.pad & 31);
   const double gd = locked ? 0.0 : gv;
   if (on) gnew[e] = gd;
   double a[NP];
@@ -2745,6 +2839,11 @@ struct mmc_ctx {
   int *ba = nullptr, *bb = nullptr;
   double *L0 = nullptr, *rad = nullptr, *area = nullptr, *inert = nullptr, *e0 = nullptr, *plastic = nullptr;
   int *inc_off = nullptr, *inc = nullptr, *adj_off = nullptr, *adj = nullptr;
+  // node-ordered gradient records of the device loop (EllOut): slot of each
+  // beam end, [6][eS] end records, [3][eS] capsule contact forces on the ends
+  int* eslot = nullptr;
+  double *erec = nullptr, *ecap = nullptr;
+  long eS = 0;
   double *x = nullptr;
   void* R = nullptr;  // FP64 planes
   double *rec = nullptr, *trial = nullptr, *cf_sph = nullptr, *cf_cap = nullptr;
@@ -2872,10 +2971,11 @@ struct mmc_ctx {
   }
 
   Lat<double> lat64() const {
-    return Lat<double>{N, B, bpos, bquat, mass, lock, ba, bb, L0, rad, area, inert, e0, plastic, inc_off, inc, adj_off, adj};
+    return Lat<double>{N, B, bpos, bquat, mass, lock, ba, bb, L0, rad, area, inert, e0, plastic, inc_off, inc, adj_off, adj, eslot};
   }
   Lat<float> lat32() const {
-    return Lat<float>{N, B, bpos, bquat, mass, lock, ba, bb, L0, rad, area, inert, e0, plastic, inc_off, inc, adj_off, adj};
+    return Lat<float>{N, B, bpos, bquat, mass, lock, ba, bb, L0, rad, area, inert, e0, plastic, inc_off, inc, adj_off, adj,
+                      eslot};
   }
   int* iflags() const { return flags + 4; }
   double* rec_slot(int s) const { return record + size_t(s) * n_inst * kRec; }
@@ -2945,7 +3045,7 @@ mmc_ctx::~mmc_ctx() {
                   rec, trial, cf_sph, cf_cap, box, cnt, start, entries, flags, scan_tmp, spart, cpart, bpart, gpart,
                   vpart, stb, stn, stout, record, scal, z, g, p, zc, gc, zt, zin, Sh, Yh, xref, pcount, plist,
                   inst_d, tn, tb, tc, td, sinst, tops_d, mask_d, smask_d, ctl, lref, dspart, cent, cflag, hlist, hcount, drows, dcount, gtd, gtb, gtc,
-                  grng, gact, tctl, rtc, rtn};
+                  grng, gact, tctl, rtc, rtn, eslot, erec, ecap};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   void* pinned[] = {host, hctl, htops, hmask, hrec, hstat};
@@ -3178,6 +3278,32 @@ void mmc_ctx::setup(const mmc_lattice* const* Ls, const mmc_config* cs, int n, i
     t = block_table(n, kDirNodes, [&](int b, int& f, int& c) { f = hinst[b].n0, c = hinst[b].N; }, r0, r1);
     Pd = int(t.size());
     for (int b = 0; b < n; b++) hinst[b].pd0 = r0[b], hinst[b].pd1 = r1[b];
+    // node-ordered record slots of the loop's gather blocks (64 nodes each):
+    // block j's slots start at base_j, slot u of lane l at base_j + 64 u + l;
+    // the block's largest degree and base ride in its table entry (pad)
+    {
+      std::vector<int> es(2 * size_t(B), 0);
+      long base = 0;
+      for (Blk& bk : t) {
+        int md = 0;
+        for (int l = 0; l < bk.count; l++) md = std::max(md, off[bk.first + l + 1] - off[bk.first + l]);
+        if (md > 31) throw Fail(MMC_ERR_CAPACITY, "a node has more than 31 incident beams");
+        if (base + long(md) * kWave >= (1L << 26)) throw Fail(MMC_ERR_CAPACITY, "too many beam ends for the gather");
+        bk.pad = int(base << 5) | md;
+        for (int l = 0; l < bk.count; l++) {
+          const int i = bk.first + l;
+          for (int u = 0; u < off[i + 1] - off[i]; u++) es[size_t(list[size_t(off[i] + u)])] = int(base + long(u) * kWave + l);
+        }
+        base += long(md) * kWave;
+      }
+      eS = std::max(base, 1L);
+      eslot = dalloc<int>(2 * size_t(B));
+      HIPCALL(hipMemcpy(eslot, es.data(), 2 * size_t(B) * sizeof(int), hipMemcpyHostToDevice));
+      erec = dalloc<double>(6 * size_t(eS));
+      ecap = dalloc<double>(3 * size_t(eS));
+      HIPCALL(hipMemset(erec, 0, 6 * size_t(eS) * sizeof(double)));
+      HIPCALL(hipMemset(ecap, 0, 3 * size_t(eS) * sizeof(double)));
+    }
     put(td, t);
     htd = t;
     gtd = dalloc<Blk>(htd.size());
@@ -3614,10 +3740,10 @@ void mmc_ctx::capture_iteration(Lat<T> Lt) {
                                                     Yh, n6, p, x, (double*)R, xref, nbuf[0], nbuf[1], dspart);
     launch_elem<T>(Lt, ctl);
     if (max_hist <= 10)
-      k_gather_g<T, 10><<<gPd, kElemThreads, 0, stream>>>(Lt, inst_d, gtd, ctl, x, rec, cf_sph, cf_cap, cent,
+      k_gather_g<T, 10><<<gPd, kElemThreads, 0, stream>>>(Lt, inst_d, gtd, ctl, x, erec, cf_sph, ecap, eS, cent,
                                                           zbuf[0], zbuf[1], gbuf[0], gbuf[1], p, Sh, Yh, n6, gpart);
     else
-      k_gather_g<T, kMaxHist><<<gPd, kElemThreads, 0, stream>>>(Lt, inst_d, gtd, ctl, x, rec, cf_sph, cf_cap,
+      k_gather_g<T, kMaxHist><<<gPd, kElemThreads, 0, stream>>>(Lt, inst_d, gtd, ctl, x, erec, cf_sph, ecap, eS,
                                                                 cent, zbuf[0], zbuf[1], gbuf[0], gbuf[1], p, Sh, Yh,
                                                                 n6, gpart);
     launch_decide(ctl);
@@ -3717,7 +3843,7 @@ void mmc_ctx::launch_elem(Lat<T> Lt, const Ctl* C) {
   const int nb = gPb + (gPc + kBW - 1) / kBW + H + int(gset.size());
   k_elem_g<T><<<nb, kBlock, 0, stream>>>(C, Lt, inst_d, gtb, gtc, x, (const double*)R, rec, ubuf[0], ubuf[1], bpart,
                                          gPb, pcount, lref, cf_sph, cf_cap, cbuf[0], cbuf[1], cpart, gPc, cflag, hlist,
-                                         hcount, H, gact, grng, dspart, gPd, cent);
+                                         hcount, H, gact, grng, dspart, gPd, cent, EllOut{erec, ecap, eS});
 }
 
 // Flags the chunks of the loop's contact table whose lists are long
@@ -3860,12 +3986,12 @@ double mmc_ctx::time_loop_kernel(int kernel, int launches) {
           break;
         case MMC_K_GATHER_G:
           if (max_hist <= 10)
-            k_gather_g<T, 10><<<gPd, kElemThreads, 0, stream>>>(Lt, inst_d, gtd, tctl, x, rec, cf_sph,
-                                                                cf_cap, cent, zbuf[0], zbuf[1], gbuf[0], gbuf[1],
+            k_gather_g<T, 10><<<gPd, kElemThreads, 0, stream>>>(Lt, inst_d, gtd, tctl, x, erec, cf_sph,
+                                                                ecap, eS, cent, zbuf[0], zbuf[1], gbuf[0], gbuf[1],
                                                                 p, Sh, Yh, n6, gpart);
           else
-            k_gather_g<T, kMaxHist><<<gPd, kElemThreads, 0, stream>>>(Lt, inst_d, gtd, tctl, x, rec, cf_sph,
-                                                                      cf_cap, cent, zbuf[0], zbuf[1], gbuf[0],
+            k_gather_g<T, kMaxHist><<<gPd, kElemThreads, 0, stream>>>(Lt, inst_d, gtd, tctl, x, erec, cf_sph,
+                                                                      ecap, eS, cent, zbuf[0], zbuf[1], gbuf[0],
                                                                       gbuf[1], p, Sh, Yh, n6, gpart);
           break;
         default:
