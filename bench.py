@@ -19,8 +19,8 @@ and moment transport, FP64 elsewhere) gives `value`; the FP64 leg runs the same
 increments.  Both legs' platen-force curves are compared with each other and
 with the oracle's FP64 curve of the same workload (tests/golden), frame by
 frame over every frame both cover: `qualifier` states whether the metric's
-error bound holds (mixed <= 1e-5 against FP64 and against the oracle, FP64 <=
-1e-6 against the oracle).
+error bound holds (mixed <= 1e-5 against FP64 and against the oracle) and,
+separately, whether the FP64 leg is within its own 1e-6 of the oracle.
 
 The JSON line also carries:
   roofline      the device loop's dominant kernel (k_gather_g: gradient
@@ -317,8 +317,9 @@ def cpu_baseline_window(cfg, budget_s, iterations_window, steps_window):
 
 def qualifier(mixed_fr, fp64_fr, fo, fixture):
     """The metric's error bound over every frame the legs and the oracle's
-    curve share: mixed within 1e-5 of the peak force against FP64 and against
-    the oracle, FP64 within 1e-6 against the oracle."""
+    curve share: the mixed leg (the value) within 1e-5 of the peak force
+    against FP64 and against the oracle (`met`); the FP64 path's own parity
+    bound, 1e-6 against the oracle, reported beside it (`fp64_within_1e-6`)."""
     q = {"mixed_vs_fp64": curve_error(mixed_fr, fp64_fr)}
     if fo is None:
         q.update(met=None, reason="no oracle curve of this workload in tests/golden")
@@ -329,7 +330,8 @@ def qualifier(mixed_fr, fp64_fr, fo, fixture):
     n = min(len(mixed_fr), len(fp64_fr), len(fo))
     q["frames_compared"] = int(n)
     q["met"] = bool(q["mixed_vs_fp64"]["max_rel_force"] <= 1e-5 and q["mixed_vs_oracle"]["max_rel_force"] <= 1e-5
-                    and q["fp64_vs_oracle"]["max_rel_force"] <= 1e-6 and n == len(mixed_fr) == len(fp64_fr))
+                    and n == len(mixed_fr) == len(fp64_fr))
+    q["fp64_within_1e-6"] = bool(q["fp64_vs_oracle"]["max_rel_force"] <= 1e-6)
     return q
 
 
@@ -388,6 +390,9 @@ def main():
     # its own, on the loop's scratch buffers)
     launches = 200
     loop_us = {k: round(sim.time_kernel(k, launches) * 1e3, 3) for k in ("dstate_g", "elem_g", "gather_g", "decide_g")}
+    # a plain 16 MiB device copy: the known-size reference of the traffic
+    # counters (profiles/), and the copy bandwidth this device reaches
+    copy_ms = sim.time_kernel("copy", 50)
     gather_ms = loop_us["gather_g"] * 1e-3
     g_bytes = gather_loop_bytes(N, cfg.history)
     ach = g_bytes / (gather_ms * 1e-3) / 1e9
@@ -396,7 +401,8 @@ def main():
             "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": "k_gather_g",
             "bytes_per_launch": g_bytes, "mean_launch_us": round(gather_ms * 1e3, 3), "launches_timed": launches,
             "loop_kernel_us": loop_us, "trial_kernel_us": round(sum(loop_us.values()), 3),
-            "loop_shape": sim.loop_shape()}
+            "loop_shape": sim.loop_shape(),
+            "copy_16MiB_GBs": round(2 * (1 << 24) / (copy_ms * 1e-3) / 1e9, 1)}
     value = K / dt
     out = {
         "metric": "load-steps/sec, mixed vs FP64 on 1 GPU, at <=1e-5 rel platen-force error",

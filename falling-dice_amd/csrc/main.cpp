@@ -4,8 +4,9 @@
 // output files (compression.meta3, compression_metrics.csv, summary.json,
 // VALIDATION.txt, convergence.csv) in the same byte formats, and the same
 // stdout/stderr lines and exit codes.  The equilibrium solve runs on the GPU.
-// Rendering and STL export are not part of this build: --mode render|hero|stl
-// exit with an error, and --mode all runs the simulation and validation only.
+// STL export runs on the GPU too (--mode stl, and --mode sim without
+// --skip-stl).  Rendering is not part of this build: --mode render|hero exit
+// with an error, and --mode all runs the simulation, STL and validation.
 // Additive options (no reference counterpart) are listed by --help.
 #include <array>
 #include <cmath>
@@ -170,13 +171,14 @@ struct Args {
   int threads = 0;
   bool quick = false, noUnload = false, skipStl = false;
   bool strictValidation = false;  // exit 1 when VALIDATION.txt holds a FAIL (additive)
+  int stlQuality = 14;
   SimulationConfig sim;
   SimulationExtensions ext;  // the additive options
 };
 
 void usage() {
   std::cout << "metamaterial_compression --mode sim|render|hero|stl|validate|convergence|all [options]\n"
-               "gfx950 build: sim, validate, convergence (all = sim + validate; render/hero/stl not built)\n"
+               "gfx950 build: sim, stl, validate, convergence (all = sim + stl + validate; render/hero not built)\n"
                "additive options: --lattice bcc|cubic --cell-size M --min-radius M --max-radius M --nu V\n"
                "  --density V --seed N --history N --grad-tol V --grad-rms-tol V --grad-max-tol V\n"
                "  --no-relative-escape --position-sigma M --radius-mode 0|1 --line-search 0|1|2 --imperfection M\n"
@@ -199,8 +201,9 @@ Args parse_args(int argc, char** argv) {
     else if (o == "--threads") a.threads = std::stoi(value());
     else if (o == "--width" || o == "--height" || o == "--spp" || o == "--depth" || o == "--frame-start" ||
              o == "--frame-end" || o == "--frame-step" || o == "--subframes" || o == "--hero-width" ||
-             o == "--hero-height" || o == "--hero-spp" || o == "--stl-quality")
-      (void)std::stoi(value());  // render/STL options: accepted, unused here
+             o == "--hero-height" || o == "--hero-spp")
+      (void)std::stoi(value());  // render options: accepted, unused here
+    else if (o == "--stl-quality") a.stlQuality = std::stoi(value());
     else if (o == "--cells-x") a.sim.cellsX = std::stoi(value());
     else if (o == "--cells-y") a.sim.cellsY = std::stoi(value());
     else if (o == "--cells-z") a.sim.cellsZ = std::stoi(value());
@@ -258,6 +261,7 @@ Args parse_args(int argc, char** argv) {
     a.sim.loadSteps = 24;
     a.sim.unloadSteps = 14;
     a.sim.maxIterations = 95;
+    a.stlQuality = 10;
   }
   if (a.noUnload) a.sim.unloadSteps = 0;
   a.sim.threads = a.threads;
@@ -372,7 +376,7 @@ int main(int argc, char** argv) {
       run_convergence(args);
       return 0;
     }
-    if (args.mode == "render" || args.mode == "hero" || args.mode == "stl")
+    if (args.mode == "render" || args.mode == "hero")
       throw std::runtime_error("--mode " + args.mode + " is not part of the gfx950 equilibrium solver build");
     SimulationResult result;
     std::unique_ptr<Simulator> sim;
@@ -383,12 +387,13 @@ int main(int argc, char** argv) {
       save_cache(args.cache, result, args.sim.fps);
       write_metrics(args.output / "compression_metrics.csv", result);
       write_summary(args.output / "summary.json", result, args.sim, sim->mass());
-      if (!args.skipStl) std::cerr << "note: STL export is not part of this build (as with --skip-stl)\n";
+      if (!args.skipStl) sim->exportSTL(args.output / "graded_bcc_energy_absorber.stl", args.stlQuality);
       if (!write_validation(args.output / "VALIDATION.txt", *sim, &result) && args.strictValidation)
         throw std::runtime_error("validation failed (VALIDATION.txt)");
-    } else if (args.mode == "validate") {
+    } else if (args.mode == "validate" || args.mode == "stl") {
       setNextSimulationExtensions(args.ext);
       sim = std::make_unique<Simulator>(args.sim);
+      if (args.mode == "stl") sim->exportSTL(args.output / "graded_bcc_energy_absorber.stl", args.stlQuality);
       const bool ok = write_validation(args.output / "VALIDATION.txt", *sim, nullptr);
       if (!ok && args.strictValidation) throw std::runtime_error("validation failed (VALIDATION.txt)");
       return 0;

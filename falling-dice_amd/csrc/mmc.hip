@@ -2922,6 +2922,7 @@ struct mmc_ctx {
   int64_t partial_builds = 0;
   int64_t loop_polls = 0;
   Ctl* tctl = nullptr;  // scratch control blocks of time_loop_kernel
+  double *cal0 = nullptr, *cal1 = nullptr;  // buffers of the MMC_K_COPY timing
   double* dspart = nullptr;    // [10][Pd] partials of the trial state kernel
   double* cent = nullptr;      // [n_inst][2] node centroid (x, z) of the trial (k_elem_g)
   // contact chunks of the loop's table with long lists (k_classify)
@@ -3045,7 +3046,7 @@ mmc_ctx::~mmc_ctx() {
                   rec, trial, cf_sph, cf_cap, box, cnt, start, entries, flags, scan_tmp, spart, cpart, bpart, gpart,
                   vpart, stb, stn, stout, record, scal, z, g, p, zc, gc, zt, zin, Sh, Yh, xref, pcount, plist,
                   inst_d, tn, tb, tc, td, sinst, tops_d, mask_d, smask_d, ctl, lref, dspart, cent, cflag, hlist, hcount, drows, dcount, gtd, gtb, gtc,
-                  grng, gact, tctl, rtc, rtn, eslot, erec, ecap};
+                  grng, gact, tctl, rtc, rtn, eslot, erec, ecap, cal0, cal1};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   void* pinned[] = {host, hctl, htops, hmask, hrec, hstat};
@@ -4641,6 +4642,24 @@ int mmc_time_kernel(mmc_ctx* c, int32_t kernel, int32_t launches, double* mean_m
     }
     if (kernel >= MMC_K_DSTATE_G && kernel <= MMC_K_DECIDE_G) {
       *mean_ms = c->time_loop_kernel(kernel, launches);
+      return;
+    }
+    if (kernel == MMC_K_COPY) {
+      constexpr long words = 1L << 21;  // 16 MiB
+      if (!c->cal0) {
+        c->cal0 = dalloc<double>(words);
+        c->cal1 = dalloc<double>(words);
+        HIPCALL(hipMemset(c->cal0, 0, words * sizeof(double)));
+      }
+      HIPCALL(hipEventRecord(c->ev0, c->stream));
+      for (int i = 0; i < launches; i++)
+        k_copy_words<<<unsigned((words + kBlock - 1) / kBlock), kBlock, 0, c->stream>>>(c->cal1, c->cal0, words);
+      HIPCALL(hipEventRecord(c->ev1, c->stream));
+      HIPCALL(hipEventSynchronize(c->ev1));
+      HIPCALL(hipGetLastError());
+      float ms = 0;
+      HIPCALL(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+      *mean_ms = double(ms) / launches;
       return;
     }
     auto run = [&](auto Lt, auto* tag) {

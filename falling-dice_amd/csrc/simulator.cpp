@@ -343,8 +343,42 @@ SimulationResult Simulator::run() {
   return r;
 }
 
-void Simulator::exportSTL(const std::filesystem::path&, int) const {
-  throw std::runtime_error("STL export is not part of the gfx950 equilibrium solver");
+namespace {
+// MC/src/sim.cpp:645-888 through mmc_export_stl: the union of the capsules and
+// joint spheres at the initial positions x0
+void export_stl_of(const std::vector<Vec3>& x0, const std::vector<int>& a, const std::vector<int>& b,
+                   const std::vector<double>& radius, double jr, const std::filesystem::path& path, int segments,
+                   mmc_stl_report* rep) {
+  if (x0.empty() || a.empty()) throw std::runtime_error("cannot mesh empty lattice");
+  std::vector<double> x(3 * x0.size());
+  for (size_t i = 0; i < x0.size(); i++) x[3 * i] = x0[i].x, x[3 * i + 1] = x0[i].y, x[3 * i + 2] = x0[i].z;
+  mmc_lattice v{};
+  v.n_nodes = int32_t(x0.size());
+  v.n_beams = int32_t(a.size());
+  v.x = x.data();
+  v.beam_a = a.data();
+  v.beam_b = b.data();
+  v.radius = radius.data();
+  const std::string p = path.string();
+  if (mmc_export_stl(&v, jr, segments, p.c_str(), rep) != MMC_OK) throw std::runtime_error(mmc_stl_last_error());
+}
+}  // namespace
+
+void Simulator::exportSTL(const std::filesystem::path& path, int radialSegments) const {
+  if (nodes_.empty() || beams_.empty()) throw std::runtime_error("cannot mesh empty lattice");
+  std::vector<Vec3> x0(nodes_.size());
+  for (size_t i = 0; i < nodes_.size(); i++) x0[i] = nodes_[i].x0;
+  std::vector<int> a(beams_.size()), b(beams_.size());
+  std::vector<double> r(beams_.size());
+  for (size_t k = 0; k < beams_.size(); k++) a[k] = beams_[k].a, b[k] = beams_[k].b, r[k] = beams_[k].radius;
+  export_stl_of(x0, a, b, r, cfg_.jointScale * cfg_.maxRadius, path, radialSegments, nullptr);
+}
+
+void Driver::exportSTL(const std::filesystem::path& path, int radialSegments, mmc_stl_report* report) const {
+  const std::vector<double>& xs = lat_.x0.empty() ? lat_.x : lat_.x0;
+  std::vector<Vec3> x0(size_t(lat_.n_nodes));
+  for (size_t i = 0; i < x0.size(); i++) x0[i] = Vec3{xs[3 * i], xs[3 * i + 1], xs[3 * i + 2]};
+  export_stl_of(x0, lat_.a, lat_.b, lat_.radius, cfg_.jointScale * cfg_.maxRadius, path, radialSegments, report);
 }
 
 // MC/src/sim.cpp:511-536 and :538-577, on the committed state, through a
@@ -1204,6 +1238,11 @@ int mmc_sim_gradient_check(mmc_sim* s, double* error, int32_t* pass) {
 }
 
 mmc_ctx* mmc_sim_context(mmc_sim* s) { return (s && s->sim) ? s->sim->context() : nullptr; }
+
+int mmc_sim_export_stl(mmc_sim* s, const char* path, int32_t radial_segments, mmc_stl_report* report) {
+  if (!path) return MMC_ERR_ARG;
+  return sim_guard(s, [&] { s->sim->exportSTL(path, radial_segments, report); });
+}
 
 int mmc_batch_create(const mmc_config* cfgs, int32_t n, int32_t precision, mmc_batch** out) {
   if (!cfgs || n <= 0 || !out) return MMC_ERR_ARG;

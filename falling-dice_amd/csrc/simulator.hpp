@@ -131,8 +131,7 @@ class Simulator {
  public:
   explicit Simulator(SimulationConfig cfg);
   SimulationResult run();
-  // Manufacturing export is outside this build (the reference's STL path,
-  // MC/src/sim.cpp:579-888): throws std::runtime_error.
+  // MC/src/sim.cpp:645-888 on the device (mmc_export_stl)
   void exportSTL(const std::filesystem::path& path, int radialSegments = 12) const;
   bool rigidRotationTest(double* error = nullptr) const;
   bool gradientCheck(double* relativeError = nullptr) const;
@@ -207,6 +206,7 @@ class Driver {
   bool advance(SimulationResult& result);
   bool rigidRotationTest(double* error = nullptr) const;
   bool gradientCheck(double* relativeError = nullptr) const;
+  void exportSTL(const std::filesystem::path& path, int radialSegments, mmc_stl_report* report = nullptr) const;
   // committed state mirrored from the device (read on demand)
   const std::vector<Node>& nodes() const;
   const std::vector<Beam>& beams() const;

@@ -128,6 +128,16 @@ FRAME_DTYPE = np.dtype([(n, "<f8") for n in FRAME_FIELDS] + [
 assert FRAME_DTYPE.itemsize == C.sizeof(Frame) == 192
 
 
+class StlReport(C.Structure):
+    _fields_ = [("voxel", C.c_double), ("signed_volume", C.c_double), ("nx", C.c_int32), ("ny", C.c_int32),
+                ("nz", C.c_int32), ("watertight", C.c_int32)] + [
+        (n, C.c_int64) for n in ("triangles", "unique_edges", "initial_boundary_edges", "repaired_loops",
+                                 "boundary_edges", "nonmanifold_edges")]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
 class StateStats(C.Structure):
     _fields_ = [(n, C.c_double) for n in (
         "sum_axial", "max_axial", "sum_rot", "max_rot", "y_min", "y_max")] + [
@@ -295,6 +305,12 @@ def lib():
         L.mmc_sim_gradient_check.argtypes = [P, dp, C.POINTER(C.c_int32)]
         L.mmc_sim_context.argtypes = [P]
         L.mmc_sim_context.restype = P
+        L.mmc_export_stl.argtypes = [C.POINTER(Lattice), C.c_double, C.c_int32, C.c_char_p, C.POINTER(StlReport)]
+        L.mmc_stl_last_error.argtypes = []
+        L.mmc_stl_last_error.restype = C.c_char_p
+        L.mmc_stl_case_table.argtypes = [C.POINTER(C.c_int32), C.POINTER(C.c_int8)]
+        L.mmc_stl_case_table.restype = None
+        L.mmc_sim_export_stl.argtypes = [P, C.c_char_p, C.c_int32, C.POINTER(StlReport)]
         L.mmc_sim_begin.argtypes = [P]
         L.mmc_sim_advance.argtypes = [P, C.POINTER(Frame), C.c_int32, C.POINTER(C.c_int32),
                                       C.POINTER(C.c_int32)]
@@ -507,6 +523,12 @@ class Simulator:
     def context(self) -> int:
         return lib().mmc_sim_context(self.h)
 
+    def export_stl(self, path, segments=12) -> dict:
+        """Simulator::exportSTL on the device; returns the audit report."""
+        rep = StlReport()
+        self._check(lib().mmc_sim_export_stl(self.h, str(path).encode(), int(segments), C.byref(rep)))
+        return rep.as_dict()
+
     def read_state(self):
         """Committed base pose (x 3N, q 4N) and plastic states (7B), node-major."""
         x, q, p = np.zeros(3 * self.n_nodes), np.zeros(4 * self.n_nodes), np.zeros(7 * self.n_beams)
@@ -558,7 +580,15 @@ class Simulator:
 
 
 KERNELS = {"state": 0, "fill": 2, "contact": 4, "beam": 5, "gather": 6, "final": 7, "direction": 8, "pair": 9,
-           "dstate_g": 10, "elem_g": 11, "gather_g": 12, "decide_g": 13}
+           "dstate_g": 10, "elem_g": 11, "gather_g": 12, "decide_g": 13, "copy": 14}
+
+
+def stl_case_table():
+    """(counts[256], tris[256, 10, 3]) of the export's cell triangulation (host only)."""
+    counts = np.zeros(256, np.int32)
+    tris = np.zeros((256, 10, 3), np.int8)
+    lib().mmc_stl_case_table(counts.ctypes.data_as(C.POINTER(C.c_int32)), tris.ctypes.data_as(C.POINTER(C.c_int8)))
+    return counts, tris
 
 
 def device_name() -> str:

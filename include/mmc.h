@@ -196,7 +196,10 @@ int mmc_read_stats(mmc_ctx* ctx, mmc_state_stats* out);
 enum { MMC_K_STATE = 0, MMC_K_FILL = 2, MMC_K_CONTACT = 4, MMC_K_BEAM = 5, MMC_K_GATHER = 6,
        MMC_K_FINAL = 7, MMC_K_DIRECTION = 8, MMC_K_PAIR = 9,
        /* the four kernels of one device-loop trial (mmc_time_kernel only) */
-       MMC_K_DSTATE_G = 10, MMC_K_ELEM_G = 11, MMC_K_GATHER_G = 12, MMC_K_DECIDE_G = 13 };
+       MMC_K_DSTATE_G = 10, MMC_K_ELEM_G = 11, MMC_K_GATHER_G = 12, MMC_K_DECIDE_G = 13,
+       /* a plain device copy of 16 MiB between two buffers of its own, the
+          known-size reference for HBM traffic counters (mmc_time_kernel only) */
+       MMC_K_COPY = 14 };
 int mmc_kernel_timer(mmc_ctx* ctx, int32_t kernel);
 int mmc_kernel_timer_read(mmc_ctx* ctx, int64_t* launches, double* mean_ms);
 /* Objective evaluations performed by the context since creation. */
@@ -330,6 +333,34 @@ int mmc_batch_info(const mmc_batch* batch, int32_t i, int32_t* n_nodes, int32_t*
 int mmc_batch_run(mmc_batch* batch, mmc_frame* frames, int32_t cap, int32_t* n_frames, int32_t* status,
                   int32_t verbose);
 mmc_ctx* mmc_batch_context(mmc_batch* batch);
+
+/* ==== manufacturing export ================================================= */
+/* What the STL audit found (the fields of the reference's .validation.txt). */
+typedef struct mmc_stl_report {
+  double voxel;          /* m */
+  double signed_volume;  /* m^3, after orientation */
+  int32_t nx, ny, nz;    /* grid points */
+  int32_t watertight;    /* no boundary and no non-manifold edge */
+  int64_t triangles, unique_edges, initial_boundary_edges, repaired_loops, boundary_edges, nonmanifold_edges;
+} mmc_stl_report;
+/* Simulator::exportSTL, MC/src/sim.cpp:645-888: the union of the beams'
+ * capsules (radius[k]) and the joints' spheres (joint_radius) at the lattice's
+ * positions x, sampled on the reference's voxel grid (voxel from
+ * radial_segments) on the device, its zero set extracted on the device, audited
+ * and written as binary STL (mm) at path, with <path stem>.validation.txt.
+ * Uses only n_nodes, n_beams, x, beam_a, beam_b and radius of the lattice.
+ * Errors carry the reference's messages (mmc_stl_last_error). */
+int mmc_export_stl(const mmc_lattice* lattice, double joint_radius, int32_t radial_segments, const char* path,
+                   mmc_stl_report* report);
+const char* mmc_stl_last_error(void);
+/* The cell triangulation of the export (no reference counterpart): for each of
+ * the 256 inside/outside patterns of a cell's corners (corner c at
+ * (c & 1, c >> 1 & 1, c >> 2 & 1), inside = bit c), counts[case] triangles and
+ * tris[case][10][3] their edge ids (edge 4a + m joins the m-th corner without
+ * bit a to that corner with bit a).  Host only. */
+void mmc_stl_case_table(int32_t* counts, int8_t* tris);
+/* Simulator::exportSTL of the facade's lattice (initial positions). */
+int mmc_sim_export_stl(mmc_sim* sim, const char* path, int32_t radial_segments, mmc_stl_report* report);
 
 #ifdef __cplusplus
 }

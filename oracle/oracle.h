@@ -44,6 +44,8 @@ double orc_beam_potential(orc* h, const double* x, const double* q);
 /* Energy change z0 -> z1 at platen top: out[0] summed per term (the
  * line_search 1 form), out[1] as the difference of total energies. */
 void orc_energy_change(orc* h, const double* z0, const double* z1, double top, double* out);
+/* (reference harness only: ref_export_stl(h, path, segments), the reference's
+ * Simulator::exportSTL, for the STL goldens) */
 int orc_rigid_rotation_test(orc* h, double* err);
 int orc_gradient_check(orc* h, double* err);
 void orc_read_state(orc* h, double* x, double* q, double* plastic);

@@ -49,6 +49,8 @@ def _load(prefix: str):
     f("commit", [P])
     f("run", [P, C.POINTER(mmc.Frame), C.c_int32, i32p, C.c_int32])
     f("beam_potential", [P, dp, dp], C.c_double)
+    if prefix == "ref":
+        f("export_stl", [P, C.c_char_p, C.c_int32])
     if prefix == "orc":
         f("energy_change", [P, dp, dp, C.c_double, dp], None)
         f("set_frame_callback", [P, FRAME_CB, P], None)
@@ -182,6 +184,11 @@ class Solver:
     def beam_potential(self, x, q):
         return self._fn("beam_potential")(self.h, _dp(np.ascontiguousarray(x, np.float64)),
                                           _dp(np.ascontiguousarray(q, np.float64)))
+
+    def export_stl(self, path, segments=12):
+        """The reference's Simulator::exportSTL (reference build only)."""
+        if self._fn("export_stl")(self.h, str(path).encode(), int(segments)) != 0:
+            raise RuntimeError(self._fn("last_error")(self.h).decode())
 
     def rigid_rotation_test(self):
         e = C.c_double()

@@ -368,6 +368,16 @@ double ref_beam_potential(ref_h* h, const double* x, const double* q) {
   }
   return s.beamPotential(xx, RR);
 }
+int ref_export_stl(ref_h* h, const char* path, int32_t segments) {
+  try {
+    h->sim->exportSTL(path, segments);
+  } catch (const std::exception& e) {
+    h->err = e.what();
+    return MMC_ERR_STATE;
+  }
+  return 0;
+}
+
 int ref_rigid_rotation_test(ref_h* h, double* err) { return h->sim->rigidRotationTest(err) ? 1 : 0; }
 int ref_gradient_check(ref_h* h, double* err) { return h->sim->gradientCheck(err) ? 1 : 0; }
 void ref_read_state(ref_h* h, double* x, double* q, double* plastic) {

@@ -12,6 +12,7 @@ Cases (arguments as the reference's CLI takes them, MC/src/main.cpp:157-181):
                    --skip-stl)
   convergence      --mode convergence (coarse, medium and fine schedules of the
                    specimen)
+  stl_specimen     --mode stl on the specimen at STL quality 12
 """
 import shutil
 import subprocess
@@ -32,8 +33,12 @@ CASES = {
     "validate_quick": ["--mode", "validate", "--quick"],
     "sim_specimen": ["--mode", "sim"] + SPECIMEN + ["--skip-stl"],
     "convergence": ["--mode", "convergence"],
+    "stl_specimen": ["--mode", "stl", "--cells-x", "2", "--cells-y", "4", "--cells-z", "2", "--stl-quality", "12"],
 }
-TEXT = ("VALIDATION.txt", "compression_metrics.csv", "summary.json", "convergence.csv")
+# the mesh of stl_specimen is the one of tests/golden/stl/specimen_q12 (same
+# lattice and quality); only the text files are kept here
+TEXT = ("VALIDATION.txt", "compression_metrics.csv", "summary.json", "convergence.csv",
+        "graded_bcc_energy_absorber.validation.txt")
 
 
 def meta3_digest(path, snapshots):

@@ -72,7 +72,15 @@ def main():
     status_path = OUT / f"{name}_oracle_status.json"
     frames_path = OUT / f"{name}_oracle_frames.npy"
 
+    # a rerun replaces an earlier partial curve only once it holds more frames
+    # (or is complete)
+    have = 0
+    if frames_path.exists() and status_path.exists() and not json.loads(status_path.read_text())["complete"]:
+        have = len(np.load(frames_path))
+
     def save(frames, error=None, done=False):
+        if len(frames) <= have and not done:
+            return
         np.save(frames_path, frames, allow_pickle=False)
         status_path.write_text(json.dumps({
             "curve": name, "frames": int(len(frames)), "complete": done, "error": error,

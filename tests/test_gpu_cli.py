@@ -29,6 +29,7 @@ import numpy as np
 import pytest
 
 import meta3
+import stlfile
 from conftest import GOLDEN
 
 pytestmark = pytest.mark.gpu
@@ -43,6 +44,7 @@ CASES = {
     "validate_quick": ["--mode", "validate", "--quick"],
     "sim_specimen": ["--mode", "sim"] + SPECIMEN + ["--skip-stl"],
     "convergence": ["--mode", "convergence"],
+    "stl_specimen": ["--mode", "stl", "--cells-x", "2", "--cells-y", "4", "--cells-z", "2", "--stl-quality", "12"],
 }
 
 
@@ -182,3 +184,31 @@ def test_convergence_study_matches_reference(gpu, program, tmp_path):
         g = np.load(gold / f"convergence_{case}_meta3.npz")
         assert h.tobytes() == g["header"].tobytes(), case
         assert np.array_equal(d["engineering_strain"], g["diagnostics"]["engineering_strain"]), case
+
+
+@pytest.mark.parametrize("program", ["dropin", "cli"])
+def test_stl_mode_matches_reference(gpu, program, tmp_path):
+    """--mode stl on the specimen: the binary STL and its validation report
+    (the mesh against tests/golden/stl/specimen_q12, the same lattice and
+    quality: shared grid-edge vertices, watertight, volume within 0.5%; see
+    tests/test_gpu_stl.py) and VALIDATION.txt."""
+    out, stdout = run_case(program, "stl_specimen", tmp_path)
+    gold = CLI / "stl_specimen"
+    assert sorted(p.name for p in out.iterdir()) == sorted(
+        ["VALIDATION.txt", "graded_bcc_energy_absorber.stl", "graded_bcc_energy_absorber.validation.txt"])
+    rep = dict(ln.split("=", 1) for ln in (out / "graded_bcc_energy_absorber.validation.txt").read_text().splitlines()
+               if "=" in ln)
+    want = dict(ln.split("=", 1) for ln in (gold / "graded_bcc_energy_absorber.validation.txt").read_text().splitlines()
+                if "=" in ln)
+    assert list(rep) == list(want)
+    assert rep["voxel_mm"] == want["voxel_mm"] and rep["grid"] == want["grid"] and rep["watertight"] == "PASS"
+    d = stlfile.digest(out / "graded_bcc_energy_absorber.stl")
+    g = np.load(GOLDEN / "stl" / "specimen_q12.npz")
+    ours = {tuple(v) for v in d["vertices"].tolist()}
+    theirs = {tuple(v) for v in g["vertices"].tolist()}
+    print(program, "vertices", len(ours), len(theirs), "only ours", len(ours - theirs), "only reference",
+          len(theirs - ours))
+    assert len(ours - theirs) <= int(rep["repaired_boundary_loops"])
+    assert len(theirs - ours) <= int(want["repaired_boundary_loops"])
+    assert abs(float(d["volume_mm3"]) - float(g["volume_mm3"])) <= 5e-3 * float(g["volume_mm3"])
+    check_validation((out / "VALIDATION.txt").read_text(), (gold / "VALIDATION.txt").read_text(), program)

