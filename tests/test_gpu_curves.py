@@ -3,11 +3,16 @@ reference curves, at the parity tolerances of the solver's specification:
 platen force within 1e-6 of the peak force on every frame for the FP64 path and
 1e-5 for the mixed path, loading/dissipated work and permanent set likewise.
 
-The curves run with the parity form of each workload (gradient tolerance 1e-7,
-per-term Armijo test, relative escape off; mmc.tight): at the workload's own
-tolerance (1e-6, the reference's total-energy Armijo test) two correct solvers
-stop anywhere inside the band that tolerance leaves, ~1e-4 of the peak force on
-the 8x8x4 curve, so only the tighter solve can pin 1e-6.  Fixtures:
+The curves run with the parity form of each workload (per-term Armijo test
+with its approximate-Wolfe completion, relative escape off; mmc.tight) at a
+gradient tolerance tight enough that the band it leaves lies inside the bound
+(1e-8 for the cubic curves, 3e-8 for the contact-heavy family, 1e-9 for the
+batched sets and the soft BCC specimen; each test states how it was chosen,
+mostly from the oracle's own curve at a ten times tighter tolerance): at the
+workload's own tolerance
+(1e-6, the reference's total-energy Armijo test) two correct solvers stop
+anywhere inside the band that tolerance leaves, ~1e-4 of the peak force on the
+8x8x4 curve, so only a tighter solve can pin 1e-6.  Fixtures:
 tests/golden/*_oracle_frames.npy (the oracle, tests/golden/make_curves.py) and
 tests/golden/bcc_222_run_ref.npy (the reference compiled from its own sources,
 tests/golden/make_golden.py)."""
@@ -21,7 +26,7 @@ from conftest import GOLDEN
 pytestmark = pytest.mark.gpu
 
 # parity bounds, relative to the oracle curve's peak platen force (and for the
-# works to the oracle's final values; permanent set absolute, it is a fraction)
+# works and the permanent set, to the oracle's final values)
 BOUND = {"fp64": 1e-6, "mixed": 1e-5}
 
 
@@ -42,9 +47,12 @@ def curve_errors(fg, fo):
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_config1_tight_curve_matches_oracle(gpu, precision):
     """8x8x4 cubic lattice, 30% strain in 60 + 60 increments (with the oracle's
-    bisections), every frame against the oracle's curve."""
+    bisections), every frame against the oracle's curve, both at gradient
+    tolerance 1e-8: at 1e-7 the oracle's own unloading frames (62-69) stand up
+    to 5.8e-6 of the peak force from its 1e-8 curve, wider than the FP64
+    bound; its 1e-8 curve is within 5.3e-8 of its 1e-9 curve."""
     mmc = gpu
-    cfg = mmc.tight(mmc.cubic_config(8, 8, 4))
+    cfg = mmc.tight(mmc.cubic_config(8, 8, 4), tol=1e-8)
     fo = np.load(GOLDEN / "cubic_8x8x4_tight_oracle_frames.npy")
     prec = mmc.MMC_FP64 if precision == "fp64" else mmc.MMC_MIXED
     fg = mmc.Simulator(cfg, prec).run()
@@ -70,12 +78,15 @@ def test_published_specimen_run_matches_reference(gpu):
     loose tolerance (rms 1.2e-3 and max 1.5e-2 in gradient units, or the
     relative escape at 1.5e-3), which leaves the specimen far from equilibrium:
     the reference's own force at 5% strain is 44% above the equilibrium the
-    parity form finds (tests/golden/bcc_222_tight_oracle_frames.npy).  Two
+    parity form finds (tests/golden/bcc_222_equilibria_oracle.npz).  Two
     solvers that sum in different orders stop at different points of that
-    band; the stated band is the measured one with a margin: platen force
-    within 10% of the peak (measured 8.3%), loading work 10% (6.2%),
-    dissipated work 35% (30%), permanent set 8% relative (5.8%); the
-    schedule, the bisections and the convergence flags exactly."""
+    band.  The bands are the spread measured over six solvers that differ
+    only in summation order (the graph loop before and after a change of its
+    wave reductions, FP64 and mixed, the block solver FP64 and mixed, the
+    host-driven loop), with a margin: platen force within 20% of the peak
+    (measured 8.3-15.8%), loading work 12% (4.5-8.8%), dissipated work 50%
+    (22-39%), permanent set 8% relative (3.2-5.8%); the schedule, the
+    bisections and the convergence flags exactly."""
     mmc = gpu
     fr = np.load(GOLDEN / "bcc_222_run_ref.npy")
     fg = mmc.Simulator(mmc.specimen_config()).run()
@@ -85,8 +96,8 @@ def test_published_specimen_run_matches_reference(gpu):
     assert float(np.max(np.abs(fg["engineering_strain"] - fr["engineering_strain"]))) <= 1e-15
     err = curve_errors(fg, fr)
     print("bcc 2x4x2 vs reference", err)
-    assert err["force"] <= 0.10
-    assert err["loading_work"] <= 0.10 and err["dissipated_work"] <= 0.35
+    assert err["force"] <= 0.20
+    assert err["loading_work"] <= 0.12 and err["dissipated_work"] <= 0.50
     assert err["permanent_set"] <= 0.08
 
 
@@ -96,15 +107,16 @@ def test_published_specimen_equilibria_match_oracle(gpu, po, precision):
     reference's algorithm (the oracle equals the reference bit for bit on this
     specimen, tests/test_oracle_pin.py): the tare and the first two load
     increments of its schedule (0, 5% and 10% strain), each solved from the
-    committed state by both to gradient tolerance 1e-8 and committed (the
+    committed state by both to gradient tolerance 1e-9 and committed (the
     oracle's side: tests/golden/bcc_222_equilibria_oracle.npz).  The
     soft, nearly incompressible BCC material (E 26 MPa, nu 0.485) is 77 times
     more compliant than the cubic workloads, so the same absolute gradient
-    tolerance leaves a band 77 times wider: at 1e-7 the curves agree to 3.7e-6
-    of the force; 1e-8 pins the 1e-6."""
+    tolerance leaves a band 77 times wider: at 1e-8 the oracle's own tare
+    stands 1.7e-6 of its force from its 1e-9 value; 1e-9 pins the 1e-6
+    (GPU against oracle 1e-7 at the tare, 3e-8 and 2e-8 under load)."""
     mmc = gpu
     g = np.load(GOLDEN / "bcc_222_equilibria_oracle.npz")  # tests/golden/make_specimen_equilibria.py
-    cfg = mmc.tight(mmc.specimen_config().replace(max_iterations=40000), tol=1e-8)
+    cfg = mmc.tight(mmc.specimen_config().replace(max_iterations=80000), tol=1e-9)
     lat = po.Solver(cfg, "orc").lattice()
     prec = mmc.MMC_FP64 if precision == "fp64" else mmc.MMC_MIXED
     ctx = mmc.Context(lat, cfg, prec)
@@ -150,14 +162,18 @@ def _window_errors(fg, fo, n):
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_config2_window_matches_oracle(gpu, precision):
     """32x32x16 cubic lattice (18,513 nodes, 53,328 beams), the 30% schedule in
-    parity form: tare and the first 12 load increments (to 6% strain) against
-    the oracle's curve."""
+    parity form at gradient tolerance 1e-8: tare and the first 12 load
+    increments (to 6% strain) against the oracle's curve.  At 1e-7 (the
+    benchmark's tolerance) the two solvers agree to 1.1-1.4e-6 of the force
+    over this window, depending on the GPU's summation order: the band 1e-7
+    leaves is as wide as the FP64 bound."""
     mmc = gpu
-    fo = _fixture("cubic_32x32x16_tight")
+    fo = _fixture("cubic_32x32x16_tight8")
     n = 13
     assert len(fo) >= n
     prec = mmc.MMC_FP64 if precision == "fp64" else mmc.MMC_MIXED
-    fg = _advance_frames(mmc.Simulator(mmc.tight(mmc.cubic_config(32, 32, 16, max_iterations=20000)), prec), n)
+    cfg = mmc.tight(mmc.cubic_config(32, 32, 16, max_iterations=40000), tol=1e-8)
+    fg = _advance_frames(mmc.Simulator(cfg, prec), n)
     assert np.array_equal(fg["engineering_strain"], fo["engineering_strain"][:n])
     assert np.all(fg["converged"] == 1)
     force, lw = _window_errors(fg, fo, n)
@@ -170,7 +186,7 @@ def _batched(mmc, names, precision):
     cfgs = []
     for name in names:
         kind, idx, _ = name.split("_")
-        cfgs.append(mmc.tight(mmc.batch_workload(kind, 0, indices=[int(idx)])[0], tol=1e-8))
+        cfgs.append(mmc.tight(mmc.batch_workload(kind, 0, indices=[int(idx)])[0], tol=1e-9))
     prec = mmc.MMC_FP64 if precision == "fp64" else mmc.MMC_MIXED
     frames, status, err = mmc.BatchSimulator(cfgs, prec).run()
     assert status == [0] * len(cfgs), err
@@ -201,7 +217,11 @@ def test_config5_sweep_designs_match_oracle(gpu, precision):
     """Four designs of the 1,024-design sweep (6^3 cells, per-design E, yield
     and hardening, 40% strain, 60 + 60), solved as one batch, each against the
     oracle's curve: force, loading and dissipated work (the sweep's energy-
-    absorption figure) and permanent set."""
+    absorption figure) and permanent set.  Gradient tolerance 1e-9: the
+    unloaded lattice stands free under gravity alone, and its height (the
+    permanent set) is the softest equilibrium of the curve: at 1e-8 the
+    oracle's own set stands 3.6e-6 from its 1e-9 value (design 0, force
+    1.5e-8); at 1e-9 the GPU agrees with the oracle to 7e-9."""
     _batched(gpu, [f"sweep_{i}_tight" for i in (0, 1, 2, 3)], precision)
 
 

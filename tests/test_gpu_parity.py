@@ -110,8 +110,9 @@ def test_objective_locked_and_inf(gpu, po):
 
 def test_minimize_commit_parity(gpu, po, loop_engine):
     """Both solvers stop anywhere inside the gradient tolerance (1e-6): the
-    equilibria agree to that band (force ~1e-6, positions ~1e-7 relative), not
-    to round-off."""
+    equilibria agree to that band, not to round-off (force to 1e-5; positions
+    to 5e-6 relative, measured 1.5e-6 at most over two summation orders of the
+    GPU's reductions)."""
     mmc = gpu
     cfg = cases(mmc)["cubic_8x8x4"]
     o, ctx, lat = make_pair(mmc, po, cfg)
@@ -128,7 +129,7 @@ def test_minimize_commit_parity(gpu, po, loop_engine):
         ctx.commit()
         xo, qo, po_ = o.read_state()
         xg, qg, pg = ctx.read_state()
-        assert rel(xg, xo) <= 1e-6
+        assert rel(xg, xo) <= 5e-6
         assert np.max(np.abs(pg - po_)) <= 2e-4 * max(1e-3, np.max(np.abs(po_)))
 
 
@@ -329,12 +330,12 @@ def test_run_config1_mixed(gpu, po, loop_engine):
     assert err["loading_work"] <= 2e-5 and err["dissipated"] <= 5e-5
 
 
-@pytest.mark.parametrize("split", ["2", "4", "16"])
+@pytest.mark.parametrize("split", ["0", "2", "4", "16"])
 def test_launch_shapes_leave_results_unchanged(gpu, split, monkeypatch):
-    """The loop's tuner picks the decision kernel's split per context; it may
-    not change a result: the 8x8x4 curve (mixed policy, 12 + 4 increments)
-    with each fixed split equals, bit for bit, the curve at the untuned split
-    (8 blocks)."""
+    """The loop's tuner picks the decision kernel's split per context, or
+    fuses the decision into the gather kernel (split 0); it may not change a
+    result: the 8x8x4 curve (mixed policy, 12 + 4 increments) with each fixed
+    shape equals, bit for bit, the curve at the untuned split (8 blocks)."""
     mmc = gpu
     cfg = mmc.cubic_config(8, 8, 4, load=12, unload=4)
     monkeypatch.setenv("MMC_DEVICE_LOOP", "graph")
