@@ -55,7 +55,12 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per k_gather_g launch from rocprofv3 FETCH_SIZE + WRITE_SIZE passes on
 # the timed window, calibrated against a device copy of known size (commands
 # and figures in profiles/); keyed by lattice cells and gradient tolerance
-GATHER_TRAFFIC_BYTES: dict = {}
+GATHER_TRAFFIC_BYTES: dict = {
+    # 32x32x16, mixed policy (FP32 history), gradient tolerance 1e-7, at a full
+    # history: FETCH_SIZE 12,292.6 KiB x 1.998 + WRITE_SIZE 2,277.9 KiB x 1.000
+    # (profiles/k_gather_g_cubic32x32x16.md)
+    (32, 32, 16, 1e-7, "mixed"): 27481635,
+}
 
 
 def workload(args):
@@ -92,13 +97,16 @@ def oracle_fixture(args):
     return (np.load(path), name) if name and path.exists() else (None, None)
 
 
-def gather_loop_bytes(N, m):
+def gather_loop_bytes(N, B, m, hist_bytes):
     """Algorithmic HBM bytes of one k_gather_g launch at a full history (m - 1
-    retained pairs), per DOF element: the trial gradient and the new pair
-    s, y written (3 doubles), the m - 1 retained s_j, y_j read for the pair
-    products (2(m - 1)), and z, z_trial, g and the direction p read (4):
-    (2m + 5) doubles, 200 B at m = 10, over 6N DOFs."""
-    return 6 * N * (2 * m + 5) * 8
+    retained pairs).  Per DOF element (6N of them): z, z_trial, g and the
+    direction p read and the trial gradient written (5 doubles), the m - 1
+    retained s_j, y_j read for the pair products and the new pair s, y written
+    (2m values of the history's element type: 4 bytes under the mixed policy,
+    8 in FP64).  Per beam: its two end records read by the gather (2 x 6
+    doubles).  At m = 10: 112 B per DOF (mixed) or 200 B (FP64), plus 96 B per
+    beam."""
+    return 6 * N * (5 * 8 + 2 * m * hist_bytes) + 2 * B * 6 * 8
 
 
 def run_leg(cfg, prec, warmup, steps, log, keep_state=False):
@@ -394,9 +402,9 @@ def main():
     # counters (profiles/), and the copy bandwidth this device reaches
     copy_ms = sim.time_kernel("copy", 50)
     gather_ms = loop_us["gather_g"] * 1e-3
-    g_bytes = gather_loop_bytes(N, cfg.history)
+    g_bytes = gather_loop_bytes(N, B, cfg.history, 4 if prec == mmc.MMC_MIXED else 8)
     ach = g_bytes / (gather_ms * 1e-3) / 1e9
-    traffic = GATHER_TRAFFIC_BYTES.get((args.nx, args.ny, args.nz, args.grad_tol))
+    traffic = GATHER_TRAFFIC_BYTES.get((args.nx, args.ny, args.nz, args.grad_tol, args.precision))
     roof = {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": "k_gather_g",
             "bytes_per_launch": g_bytes, "mean_launch_us": round(gather_ms * 1e3, 3), "launches_timed": launches,
