@@ -1466,10 +1466,8 @@ __device__ __forceinline__ void finish_node(double (&q)[6], unsigned lk, double 
 }
 
 // p = -(cg g + sum_j cs_j s_j + cy_j y_j), locked and clamped; partial g.p
-// (H: the history element type; the sums run in double)
-template <class H>
 __device__ __forceinline__ void direction_body(int N, size_t n6, const double* __restrict__ g,
-                                                      const H* __restrict__ Sh, const H* __restrict__ Yh,
+                                                      const double* __restrict__ Sh, const double* __restrict__ Yh,
                                                       const DirCoef& C, double* __restrict__ p,
                                                       const uint8_t* __restrict__ lock, double mt, double mr,
                                                       double* __restrict__ part) {
@@ -1483,10 +1481,10 @@ __device__ __forceinline__ void direction_body(int N, size_t n6, const double* _
       q[d] = C.cg * gv[d];
     }
     for (int j = 0; j < C.k; j++) {
-      const H* s = Sh + n6 * C.slot[j];
-      const H* y = Yh + n6 * C.slot[j];
+      const double* s = Sh + n6 * C.slot[j];
+      const double* y = Yh + n6 * C.slot[j];
 #pragma unroll
-      for (int d = 0; d < 6; d++) q[d] += C.cs[j] * double(s[d * N + i]) + C.cy[j] * double(y[d * N + i]);
+      for (int d = 0; d < 6; d++) q[d] += C.cs[j] * s[d * N + i] + C.cy[j] * y[d * N + i];
     }
 #pragma unroll
     for (int d = 0; d < 6; d++) q[d] = -q[d];
@@ -1504,10 +1502,9 @@ __device__ __forceinline__ void direction_body(int N, size_t n6, const double* _
 // from LDS.  Same arithmetic per element as direction_body.  Writes p and
 // returns this element's direction (0 past the block's nodes) and g.p term.
 constexpr int kDirNodes = kElemNodes;
-template <class H>
 __device__ __forceinline__ double direction_elem(size_t e, bool on, bool locked, double gv, double cg, int k,
-                                                 const DirCoef& C, const H* __restrict__ Sh,
-                                                 const H* __restrict__ Yh, size_t n6, double mt, double mr,
+                                                 const DirCoef& C, const double* __restrict__ Sh,
+                                                 const double* __restrict__ Yh, size_t n6, double mt, double mr,
                                                  double* __restrict__ p, double& dv) {
   __shared__ double qs[6][kDirNodes];
   const int d = threadIdx.x / kDirNodes, l = threadIdx.x % kDirNodes;
@@ -1544,9 +1541,8 @@ __device__ __forceinline__ double direction_elem(size_t e, bool on, bool locked,
   return q;
 }
 
-template <class H>
 __global__ __launch_bounds__(kBlock) void k_direction(int N, size_t n6, const double* __restrict__ g,
-                                                      const H* __restrict__ Sh, const H* __restrict__ Yh,
+                                                      const double* __restrict__ Sh, const double* __restrict__ Yh,
                                                       DirCoef C, double* __restrict__ p,
                                                       const uint8_t* __restrict__ lock, double mt, double mr,
                                                       double* __restrict__ part) {
@@ -1556,27 +1552,24 @@ __global__ __launch_bounds__(kBlock) void k_direction(int N, size_t n6, const do
 
 // s = zc - z, y = gc - g into history slot t.  Partials [4*nold + 5][P]: for
 // each retained pair j: s_j.y, y_j.y, s_j.gc, y_j.gc; then s.y, s.s, y.y,
-// s.gc, y.gc.  H: the history element type; s and y are rounded to it before
-// their products, so that every product is of the stored pairs.
-template <class H>
+// s.gc, y.gc.
 __device__ __forceinline__ void pair_dots_body(int N, size_t n6, const double* __restrict__ z,
                                                       const double* __restrict__ zc, const double* __restrict__ g,
-                                                      const double* __restrict__ gc, H* __restrict__ Sh,
-                                                      H* __restrict__ Yh, const PairArgs& A, double* __restrict__ part,
+                                                      const double* __restrict__ gc, double* __restrict__ Sh,
+                                                      double* __restrict__ Yh, const PairArgs& A, double* __restrict__ part,
                                                       int i, bool on) {
   const int P = gridDim.x;
   double sv[6] = {0, 0, 0, 0, 0, 0}, yv[6] = {0, 0, 0, 0, 0, 0}, gv[6] = {0, 0, 0, 0, 0, 0};
   if (on) {
-    H* st = Sh + n6 * A.t;
-    H* yt = Yh + n6 * A.t;
+    double* st = Sh + n6 * A.t;
+    double* yt = Yh + n6 * A.t;
 #pragma unroll
     for (int d = 0; d < 6; d++) {
       gv[d] = gc[d * N + i];
-      const H sh = H(zc[d * N + i] - z[d * N + i]), yh = H(gv[d] - g[d * N + i]);
-      sv[d] = double(sh);
-      yv[d] = double(yh);
-      st[d * N + i] = sh;
-      yt[d * N + i] = yh;
+      sv[d] = zc[d * N + i] - z[d * N + i];
+      yv[d] = gv[d] - g[d * N + i];
+      st[d * N + i] = sv[d];
+      yt[d * N + i] = yv[d];
     }
   }
   // all 4*nold + 5 partial dots in registers (unrolled over the maximum
@@ -1589,11 +1582,11 @@ __device__ __forceinline__ void pair_dots_body(int N, size_t n6, const double* _
 #pragma unroll
     for (int jj = 0; jj < kMaxHist; jj++) {
       if (jj < A.nold) {
-        const H* sj = Sh + n6 * A.slot[jj];
-        const H* yj = Yh + n6 * A.slot[jj];
+        const double* sj = Sh + n6 * A.slot[jj];
+        const double* yj = Yh + n6 * A.slot[jj];
 #pragma unroll
         for (int d = 0; d < 6; d++) {
-          const double a = double(sj[d * N + i]), b = double(yj[d * N + i]);
+          const double a = sj[d * N + i], b = yj[d * N + i];
           acc[4 * jj + 0] += a * yv[d];
           acc[4 * jj + 1] += b * yv[d];
           acc[4 * jj + 2] += a * gv[d];
@@ -1629,11 +1622,10 @@ __device__ __forceinline__ void pair_dots_body(int N, size_t n6, const double* _
       for (int r = 0; r < 16 && c0 + r < K; r++) part[(c0 + r) * P + blockIdx.x] = v[r];
   }
 }
-template <class H>
 __global__ __launch_bounds__(kBlock) void k_pair_dots(int N, size_t n6, const double* __restrict__ z,
                                                       const double* __restrict__ zc, const double* __restrict__ g,
-                                                      const double* __restrict__ gc, H* __restrict__ Sh,
-                                                      H* __restrict__ Yh, PairArgs A, double* __restrict__ part) {
+                                                      const double* __restrict__ gc, double* __restrict__ Sh,
+                                                      double* __restrict__ Yh, PairArgs A, double* __restrict__ part) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
   pair_dots_body(N, n6, z, zc, g, gc, Sh, Yh, A, part, i, i < N);
 }
@@ -1912,7 +1904,7 @@ __global__ __launch_bounds__(kElemThreads) void k_dstate_g(Lat<T> L, const Inst*
                                                            const Blk* __restrict__ td, const Ctl* __restrict__ Cs,
                                                            double* __restrict__ z0, double* __restrict__ z1,
                                                            const double* __restrict__ g0, const double* __restrict__ g1,
-                                                           const T* __restrict__ Sh, const T* __restrict__ Yh,
+                                                           const double* __restrict__ Sh, const double* __restrict__ Yh,
                                                            size_t n6, double* __restrict__ p, double* __restrict__ x,
                                                            double* __restrict__ R, const double* __restrict__ xref,
                                                            double* __restrict__ nb0, double* __restrict__ nb1,
@@ -2271,8 +2263,8 @@ __global__ __launch_bounds__(kElemThreads) void k_gather_g(Lat<T> L, const Inst*
                                                            const double* __restrict__ cent,
                                                            const double* __restrict__ z0, const double* __restrict__ z1,
                                                            double* __restrict__ g0, double* __restrict__ g1,
-                                                           const double* __restrict__ p, T* __restrict__ Sh,
-                                                           T* __restrict__ Yh, size_t n6,
+                                                           const double* __restrict__ p, double* __restrict__ Sh,
+                                                           double* __restrict__ Yh, size_t n6,
                                                            double* __restrict__ part, FuseArgs fa) {
   const Blk bk = td[blockIdx.x];
   const Ctl* C = Cs + bk.inst;
@@ -2319,13 +2311,10 @@ __global__ __launch_bounds__(kElemThreads) void k_gather_g(Lat<T> L, const Inst*
   a[2] = on ? gd * pv : 0.0;
   int NR = 3;
   if (m > 0) {
-    // the pair in the history's element type; its products are of the
-    // rounded values, as every later product with it
-    const T sh = T(zcv - zv), yh = T(gd - gov);
-    const double sv = on ? double(sh) : 0.0, yv = on ? double(yh) : 0.0, gc = on ? gd : 0.0;
+    const double sv = on ? zcv - zv : 0.0, yv = on ? gd - gov : 0.0, gc = on ? gd : 0.0;
     if (on) {
-      Sh[size_t(spare) * n6 + e] = sh;
-      Yh[size_t(spare) * n6 + e] = yh;
+      Sh[size_t(spare) * n6 + e] = sv;
+      Yh[size_t(spare) * n6 + e] = yv;
     }
 #pragma unroll
     for (int jj = 0; jj < MH; jj++) {
@@ -2467,7 +2456,7 @@ template <class T, int MH>
 __global__ __launch_bounds__(kSolveThreads) void k_solve_block(
     Lat<T> L, const Inst* __restrict__ I, Ctl* __restrict__ Cs, const int* __restrict__ act, int max_trials,
     double* __restrict__ z0, double* __restrict__ z1, double* __restrict__ g0, double* __restrict__ g1,
-    T* __restrict__ Sh, T* __restrict__ Yh, size_t n6, double* __restrict__ p, double* __restrict__ x,
+    double* __restrict__ Sh, double* __restrict__ Yh, size_t n6, double* __restrict__ p, double* __restrict__ x,
     double* __restrict__ R, const double* __restrict__ xref, double* __restrict__ nb0, double* __restrict__ nb1,
     double* __restrict__ ub0, double* __restrict__ ub1, double* __restrict__ cb0, double* __restrict__ cb1,
     const unsigned* __restrict__ pcount, const ListRef* __restrict__ lr, double* __restrict__ rec,
@@ -2647,8 +2636,8 @@ __global__ __launch_bounds__(kSolveThreads) void k_solve_block(
           }
           a[2] += gd * p[e];
           if (m > 0) {
-            Sh[size_t(spare) * n6 + e] = T(zcand[e] - zcur[e]);
-            Yh[size_t(spare) * n6 + e] = T(gd - gcur[e]);
+            Sh[size_t(spare) * n6 + e] = zcand[e] - zcur[e];
+            Yh[size_t(spare) * n6 + e] = gd - gcur[e];
           }
         }
       }
@@ -2948,14 +2937,7 @@ struct mmc_ctx {
   double* hrec = nullptr;    // pinned readback of one record slot
   double* scal = nullptr;    // misc scalars (dots)
   double *z = nullptr, *g = nullptr, *p = nullptr, *zc = nullptr, *gc = nullptr, *zt = nullptr, *zin = nullptr;
-  // curvature pairs s, y: [m + 1][6N] of the element type (double; float
-  // under the mixed policy, which halves the history traffic of the
-  // direction and pair-product passes)
-  void *Sh = nullptr, *Yh = nullptr;
-  template <class H>
-  H* hs_() const { return static_cast<H*>(Sh); }
-  template <class H>
-  H* hy_() const { return static_cast<H*>(Yh); }
+  double *Sh = nullptr, *Yh = nullptr;  // [m + 1][6N]
   int hist_slots = 1;
   int max_hist = 0;  // longest history over the instances
   // device loop engine: the graph-replayed loop, or (MMC_DEVICE_LOOP=block,
@@ -3450,11 +3432,8 @@ void mmc_ctx::setup(const mmc_lattice* const* Ls, const mmc_config* cs, int n, i
     const char* eng = std::getenv("MMC_DEVICE_LOOP");
     block_engine = eng && std::strcmp(eng, "block") == 0 && max_n <= kBlockEngineNodes;
   }
-  {
-    const size_t hbytes = precision == MMC_MIXED ? sizeof(float) : sizeof(double);
-    Sh = dalloc<char>(n6 * hist_slots * hbytes);
-    Yh = dalloc<char>(n6 * hist_slots * hbytes);
-  }
+  Sh = dalloc<double>(n6 * hist_slots);
+  Yh = dalloc<double>(n6 * hist_slots);
   HIPCALL(hipHostMalloc((void**)&host, kHost * sizeof(double), hipHostMallocDefault));
   HIPCALL(hipHostMalloc((void**)&hctl, size_t(n) * sizeof(Ctl), hipHostMallocDefault));
   HIPCALL(hipHostMalloc((void**)&htops, size_t(n) * sizeof(double), hipHostMallocDefault));
@@ -3821,8 +3800,8 @@ void mmc_ctx::capture_iteration(Lat<T> Lt) {
   const size_t n6 = 6 * size_t(N);
   HIPCALL(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
   for (int u = 0; u < kGraphIterations; u++) {
-    k_dstate_g<T><<<gPd, kElemThreads, 0, stream>>>(Lt, inst_d, gtd, ctl, zbuf[0], zbuf[1], gbuf[0], gbuf[1], hs_<T>(),
-                                                    hy_<T>(), n6, p, x, (double*)R, xref, nbuf[0], nbuf[1], dspart);
+    k_dstate_g<T><<<gPd, kElemThreads, 0, stream>>>(Lt, inst_d, gtd, ctl, zbuf[0], zbuf[1], gbuf[0], gbuf[1], Sh,
+                                                    Yh, n6, p, x, (double*)R, xref, nbuf[0], nbuf[1], dspart);
     launch_elem<T>(Lt, ctl);
     launch_gather<T>(Lt, ctl);
     if (dec_split != 0) launch_decide(ctl);
@@ -3862,11 +3841,11 @@ void mmc_ctx::solve_blocks(Lat<T> Lt, const std::vector<int>& set, int max_trial
   const int nb = int(set.size());
   if (max_hist <= 10)
     k_solve_block<T, 10><<<nb, kSolveThreads, 0, stream>>>(
-        Lt, inst_d, ctl, gact, max_trials, zbuf[0], zbuf[1], gbuf[0], gbuf[1], hs_<T>(), hy_<T>(), n6, p, x, (double*)R, xref,
+        Lt, inst_d, ctl, gact, max_trials, zbuf[0], zbuf[1], gbuf[0], gbuf[1], Sh, Yh, n6, p, x, (double*)R, xref,
         nbuf[0], nbuf[1], ubuf[0], ubuf[1], cbuf[0], cbuf[1], pcount, lref, rec, cf_sph, cf_cap, iflags());
   else
     k_solve_block<T, kMaxHist><<<nb, kSolveThreads, 0, stream>>>(
-        Lt, inst_d, ctl, gact, max_trials, zbuf[0], zbuf[1], gbuf[0], gbuf[1], hs_<T>(), hy_<T>(), n6, p, x, (double*)R, xref,
+        Lt, inst_d, ctl, gact, max_trials, zbuf[0], zbuf[1], gbuf[0], gbuf[1], Sh, Yh, n6, p, x, (double*)R, xref,
         nbuf[0], nbuf[1], ubuf[0], ubuf[1], cbuf[0], cbuf[1], pcount, lref, rec, cf_sph, cf_cap, iflags());
   HIPCALL(hipGetLastError());
   // the set is staged from host memory that the caller may reuse
@@ -3924,7 +3903,7 @@ void mmc_ctx::launch_gather(Lat<T> Lt, Ctl* C) {
   const FuseArgs fa{grng, dspart, cpart, bpart, gPc, gPb * kBW, iflags(), dcount, drows};
   auto go = [&](auto kern) {
     kern<<<gPd, kElemThreads, 0, stream>>>(Lt, inst_d, gtd, C, x, erec, cf_sph, ecap, eS, cent, zbuf[0], zbuf[1],
-                                           gbuf[0], gbuf[1], p, hs_<T>(), hy_<T>(), n6, gpart, fa);
+                                           gbuf[0], gbuf[1], p, Sh, Yh, n6, gpart, fa);
   };
   if (max_hist <= 10) {
     if (dec_split == 0)
@@ -4089,7 +4068,7 @@ double mmc_ctx::time_loop_kernel(int kernel, int launches) {
       switch (kernel) {
         case MMC_K_DSTATE_G:
           k_dstate_g<T><<<gPd, kElemThreads, 0, stream>>>(Lt, inst_d, gtd, tctl, zbuf[0], zbuf[1], gbuf[0],
-                                                          gbuf[1], hs_<T>(), hy_<T>(), n6, p, x, (double*)R, xref, nbuf[0],
+                                                          gbuf[1], Sh, Yh, n6, p, x, (double*)R, xref, nbuf[0],
                                                           nbuf[1], dspart);
           break;
         case MMC_K_ELEM_G:
@@ -4498,11 +4477,7 @@ void mmc_ctx::minimize_host(double top, mmc_opt_stats& st) {
   static thread_local double SY[kMaxHist][kMaxHist], YY[kMaxHist][kMaxHist], uu[kMaxHist], vv[kMaxHist];
   auto direction = [&](const DirCoef& C) {
     tick(MMC_K_DIRECTION);
-    if (precision == MMC_MIXED)
-      k_direction<float><<<P, kBlock, 0, stream>>>(N, n6, g, hs_<float>(), hy_<float>(), C, p, lock, mt, mr, vpart);
-    else
-      k_direction<double><<<P, kBlock, 0, stream>>>(N, n6, g, hs_<double>(), hy_<double>(), C, p, lock, mt, mr,
-                                                    vpart);
+    k_direction<<<P, kBlock, 0, stream>>>(N, n6, g, Sh, Yh, C, p, lock, mt, mr, vpart);
     tock(MMC_K_DIRECTION);
     k_sum_rows<<<1, kBlock, 0, stream>>>(vpart, P, 1, scal);
     read(host + 32, scal, sizeof(double));
@@ -4545,10 +4520,7 @@ void mmc_ctx::minimize_host(double top, mmc_opt_stats& st) {
     if (m > 0) {  // curvature pair, MC/src/sim.cpp:205-206
       PairArgs A = pair_slots(m, count, head);
       tick(MMC_K_PAIR);
-      if (precision == MMC_MIXED)
-        k_pair_dots<float><<<P, kBlock, 0, stream>>>(N, n6, z, zc, g, gc, hs_<float>(), hy_<float>(), A, vpart);
-      else
-        k_pair_dots<double><<<P, kBlock, 0, stream>>>(N, n6, z, zc, g, gc, hs_<double>(), hy_<double>(), A, vpart);
+      k_pair_dots<<<P, kBlock, 0, stream>>>(N, n6, z, zc, g, gc, Sh, Yh, A, vpart);
       tock(MMC_K_PAIR);
       const int K = 4 * A.nold + 5;
       k_sum_rows<<<1, kBlock, 0, stream>>>(vpart, P, K, scal);
