@@ -56,10 +56,10 @@ HBM_PEAK_GBS = 8000.0
 # the timed window, calibrated against a device copy of known size (commands
 # and figures in profiles/); keyed by lattice cells and gradient tolerance
 GATHER_TRAFFIC_BYTES: dict = {
-    # 32x32x16, mixed policy (FP32 history), gradient tolerance 1e-7, at a full
-    # history: FETCH_SIZE 12,292.6 KiB x 1.998 + WRITE_SIZE 2,277.9 KiB x 1.000
+    # 32x32x16, gradient tolerance 1e-7, mixed policy, at a full history:
+    # FETCH_SIZE 16,126.4 KiB x 1.999 + WRITE_SIZE 3,127.6 KiB x 1.000
     # (profiles/k_gather_g_cubic32x32x16.md)
-    (32, 32, 16, 1e-7, "mixed"): 27481635,
+    (32, 32, 16, 1e-7, "mixed"): 36219501,
 }
 
 
@@ -97,16 +97,14 @@ def oracle_fixture(args):
     return (np.load(path), name) if name and path.exists() else (None, None)
 
 
-def gather_loop_bytes(N, B, m, hist_bytes):
+def gather_loop_bytes(N, B, m):
     """Algorithmic HBM bytes of one k_gather_g launch at a full history (m - 1
     retained pairs).  Per DOF element (6N of them): z, z_trial, g and the
     direction p read and the trial gradient written (5 doubles), the m - 1
     retained s_j, y_j read for the pair products and the new pair s, y written
-    (2m values of the history's element type: 4 bytes under the mixed policy,
-    8 in FP64).  Per beam: its two end records read by the gather (2 x 6
-    doubles).  At m = 10: 112 B per DOF (mixed) or 200 B (FP64), plus 96 B per
-    beam."""
-    return 6 * N * (5 * 8 + 2 * m * hist_bytes) + 2 * B * 6 * 8
+    (2m doubles).  Per beam: its two end records read by the gather (2 x 6
+    doubles).  At m = 10: 200 B per DOF plus 96 B per beam."""
+    return 6 * N * (5 + 2 * m) * 8 + 2 * B * 6 * 8
 
 
 def run_leg(cfg, prec, warmup, steps, log, keep_state=False):
@@ -402,7 +400,7 @@ def main():
     # counters (profiles/), and the copy bandwidth this device reaches
     copy_ms = sim.time_kernel("copy", 50)
     gather_ms = loop_us["gather_g"] * 1e-3
-    g_bytes = gather_loop_bytes(N, B, cfg.history, 4 if prec == mmc.MMC_MIXED else 8)
+    g_bytes = gather_loop_bytes(N, B, cfg.history)
     ach = g_bytes / (gather_ms * 1e-3) / 1e9
     traffic = GATHER_TRAFFIC_BYTES.get((args.nx, args.ny, args.nz, args.grad_tol, args.precision))
     roof = {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
