@@ -32,6 +32,7 @@
 #include <vector>
 
 #include "../../include/mmc.h"
+#include "capture.hpp"
 #include "kernels.hpp"
 
 using namespace mmc;
@@ -2896,6 +2897,14 @@ struct mmc_ctx {
   uint8_t* lock = nullptr;
   int *ba = nullptr, *bb = nullptr;
   double *L0 = nullptr, *rad = nullptr, *area = nullptr, *inert = nullptr, *e0 = nullptr, *plastic = nullptr;
+  // labels of the captured bodies (Node::layer, Beam::layer, Beam::family)
+  int *nlayer = nullptr, *blayer = nullptr, *bfam = nullptr;
+  // capture: previous committed poses (mmc_capture_mark), the bodies of the
+  // last capture and its lattice table
+  double *prev_x = nullptr, *prev_q = nullptr;
+  mmc_snapshot* snap = nullptr;
+  long snap_cap = 0;
+  CapInst* capt = nullptr;
   int *inc_off = nullptr, *inc = nullptr, *adj_off = nullptr, *adj = nullptr;
   // node-ordered gradient records of the device loop (EllOut): slot of each
   // beam end, [6][eS] end records, [3][eS] capsule contact forces on the ends
@@ -3109,7 +3118,8 @@ mmc_ctx::~mmc_ctx() {
                   rec, trial, cf_sph, cf_cap, box, cnt, start, entries, flags, scan_tmp, spart, cpart, bpart, gpart,
                   vpart, stb, stn, stout, record, scal, z, g, p, zc, gc, zt, zin, Sh, Yh, xref, pcount, plist,
                   inst_d, tn, tb, tc, td, sinst, tops_d, mask_d, smask_d, ctl, lref, dspart, cent, cflag, hlist, hcount, drows, dcount, gtd, gtb, gtc,
-                  grng, gact, tctl, rtc, rtn, eslot, erec, ecap, cal0, cal1};
+                  grng, gact, tctl, rtc, rtn, eslot, erec, ecap, cal0, cal1, nlayer, blayer, bfam, prev_x, prev_q,
+                  snap, capt};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   void* pinned[] = {host, hctl, htops, hmask, hrec, hstat};
@@ -3273,6 +3283,22 @@ void mmc_ctx::setup(const mmc_lattice* const* Ls, const mmc_config* cs, int n, i
   plastic = dalloc<double>(7 * size_t(B));
   soa(B, 7, false, [&](int b, int i, int d) { return Ls[b]->plastic ? Ls[b]->plastic[7 * size_t(i) + d] : 0.0; });
   up(plastic, h.data(), 7 * size_t(B));
+  {
+    std::vector<int> lab(std::max(N, B));
+    auto labels = [&](int*& dst, int count, bool nodes, const int32_t* mmc_lattice::*field) {
+      for (int b = 0; b < n; b++) {
+        const Inst& in = hinst[b];
+        const int32_t* src = Ls[b]->*field;
+        const int off = nodes ? in.n0 : in.b0, cntb = nodes ? in.N : in.B;
+        for (int i = 0; i < cntb; i++) lab[size_t(off + i)] = src ? src[i] : 0;
+      }
+      dst = dalloc<int>(count);
+      HIPCALL(hipMemcpy(dst, lab.data(), size_t(count) * sizeof(int), hipMemcpyHostToDevice));
+    };
+    labels(nlayer, N, true, &mmc_lattice::node_layer);
+    labels(blayer, B, false, &mmc_lattice::beam_layer);
+    labels(bfam, B, false, &mmc_lattice::beam_family);
+  }
   // ---- CSR beam incidence, ordered by beam index ----
   std::vector<int> off(N + 1, 0), list(2 * size_t(B));
   for (int k = 0; k < B; k++) {
@@ -4694,6 +4720,53 @@ int mmc_read_stats(mmc_ctx* c, mmc_state_stats* o) {
     return MMC_ERR_ARG;
   }
   return mmc_read_stats_batch(c, o);
+}
+
+int mmc_capture_mark(mmc_ctx* c) {
+  return guarded(c, [&] {
+    if (!c->prev_x) {
+      c->prev_x = dalloc<double>(3 * size_t(c->N));
+      c->prev_q = dalloc<double>(4 * size_t(c->N));
+    }
+    launch_capture_mark(c->bpos, c->bquat, c->prev_x, c->prev_q, c->N, c->stream);
+    HIPCALL(hipGetLastError());
+  });
+}
+
+int mmc_capture(mmc_ctx* c, const double* tops, const int32_t* mask, const int32_t* velocities, mmc_snapshot* out) {
+  return guarded(c, [&] {
+    if (!tops || !out) throw Fail(MMC_ERR_ARG, "NULL argument");
+    std::vector<CapInst> ci;
+    long total = 0;
+    int per = 0;
+    for (int b = 0; b < c->n_inst; b++) {
+      if (mask && !mask[b]) continue;
+      const bool vel = velocities && velocities[b];
+      if (vel && !c->prev_x) throw Fail(MMC_ERR_STATE, "no previous poses for the velocities (mmc_capture_mark)");
+      const Inst& in = c->hinst[size_t(b)];
+      const long bodies = long(in.B) + in.N + MMC_MACHINE_BODIES;
+      ci.push_back(CapInst{total, in.n0, in.N, in.b0, in.B, vel ? 1 : 0, tops[b],
+                           1.0 / std::max(1e-6, c->icfg[size_t(b)].step_time), in.S.jr});
+      total += bodies;
+      per = std::max(per, int((bodies + 255) / 256));
+    }
+    if (ci.empty()) return;
+    if (ci.size() > 65535) throw Fail(MMC_ERR_CAPACITY, "a capture takes at most 65,535 lattices");
+    if (total > c->snap_cap) {
+      if (c->snap) HIPCALL(hipFree(c->snap));
+      c->snap = nullptr;
+      c->snap = dalloc<mmc_snapshot>(size_t(total));
+      c->snap_cap = total;
+    }
+    if (!c->capt) c->capt = dalloc<CapInst>(size_t(c->n_inst));
+    HIPCALL(hipStreamSynchronize(c->stream));
+    HIPCALL(hipMemcpy(c->capt, ci.data(), ci.size() * sizeof(CapInst), hipMemcpyHostToDevice));
+    CapArgs a{c->capt, int(ci.size()), c->N, c->B, per, c->bpos, c->bquat, c->prev_x, c->prev_q, c->ba, c->bb,
+              c->rad, c->plastic, c->nlayer, c->blayer, c->bfam, c->snap};
+    launch_capture(a, c->stream);
+    HIPCALL(hipGetLastError());
+    c->read(out, c->snap, size_t(total) * sizeof(mmc_snapshot));
+  });
 }
 
 int mmc_kernel_timer(mmc_ctx* c, int32_t kernel) {

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstddef>
 #include <cstdlib>
 #include <cstring>
 #include <functional>
@@ -24,11 +25,9 @@ constexpr double kPi = 3.1415926535897932384626433832795;
 constexpr double kInf = std::numeric_limits<double>::infinity();
 
 Vec3 add(Vec3 a, Vec3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
-Vec3 sub(Vec3 a, Vec3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
 Vec3 mul(Vec3 a, double s) { return {a.x * s, a.y * s, a.z * s}; }
 double dot3(Vec3 a, Vec3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 Vec3 cross3(Vec3 a, Vec3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
-double len3(Vec3 a) { return std::sqrt(dot3(a, a)); }
 Vec3 unit3(Vec3 v) {
   double l2 = dot3(v, v);
   return (std::isfinite(l2) && l2 > 1e-30) ? mul(v, 1.0 / std::sqrt(l2)) : Vec3{0, 1, 0};
@@ -42,7 +41,6 @@ Quat qm(Quat a, Quat b) {
   return {a.w * b.w - a.x * b.x - a.y * b.y - a.z * b.z, a.w * b.x + a.x * b.w + a.y * b.z - a.z * b.y,
           a.w * b.y - a.x * b.z + a.y * b.w + a.z * b.x, a.w * b.z + a.x * b.y - a.y * b.x + a.z * b.w};
 }
-Quat qc(Quat q) { return {q.w, -q.x, -q.y, -q.z}; }
 Vec3 qr(Quat q, Vec3 v) {
   Vec3 u{q.x, q.y, q.z};
   Vec3 t = mul(cross3(u, v), 2.0);
@@ -52,43 +50,6 @@ Quat q_axis_angle(Vec3 axis, double angle) {
   axis = unit3(axis);
   double s = std::sin(0.5 * angle);
   return qn({std::cos(0.5 * angle), axis.x * s, axis.y * s, axis.z * s});
-}
-Vec3 q_log(Quat q) {  // MC/include/core.hpp:52
-  q = qn(q);
-  if (q.w < 0) q = {-q.w, -q.x, -q.y, -q.z};
-  double s = std::sqrt(q.x * q.x + q.y * q.y + q.z * q.z);
-  if (s < 1e-12) return {2 * q.x, 2 * q.y, 2 * q.z};
-  double a = 2 * std::atan2(s, std::max(-1.0, std::min(1.0, q.w)));
-  return mul(Vec3{q.x, q.y, q.z}, a / s);
-}
-// Rotation matrix columns -> quaternion (MC/include/core.hpp:43)
-Quat q_from_axes(Vec3 X, Vec3 Y, Vec3 Z) {
-  double m00 = X.x, m01 = Y.x, m02 = Z.x, m10 = X.y, m11 = Y.y, m12 = Z.y, m20 = X.z, m21 = Y.z, m22 = Z.z;
-  Quat q;
-  double tr = m00 + m11 + m22;
-  if (tr > 0) {
-    double s = std::sqrt(tr + 1) * 2;
-    q = {0.25 * s, (m21 - m12) / s, (m02 - m20) / s, (m10 - m01) / s};
-  } else if (m00 > m11 && m00 > m22) {
-    double s = std::sqrt(1 + m00 - m11 - m22) * 2;
-    q = {(m21 - m12) / s, 0.25 * s, (m01 + m10) / s, (m02 + m20) / s};
-  } else if (m11 > m22) {
-    double s = std::sqrt(1 + m11 - m00 - m22) * 2;
-    q = {(m02 - m20) / s, (m01 + m10) / s, 0.25 * s, (m12 + m21) / s};
-  } else {
-    double s = std::sqrt(1 + m22 - m00 - m11) * 2;
-    q = {(m10 - m01) / s, (m02 + m20) / s, (m12 + m21) / s, 0.25 * s};
-  }
-  return qn(q);
-}
-// Orientation whose x axis follows d (MC/src/sim.cpp:41)
-Quat align_x(Vec3 d) {
-  Vec3 X = unit3(d);
-  Vec3 h = std::abs(X.z) < 0.8 ? Vec3{0, 0, 1} : Vec3{0, 1, 0};
-  Vec3 Y = unit3(cross3(h, X));
-  Vec3 Z = unit3(cross3(X, Y));
-  Y = unit3(cross3(Z, X));
-  return q_from_axes(X, Y, Z);
 }
 uint64_t splitmix(uint64_t v) {
   v += 0x9e3779b97f4a7c15ULL;
@@ -475,81 +436,21 @@ double Driver::topY(double s, bool loading) const {
   return loading ? lat_.top0 * (1.0 - t) + low * t : low * (1.0 - t) + lat_.top0 * t;
 }
 
-// MC/src/sim.cpp:215-310
-std::vector<Snapshot> Driver::capture(double top, const std::vector<Vec3>* prevX,
-                                         const std::vector<Quat>* prevR) const {
-  sync();
-  std::vector<Snapshot> out;
-  out.reserve(beams_.size() + nodes_.size() + 10);
-  const double dt = std::max(1e-6, cfg_.stepTime);
-  auto vel = [&](int i) {
-    if (!prevX || prevX->size() != nodes_.size()) return Vec3{};
-    return mul(sub(nodes_[size_t(i)].x, (*prevX)[size_t(i)]), 1.0 / dt);
-  };
-  auto omega = [&](int i) {
-    if (!prevR || prevR->size() != nodes_.size()) return Vec3{};
-    Quat dq = qn(qm(nodes_[size_t(i)].R, qc((*prevR)[size_t(i)])));
-    return mul(q_log(dq), 1.0 / dt);
-  };
-  for (size_t bi = 0; bi < beams_.size(); ++bi) {
-    const Beam& e = beams_[bi];
-    const Vec3 a = nodes_[size_t(e.a)].x, b = nodes_[size_t(e.b)].x, d = sub(b, a);
-    const double L = std::max(1e-8, len3(d));
-    const double severity = std::max(std::abs(e.plastic.epsp) / 0.10, len3(e.plastic.rotp) / 0.42);
-    int material = 0;
-    if (severity > 0.28) material = 1;
-    if (severity > 0.62) material = 2;
-    if (severity > 1.05) material = 3;
-    Snapshot s;
-    s.p = mul(add(a, b), 0.5);
-    s.q = align_x(d);
-    s.v = mul(add(vel(e.a), vel(e.b)), 0.5);
-    s.w = mul(add(omega(e.a), omega(e.b)), 0.5);
-    s.half = {0.5 * L, e.radius, e.radius};
-    s.material = material;
-    s.id = int(bi);
-    s.kind = int(BodyKind::Strut);
-    s.moduleClass = e.family;
-    s.ring = e.layer;
-    s.slot = int(bi);
-    out.push_back(s);
-  }
-  const double jr = cfg_.jointScale * cfg_.maxRadius;
-  for (size_t ni = 0; ni < nodes_.size(); ++ni) {
-    Snapshot s;
-    s.p = nodes_[ni].x;
-    s.q = nodes_[ni].R;
-    s.v = vel(int(ni));
-    s.w = omega(int(ni));
-    s.half = Vec3{jr, jr, jr};
-    s.material = 4;
-    s.id = int(beams_.size() + ni);
-    s.kind = int(BodyKind::Joint);
-    s.moduleClass = nodes_[ni].layer;
-    s.ring = nodes_[ni].layer;
-    s.slot = int(ni);
-    out.push_back(s);
-  }
-  int id = int(beams_.size() + nodes_.size());
-  auto box = [&](Vec3 p, Vec3 half, int material, BodyKind kind) {
-    Snapshot s;
-    s.p = p;
-    s.q = Quat{};
-    s.half = half;
-    s.material = material;
-    s.id = id++;
-    s.kind = int(kind);
-    out.push_back(s);
-  };
-  // rigid test machine: platens at y = 0 and y = top, base, frame (MC/src/sim.cpp:297-307)
-  box({0, -0.0030, 0}, {0.026, 0.0030, 0.026}, 5, BodyKind::Platen);
-  box({0, top + 0.0030, 0}, {0.026, 0.0030, 0.026}, 5, BodyKind::Platen);
-  box({0, -0.0095, 0}, {0.036, 0.0035, 0.036}, 6, BodyKind::Machine);
-  const double columnY = 0.054;
-  box({-0.037, columnY, -0.049}, {0.0026, 0.061, 0.0026}, 6, BodyKind::Machine);
-  box({0.037, columnY, -0.049}, {0.0026, 0.061, 0.0026}, 6, BodyKind::Machine);
-  box({0, 0.111, -0.049}, {0.042, 0.0032, 0.0036}, 6, BodyKind::Machine);
-  box({0, top + 0.0080, -0.001}, {0.0105, 0.0025, 0.0105}, 7, BodyKind::Machine);
+// MC/src/sim.cpp:215-310, computed on the device from the committed state
+// (mmc_capture): the velocities are those against the poses stored by the
+// mmc_capture_mark before this increment's minimisation, or zero on the
+// schedule's first frame (the reference's prevX = prevR = nullptr).
+static_assert(sizeof(Snapshot) == sizeof(mmc_snapshot), "Snapshot and mmc_snapshot differ in size");
+static_assert(offsetof(Snapshot, q) == offsetof(mmc_snapshot, q) && offsetof(Snapshot, v) == offsetof(mmc_snapshot, v) &&
+                  offsetof(Snapshot, w) == offsetof(mmc_snapshot, w) &&
+                  offsetof(Snapshot, half) == offsetof(mmc_snapshot, half) &&
+                  offsetof(Snapshot, material) == offsetof(mmc_snapshot, material) &&
+                  offsetof(Snapshot, slot) == offsetof(mmc_snapshot, slot),
+              "Snapshot and mmc_snapshot differ in layout");
+std::vector<Snapshot> Driver::capture(double top, bool velocities) const {
+  std::vector<Snapshot> out(size_t(lat_.n_beams) + size_t(lat_.n_nodes) + MMC_MACHINE_BODIES);
+  const int32_t one = 1, vel = velocities ? 1 : 0;
+  check(mmc_capture(ctx_, &top, &one, &vel, reinterpret_cast<mmc_snapshot*>(out.data())));
   return out;
 }
 
@@ -616,17 +517,9 @@ void Driver::begin() {
 
 // One equilibrium increment with adaptive bisection, MC/src/sim.cpp:390-454.
 void Driver::advanceTo(double targetTop, bool loading, int refinement, SimulationResult& result) {
-  std::vector<Vec3> previousX;
-  std::vector<Quat> previousR;
-  if (ext_.captureFrames) {
-    sync();
-    previousX.resize(nodes_.size());
-    previousR.resize(nodes_.size());
-    for (size_t i = 0; i < nodes_.size(); ++i) {
-      previousX[i] = nodes_[i].x;
-      previousR[i] = nodes_[i].R;
-    }
-  }
+  // the committed poses before this increment, for the frame's velocities
+  // (MC/src/sim.cpp:391-397), kept on the device
+  if (ext_.captureFrames) check(mmc_capture_mark(ctx_));
   mmc_opt_stats stats{};
   mmc_objective_out fin{};
   check(mmc_minimize(ctx_, targetTop, &stats, &fin, nullptr));
@@ -660,7 +553,7 @@ void Driver::advanceTo(double targetTop, bool loading, int refinement, Simulatio
   prevForce_ = correctedForce;
   currentTop_ = targetTop;
   if (ext_.captureFrames)
-    result.frames.push_back(capture(targetTop, first_ ? nullptr : &previousX, first_ ? nullptr : &previousR));
+    result.frames.push_back(capture(targetTop, !first_));
   result.diagnostics.push_back(diagnostics(time_, targetTop, loading, fin, stats));
   const FrameDiagnostics& d = result.diagnostics.back();
   if (ext_.verbose)
@@ -923,7 +816,31 @@ bool per_lattice(const std::vector<mmc_config>& cfgs) {
 }  // namespace
 
 std::vector<BatchSimulator::Outcome> BatchSimulator::run(bool verbose) {
+  if (observer_) check(mmc_capture_mark(ctx_));
   return per_lattice(c_) ? run_per_lattice(verbose) : run_rounds(verbose);
+}
+
+void BatchSimulator::observe(const std::vector<int32_t>& mask, const std::vector<double>& tops,
+                             const std::vector<int>& frames, const std::vector<int32_t>& first) {
+  const int n = instances();
+  std::vector<int32_t> vel(size_t(n), 0);
+  int64_t total = 0;
+  for (int b = 0; b < n; b++)
+    if (mask[size_t(b)]) {
+      vel[size_t(b)] = !first[size_t(b)];
+      total += int64_t(lats_[size_t(b)].n_beams) + lats_[size_t(b)].n_nodes + MMC_MACHINE_BODIES;
+    }
+  if (total == 0) return;
+  bodies_.resize(size_t(total));
+  check(mmc_capture(ctx_, tops.data(), mask.data(), vel.data(), bodies_.data()));
+  check(mmc_capture_mark(ctx_));
+  int64_t o = 0;
+  for (int b = 0; b < n; b++)
+    if (mask[size_t(b)]) {
+      const int64_t count = int64_t(lats_[size_t(b)].n_beams) + lats_[size_t(b)].n_nodes + MMC_MACHINE_BODIES;
+      observer_(b, frames[size_t(b)], bodies_.data() + o, count);
+      o += count;
+    }
 }
 
 // Rounds: every lattice with a target solves in one mmc_minimize_batch; the
@@ -940,6 +857,8 @@ std::vector<BatchSimulator::Outcome> BatchSimulator::run_rounds(bool verbose) {
   std::vector<mmc_opt_stats> stats(nz);
   std::vector<mmc_objective_out> fin(nz);
   std::vector<mmc_state_stats> st(nz);
+  std::vector<int32_t> first(nz);
+  std::vector<int> nframe(nz);
   const auto t_run = std::chrono::steady_clock::now();
   for (;;) {
     int any = 0;
@@ -981,10 +900,14 @@ std::vector<BatchSimulator::Outcome> BatchSimulator::run_rounds(bool verbose) {
     check(mmc_commit_batch(ctx_, mask.data()));
     check(mmc_read_stats_batch(ctx_, st.data()));
     phase_s_[1] += std::chrono::duration<double>(std::chrono::steady_clock::now() - t_commit).count();
-    for (int b = 0; b < n; b++)
+    for (int b = 0; b < n; b++) {
+      first[size_t(b)] = sch[size_t(b)].first;
       if (mask[size_t(b)])
         record(sch[size_t(b)], cfgs_[size_t(b)], lats_[size_t(b)], fin[size_t(b)], stats[size_t(b)], st[size_t(b)],
                out[size_t(b)]);
+      nframe[size_t(b)] = int(out[size_t(b)].result.diagnostics.size()) - 1;
+    }
+    if (observer_) observe(mask, tops, nframe, first);
   }
   phase_s_[2] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_run).count();
   if (verbose)
@@ -1009,6 +932,9 @@ std::vector<BatchSimulator::Outcome> BatchSimulator::run_per_lattice(bool verbos
   std::vector<mmc_opt_stats> stats(nz);
   std::vector<mmc_objective_out> fin(nz);
   std::vector<mmc_state_stats> st(nz);
+  std::vector<int32_t> first(nz);
+  std::vector<int> nframe(nz);
+  std::vector<double> ctop(nz);
   const auto t_run = std::chrono::steady_clock::now();
   // the lattices of `start` whose schedules go on begin their next solves
   auto launch = [&] {
@@ -1065,10 +991,17 @@ std::vector<BatchSimulator::Outcome> BatchSimulator::run_per_lattice(bool verbos
       check(mmc_commit_batch(ctx_, mask.data()));
       check(mmc_read_stats_batch(ctx_, st.data()));
       phase_s_[1] += std::chrono::duration<double>(std::chrono::steady_clock::now() - t_commit).count();
-      for (int b = 0; b < n; b++)
-        if (mask[size_t(b)])
+      for (int b = 0; b < n; b++) {
+        first[size_t(b)] = sch[size_t(b)].first;
+        if (mask[size_t(b)]) {
+          // the committed target, for the frame's moving platen
+          ctop[size_t(b)] = sch[size_t(b)].stack.back().top;
           record(sch[size_t(b)], cfgs_[size_t(b)], lats_[size_t(b)], fin[size_t(b)], stats[size_t(b)],
                  st[size_t(b)], out[size_t(b)]);
+        }
+        nframe[size_t(b)] = int(out[size_t(b)].result.diagnostics.size()) - 1;
+      }
+      if (observer_) observe(mask, ctop, nframe, first);
     }
     in_flight += launch();
   }
@@ -1294,6 +1227,17 @@ int mmc_batch_run(mmc_batch* s, mmc_frame* frames, int32_t cap, int32_t* n_frame
     s->err = e.what();
     return MMC_ERR_HIP;
   }
+}
+
+int mmc_batch_set_frame_observer(mmc_batch* s, mmc_batch_frame_fn fn, void* user) {
+  if (!s || !s->sim) return MMC_ERR_ARG;
+  if (fn)
+    s->sim->setFrameObserver([fn, user](int lattice, int frame, const mmc_snapshot* bodies, int64_t count) {
+      fn(int32_t(lattice), int32_t(frame), bodies, count, user);
+    });
+  else
+    s->sim->setFrameObserver(nullptr);
+  return MMC_OK;
 }
 
 mmc_ctx* mmc_batch_context(mmc_batch* s) { return (s && s->sim) ? s->sim->context() : nullptr; }

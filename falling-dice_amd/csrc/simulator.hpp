@@ -19,6 +19,7 @@
 #pragma once
 #include <cstdint>
 #include <filesystem>
+#include <functional>
 #include <limits>
 #include <memory>
 #include <string>
@@ -243,7 +244,7 @@ class Driver {
   void check(int rc) const;
   void sync() const;
   double topY(double s, bool loading) const;
-  std::vector<Snapshot> capture(double top, const std::vector<Vec3>* prevX, const std::vector<Quat>* prevR) const;
+  std::vector<Snapshot> capture(double top, bool velocities) const;
   FrameDiagnostics diagnostics(double time, double top, bool loading, const mmc_objective_out& o,
                                const mmc_opt_stats& s) const;
 };
@@ -273,8 +274,19 @@ class BatchSimulator {
   const HostLattice& lattice(int i) const { return lats_[size_t(i)]; }
   mmc_ctx* context() const { return ctx_; }
   int64_t rounds() const { return rounds_; }
+  // Per-frame bodies of every lattice (mmc_batch_set_frame_observer); empty:
+  // no capture.
+  using FrameObserver = std::function<void(int lattice, int frame, const mmc_snapshot* bodies, int64_t count)>;
+  void setFrameObserver(FrameObserver f) { observer_ = std::move(f); }
 
  private:
+  FrameObserver observer_;
+  std::vector<mmc_snapshot> bodies_;  // host staging of one capture
+  // The frames of the lattices in `mask` just committed (velocities unless
+  // it is a lattice's first frame), handed to the observer; then the
+  // committed poses of every lattice become the previous poses.
+  void observe(const std::vector<int32_t>& mask, const std::vector<double>& tops, const std::vector<int>& frames,
+               const std::vector<int32_t>& first);
   std::vector<SimulationConfig> cfgs_;
   std::vector<mmc_config> c_;
   std::vector<HostLattice> lats_;

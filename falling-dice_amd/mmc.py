@@ -127,6 +127,15 @@ FRAME_DTYPE = np.dtype([(n, "<f8") for n in FRAME_FIELDS] + [
     ("iterations", "<i4"), ("converged", "<u4"), ("loading", "<u4")])
 assert FRAME_DTYPE.itemsize == C.sizeof(Frame) == 192
 
+# One captured body (mmc_snapshot; DiskSnapshot of the reference's META3 cache)
+SNAPSHOT_DTYPE = np.dtype([("p", "<f8", (3,)), ("q", "<f8", (4,)), ("v", "<f8", (3,)), ("w", "<f8", (3,)),
+                           ("half", "<f8", (3,)), ("material", "<i4"), ("id", "<i4"), ("kind", "<i4"),
+                           ("module_class", "<i4"), ("ring", "<i4"), ("slot", "<i4")])
+assert SNAPSHOT_DTYPE.itemsize == 152
+MACHINE_BODIES = 7
+# mmc_batch_frame_fn(lattice, frame, bodies, count, user)
+FRAME_OBSERVER = C.CFUNCTYPE(None, C.c_int32, C.c_int32, C.c_void_p, C.c_int64, C.c_void_p)
+
 
 class StlReport(C.Structure):
     _fields_ = [("voxel", C.c_double), ("signed_volume", C.c_double), ("nx", C.c_int32), ("ny", C.c_int32),
@@ -290,6 +299,8 @@ def lib():
         L.mmc_read_stats.argtypes = [P, C.POINTER(StateStats)]
         L.mmc_beam_potential.argtypes = [P, dp, dp, dp]
         L.mmc_beam_potential_noalloc.argtypes = [P, P, P, P, P]
+        L.mmc_capture.argtypes = [P, dp, C.POINTER(C.c_int32), C.POINTER(C.c_int32), P]
+        L.mmc_capture_mark.argtypes = [P]
         L.mmc_config_defaults.argtypes = [C.POINTER(Config)]
         L.mmc_config_defaults.restype = None
         L.mmc_sim_create.argtypes = [C.POINTER(Config), C.c_int32, C.POINTER(P)]
@@ -328,6 +339,7 @@ def lib():
         L.mmc_batch_info.argtypes = [P, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32), dp, dp, dp]
         L.mmc_batch_run.argtypes = [P, C.POINTER(Frame), C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                     C.c_int32]
+        L.mmc_batch_set_frame_observer.argtypes = [P, FRAME_OBSERVER, P]
         L.mmc_batch_context.argtypes = [P]
         L.mmc_batch_context.restype = P
         L.mmc_lattice_build.argtypes = [C.POINTER(Config), C.POINTER(Lattice), C.POINTER(P)]
@@ -458,6 +470,27 @@ class Context:
         q = np.ascontiguousarray(q, dtype=np.float64)
         self._check(lib().mmc_beam_potential(self.h, _dp(x), _dp(q), _dp(out)))
         return float(out[0])
+
+    def capture_mark(self):
+        """Stores the committed poses as the previous poses of the next capture."""
+        self._check(lib().mmc_capture_mark(self.h))
+
+    def capture(self, top, velocities=False):
+        """The bodies of the committed frame (SNAPSHOT_DTYPE, B + N + 7), with
+        velocities against the poses of the last capture_mark when asked."""
+        return capture(self.h, [top], [1], [int(velocities)], self.n_beams + self.n_nodes + MACHINE_BODIES)
+
+
+def capture(ctx, tops, mask, velocities, total):
+    """mmc_capture on a context handle: `total` bodies of the masked lattices."""
+    out = np.zeros(total, SNAPSHOT_DTYPE)
+    t = np.ascontiguousarray(tops, dtype=np.float64)
+    m = (C.c_int32 * len(mask))(*mask)
+    v = (C.c_int32 * len(velocities))(*velocities)
+    rc = lib().mmc_capture(ctx, _dp(t), m, v, out.ctypes.data_as(C.c_void_p))
+    if rc != 0:
+        raise MMCError(lib().mmc_last_error(ctx).decode())
+    return out
 
 
 class Simulator:
@@ -634,8 +667,24 @@ class BatchSimulator:
     def evaluations(self) -> int:
         return int(lib().mmc_evaluations(self.context()))
 
-    def run(self, verbose=False):
-        """Returns (frames per lattice, status per lattice, first error text)."""
+    def run(self, verbose=False, on_bodies=None):
+        """Returns (frames per lattice, status per lattice, first error text).
+        on_bodies(lattice, frame, bodies) receives every committed frame's
+        bodies (SNAPSHOT_DTYPE) as the device captures them."""
+        cb = None
+        if on_bodies is not None:
+            def observe(lattice, frame, ptr, count, _user):
+                raw = C.string_at(ptr, int(count) * SNAPSHOT_DTYPE.itemsize)
+                on_bodies(int(lattice), int(frame), np.frombuffer(raw, SNAPSHOT_DTYPE).copy())
+            cb = FRAME_OBSERVER(observe)
+            lib().mmc_batch_set_frame_observer(self.h, cb, None)
+        try:
+            return self._run(verbose)
+        finally:
+            if cb is not None:
+                lib().mmc_batch_set_frame_observer(self.h, FRAME_OBSERVER(), None)
+
+    def _run(self, verbose):
         cap = 4 * max(c.load_steps + c.unload_steps + 1 for c in self.cfgs) + 256
         frames = (Frame * (cap * self.n))()
         nf = (C.c_int32 * self.n)()

@@ -148,6 +148,18 @@ typedef struct mmc_frame {
  * bendPlastic, torsionPlastic (flags as 0/1). */
 #define MMC_TRIAL_STRIDE 17
 
+/* One body of a captured frame: the byte layout of DiskSnapshot,
+ * MC/src/main.cpp:19-22, which is the layout of Snapshot, MC/include/sim.hpp:7
+ * (position, orientation w,x,y,z, velocity, angular velocity, half extents,
+ * render material, id, BodyKind, module class, ring, slot). */
+typedef struct mmc_snapshot {
+  double p[3], q[4], v[3], w[3], half[3];
+  int32_t material, id, kind, module_class, ring, slot;
+} mmc_snapshot;
+/* Rigid machine bodies that close every captured frame (MC/src/sim.cpp:297-307):
+ * a lattice of N nodes and B beams captures B + N + MMC_MACHINE_BODIES bodies. */
+#define MMC_MACHINE_BODIES 7
+
 /* Plastic-state and geometry reductions of diagnostics(), MC/src/sim.cpp:339-370. */
 typedef struct mmc_state_stats {
   double sum_axial, max_axial, sum_rot, max_rot, y_min, y_max;
@@ -188,6 +200,24 @@ int mmc_read_state(mmc_ctx* ctx, double* x, double* q, double* plastic);
 int mmc_read_trial(mmc_ctx* ctx, double* trial);
 /* Device reductions over the committed state for diagnostics (sim.cpp:339-370). */
 int mmc_read_stats(mmc_ctx* ctx, mmc_state_stats* out);
+
+/* Simulator::capture(top, prevX, prevR), MC/src/sim.cpp:215-310, computed on the
+ * device from the committed state: per lattice, one strut body per beam (beam
+ * order), one joint body per node (node order), then the seven machine bodies
+ * of the platen at tops[b].  Lattice b takes part when mask[b] != 0 (all when
+ * mask is NULL); the bodies of the lattices taking part follow one another in
+ * lattice order in out (B_b + N_b + MMC_MACHINE_BODIES each).  With
+ * velocities[b] != 0 the bodies carry the velocities of the reference's
+ * prevX/prevR form against the poses stored by the last mmc_capture_mark
+ * ((x - x_prev) / dt and quatLog(normalized(R conj(R_prev))) / dt, dt =
+ * max(1e-6, step_time)); with velocities NULL or velocities[b] == 0 they carry
+ * zeros (the reference's first frame, prevX = prevR = nullptr). */
+int mmc_capture(mmc_ctx* ctx, const double* tops, const int32_t* mask, const int32_t* velocities,
+                mmc_snapshot* out);
+/* Stores the committed node poses of every lattice as the previous poses of
+ * the next mmc_capture (the copies of nodes_[i].x and .R that advanceTo keeps
+ * before each minimisation, MC/src/sim.cpp:391-397); on the device. */
+int mmc_capture_mark(mmc_ctx* ctx);
 
 /* Kernel timer (no reference counterpart): HIP events recorded on the context
  * stream around every launch of one objective kernel (MMC_K_*) while enabled.
@@ -336,6 +366,16 @@ int mmc_batch_info(const mmc_batch* batch, int32_t i, int32_t* n_nodes, int32_t*
 int mmc_batch_run(mmc_batch* batch, mmc_frame* frames, int32_t cap, int32_t* n_frames, int32_t* status,
                   int32_t verbose);
 mmc_ctx* mmc_batch_context(mmc_batch* batch);
+/* Frame observer of mmc_batch_run (no reference counterpart; the batched form
+ * of SimulationResult::frames, MC/src/sim.cpp:436): after every committed
+ * increment, fn(lattice, frame, bodies, count, user) receives that lattice's
+ * frame as Simulator::capture gives it (B + N + MMC_MACHINE_BODIES bodies,
+ * velocities against the lattice's previous frame, none on its first),
+ * captured on the device for all lattices committed together; `bodies` is
+ * valid during the call only.  fn NULL (the default) captures nothing. */
+typedef void (*mmc_batch_frame_fn)(int32_t lattice, int32_t frame, const mmc_snapshot* bodies, int64_t count,
+                                   void* user);
+int mmc_batch_set_frame_observer(mmc_batch* batch, mmc_batch_frame_fn fn, void* user);
 
 /* ==== manufacturing export ================================================= */
 /* What the STL audit found (the fields of the reference's .validation.txt). */

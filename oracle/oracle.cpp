@@ -1487,6 +1487,104 @@ void orc_set_pose(orc* h, const double* x, const double* q) {
   }
 }
 
+namespace {
+// quatFromAxes, MC/include/core.hpp:43: the rotation with columns X, Y, Z
+Q4 from_axes(V3 X, V3 Y, V3 Z) {
+  double m00 = X.x, m01 = Y.x, m02 = Z.x, m10 = X.y, m11 = Y.y, m12 = Z.y, m20 = X.z, m21 = Y.z, m22 = Z.z;
+  Q4 q;
+  double tr = m00 + m11 + m22;
+  if (tr > 0) {
+    double s = std::sqrt(tr + 1) * 2;
+    q = {0.25 * s, (m21 - m12) / s, (m02 - m20) / s, (m10 - m01) / s};
+  } else if (m00 > m11 && m00 > m22) {
+    double s = std::sqrt(1 + m00 - m11 - m22) * 2;
+    q = {(m21 - m12) / s, 0.25 * s, (m01 + m10) / s, (m02 + m20) / s};
+  } else if (m11 > m22) {
+    double s = std::sqrt(1 + m11 - m00 - m22) * 2;
+    q = {(m02 - m20) / s, (m01 + m10) / s, 0.25 * s, (m12 + m21) / s};
+  } else {
+    double s = std::sqrt(1 + m22 - m00 - m11) * 2;
+    q = {(m10 - m01) / s, (m02 + m20) / s, (m12 + m21) / s, 0.25 * s};
+  }
+  return qnorm(q);
+}
+// alignX, MC/src/sim.cpp:41, with orthogonal, MC/include/core.hpp:47
+Q4 align_x(V3 d) {
+  V3 X = unit(d);
+  V3 h = std::abs(X.z) < .8 ? V3{0, 0, 1} : V3{0, 1, 0};
+  V3 Y = unit(crossv(h, X));
+  V3 Z = unit(crossv(X, Y));
+  Y = unit(crossv(Z, X));
+  return from_axes(X, Y, Z);
+}
+void put(mmc_snapshot& s, V3 p, Q4 q, V3 v, V3 w, V3 half, int material, int id, int kind, int module_class,
+         int ring, int slot) {
+  const double vals[16] = {p.x, p.y, p.z, q.w, q.x, q.y, q.z, v.x, v.y, v.z, w.x, w.y, w.z, half.x, half.y, half.z};
+  std::memcpy(s.p, vals, 3 * sizeof(double));
+  std::memcpy(s.q, vals + 3, 4 * sizeof(double));
+  std::memcpy(s.v, vals + 7, 3 * sizeof(double));
+  std::memcpy(s.w, vals + 10, 3 * sizeof(double));
+  std::memcpy(s.half, vals + 13, 3 * sizeof(double));
+  s.material = material;
+  s.id = id;
+  s.kind = kind;
+  s.module_class = module_class;
+  s.ring = ring;
+  s.slot = slot;
+}
+}  // namespace
+
+// Simulator::capture(top, prevX, prevR), MC/src/sim.cpp:215-310: struts (beam
+// order), joints (node order), then the rigid machine; velocities from the
+// previous poses prev_x (3N) / prev_q (4N), zero when they are NULL.
+void orc_capture(orc* h, double top, const double* prev_x, const double* prev_q, mmc_snapshot* out) {
+  const std::vector<Node>& nodes = h->nodes;
+  const std::vector<Beam>& beams = h->beams;
+  const double dt = std::max(1e-6, h->cfg.step_time);
+  auto vel = [&](int i) {
+    if (!prev_x) return V3{};
+    V3 px{prev_x[3 * size_t(i)], prev_x[3 * size_t(i) + 1], prev_x[3 * size_t(i) + 2]};
+    return divs(sub(nodes[size_t(i)].x, px), dt);
+  };
+  auto omega = [&](int i) {
+    if (!prev_q) return V3{};
+    const double* r = prev_q + 4 * size_t(i);
+    Q4 dq = qnorm(qmul(nodes[size_t(i)].R, qconj(Q4{r[0], r[1], r[2], r[3]})));
+    return divs(qlog(dq), dt);
+  };
+  size_t o = 0;
+  for (size_t bi = 0; bi < beams.size(); bi++) {
+    const Beam& e = beams[bi];
+    V3 a = nodes[size_t(e.a)].x, b = nodes[size_t(e.b)].x, d = sub(b, a);
+    double L = std::max(1e-8, len(d));
+    double pAxial = std::abs(e.plastic.epsp), pRot = len(e.plastic.rotp);
+    double severity = std::max(pAxial / 0.10, pRot / 0.42);
+    int material = 0;
+    if (severity > 0.28) material = 1;
+    if (severity > 0.62) material = 2;
+    if (severity > 1.05) material = 3;
+    put(out[o++], scl(add(a, b), 0.5), align_x(d), scl(add(vel(e.a), vel(e.b)), 0.5),
+        scl(add(omega(e.a), omega(e.b)), 0.5), V3{0.5 * L, e.radius, e.radius}, material, int(bi), 0, e.family,
+        e.layer, int(bi));
+  }
+  const double J = h->jr();
+  for (size_t ni = 0; ni < nodes.size(); ni++)
+    put(out[o++], nodes[ni].x, nodes[ni].R, vel(int(ni)), omega(int(ni)), V3{J, J, J}, 4,
+        int(beams.size() + ni), 1, nodes[ni].layer, nodes[ni].layer, int(ni));
+  int id = int(beams.size() + nodes.size());
+  auto box = [&](V3 p, V3 half, int material, int kind) {
+    put(out[o++], p, Q4{}, V3{}, V3{}, half, material, id++, kind, -1, -1, -1);
+  };
+  box({0, -0.0030, 0}, {0.026, 0.0030, 0.026}, 5, 2);
+  box({0, top + 0.0030, 0}, {0.026, 0.0030, 0.026}, 5, 2);
+  box({0, -0.0095, 0}, {0.036, 0.0035, 0.036}, 6, 3);
+  const double columnY = 0.054;
+  box({-0.037, columnY, -0.049}, {0.0026, 0.061, 0.0026}, 6, 3);
+  box({0.037, columnY, -0.049}, {0.0026, 0.061, 0.0026}, 6, 3);
+  box({0, 0.111, -0.049}, {0.042, 0.0032, 0.0036}, 6, 3);
+  box({0, top + 0.0080, -0.001}, {0.0105, 0.0025, 0.0105}, 7, 3);
+}
+
 void orc_scalar_return(double total, double pc, double ac, double k, double y, double H, double* o) {
   Ret1 r = ret1(total, pc, ac, k, y, H);
   o[0] = r.force;

@@ -54,6 +54,11 @@ void orc_set_plastic(orc* h, const double* plastic);
 /* Overwrites the committed base pose (x 3N, q 4N). */
 void orc_set_pose(orc* h, const double* x, const double* q);
 
+/* Simulator::capture(top, prevX, prevR) (MC/src/sim.cpp:215-310) of the
+ * committed state: B + N + MMC_MACHINE_BODIES bodies into out; prev_x (3N) and
+ * prev_q (4N) give the velocities, zero when NULL. */
+void orc_capture(orc* h, double top, const double* prev_x, const double* prev_q, mmc_snapshot* out);
+
 /* Closed-form helpers exposed for known-answer tests. */
 void orc_scalar_return(double total, double pc, double ac, double k, double y, double H, double* out7);
 void orc_vector_return(const double* t, const double* pc, double ac, double k, double y, double H, double* out12);

@@ -54,6 +54,7 @@ def _load(prefix: str):
     if prefix == "orc":
         f("energy_change", [P, dp, dp, C.c_double, dp], None)
         f("set_frame_callback", [P, FRAME_CB, P], None)
+    f("capture", [P, C.c_double, dp, dp, P], None)
     f("rigid_rotation_test", [P, dp])
     f("gradient_check", [P, dp])
     f("read_state", [P, dp, dp, dp], None)
@@ -189,6 +190,15 @@ class Solver:
         """The reference's Simulator::exportSTL (reference build only)."""
         if self._fn("export_stl")(self.h, str(path).encode(), int(segments)) != 0:
             raise RuntimeError(self._fn("last_error")(self.h).decode())
+
+    def capture(self, top, prev_x=None, prev_q=None):
+        """Simulator::capture of the committed state (mmc.SNAPSHOT_DTYPE):
+        velocities from the previous poses (node-major x 3N, q 4N) when given."""
+        out = np.zeros(self.n_beams + self.n_nodes + mmc.MACHINE_BODIES, mmc.SNAPSHOT_DTYPE)
+        px = None if prev_x is None else np.ascontiguousarray(prev_x, np.float64)
+        pq = None if prev_q is None else np.ascontiguousarray(prev_q, np.float64)
+        self._fn("capture")(self.h, top, _dp(px), _dp(pq), out.ctypes.data_as(C.c_void_p))
+        return out
 
     def rigid_rotation_test(self):
         e = C.c_double()

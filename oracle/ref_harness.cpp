@@ -378,6 +378,37 @@ int ref_export_stl(ref_h* h, const char* path, int32_t segments) {
   return 0;
 }
 
+void ref_capture(ref_h* h, double top, const double* px, const double* pq, mmc_snapshot* out) {
+  const size_t N = h->sim->nodes_.size();
+  std::vector<Vec3> X;
+  std::vector<Quat> R;
+  if (px) {
+    X.resize(N);
+    for (size_t i = 0; i < N; i++) X[i] = Vec3{px[3 * i], px[3 * i + 1], px[3 * i + 2]};
+  }
+  if (pq) {
+    R.resize(N);
+    for (size_t i = 0; i < N; i++) R[i] = Quat{pq[4 * i], pq[4 * i + 1], pq[4 * i + 2], pq[4 * i + 3]};
+  }
+  std::vector<Snapshot> s = h->sim->capture(top, px ? &X : nullptr, pq ? &R : nullptr);
+  for (size_t k = 0; k < s.size(); k++) {
+    mmc_snapshot& o = out[k];
+    const Snapshot& a = s[k];
+    const double v[16] = {a.p.x, a.p.y, a.p.z, a.q.w, a.q.x, a.q.y, a.q.z, a.v.x, a.v.y, a.v.z,
+                          a.w.x, a.w.y, a.w.z, a.half.x, a.half.y, a.half.z};
+    std::memcpy(o.p, v, 3 * sizeof(double));
+    std::memcpy(o.q, v + 3, 4 * sizeof(double));
+    std::memcpy(o.v, v + 7, 3 * sizeof(double));
+    std::memcpy(o.w, v + 10, 3 * sizeof(double));
+    std::memcpy(o.half, v + 13, 3 * sizeof(double));
+    o.material = a.material;
+    o.id = a.id;
+    o.kind = a.kind;
+    o.module_class = a.moduleClass;
+    o.ring = a.ring;
+    o.slot = a.slot;
+  }
+}
 int ref_rigid_rotation_test(ref_h* h, double* err) { return h->sim->rigidRotationTest(err) ? 1 : 0; }
 int ref_gradient_check(ref_h* h, double* err) { return h->sim->gradientCheck(err) ? 1 : 0; }
 void ref_read_state(ref_h* h, double* x, double* q, double* plastic) {

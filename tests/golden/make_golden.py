@@ -18,6 +18,12 @@ Fixtures:
   cubic_8x8x4_oracle_frames.npy
                            the 8x8x4 cubic workload, 60 + 60 increments, gradient
                            tolerance 1e-6 (oracle)
+  bcc_222_capture_ref.npz  Simulator::capture (MC/src/sim.cpp:215-310) of the
+                           specimen at a seeded committed pose and plastic state
+                           (all four strut material classes), with velocities
+                           against a seeded previous pose and without
+                           (python tests/golden/make_golden.py capture writes
+                           this one alone)
 """
 import sys
 from pathlib import Path
@@ -71,7 +77,42 @@ def cubic_444_config():
         relative_escape=1, gradient_max_tolerance=1.5e-2, max_iterations=600)
 
 
+def seeded_pose(lat, seed, t_sigma, r_sigma):
+    """A pose near the lattice's base pose: positions moved by N(0, t_sigma),
+    orientations turned by rotation vectors N(0, r_sigma) (node-major x, q)."""
+    rng = np.random.default_rng(seed)
+    n = len(lat.mass)
+    x = lat.x.reshape(n, 3) + rng.normal(0, t_sigma, (n, 3))
+    q = np.zeros((n, 4))
+    for i, r in enumerate(rng.normal(0, r_sigma, (n, 3))):
+        q[i] = po.helper("orc", "quat_exp", r)
+    return x.reshape(-1), q.reshape(-1)
+
+
+def capture_fixture():
+    cfg = bcc_222_config()
+    lat = po.Solver(cfg, "ref").lattice()
+    r = po.Solver(cfg, "ref", lattice=lat)
+    prev_x, prev_q = seeded_pose(lat, 11, 2e-4, 0.02)
+    x, q = seeded_pose(lat, 12, 6e-4, 0.3)
+    rng = np.random.default_rng(13)
+    plastic = np.zeros((r.n_beams, 7))
+    plastic[:, 0] = rng.normal(0, 0.08, r.n_beams)   # |epsp| / 0.10 spans the severity classes
+    plastic[:, 2:5] = rng.normal(0, 0.25, (r.n_beams, 3))
+    plastic = plastic.reshape(-1)
+    r.set_pose(x, q)
+    r.set_plastic(plastic)
+    top = r.top0 - 0.013
+    data = dict(x=x, q=q, prev_x=prev_x, prev_q=prev_q, plastic=plastic, top=np.array([top]),
+                bodies=r.capture(top, prev_x, prev_q), bodies_first=r.capture(top))
+    np.savez_compressed(OUT / "bcc_222_capture_ref.npz", **data)
+    print("capture: bodies", len(data["bodies"]), "materials", np.bincount(data["bodies"]["material"]))
+
+
 def main():
+    if len(sys.argv) > 1 and sys.argv[1] == "capture":
+        capture_fixture()
+        return
     if not po.available("ref"):
         raise SystemExit("oracle/_ref/libmmcref.so missing: run make -C oracle with the reference present")
     # ---- 2x4x2 BCC specimen ----
@@ -125,6 +166,8 @@ def main():
     frames = po.Solver(mmc.cubic_config(8, 8, 4), "orc").run()
     np.save(OUT / "cubic_8x8x4_oracle_frames.npy", frames)
     print("cubic_8x8x4: frames", len(frames), "peak", frames["reaction_force"].max())
+
+    capture_fixture()
 
 
 if __name__ == "__main__":

@@ -317,3 +317,70 @@ def test_approximate_wolfe_line_search_reaches_the_same_equilibrium(mmc, po):
         deep[ls] = st
     assert deep[2].converged and deep[2].max <= 3e-9
     assert not deep[1].converged
+
+
+def test_capture_closed_forms(mmc, po):
+    """Simulator::capture (MC/src/sim.cpp:215-310) against what its bodies
+    must be: a strut's orientation turns the x axis onto its chord (alignX)
+    with a proper rotation, it sits at the chord's midpoint with half extents
+    (L/2, r, r); a joint's velocity is its displacement over the step time and
+    its angular velocity the rotation vector it turned through over the step
+    time; a strut moves with the mean of its two joints; the material class
+    follows the plastic severity thresholds; the machine closes the frame with
+    the moving platen at the frame's top."""
+    cfg = mmc.default_config().replace(cells_x=2, cells_y=3, cells_z=2)
+    s = po.Solver(cfg, "orc")
+    lat = s.lattice()
+    N, B = s.n_nodes, s.n_beams
+    rng = np.random.default_rng(5)
+    x0 = lat.x.reshape(N, 3)
+    dx = rng.normal(0, 3e-4, (N, 3))
+    rho = rng.normal(0, 0.2, (N, 3))
+    q0 = lat.q.reshape(N, 4)
+    assert np.allclose(q0, [1, 0, 0, 0])  # the lattice starts unrotated
+    q1 = np.array([po.helper("orc", "quat_exp", r) for r in rho])
+    sev = np.array([0.0, 0.2, 0.3, 0.5, 0.7, 1.0, 1.2])[np.arange(B) % 7]
+    plastic = np.zeros((B, 7))
+    plastic[:, 0] = 0.10 * sev * np.where(np.arange(B) % 2, 1, -1)
+    s.set_pose((x0 + dx).reshape(-1), q1.reshape(-1))
+    s.set_plastic(plastic.reshape(-1))
+    top = s.top0 - 0.004
+    b = s.capture(top, lat.x, lat.q)
+    dt = cfg.step_time
+    x1 = x0 + dx
+    assert len(b) == B + N + 7
+    # struts
+    a_, b_ = lat.a, lat.b
+    d = x1[b_] - x1[a_]
+    L = np.linalg.norm(d, axis=1)
+    for k in range(B):
+        Rm = quat_to_mat(b["q"][k])
+        assert np.allclose(Rm @ [1, 0, 0], d[k] / L[k], atol=1e-14)
+        assert np.allclose(Rm.T @ Rm, np.eye(3), atol=1e-14) and np.linalg.det(Rm) > 0
+    assert np.allclose(b["p"][:B], 0.5 * (x1[a_] + x1[b_]), rtol=0, atol=1e-17)
+    assert np.allclose(b["half"][:B], np.c_[0.5 * L, lat.radius, lat.radius], rtol=1e-15, atol=0)
+    assert np.array_equal(b["material"][:B], np.searchsorted([0.28, 0.62, 1.05], sev, side="right"))
+    v = dx / dt
+    assert np.allclose(b["v"][:B], 0.5 * (v[a_] + v[b_]), rtol=1e-12, atol=1e-15)
+    assert np.allclose(b["w"][:B], 0.5 * (rho[a_] + rho[b_]) / dt, rtol=1e-10, atol=1e-12)
+    assert np.array_equal(b["id"][:B], np.arange(B)) and np.array_equal(b["slot"][:B], np.arange(B))
+    assert np.array_equal(b["kind"][:B], np.zeros(B)) and np.array_equal(b["ring"][:B], lat.beam_layer)
+    assert np.array_equal(b["module_class"][:B], lat.beam_family)
+    # joints
+    j = b[B:B + N]
+    jr = cfg.joint_scale * cfg.max_radius
+    assert np.array_equal(j["p"], x1) and np.array_equal(j["q"], q1)
+    assert np.allclose(j["v"], v, rtol=1e-12, atol=1e-15)
+    assert np.allclose(j["w"], rho / dt, rtol=1e-10, atol=1e-12)
+    assert np.all(j["half"] == jr) and np.all(j["material"] == 4) and np.all(j["kind"] == 1)
+    assert np.array_equal(j["id"], B + np.arange(N)) and np.array_equal(j["ring"], lat.node_layer)
+    # machine
+    m = b[B + N:]
+    assert np.array_equal(m["kind"], [2, 2, 3, 3, 3, 3, 3]) and np.array_equal(m["id"], B + N + np.arange(7))
+    assert m["p"][1][1] == top + 0.0030 and m["p"][6][1] == top + 0.0080 and m["p"][0][1] == -0.0030
+    assert np.all(m["q"] == [1, 0, 0, 0]) and np.all(m["v"] == 0) and np.all(m["slot"] == -1)
+    # first frame: no previous pose, no motion
+    f = s.capture(top)
+    assert np.all(f["v"] == 0) and np.all(f["w"] == 0)
+    for name in ("p", "q", "half", "material", "id", "kind", "module_class", "ring", "slot"):
+        assert np.array_equal(f[name], b[name]), name

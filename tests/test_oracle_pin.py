@@ -135,3 +135,32 @@ def test_live_reference_minimize_commit(mmc, po):
         r.commit()
         for a, b in zip(o.read_state(), r.read_state()):
             assert np.array_equal(a, b)
+
+
+def test_capture_fixture(mmc, po):
+    """Simulator::capture (MC/src/sim.cpp:215-310) at the fixture's committed
+    pose and plastic state: with velocities against its previous pose and
+    without (the first frame), every body bit for bit."""
+    g = np.load(GOLDEN / "bcc_222_capture_ref.npz")
+    s = po.Solver(bcc_222(mmc), "orc")
+    s.set_pose(g["x"], g["q"])
+    s.set_plastic(g["plastic"])
+    top = float(g["top"][0])
+    assert s.capture(top, g["prev_x"], g["prev_q"]).tobytes() == g["bodies"].tobytes()
+    assert s.capture(top).tobytes() == g["bodies_first"].tobytes()
+
+
+@needs_ref
+@pytest.mark.parametrize("seed", [1, 2])
+def test_live_reference_capture(mmc, po, seed):
+    """capture after a committed minimisation (yielded struts, contact) against
+    the reference's, velocities from the pose before it."""
+    cfg = bcc_222(mmc)
+    o, r = po.Solver(cfg, "orc"), po.Solver(cfg, "ref")
+    top = o.top0 - 0.1 * seed * o.height
+    px, pq, _ = o.read_state()
+    for s in (o, r):
+        s.minimize(top)
+        s.commit()
+    assert o.capture(top, px, pq).tobytes() == r.capture(top, px, pq).tobytes()
+    assert o.capture(top).tobytes() == r.capture(top).tobytes()
