@@ -33,6 +33,7 @@
 
 #include "../../include/mmc.h"
 #include "capture.hpp"
+#include "generate.hpp"
 #include "kernels.hpp"
 
 using namespace mmc;
@@ -3054,6 +3055,15 @@ struct mmc_ctx {
   double* rec_slot(int s) const { return record + size_t(s) * n_inst * kRec; }
   ~mmc_ctx();
   void setup(const mmc_lattice* const* L, const mmc_config* c, int n, int prec);
+  void setup_generated(const mmc_config* c, int n, int prec);
+  void begin_setup(const mmc_config* c, int n, int prec, const std::vector<int>& nn, const std::vector<int>& nb);
+  void upload_lattices(const mmc_lattice* const* L, std::vector<int>& off, std::vector<int>& list);
+  void finish_setup(const std::vector<int>& off, const std::vector<int>& list);
+  // envelope and mass of every lattice (mmc_lattice_info)
+  struct LatInfo {
+    double width = 0, height = 0, depth = 0, mass = 0;
+  };
+  std::vector<LatInfo> hlat;
   void ensure_entries(size_t need);
   void upload_inst() {
     if (inst_d) HIPCALL(hipMemcpy(inst_d, hinst.data(), hinst.size() * sizeof(Inst), hipMemcpyHostToDevice));
@@ -3159,6 +3169,80 @@ std::vector<Blk> block_table(int n, int per, F range, std::vector<int>& b0, std:
 }  // namespace
 
 void mmc_ctx::setup(const mmc_lattice* const* Ls, const mmc_config* cs, int n, int prec) {
+  std::vector<int> nn(size_t(std::max(n, 0))), nb(size_t(std::max(n, 0)));
+  for (int b = 0; b < n; b++) nn[size_t(b)] = Ls[b]->n_nodes, nb[size_t(b)] = Ls[b]->n_beams;
+  begin_setup(cs, n, prec, nn, nb);
+  std::vector<int> off, list;
+  upload_lattices(Ls, off, list);
+  finish_setup(off, list);
+}
+
+// Lattices built on the device (generate_lattices) straight into the SoA.
+void mmc_ctx::setup_generated(const mmc_config* cs, int n, int prec) {
+  if (n <= 0) throw Fail(MMC_ERR_ARG, "a context needs at least one lattice");
+  std::vector<int> nn(static_cast<size_t>(n)), nb(static_cast<size_t>(n));
+  for (int b = 0; b < n; b++) {
+    GenCounts k;
+    try {
+      k = gen_counts(cs[b]);
+    } catch (const std::exception& e) {
+      throw Fail(MMC_ERR_ARG, e.what());
+    }
+    nn[size_t(b)] = k.nodes, nb[size_t(b)] = k.beams;
+  }
+  begin_setup(cs, n, prec, nn, nb);
+  std::vector<int> n0(static_cast<size_t>(n)), b0(static_cast<size_t>(n));
+  long pairs = 0;
+  for (int b = 0; b < n; b++) {
+    n0[size_t(b)] = hinst[size_t(b)].n0, b0[size_t(b)] = hinst[size_t(b)].b0;
+    pairs += gen_counts(icfg[size_t(b)]).connected;
+  }
+  bpos = dalloc<double>(3 * size_t(N));
+  bquat = dalloc<double>(4 * size_t(N));
+  mass = dalloc<double>(N);
+  lock = dalloc<uint8_t>(N);
+  ba = dalloc<int>(B);
+  bb = dalloc<int>(B);
+  L0 = dalloc<double>(B);
+  rad = dalloc<double>(B);
+  area = dalloc<double>(B);
+  inert = dalloc<double>(B);
+  e0 = dalloc<double>(3 * size_t(B));
+  plastic = dalloc<double>(7 * size_t(B));
+  HIPCALL(hipMemset(plastic, 0, 7 * size_t(B) * sizeof(double)));
+  nlayer = dalloc<int>(N);
+  blayer = dalloc<int>(B);
+  bfam = dalloc<int>(B);
+  inc_off = dalloc<int>(N + 1);
+  inc = dalloc<int>(2 * size_t(B));
+  adj_off = dalloc<int>(N + 1);
+  adj = dalloc<int>(2 * size_t(pairs));
+  std::vector<GenResult> res;
+  try {
+    res = generate_lattices(icfg, n0, b0, N, B,
+                            GenOut{bpos, bquat, mass, lock, ba, bb, L0, rad, area, inert, e0, nlayer, blayer, bfam,
+                                   inc_off, inc, adj_off, adj},
+                            stream);
+  } catch (const std::exception& e) {
+    throw Fail(MMC_ERR_HIP, e.what());
+  }
+  for (int b = 0; b < n; b++) {
+    Inst& in = hinst[size_t(b)];
+    const GenResult& r = res[size_t(b)];
+    in.n_free = 6.0 * in.N - (r.lock_a == r.lock_b ? 2 : 3);
+    LatInfo& li = hlat[size_t(b)];
+    li.mass = r.total_mass;
+  }
+  // the gather's record slots are laid out on the host from the incidence
+  std::vector<int> off(size_t(N) + 1), list(2 * size_t(B));
+  HIPCALL(hipMemcpy(off.data(), inc_off, (size_t(N) + 1) * sizeof(int), hipMemcpyDeviceToHost));
+  HIPCALL(hipMemcpy(list.data(), inc, 2 * size_t(B) * sizeof(int), hipMemcpyDeviceToHost));
+  finish_setup(off, list);
+}
+
+// Instances, constants and counts shared by both ways of supplying lattices.
+void mmc_ctx::begin_setup(const mmc_config* cs, int n, int prec, const std::vector<int>& nn,
+                          const std::vector<int>& nb) {
   if (n <= 0) throw Fail(MMC_ERR_ARG, "a context needs at least one lattice");
   precision = prec;
   n_inst = n;
@@ -3168,12 +3252,12 @@ void mmc_ctx::setup(const mmc_lattice* const* Ls, const mmc_config* cs, int n, i
   for (int b = 0; b < n; b++) {
     mmc_config& c = icfg[b];
     c.translation_scale = c.cell_size;  // MC/src/sim.cpp:49
-    if (Ls[b]->n_nodes <= 0 || Ls[b]->n_beams <= 0) throw Fail(MMC_ERR_ARG, "lattice has no nodes or beams");
+    if (nn[size_t(b)] <= 0 || nb[size_t(b)] <= 0) throw Fail(MMC_ERR_ARG, "lattice has no nodes or beams");
     if (c.history < 0 || c.history > kMaxHist) throw Fail(MMC_ERR_ARG, "history must be in [0, 16]");
     if (c.line_search < 0 || c.line_search > 2) throw Fail(MMC_ERR_ARG, "line_search must be 0, 1 or 2");
     max_hist = std::max(max_hist, c.history);
-    tn_ += Ls[b]->n_nodes;
-    tb_ += Ls[b]->n_beams;
+    tn_ += nn[size_t(b)];
+    tb_ += nb[size_t(b)];
   }
   if (tn_ + tb_ > 0x3fffffffLL) throw Fail(MMC_ERR_CAPACITY, "batch exceeds 2^30 nodes + beams");
   cfg = icfg[0];
@@ -3191,9 +3275,9 @@ void mmc_ctx::setup(const mmc_lattice* const* Ls, const mmc_config* cs, int n, i
       Inst& in = hinst[b];
       const mmc_config& c = icfg[b];
       in.n0 = n0;
-      in.N = Ls[b]->n_nodes;
+      in.N = nn[size_t(b)];
       in.b0 = b0;
-      in.B = Ls[b]->n_beams;
+      in.B = nb[size_t(b)];
       in.s0 = n0 + b0;  // shapes of instance b: its spheres, then its capsules
       in.NS = in.N + in.B;
       n0 += in.N;
@@ -3215,6 +3299,26 @@ void mmc_ctx::setup(const mmc_lattice* const* Ls, const mmc_config* cs, int n, i
       in.mt = c.max_translation_step / c.translation_scale;
       in.mr = c.max_rotation_step;
     }
+  }
+  hlat.assign(size_t(n), LatInfo{});
+  for (int b = 0; b < n; b++) {
+    const mmc_config& c = icfg[size_t(b)];
+    LatInfo& li = hlat[size_t(b)];
+    li.width = c.cells_x * c.cell_size;
+    li.height = c.cells_y * c.cell_size;
+    li.depth = c.cells_z * c.cell_size;
+  }
+}
+
+// Lattices handed over on the host: staged into SoA planes, incidence and
+// exclusion adjacency built here; off and list receive the incidence.
+void mmc_ctx::upload_lattices(const mmc_lattice* const* Ls, std::vector<int>& off, std::vector<int>& list) {
+  const int n = n_inst;
+  for (int b = 0; b < n; b++) {
+    LatInfo& li = hlat[size_t(b)];
+    li.width = Ls[b]->width, li.height = Ls[b]->height, li.depth = Ls[b]->depth;
+    li.mass = 0;
+    for (int i = 0; i < Ls[b]->n_nodes; i++) li.mass += Ls[b]->mass[i];
   }
   // ---- host-side SoA staging of the concatenated lattice ----
   std::vector<double> h(7 * size_t(std::max(N, B)));
@@ -3300,7 +3404,8 @@ void mmc_ctx::setup(const mmc_lattice* const* Ls, const mmc_config* cs, int n, i
     labels(bfam, B, false, &mmc_lattice::beam_family);
   }
   // ---- CSR beam incidence, ordered by beam index ----
-  std::vector<int> off(N + 1, 0), list(2 * size_t(B));
+  off.assign(size_t(N) + 1, 0);
+  list.assign(2 * size_t(B), 0);
   for (int k = 0; k < B; k++) {
     off[ha[k] + 1]++;
     off[hb[k] + 1]++;
@@ -3341,6 +3446,11 @@ void mmc_ctx::setup(const mmc_lattice* const* Ls, const mmc_config* cs, int n, i
   adj = dalloc<int>(alist.size());
   HIPCALL(hipMemcpy(adj_off, aoff.data(), (N + 1) * sizeof(int), hipMemcpyHostToDevice));
   if (!alist.empty()) HIPCALL(hipMemcpy(adj, alist.data(), alist.size() * sizeof(int), hipMemcpyHostToDevice));
+}
+
+// Block tables, record slots and work buffers (off, list: the beam incidence).
+void mmc_ctx::finish_setup(const std::vector<int>& off, const std::vector<int>& list) {
+  const int n = n_inst;
   // ---- block tables ----
   {
     std::vector<int> r0, r1;
@@ -3767,6 +3877,93 @@ void mmc_destroy(mmc_ctx* c) { delete c; }
 const char* mmc_last_error(const mmc_ctx* c) { return c ? c->err.c_str() : g_create_error.c_str(); }
 
 int mmc_instances(const mmc_ctx* c) { return c ? c->n_inst : 0; }
+
+int mmc_create_generated(const mmc_config* cfgs, int32_t n, int32_t precision, mmc_ctx** out) {
+  if (!cfgs || n <= 0 || !out) {
+    g_create_error = "NULL or empty argument";
+    return MMC_ERR_ARG;
+  }
+  *out = nullptr;
+  if (precision != MMC_FP64 && precision != MMC_MIXED) {
+    g_create_error = "unknown precision policy";
+    return MMC_ERR_ARG;
+  }
+  auto* c = new mmc_ctx();
+  try {
+    c->setup_generated(cfgs, n, precision);
+  } catch (const Fail& e) {
+    g_create_error = e.what();
+    delete c;
+    return e.code;
+  } catch (const std::exception& e) {
+    g_create_error = e.what();
+    delete c;
+    return MMC_ERR_HIP;
+  }
+  *out = c;
+  return MMC_OK;
+}
+
+int mmc_lattice_info_get(mmc_ctx* c, int32_t b, mmc_lattice_info* o) {
+  return guarded(c, [&] {
+    if (!o || b < 0 || b >= c->n_inst) throw Fail(MMC_ERR_ARG, "lattice index out of range");
+    const Inst& in = c->hinst[size_t(b)];
+    const mmc_ctx::LatInfo& li = c->hlat[size_t(b)];
+    int excl = 0;
+    HIPCALL(hipMemcpy(&excl, c->adj_off + in.n0 + in.N, sizeof(int), hipMemcpyDeviceToHost));
+    int first = 0;
+    HIPCALL(hipMemcpy(&first, c->adj_off + in.n0, sizeof(int), hipMemcpyDeviceToHost));
+    *o = mmc_lattice_info{in.N, in.B, int64_t(excl - first) / 2, li.width, li.height, li.depth, li.mass};
+  });
+}
+
+int mmc_read_lattice(mmc_ctx* c, int32_t b, double* x, double* mass, uint8_t* locked, int32_t* beam_a,
+                     int32_t* beam_b, double* L0, double* radius, double* area, double* inertia, double* e0,
+                     int32_t* node_layer, int32_t* beam_layer, int32_t* beam_family, int32_t* adj_off,
+                     int32_t* adj) {
+  return guarded(c, [&] {
+    if (b < 0 || b >= c->n_inst) throw Fail(MMC_ERR_ARG, "lattice index out of range");
+    const Inst& in = c->hinst[size_t(b)];
+    const int N = in.N, B = in.B;
+    // planes of this lattice's range, node-major on the host
+    auto planes = [&](double* dst, const double* src, int total, int first, int cnt, int k) {
+      if (!dst) return;
+      std::vector<double> t(size_t(cnt) * k);
+      for (int d = 0; d < k; d++)
+        HIPCALL(hipMemcpy(t.data() + size_t(d) * cnt, src + size_t(d) * total + first, size_t(cnt) * sizeof(double),
+                          hipMemcpyDeviceToHost));
+      aos_from_soa(t.data(), dst, cnt, k);
+    };
+    auto ints = [&](int32_t* dst, const int* src, int first, int cnt, int shift) {
+      if (!dst) return;
+      HIPCALL(hipMemcpy(dst, src + first, size_t(cnt) * sizeof(int), hipMemcpyDeviceToHost));
+      for (int i = 0; i < cnt; i++) dst[i] -= shift;
+    };
+    planes(x, c->bpos, c->N, in.n0, N, 3);
+    planes(mass, c->mass, c->N, in.n0, N, 1);
+    if (locked) {
+      std::vector<uint8_t> lk(static_cast<size_t>(N));
+      HIPCALL(hipMemcpy(lk.data(), c->lock + in.n0, size_t(N), hipMemcpyDeviceToHost));
+      for (int i = 0; i < N; i++)
+        for (int d = 0; d < 6; d++) locked[6 * size_t(i) + d] = uint8_t((lk[size_t(i)] >> d) & 1u);
+    }
+    ints(beam_a, c->ba, in.b0, B, in.n0);
+    ints(beam_b, c->bb, in.b0, B, in.n0);
+    planes(L0, c->L0, c->B, in.b0, B, 1);
+    planes(radius, c->rad, c->B, in.b0, B, 1);
+    planes(area, c->area, c->B, in.b0, B, 1);
+    planes(inertia, c->inert, c->B, in.b0, B, 1);
+    planes(e0, c->e0, c->B, in.b0, B, 3);
+    ints(node_layer, c->nlayer, in.n0, N, 0);
+    ints(beam_layer, c->blayer, in.b0, B, 0);
+    ints(beam_family, c->bfam, in.b0, B, 0);
+    std::vector<int> ao(size_t(N) + 1);
+    HIPCALL(hipMemcpy(ao.data(), c->adj_off + in.n0, (size_t(N) + 1) * sizeof(int), hipMemcpyDeviceToHost));
+    if (adj_off)
+      for (int i = 0; i <= N; i++) adj_off[i] = ao[size_t(i)] - ao[0];
+    if (adj && ao[size_t(N)] > ao[0]) ints(adj, c->adj, ao[0], ao[size_t(N)] - ao[0], in.n0);
+  });
+}
 
 int mmc_objective(mmc_ctx* c, const double* zh, double top, int32_t trials, mmc_objective_out* out, double* grad) {
   return guarded(c, [&] {

@@ -690,15 +690,38 @@ bool Driver::gradientCheck(double* relativeError) const {
 // ---------------------------------------------------------------------------
 // BatchSimulator
 // ---------------------------------------------------------------------------
-BatchSimulator::BatchSimulator(std::vector<mmc_config> cfgs, int precision) : c_(std::move(cfgs)) {
+BatchSimulator::BatchSimulator(std::vector<mmc_config> cfgs, int precision, bool generate) : c_(std::move(cfgs)) {
   if (c_.empty()) throw std::runtime_error("a batch needs at least one lattice");
   const size_t n = c_.size();
   cfgs_.resize(n);
   lats_.resize(n);
-  std::vector<mmc_lattice> views(n);
   for (size_t b = 0; b < n; b++) {
     c_[b].translation_scale = c_[b].cell_size;  // MC/src/sim.cpp:49
     from_c(c_[b], &cfgs_[b], nullptr);
+  }
+  if (generate) {
+    // built on the device: the host keeps each lattice's counts, envelope,
+    // mass and platen (MC/src/sim.cpp:56,131,134)
+    int rc = mmc_create_generated(c_.data(), int32_t(n), precision, &ctx_);
+    if (rc != MMC_OK) throw std::runtime_error(mmc_last_error(nullptr));
+    for (size_t b = 0; b < n; b++) {
+      mmc_lattice_info info{};
+      check(mmc_lattice_info_get(ctx_, int32_t(b), &info));
+      HostLattice& L = lats_[b];
+      L.n_nodes = info.n_nodes;
+      L.n_beams = info.n_beams;
+      L.width = info.width;
+      L.height = info.height;
+      L.depth = info.depth;
+      L.area_ref = L.width * L.depth;
+      L.joint_radius = c_[b].joint_scale * c_[b].max_radius;
+      L.total_mass = info.mass;
+      L.top0 = L.joint_radius + L.height + L.joint_radius;
+    }
+    return;
+  }
+  std::vector<mmc_lattice> views(n);
+  for (size_t b = 0; b < n; b++) {
     lats_[b] = build_lattice(c_[b]);
     views[b] = lats_[b].view();
   }
@@ -1177,12 +1200,12 @@ int mmc_sim_export_stl(mmc_sim* s, const char* path, int32_t radial_segments, mm
   return sim_guard(s, [&] { s->sim->exportSTL(path, radial_segments, report); });
 }
 
-int mmc_batch_create(const mmc_config* cfgs, int32_t n, int32_t precision, mmc_batch** out) {
+static int batch_create(const mmc_config* cfgs, int32_t n, int32_t precision, mmc_batch** out, bool generate) {
   if (!cfgs || n <= 0 || !out) return MMC_ERR_ARG;
   *out = nullptr;
   auto* s = new mmc_batch();
   try {
-    s->sim = std::make_unique<meta::BatchSimulator>(std::vector<mmc_config>(cfgs, cfgs + n), precision);
+    s->sim = std::make_unique<meta::BatchSimulator>(std::vector<mmc_config>(cfgs, cfgs + n), precision, generate);
   } catch (const std::exception& e) {
     s->err = e.what();
     g_sim_create_error = e.what();
@@ -1191,6 +1214,14 @@ int mmc_batch_create(const mmc_config* cfgs, int32_t n, int32_t precision, mmc_b
   }
   *out = s;
   return MMC_OK;
+}
+
+int mmc_batch_create(const mmc_config* cfgs, int32_t n, int32_t precision, mmc_batch** out) {
+  return batch_create(cfgs, n, precision, out, false);
+}
+
+int mmc_batch_create_generated(const mmc_config* cfgs, int32_t n, int32_t precision, mmc_batch** out) {
+  return batch_create(cfgs, n, precision, out, true);
 }
 
 void mmc_batch_destroy(mmc_batch* s) { delete s; }

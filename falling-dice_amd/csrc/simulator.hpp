@@ -257,7 +257,9 @@ class Driver {
 // reference's error text while the others continue.
 class BatchSimulator {
  public:
-  BatchSimulator(std::vector<mmc_config> cfgs, int precision);
+  // generate: the lattices are built on the device (mmc_create_generated);
+  // otherwise on the host (mmc_lattice_build) and uploaded
+  BatchSimulator(std::vector<mmc_config> cfgs, int precision, bool generate = false);
   ~BatchSimulator();
   BatchSimulator(const BatchSimulator&) = delete;
   BatchSimulator& operator=(const BatchSimulator&) = delete;

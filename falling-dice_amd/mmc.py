@@ -137,6 +137,11 @@ MACHINE_BODIES = 7
 FRAME_OBSERVER = C.CFUNCTYPE(None, C.c_int32, C.c_int32, C.c_void_p, C.c_int64, C.c_void_p)
 
 
+class LatticeInfo(C.Structure):
+    _fields_ = [("n_nodes", C.c_int32), ("n_beams", C.c_int32), ("n_exclusions", C.c_int64),
+                ("width", C.c_double), ("height", C.c_double), ("depth", C.c_double), ("mass", C.c_double)]
+
+
 class StlReport(C.Structure):
     _fields_ = [("voxel", C.c_double), ("signed_volume", C.c_double), ("nx", C.c_int32), ("ny", C.c_int32),
                 ("nz", C.c_int32), ("watertight", C.c_int32)] + [
@@ -300,6 +305,12 @@ def lib():
         L.mmc_beam_potential.argtypes = [P, dp, dp, dp]
         L.mmc_beam_potential_noalloc.argtypes = [P, P, P, P, P]
         L.mmc_capture.argtypes = [P, dp, C.POINTER(C.c_int32), C.POINTER(C.c_int32), P]
+        L.mmc_create_generated.argtypes = [C.POINTER(Config), C.c_int32, C.c_int32, C.POINTER(P)]
+        L.mmc_lattice_info_get.argtypes = [P, C.c_int32, C.POINTER(LatticeInfo)]
+        i32p = C.POINTER(C.c_int32)
+        L.mmc_read_lattice.argtypes = [P, C.c_int32, dp, dp, C.POINTER(C.c_uint8), i32p, i32p, dp, dp, dp, dp, dp,
+                                       i32p, i32p, i32p, i32p, i32p]
+        L.mmc_batch_create_generated.argtypes = [C.POINTER(Config), C.c_int32, C.c_int32, C.POINTER(P)]
         L.mmc_capture_mark.argtypes = [P]
         L.mmc_config_defaults.argtypes = [C.POINTER(Config)]
         L.mmc_config_defaults.restype = None
@@ -408,17 +419,39 @@ def build_lattice(cfg: Config) -> LatticeArrays:
 class Context:
     """One device context (mmc_ctx): the hot path of a single lattice."""
 
-    def __init__(self, lat: LatticeArrays, cfg: Config, precision: int = MMC_FP64):
+    def __init__(self, lat: LatticeArrays | None, cfg: Config, precision: int = MMC_FP64):
         L = lib()
+        self.cfg = cfg
+        h = C.c_void_p()
+        if lat is None:
+            # the lattice built on the device (mmc_create_generated)
+            rc = L.mmc_create_generated(C.byref(cfg), 1, precision, C.byref(h))
+            if rc != 0:
+                raise MMCError(f"mmc_create_generated failed ({rc}): {L.mmc_last_error(None).decode()}")
+            self.h = h
+            info = self.lattice_info(0)
+            self.n_nodes, self.n_beams = info.n_nodes, info.n_beams
+            return
         self._lat = lat
         self._ls = lat.struct()
-        self.cfg = cfg
         self.n_nodes, self.n_beams = len(lat.mass), len(lat.L0)
-        h = C.c_void_p()
         rc = L.mmc_create(C.byref(self._ls), C.byref(cfg), precision, C.byref(h))
         if rc != 0:
             raise MMCError(f"mmc_create failed ({rc}): {L.mmc_last_error(None).decode()}")
         self.h = h
+
+    @classmethod
+    def generated(cls, cfg: Config, precision: int = MMC_FP64) -> "Context":
+        """A context whose lattice is built on the device."""
+        return cls(None, cfg, precision)
+
+    def lattice_info(self, b: int = 0) -> LatticeInfo:
+        return lattice_info(self.h, b)
+
+    def read_lattice(self, b: int = 0):
+        """Lattice b's construction as the device holds it: (LatticeArrays
+        without plastic state and exclusion keys, adj_off, adj)."""
+        return read_lattice(self.h, b)
 
     def _check(self, rc):
         if rc != 0:
@@ -479,6 +512,35 @@ class Context:
         """The bodies of the committed frame (SNAPSHOT_DTYPE, B + N + 7), with
         velocities against the poses of the last capture_mark when asked."""
         return capture(self.h, [top], [1], [int(velocities)], self.n_beams + self.n_nodes + MACHINE_BODIES)
+
+
+def lattice_info(ctx, b: int) -> LatticeInfo:
+    info = LatticeInfo()
+    if lib().mmc_lattice_info_get(ctx, b, C.byref(info)) != 0:
+        raise MMCError(lib().mmc_last_error(ctx).decode())
+    return info
+
+
+def read_lattice(ctx, b: int):
+    """mmc_read_lattice of lattice b of a context handle."""
+    info = lattice_info(ctx, b)
+    N, B = info.n_nodes, info.n_beams
+    a = dict(x=np.zeros(3 * N), mass=np.zeros(N), locked=np.zeros(6 * N, np.uint8), a=np.zeros(B, np.int32),
+             b=np.zeros(B, np.int32), L0=np.zeros(B), radius=np.zeros(B), area=np.zeros(B), inertia=np.zeros(B),
+             e0=np.zeros(3 * B), node_layer=np.zeros(N, np.int32), beam_layer=np.zeros(B, np.int32),
+             beam_family=np.zeros(B, np.int32))
+    adj_off = np.zeros(N + 1, np.int32)
+    adj = np.zeros(max(1, 2 * info.n_exclusions), np.int32)
+    i32 = lambda v: v.ctypes.data_as(C.POINTER(C.c_int32))  # noqa: E731
+    rc = lib().mmc_read_lattice(ctx, b, _dp(a["x"]), _dp(a["mass"]), a["locked"].ctypes.data_as(C.POINTER(C.c_uint8)),
+                                i32(a["a"]), i32(a["b"]), _dp(a["L0"]), _dp(a["radius"]), _dp(a["area"]),
+                                _dp(a["inertia"]), _dp(a["e0"]), i32(a["node_layer"]), i32(a["beam_layer"]),
+                                i32(a["beam_family"]), i32(adj_off), i32(adj))
+    if rc != 0:
+        raise MMCError(lib().mmc_last_error(ctx).decode())
+    lat = LatticeArrays(width=info.width, height=info.height, depth=info.depth, q=None, plastic=None,
+                        connected=np.zeros(0, np.uint64), **a)
+    return lat, adj_off, adj[:2 * info.n_exclusions]
 
 
 def capture(ctx, tops, mask, velocities, total):
@@ -634,12 +696,13 @@ class BatchSimulator:
     """Many lattices solved together (mmc_batch_*): each with the schedule of
     Simulator.run under its own config."""
 
-    def __init__(self, cfgs, precision: int = MMC_FP64):
+    def __init__(self, cfgs, precision: int = MMC_FP64, device_lattices: bool = False):
         L = lib()
         n = len(cfgs)
         arr = (Config * n)(*cfgs)
         h = C.c_void_p()
-        rc = L.mmc_batch_create(arr, n, precision, C.byref(h))
+        create = L.mmc_batch_create_generated if device_lattices else L.mmc_batch_create
+        rc = create(arr, n, precision, C.byref(h))
         if rc != 0:
             msg = L.mmc_batch_last_error(h).decode() if h else L.mmc_batch_last_error(None).decode()
             if h:

@@ -280,6 +280,37 @@ int mmc_beam_potential_noalloc(mmc_ctx* ctx, const double* x, const double* q, d
 /* n lattices with one config each (cfgs[i].history etc. may differ). */
 int mmc_create_batch(const mmc_lattice* lattices, const mmc_config* cfgs, int32_t n, int32_t precision,
                      mmc_ctx** out);
+/* A batched context whose lattices are built on the device (no reference
+ * counterpart as an entry point; the construction is Simulator::build and
+ * distributeMass, MC/src/sim.cpp:49-135, for the BCC family, and the simple
+ * cubic family of mmc_lattice_build): lattice i from cfgs[i] exactly as
+ * mmc_lattice_build builds it, generated in the context's device buffers with
+ * its beam incidence, lumped masses, locks and exclusion adjacency; no host
+ * lattice exists.  Integer data (topology, layers, families, locks, exclusion
+ * pairs) and whatever is made of sums, products, quotients and square roots
+ * equal mmc_lattice_build's bit for bit; the second moments pi r^4 / 4 and the
+ * Gaussian jitter of the cubic family (position_sigma > 0) use the device's
+ * pow, log, sin and cos, within a few units in the last place of the host's. */
+int mmc_create_generated(const mmc_config* cfgs, int32_t n, int32_t precision, mmc_ctx** out);
+/* Lattice b of a context: node and beam counts, exclusion pairs (each pair
+ * counted once), envelope, and total mass (for generated lattices that of
+ * distributeMass, MC/src/sim.cpp:134: beams in order, then joints; for
+ * lattices handed to mmc_create* the sum of their node masses). */
+typedef struct mmc_lattice_info {
+  int32_t n_nodes, n_beams;
+  int64_t n_exclusions;
+  double width, height, depth, mass;
+} mmc_lattice_info;
+int mmc_lattice_info_get(mmc_ctx* ctx, int32_t b, mmc_lattice_info* out);
+/* Reads lattice b's construction back in mmc_lattice's node-major layout
+ * (local node indices): x 3N, mass N, locked 6N, beam_a, beam_b, L0, radius,
+ * area, inertia B, e0 3B, node_layer N, beam_layer and beam_family B, and the
+ * sorted exclusion adjacency (adj_off N + 1, adj 2 * n_exclusions).  Any
+ * pointer may be NULL. */
+int mmc_read_lattice(mmc_ctx* ctx, int32_t b, double* x, double* mass, uint8_t* locked, int32_t* beam_a,
+                     int32_t* beam_b, double* L0, double* radius, double* area, double* inertia, double* e0,
+                     int32_t* node_layer, int32_t* beam_layer, int32_t* beam_family, int32_t* adj_off,
+                     int32_t* adj);
 /* Number of lattices held by ctx (1 for mmc_create). */
 int mmc_instances(const mmc_ctx* ctx);
 /* mmc_minimize for every instance b with active[b] != 0 (all when active is
@@ -351,6 +382,9 @@ mmc_ctx* mmc_sim_context(mmc_sim* sim);
  * material, radii, schedule), solved together on one device context. */
 typedef struct mmc_batch mmc_batch;
 int mmc_batch_create(const mmc_config* cfgs, int32_t n, int32_t precision, mmc_batch** out);
+/* The same with the lattices built on the device (mmc_create_generated):
+ * no host lattice is built or uploaded. */
+int mmc_batch_create_generated(const mmc_config* cfgs, int32_t n, int32_t precision, mmc_batch** out);
 void mmc_batch_destroy(mmc_batch* batch);
 /* First failure message (a lattice whose equilibrium failed), or the creation
  * error when batch is NULL.  Never NULL. */
