@@ -73,9 +73,14 @@ def workload(args):
         cfg = mmc.cubic_config(args.nx, args.ny, args.nz, cell=cell, r_bot=3e-4, r_top=7e-4, strain=0.5,
                                load=100, unload=100, grad_tol=args.grad_tol, max_iterations=args.max_iterations,
                                sigma=0.02 * cell, radius_mode=1)
+        # the same line-search form as the 32x32x16 workload: the reference's
+        # total-energy Armijo test cannot resolve decreases at tolerance 1e-7
+        cfg = cfg.replace(line_search=args.line_search)
         name = (f"cubic {args.nx}x{args.ny}x{args.nz} cells, cell 5 mm, node jitter N(0, (0.1 mm)^2), "
                 f"per-beam r U(0.3, 0.7) mm, E 2.0 GPa, yield 50 MPa, H 5%E, 50% strain, 100+100 increments, "
-                f"L-BFGS m=10, grad tol {args.grad_tol:g}, max {args.max_iterations} iterations")
+                f"L-BFGS m=10, grad tol {args.grad_tol:g}"
+                + {0: "", 1: ", per-term Armijo", 2: ", per-term Armijo + approximate Wolfe (parity form)"}[args.line_search]
+                + f", max {args.max_iterations} iterations")
         return cfg, name
     cfg = mmc.cubic_config(args.nx, args.ny, args.nz, load=args.load_steps, unload=args.unload_steps,
                            grad_tol=args.grad_tol, strain=args.strain, max_iterations=args.max_iterations)
