@@ -191,6 +191,8 @@ def run_batch(args, log):
     count = args.lattices or (256 if kind == "uneven" else 1024)
     cfgs = mmc.batch_workload(kind, count, args.load_steps, args.unload_steps)
     prec = mmc.MMC_FP64 if args.precision == "fp64" else mmc.MMC_MIXED
+    if args.shards:
+        os.environ["MMC_BATCH_SHARDS"] = str(args.shards)
     t0 = time.perf_counter()
     sim = mmc.BatchSimulator(cfgs, prec, device_lattices=not args.host_lattices)
     setup = time.perf_counter() - t0
@@ -206,7 +208,7 @@ def run_batch(args, log):
         "metric": "lattices x load-steps/sec (batched)", "value": round(len(ok) * steps / dt, 3),
         "unit": "lattice-load-steps/s", "n_gpus": 1, "higher_is_better": True,
         "designs_per_s": round(len(ok) / dt, 4), "seconds": round(dt, 2), "setup_s": round(setup, 3),
-        "lattices_built_on": source,
+        "lattices_built_on": source, "shards": int(os.environ.get("MMC_BATCH_SHARDS", "1")),
         "dtype": "f64" if prec == 0 else "f32-element/f64-reduction", "data": "synthetic (seeded lattices)",
         "config": {"workload": kind, "lattices": count, "nodes": int(sum(sim.n_nodes)),
                    "beams": int(sum(sim.n_beams)), "increments": f"{args.load_steps}+{args.unload_steps} + tare"},
@@ -373,6 +375,8 @@ def main():
                          "window of --steps increments); uneven / sweep: batched lattice sets")
     ap.add_argument("--lattices", type=int, default=0, help="batch size (default 256 uneven, 1024 sweep)")
     ap.add_argument("--oracle-sample", type=int, default=0, help="batched: lattices checked against the oracle")
+    ap.add_argument("--shards", type=int, default=0,
+                    help="batched: independent contexts the batch is split over (MMC_BATCH_SHARDS)")
     ap.add_argument("--host-lattices", action="store_true",
                     help="batched: build the lattices on the host and upload them (default: on the device)")
     ap.add_argument("--single-sample", type=int, default=0,

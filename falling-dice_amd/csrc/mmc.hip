@@ -2869,6 +2869,16 @@ constexpr int kStaticScratchDoubles = 1 << 16;  // 512 KiB
 static __device__ __align__(256) double scratch[kStaticScratchDoubles];
 constexpr int kMaxDevices = 64;
 std::mutex g_scratch_mu;
+// Contexts may be driven from several host threads at once (a sharded batch).
+// A graph is recorded by stream capture, and a capture in progress is
+// invalidated by any operation that synchronises with its stream from
+// elsewhere in the process: copies and memsets on the legacy default stream
+// (which synchronise with every blocking stream) and frees (which
+// synchronise the device).  Every capture, and every run-phase section that
+// issues such operations (list builds, table uploads, kernel timing, capture
+// of frames, lattice readback, set-up and tear-down), holds this lock.
+std::recursive_mutex g_sync_mu;
+using SyncLock = std::lock_guard<std::recursive_mutex>;
 double* g_scratch_ptr[kMaxDevices] = {};
 
 // Takes the device copy's address for `device` (current) once; throws on failure.
@@ -3115,6 +3125,7 @@ struct mmc_ctx {
 };
 
 mmc_ctx::~mmc_ctx() {
+  SyncLock sync_lock(g_sync_mu);
   if (std::getenv("MMC_LOOP_TIMING"))
     std::fprintf(stderr,
                  "[loop] %d instances: initial evaluations %.3f s, captures %lld (%.3f s), replay loop %.3f s "
@@ -3169,6 +3180,7 @@ std::vector<Blk> block_table(int n, int per, F range, std::vector<int>& b0, std:
 }  // namespace
 
 void mmc_ctx::setup(const mmc_lattice* const* Ls, const mmc_config* cs, int n, int prec) {
+  SyncLock sync_lock(g_sync_mu);
   std::vector<int> nn(size_t(std::max(n, 0))), nb(size_t(std::max(n, 0)));
   for (int b = 0; b < n; b++) nn[size_t(b)] = Ls[b]->n_nodes, nb[size_t(b)] = Ls[b]->n_beams;
   begin_setup(cs, n, prec, nn, nb);
@@ -3179,6 +3191,7 @@ void mmc_ctx::setup(const mmc_lattice* const* Ls, const mmc_config* cs, int n, i
 
 // Lattices built on the device (generate_lattices) straight into the SoA.
 void mmc_ctx::setup_generated(const mmc_config* cs, int n, int prec) {
+  SyncLock sync_lock(g_sync_mu);
   if (n <= 0) throw Fail(MMC_ERR_ARG, "a context needs at least one lattice");
   std::vector<int> nn(static_cast<size_t>(n)), nb(static_cast<size_t>(n));
   for (int b = 0; b < n; b++) {
@@ -3621,6 +3634,7 @@ void mmc_ctx::ensure_entries(size_t need) {
 // synchronisations size the grid entries and the pair list exactly.
 template <class T>
 void mmc_ctx::build_list(Lat<T> Lt, const Blk* ctab, int cblocks, const Blk* ntab, int nblocks) {
+  SyncLock sync_lock(g_sync_mu);
   const auto t_build = std::chrono::steady_clock::now();
   const bool part = ctab != nullptr;
   if (!part) {
@@ -3906,6 +3920,7 @@ int mmc_create_generated(const mmc_config* cfgs, int32_t n, int32_t precision, m
 
 int mmc_lattice_info_get(mmc_ctx* c, int32_t b, mmc_lattice_info* o) {
   return guarded(c, [&] {
+    SyncLock sync_lock(g_sync_mu);
     if (!o || b < 0 || b >= c->n_inst) throw Fail(MMC_ERR_ARG, "lattice index out of range");
     const Inst& in = c->hinst[size_t(b)];
     const mmc_ctx::LatInfo& li = c->hlat[size_t(b)];
@@ -3922,6 +3937,7 @@ int mmc_read_lattice(mmc_ctx* c, int32_t b, double* x, double* mass, uint8_t* lo
                      int32_t* node_layer, int32_t* beam_layer, int32_t* beam_family, int32_t* adj_off,
                      int32_t* adj) {
   return guarded(c, [&] {
+    SyncLock sync_lock(g_sync_mu);
     if (b < 0 || b >= c->n_inst) throw Fail(MMC_ERR_ARG, "lattice index out of range");
     const Inst& in = c->hinst[size_t(b)];
     const int N = in.N, B = in.B;
@@ -4037,6 +4053,7 @@ void mmc_ctx::capture_iteration(Lat<T> Lt) {
 // in x): the other instances' lists and reference positions are left as they
 // are, and stay exact for them.
 void mmc_ctx::rebuild_held(const std::vector<int>& held) {
+  SyncLock sync_lock(g_sync_mu);
   if (held.size() == size_t(n_inst)) {
     rebuild();
     return;
@@ -4229,6 +4246,7 @@ void mmc_ctx::tune() {
 }
 
 void mmc_ctx::capture(const std::vector<int>& set) {
+  SyncLock sync_lock(g_sync_mu);
   HIPCALL(hipStreamSynchronize(stream));
   graph_tables(set);
   if (!tuned) tune();
@@ -4256,6 +4274,7 @@ __global__ __launch_bounds__(kBlock) void k_copy_words(double* __restrict__ dst,
 }
 }  // namespace
 double mmc_ctx::time_loop_kernel(int kernel, int launches) {
+  SyncLock sync_lock(g_sync_mu);
   if (gset.empty()) throw Fail(MMC_ERR_STATE, "the device loop has not run yet (no mmc_minimize)");
   if (!ev0) {
     HIPCALL(hipEventCreate(&ev0));
@@ -4921,6 +4940,7 @@ int mmc_read_stats(mmc_ctx* c, mmc_state_stats* o) {
 
 int mmc_capture_mark(mmc_ctx* c) {
   return guarded(c, [&] {
+    SyncLock sync_lock(g_sync_mu);
     if (!c->prev_x) {
       c->prev_x = dalloc<double>(3 * size_t(c->N));
       c->prev_q = dalloc<double>(4 * size_t(c->N));
@@ -4932,6 +4952,7 @@ int mmc_capture_mark(mmc_ctx* c) {
 
 int mmc_capture(mmc_ctx* c, const double* tops, const int32_t* mask, const int32_t* velocities, mmc_snapshot* out) {
   return guarded(c, [&] {
+    SyncLock sync_lock(g_sync_mu);
     if (!tops || !out) throw Fail(MMC_ERR_ARG, "NULL argument");
     std::vector<CapInst> ci;
     long total = 0;
@@ -4991,6 +5012,7 @@ int64_t mmc_evaluations(const mmc_ctx* c) { return c ? c->total_evaluations : 0;
 
 int mmc_time_kernel(mmc_ctx* c, int32_t kernel, int32_t launches, double* mean_ms) {
   return guarded(c, [&] {
+    SyncLock sync_lock(g_sync_mu);
     if (launches <= 0 || !mean_ms) throw Fail(MMC_ERR_ARG, "launches must be positive");
     if (!c->ev0) {
       HIPCALL(hipEventCreate(&c->ev0));

@@ -12,6 +12,8 @@
 #include <functional>
 #include <iomanip>
 #include <iostream>
+#include <mutex>
+#include <thread>
 #include <limits>
 #include <mutex>
 #include <sstream>
@@ -690,7 +692,8 @@ bool Driver::gradientCheck(double* relativeError) const {
 // ---------------------------------------------------------------------------
 // BatchSimulator
 // ---------------------------------------------------------------------------
-BatchSimulator::BatchSimulator(std::vector<mmc_config> cfgs, int precision, bool generate) : c_(std::move(cfgs)) {
+BatchSimulator::BatchSimulator(std::vector<mmc_config> cfgs, int precision, bool generate, int shards)
+    : c_(std::move(cfgs)) {
   if (c_.empty()) throw std::runtime_error("a batch needs at least one lattice");
   const size_t n = c_.size();
   cfgs_.resize(n);
@@ -698,6 +701,23 @@ BatchSimulator::BatchSimulator(std::vector<mmc_config> cfgs, int precision, bool
   for (size_t b = 0; b < n; b++) {
     c_[b].translation_scale = c_[b].cell_size;  // MC/src/sim.cpp:49
     from_c(c_[b], &cfgs_[b], nullptr);
+  }
+  if (shards <= 0) {
+    const char* e = std::getenv("MMC_BATCH_SHARDS");
+    shards = e ? std::max(1, std::atoi(e)) : 1;
+  }
+  shards = std::min<int>(shards, int(n));
+  if (shards > 1) {
+    shard_lattices_.resize(size_t(shards));
+    for (size_t b = 0; b < n; b++) shard_lattices_[b % size_t(shards)].push_back(int(b));
+    for (const std::vector<int>& idx : shard_lattices_) {
+      std::vector<mmc_config> sub;
+      for (int b : idx) sub.push_back(c_[size_t(b)]);
+      shards_.push_back(std::make_unique<BatchSimulator>(std::move(sub), precision, generate, 1));
+      const BatchSimulator& sh = *shards_.back();
+      for (size_t k = 0; k < idx.size(); k++) lats_[size_t(idx[k])] = sh.lats_[k];
+    }
+    return;
   }
   if (generate) {
     // built on the device: the host keeps each lattice's counts, envelope,
@@ -839,8 +859,57 @@ bool per_lattice(const std::vector<mmc_config>& cfgs) {
 }  // namespace
 
 std::vector<BatchSimulator::Outcome> BatchSimulator::run(bool verbose) {
+  if (!shards_.empty()) return run_shards(verbose);
   if (observer_) check(mmc_capture_mark(ctx_));
   return per_lattice(c_) ? run_per_lattice(verbose) : run_rounds(verbose);
+}
+
+// Every shard on a host thread of its own (each shard's context has its own
+// stream); frames and outcomes land at the lattices' indices in the batch.
+std::vector<BatchSimulator::Outcome> BatchSimulator::run_shards(bool verbose) {
+  const size_t K = shards_.size();
+  std::vector<Outcome> out(lats_.size());
+  std::vector<std::exception_ptr> err(K);
+  std::mutex mu;  // the observer is called from every shard's thread
+  for (size_t k = 0; k < K; k++) {
+    if (observer_) {
+      const std::vector<int>& idx = shard_lattices_[k];
+      shards_[k]->setFrameObserver([this, &mu, &idx](int lattice, int frame, const mmc_snapshot* b, int64_t c) {
+        std::lock_guard<std::mutex> lock(mu);
+        observer_(idx[size_t(lattice)], frame, b, c);
+      });
+    } else {
+      shards_[k]->setFrameObserver(nullptr);
+    }
+  }
+  std::vector<std::thread> threads;
+  for (size_t k = 0; k < K; k++)
+    threads.emplace_back([&, k] {
+      try {
+        std::vector<Outcome> r = shards_[k]->run(verbose && k == 0);
+        const std::vector<int>& idx = shard_lattices_[k];
+        for (size_t j = 0; j < idx.size(); j++) out[size_t(idx[j])] = std::move(r[j]);
+      } catch (...) {
+        err[k] = std::current_exception();
+      }
+    });
+  for (std::thread& t : threads) t.join();
+  for (const std::exception_ptr& e : err)
+    if (e) std::rethrow_exception(e);
+  return out;
+}
+
+int64_t BatchSimulator::rounds() const {
+  int64_t r = rounds_;
+  for (const auto& sh : shards_) r = std::max(r, sh->rounds());
+  return r;
+}
+
+int64_t BatchSimulator::evaluations() const {
+  if (shards_.empty()) return mmc_evaluations(ctx_);
+  int64_t e = 0;
+  for (const auto& sh : shards_) e += sh->evaluations();
+  return e;
 }
 
 void BatchSimulator::observe(const std::vector<int32_t>& mask, const std::vector<double>& tops,
@@ -1270,6 +1339,8 @@ int mmc_batch_set_frame_observer(mmc_batch* s, mmc_batch_frame_fn fn, void* user
     s->sim->setFrameObserver(nullptr);
   return MMC_OK;
 }
+
+int64_t mmc_batch_evaluations(const mmc_batch* s) { return (s && s->sim) ? s->sim->evaluations() : 0; }
 
 mmc_ctx* mmc_batch_context(mmc_batch* s) { return (s && s->sim) ? s->sim->context() : nullptr; }
 

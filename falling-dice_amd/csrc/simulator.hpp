@@ -258,8 +258,13 @@ class Driver {
 class BatchSimulator {
  public:
   // generate: the lattices are built on the device (mmc_create_generated);
-  // otherwise on the host (mmc_lattice_build) and uploaded
-  BatchSimulator(std::vector<mmc_config> cfgs, int precision, bool generate = false);
+  // otherwise on the host (mmc_lattice_build) and uploaded.  shards: the
+  // lattices are split over that many independent contexts (lattice b to
+  // shard b mod shards), each with its own stream and host thread, so that a
+  // shard's rounds wait only for that shard's slowest lattice; 0 reads
+  // MMC_BATCH_SHARDS (default 1).  A lattice's results do not depend on the
+  // other lattices of its context, so they do not depend on the sharding.
+  BatchSimulator(std::vector<mmc_config> cfgs, int precision, bool generate = false, int shards = 0);
   ~BatchSimulator();
   BatchSimulator(const BatchSimulator&) = delete;
   BatchSimulator& operator=(const BatchSimulator&) = delete;
@@ -274,8 +279,12 @@ class BatchSimulator {
   std::vector<Outcome> run(bool verbose = false);
   int instances() const { return int(lats_.size()); }
   const HostLattice& lattice(int i) const { return lats_[size_t(i)]; }
-  mmc_ctx* context() const { return ctx_; }
-  int64_t rounds() const { return rounds_; }
+  // the device context (the first shard's when sharded)
+  mmc_ctx* context() const { return shards_.empty() ? ctx_ : shards_[0]->context(); }
+  int64_t rounds() const;
+  int shards() const { return shards_.empty() ? 1 : int(shards_.size()); }
+  // objective evaluations over every shard
+  int64_t evaluations() const;
   // Per-frame bodies of every lattice (mmc_batch_set_frame_observer); empty:
   // no capture.
   using FrameObserver = std::function<void(int lattice, int frame, const mmc_snapshot* bodies, int64_t count)>;
@@ -283,6 +292,9 @@ class BatchSimulator {
 
  private:
   FrameObserver observer_;
+  std::vector<std::unique_ptr<BatchSimulator>> shards_;  // empty: one context (ctx_)
+  std::vector<std::vector<int>> shard_lattices_;        // the lattices of each shard
+  std::vector<Outcome> run_shards(bool verbose);
   std::vector<mmc_snapshot> bodies_;  // host staging of one capture
   // The frames of the lattices in `mask` just committed (velocities unless
   // it is a lattice's first frame), handed to the observer; then the

@@ -351,6 +351,8 @@ def lib():
         L.mmc_batch_run.argtypes = [P, C.POINTER(Frame), C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                     C.c_int32]
         L.mmc_batch_set_frame_observer.argtypes = [P, FRAME_OBSERVER, P]
+        L.mmc_batch_evaluations.argtypes = [P]
+        L.mmc_batch_evaluations.restype = C.c_int64
         L.mmc_batch_context.argtypes = [P]
         L.mmc_batch_context.restype = P
         L.mmc_lattice_build.argtypes = [C.POINTER(Config), C.POINTER(Lattice), C.POINTER(P)]
@@ -728,7 +730,7 @@ class BatchSimulator:
         return lib().mmc_batch_context(self.h)
 
     def evaluations(self) -> int:
-        return int(lib().mmc_evaluations(self.context()))
+        return int(lib().mmc_batch_evaluations(self.h))
 
     def run(self, verbose=False, on_bodies=None):
         """Returns (frames per lattice, status per lattice, first error text).

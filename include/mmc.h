@@ -399,7 +399,12 @@ int mmc_batch_info(const mmc_batch* batch, int32_t i, int32_t* n_nodes, int32_t*
  * to stderr about every 10 s.  Returns MMC_OK unless the device fails. */
 int mmc_batch_run(mmc_batch* batch, mmc_frame* frames, int32_t cap, int32_t* n_frames, int32_t* status,
                   int32_t verbose);
+/* The device context of the batch (with MMC_BATCH_SHARDS > 1 the batch is
+ * split over that many contexts, lattice b in context b mod shards, each run
+ * by a host thread of its own on its own stream; this returns the first). */
 mmc_ctx* mmc_batch_context(mmc_batch* batch);
+/* Objective evaluations of the batch's contexts since creation. */
+int64_t mmc_batch_evaluations(const mmc_batch* batch);
 /* Frame observer of mmc_batch_run (no reference counterpart; the batched form
  * of SimulationResult::frames, MC/src/sim.cpp:436): after every committed
  * increment, fn(lattice, frame, bodies, count, user) receives that lattice's

@@ -11,7 +11,9 @@ mmc_minimize_batch (every lattice waits for the slowest of the round) or, with
 MMC_BATCH_PER_LATTICE=1 and the graph loop, advances every lattice through its
 own schedule (mmc_batch_start / advance / finish: a lattice starts its next
 solve while others are still in theirs); both must give the single runs'
-frames."""
+frames.  With MMC_BATCH_SHARDS > 1 the batch is split over independent
+contexts run by host threads of their own, which must change nothing
+either."""
 import numpy as np
 import pytest
 
@@ -75,3 +77,11 @@ def test_batch_per_lattice_isolates_a_failed_lattice(gpu, monkeypatch):
     monkeypatch.setenv("MMC_DEVICE_LOOP", "graph")
     monkeypatch.setenv("MMC_BATCH_PER_LATTICE", "1")
     test_batch_isolates_a_failed_lattice(gpu, "graph")
+
+
+@pytest.mark.parametrize("shards", [2, 3])
+def test_sharded_batch_equals_single_runs(gpu, shards, monkeypatch):
+    """The four lattices over two and three shards (contexts on their own
+    streams and host threads), lattice b in shard b mod shards."""
+    monkeypatch.setenv("MMC_BATCH_SHARDS", str(shards))
+    _batch_vs_single(gpu, "fp64")

@@ -85,14 +85,17 @@ def _batch_cfgs(mmc):
             mmc.cubic_config(4, 3, 3, load=3, unload=2, sigma=1e-4, radius_mode=1, r_bot=3e-4, r_top=7e-4)]
 
 
-@pytest.mark.parametrize("driver", ["rounds", "per_lattice"])
+@pytest.mark.parametrize("driver", ["rounds", "per_lattice", "sharded"])
 def test_batch_frames_carry_their_bodies(gpu, driver, monkeypatch):
     mmc = gpu
     if driver == "per_lattice":
         monkeypatch.setenv("MMC_DEVICE_LOOP", "graph")
         monkeypatch.setenv("MMC_BATCH_PER_LATTICE", "1")
     cfgs = _batch_cfgs(mmc)
+    if driver == "sharded":
+        monkeypatch.setenv("MMC_BATCH_SHARDS", "2")
     frames, bodies = _run_with_bodies(mmc, cfgs)
+    monkeypatch.delenv("MMC_BATCH_SHARDS", raising=False)
     assert set(bodies) == {(b, f) for b, fr in enumerate(frames) for f in range(len(fr))}
     for b, cfg in enumerate(cfgs):
         alone, ab = _run_with_bodies(mmc, [cfg])
