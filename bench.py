@@ -266,6 +266,16 @@ def run_batch(args, log):
     print(json.dumps(out))
 
 
+def reference_speed_ratio():
+    """The oracle's wall time over the compiled reference's on identical work,
+    recorded in the build container (tests/golden/make_speed_ratio.py)."""
+    path = ROOT / "tests" / "golden" / "oracle_speed_ratio.json"
+    if not path.exists():
+        return None
+    cases = json.loads(path.read_text())["cases"]
+    return {c["case"]: c["oracle_over_reference"] for c in cases}
+
+
 def cpu_baseline_sample(cfg, state, top, budget_s, gpu_iterations, steps, gpu_seconds):
     """The oracle (one host core) on the first timed increment, from the FP64
     leg's committed state after the warmup: its L-BFGS loop capped so that it
@@ -297,6 +307,7 @@ def cpu_baseline_sample(cfg, state, top, budget_s, gpu_iterations, steps, gpu_se
                       f"evaluations, {dt:.1f} s on one host core, {per_it * 1e3:.2f} ms per iteration), scaled to "
                       f"the {gpu_iterations} iterations of the {steps} timed increments",
             "seconds_per_iteration": round(per_it, 6), "projected_seconds": round(proj, 1),
+            "oracle_over_reference_time": reference_speed_ratio(),
             "gpu_seconds_per_iteration": round(gpu_seconds / max(1, gpu_iterations), 8),
             "gpu_over_cpu_per_iteration": round(per_it / max(gpu_seconds / max(1, gpu_iterations), 1e-12), 1)}
 
