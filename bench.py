@@ -431,6 +431,10 @@ def main():
     roof = {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": "k_gather_g",
             "bytes_per_launch": g_bytes, "mean_launch_us": round(gather_ms * 1e3, 3), "launches_timed": launches,
+            # the L-BFGS vector traffic alone (200 B per DOF at a full history,
+            # without the beam-end record reads of the gradient assembly)
+            "vector_bytes_per_launch": 6 * N * (5 + 2 * cfg.history) * 8,
+            "frac_vector_bytes": round(6 * N * (5 + 2 * cfg.history) * 8 / (gather_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
             "loop_kernel_us": loop_us, "trial_kernel_us": round(sum(loop_us.values()), 3),
             "loop_shape": sim.loop_shape(),
             "copy_16MiB_GBs": round(2 * (1 << 24) / (copy_ms * 1e-3) / 1e9, 1)}
