@@ -144,3 +144,25 @@ def test_cli_help_and_errors():
     r = subprocess.run([str(cli()), "--mode", "render", "--output", "/tmp/mmc_cli_test"], capture_output=True,
                        text=True)
     assert r.returncode == 1 and "render" in r.stderr
+
+
+def test_device_construction_and_capture_reject_bad_arguments(mmc):
+    """mmc_create_generated checks every configuration before it touches a
+    device (cell counts, family), and the capture, lattice-readback and
+    frame-observer entry points reject a missing context or batch."""
+    lib = mmc.lib()
+    h = C.c_void_p()
+    bad = mmc.cubic_config(2, 2, 2).replace(cells_x=0)
+    assert lib.mmc_create_generated(C.byref(bad), 1, 0, C.byref(h)) == mmc.MMC_ERR_ARG
+    assert "cell counts must be positive" in lib.mmc_last_error(None).decode()
+    fam = mmc.cubic_config(2, 2, 2).replace(lattice=7)
+    assert lib.mmc_create_generated(C.byref(fam), 1, 0, C.byref(h)) == mmc.MMC_ERR_ARG
+    assert "unknown lattice family" in lib.mmc_last_error(None).decode()
+    good = mmc.cubic_config(2, 2, 2)
+    assert lib.mmc_create_generated(C.byref(good), 0, 0, C.byref(h)) == mmc.MMC_ERR_ARG
+    assert lib.mmc_create_generated(C.byref(good), 1, 5, C.byref(h)) == mmc.MMC_ERR_ARG
+    assert lib.mmc_capture(None, None, None, None, None) == mmc.MMC_ERR_ARG
+    assert lib.mmc_capture_mark(None) == mmc.MMC_ERR_ARG
+    assert lib.mmc_lattice_info_get(None, 0, None) == mmc.MMC_ERR_ARG
+    assert lib.mmc_batch_set_frame_observer(None, mmc.FRAME_OBSERVER(), None) == mmc.MMC_ERR_ARG
+    assert lib.mmc_batch_evaluations(None) == 0
