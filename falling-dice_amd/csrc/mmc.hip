@@ -23,6 +23,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstddef>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -1227,10 +1228,17 @@ constexpr int kRec = 16;
 // The record of instance in from its 20 reduced partial rows: 9 of the state
 // kernel, 4 of the contact shapes, 4 of the beams, 3 of the gradient (fl: the
 // instance's flag word).
-__device__ __forceinline__ void record_from_rows(const Inst& in, const double* __restrict__ rows, int fl,
+// The instance constants a record needs (the decision kernels read them
+// ahead of the decision).
+struct RecConst {
+  int N;
+  double centering, n_free;
+};
+__device__ __forceinline__ RecConst rec_const(const Inst& in) { return {in.N, in.S.centering, in.n_free}; }
+__device__ __forceinline__ void record_from_rows(const RecConst& in, const double* __restrict__ rows, int fl,
                                                  const double* __restrict__ refrec, double* __restrict__ out) {
   const int N = in.N;
-  const double centering = in.S.centering, n_free = in.n_free;
+  const double centering = in.centering, n_free = in.n_free;
   const double* s = rows;
   const double* c = rows + 9;
   const double* b = rows + 13;
@@ -1353,7 +1361,7 @@ __device__ __forceinline__ void final_body(const Inst& in, const double* __restr
   if (threadIdx.x == 0) {
     const int fl = iflag[0];
     iflag[0] = 0;
-    record_from_rows(in, rows, fl, refrec, out);
+    record_from_rows(rec_const(in), rows, fl, refrec, out);
   }
 }
 // one block per instance; records at out + inst * kRec
@@ -1708,9 +1716,12 @@ __device__ inline void history_update_wave(Ctl& c, const double* d) {
     return;
   }
   // every read of the old order precedes the first write (one wavefront:
-  // its LDS accesses complete in program order)
+  // its LDS accesses complete in program order, and the fence keeps the
+  // compiler from moving a write above another lane's read)
   const int evicted = off ? c.order[0] : -1;
   const int j = l < nold ? c.order[l + off] : 0;
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
   if (l < nold) {
     c.SY[j][t] = d[4 * l + 0];
     c.YY[j][t] = c.YY[t][j] = d[4 * l + 1];
@@ -1820,7 +1831,13 @@ __host__ __device__ inline PairArgs pair_slots(int m, int count, int head) {
 // each step broadcasts one solved component and every lane updates its own
 // residual, so the dependent chain is k steps instead of k(k+1)/2.  Each lane
 // reads its rows of S'Y and Y'Y (and its column of S'Y) into registers first,
-// so a step is a division, a lane read and a multiply-add.
+// so a step is a division, a lane read and a multiply-add.  The division by
+// the lane's own diagonal element d goes through its reciprocal, computed
+// before the chain: q0 = r (1/d), then one correction q0 + (r - q0 d)(1/d),
+// which is the correctly rounded quotient r / d (Markstein, IBM J. Res. Dev.
+// 34, 1990: a reciprocal within half an ulp and a quotient within one ulp;
+// no overflow or underflow here), so the result is that of the division,
+// bit for bit, at three dependent operations per step.
 template <int MH = kMaxHist>
 __device__ void compact_coefficients_wave(const Ctl& c, DirCoef& C) {
   const int k = c.count, l = threadIdx.x & (kWave - 1);
@@ -1837,6 +1854,11 @@ __device__ void compact_coefficients_wave(const Ctl& c, DirCoef& C) {
   const double sy = c.SY[last][last], yy = c.YY[last][last];
   const double gamma = (sy > 1e-20 && yy > 1e-20) ? fmin(1e4, fmax(1e-7, sy / yy)) : 0.10;
   const double dl = on ? c.SY[sl][sl] : 1.0;
+  const double rdl = 1.0 / dl;
+  auto quot = [&](double x) {
+    const double q0 = x * rdl;
+    return fma(fma(-q0, dl, x), rdl, q0);
+  };
   // order[] entries past count are stale but valid slot numbers
   double syr[MH], syc[MH], yyr[MH];
 #pragma unroll
@@ -1851,27 +1873,26 @@ __device__ void compact_coefficients_wave(const Ctl& c, DirCoef& C) {
 #pragma unroll
   for (int i = MH - 1; i >= 0; i--) {
     if (i < k) {
-      const double wi = read_lane(r / dl, i);
+      const double wi = read_lane(quot(r), i);
       if (l == i) w = wi;
       if (l < i) r -= syr[i] * wi;
     }
   }
-  // t = (D + gamma Y'Y) w - gamma v
+  // t = (D + gamma Y'Y) w - gamma v (the lane reads first, all independent of
+  // the accumulation; lanes past k hold w = 0 and are not added)
   double t = on ? c.SY[sl][sl] * w - gamma * c.v[sl] : 0.0;
+  double wr[MH];
 #pragma unroll
-  for (int j = 0; j < MH; j++) {
-    if (j < k) {
-      const double wj = read_lane(w, j);
-      if (on) t += gamma * yyr[j] * wj;
-    }
-  }
+  for (int j = 0; j < MH; j++) wr[j] = read_lane(w, j);
+#pragma unroll
+  for (int j = 0; j < MH; j++) t = (on && j < k) ? t + gamma * yyr[j] * wr[j] : t;
   // a = R^{-T} t
   double a = 0.0;
   r = t;
 #pragma unroll
   for (int i = 0; i < MH; i++) {
     if (i < k) {
-      const double ai = read_lane(r / dl, i);
+      const double ai = read_lane(quot(r), i);
       if (l == i) a = ai;
       if (on && l > i) r -= syc[i] * ai;
     }
@@ -2131,16 +2152,26 @@ __device__ __forceinline__ void ctl_store(Ctl* C, const Ctl& sc) {
 // calls it; returns after a barrier); decide_block copies the control block
 // in from global memory and back.
 template <int NT, int MH = kMaxHist>
-__device__ void decide_core(Ctl& sc, const Inst& in, const double* rows, int* __restrict__ iflag) {
-  __shared__ int accepted, need_coef;
-  const int fl = *iflag;
+__device__ void decide_core(Ctl& sc, const RecConst& in, const double* rows, int fl, int* __restrict__ iflag) {
+  static_assert(NT >= 2 * kWave, "the decision runs on two wavefronts");
+  __shared__ int accepted;
+  // The pair update and the next direction's coefficients do not depend on
+  // the acceptance test, only on whether it passes: wavefront 1 computes them
+  // on a copy of the control block (spec) while thread 0 decides, and they
+  // are kept on acceptance, the common outcome.  Coefficients solved after an
+  // acceptance that ends the solve (converged, iteration cap) are unused.
+  __shared__ Ctl spec;
+  constexpr int kHistWord = int(offsetof(Ctl, coef) / 8);  // coef, order, SY, YY, u, v to the end
+  static_assert(offsetof(Ctl, coef) % 8 == 0, "the history tail is copied as 8-byte words");
+  __syncthreads();
+  for (int w = threadIdx.x; w < kCtlWords; w += NT)
+    reinterpret_cast<double*>(&spec)[w] = reinterpret_cast<const double*>(&sc)[w];
   __syncthreads();
   if (threadIdx.x == 0) {
     Ctl* c = &sc;
     *iflag = 0;
     c->evals++;
     accepted = 0;
-    need_coef = 0;
     bool discard = false;
     if (c->phase != PH_SEARCH) {
       const double dg = rows[20];
@@ -2169,6 +2200,22 @@ __device__ void decide_core(Ctl& sc, const Inst& in, const double* rows, int* __
           ok = r[14] <= (2.0 * c->armijo - 1.0) * c->dg;
         if (ok) {
           accepted = 1;
+          c->par ^= 1;
+          for (int k = 0; k < kRec; k++) c->cur[k] = c->cand[k];
+          c->it++;
+          c->retry = 0;
+          if (c->it >= c->max_it) {
+            c->status = ST_ITERCAP;
+          } else {
+            c->iterations = c->it + 1;
+            const double* rc = c->cur;
+            const double rel = rc[6] / fmax(1e-6, c->ts * fmax(1.0, rc[4]));
+            const bool conv = (rc[6] <= c->tol_rms && rc[7] <= c->tol_max) || (c->rel_escape && rel < c->rel_tol);
+            if (conv && rc[5] <= c->pen_tol)
+              c->status = ST_CONVERGED;
+            else
+              c->phase = PH_DIRECTION;
+          }
         } else {
           c->step *= 0.5;
           c->bt++;
@@ -2184,43 +2231,33 @@ __device__ void decide_core(Ctl& sc, const Inst& in, const double* rows, int* __
         }
       }
     }
+  } else if (threadIdx.x >= kWave && threadIdx.x < 2 * kWave) {
+    // pair update from the products in rows 21.. and the coefficients, by one
+    // wavefront: its LDS accesses complete in program order, and the fence
+    // keeps the compiler from moving a lane's reads of the updated history
+    // (other lanes' writes) above the update
+    if (spec.m > 0) history_update_wave(spec, rows + 21);
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    compact_coefficients_wave<MH>(spec, spec.coef);
   }
   __syncthreads();
   if (accepted) {
-    // pair update from the products in rows 21.. (one wavefront)
-    if (sc.m > 0 && threadIdx.x < kWave) history_update_wave(sc, rows + 21);
-    __syncthreads();
+    for (int w = kHistWord + threadIdx.x; w < kCtlWords; w += NT)
+      reinterpret_cast<double*>(&sc)[w] = reinterpret_cast<const double*>(&spec)[w];
     if (threadIdx.x == 0) {
-      Ctl* c = &sc;
-      c->par ^= 1;
-      for (int k = 0; k < kRec; k++) c->cur[k] = c->cand[k];
-      c->it++;
-      c->retry = 0;
-      if (c->it >= c->max_it) {
-        c->status = ST_ITERCAP;
-      } else {
-        c->iterations = c->it + 1;
-        const double* r = c->cur;
-        const double rel = r[6] / fmax(1e-6, c->ts * fmax(1.0, r[4]));
-        const bool ok = (r[6] <= c->tol_rms && r[7] <= c->tol_max) || (c->rel_escape && rel < c->rel_tol);
-        if (ok && r[5] <= c->pen_tol) {
-          c->status = ST_CONVERGED;
-        } else {
-          need_coef = 1;
-          c->phase = PH_DIRECTION;
-        }
-      }
+      sc.count = spec.count;
+      sc.spare = spec.spare;
     }
-    __syncthreads();
-    if (need_coef && threadIdx.x < kWave) compact_coefficients_wave<MH>(sc, sc.coef);
   }
   __syncthreads();
 }
 template <int NT, int MH = kMaxHist>
-__device__ void decide_block(Ctl* __restrict__ C, const Inst& in, const double* rows, int* __restrict__ iflag) {
+__device__ void decide_block(Ctl* __restrict__ C, const RecConst& in, const double* rows, int fl,
+                             int* __restrict__ iflag) {
   __shared__ Ctl sc;
   ctl_load<NT>(sc, C);
-  decide_core<NT, MH>(sc, in, rows, iflag);
+  decide_core<NT, MH>(sc, in, rows, fl, iflag);
   ctl_store<NT>(C, sc);
 }
 
@@ -2340,6 +2377,8 @@ __global__ __launch_bounds__(kElemThreads) void k_gather_g(Lat<T> L, const Inst*
   if (int(threadIdx.x) < NR) part[size_t(threadIdx.x) * Pd + blockIdx.x] = t;
   if constexpr (FD) {
     constexpr int NW = kElemThreads / kWave;
+    const int fl = fa.iflags[bk.inst];
+    const RecConst rc = rec_const(in);
     const GRng g = fa.G[bk.inst];
     const int q = blockIdx.x - g.pd0, nblk = g.pd1 - g.pd0;
     const GraphParts P{fa.dspart, fa.cpart, fa.bpart, part, Pd, fa.Pc, fa.Pb};
@@ -2369,7 +2408,7 @@ __global__ __launch_bounds__(kElemThreads) void k_gather_g(Lat<T> L, const Inst*
                                  [&](int jj, double v) { rows[gather_row(jj)] = v; });
     if (int(threadIdx.x) < kEarlyRows) rows[early_row(threadIdx.x)] = drow[early_row(threadIdx.x)];
     __syncthreads();
-    decide_block<kElemThreads, MH>(Cs + bk.inst, in, rows, fa.iflags + bk.inst);
+    decide_block<kElemThreads, MH>(Cs + bk.inst, rc, rows, fl, fa.iflags + bk.inst);
   }
 }
 
@@ -2396,6 +2435,10 @@ __global__ __launch_bounds__(kBlock) void k_decide_g(Ctl* __restrict__ Cs, const
   Ctl* C = Cs + b;
   const Inst& in = I[b];
   if (!ctl_trial_phase(C)) return;
+  // what the deciding block needs besides the rows, read ahead (the release
+  // fence below waits for these loads)
+  const int fl = iflags[b];
+  const RecConst rc = rec_const(in);
   __shared__ int last;
   {
     const int m = C->m, count = C->count;
@@ -2422,7 +2465,7 @@ __global__ __launch_bounds__(kBlock) void k_decide_g(Ctl* __restrict__ Cs, const
   __shared__ double rows[kGDecRows];
   if (int(threadIdx.x) < kGDecRows) rows[threadIdx.x] = drows[size_t(b) * kGDecRows + threadIdx.x];
   __syncthreads();
-  decide_block<kBlock>(C, in, rows, iflags + b);
+  decide_block<kBlock>(C, rc, rows, fl, iflags + b);
 }
 
 // ---------------------------------------------------------------------------
@@ -2687,7 +2730,7 @@ __global__ __launch_bounds__(kSolveThreads) void k_solve_block(
       if (tid < NR) rows[tid < 3 ? 17 + tid : 18 + tid] = tt;
     }
     __syncthreads();
-    decide_core<kSolveThreads, MH>(sc, in, rows, iflags + b);
+    decide_core<kSolveThreads, MH>(sc, rec_const(in), rows, iflags[b], iflags + b);
   }
   ctl_store<kSolveThreads>(Cs + b, sc);
 }
