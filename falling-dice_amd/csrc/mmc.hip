@@ -2415,8 +2415,9 @@ __global__ __launch_bounds__(kElemThreads) void k_gather_g(Lat<T> L, const Inst*
 // The decision of one trial: D blocks per instance, each folding its
 // share of the decision rows (rows_pass) into drows, so that the row loads are
 // spread over several compute units; the block that arrives last at the
-// instance's counter reads the folded rows and decides (decide_block), and
-// resets the counter for the next trial.  Arrival: a block barrier completes
+// instance's counter reads the folded rows and decides (decide_core on the
+// control block, which every block reads at its start), and resets the
+// counter for the next trial.  Arrival: a block barrier completes
 // the block's row stores, one thread releases them (device-scope release
 // fence: L2 write-back, for the other XCDs) and counts the arrival; only the
 // last arriver acquires (invalidates its cached lines) before reading.
@@ -2436,9 +2437,15 @@ __global__ __launch_bounds__(kBlock) void k_decide_g(Ctl* __restrict__ Cs, const
   const Inst& in = I[b];
   if (!ctl_trial_phase(C)) return;
   // what the deciding block needs besides the rows, read ahead (the release
-  // fence below waits for these loads)
+  // fence below waits for these loads): the flag word, the instance
+  // constants and the control block (only the deciding block writes it)
   const int fl = iflags[b];
   const RecConst rc = rec_const(in);
+  constexpr int RD = (kCtlWords + kBlock - 1) / kBlock;
+  double cr[RD];
+#pragma unroll
+  for (int q = 0; q < RD; q++)
+    cr[q] = reinterpret_cast<const double*>(C)[min(int(threadIdx.x) + q * kBlock, kCtlWords - 1)];
   __shared__ int last;
   {
     const int m = C->m, count = C->count;
@@ -2463,9 +2470,16 @@ __global__ __launch_bounds__(kBlock) void k_decide_g(Ctl* __restrict__ Cs, const
   __syncthreads();
   if (!last) return;
   __shared__ double rows[kGDecRows];
+  __shared__ Ctl sc;
   if (int(threadIdx.x) < kGDecRows) rows[threadIdx.x] = drows[size_t(b) * kGDecRows + threadIdx.x];
+#pragma unroll
+  for (int q = 0; q < RD; q++) {
+    const int w = threadIdx.x + q * kBlock;
+    if (w < kCtlWords) reinterpret_cast<double*>(&sc)[w] = cr[q];
+  }
   __syncthreads();
-  decide_block<kBlock>(C, rc, rows, fl, iflags + b);
+  decide_core<kBlock>(sc, rc, rows, fl, iflags + b);
+  ctl_store<kBlock>(C, sc);
 }
 
 // ---------------------------------------------------------------------------
