@@ -233,7 +233,8 @@ __global__ __launch_bounds__(kBlock) void k_count(Lat<T> L, const Inst* __restri
   double pad = r + S.margin + 0.5 * S.skin;
   float lo[3] = {round_down(fmin(a.x, b.x) - pad), round_down(fmin(a.y, b.y) - pad), round_down(fmin(a.z, b.z) - pad)};
   float hi[3] = {round_up(fmax(a.x, b.x) + pad), round_up(fmax(a.y, b.y) + pad), round_up(fmax(a.z, b.z) + pad)};
-  reinterpret_cast<float4*>(box)[2 * s] = make_float4(lo[0], lo[1], lo[2], 0.f);
+  // (the low corner's w carries the shape's instance, for the pair walk)
+  reinterpret_cast<float4*>(box)[2 * s] = make_float4(lo[0], lo[1], lo[2], __int_as_float(bk.inst));
   reinterpret_cast<float4*>(box)[2 * s + 1] = make_float4(hi[0], hi[1], hi[2], 0.f);
   int c0[3], c1[3];
   bool ok = true;
@@ -257,7 +258,8 @@ __global__ __launch_bounds__(kBlock) void k_count(Lat<T> L, const Inst* __restri
       for (int i = c0[0]; i <= c1[0]; i++) atomicAdd(&cnt[cell_hash(i, j, k, bk.inst, S.table_mask)], 1u);
 }
 
-// Boxes are packed as two float4 (lo, hi) per shape: one cache line per lookup.
+// Boxes are packed as two float4 (lo, hi) per shape: one cache line per lookup
+// (lo.w: the shape's instance).
 __device__ __forceinline__ bool load_cells(const float* box, int s, double inv, float lo[3], float hi[3], int c0[3],
                                            int c1[3]) {
   const float4 l = reinterpret_cast<const float4*>(box)[2 * s], h = reinterpret_cast<const float4*>(box)[2 * s + 1];
@@ -357,7 +359,7 @@ constexpr int kPairShapes = kBlock / kPairLanes;          // shapes per workgrou
 constexpr int kPairSub = kBlock / kPairShapes;            // workgroups per table block
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_pairs(Lat<T> L, const Inst* __restrict__ I, const Blk* __restrict__ tc,
-                                                  int slab_in, const int* __restrict__ sinst,
+                                                  int slab_in,
                                                   const double* __restrict__ x, const float* __restrict__ box,
                                                   const unsigned* __restrict__ start, const int* __restrict__ entries,
                                                   unsigned* __restrict__ pcount, int* __restrict__ plist,
@@ -407,27 +409,39 @@ __global__ __launch_bounds__(kBlock) void k_pairs(Lat<T> L, const Inst* __restri
     unsigned rounds = len;
 #pragma unroll
     for (int o = kWave / 2; o > 0; o >>= 1) rounds = max(rounds, unsigned(__shfl_xor(int(rounds), o, kWave)));
+    // the bucket walk as a two-stage pipeline: while round r's candidate is
+    // tested, the entry of round r + 2 and the box of round r + 1's entry are
+    // already in flight, so that a round waits for one memory latency at most
+    const float4* box4 = reinterpret_cast<const float4*>(box);
+    int on1 = len > 0 ? entries[e0] : -1, on2 = len > 1 ? entries[e0 + 1] : -1;
+    float4 bl = make_float4(0.f, 0.f, 0.f, 0.f), bh = bl;
+    if (on1 >= 0) {
+      bl = box4[2 * on1];
+      bh = box4[2 * on1 + 1];
+    }
     for (unsigned r = 0; r < rounds; r++) {
       bool found = false;
-      int o = -1;
-      if (r < len) {
-        o = entries[e0 + r];
-        if (o != s && sinst[o] == bk.inst) {
-          const float4 ol = reinterpret_cast<const float4*>(box)[2 * o];
-          const float4 oh = reinterpret_cast<const float4*>(box)[2 * o + 1];
-          const float olo[3] = {ol.x, ol.y, ol.z}, ohi[3] = {oh.x, oh.y, oh.z};
-          if (olo[0] <= hi[0] && lo[0] <= ohi[0] && olo[1] <= hi[1] && lo[1] <= ohi[1] && olo[2] <= hi[2] &&
-              lo[2] <= ohi[2] && cell_of(fmaxf(lo[0], olo[0]), S.inv_cell) == ci &&
-              cell_of(fmaxf(lo[1], olo[1]), S.inv_cell) == cj && cell_of(fmaxf(lo[2], olo[2]), S.inv_cell) == ck) {
-            v3<double> oa, ob;
-            double orr;
-            int o0, o1;
-            shape_geom(L, in, x, o, oa, ob, orr, o0, o1);
-            const bool low = s < o, ocap = o - in.s0 >= in.N;
-            closest cp = low ? shape_closest(scap, sa, sb, ocap, oa, ob) : shape_closest(ocap, oa, ob, scap, sa, sb);
-            const double dist = norm(cp.pb - cp.pa);
-            found = dist < sr + orr + S.margin + S.skin && !excluded(L, s0, s1, o0, o1);
-          }
+      const int o = on1;
+      const float4 ol = bl, oh = bh;
+      on1 = on2;
+      on2 = r + 2 < len ? entries[e0 + r + 2] : -1;
+      if (on1 >= 0) {
+        bl = box4[2 * on1];
+        bh = box4[2 * on1 + 1];
+      }
+      if (o >= 0 && o != s && __float_as_int(ol.w) == bk.inst) {
+        const float olo[3] = {ol.x, ol.y, ol.z}, ohi[3] = {oh.x, oh.y, oh.z};
+        if (olo[0] <= hi[0] && lo[0] <= ohi[0] && olo[1] <= hi[1] && lo[1] <= ohi[1] && olo[2] <= hi[2] &&
+            lo[2] <= ohi[2] && cell_of(fmaxf(lo[0], olo[0]), S.inv_cell) == ci &&
+            cell_of(fmaxf(lo[1], olo[1]), S.inv_cell) == cj && cell_of(fmaxf(lo[2], olo[2]), S.inv_cell) == ck) {
+          v3<double> oa, ob;
+          double orr;
+          int o0, o1;
+          shape_geom(L, in, x, o, oa, ob, orr, o0, o1);
+          const bool low = s < o, ocap = o - in.s0 >= in.N;
+          closest cp = low ? shape_closest(scap, sa, sb, ocap, oa, ob) : shape_closest(ocap, oa, ob, scap, sa, sb);
+          const double dist = norm(cp.pb - cp.pa);
+          found = dist < sr + orr + S.margin + S.skin && !excluded(L, s0, s1, o0, o1);
         }
       }
       const unsigned long long m = __ballot(found) & segmask;
@@ -2991,7 +3005,6 @@ struct mmc_ctx {
   // instances and block tables
   Inst* inst_d = nullptr;
   Blk *tn = nullptr, *tb = nullptr, *tc = nullptr, *td = nullptr;
-  int* sinst = nullptr;       // instance of every shape
   double* tops_d = nullptr;   // platen of every instance (host-driven evaluations)
   double* htops = nullptr;    // pinned staging of tops_d
   int* mask_d = nullptr;      // commit mask
@@ -3195,7 +3208,7 @@ mmc_ctx::~mmc_ctx() {
   void* ptrs[] = {bpos, bquat, mass, lock, ba, bb, L0, rad, area, inert, e0, plastic, inc_off, inc, adj_off, adj, x, R,
                   rec, trial, cf_sph, cf_cap, box, cnt, start, entries, flags, scan_tmp, spart, cpart, bpart, gpart,
                   vpart, stb, stn, stout, record, scal, z, g, p, zc, gc, zt, zin, Sh, Yh, xref, pcount, plist,
-                  inst_d, tn, tb, tc, td, sinst, tops_d, mask_d, smask_d, ctl, lref, dspart, cent, cflag, hlist, hcount, drows, dcount, gtd, gtb, gtc,
+                  inst_d, tn, tb, tc, td, tops_d, mask_d, smask_d, ctl, lref, dspart, cent, cflag, hlist, hcount, drows, dcount, gtd, gtb, gtc,
                   grng, gact, tctl, rtc, rtn, eslot, erec, ecap, cal0, cal1, nlayer, blayer, bfam, prev_x, prev_q,
                   snap, capt};
   for (void* q : ptrs)
@@ -3581,11 +3594,6 @@ void mmc_ctx::finish_setup(const std::vector<int>& off, const std::vector<int>& 
     gtc = dalloc<Blk>(htc.size() * kBW);
     grng = dalloc<GRng>(n);
     gact = dalloc<int>(n);
-    std::vector<int> si(NS);
-    for (int b = 0; b < n; b++)
-      for (int s = 0; s < hinst[b].NS; s++) si[hinst[b].s0 + s] = b;
-    sinst = dalloc<int>(NS);
-    HIPCALL(hipMemcpy(sinst, si.data(), NS * sizeof(int), hipMemcpyHostToDevice));
   }
   // ---- evaluation buffers ----
   x = dalloc<double>(3 * size_t(N));
@@ -3708,7 +3716,7 @@ void mmc_ctx::build_list(Lat<T> Lt, const Blk* ctab, int cblocks, const Blk* nta
   tick(MMC_K_FILL);
   k_fill<<<cblocks, kBlock, 0, stream>>>(inst_d, ctab, box, cnt, start, entries, flags);
   tock(MMC_K_FILL);
-  k_pairs<T><<<cblocks * kPairSub, kBlock, 0, stream>>>(Lt, inst_d, ctab, slab, sinst, x, box, start, entries,
+  k_pairs<T><<<cblocks * kPairSub, kBlock, 0, stream>>>(Lt, inst_d, ctab, slab, x, box, start, entries,
                                                         pcount, plist, flags);
   int fl = 0;
   read(&fl, flags, sizeof(int));
