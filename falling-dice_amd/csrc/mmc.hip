@@ -347,8 +347,9 @@ __device__ __forceinline__ closest shape_closest(bool capA, v3<double> a0, v3<do
 // skin, written to the shape's slab of S.slab slots; pcount gets the true
 // count, and a count above the slab raises flag 8 (the host widens the slab
 // and rebuilds).  kPairLanes lanes per shape (four shapes per wavefront): lane
-// c of a shape's segment takes cells c, c + kPairLanes, ... of the shape's
-// box, and the lanes walk their buckets in step, one entry each per round; a
+// c of a shape's segment looks up cells c, c + kPairLanes, ... of the shape's
+// box, kPairLanes cells at a time, and the segment then walks those cells'
+// bucket entries kPairLanes at a time, one entry per lane per round; a
 // round's partners are appended in lane order.  Every overlapping pair is
 // kept only in the cell that holds the low corner of the two boxes' overlap
 // (once per partner); the segment's first lane then sorts the list and drops
@@ -406,34 +407,43 @@ __global__ __launch_bounds__(kBlock) void k_pairs(Lat<T> L, const Inst* __restri
       e1 = max(e0, min(start[h + 1], (unsigned)S.entry_cap));
     }
     const unsigned len = e1 - e0;
-    unsigned rounds = len;
+    // the segment's bucket entries over these cells, shared out evenly: item
+    // t (running count over the lanes' cells) is taken by lane t % kPairLanes
+    // in round t / kPairLanes, which finds the item's cell by a binary search
+    // over the lanes' running counts (the segment's longest bucket no longer
+    // sets the number of rounds)
+    unsigned excl = len;
+#pragma unroll
+    for (int d = 1; d < kPairLanes; d <<= 1) {
+      const unsigned v = unsigned(__shfl_up(int(excl), d, kPairLanes));
+      if (sl >= d) excl += v;
+    }
+    const unsigned total = unsigned(__shfl(int(excl), kPairLanes - 1, kPairLanes));
+    excl -= len;
+    unsigned rounds = (total + kPairLanes - 1) / kPairLanes;
 #pragma unroll
     for (int o = kWave / 2; o > 0; o >>= 1) rounds = max(rounds, unsigned(__shfl_xor(int(rounds), o, kWave)));
-    // the bucket walk as a two-stage pipeline: while round r's candidate is
-    // tested, the entry of round r + 2 and the box of round r + 1's entry are
-    // already in flight, so that a round waits for one memory latency at most
     const float4* box4 = reinterpret_cast<const float4*>(box);
-    int on1 = len > 0 ? entries[e0] : -1, on2 = len > 1 ? entries[e0 + 1] : -1;
-    float4 bl = make_float4(0.f, 0.f, 0.f, 0.f), bh = bl;
-    if (on1 >= 0) {
-      bl = box4[2 * on1];
-      bh = box4[2 * on1 + 1];
-    }
     for (unsigned r = 0; r < rounds; r++) {
+      const unsigned item = r * kPairLanes + sl;
+      int own = 0;  // the last lane whose running count is at most item
+#pragma unroll
+      for (int step = kPairLanes / 2; step > 0; step >>= 1)
+        if (unsigned(__shfl(int(excl), own + step, kPairLanes)) <= item) own += step;
+      const unsigned oe = unsigned(__shfl(int(e0), own, kPairLanes)) + item -
+                          unsigned(__shfl(int(excl), own, kPairLanes));
+      const int oci = __shfl(ci, own, kPairLanes), ocj = __shfl(cj, own, kPairLanes),
+                ock = __shfl(ck, own, kPairLanes);
       bool found = false;
-      const int o = on1;
-      const float4 ol = bl, oh = bh;
-      on1 = on2;
-      on2 = r + 2 < len ? entries[e0 + r + 2] : -1;
-      if (on1 >= 0) {
-        bl = box4[2 * on1];
-        bh = box4[2 * on1 + 1];
-      }
+      int o = -1;
+      if (item < total) o = entries[oe];
+      const float4 ol = o >= 0 ? box4[2 * o] : make_float4(0.f, 0.f, 0.f, 0.f);
+      const float4 oh = o >= 0 ? box4[2 * o + 1] : ol;
       if (o >= 0 && o != s && __float_as_int(ol.w) == bk.inst) {
         const float olo[3] = {ol.x, ol.y, ol.z}, ohi[3] = {oh.x, oh.y, oh.z};
         if (olo[0] <= hi[0] && lo[0] <= ohi[0] && olo[1] <= hi[1] && lo[1] <= ohi[1] && olo[2] <= hi[2] &&
-            lo[2] <= ohi[2] && cell_of(fmaxf(lo[0], olo[0]), S.inv_cell) == ci &&
-            cell_of(fmaxf(lo[1], olo[1]), S.inv_cell) == cj && cell_of(fmaxf(lo[2], olo[2]), S.inv_cell) == ck) {
+            lo[2] <= ohi[2] && cell_of(fmaxf(lo[0], olo[0]), S.inv_cell) == oci &&
+            cell_of(fmaxf(lo[1], olo[1]), S.inv_cell) == ocj && cell_of(fmaxf(lo[2], olo[2]), S.inv_cell) == ock) {
           v3<double> oa, ob;
           double orr;
           int o0, o1;
