@@ -2446,8 +2446,9 @@ __global__ __launch_bounds__(kElemThreads) void k_gather_g(Lat<T> L, const Inst*
 // fence: L2 write-back, for the other XCDs) and counts the arrival; only the
 // last arriver acquires (invalidates its cached lines) before reading.
 // D: blocks per instance, chosen by the loop's tuner (any D gives the same
-// rows: each row is folded by one wavefront in one fixed order).
-template <int D>
+// rows: each row is folded by one wavefront in one fixed order).  MH: the
+// history capacity the decision's coefficient solve is built for.
+template <int D, int MH>
 __global__ __launch_bounds__(kBlock) void k_decide_g(Ctl* __restrict__ Cs, const Inst* __restrict__ I,
                                                      const int* __restrict__ act, const GRng* __restrict__ G,
                                                      const double* __restrict__ dspart, int Pd,
@@ -2477,8 +2478,11 @@ __global__ __launch_bounds__(kBlock) void k_decide_g(Ctl* __restrict__ Cs, const
     const GraphParts P{dspart, cpart, bpart, gpart, Pd, Pc, Pb};
     const GRng g = G[b];
     // first-round columns per lane: as many as 32 registers per row pair hold
-    // (the beam and contact rows have one column per 64 items)
-    constexpr int RD = NQ <= 2 ? 16 : (NQ <= 4 ? 8 : kDecColRounds);
+    // (the beam and contact rows have one column per 64 items); with many
+    // rows per wave (few blocks per lattice, chosen for batches of small
+    // lattices) two, so that the fold's registers leave room for more
+    // resident blocks
+    constexpr int RD = NQ <= 2 ? 16 : (NQ <= 4 ? 8 : 2);
     double* out = drows + size_t(b) * kGDecRows;
     rows_pass<NQ, RD>([&](int j) { return graph_row(j, g, P); }, dspart + g.pd0, R, part * kDecWaves, D * kDecWaves,
                       [&](int j, double t) { out[j] = t; });
@@ -2502,7 +2506,7 @@ __global__ __launch_bounds__(kBlock) void k_decide_g(Ctl* __restrict__ Cs, const
     if (w < kCtlWords) reinterpret_cast<double*>(&sc)[w] = cr[q];
   }
   __syncthreads();
-  decide_core<kBlock>(sc, rc, rows, fl, iflags + b);
+  decide_core<kBlock, MH>(sc, rc, rows, fl, iflags + b);
   ctl_store<kBlock>(C, sc);
 }
 
@@ -4257,22 +4261,23 @@ void mmc_ctx::classify() {
 void mmc_ctx::launch_decide(Ctl* C) {
   const int na = int(gset.size());
   const int cw = gPc, bw = gPb * kBW;  // columns of the partials (one per 64 items)
+  auto go = [&](auto kern, int d) {
+    kern<<<na * d, kBlock, 0, stream>>>(C, inst_d, gact, grng, dspart, gPd, cpart, cw, bpart, bw, gpart, iflags(),
+                                        drows, dcount);
+  };
+  const bool small = max_hist <= 10;
   switch (dec_split) {
     case 2:
-      k_decide_g<2><<<na * 2, kBlock, 0, stream>>>(C, inst_d, gact, grng, dspart, gPd, cpart, cw, bpart, bw, gpart,
-                                                   iflags(), drows, dcount);
+      small ? go(k_decide_g<2, 10>, 2) : go(k_decide_g<2, kMaxHist>, 2);
       break;
     case 4:
-      k_decide_g<4><<<na * 4, kBlock, 0, stream>>>(C, inst_d, gact, grng, dspart, gPd, cpart, cw, bpart, bw, gpart,
-                                                   iflags(), drows, dcount);
+      small ? go(k_decide_g<4, 10>, 4) : go(k_decide_g<4, kMaxHist>, 4);
       break;
     case 16:
-      k_decide_g<16><<<na * 16, kBlock, 0, stream>>>(C, inst_d, gact, grng, dspart, gPd, cpart, cw, bpart, bw,
-                                                     gpart, iflags(), drows, dcount);
+      small ? go(k_decide_g<16, 10>, 16) : go(k_decide_g<16, kMaxHist>, 16);
       break;
     default:
-      k_decide_g<8><<<na * 8, kBlock, 0, stream>>>(C, inst_d, gact, grng, dspart, gPd, cpart, cw, bpart, bw, gpart,
-                                                   iflags(), drows, dcount);
+      small ? go(k_decide_g<8, 10>, 8) : go(k_decide_g<8, kMaxHist>, 8);
   }
 }
 
