@@ -1536,6 +1536,7 @@ __device__ __forceinline__ void direction_body(int N, size_t n6, const double* _
 // from LDS.  Same arithmetic per element as direction_body.  Writes p and
 // returns this element's direction (0 past the block's nodes) and g.p term.
 constexpr int kDirNodes = kElemNodes;
+template <int MH>
 __device__ __forceinline__ double direction_elem(size_t e, bool on, bool locked, double gv, double cg, int k,
                                                  const DirCoef& C, const double* __restrict__ Sh,
                                                  const double* __restrict__ Yh, size_t n6, double mt, double mr,
@@ -1545,9 +1546,9 @@ __device__ __forceinline__ double direction_elem(size_t e, bool on, bool locked,
   double q = 0.0;
   if (on) {
     // every history load issued before the first use
-    double hs[kMaxHist], hy[kMaxHist];
+    double hs[MH], hy[MH];
 #pragma unroll
-    for (int j = 0; j < kMaxHist; j++) {
+    for (int j = 0; j < MH; j++) {
       if (j < k) {
         hs[j] = Sh[n6 * C.slot[j] + e];
         hy[j] = Yh[n6 * C.slot[j] + e];
@@ -1555,7 +1556,7 @@ __device__ __forceinline__ double direction_elem(size_t e, bool on, bool locked,
     }
     q = cg * gv;
 #pragma unroll
-    for (int j = 0; j < kMaxHist; j++)
+    for (int j = 0; j < MH; j++)
       if (j < k) q += C.cs[j] * hs[j] + C.cy[j] * hy[j];
     q = -q;
     if (locked) q = 0.0;
@@ -1946,7 +1947,7 @@ __device__ __forceinline__ bool ctl_trial_phase(const Ctl* C) {
 // phase SEARCH takes the stored p at the control block's step.  Wave 0 then
 // evaluates the pose of the block's nodes (state_node: partial rows 0..8 as
 // k_state).  partials [10][Pd].
-template <class T>
+template <class T, int MH>
 __global__ __launch_bounds__(kElemThreads) void k_dstate_g(Lat<T> L, const Inst* __restrict__ I,
                                                            const Blk* __restrict__ td, const Ctl* __restrict__ Cs,
                                                            double* __restrict__ z0, double* __restrict__ z1,
@@ -1980,7 +1981,7 @@ __global__ __launch_bounds__(kElemThreads) void k_dstate_g(Lat<T> L, const Inst*
     step = 1.0;
     const double gv = (par ? g1 : g0)[e];
     const bool steep = phase == PH_STEEPEST;
-    q = direction_elem(e, on, on && locked, gv, steep ? 1.0 : C->coef.cg, steep ? 0 : C->coef.k, C->coef, Sh, Yh,
+    q = direction_elem<MH>(e, on, on && locked, gv, steep ? 1.0 : C->coef.cg, steep ? 0 : C->coef.k, C->coef, Sh, Yh,
                        n6, in.mt, in.mr, p, dv);
   }
   // trial DOF value, as state_body
@@ -3074,6 +3075,8 @@ struct mmc_ctx {
   // gather with the decision fused
   double tune_dec_us[5] = {0, 0, 0, 0, 0};
   template <class T>
+  void launch_dstate(Lat<T> Lt, const Ctl* C);
+  template <class T>
   void launch_elem(Lat<T> Lt, const Ctl* C);
   template <class T>
   void launch_gather(Lat<T> Lt, Ctl* C);
@@ -4115,11 +4118,9 @@ void mmc_ctx::capture_iteration(Lat<T> Lt) {
   if (graph) HIPCALL(hipGraphDestroy(graph));
   graph_exec = nullptr;
   graph = nullptr;
-  const size_t n6 = 6 * size_t(N);
   HIPCALL(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
   for (int u = 0; u < kGraphIterations; u++) {
-    k_dstate_g<T><<<gPd, kElemThreads, 0, stream>>>(Lt, inst_d, gtd, ctl, zbuf[0], zbuf[1], gbuf[0], gbuf[1], Sh,
-                                                    Yh, n6, p, x, (double*)R, xref, nbuf[0], nbuf[1], dspart);
+    launch_dstate<T>(Lt, ctl);
     launch_elem<T>(Lt, ctl);
     launch_gather<T>(Lt, ctl);
     if (dec_split != 0) launch_decide(ctl);
@@ -4235,6 +4236,20 @@ void mmc_ctx::launch_gather(Lat<T> Lt, Ctl* C) {
     else
       go(k_gather_g<T, kMaxHist, false>);
   }
+}
+
+// The direction kernel, built for the longest history of the instances.
+template <class T>
+void mmc_ctx::launch_dstate(Lat<T> Lt, const Ctl* C) {
+  const size_t n6 = 6 * size_t(N);
+  auto go = [&](auto kern) {
+    kern<<<gPd, kElemThreads, 0, stream>>>(Lt, inst_d, gtd, C, zbuf[0], zbuf[1], gbuf[0], gbuf[1], Sh, Yh, n6, p, x,
+                                           (double*)R, xref, nbuf[0], nbuf[1], dspart);
+  };
+  if (max_hist <= 10)
+    go(k_dstate_g<T, 10>);
+  else
+    go(k_dstate_g<T, kMaxHist>);
 }
 
 template <class T>
@@ -4380,7 +4395,6 @@ double mmc_ctx::time_loop_kernel(int kernel, int launches) {
   HIPCALL(hipMemcpy(tinit, h.data(), h.size() * sizeof(Ctl), hipMemcpyHostToDevice));
   const long words = long(n_inst) * long(sizeof(Ctl) / 8);
   const int wblocks = int((words + kBlock - 1) / kBlock);
-  const size_t n6 = 6 * size_t(N);
   auto loop = [&](auto Lt, auto* tag, bool restore, bool work) {
     using T = std::remove_pointer_t<decltype(tag)>;
     HIPCALL(hipEventRecord(ev0, stream));
@@ -4389,9 +4403,7 @@ double mmc_ctx::time_loop_kernel(int kernel, int launches) {
       if (!work) continue;
       switch (kernel) {
         case MMC_K_DSTATE_G:
-          k_dstate_g<T><<<gPd, kElemThreads, 0, stream>>>(Lt, inst_d, gtd, tctl, zbuf[0], zbuf[1], gbuf[0],
-                                                          gbuf[1], Sh, Yh, n6, p, x, (double*)R, xref, nbuf[0],
-                                                          nbuf[1], dspart);
+          launch_dstate<T>(Lt, tctl);
           break;
         case MMC_K_ELEM_G:
           launch_elem<T>(Lt, tctl);
