@@ -1249,9 +1249,6 @@ __global__ __launch_bounds__(kElemThreads) void k_gather(Lat<T> L, const Inst* _
 // 13 centering energy, 14 g.p
 constexpr int kRec = 16;
 
-// The record of instance in from its 20 reduced partial rows: 9 of the state
-// kernel, 4 of the contact shapes, 4 of the beams, 3 of the gradient (fl: the
-// instance's flag word).
 // The instance constants a record needs (the decision kernels read them
 // ahead of the decision).
 struct RecConst {
@@ -1259,6 +1256,9 @@ struct RecConst {
   double centering, n_free;
 };
 __device__ __forceinline__ RecConst rec_const(const Inst& in) { return {in.N, in.S.centering, in.n_free}; }
+// The record of instance in from its 20 reduced partial rows: 9 of the state
+// kernel, 4 of the contact shapes, 4 of the beams, 3 of the gradient (fl: the
+// instance's flag word).
 __device__ __forceinline__ void record_from_rows(const RecConst& in, const double* __restrict__ rows, int fl,
                                                  const double* __restrict__ refrec, double* __restrict__ out) {
   const int N = in.N;
