@@ -2447,8 +2447,11 @@ __global__ __launch_bounds__(kElemThreads) void k_gather_g(Lat<T> L, const Inst*
 // fence: L2 write-back, for the other XCDs) and counts the arrival; only the
 // last arriver acquires (invalidates its cached lines) before reading.
 // D: blocks per instance, chosen by the loop's tuner (any D gives the same
-// rows: each row is folded by one wavefront in one fixed order).  MH: the
-// history capacity the decision's coefficient solve is built for.
+// rows: each row is folded by one wavefront in one fixed order).  D = 1: the
+// instance's one block folds every row into LDS and decides, with no arrival
+// and no fence (on batches of many lattices each block's L2 write-back is the
+// decision's largest cost).  MH: the history capacity the decision's
+// coefficient solve is built for.
 template <int D, int MH>
 __global__ __launch_bounds__(kBlock) void k_decide_g(Ctl* __restrict__ Cs, const Inst* __restrict__ I,
                                                      const int* __restrict__ act, const GRng* __restrict__ G,
@@ -2457,7 +2460,9 @@ __global__ __launch_bounds__(kBlock) void k_decide_g(Ctl* __restrict__ Cs, const
                                                      const double* __restrict__ bpart, int Pb,
                                                      const double* __restrict__ gpart, int* __restrict__ iflags,
                                                      double* drows, unsigned* dcount) {
-  constexpr int NQ = (kGDecRows + D * kDecWaves - 1) / (D * kDecWaves);
+  // rows at a full history of MH pairs (MH - 1 retained and the new one)
+  constexpr int kRows = 21 + 4 * (MH - 1) + 5;
+  constexpr int NQ = (kRows + D * kDecWaves - 1) / (D * kDecWaves);
   const int b = act[blockIdx.x / D], part = blockIdx.x % D;
   Ctl* C = Cs + b;
   const Inst& in = I[b];
@@ -2473,6 +2478,7 @@ __global__ __launch_bounds__(kBlock) void k_decide_g(Ctl* __restrict__ Cs, const
   for (int q = 0; q < RD; q++)
     cr[q] = reinterpret_cast<const double*>(C)[min(int(threadIdx.x) + q * kBlock, kCtlWords - 1)];
   __shared__ int last;
+  __shared__ double rows[kGDecRows];
   {
     const int m = C->m, count = C->count;
     const int R = 21 + (m > 0 ? 4 * (count - (count == m ? 1 : 0)) + 5 : 0);
@@ -2485,22 +2491,28 @@ __global__ __launch_bounds__(kBlock) void k_decide_g(Ctl* __restrict__ Cs, const
     // resident blocks
     constexpr int RD = NQ <= 2 ? 16 : (NQ <= 4 ? 8 : 2);
     double* out = drows + size_t(b) * kGDecRows;
-    rows_pass<NQ, RD>([&](int j) { return graph_row(j, g, P); }, dspart + g.pd0, R, part * kDecWaves, D * kDecWaves,
-                      [&](int j, double t) { out[j] = t; });
+    // (rows_pass ends with a block barrier)
+    if constexpr (D == 1)
+      rows_pass<NQ, RD>([&](int j) { return graph_row(j, g, P); }, dspart + g.pd0, R, 0, kDecWaves,
+                        [&](int j, double t) { rows[j] = t; });
+    else
+      rows_pass<NQ, RD>([&](int j) { return graph_row(j, g, P); }, dspart + g.pd0, R, part * kDecWaves,
+                        D * kDecWaves, [&](int j, double t) { out[j] = t; });
   }
-  if (threadIdx.x == 0) {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    last = atomicAdd(dcount + b, 1u) == unsigned(D - 1);
-    if (last) {
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-      dcount[b] = 0;
+  if constexpr (D > 1) {
+    if (threadIdx.x == 0) {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      last = atomicAdd(dcount + b, 1u) == unsigned(D - 1);
+      if (last) {
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        dcount[b] = 0;
+      }
     }
+    __syncthreads();
+    if (!last) return;
+    if (int(threadIdx.x) < kGDecRows) rows[threadIdx.x] = drows[size_t(b) * kGDecRows + threadIdx.x];
   }
-  __syncthreads();
-  if (!last) return;
-  __shared__ double rows[kGDecRows];
   __shared__ Ctl sc;
-  if (int(threadIdx.x) < kGDecRows) rows[threadIdx.x] = drows[size_t(b) * kGDecRows + threadIdx.x];
 #pragma unroll
   for (int q = 0; q < RD; q++) {
     const int w = threadIdx.x + q * kBlock;
@@ -3073,7 +3085,7 @@ struct mmc_ctx {
   bool tuned = false;
   // tuner timings (us): gather + decision at splits 2, 4, 8, 16, then the
   // gather with the decision fused
-  double tune_dec_us[5] = {0, 0, 0, 0, 0};
+  double tune_dec_us[6] = {0, 0, 0, 0, 0, 0};
   template <class T>
   void launch_dstate(Lat<T> Lt, const Ctl* C);
   template <class T>
@@ -4282,6 +4294,9 @@ void mmc_ctx::launch_decide(Ctl* C) {
   };
   const bool small = max_hist <= 10;
   switch (dec_split) {
+    case 1:
+      small ? go(k_decide_g<1, 10>, 1) : go(k_decide_g<1, kMaxHist>, 1);
+      break;
     case 2:
       small ? go(k_decide_g<2, 10>, 2) : go(k_decide_g<2, kMaxHist>, 2);
       break;
@@ -4298,7 +4313,7 @@ void mmc_ctx::launch_decide(Ctl* C) {
 
 // Launch-shape tuner of the device loop, run at a context's first capture on
 // the state of its first evaluation: the gather and decision of a trial as two
-// kernels, the decision at 2, 4, 8 and 16 blocks per instance, and as one
+// kernels, the decision at 1, 2, 4, 8 and 16 blocks per instance, and as one
 // (the decision in the gather's last block per instance), each timed with HIP
 // events over back-to-back launches (time_loop_kernel).  Of the splits, the
 // most blocks within 3% of the fastest; the fused form when it beats that.
@@ -4311,18 +4326,19 @@ void mmc_ctx::tune() {
   const char* fd = std::getenv("MMC_DECIDE_BLOCKS");
   if (fd) {
     const int d = std::atoi(fd);
-    if (d != 0 && d != 2 && d != 4 && d != 8 && d != 16)
-      throw Fail(MMC_ERR_ARG, "MMC_DECIDE_BLOCKS must be 0, 2, 4, 8 or 16");
+    if (d != 0 && d != 1 && d != 2 && d != 4 && d != 8 && d != 16)
+      throw Fail(MMC_ERR_ARG, "MMC_DECIDE_BLOCKS must be 0, 1, 2, 4, 8 or 16");
     dec_split = d;
     return;
   }
   if (env && std::strcmp(env, "0") == 0) return;
   constexpr int kLaunches = 20;
-  const int splits[4] = {2, 4, 8, 16};
+  constexpr int kSplits = 5;
+  const int splits[kSplits] = {1, 2, 4, 8, 16};
   dec_split = 8;
   const double gather = 1e3 * time_loop_kernel(MMC_K_GATHER_G, kLaunches);
   int best = 0;
-  for (int k = 0; k < 4; k++) {
+  for (int k = 0; k < kSplits; k++) {
     dec_split = splits[k];
     tune_dec_us[k] = gather + 1e3 * time_loop_kernel(MMC_K_DECIDE_G, kLaunches);
     if (tune_dec_us[k] < tune_dec_us[best]) best = k;
@@ -4330,14 +4346,14 @@ void mmc_ctx::tune() {
   // the most blocks within 3% of the fastest: the timed launches find the
   // partial rows in cache, while in the loop the other kernels have just
   // written them, and more blocks spread those colder loads
-  for (int k = 3; k > best; k--)
+  for (int k = kSplits - 1; k > best; k--)
     if (tune_dec_us[k] <= 1.03 * tune_dec_us[best]) {
       best = k;
       break;
     }
   dec_split = 0;
-  tune_dec_us[4] = 1e3 * time_loop_kernel(MMC_K_GATHER_G, kLaunches);
-  if (tune_dec_us[4] >= tune_dec_us[best]) dec_split = splits[best];
+  tune_dec_us[kSplits] = 1e3 * time_loop_kernel(MMC_K_GATHER_G, kLaunches);
+  if (tune_dec_us[kSplits] >= tune_dec_us[best]) dec_split = splits[best];
 }
 
 void mmc_ctx::capture(const std::vector<int>& set) {
@@ -5179,7 +5195,7 @@ int mmc_loop_shape(mmc_ctx* c, int32_t* decide_blocks, double* decide_us) {
   return guarded(c, [&] {
     if (decide_blocks) *decide_blocks = c->dec_split;
     if (decide_us)
-      for (int k = 0; k < 5; k++) decide_us[k] = c->tune_dec_us[k];
+      for (int k = 0; k < 6; k++) decide_us[k] = c->tune_dec_us[k];
   });
 }
 

@@ -664,10 +664,10 @@ class Simulator:
         of the decision kernel; 0: fused into the gather kernel), and the
         gather + decision times (us) of each candidate."""
         d = C.c_int32()
-        du = (C.c_double * 5)()
+        du = (C.c_double * 6)()
         lib().mmc_loop_shape(self.context(), C.byref(d), du)
         return {"decide_blocks": d.value,
-                "gather_decide_us": {str(k): round(v, 3) for k, v in zip((2, 4, 8, 16, "fused"), du)}}
+                "gather_decide_us": {str(k): round(v, 3) for k, v in zip((1, 2, 4, 8, 16, "fused"), du)}}
 
     def time_kernel(self, kernel: str, launches: int = 200) -> float:
         """Mean duration (ms) of one objective kernel, HIP events on the solver stream."""

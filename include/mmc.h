@@ -246,10 +246,10 @@ int64_t mmc_evaluations(const mmc_ctx* ctx);
 int mmc_time_kernel(mmc_ctx* ctx, int32_t kernel, int32_t launches, double* mean_ms);
 /* Launch shape the device loop's tuner chose at the context's first
  * minimisation (no reference counterpart): blocks per lattice of the decision
- * kernel (2, 4, 8 or 16), or 0 when the decision runs in the gather kernel's
- * last block per lattice, with the mean times (us) it measured for each
- * candidate (decide_us[5]: gather + decision at the splits 2, 4, 8 and 16,
- * then the gather with the decision fused; zeros before the first
+ * kernel (1, 2, 4, 8 or 16), or 0 when the decision runs in the gather
+ * kernel's last block per lattice, with the mean times (us) it measured for
+ * each candidate (decide_us[6]: gather + decision at the splits 1, 2, 4, 8
+ * and 16, then the gather with the decision fused; zeros before the first
  * minimisation or with MMC_TUNE=0).  The shape changes no result.  Either
  * pointer may be NULL. */
 int mmc_loop_shape(mmc_ctx* ctx, int32_t* decide_blocks, double* decide_us);

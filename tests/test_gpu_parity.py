@@ -330,7 +330,7 @@ def test_run_config1_mixed(gpu, po, loop_engine):
     assert err["loading_work"] <= 2e-5 and err["dissipated"] <= 5e-5
 
 
-@pytest.mark.parametrize("split", ["0", "2", "4", "16"])
+@pytest.mark.parametrize("split", ["0", "1", "2", "4", "16"])
 def test_launch_shapes_leave_results_unchanged(gpu, split, monkeypatch):
     """The loop's tuner picks the decision kernel's split per context, or
     fuses the decision into the gather kernel (split 0); it may not change a
