@@ -3077,15 +3077,21 @@ struct mmc_ctx {
   int gPd = 0, gPb = 0, gPc = 0;
   std::vector<int> gset;
   int64_t captures = 0;
-  // launch shapes of the loop, chosen by tune() at the first capture (neither
-  // changes a result): items per block of the element kernel, blocks per
-  // instance of the decision kernel; and the timings the choice was made on
-  // (dec_split 0: the decision runs in the gather kernel's last block)
+  // launch shape of the loop (it changes no result): blocks per instance of
+  // the decision kernel (0: the decision runs in the gather kernel's last
+  // block), chosen by tune() at the first capture over a set of each size
+  // class (sets of 1, 2, 3-4, 5-8, ... instances) and reused for later sets
+  // of that class; and the timings the current choice was made on
   int dec_split = 8;
-  bool tuned = false;
-  // tuner timings (us): gather + decision at splits 2, 4, 8, 16, then the
+  // tuner timings (us): gather + decision at splits 1, 2, 4, 8, 16, then the
   // gather with the decision fused
   double tune_dec_us[6] = {0, 0, 0, 0, 0, 0};
+  struct LoopShape {
+    bool tuned = false;
+    int split = 8;
+    double us[6] = {0, 0, 0, 0, 0, 0};
+  };
+  LoopShape shapes[32];
   template <class T>
   void launch_dstate(Lat<T> Lt, const Ctl* C);
   template <class T>
@@ -4311,17 +4317,23 @@ void mmc_ctx::launch_decide(Ctl* C) {
   }
 }
 
-// Launch-shape tuner of the device loop, run at a context's first capture on
-// the state of its first evaluation: the gather and decision of a trial as two
-// kernels, the decision at 1, 2, 4, 8 and 16 blocks per instance, and as one
-// (the decision in the gather's last block per instance), each timed with HIP
+// Launch-shape tuner of the device loop, run at the first capture over a set
+// of each size class (a batch's graph is captured again over fewer instances
+// as it drains; few instances favour many blocks each, many instances one),
+// on the current state: the gather and decision of a trial as two kernels,
+// the decision at 1, 2, 4, 8 and 16 blocks per instance, and as one (the
+// decision in the gather's last block per instance), each timed with HIP
 // events over back-to-back launches (time_loop_kernel).  Of the splits, the
 // most blocks within 3% of the fastest; the fused form when it beats that.
 // The choice changes no result (one fixed fold order per row), so the tuning
-// is free to differ between contexts.  MMC_TUNE=0 keeps the split 8;
-// MMC_DECIDE_BLOCKS fixes the shape (0: fused; tests).
+// is free to differ between contexts and size classes.  A tune in the middle
+// of a solve changes no state: the timed kernels read a scratch copy of the
+// control blocks and write only what the loop rewrites before its next use
+// (the candidate gradient, the spare history slot, partials, folded rows,
+// the arrival counters, left at zero, and the evaluation flags, zero between
+// two polls).  MMC_TUNE=0 keeps the split 8; MMC_DECIDE_BLOCKS fixes the
+// shape (0: fused; tests).
 void mmc_ctx::tune() {
-  tuned = true;
   const char* env = std::getenv("MMC_TUNE");
   const char* fd = std::getenv("MMC_DECIDE_BLOCKS");
   if (fd) {
@@ -4360,7 +4372,19 @@ void mmc_ctx::capture(const std::vector<int>& set) {
   SyncLock sync_lock(g_sync_mu);
   HIPCALL(hipStreamSynchronize(stream));
   graph_tables(set);
-  if (!tuned) tune();
+  // size class: ceil(log2(instances))
+  int cls = 0;
+  while ((size_t(1) << cls) < set.size()) cls++;
+  LoopShape& shape = shapes[cls];
+  if (!shape.tuned) {
+    tune();
+    shape.tuned = true;
+    shape.split = dec_split;
+    std::copy(tune_dec_us, tune_dec_us + 6, shape.us);
+  } else {
+    dec_split = shape.split;
+    std::copy(shape.us, shape.us + 6, tune_dec_us);
+  }
   captures++;
   if (precision == MMC_MIXED)
     capture_iteration<float>(lat32());

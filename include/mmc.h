@@ -244,11 +244,13 @@ int64_t mmc_evaluations(const mmc_ctx* ctx);
  * they write only the loop's scratch buffers, so the solver state is left as
  * it was. */
 int mmc_time_kernel(mmc_ctx* ctx, int32_t kernel, int32_t launches, double* mean_ms);
-/* Launch shape the device loop's tuner chose at the context's first
- * minimisation (no reference counterpart): blocks per lattice of the decision
- * kernel (1, 2, 4, 8 or 16), or 0 when the decision runs in the gather
- * kernel's last block per lattice, with the mean times (us) it measured for
- * each candidate (decide_us[6]: gather + decision at the splits 1, 2, 4, 8
+/* Launch shape the device loop's tuner chose for the lattices of the
+ * context's latest graph capture (no reference counterpart; the tuner runs at
+ * the first capture over a set of each size class, 1, 2, 3-4, 5-8, ...
+ * lattices, and reuses its choice for later sets of that class): blocks per
+ * lattice of the decision kernel (1, 2, 4, 8 or 16), or 0 when the decision
+ * runs in the gather kernel's last block per lattice, with the mean times
+ * (us) it measured for each candidate (decide_us[6]: gather + decision at the splits 1, 2, 4, 8
  * and 16, then the gather with the decision fused; zeros before the first
  * minimisation or with MMC_TUNE=0).  The shape changes no result.  Either
  * pointer may be NULL. */
