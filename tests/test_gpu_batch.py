@@ -85,3 +85,20 @@ def test_sharded_batch_equals_single_runs(gpu, shards, monkeypatch):
     streams and host threads), lattice b in shard b mod shards."""
     monkeypatch.setenv("MMC_BATCH_SHARDS", str(shards))
     _batch_vs_single(gpu, "fp64")
+
+
+@pytest.mark.parametrize("split", ["1", "16"])
+def test_batch_decision_split_equals_single_runs(gpu, split, monkeypatch):
+    """The decision kernel fixed at one block per lattice (every row folded by
+    the deciding block, no arrival counter) and at sixteen, over the four
+    lattices of one batch (the tuner's choice per set size bypassed), against
+    single runs at the tuner's shape: bit for bit, mixed policy."""
+    mmc = gpu
+    monkeypatch.setenv("MMC_DEVICE_LOOP", "graph")
+    cfgs = _configs(mmc)
+    monkeypatch.setenv("MMC_DECIDE_BLOCKS", split)
+    frames, status, err = mmc.BatchSimulator(cfgs, mmc.MMC_MIXED).run()
+    monkeypatch.delenv("MMC_DECIDE_BLOCKS")
+    assert status == [0] * len(cfgs), err
+    for cfg, fb in zip(cfgs, frames):
+        _same(fb, mmc.Simulator(cfg, mmc.MMC_MIXED).run())
