@@ -2487,9 +2487,10 @@ __global__ __launch_bounds__(kBlock) void k_decide_g(Ctl* __restrict__ Cs, const
     // first-round columns per lane: as many as 32 registers per row pair hold
     // (the beam and contact rows have one column per 64 items); with many
     // rows per wave (few blocks per lattice, chosen for batches of small
-    // lattices) two, so that the fold's registers leave room for more
-    // resident blocks
-    constexpr int RD = NQ <= 2 ? 16 : (NQ <= 4 ? 8 : 2);
+    // lattices) two, and one at one block per lattice (chosen for many small
+    // lattices), so that the fold's registers leave room for more resident
+    // blocks
+    constexpr int RD = NQ <= 2 ? 16 : (NQ <= 4 ? 8 : (D == 1 ? 1 : 2));
     double* out = drows + size_t(b) * kGDecRows;
     // (rows_pass ends with a block barrier)
     if constexpr (D == 1)
